@@ -1,0 +1,99 @@
+"""ctypes binding of libamaru_hip.so (the C ABI in include/amaru_hip.h).
+
+There is no fallback: if the library is missing or does not load, every
+entry point raises.  The signatures below are the ones a Julia ``ccall``
+glue would use.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libamaru_hip.so")
+
+AMR_OK = 0
+AMR_HEX8, AMR_HEX20, AMR_TET10 = 1, 2, 3
+AMR_MAT_LE, AMR_MAT_VM, AMR_MAT_DP = 0, 1, 2
+AMR_PREC_FP64, AMR_PREC_MIXED = 0, 1
+AMR_PRECOND_JACOBI, AMR_PRECOND_BLOCK_JACOBI = 0, 1
+ERRORS = {-1: "AMR_ERR_ARG", -2: "AMR_ERR_HIP", -3: "AMR_ERR_JACOBIAN", -4: "AMR_ERR_MATERIAL",
+          -5: "AMR_ERR_SOLVE", -6: "AMR_ERR_NAN", -7: "AMR_ERR_NOMEM"}
+
+# every function the header declares, with (restype, argtypes)
+P = ctypes.c_void_p
+I32, I64, D = ctypes.c_int, ctypes.c_int64, ctypes.c_double
+
+
+class SolveOpts(ctypes.Structure):
+    _fields_ = [("precision", ctypes.c_int), ("precond", ctypes.c_int), ("maxiter", ctypes.c_int),
+                ("check_every", ctypes.c_int), ("rtol", ctypes.c_double), ("inner_rtol", ctypes.c_double)]
+
+
+class SolveInfo(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int), ("refinements", ctypes.c_int), ("converged", ctypes.c_int),
+                ("pad", ctypes.c_int), ("relres", ctypes.c_double)]
+
+
+SIGNATURES = {
+    "amr_last_error": (ctypes.c_char_p, []),
+    "amr_device_count": (I32, [P]),
+    "amr_model_create": (I32, [P, I32, I32, I32, I64, P, I64, P, I32, P, P, P, P, I64, I64]),
+    "amr_model_destroy": (I32, [P]),
+    "amr_model_set_dofs": (I32, [P, P, I64, I64]),
+    "amr_model_info": (I32, [P, P]),
+    "amr_state_set": (I32, [P, P]),
+    "amr_state_get": (I32, [P, P, I32]),
+    "amr_state_commit": (I32, [P]),
+    "amr_state_restore": (I32, [P]),
+    "amr_assemble": (I32, [P]),
+    "amr_get_Ke": (I32, [P, P]),
+    "amr_get_K_csc": (I32, [P, P, P, P, P]),
+    "amr_solve_system": (I32, [P, P, P, ctypes.POINTER(SolveOpts), ctypes.POINTER(SolveInfo)]),
+    "amr_update_state": (I32, [P, P, P]),
+}
+
+_lib = None
+
+
+class AmaruError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+        self.msg = msg
+
+
+def lib():
+    """Load the HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with amaru.jl_amd/build.py "
+                           "(the HIP path has no CPU fallback)")
+    L = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def check(rc):
+    if rc != AMR_OK:
+        raise AmaruError(rc, lib().amr_last_error().decode(errors="replace"))
+    return rc
+
+
+def ptr(a):
+    if a is None:
+        return None
+    assert isinstance(a, np.ndarray) and a.flags["C_CONTIGUOUS"], "need a C-contiguous array"
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def device_count():
+    n = ctypes.c_int(0)
+    check(lib().amr_device_count(ctypes.byref(n)))
+    return n.value

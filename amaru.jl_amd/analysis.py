@@ -1,0 +1,216 @@
+"""MechAnalysis / addstage! / solve! on the device (host control flow).
+
+``solve`` is the reference's ``solve!(ana::MechAnalysis; args...)``
+(src/mech/mech-solver.jl:155-181) → ``stage_iterator!`` (src/solver.jl:
+82-198) → ``mech_stage_solver!`` (src/mech/mech-solver.jl:186-492), with
+the hot calls going through the C ABI:
+
+    mount_K(active_elems, ndofs)       -> DeviceModel.assemble      (K stays on the GPU)
+    solve_system!(K, ΔUitr, Rtr, nu)   -> DeviceModel.solve_system  (PCG on the GPU)
+    copyto!.(State, StateBk)           -> implicit: updates read the committed state
+    update_state!(active_elems, ΔUt)   -> DeviceModel.update_state
+    copyto!.(StateBk, State)           -> DeviceModel.commit
+
+The increment / Newton-Raphson / auto-increment logic is kept as the
+reference has it (scheme :FE).
+"""
+import math
+
+import numpy as np
+
+from .model import DeviceModel
+
+EPS = 2.220446049250313e-16
+
+SOLVER_DEFAULTS = dict(tol=0.01, rspan=0.01, rtol=0.01, dT0=0.01, dTmin=1e-7, dTmax=0.1, scheme="FE", maxits=5,
+                       autoinc=False, quiet=True,
+                       # device options (additions of this implementation)
+                       device=0, precision="fp64", pcg_rtol=1e-12, inner_rtol=1e-4, pcg_maxiter=200000)
+
+
+class Stage:
+    """Stage (src/stage.jl:1-23)."""
+
+    def __init__(self, bcs, nincs=1, nouts=0):
+        self.bcs = list(bcs)
+        self.nincs = int(nincs)
+        self.nouts = int(nouts)
+        self.status = "pending"
+
+
+class MechAnalysis:
+    """MechAnalysis(model) (src/mech/mech-solver.jl:43-74)."""
+
+    def __init__(self, model):
+        self.model = model
+        self.stages = []
+        self.loggers = []      # callables f(model, T) after each converged increment
+        self.log = []          # (stage, inc, it, residue) like solve.log
+        self.stats = dict(nr_iterations=0, pcg_iterations=0, increments=0, dof_solves=0)
+        self.dev = None
+
+
+def addstage(ana, bcs, nincs=1, nouts=0):
+    """addstage!(ana, bcs; nincs, nouts) (src/analysis.jl:27-32)."""
+    st = Stage(bcs, nincs, nouts)
+    ana.stages.append(st)
+    return st
+
+
+def solve(ana, **kwargs):
+    """solve!(ana; args...) → (success, message)."""
+    unknown = set(kwargs) - set(SOLVER_DEFAULTS)
+    if unknown:
+        raise TypeError(f"solve!: unknown arguments {sorted(unknown)}")
+    args = dict(SOLVER_DEFAULTS, **kwargs)
+    if args["scheme"] != "FE":
+        raise NotImplementedError("only the :FE predictor-corrector scheme is implemented on the device")
+    if not 1 <= args["maxits"] <= 10:
+        raise ValueError("maxits must be in 1..10")
+    todo = [s for s in ana.stages if s.status != "done"]
+    if not todo:
+        raise RuntimeError("stage_iterator!: No stages have been set")
+    ok, msg = True, None
+    for stage in todo:
+        stage.status = "solving"
+        nincs = stage.nincs
+        if stage.nouts > 0 and not args["autoinc"] and stage.nouts > nincs:
+            nincs = stage.nouts
+        stage.nincs = nincs
+        ok, msg = _stage_solver(ana, stage, args)
+        stage.status = "done" if ok else "failed"
+        if not ok:
+            break
+    return ok, msg
+
+
+def _stage_solver(ana, stage, a):
+    model = ana.model
+    tol, rtol, rspan = a["tol"], a["rtol"], a["rspan"]
+    dTmin, dTmax, maxits, autoinc = a["dTmin"], a["dTmax"], a["maxits"], a["autoinc"]
+    bcs = stage.bcs
+    ndofs, nu = model.configure_dofs(bcs)
+    if ana.dev is None:
+        ana.dev = DeviceModel(model, device=a["device"])
+    else:
+        ana.dev.set_dofs()
+    dev = ana.dev
+    Uex, Fex = model.bc_vals(bcs)
+    solve_kw = dict(precision=a["precision"], rtol=a["pcg_rtol"], inner_rtol=a["inner_rtol"],
+                    maxiter=a["pcg_maxiter"])
+    # eq id -> (node, dir)
+    inv = np.zeros(ndofs, dtype=np.int64)
+    live = model.eq.ravel() >= 0
+    inv[model.eq.ravel()[live]] = np.nonzero(live)[0]
+
+    nouts = stage.nouts
+    dTcheck = 1.0 / nouts if nouts > 0 else 1.0
+    Tcheck = dTcheck
+    T = 0.0
+    dT = 1.0 / stage.nincs
+    if autoinc:
+        dT = min(dT, dTmax, dTcheck, a["dT0"])
+    dTbk = 0.0
+    inc = 0
+    U = np.zeros(ndofs)
+    F = np.zeros(ndofs)
+    R = np.zeros(ndofs)
+    Rc = np.zeros(ndofs)
+    dFin = np.zeros(ndofs)
+    still_linear = True
+    status = None
+    while T < 1.0 - dTmin:
+        inc += 1
+        dUex, dFex = dT * Uex, dT * Fex
+        dTcr = min(rspan, 1 - T)
+        acr = min(dT / dTcr, 1.0)
+        if T < 1 - rspan:
+            dFex = dFex + acr * Rc
+        R = dFex.copy()
+        dUa = np.zeros(ndofs)
+        dUi = dUex.copy()
+        res = res1 = 0.0
+        converged = syserror = False
+        sysmsg = None
+        for it in range(1, maxits + 1):
+            if it > 1:
+                dUi = np.zeros(ndofs)
+            lastres = res
+            try:
+                dev.assemble()                                  # mount_K
+                dUitr = np.ascontiguousarray(dUi, dtype=float)
+                Rtr = np.ascontiguousarray(R, dtype=float)
+                info = dev.solve_system(dUitr, Rtr, **solve_kw)   # solve_system!
+                ana.stats["pcg_iterations"] += info["iterations"]
+                dUt = dUa + dUitr
+                dFin = dev.update_state(dUt)                    # update_state! (from committed state)
+            except RuntimeError as err:
+                syserror = True
+                sysmsg = str(err)
+                break
+            ana.stats["nr_iterations"] += 1
+            dUi = dUitr
+            dUa = dUa + dUi
+            R = dFex - dFin
+            R[nu:] = 0.0
+            res = float(np.max(np.abs(R))) if ndofs else 0.0
+            err = float(np.max(np.abs(dUi)) / np.max(np.abs(dUa))) if ndofs else 0.0
+            ana.log.append((len(ana.stages), inc, it, res))
+            if it == 1:
+                res1 = res
+            if it > 1:
+                still_linear = False
+            if res < tol or err < rtol:
+                converged = True
+                break
+            if math.isnan(res):
+                break
+            if it > 1 and res > lastres:
+                break
+        if syserror:
+            converged = False
+        if converged:
+            U += dUa
+            F += dFin
+            Rc = (1.0 - acr) * Rc + R
+            dev.commit()
+            model.uvals.ravel()[inv] += dUa
+            model.fvals.ravel()[inv] += dFin
+            ana.stats["increments"] += 1
+            ana.stats["dof_solves"] += nu
+            T += dT
+            if T > Tcheck - dTmin:
+                Tcheck += dTcheck
+            model.state = dev.state_get(committed=True)
+            for lg in ana.loggers:
+                lg(model, T)
+            if autoinc:
+                if dTbk > 0.0:
+                    dT = min(dTbk, Tcheck - T)
+                    dTbk = 0.0
+                else:
+                    q = max(1 + math.tanh(math.log10(tol / (res1 + EPS))), 1.1)
+                    dTtr = min(q * dT, dTmax, 1 - T)
+                    if T + dTtr > Tcheck - dTmin:
+                        dTbk = dT
+                        dT = Tcheck - T
+                    else:
+                        dT = dTtr
+                        dTbk = 0.0
+        else:
+            inc -= 1
+            dev.restore()
+            if autoinc:
+                q = 1 + math.tanh(math.log10(tol / (res1 + EPS)))
+                q = min(max(q, 0.2), 0.9)
+                if syserror:
+                    q = 0.7
+                dT = float(f"{q * dT:.3g}")
+                if dT < dTmin:
+                    status = "Solver did not converge."
+                    break
+            else:
+                status = "Solver did not converge. Try `autoinc=true`. " + (sysmsg or "")
+                break
+    model.state = dev.state_get(committed=True)
+    return status is None, status

@@ -1,0 +1,99 @@
+// Internal declarations shared by the HIP library's translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/amaru_hip.h"
+
+#define AMR_NSTATE 14
+#define AMR_MAXNN 20
+#define AMR_MAXIP 27
+
+// material table entry (device and host)
+struct AmrMat {
+    int kind;  // AMR_MAT_*
+    int pad;
+    double p[6];  // LE: E nu; VM: E nu fy H; DP: E nu alpha kappa H
+};
+
+void amr_set_error(const std::string &msg);
+
+#define AMR_HIP(call)                                                                  \
+    do {                                                                               \
+        hipError_t _e = (call);                                                        \
+        if (_e != hipSuccess) {                                                        \
+            amr_set_error(std::string("HIP error: ") + hipGetErrorString(_e) + " at " + \
+                          __FILE__ + ":" + std::to_string(__LINE__));                  \
+            return AMR_ERR_HIP;                                                        \
+        }                                                                              \
+    } while (0)
+
+// Host-built sparsity structure (node-block BSR over all nodes).
+struct AmrPattern {
+    int64_t nnodes = 0, nblk = 0, ncontrib = 0;
+    std::vector<int32_t> rowptr;   // nnodes+1
+    std::vector<int32_t> col;      // nblk
+    std::vector<int32_t> diag;     // nnodes: block index of (n,n)
+    std::vector<int64_t> cptr;     // nblk+1: contributions of each block
+    std::vector<int32_t> csrc;     // ncontrib: e*nn*nn + la*nn + lb (ascending e per block)
+    std::vector<int32_t> fptr;     // nnodes+1: force-gather contributions per node
+    std::vector<int32_t> fsrc;     // e*nn + la (ascending e per node)
+};
+
+int amr_build_pattern(int nn, int64_t nnodes, int64_t ne, const int32_t *conn, AmrPattern &P);
+
+// Per-model device resources.
+struct amr_model {
+    int etype = 0, nn = 0, nips = 0, device = 0;
+    int64_t nnodes = 0, ne = 0, ndofs = 0, nu = 0, nip_total = 0;
+    AmrPattern pat;
+    std::vector<int32_t> eq;      // nnodes*3 eq ids (-1: no dof)
+    std::vector<uint8_t> free_h;  // nnodes*3: 1 if the dof is unknown
+
+    // device buffers
+    double *X = nullptr;          // nnodes*3
+    int32_t *conn = nullptr;      // ne*nn
+    int32_t *emat = nullptr;      // ne
+    AmrMat *mats = nullptr;       // nmats
+    int nmats = 0;
+    bool all_elastic = true;
+    double *dNdR = nullptr;       // nips*nn*3
+    double *wip = nullptr;        // nips
+    double *st_cur = nullptr;     // 14*nip_total SoA (trial/current)
+    double *st_com = nullptr;     // 14*nip_total SoA (committed)
+    double *Ke = nullptr;         // ne*nn*nn*9 (block layout)
+    int32_t *rowptr = nullptr, *col = nullptr, *diag = nullptr;
+    int64_t *cptr = nullptr;
+    int32_t *csrc = nullptr;
+    int32_t *fptr = nullptr, *fsrc = nullptr;
+    double *Kv = nullptr;         // nblk*9 FP64 values
+    float *Kv32 = nullptr;        // nblk*9 FP32 values
+    uint8_t *Kflag = nullptr;     // nblk*9 survivor flags (exact pattern)
+    double *dinv = nullptr;       // 3*nnodes Jacobi inverse (0 at prescribed)
+    float *dinv32 = nullptr;
+    double *mask = nullptr;       // 3*nnodes: 1 free, 0 prescribed
+    int32_t *eqd = nullptr;       // 3*nnodes eq ids (device)
+    double *dF = nullptr;         // ne*nn*3 element internal forces
+    int32_t *estat = nullptr;     // ne status codes
+    int32_t *err = nullptr;       // device error word
+    // solver workspace (3*nnodes each)
+    double *v[12] = {};
+    float *f[8] = {};
+    double *part = nullptr;       // reduction partials
+    double *scal = nullptr;       // device scalars
+    int nparts = 0;
+    hipStream_t stream = nullptr;
+    bool assembled = false;
+};
+
+// kernels (kernels_elem.hip)
+int amr_launch_ke(amr_model *m);
+int amr_launch_assemble(amr_model *m, bool flags);
+int amr_launch_update(amr_model *m, const double *dU_node /*device, node order*/, double *dFin_node);
+
+// solver (kernels_solve.hip)
+int amr_pcg_solve(amr_model *m, const double *b, double *x, const amr_solve_opts *o, amr_solve_info *info);
+int amr_spmv_full(amr_model *m, const double *x, double *y, int mask_mode);

@@ -1,0 +1,652 @@
+// C ABI of libamaru_hip.so (declared in include/amaru_hip.h).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+
+#include "amr_internal.h"
+
+static thread_local std::string g_err;
+
+void amr_set_error(const std::string &msg) { g_err = msg; }
+
+extern "C" const char *amr_last_error(void) { return g_err.c_str(); }
+
+extern "C" int amr_device_count(int *n)
+{
+    if (!n) return AMR_ERR_ARG;
+    AMR_HIP(hipGetDeviceCount(n));
+    return AMR_OK;
+}
+
+// ------------------------------------------------ shapes and quadrature
+// Natural coordinates of the HEX20 nodes (first 8: HEX8 corners), in the
+// reference's node numbering (src/shape/solids3d.jl:459-468, 557-580).
+static const double kHexNodes[20][3] = {
+    {-1, -1, -1}, {1, -1, -1}, {1, 1, -1}, {-1, 1, -1}, {-1, -1, 1}, {1, -1, 1}, {1, 1, 1}, {-1, 1, 1},
+    {0, -1, -1},  {1, 0, -1},  {0, 1, -1}, {-1, 0, -1}, {0, -1, 1},  {1, 0, 1},  {0, 1, 1}, {-1, 0, 1},
+    {-1, -1, 0},  {1, -1, 0},  {1, 1, 0},  {-1, 1, 0}};
+static const int kTetEdges[6][2] = {{0, 1}, {1, 2}, {2, 0}, {0, 3}, {1, 3}, {2, 3}};
+
+// dN/dR at R (nn x 3).  HEX8 N_a = (1+r ra)(1+s sa)(1+t ta)/8
+// (src/shape/solids3d.jl:472-504); HEX20 serendipity (:584-691); TET10 in
+// volume coordinates (:124-191).
+static void shape_deriv(int etype, const double *R, double *D)
+{
+    const double r = R[0], s = R[1], t = R[2];
+    if (etype == AMR_HEX8) {
+        for (int a = 0; a < 8; a++) {
+            const double ra = kHexNodes[a][0], sa = kHexNodes[a][1], ta = kHexNodes[a][2];
+            const double fr = 1.0 + ra * r, fs = 1.0 + sa * s, ft = 1.0 + ta * t;
+            D[a * 3 + 0] = 0.125 * ra * fs * ft;
+            D[a * 3 + 1] = 0.125 * sa * fr * ft;
+            D[a * 3 + 2] = 0.125 * ta * fr * fs;
+        }
+    } else if (etype == AMR_HEX20) {
+        for (int a = 0; a < 8; a++) {
+            const double ra = kHexNodes[a][0], sa = kHexNodes[a][1], ta = kHexNodes[a][2];
+            const double fr = 1.0 + ra * r, fs = 1.0 + sa * s, ft = 1.0 + ta * t;
+            const double g = ra * r + sa * s + ta * t - 2.0;
+            D[a * 3 + 0] = 0.125 * ra * fs * ft * (g + fr);
+            D[a * 3 + 1] = 0.125 * sa * fr * ft * (g + fs);
+            D[a * 3 + 2] = 0.125 * ta * fr * fs * (g + ft);
+        }
+        for (int a = 8; a < 20; a++) {
+            const double ra = kHexNodes[a][0], sa = kHexNodes[a][1], ta = kHexNodes[a][2];
+            if (ra == 0.0) {
+                const double fs = 1.0 + sa * s, ft = 1.0 + ta * t, q = 1.0 - r * r;
+                D[a * 3 + 0] = -0.5 * r * fs * ft;
+                D[a * 3 + 1] = 0.25 * q * sa * ft;
+                D[a * 3 + 2] = 0.25 * q * fs * ta;
+            } else if (sa == 0.0) {
+                const double fr = 1.0 + ra * r, ft = 1.0 + ta * t, q = 1.0 - s * s;
+                D[a * 3 + 0] = 0.25 * q * ra * ft;
+                D[a * 3 + 1] = -0.5 * s * fr * ft;
+                D[a * 3 + 2] = 0.25 * q * fr * ta;
+            } else {
+                const double fr = 1.0 + ra * r, fs = 1.0 + sa * s, q = 1.0 - t * t;
+                D[a * 3 + 0] = 0.25 * q * ra * fs;
+                D[a * 3 + 1] = 0.25 * q * fr * sa;
+                D[a * 3 + 2] = -0.5 * t * fr * fs;
+            }
+        }
+    } else {
+        double L[4] = {1.0 - r - s - t, r, s, t};
+        double dL[4][3] = {{-1, -1, -1}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+        for (int i = 0; i < 4; i++)
+            for (int k = 0; k < 3; k++) D[i * 3 + k] = (4.0 * L[i] - 1.0) * dL[i][k];
+        for (int e = 0; e < 6; e++) {
+            const int i = kTetEdges[e][0], j = kTetEdges[e][1];
+            for (int k = 0; k < 3; k++) D[(4 + e) * 3 + k] = 4.0 * (dL[i][k] * L[j] + L[i] * dL[j][k]);
+        }
+    }
+}
+
+// IP rules, as tabulated by the reference (src/shape/quadrature.jl:110-114,
+// 167-206): HEX 2x2x2 and 3x3x3 (r fastest), TET 4-point.
+static int ip_rule(int etype, int nips, double *R, double *w)
+{
+    const double g2 = 0.577350269189626, g3 = 0.774596669241483;
+    const double w3[4] = {0.171467764060357, 0.274348422496571, 0.438957475994513, 0.702331961591221};
+    if (etype == AMR_HEX8 || etype == AMR_HEX20) {
+        if (nips == 0 || nips == 8) {
+            int q = 0;
+            for (int k = 0; k < 2; k++)
+                for (int j = 0; j < 2; j++)
+                    for (int i = 0; i < 2; i++, q++) {
+                        R[q * 3] = i ? g2 : -g2;
+                        R[q * 3 + 1] = j ? g2 : -g2;
+                        R[q * 3 + 2] = k ? g2 : -g2;
+                        w[q] = 1.0;
+                    }
+            return 8;
+        }
+        if (nips == 27) {
+            const double pts[3] = {-g3, 0.0, g3};
+            int q = 0;
+            for (int k = 0; k < 3; k++)
+                for (int j = 0; j < 3; j++)
+                    for (int i = 0; i < 3; i++, q++) {
+                        R[q * 3] = pts[i];
+                        R[q * 3 + 1] = pts[j];
+                        R[q * 3 + 2] = pts[k];
+                        w[q] = w3[(i == 1) + (j == 1) + (k == 1)];
+                    }
+            return 27;
+        }
+        return -1;
+    }
+    if (etype == AMR_TET10 && (nips == 0 || nips == 4)) {
+        const double a = 0.5854101966249685, b = 0.1381966011250105;
+        const double P[4][3] = {{a, b, b}, {b, a, b}, {b, b, a}, {b, b, b}};
+        for (int q = 0; q < 4; q++) {
+            for (int k = 0; k < 3; k++) R[q * 3 + k] = P[q][k];
+            w[q] = 0.04166666666666667;
+        }
+        return 4;
+    }
+    return -1;
+}
+
+static int nn_of(int etype)
+{
+    return etype == AMR_HEX8 ? 8 : etype == AMR_HEX20 ? 20 : etype == AMR_TET10 ? 10 : -1;
+}
+
+// ------------------------------------------------------------- helpers
+
+template <typename T>
+static int dalloc(T **p, size_t n)
+{
+    if (n == 0) n = 1;
+    hipError_t e = hipMalloc((void **)p, n * sizeof(T));
+    if (e != hipSuccess) {
+        *p = nullptr;
+        amr_set_error(std::string("hipMalloc of ") + std::to_string(n * sizeof(T)) + " bytes failed: " +
+                      hipGetErrorString(e));
+        return e == hipErrorOutOfMemory ? AMR_ERR_NOMEM : AMR_ERR_HIP;
+    }
+    return AMR_OK;
+}
+
+template <typename T>
+static int h2d(T *d, const T *h, size_t n, hipStream_t s)
+{
+    if (n == 0) return AMR_OK;
+    AMR_HIP(hipMemcpyAsync(d, h, n * sizeof(T), hipMemcpyHostToDevice, s));
+    return AMR_OK;
+}
+
+template <typename T>
+static int d2h(T *h, const T *d, size_t n, hipStream_t s)
+{
+    if (n == 0) return AMR_OK;
+    AMR_HIP(hipMemcpyAsync(h, d, n * sizeof(T), hipMemcpyDeviceToHost, s));
+    return AMR_OK;
+}
+
+#define TRY(x)                 \
+    do {                       \
+        int _rc = (x);         \
+        if (_rc) return _rc;   \
+    } while (0)
+
+static void free_model(amr_model *m)
+{
+    if (!m) return;
+    hipSetDevice(m->device);
+    void *ptrs[] = {m->X,    m->conn, m->emat, m->mats,  m->dNdR, m->wip,  m->st_cur, m->st_com, m->Ke,
+                    m->rowptr, m->col, m->diag, m->cptr,  m->csrc, m->fptr, m->fsrc,   m->Kv,     m->Kv32,
+                    m->Kflag, m->dinv, m->dinv32, m->mask, m->eqd,  m->dF,   m->estat,  m->err,    m->part,
+                    m->scal};
+    for (void *p : ptrs)
+        if (p) hipFree(p);
+    for (double *p : m->v)
+        if (p) hipFree(p);
+    for (float *p : m->f)
+        if (p) hipFree(p);
+    if (m->stream) hipStreamDestroy(m->stream);
+    delete m;
+}
+
+static int upload_dofs(amr_model *m, const int32_t *eq_id, int64_t ndofs, int64_t nu)
+{
+    if (ndofs < 0 || nu < 0 || nu > ndofs) {
+        amr_set_error("invalid ndofs/nu");
+        return AMR_ERR_ARG;
+    }
+    m->ndofs = ndofs;
+    m->nu = nu;
+    m->eq.assign(eq_id, eq_id + m->nnodes * 3);
+    m->free_h.assign(m->nnodes * 3, 0);
+    std::vector<double> mask(m->nnodes * 3, 0.0);
+    std::vector<char> seen(ndofs, 0);
+    for (int64_t k = 0; k < m->nnodes * 3; k++) {
+        int32_t q = eq_id[k];
+        if (q < -1 || q >= ndofs || (q >= 0 && seen[q])) {
+            amr_set_error("eq_id is not an injective map into [0, ndofs)");
+            return AMR_ERR_ARG;
+        }
+        if (q >= 0) {
+            seen[q] = 1;
+            if (q < nu) {
+                m->free_h[k] = 1;
+                mask[k] = 1.0;
+            }
+        }
+    }
+    TRY(h2d(m->mask, mask.data(), mask.size(), m->stream));
+    TRY(h2d(m->eqd, eq_id, (size_t)m->nnodes * 3, m->stream));
+    AMR_HIP(hipStreamSynchronize(m->stream));
+    m->assembled = false;
+    return AMR_OK;
+}
+
+extern "C" int amr_model_create(amr_model **out, int device, int etype, int nips, int64_t nnodes,
+                                const double *coords, int64_t ne, const int32_t *conn, int nmats,
+                                const int32_t *mat_kind, const double *mat_params, const int32_t *emat,
+                                const int32_t *eq_id, int64_t ndofs, int64_t nu)
+{
+    if (!out || !coords || !conn || !mat_kind || !mat_params || !emat || !eq_id || nnodes <= 0 || ne < 0 ||
+        nmats <= 0) {
+        amr_set_error("amr_model_create: null pointer or empty model");
+        return AMR_ERR_ARG;
+    }
+    *out = nullptr;
+    const int nn = nn_of(etype);
+    if (nn < 0) {
+        amr_set_error("amr_model_create: unknown element type " + std::to_string(etype));
+        return AMR_ERR_ARG;
+    }
+    double R[AMR_MAXIP * 3], w[AMR_MAXIP];
+    const int nq = ip_rule(etype, nips, R, w);
+    if (nq < 0) {
+        amr_set_error("amr_model_create: element type " + std::to_string(etype) + " has no " +
+                      std::to_string(nips) + "-point rule");
+        return AMR_ERR_ARG;
+    }
+    for (int64_t e = 0; e < ne; e++)
+        if (emat[e] < 0 || emat[e] >= nmats) {
+            amr_set_error("amr_model_create: element material index out of range");
+            return AMR_ERR_ARG;
+        }
+    std::vector<AmrMat> mats(nmats);
+    bool all_el = true;
+    for (int i = 0; i < nmats; i++) {
+        if (mat_kind[i] < AMR_MAT_LE || mat_kind[i] > AMR_MAT_DP) {
+            amr_set_error("amr_model_create: unknown material kind");
+            return AMR_ERR_ARG;
+        }
+        mats[i].kind = mat_kind[i];
+        mats[i].pad = 0;
+        for (int k = 0; k < 6; k++) mats[i].p[k] = mat_params[i * 6 + k];
+        if (mat_kind[i] != AMR_MAT_LE) all_el = false;
+    }
+
+    std::unique_ptr<amr_model, void (*)(amr_model *)> m(new amr_model, free_model);
+    m->etype = etype;
+    m->nn = nn;
+    m->nips = nq;
+    m->device = device;
+    m->nnodes = nnodes;
+    m->ne = ne;
+    m->nip_total = ne * nq;
+    m->nmats = nmats;
+    m->all_elastic = all_el;
+    TRY(amr_build_pattern(nn, nnodes, ne, conn, m->pat));
+
+    AMR_HIP(hipSetDevice(device));
+    AMR_HIP(hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking));
+    const size_t n3 = (size_t)nnodes * 3;
+    TRY(dalloc(&m->X, n3));
+    TRY(dalloc(&m->conn, (size_t)ne * nn));
+    TRY(dalloc(&m->emat, (size_t)ne));
+    TRY(dalloc(&m->mats, (size_t)nmats));
+    TRY(dalloc(&m->dNdR, (size_t)nq * nn * 3));
+    TRY(dalloc(&m->wip, (size_t)nq));
+    TRY(dalloc(&m->st_cur, (size_t)AMR_NSTATE * m->nip_total));
+    TRY(dalloc(&m->st_com, (size_t)AMR_NSTATE * m->nip_total));
+    TRY(dalloc(&m->Ke, (size_t)ne * nn * nn * 9));
+    TRY(dalloc(&m->rowptr, (size_t)nnodes + 1));
+    TRY(dalloc(&m->col, (size_t)m->pat.nblk));
+    TRY(dalloc(&m->diag, (size_t)nnodes));
+    TRY(dalloc(&m->cptr, (size_t)m->pat.nblk + 1));
+    TRY(dalloc(&m->csrc, (size_t)m->pat.ncontrib));
+    TRY(dalloc(&m->fptr, (size_t)nnodes + 1));
+    TRY(dalloc(&m->fsrc, m->pat.fsrc.size()));
+    TRY(dalloc(&m->Kv, (size_t)m->pat.nblk * 9));
+    TRY(dalloc(&m->Kv32, (size_t)m->pat.nblk * 9));
+    TRY(dalloc(&m->Kflag, (size_t)m->pat.nblk * 9));
+    TRY(dalloc(&m->dinv, n3));
+    TRY(dalloc(&m->dinv32, n3));
+    TRY(dalloc(&m->mask, n3));
+    TRY(dalloc(&m->eqd, n3));
+    TRY(dalloc(&m->dF, (size_t)ne * nn * 3));
+    TRY(dalloc(&m->estat, (size_t)ne));
+    TRY(dalloc(&m->err, 1));
+    TRY(dalloc(&m->part, 8192));
+    TRY(dalloc(&m->scal, 64));
+    for (double *&p : m->v) TRY(dalloc(&p, n3));
+    for (float *&p : m->f) TRY(dalloc(&p, n3));
+
+    std::vector<double> dNdR((size_t)nq * nn * 3);
+    for (int q = 0; q < nq; q++) shape_deriv(etype, R + 3 * q, dNdR.data() + (size_t)q * nn * 3);
+    hipStream_t s = m->stream;
+    TRY(h2d(m->X, coords, n3, s));
+    TRY(h2d(m->conn, conn, (size_t)ne * nn, s));
+    TRY(h2d(m->emat, emat, (size_t)ne, s));
+    TRY(h2d(m->mats, mats.data(), mats.size(), s));
+    TRY(h2d(m->dNdR, dNdR.data(), dNdR.size(), s));
+    TRY(h2d(m->wip, w, (size_t)nq, s));
+    TRY(h2d(m->rowptr, m->pat.rowptr.data(), m->pat.rowptr.size(), s));
+    TRY(h2d(m->col, m->pat.col.data(), m->pat.col.size(), s));
+    TRY(h2d(m->diag, m->pat.diag.data(), m->pat.diag.size(), s));
+    TRY(h2d(m->cptr, m->pat.cptr.data(), m->pat.cptr.size(), s));
+    TRY(h2d(m->csrc, m->pat.csrc.data(), m->pat.csrc.size(), s));
+    TRY(h2d(m->fptr, m->pat.fptr.data(), m->pat.fptr.size(), s));
+    TRY(h2d(m->fsrc, m->pat.fsrc.data(), m->pat.fsrc.size(), s));
+    AMR_HIP(hipMemsetAsync(m->st_cur, 0, sizeof(double) * AMR_NSTATE * m->nip_total, s));
+    AMR_HIP(hipMemsetAsync(m->st_com, 0, sizeof(double) * AMR_NSTATE * m->nip_total, s));
+    AMR_HIP(hipMemsetAsync(m->scal, 0, sizeof(double) * 64, s));
+    AMR_HIP(hipMemsetAsync(m->err, 0, sizeof(int32_t), s));
+    TRY(upload_dofs(m.get(), eq_id, ndofs, nu));
+    // host copies of the big maps are no longer needed
+    std::vector<int32_t>().swap(m->pat.csrc);
+    std::vector<int64_t>().swap(m->pat.cptr);
+    std::vector<int32_t>().swap(m->pat.fsrc);
+    *out = m.release();
+    return AMR_OK;
+}
+
+extern "C" int amr_model_destroy(amr_model *m)
+{
+    free_model(m);
+    return AMR_OK;
+}
+
+extern "C" int amr_model_set_dofs(amr_model *m, const int32_t *eq_id, int64_t ndofs, int64_t nu)
+{
+    if (!m || !eq_id) return AMR_ERR_ARG;
+    AMR_HIP(hipSetDevice(m->device));
+    return upload_dofs(m, eq_id, ndofs, nu);
+}
+
+extern "C" int amr_model_info(amr_model *m, int64_t *out)
+{
+    if (!m || !out) return AMR_ERR_ARG;
+    out[0] = m->nnodes;
+    out[1] = m->ne;
+    out[2] = m->nips;
+    out[3] = m->pat.nblk;
+    out[4] = m->pat.nblk * 9;
+    out[5] = m->pat.ncontrib;
+    out[6] = m->ndofs;
+    out[7] = m->nu;
+    return AMR_OK;
+}
+
+// ----------------------------------------------------------------- state
+
+extern "C" int amr_state_set(amr_model *m, const double *st)
+{
+    if (!m || !st) return AMR_ERR_ARG;
+    AMR_HIP(hipSetDevice(m->device));
+    const int64_t n = m->nip_total;
+    std::vector<double> soa((size_t)n * AMR_NSTATE);
+    for (int64_t i = 0; i < n; i++)
+        for (int f = 0; f < AMR_NSTATE; f++) soa[f * n + i] = st[i * AMR_NSTATE + f];
+    TRY(h2d(m->st_cur, soa.data(), soa.size(), m->stream));
+    TRY(h2d(m->st_com, soa.data(), soa.size(), m->stream));
+    AMR_HIP(hipStreamSynchronize(m->stream));
+    return AMR_OK;
+}
+
+extern "C" int amr_state_get(amr_model *m, double *st, int committed)
+{
+    if (!m || !st) return AMR_ERR_ARG;
+    AMR_HIP(hipSetDevice(m->device));
+    const int64_t n = m->nip_total;
+    std::vector<double> soa((size_t)n * AMR_NSTATE);
+    TRY(d2h(soa.data(), committed ? m->st_com : m->st_cur, soa.size(), m->stream));
+    AMR_HIP(hipStreamSynchronize(m->stream));
+    for (int64_t i = 0; i < n; i++)
+        for (int f = 0; f < AMR_NSTATE; f++) st[i * AMR_NSTATE + f] = soa[f * n + i];
+    return AMR_OK;
+}
+
+extern "C" int amr_state_commit(amr_model *m)
+{
+    if (!m) return AMR_ERR_ARG;
+    AMR_HIP(hipSetDevice(m->device));
+    AMR_HIP(hipMemcpyAsync(m->st_com, m->st_cur, sizeof(double) * AMR_NSTATE * m->nip_total,
+                           hipMemcpyDeviceToDevice, m->stream));
+    return AMR_OK;
+}
+
+extern "C" int amr_state_restore(amr_model *m)
+{
+    if (!m) return AMR_ERR_ARG;
+    AMR_HIP(hipSetDevice(m->device));
+    AMR_HIP(hipMemcpyAsync(m->st_cur, m->st_com, sizeof(double) * AMR_NSTATE * m->nip_total,
+                           hipMemcpyDeviceToDevice, m->stream));
+    return AMR_OK;
+}
+
+// -------------------------------------------------------------- mount_K
+
+extern "C" int amr_assemble(amr_model *m)
+{
+    if (!m) return AMR_ERR_ARG;
+    AMR_HIP(hipSetDevice(m->device));
+    AMR_HIP(hipMemsetAsync(m->err, 0x7f, sizeof(int32_t), m->stream));  // "no bad element"
+    TRY(amr_launch_ke(m));
+    TRY(amr_launch_assemble(m, true));
+    int32_t err = 0;
+    TRY(d2h(&err, m->err, 1, m->stream));
+    AMR_HIP(hipStreamSynchronize(m->stream));
+    if (err != 0x7f7f7f7f) {
+        amr_set_error("Negative Jacobian determinant in cell " + std::to_string(err));
+        return AMR_ERR_JACOBIAN;
+    }
+    m->assembled = true;
+    return AMR_OK;
+}
+
+extern "C" int amr_get_Ke(amr_model *m, double *Ke)
+{
+    if (!m || !Ke) return AMR_ERR_ARG;
+    AMR_HIP(hipSetDevice(m->device));
+    const int nn = m->nn, nd = 3 * nn;
+    std::vector<double> blk((size_t)m->ne * nn * nn * 9);
+    TRY(d2h(blk.data(), m->Ke, blk.size(), m->stream));
+    AMR_HIP(hipStreamSynchronize(m->stream));
+    for (int64_t e = 0; e < m->ne; e++)
+        for (int a = 0; a < nn; a++)
+            for (int b = 0; b < nn; b++)
+                for (int r = 0; r < 3; r++)
+                    for (int c = 0; c < 3; c++)
+                        Ke[(e * nd + 3 * a + r) * nd + 3 * b + c] = blk[((e * nn + a) * nn + b) * 9 + r * 3 + c];
+    return AMR_OK;
+}
+
+extern "C" int amr_get_K_csc(amr_model *m, int64_t *nnz, int64_t *colptr, int32_t *rowval, double *nzval)
+{
+    if (!m || !nnz) return AMR_ERR_ARG;
+    if (!m->assembled) {
+        amr_set_error("amr_get_K_csc: K has not been assembled");
+        return AMR_ERR_ARG;
+    }
+    AMR_HIP(hipSetDevice(m->device));
+    const int64_t nb = m->pat.nblk;
+    std::vector<double> val((size_t)nb * 9);
+    std::vector<uint8_t> fl((size_t)nb * 9);
+    TRY(d2h(val.data(), m->Kv, val.size(), m->stream));
+    TRY(d2h(fl.data(), m->Kflag, fl.size(), m->stream));
+    AMR_HIP(hipStreamSynchronize(m->stream));
+    std::vector<int64_t> cnt(m->ndofs + 1, 0);
+    int64_t total = 0;
+    for (int64_t A = 0; A < m->nnodes; A++)
+        for (int32_t k = m->pat.rowptr[A]; k < m->pat.rowptr[A + 1]; k++) {
+            const int64_t B = m->pat.col[k];
+            for (int r = 0; r < 3; r++)
+                for (int c = 0; c < 3; c++) {
+                    if (!fl[(size_t)k * 9 + r * 3 + c]) continue;
+                    const int32_t i = m->eq[A * 3 + r], j = m->eq[B * 3 + c];
+                    if (i < 0 || j < 0) continue;
+                    cnt[j + 1]++;
+                    total++;
+                }
+        }
+    *nnz = total;
+    if (!colptr) return AMR_OK;
+    if (!rowval || !nzval) return AMR_ERR_ARG;
+    for (int64_t j = 0; j < m->ndofs; j++) cnt[j + 1] += cnt[j];
+    std::copy(cnt.begin(), cnt.end(), colptr);
+    std::vector<int64_t> pos(cnt.begin(), cnt.end() - 1);
+    for (int64_t A = 0; A < m->nnodes; A++)
+        for (int32_t k = m->pat.rowptr[A]; k < m->pat.rowptr[A + 1]; k++) {
+            const int64_t B = m->pat.col[k];
+            for (int r = 0; r < 3; r++)
+                for (int c = 0; c < 3; c++) {
+                    if (!fl[(size_t)k * 9 + r * 3 + c]) continue;
+                    const int32_t i = m->eq[A * 3 + r], j = m->eq[B * 3 + c];
+                    if (i < 0 || j < 0) continue;
+                    int64_t p = pos[j]++;
+                    rowval[p] = i;
+                    nzval[p] = val[(size_t)k * 9 + r * 3 + c];
+                }
+        }
+    // sort row ids within each column
+    std::vector<std::pair<int32_t, double>> tmp;
+    for (int64_t j = 0; j < m->ndofs; j++) {
+        const int64_t b = colptr[j], e = colptr[j + 1];
+        tmp.resize(e - b);
+        for (int64_t p = b; p < e; p++) tmp[p - b] = {rowval[p], nzval[p]};
+        std::sort(tmp.begin(), tmp.end(),
+                  [](const std::pair<int32_t, double> &x, const std::pair<int32_t, double> &y) {
+                      return x.first < y.first;
+                  });
+        for (int64_t p = b; p < e; p++) {
+            rowval[p] = tmp[p - b].first;
+            nzval[p] = tmp[p - b].second;
+        }
+    }
+    return AMR_OK;
+}
+
+// ------------------------------------------------------- solve_system!
+
+static void eq_to_node(const amr_model *m, const double *eqv, std::vector<double> &node)
+{
+    node.assign(m->nnodes * 3, 0.0);
+    for (int64_t k = 0; k < m->nnodes * 3; k++)
+        if (m->eq[k] >= 0) node[k] = eqv[m->eq[k]];
+}
+
+static void node_to_eq(const amr_model *m, const std::vector<double> &node, double *eqv)
+{
+    for (int64_t k = 0; k < m->nnodes * 3; k++)
+        if (m->eq[k] >= 0) eqv[m->eq[k]] = node[k];
+}
+
+__global__ static void prep_rhs_kernel(int64_t n, const double *__restrict__ mask, const double *__restrict__ u,
+                                       double *__restrict__ u2)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) u2[i] = mask[i] != 0.0 ? 0.0 : u[i];
+}
+
+__global__ static void rhs_kernel(int64_t n, const double *__restrict__ mask, const double *__restrict__ f,
+                                  const double *__restrict__ ky, double *__restrict__ b)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = mask[i] != 0.0 ? f[i] - ky[i] : 0.0;
+}
+
+__global__ static void combine_kernel(int64_t n, const double *__restrict__ mask, const double *__restrict__ x,
+                                      double *__restrict__ u2)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && mask[i] != 0.0) u2[i] = x[i];
+}
+
+static amr_solve_opts default_opts()
+{
+    amr_solve_opts o;
+    o.precision = AMR_PREC_FP64;
+    o.precond = AMR_PRECOND_JACOBI;
+    o.maxiter = 200000;
+    o.check_every = 32;
+    o.rtol = 1e-12;
+    o.inner_rtol = 1e-4;
+    return o;
+}
+
+extern "C" int amr_solve_system(amr_model *m, double *U, double *F, const amr_solve_opts *opts,
+                                amr_solve_info *info)
+{
+    if (!m || !U || !F) return AMR_ERR_ARG;
+    if (!m->assembled) {
+        amr_set_error("amr_solve_system: K has not been assembled");
+        return AMR_ERR_ARG;
+    }
+    AMR_HIP(hipSetDevice(m->device));
+    amr_solve_opts o = opts ? *opts : default_opts();
+    const int64_t n = m->nnodes * 3;
+    std::vector<double> un, fn;
+    eq_to_node(m, U, un);
+    eq_to_node(m, F, fn);
+    double *du = m->v[6], *df = m->v[7], *u2 = m->v[8], *ky = m->v[9], *b = m->v[10], *x = m->v[11];
+    hipStream_t s = m->stream;
+    TRY(h2d(du, un.data(), (size_t)n, s));
+    TRY(h2d(df, fn.data(), (size_t)n, s));
+    const unsigned g = (unsigned)((n + 255) / 256);
+    hipLaunchKernelGGL(prep_rhs_kernel, dim3(g), dim3(256), 0, s, n, m->mask, du, u2);
+    TRY(amr_spmv_full(m, u2, ky, 0));
+    hipLaunchKernelGGL(rhs_kernel, dim3(g), dim3(256), 0, s, n, m->mask, df, ky, b);
+    AMR_HIP(hipGetLastError());
+    amr_solve_info inf{};
+    int rc = AMR_OK;
+    if (m->nu > 0) rc = amr_pcg_solve(m, b, x, &o, &inf);
+    if (info) *info = inf;
+    if (rc) return rc;
+    if (m->nu > 0) hipLaunchKernelGGL(combine_kernel, dim3(g), dim3(256), 0, s, n, m->mask, x, u2);
+    TRY(amr_spmv_full(m, u2, ky, 0));
+    std::vector<double> uo(n), ko(n);
+    TRY(d2h(uo.data(), u2, (size_t)n, s));
+    TRY(d2h(ko.data(), ky, (size_t)n, s));
+    AMR_HIP(hipStreamSynchronize(s));
+    double umax = 0.0;
+    for (int64_t k = 0; k < n; k++)
+        if (m->free_h[k]) {
+            if (!std::isfinite(uo[k])) umax = INFINITY;
+            umax = std::max(umax, std::fabs(uo[k]));
+        }
+    if (umax > 1e8) {
+        amr_set_error("\nsolve_system!: Possible syngular matrix");
+        return AMR_ERR_SOLVE;
+    }
+    for (int64_t k = 0; k < n; k++) {
+        const int32_t q = m->eq[k];
+        if (q < 0) continue;
+        if (m->free_h[k]) U[q] = uo[k];
+        else F[q] = ko[k];
+    }
+    return AMR_OK;
+}
+
+// -------------------------------------------------------- update_state!
+
+extern "C" int amr_update_state(amr_model *m, const double *dUt, double *dFin)
+{
+    if (!m || !dUt || !dFin) return AMR_ERR_ARG;
+    AMR_HIP(hipSetDevice(m->device));
+    const int64_t n = m->nnodes * 3;
+    std::vector<double> un;
+    eq_to_node(m, dUt, un);
+    double *du = m->v[6], *fo = m->v[7];
+    TRY(h2d(du, un.data(), (size_t)n, m->stream));
+    TRY(amr_launch_update(m, du, fo));
+    std::vector<double> fn(n);
+    std::vector<int32_t> st(m->ne);
+    TRY(d2h(fn.data(), fo, (size_t)n, m->stream));
+    TRY(d2h(st.data(), m->estat, (size_t)m->ne, m->stream));
+    AMR_HIP(hipStreamSynchronize(m->stream));
+    std::fill(dFin, dFin + m->ndofs, 0.0);
+    node_to_eq(m, fn, dFin);
+    for (int64_t e = 0; e < m->ne; e++)
+        if (st[e]) {
+            if (st[e] == 2) {
+                amr_set_error("Negative Jacobian determinant in cell " + std::to_string(e + 1));
+                return AMR_ERR_JACOBIAN;
+            }
+            amr_set_error("VonMisses: Negative value for √J2D (cell " + std::to_string(e + 1) + ")");
+            return AMR_ERR_MATERIAL;
+        }
+    for (int64_t i = 0; i < m->ndofs; i++)
+        if (std::isnan(dFin[i])) {
+            amr_set_error("solve_system!: NaN values in internal forces vector");
+            return AMR_ERR_NAN;
+        }
+    return AMR_OK;
+}

@@ -1,0 +1,620 @@
+// Element-level HIP kernels for gfx950: element stiffness, deterministic
+// assembly gather, per-IP constitutive update and internal-force gather.
+//
+// Floating-point order.  Every kernel here evaluates its sums in one fixed
+// order: dot products are fma chains in ascending index order starting from
+// 0, K += coef*t and dF += coef*t are a single fma, divisions stay divisions,
+// and the library is compiled with -ffp-contract=off.  Structural zeros of
+// the Mandel B matrix are skipped; an fma with a zero factor leaves a
+// non-zero accumulator unchanged, so skipping them does not change a bit.
+#include <cmath>
+
+#include "amr_internal.h"
+
+namespace {
+
+constexpr double SR2 = 1.4142135623730951;  // sqrt(2.0)
+constexpr double DEPS = 2.220446049250313e-16;  // eps(Float64)
+
+// ------------------------------------------------------------- materials
+
+// Elastic Mandel stiffness, src/mech/mat/linear-elastic.jl:110-117.
+__device__ __host__ inline void calc_De(double E, double nu, double *D)
+{
+    double c = E / ((1.0 + nu) * (1.0 - 2.0 * nu));
+    double a = c * (1.0 - nu), b = c * nu, g = c * (1.0 - 2.0 * nu);
+    for (int i = 0; i < 36; i++) D[i] = 0.0;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) D[i * 6 + j] = (i == j) ? a : b;
+    for (int i = 3; i < 6; i++) D[i * 6 + i] = g;
+}
+
+__device__ inline void mat6_vec(const double *A, const double *x, double *y)
+{
+    for (int i = 0; i < 6; i++) {
+        double acc = 0.0;
+        for (int j = 0; j < 6; j++) acc = fma(A[i * 6 + j], x[j], acc);
+        y[i] = acc;
+    }
+}
+
+__device__ inline double dot6(const double *a, const double *b)
+{
+    double acc = 0.0;
+    for (int i = 0; i < 6; i++) acc = fma(a[i], b[i], acc);
+    return acc;
+}
+
+// deviatoric part via the 6x6 projector product (src/tools/tensors.jl:10-18)
+__device__ inline void deviator(const double *x, double *s)
+{
+    const double d = 2.0 / 3.0, o = -1.0 / 3.0;
+    s[0] = fma(o, x[2], fma(o, x[1], d * x[0]));
+    s[1] = fma(o, x[2], fma(d, x[1], o * x[0]));
+    s[2] = fma(d, x[2], fma(o, x[1], o * x[0]));
+    s[3] = x[3];
+    s[4] = x[4];
+    s[5] = x[5];
+}
+
+// J2 of a Mandel vector (src/tools/tensors.jl:24-27)
+__device__ inline double j2_of(const double *x)
+{
+    double t23 = x[3] / SR2, t13 = x[4] / SR2, t12 = x[5] / SR2;
+    double a = x[0] - x[1], b = x[1] - x[2], c = x[2] - x[0];
+    return (1.0 / 6.0) * (a * a + b * b + c * c) + t23 * t23 + t13 * t13 + t12 * t12;
+}
+
+// calcD for LE / VM 3D / DP (src/mech/mat/linear-elastic.jl:125-127,
+// von-mises.jl:112-125, drucker-prager.jl:86-109).  st: sig[6] .. dlam.
+__device__ void calc_D(const AmrMat &m, const double *st, double *D)
+{
+    calc_De(m.p[0], m.p[1], D);
+    if (m.kind == AMR_MAT_LE || st[13] == 0.0) return;
+    double De[36];
+    for (int i = 0; i < 36; i++) De[i] = D[i];
+    if (m.kind == AMR_MAT_VM) {
+        double s[6], n[6], v[6];
+        deviator(st, s);
+        double nrm = sqrt(dot6(s, s));
+        for (int i = 0; i < 6; i++) n[i] = (sqrt(1.5) * s[i]) / nrm;
+        mat6_vec(De, n, v);
+        double den = dot6(n, v) - sqrt(1.5) * (-m.p[3]);
+        for (int i = 0; i < 6; i++)
+            for (int j = 0; j < 6; j++) D[i * 6 + j] = De[i * 6 + j] - (v[i] * v[j]) / den;
+    } else {
+        double alpha = m.p[2], H = m.p[4];
+        double Nu[6], V[6], a[6], b[6];
+        const double I2[6] = {1, 1, 1, 0, 0, 0};
+        if (j2_of(st) != 0.0) {
+            double s[6];
+            deviator(st, s);
+            double ns = sqrt(dot6(s, s));
+            for (int i = 0; i < 6; i++) V[i] = alpha * I2[i] + (s[i] / ns) / SR2;
+            double nv = sqrt(dot6(V, V));
+            for (int i = 0; i < 6; i++) Nu[i] = V[i] / nv;
+        } else {
+            for (int i = 0; i < 6; i++) Nu[i] = V[i] = (1.0 / sqrt(3.0)) * I2[i];
+        }
+        mat6_vec(De, Nu, a);
+        mat6_vec(De, V, b);
+        double den = dot6(b, Nu) + H;
+        for (int i = 0; i < 6; i++)
+            for (int j = 0; j < 6; j++) D[i * 6 + j] = De[i * 6 + j] - (a[i] * b[j]) / den;
+    }
+}
+
+// update_state! for LE / VM 3D / DP (src/mech/mat/linear-elastic.jl:130-137,
+// von-mises.jl:128-156, drucker-prager.jl:112-149).  st is updated in place;
+// returns 0, or 1 on a material failure (VM "negative sqrt(J2D)").
+__device__ int update_ip(const AmrMat &m, double *st, const double *de, double *ds)
+{
+    double De[36], sini[6], str[6], t[6];
+    calc_De(m.p[0], m.p[1], De);
+    for (int i = 0; i < 6; i++) sini[i] = st[i];
+    mat6_vec(De, de, t);
+    for (int i = 0; i < 6; i++) str[i] = st[i] + t[i];
+    if (m.kind == AMR_MAT_LE) {
+        for (int i = 0; i < 6; i++) {
+            st[6 + i] += de[i];
+            st[i] = str[i];
+            ds[i] = t[i];
+        }
+        return 0;
+    }
+    if (m.kind == AMR_MAT_VM) {
+        double E = m.p[0], nu = m.p[1], fy = m.p[2], H = m.p[3];
+        double j2 = j2_of(str);
+        double ftr = sqrt(3.0 * j2) - fy - H * st[12];
+        if (ftr < 1e-8) {
+            st[13] = 0.0;
+            for (int i = 0; i < 6; i++) st[i] = str[i];
+        } else {
+            double G = E / (2.0 * (1.0 + nu));
+            double dl = ftr / (3.0 * G + sqrt(1.5) * H);
+            st[13] = dl;
+            double sj = sqrt(j2);
+            if (!(sj - dl * sqrt(3.0) * G >= 0.0)) return 1;
+            double f = 1.0 - sqrt(3.0) * G * dl / sj;
+            double s[6];
+            deviator(str, s);
+            for (int i = 0; i < 6; i++) s[i] = f * s[i];
+            double ns = sqrt(dot6(s, s));
+            double c = sqrt(6.0) * G * dl;
+            for (int i = 0; i < 6; i++) st[i] = str[i] - (c * s[i]) / ns;
+            st[12] += dl;
+        }
+    } else {
+        double E = m.p[0], nu = m.p[1], alpha = m.p[2], kappa = m.p[3], H = m.p[4];
+        double j1 = str[0] + str[1] + str[2];
+        double j2 = j2_of(str);
+        double ftr = alpha * j1 + sqrt(j2) - kappa - H * st[12];
+        if (ftr < 1e-8) {
+            st[13] = 0.0;
+            for (int i = 0; i < 6; i++) st[i] = str[i];
+        } else {
+            double K = E / (3.0 * (1.0 - 2.0 * nu)), G = E / (2.0 * (1.0 + nu));
+            double n = 1.0 / sqrt(3.0 * alpha * alpha + 0.5);
+            double sj = sqrt(j2);
+            if (sj - st[13] * n * G > 0.0) {  // previous dgamma, as the reference
+                double dg = ftr / (9.0 * alpha * alpha * n * K + n * G + H);
+                st[13] = dg;
+                double j1n = j1 - 9.0 * dg * alpha * n * K;
+                double mm = 1.0 - dg * n * G / sj;
+                double s[6];
+                deviator(str, s);
+                for (int i = 0; i < 6; i++) st[i] = mm * s[i] + (i < 3 ? j1n / 3.0 : 0.0);
+            } else {
+                double dg = (alpha * j1 - kappa - H * st[12]) / (3.0 * sqrt(3.0) * alpha * K + H);
+                st[13] = dg;
+                double j1n = j1 - 3.0 * sqrt(3.0) * dg * K;
+                for (int i = 0; i < 6; i++) st[i] = (i < 3 ? j1n / 3.0 : 0.0);
+            }
+            st[12] += st[13];
+        }
+    }
+    for (int i = 0; i < 6; i++) {
+        st[6 + i] += de[i];
+        ds[i] = st[i] - sini[i];
+    }
+    return 0;
+}
+
+// Jacobian inverse by cofactors; returns det.  Same expressions as the
+// oracle's ip_geometry.
+__device__ inline double inv3(const double *J, double *inv)
+{
+    double c00 = J[4] * J[8] - J[5] * J[7];
+    double c01 = J[5] * J[6] - J[3] * J[8];
+    double c02 = J[3] * J[7] - J[4] * J[6];
+    double det = J[0] * c00 + J[1] * c01 + J[2] * c02;
+    inv[0] = c00 / det;
+    inv[3] = c01 / det;
+    inv[6] = c02 / det;
+    inv[1] = (J[2] * J[7] - J[1] * J[8]) / det;
+    inv[4] = (J[0] * J[8] - J[2] * J[6]) / det;
+    inv[7] = (J[1] * J[6] - J[0] * J[7]) / det;
+    inv[2] = (J[1] * J[5] - J[2] * J[4]) / det;
+    inv[5] = (J[2] * J[3] - J[0] * J[5]) / det;
+    inv[8] = (J[0] * J[4] - J[1] * J[3]) / det;
+    return det;
+}
+
+__device__ inline double inv3_entry(const double *J, int k, double *det_out)
+{
+    double c00 = J[4] * J[8] - J[5] * J[7];
+    double c01 = J[5] * J[6] - J[3] * J[8];
+    double c02 = J[3] * J[7] - J[4] * J[6];
+    double det = J[0] * c00 + J[1] * c01 + J[2] * c02;
+    *det_out = det;
+    double num;
+    switch (k) {
+    case 0: num = c00; break;
+    case 3: num = c01; break;
+    case 6: num = c02; break;
+    case 1: num = J[2] * J[7] - J[1] * J[8]; break;
+    case 4: num = J[0] * J[8] - J[2] * J[6]; break;
+    case 7: num = J[1] * J[6] - J[0] * J[7]; break;
+    case 2: num = J[1] * J[5] - J[2] * J[4]; break;
+    case 5: num = J[2] * J[3] - J[0] * J[5]; break;
+    default: num = J[0] * J[4] - J[1] * J[3]; break;
+    }
+    return num / det;
+}
+
+// ---------------------------------------------------- element stiffness
+//
+// One 64-lane workgroup per element.  Lane (ch, b) = lane ch*NN + b owns the
+// node blocks (a, b) for a in chunk ch (ACH consecutive row nodes) and keeps
+// their 9*ACH entries in registers across all IPs.  Per IP: 9 lanes build J
+// and its inverse, NN lanes build dN/dX (and /sqrt2) into LDS, one lane
+// evaluates calcD for plastic states, then every lane forms DB for its
+// column node (54 fma) and updates its blocks (36 fma per block).
+
+template <int NN, int NQ, int ACH>
+__global__ void __launch_bounds__(64) ke_kernel(int64_t ne, const double *__restrict__ X,
+                                                const int32_t *__restrict__ conn,
+                                                const int32_t *__restrict__ emat,
+                                                const AmrMat *__restrict__ mats,
+                                                const double *__restrict__ dNdR,
+                                                const double *__restrict__ wip,
+                                                const double *__restrict__ st, int64_t nip,
+                                                int all_elastic, double *__restrict__ Ke,
+                                                int32_t *__restrict__ err)
+{
+    constexpr int NCH = (NN + ACH - 1) / ACH;
+    const int64_t e = blockIdx.x;
+    const int lane = threadIdx.x;
+    __shared__ double sC[NN * 3];
+    __shared__ double sJ[9], sInv[9], sCoef;
+    __shared__ double sG[NN * 6];
+    __shared__ double sD[36];
+    __shared__ int sBad;
+
+    for (int i = lane; i < NN * 3; i += 64) sC[i] = X[(int64_t)conn[e * NN + i / 3] * 3 + i % 3];
+    const AmrMat m = mats[emat[e]];
+    if (lane == 0) {
+        sBad = 0;
+        if (all_elastic || m.kind == AMR_MAT_LE) calc_De(m.p[0], m.p[1], sD);
+    }
+    const bool elastic = all_elastic || m.kind == AMR_MAT_LE;
+    const int b = lane % NN, ch = lane / NN;
+    const bool active = lane < NN * NCH;
+    double acc[ACH][9];
+#pragma unroll
+    for (int i = 0; i < ACH; i++)
+#pragma unroll
+        for (int s = 0; s < 9; s++) acc[i][s] = 0.0;
+    __syncthreads();
+
+    for (int q = 0; q < NQ; q++) {
+        const double *dq = dNdR + q * NN * 3;
+        if (lane < 9) {
+            int i = lane / 3, j = lane % 3;
+            double a = 0.0;
+            for (int n = 0; n < NN; n++) a = fma(sC[n * 3 + i], dq[n * 3 + j], a);
+            sJ[lane] = a;
+        }
+        __syncthreads();
+        if (lane < 9) {
+            double det;
+            sInv[lane] = inv3_entry(sJ, lane, &det);
+            if (lane == 0) {
+                sCoef = det * wip[q];
+                if (!(det > 0.0)) sBad = 1;
+            }
+        } else if (lane == 32 && !elastic) {
+            double s14[AMR_NSTATE];
+            int64_t ip = e * NQ + q;
+            for (int f = 0; f < AMR_NSTATE; f++) s14[f] = st[f * nip + ip];
+            calc_D(m, s14, sD);
+        }
+        __syncthreads();
+        if (lane < NN) {
+            double g[3];
+            for (int j = 0; j < 3; j++) {
+                double a = 0.0;
+                for (int i = 0; i < 3; i++) a = fma(dq[lane * 3 + i], sInv[i * 3 + j], a);
+                g[j] = a;
+            }
+            sG[lane * 6 + 0] = g[0];
+            sG[lane * 6 + 1] = g[1];
+            sG[lane * 6 + 2] = g[2];
+            sG[lane * 6 + 3] = g[0] / SR2;
+            sG[lane * 6 + 4] = g[1] / SR2;
+            sG[lane * 6 + 5] = g[2] / SR2;
+        }
+        __syncthreads();
+        if (active) {
+            const double coef = sCoef;
+            const double gx = sG[b * 6], gy = sG[b * 6 + 1], gz = sG[b * 6 + 2];
+            const double hx = sG[b * 6 + 3], hy = sG[b * 6 + 4], hz = sG[b * 6 + 5];
+            double DB[6][3];
+#pragma unroll
+            for (int k = 0; k < 6; k++) {
+                DB[k][0] = fma(sD[k * 6 + 5], hy, fma(sD[k * 6 + 4], hz, fma(sD[k * 6 + 0], gx, 0.0)));
+                DB[k][1] = fma(sD[k * 6 + 5], hx, fma(sD[k * 6 + 3], hz, fma(sD[k * 6 + 1], gy, 0.0)));
+                DB[k][2] = fma(sD[k * 6 + 4], hx, fma(sD[k * 6 + 3], hy, fma(sD[k * 6 + 2], gz, 0.0)));
+            }
+#pragma unroll
+            for (int ia = 0; ia < ACH; ia++) {
+                const int a = ch * ACH + ia;
+                if (a < NN) {
+                    const double ax = sG[a * 6], ay = sG[a * 6 + 1], az = sG[a * 6 + 2];
+                    const double bx = sG[a * 6 + 3], by = sG[a * 6 + 4], bz = sG[a * 6 + 5];
+#pragma unroll
+                    for (int c = 0; c < 3; c++) {
+                        // row r=0 (x): B rows 0 (ax), 4 (bz), 5 (by)
+                        double t0 = fma(by, DB[5][c], fma(bz, DB[4][c], fma(ax, DB[0][c], 0.0)));
+                        // r=1 (y): rows 1 (ay), 3 (bz), 5 (bx)
+                        double t1 = fma(bx, DB[5][c], fma(bz, DB[3][c], fma(ay, DB[1][c], 0.0)));
+                        // r=2 (z): rows 2 (az), 3 (by), 4 (bx)
+                        double t2 = fma(bx, DB[4][c], fma(by, DB[3][c], fma(az, DB[2][c], 0.0)));
+                        acc[ia][0 * 3 + c] = fma(coef, t0, acc[ia][0 * 3 + c]);
+                        acc[ia][1 * 3 + c] = fma(coef, t1, acc[ia][1 * 3 + c]);
+                        acc[ia][2 * 3 + c] = fma(coef, t2, acc[ia][2 * 3 + c]);
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (active) {
+#pragma unroll
+        for (int ia = 0; ia < ACH; ia++) {
+            const int a = ch * ACH + ia;
+            if (a < NN) {
+                double *o = Ke + ((e * NN + a) * NN + b) * 9;
+#pragma unroll
+                for (int s = 0; s < 9; s++) o[s] = acc[ia][s];
+            }
+        }
+    }
+    if (lane == 0 && sBad) atomicMin(err, (int32_t)(e + 1));
+}
+
+// --------------------------------------------------------------- assembly
+//
+// One thread per K block: sum the element blocks listed for it in element
+// order, each scalar only if |v| >= eps (mount_K's filter), starting from
+// 0.0 (0.0 + v1 == v1).  Writes FP64 and FP32 values and, when asked, the
+// survivor flags that define the reference's scalar pattern.
+__global__ void __launch_bounds__(256) assemble_kernel(int64_t nblk, const int64_t *__restrict__ cptr,
+                                                       const int32_t *__restrict__ csrc,
+                                                       const double *__restrict__ Ke,
+                                                       double *__restrict__ Kv, float *__restrict__ Kv32,
+                                                       uint8_t *__restrict__ flags)
+{
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nblk) return;
+    double acc[9];
+    unsigned fl = 0;
+#pragma unroll
+    for (int s = 0; s < 9; s++) acc[s] = 0.0;
+    for (int64_t p = cptr[k]; p < cptr[k + 1]; p++) {
+        const double *v = Ke + (int64_t)csrc[p] * 9;
+#pragma unroll
+        for (int s = 0; s < 9; s++) {
+            double x = v[s];
+            if (!(fabs(x) < DEPS)) {
+                acc[s] += x;
+                fl |= 1u << s;
+            }
+        }
+    }
+#pragma unroll
+    for (int s = 0; s < 9; s++) {
+        Kv[k * 9 + s] = acc[s];
+        Kv32[k * 9 + s] = (float)acc[s];
+    }
+    if (flags) {
+#pragma unroll
+        for (int s = 0; s < 9; s++) flags[k * 9 + s] = (fl >> s) & 1u;
+    }
+}
+
+// Jacobi inverse diagonal (0 at prescribed dofs)
+__global__ void jacobi_kernel(int64_t nnodes, const int32_t *__restrict__ diag, const double *__restrict__ Kv,
+                              const double *__restrict__ mask, double *__restrict__ dinv,
+                              float *__restrict__ dinv32)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nnodes * 3) return;
+    int64_t n = i / 3;
+    int d = (int)(i % 3);
+    const int32_t db = diag[n];  // -1 for a node in no element (it has no dofs)
+    double kd = db >= 0 ? Kv[(int64_t)db * 9 + d * 4] : 1.0;
+    bool fr = mask[i] != 0.0 && db >= 0;
+    dinv[i] = fr ? 1.0 / kd : 0.0;
+    dinv32[i] = fr ? 1.0f / (float)kd : 0.0f;
+}
+
+// ------------------------------------------------------------- IP update
+//
+// EPW elements per 256-thread workgroup.  Phase A: one thread per IP builds
+// J, its inverse and coef, streams the strain increment deps = B dU over the
+// element's nodes (the oracle's column-ascending fma chains), runs the
+// constitutive update from the committed state into the current state and
+// keeps (inv J, coef, dsig) in LDS.  Phase B: one thread per element column
+// accumulates dF[c] = fma(coef_q, (B' dsig_q)[c], dF[c]) over the IPs in
+// order.
+template <int NN, int NQ, int EPW>
+__global__ void __launch_bounds__(256) update_kernel(int64_t ne, const double *__restrict__ X,
+                                                     const int32_t *__restrict__ conn,
+                                                     const int32_t *__restrict__ emat,
+                                                     const AmrMat *__restrict__ mats,
+                                                     const double *__restrict__ dNdR,
+                                                     const double *__restrict__ wip,
+                                                     const double *__restrict__ st_in,
+                                                     double *__restrict__ st_out, int64_t nip,
+                                                     const double *__restrict__ dU,
+                                                     double *__restrict__ dF,
+                                                     int32_t *__restrict__ estat)
+{
+    constexpr int ND = 3 * NN;
+    __shared__ double sC[EPW][NN * 3];
+    __shared__ double sU[EPW][ND];
+    __shared__ double sInv[EPW * NQ][9];
+    __shared__ double sCoef[EPW * NQ];
+    __shared__ double sDs[EPW * NQ][6];
+    __shared__ int sStat[EPW * NQ];
+    const int64_t e0 = (int64_t)blockIdx.x * EPW;
+    const int tid = threadIdx.x;
+
+    for (int i = tid; i < EPW * NN * 3; i += blockDim.x) {
+        int el = i / (NN * 3), r = i % (NN * 3);
+        int64_t e = e0 + el;
+        if (e < ne) {
+            int64_t node = conn[e * NN + r / 3];
+            sC[el][r] = X[node * 3 + r % 3];
+            sU[el][r] = dU[node * 3 + r % 3];
+        }
+    }
+    __syncthreads();
+
+    if (tid < EPW * NQ) {
+        const int el = tid / NQ, q = tid % NQ;
+        const int64_t e = e0 + el;
+        int stat = 0;
+        if (e < ne) {
+            const double *dq = dNdR + q * NN * 3;
+            double J[9], inv[9];
+            for (int i = 0; i < 3; i++)
+                for (int j = 0; j < 3; j++) {
+                    double a = 0.0;
+                    for (int n = 0; n < NN; n++) a = fma(sC[el][n * 3 + i], dq[n * 3 + j], a);
+                    J[i * 3 + j] = a;
+                }
+            double det = inv3(J, inv);
+            if (!(det > 0.0)) stat = 2;
+            double de[6] = {0, 0, 0, 0, 0, 0};
+            for (int n = 0; n < NN; n++) {
+                double g[3];
+                for (int j = 0; j < 3; j++) {
+                    double a = 0.0;
+                    for (int i = 0; i < 3; i++) a = fma(dq[n * 3 + i], inv[i * 3 + j], a);
+                    g[j] = a;
+                }
+                const double hx = g[0] / SR2, hy = g[1] / SR2, hz = g[2] / SR2;
+                const double u0 = sU[el][n * 3], u1 = sU[el][n * 3 + 1], u2 = sU[el][n * 3 + 2];
+                de[0] = fma(g[0], u0, de[0]);
+                de[1] = fma(g[1], u1, de[1]);
+                de[2] = fma(g[2], u2, de[2]);
+                de[3] = fma(hy, u2, fma(hz, u1, de[3]));
+                de[4] = fma(hx, u2, fma(hz, u0, de[4]));
+                de[5] = fma(hx, u1, fma(hy, u0, de[5]));
+            }
+            const int64_t ip = e * NQ + q;
+            double s14[AMR_NSTATE], ds[6];
+            for (int f = 0; f < AMR_NSTATE; f++) s14[f] = st_in[f * nip + ip];
+            const AmrMat m = mats[emat[e]];
+            if (update_ip(m, s14, de, ds)) stat = stat ? stat : 1;
+            for (int f = 0; f < AMR_NSTATE; f++) st_out[f * nip + ip] = s14[f];
+            for (int i = 0; i < 9; i++) sInv[tid][i] = inv[i];
+            sCoef[tid] = det * wip[q];
+            for (int k = 0; k < 6; k++) sDs[tid][k] = ds[k];
+        }
+        sStat[tid] = stat;
+    }
+    __syncthreads();
+
+    for (int it = tid; it < EPW * ND; it += blockDim.x) {
+        const int el = it / ND, c = it % ND;
+        const int64_t e = e0 + el;
+        if (e >= ne) continue;
+        const int n = c / 3, d = c % 3;
+        double F = 0.0;
+        for (int q = 0; q < NQ; q++) {
+            const int slot = el * NQ + q;
+            const double *dq = dNdR + q * NN * 3;
+            double g[3];
+            for (int j = 0; j < 3; j++) {
+                double a = 0.0;
+                for (int i = 0; i < 3; i++) a = fma(dq[n * 3 + i], sInv[slot][i * 3 + j], a);
+                g[j] = a;
+            }
+            const double *s = sDs[slot];
+            double t;
+            if (d == 0) t = fma(g[1] / SR2, s[5], fma(g[2] / SR2, s[4], fma(g[0], s[0], 0.0)));
+            else if (d == 1) t = fma(g[0] / SR2, s[5], fma(g[2] / SR2, s[3], fma(g[1], s[1], 0.0)));
+            else t = fma(g[0] / SR2, s[4], fma(g[1] / SR2, s[3], fma(g[2], s[2], 0.0)));
+            F = fma(sCoef[slot], t, F);
+        }
+        dF[e * ND + c] = F;
+    }
+    if (tid < EPW) {
+        const int64_t e = e0 + tid;
+        if (e < ne) {
+            int s = 0;
+            for (int q = 0; q < NQ; q++) {
+                int v = sStat[tid * NQ + q];
+                if (v) {
+                    s = v;
+                    break;
+                }
+            }
+            estat[e] = s;
+        }
+    }
+}
+
+// dFin[3n+d] = sum over (e, la) incident to node n, in element order
+__global__ void force_gather_kernel(int64_t nnodes, int nn, const int32_t *__restrict__ fptr,
+                                    const int32_t *__restrict__ fsrc, const double *__restrict__ dF,
+                                    double *__restrict__ out)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nnodes * 3) return;
+    int64_t n = i / 3;
+    int d = (int)(i % 3);
+    double acc = 0.0;
+    for (int32_t p = fptr[n]; p < fptr[n + 1]; p++) acc += dF[(int64_t)fsrc[p] * 3 + d];
+    out[i] = acc;
+}
+
+template <int NN, int NQ, int ACH>
+int launch_ke_t(amr_model *m)
+{
+    if (m->ne == 0) return AMR_OK;
+    hipLaunchKernelGGL((ke_kernel<NN, NQ, ACH>), dim3((unsigned)m->ne), dim3(64), 0, m->stream, m->ne, m->X,
+                       m->conn, m->emat, m->mats, m->dNdR, m->wip, m->st_cur, m->nip_total,
+                       m->all_elastic ? 1 : 0, m->Ke, m->err);
+    AMR_HIP(hipGetLastError());
+    return AMR_OK;
+}
+
+template <int NN, int NQ, int EPW>
+int launch_update_t(amr_model *m, const double *dU)
+{
+    if (m->ne == 0) return AMR_OK;
+    unsigned grid = (unsigned)((m->ne + EPW - 1) / EPW);
+    hipLaunchKernelGGL((update_kernel<NN, NQ, EPW>), dim3(grid), dim3(256), 0, m->stream, m->ne, m->X, m->conn,
+                       m->emat, m->mats, m->dNdR, m->wip, m->st_com, m->st_cur, m->nip_total, dU, m->dF,
+                       m->estat);
+    AMR_HIP(hipGetLastError());
+    return AMR_OK;
+}
+
+}  // namespace
+
+int amr_launch_ke(amr_model *m)
+{
+    switch (m->etype) {
+    case AMR_HEX8: return launch_ke_t<8, 8, 1>(m);
+    case AMR_HEX20: return m->nips == 27 ? launch_ke_t<20, 27, 7>(m) : launch_ke_t<20, 8, 7>(m);
+    case AMR_TET10: return launch_ke_t<10, 4, 2>(m);
+    }
+    amr_set_error("unknown element type");
+    return AMR_ERR_ARG;
+}
+
+int amr_launch_assemble(amr_model *m, bool flags)
+{
+    if (m->pat.nblk == 0) return AMR_OK;
+    unsigned grid = (unsigned)((m->pat.nblk + 255) / 256);
+    hipLaunchKernelGGL(assemble_kernel, dim3(grid), dim3(256), 0, m->stream, m->pat.nblk, m->cptr, m->csrc, m->Ke,
+                       m->Kv, m->Kv32, flags ? m->Kflag : nullptr);
+    AMR_HIP(hipGetLastError());
+    unsigned g2 = (unsigned)((m->nnodes * 3 + 255) / 256);
+    hipLaunchKernelGGL(jacobi_kernel, dim3(g2), dim3(256), 0, m->stream, m->nnodes, m->diag, m->Kv, m->mask,
+                       m->dinv, m->dinv32);
+    AMR_HIP(hipGetLastError());
+    return AMR_OK;
+}
+
+int amr_launch_update(amr_model *m, const double *dU, double *dFin)
+{
+    int rc;
+    switch (m->etype) {
+    case AMR_HEX8: rc = launch_update_t<8, 8, 32>(m, dU); break;
+    case AMR_HEX20: rc = m->nips == 27 ? launch_update_t<20, 27, 9>(m, dU) : launch_update_t<20, 8, 16>(m, dU); break;
+    case AMR_TET10: rc = launch_update_t<10, 4, 32>(m, dU); break;
+    default: amr_set_error("unknown element type"); return AMR_ERR_ARG;
+    }
+    if (rc) return rc;
+    unsigned g = (unsigned)((m->nnodes * 3 + 255) / 256);
+    hipLaunchKernelGGL(force_gather_kernel, dim3(g), dim3(256), 0, m->stream, m->nnodes, m->nn, m->fptr, m->fsrc,
+                       m->dF, dFin);
+    AMR_HIP(hipGetLastError());
+    return AMR_OK;
+}

@@ -1,0 +1,102 @@
+// Host-side construction of the node-block sparsity structure and of the
+// deterministic gather maps used by assembly and by the internal-force sum.
+//
+// K is stored as 3x3 blocks, one block row per node, block columns sorted by
+// node id.  A block (A,B) exists when nodes A and B share an element; the
+// scalar pattern the reference builds (src/mech/mech-solver.jl:86-102, drop
+// |Ke_ij| < eps) is a subset of it and is recovered from per-scalar survivor
+// flags written at assembly.
+//
+// Gather maps replace the reference's triplet push!/sparse() and its
+// ΔFin[map] .+= ΔF scatter (src/mech/mech-solver.jl:129-136): for every
+// block, the list of element blocks (e, la, lb) that land in it, in
+// ascending element order; for every node, the list of (e, la).  Summing
+// each list in order reproduces the reference's element-order sums without
+// atomics.
+#include <algorithm>
+#include <cstring>
+
+#include "amr_internal.h"
+
+int amr_build_pattern(int nn, int64_t nnodes, int64_t ne, const int32_t *conn, AmrPattern &P)
+{
+    P.nnodes = nnodes;
+    // node -> elements incidence, ascending element order
+    std::vector<int32_t> ninc(nnodes + 1, 0);
+    for (int64_t e = 0; e < ne; e++)
+        for (int a = 0; a < nn; a++) {
+            int32_t n = conn[e * nn + a];
+            if (n < 0 || n >= nnodes) {
+                amr_set_error("connectivity references node " + std::to_string(n) + " outside [0, " +
+                              std::to_string(nnodes) + ")");
+                return AMR_ERR_ARG;
+            }
+            ninc[n + 1]++;
+        }
+    for (int64_t n = 0; n < nnodes; n++) ninc[n + 1] += ninc[n];
+    std::vector<int32_t> inc_e(ninc[nnodes]), inc_a(ninc[nnodes]);
+    {
+        std::vector<int32_t> pos(ninc.begin(), ninc.end() - 1);
+        for (int64_t e = 0; e < ne; e++)
+            for (int a = 0; a < nn; a++) {
+                int32_t n = conn[e * nn + a];
+                inc_e[pos[n]] = (int32_t)e;
+                inc_a[pos[n]] = a;
+                pos[n]++;
+            }
+    }
+    // force gather lists: exactly the incidence lists (ascending e)
+    P.fptr.assign(ninc.begin(), ninc.end());
+    P.fsrc.resize(ninc[nnodes]);
+    for (size_t i = 0; i < P.fsrc.size(); i++) P.fsrc[i] = inc_e[i] * nn + inc_a[i];
+
+    // block rows: sorted unique neighbours of each node
+    P.rowptr.assign(nnodes + 1, 0);
+    std::vector<std::vector<int32_t>> nbr(nnodes);
+    for (int64_t n = 0; n < nnodes; n++) {
+        std::vector<int32_t> &v = nbr[n];
+        v.reserve((ninc[n + 1] - ninc[n]) * nn);
+        for (int32_t i = ninc[n]; i < ninc[n + 1]; i++) {
+            const int32_t *el = conn + (int64_t)inc_e[i] * nn;
+            v.insert(v.end(), el, el + nn);
+        }
+        std::sort(v.begin(), v.end());
+        v.erase(std::unique(v.begin(), v.end()), v.end());
+        P.rowptr[n + 1] = P.rowptr[n] + (int32_t)v.size();
+    }
+    P.nblk = P.rowptr[nnodes];
+    P.col.resize(P.nblk);
+    P.diag.assign(nnodes, -1);
+    for (int64_t n = 0; n < nnodes; n++) {
+        std::copy(nbr[n].begin(), nbr[n].end(), P.col.begin() + P.rowptr[n]);
+        auto it = std::lower_bound(nbr[n].begin(), nbr[n].end(), (int32_t)n);
+        if (it != nbr[n].end() && *it == n) P.diag[n] = P.rowptr[n] + (int32_t)(it - nbr[n].begin());
+        std::vector<int32_t>().swap(nbr[n]);
+    }
+
+    // block contribution lists: pass 1 counts, pass 2 fills, both in
+    // ascending element order so each list is sorted by element
+    auto find_blk = [&](int32_t A, int32_t B) -> int64_t {
+        const int32_t *b = P.col.data() + P.rowptr[A], *e = P.col.data() + P.rowptr[A + 1];
+        return (int64_t)(std::lower_bound(b, e, B) - P.col.data());
+    };
+    P.cptr.assign(P.nblk + 1, 0);
+    std::vector<int64_t> slot((size_t)ne * nn * nn);
+    for (int64_t e = 0; e < ne; e++)
+        for (int a = 0; a < nn; a++)
+            for (int b = 0; b < nn; b++) {
+                int64_t k = find_blk(conn[e * nn + a], conn[e * nn + b]);
+                slot[(e * nn + a) * nn + b] = k;
+                P.cptr[k + 1]++;
+            }
+    for (int64_t k = 0; k < P.nblk; k++) P.cptr[k + 1] += P.cptr[k];
+    P.ncontrib = P.cptr[P.nblk];
+    if ((int64_t)ne * nn * nn > INT32_MAX) {
+        amr_set_error("model too large for 32-bit element-block ids");
+        return AMR_ERR_ARG;
+    }
+    P.csrc.resize(P.ncontrib);
+    std::vector<int64_t> pos(P.cptr.begin(), P.cptr.end() - 1);
+    for (int64_t i = 0; i < (int64_t)slot.size(); i++) P.csrc[pos[slot[i]]++] = (int32_t)i;
+    return AMR_OK;
+}

@@ -1,0 +1,455 @@
+"""FE model, materials and boundary conditions (host side).
+
+Mirrors the reference's model API: ``FEModel(mesh, matbinds, ctx)``
+(src/fe-model.jl:66-255), the material constructors (LinearElastic
+src/mech/mat/linear-elastic.jl:5-20, VonMises von-mises.jl:4-26,
+DruckerPrager drucker-prager.jl:5-34), the boundary conditions NodeBC /
+SurfaceBC / EdgeBC / BodyC (src/bc.jl:10-153), ``configure_dofs!``
+(src/bc.jl:198-233) and ``get_bc_vals`` (src/bc.jl:237-249) with the
+distributed loads of src/mech/elem/distributed.jl:75-210.
+"""
+import numpy as np
+
+from . import _lib
+from .expr import compile_expr
+from .mesh import HEX8, HEX20, NNODES, TET10
+
+ETYPE = {HEX8: _lib.AMR_HEX8, HEX20: _lib.AMR_HEX20, TET10: _lib.AMR_TET10}
+NSTATE = 14
+
+
+# ------------------------------------------------------------- materials
+
+class Material:
+    kind = None
+    names = ()
+
+    def params(self):
+        p = np.zeros(6)
+        v = [getattr(self, n) for n in self.names]
+        p[:len(v)] = v
+        return p
+
+
+class LinearElastic(Material):
+    """LinearElastic(E, nu) (src/mech/mat/linear-elastic.jl:5-20)."""
+    kind = _lib.AMR_MAT_LE
+    names = ("E", "nu")
+
+    def __init__(self, E, nu=0.0):
+        if not E > 0.0:
+            raise ValueError("LinearElastic: E>0 required")
+        if not 0.0 <= nu < 0.5:
+            raise ValueError("LinearElastic: 0<=nu<0.5 required")
+        self.E, self.nu = float(E), float(nu)
+
+
+class VonMises(Material):
+    """VonMises(E, nu, fy, H) with linear hardening (src/mech/mat/von-mises.jl:4-26)."""
+    kind = _lib.AMR_MAT_VM
+    names = ("E", "nu", "fy", "H")
+
+    def __init__(self, E, fy, nu=0.0, H=0.0, rho=0.0):
+        if not (E > 0.0 and 0.0 <= nu < 0.5 and fy >= 0.0 and H >= 0.0):
+            raise ValueError("VonMises: invalid parameters")
+        self.E, self.nu, self.fy, self.H = float(E), float(nu), float(fy), float(H)
+
+
+class DruckerPrager(Material):
+    """DruckerPrager(E, nu, alpha, kappa, H) (src/mech/mat/drucker-prager.jl:5-34)."""
+    kind = _lib.AMR_MAT_DP
+    names = ("E", "nu", "alpha", "kappa", "H")
+
+    def __init__(self, E, nu=0.0, alpha=0.0, kappa=0.0, H=0.0, rho=0.0):
+        if not (E > 0.0 and 0.0 <= nu < 0.5 and alpha >= 0.0 and kappa > 0.0 and H >= 0.0):
+            raise ValueError("DruckerPrager: invalid parameters")
+        self.E, self.nu, self.alpha, self.kappa, self.H = float(E), float(nu), float(alpha), float(kappa), float(H)
+
+
+class MechSolid:
+    """Marker for the isoparametric 3D bulk element (src/mech/elem/mech-solid.jl)."""
+
+
+# ------------------------------------------------------- boundary conditions
+
+class BC:
+    kind = None
+
+    def __init__(self, **conds):
+        if not conds:
+            raise ValueError(f"{type(self).__name__} must have at least one condition")
+        self.conds = dict(conds)
+
+
+class NodeBC(BC):
+    kind = "node"
+
+
+class SurfaceBC(BC):
+    kind = "surface"
+
+
+FaceBC = SurfaceBC
+
+
+class EdgeBC(BC):
+    kind = "edge"
+
+
+class BodyC(BC):
+    kind = "body"
+
+
+ESSENTIAL = ("ux", "uy", "uz")
+DIR = {"x": 0, "y": 1, "z": 2, "n": -1}
+
+
+# ------------------------------------------------ facet shapes and rules
+
+def _facet_tables(npts):
+    """Shape functions / derivatives of QUAD4, QUAD8, TRI6, LIN2, LIN3 at the
+    default rule (QUAD_IP4, TRI_IP3, LIN_IP2, LIN_IP3; src/shape/quadrature.jl)."""
+    g2 = 0.577350269189626
+    if npts in (4, 8):
+        pts = [(-g2, -g2), (g2, -g2), (-g2, g2), (g2, g2)]
+        w = [1.0] * 4
+    elif npts == 6:
+        pts = [(1 / 6, 1 / 6), (2 / 3, 1 / 6), (1 / 6, 2 / 3)]
+        w = [1 / 6] * 3
+    elif npts == 2:
+        a = 0.577350269189625764509149
+        pts, w = [(-a, 0.0), (a, 0.0)], [1.0, 1.0]
+    else:
+        a = 0.774596669241483377035835
+        pts = [(-a, 0.0), (0.0, 0.0), (a, 0.0)]
+        w = [0.555555555555555555555556, 0.888888888888888888888889, 0.555555555555555555555556]
+    Ns, Ds = [], []
+    for r, s in pts:
+        if npts == 4:
+            nat = [(-1, -1), (1, -1), (1, 1), (-1, 1)]
+            N = [0.25 * (1 + a * r) * (1 + b * s) for a, b in nat]
+            D = [[0.25 * a * (1 + b * s), 0.25 * b * (1 + a * r)] for a, b in nat]
+        elif npts == 8:
+            nat = [(-1, -1), (1, -1), (1, 1), (-1, 1), (0, -1), (1, 0), (0, 1), (-1, 0)]
+            N, D = [], []
+            for a, b in nat:
+                if a and b:
+                    N.append(0.25 * (1 + a * r) * (1 + b * s) * (a * r + b * s - 1))
+                    D.append([0.25 * a * (1 + b * s) * (2 * a * r + b * s), 0.25 * b * (1 + a * r) * (a * r + 2 * b * s)])
+                elif a == 0:
+                    N.append(0.5 * (1 - r * r) * (1 + b * s))
+                    D.append([-r * (1 + b * s), 0.5 * b * (1 - r * r)])
+                else:
+                    N.append(0.5 * (1 + a * r) * (1 - s * s))
+                    D.append([0.5 * a * (1 - s * s), -s * (1 + a * r)])
+        elif npts == 6:
+            L0 = 1 - r - s
+            N = [L0 * (2 * L0 - 1), r * (2 * r - 1), s * (2 * s - 1), 4 * r * L0, 4 * r * s, 4 * s * L0]
+            D = [[-(4 * L0 - 1), -(4 * L0 - 1)], [4 * r - 1, 0.0], [0.0, 4 * s - 1], [4 * (L0 - r), -4 * r],
+                 [4 * s, 4 * r], [-4 * s, 4 * (L0 - s)]]
+        elif npts == 2:
+            N = [0.5 * (1 - r), 0.5 * (1 + r)]
+            D = [[-0.5], [0.5]]
+        else:
+            N = [0.5 * r * (r - 1), 0.5 * r * (r + 1), 1 - r * r]
+            D = [[r - 0.5], [r + 0.5], [-2 * r]]
+        Ns.append(N)
+        Ds.append(D)
+    return np.array(Ns), np.array(Ds), np.array(w)
+
+
+def distributed_forces(C, key, val, t=0.0):
+    """mech_boundary_forces (src/mech/elem/distributed.jl:75-150) for many
+    facets/edges at once: C is (nf, npts, 3); returns (nf, npts, 3)."""
+    nf, npts, _ = C.shape
+    N, D, w = _facet_tables(npts)
+    f = compile_expr(val)
+    edge = key[0] == "q"
+    F = np.zeros((nf, npts, 3))
+    for q in range(len(w)):
+        J = np.einsum("fak,ai->fki", C, D[q])        # (nf, 3, ndim_facet)
+        X = np.einsum("fak,a->fk", C, N[q])
+        vip = f(X[:, 0], X[:, 1], X[:, 2], t)
+        Q = np.zeros((nf, 3))
+        d = DIR[key[1]]
+        if d >= 0:
+            Q[:, d] = vip
+        else:
+            n = np.cross(J[:, :, 0], J[:, :, 1])
+            Q = vip[:, None] * n / np.linalg.norm(n, axis=1)[:, None]
+        if edge:
+            jn = np.linalg.norm(J[:, :, 0], axis=1)
+        else:
+            jn = np.linalg.norm(np.cross(J[:, :, 0], J[:, :, 1]), axis=1)
+        F += (jn * w[q])[:, None, None] * N[q][None, :, None] * Q[:, None, :]
+    return F
+
+
+# ------------------------------------------------------------------ model
+
+class FEModel:
+    """FEModel(mesh, matbinds) for one solid element shape.
+
+    matbinds: list of ``(filter, MechSolid, material)`` or
+    ``(filter, material)``; filter is "all", a mesh tag, or an expression
+    over node coordinates that every node of an element must satisfy.
+    nips selects the IP rule (0 = shape default; 27 is
+    ``changequadrature!(elems, 27)`` for HEX20, src/element.jl:185-188).
+    """
+
+    def __init__(self, mesh, matbinds, nips=0):
+        self.mesh = mesh
+        self.shape = mesh.cellshape
+        self.etype = ETYPE[self.shape]
+        self.nen = NNODES[self.shape]
+        self.coords = mesh.coords
+        self.conn = mesh.conn
+        self.nnodes, self.ne = mesh.nnodes, mesh.ne
+        self.nips = self._rule(nips)
+        mats, emat = [], -np.ones(self.ne, dtype=np.int32)
+        for bind in matbinds:
+            flt, mat = bind[0], bind[-1]
+            if not isinstance(mat, Material):
+                raise TypeError("FEModel: material binding must end with a material instance")
+            sel = self.elem_filter(flt)
+            mats.append(mat)
+            emat[sel] = len(mats) - 1
+        if (emat < 0).any():
+            raise ValueError(f"FEModel: missing material definition for {int((emat < 0).sum())} elements")
+        self.mats = mats
+        self.emat = emat
+        self.state = np.zeros((self.ne * self.nips, NSTATE))
+        self.uvals = np.zeros((self.nnodes, 3))   # dof.vals[ux,uy,uz]
+        self.fvals = np.zeros((self.nnodes, 3))   # dof.vals[fx,fy,fz]
+        self.eq = None
+
+    def _rule(self, nips):
+        if self.shape in (HEX8, HEX20):
+            if nips in (0, 8):
+                return 8
+            if nips == 27:
+                return 27
+        elif nips in (0, 4):
+            return 4
+        raise ValueError(f"{self.shape} cannot use {nips} integration points")
+
+    def changequadrature(self, nips):
+        """changequadrature!(elems, n): new rule, states re-initialised."""
+        self.nips = self._rule(nips)
+        self.state = np.zeros((self.ne * self.nips, NSTATE))
+
+    # -------------------------------------------------------- filters
+    def _node_mask(self, expr):
+        X = self.coords
+        return np.asarray(compile_expr(expr)(X[:, 0], X[:, 1], X[:, 2]), dtype=bool)
+
+    def node_filter(self, expr):
+        return np.nonzero(self._node_mask(expr))[0]
+
+    def elem_filter(self, flt):
+        if flt in ("all", ":bulks", ":solids", "bulks", "solids"):
+            return np.arange(self.ne)
+        if isinstance(flt, str) and flt in set(self.mesh.tags.tolist()):
+            return np.nonzero(self.mesh.tags == flt)[0]
+        m = self._node_mask(flt)
+        return np.nonzero(m[self.conn].all(axis=1))[0]
+
+    def _sel(self, bc, flt):
+        if bc.kind == "node":
+            n = self.node_filter(flt)
+            return n, n
+        if bc.kind == "body":
+            e = self.elem_filter(flt)
+            return e, self.conn[e].ravel()
+        owners, nodes = self.mesh.facets() if bc.kind == "surface" else self.mesh.edges()
+        m = self._node_mask(flt)
+        keep = m[nodes].all(axis=1)
+        return (owners[keep], nodes[keep]), nodes[keep].ravel()
+
+    # ----------------------------------------------------------- dofs
+    def configure_dofs(self, bcs):
+        """configure_dofs! (src/bc.jl:198-233): DOFs of the active nodes in node
+        order (ux,uy,uz), stable partition unknowns-first, eq ids = position."""
+        active = np.unique(self.conn)
+        presc = np.zeros((self.nnodes, 3), dtype=bool)
+        self._bcsel = []
+        for flt, bc in bcs:
+            sel, nodes = self._sel(bc, flt)
+            for key in bc.conds:
+                if key in ESSENTIAL:
+                    presc[nodes, DIR[key[1]]] = True
+            self._bcsel.append(sel)
+        dof = (active[:, None] * 3 + np.arange(3)[None, :]).ravel()
+        p = presc.ravel()[dof]
+        order = np.concatenate([dof[~p], dof[p]])
+        eq = -np.ones(self.nnodes * 3, dtype=np.int32)
+        eq[order] = np.arange(order.size, dtype=np.int32)
+        self.eq = eq.reshape(self.nnodes, 3)
+        self.ndofs = int(order.size)
+        self.nu = int((~p).sum())
+        return self.ndofs, self.nu
+
+    def bc_vals(self, bcs, t=0.0):
+        """get_bc_vals (src/bc.jl:237-249): essential values U and natural
+        loads F, in eq order."""
+        U = np.zeros(self.ndofs)
+        F = np.zeros(self.ndofs)
+        X = self.coords
+        for (flt, bc), sel in zip(bcs, self._bcsel):
+            for key, val in bc.conds.items():
+                f = compile_expr(val)
+                if bc.kind == "node":
+                    d = DIR[key[1]]
+                    v = f(X[sel, 0], X[sel, 1], X[sel, 2], t)
+                    if key in ESSENTIAL:
+                        U[self.eq[sel, d]] = v
+                    elif key[0] == "f":
+                        np.add.at(F, self.eq[sel, d], v)
+                    continue
+                if bc.kind == "body":
+                    nodes = self.conn[sel]
+                else:
+                    nodes = sel[1]
+                if key in ESSENTIAL:
+                    nn = nodes.ravel()
+                    U[self.eq[nn, DIR[key[1]]]] = f(X[nn, 0], X[nn, 1], X[nn, 2], t)
+                    continue
+                if bc.kind == "body":
+                    Fd = self._body_forces(sel, key, f)
+                else:
+                    Fd = distributed_forces(X[nodes], key, val, t)
+                np.add.at(F, self.eq[nodes].reshape(-1), Fd.reshape(-1))
+        return U, F
+
+    def _body_forces(self, elems, key, f):
+        """mech_solid_body_forces (src/mech/elem/distributed.jl:155-210) with
+        the shape's default rule."""
+        from .shapes import rule_tables
+        N, D, w = rule_tables(self.shape, 0)
+        C = self.coords[self.conn[elems]]
+        F = np.zeros(C.shape)
+        d = DIR[key[1]]
+        for q in range(len(w)):
+            J = np.einsum("eak,ai->eki", C, D[q])
+            X = np.einsum("eak,a->ek", C, N[q])
+            vip = f(X[:, 0], X[:, 1], X[:, 2])
+            coef = np.linalg.det(J) * w[q]
+            F[:, :, d] += (coef * vip)[:, None] * N[q][None, :]
+        return F
+
+    # -------------------------------------------------------- outputs
+    def ip_stress(self):
+        """sigma_xx..sigma_xy per IP (stress_strain_dict,
+        src/tools/tensors.jl:162-215: shear = Mandel / sqrt(2))."""
+        s = self.state[:, :6].copy()
+        s[:, 3:] /= np.sqrt(2.0)
+        return s
+
+    def node_data(self):
+        """get_data(model.nodes) (src/node.jl:270-281): ux uy uz fx fy fz."""
+        u, f = self.uvals, self.fvals
+        return {"ux": u[:, 0], "uy": u[:, 1], "uz": u[:, 2], "fx": f[:, 0], "fy": f[:, 1], "fz": f[:, 2]}
+
+
+class DeviceModel:
+    """An FEModel resident on one GPU behind the C ABI."""
+
+    def __init__(self, model, device=0):
+        import ctypes
+        L = _lib.lib()
+        if model.eq is None:
+            model.configure_dofs([])
+        self.model = model
+        kinds = np.array([m.kind for m in model.mats], dtype=np.int32)
+        prms = np.ascontiguousarray(np.stack([m.params() for m in model.mats]))
+        self._h = ctypes.c_void_p()
+        self._keep = (np.ascontiguousarray(model.coords), np.ascontiguousarray(model.conn, dtype=np.int32),
+                      kinds, prms, np.ascontiguousarray(model.emat, dtype=np.int32),
+                      np.ascontiguousarray(model.eq.ravel(), dtype=np.int32))
+        X, C, K, Pm, E, Q = self._keep
+        _lib.check(L.amr_model_create(ctypes.byref(self._h), device, model.etype, model.nips, model.nnodes,
+                                      _lib.ptr(X), model.ne, _lib.ptr(C), len(model.mats), _lib.ptr(K),
+                                      _lib.ptr(Pm), _lib.ptr(E), _lib.ptr(Q), model.ndofs, model.nu))
+        if model.state.any():
+            self.state_set(model.state)
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if self._h:
+            _lib.lib().amr_model_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self):
+        out = np.zeros(8, dtype=np.int64)
+        _lib.check(_lib.lib().amr_model_info(self._h, _lib.ptr(out)))
+        return dict(zip(["nnodes", "ne", "nips", "nblk", "nnz_max", "ncontrib", "ndofs", "nu"], out.tolist()))
+
+    def set_dofs(self):
+        m = self.model
+        eq = np.ascontiguousarray(m.eq.ravel(), dtype=np.int32)
+        _lib.check(_lib.lib().amr_model_set_dofs(self._h, _lib.ptr(eq), m.ndofs, m.nu))
+
+    def state_set(self, st):
+        st = np.ascontiguousarray(st, dtype=float)
+        _lib.check(_lib.lib().amr_state_set(self._h, _lib.ptr(st)))
+
+    def state_get(self, committed=False):
+        st = np.zeros((self.model.ne * self.model.nips, NSTATE))
+        _lib.check(_lib.lib().amr_state_get(self._h, _lib.ptr(st), 1 if committed else 0))
+        return st
+
+    def commit(self):
+        _lib.check(_lib.lib().amr_state_commit(self._h))
+
+    def restore(self):
+        _lib.check(_lib.lib().amr_state_restore(self._h))
+
+    def assemble(self):
+        _lib.check(_lib.lib().amr_assemble(self._h))
+
+    def Ke(self):
+        nd = 3 * self.model.nen
+        out = np.zeros((self.model.ne, nd, nd))
+        _lib.check(_lib.lib().amr_get_Ke(self._h, _lib.ptr(out)))
+        return out
+
+    def K_csc(self):
+        import ctypes
+        import scipy.sparse as sp
+        L = _lib.lib()
+        nnz = ctypes.c_int64(0)
+        _lib.check(L.amr_get_K_csc(self._h, ctypes.byref(nnz), None, None, None))
+        n = self.model.ndofs
+        colptr = np.zeros(n + 1, dtype=np.int64)
+        rowval = np.zeros(max(nnz.value, 1), dtype=np.int32)
+        nzval = np.zeros(max(nnz.value, 1))
+        _lib.check(L.amr_get_K_csc(self._h, ctypes.byref(nnz), _lib.ptr(colptr), _lib.ptr(rowval), _lib.ptr(nzval)))
+        return sp.csc_matrix((nzval[:nnz.value], rowval[:nnz.value], colptr), shape=(n, n))
+
+    def solve_system(self, U, F, precision="fp64", rtol=1e-12, inner_rtol=1e-4, maxiter=200000, check_every=32):
+        """solve_system!(K, U, F, nu): mutates U[:nu] and F[nu:]."""
+        import ctypes
+        o = _lib.SolveOpts(_lib.AMR_PREC_MIXED if precision == "mixed" else _lib.AMR_PREC_FP64,
+                           _lib.AMR_PRECOND_JACOBI, int(maxiter), int(check_every), float(rtol), float(inner_rtol))
+        info = _lib.SolveInfo()
+        assert U.flags["C_CONTIGUOUS"] and F.flags["C_CONTIGUOUS"] and U.dtype == F.dtype == np.float64
+        rc = _lib.lib().amr_solve_system(self._h, _lib.ptr(U), _lib.ptr(F), ctypes.byref(o), ctypes.byref(info))
+        _lib.check(rc)
+        return {"iterations": info.iterations, "refinements": info.refinements, "relres": info.relres,
+                "converged": bool(info.converged)}
+
+    def update_state(self, dUt):
+        """update_state!(elems, ΔUt, t) → ΔFin (raises on failure)."""
+        dUt = np.ascontiguousarray(dUt, dtype=float)
+        out = np.zeros(self.model.ndofs)
+        _lib.check(_lib.lib().amr_update_state(self._h, _lib.ptr(dUt), _lib.ptr(out)))
+        return out
