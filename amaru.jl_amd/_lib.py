@@ -35,6 +35,11 @@ class SolveInfo(ctypes.Structure):
                 ("pad", ctypes.c_int), ("relres", ctypes.c_double)]
 
 
+class NRInfo(ctypes.Structure):
+    _fields_ = [("res", ctypes.c_double), ("err", ctypes.c_double), ("relres", ctypes.c_double),
+                ("pcg_iterations", ctypes.c_int), ("status", ctypes.c_int)]
+
+
 SIGNATURES = {
     "amr_last_error": (ctypes.c_char_p, []),
     "amr_device_count": (I32, [P]),
@@ -51,6 +56,13 @@ SIGNATURES = {
     "amr_get_K_csc": (I32, [P, P, P, P, P]),
     "amr_solve_system": (I32, [P, P, P, ctypes.POINTER(SolveOpts), ctypes.POINTER(SolveInfo)]),
     "amr_update_state": (I32, [P, P, P]),
+    "amr_stage_begin": (I32, [P, P, P]),
+    "amr_increment_begin": (I32, [P, D, D, I32]),
+    "amr_nr_iteration": (I32, [P, I32, ctypes.POINTER(SolveOpts), ctypes.POINTER(NRInfo)]),
+    "amr_increment_commit": (I32, [P]),
+    "amr_increment_reject": (I32, [P]),
+    "amr_stage_get": (I32, [P, P, P, I32]),
+    "amr_timing": (I32, [P, I32, P, P]),
 }
 
 _lib = None

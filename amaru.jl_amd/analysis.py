@@ -25,7 +25,7 @@ EPS = 2.220446049250313e-16
 SOLVER_DEFAULTS = dict(tol=0.01, rspan=0.01, rtol=0.01, dT0=0.01, dTmin=1e-7, dTmax=0.1, scheme="FE", maxits=5,
                        autoinc=False, quiet=True,
                        # device options (additions of this implementation)
-                       device=0, precision="fp64", pcg_rtol=1e-12, inner_rtol=1e-4, pcg_maxiter=200000)
+                       device=0, precision="fp64", pcg_rtol=1e-10, inner_rtol=1e-4, pcg_maxiter=200000)
 
 
 class Stage:
@@ -96,12 +96,21 @@ def _stage_solver(ana, stage, a):
         ana.dev.set_dofs()
     dev = ana.dev
     Uex, Fex = model.bc_vals(bcs)
-    solve_kw = dict(precision=a["precision"], rtol=a["pcg_rtol"], inner_rtol=a["inner_rtol"],
-                    maxiter=a["pcg_maxiter"])
-    # eq id -> (node, dir)
-    inv = np.zeros(ndofs, dtype=np.int64)
+    dev.stage_begin(Uex, Fex)
+    opts = DeviceModel.opts(precision=a["precision"], rtol=a["pcg_rtol"], inner_rtol=a["inner_rtol"],
+                            maxiter=a["pcg_maxiter"])
+    # eq id -> flat (node, dir) index; dof.vals start values of this stage
+    flat = np.zeros(ndofs, dtype=np.int64)
     live = model.eq.ravel() >= 0
-    inv[model.eq.ravel()[live]] = np.nonzero(live)[0]
+    flat[model.eq.ravel()[live]] = np.nonzero(live)[0]
+    u0, f0 = model.uvals.copy(), model.fvals.copy()
+
+    def sync_nodal():
+        U, F = dev.stage_get()
+        model.uvals = u0.copy()
+        model.fvals = f0.copy()
+        model.uvals.ravel()[flat] += U
+        model.fvals.ravel()[flat] += F
 
     nouts = stage.nouts
     dTcheck = 1.0 / nouts if nouts > 0 else 1.0
@@ -112,54 +121,28 @@ def _stage_solver(ana, stage, a):
         dT = min(dT, dTmax, dTcheck, a["dT0"])
     dTbk = 0.0
     inc = 0
-    U = np.zeros(ndofs)
-    F = np.zeros(ndofs)
-    R = np.zeros(ndofs)
-    Rc = np.zeros(ndofs)
-    dFin = np.zeros(ndofs)
-    still_linear = True
     status = None
     while T < 1.0 - dTmin:
         inc += 1
-        dUex, dFex = dT * Uex, dT * Fex
         dTcr = min(rspan, 1 - T)
         acr = min(dT / dTcr, 1.0)
-        if T < 1 - rspan:
-            dFex = dFex + acr * Rc
-        R = dFex.copy()
-        dUa = np.zeros(ndofs)
-        dUi = dUex.copy()
+        dev.increment_begin(dT, acr, T < 1 - rspan)
         res = res1 = 0.0
         converged = syserror = False
         sysmsg = None
         for it in range(1, maxits + 1):
-            if it > 1:
-                dUi = np.zeros(ndofs)
             lastres = res
-            try:
-                dev.assemble()                                  # mount_K
-                dUitr = np.ascontiguousarray(dUi, dtype=float)
-                Rtr = np.ascontiguousarray(R, dtype=float)
-                info = dev.solve_system(dUitr, Rtr, **solve_kw)   # solve_system!
-                ana.stats["pcg_iterations"] += info["iterations"]
-                dUt = dUa + dUitr
-                dFin = dev.update_state(dUt)                    # update_state! (from committed state)
-            except RuntimeError as err:
+            r = dev.nr_iteration(it, opts)   # mount_K, solve_system!, update_state!, R, norms
+            ana.stats["pcg_iterations"] += r["pcg_iterations"]
+            if r["status"] != 0:
                 syserror = True
-                sysmsg = str(err)
+                sysmsg = r["message"]
                 break
             ana.stats["nr_iterations"] += 1
-            dUi = dUitr
-            dUa = dUa + dUi
-            R = dFex - dFin
-            R[nu:] = 0.0
-            res = float(np.max(np.abs(R))) if ndofs else 0.0
-            err = float(np.max(np.abs(dUi)) / np.max(np.abs(dUa))) if ndofs else 0.0
+            res, err = r["res"], r["err"]
             ana.log.append((len(ana.stages), inc, it, res))
             if it == 1:
                 res1 = res
-            if it > 1:
-                still_linear = False
             if res < tol or err < rtol:
                 converged = True
                 break
@@ -170,20 +153,17 @@ def _stage_solver(ana, stage, a):
         if syserror:
             converged = False
         if converged:
-            U += dUa
-            F += dFin
-            Rc = (1.0 - acr) * Rc + R
-            dev.commit()
-            model.uvals.ravel()[inv] += dUa
-            model.fvals.ravel()[inv] += dFin
+            dev.increment_commit()
             ana.stats["increments"] += 1
             ana.stats["dof_solves"] += nu
             T += dT
             if T > Tcheck - dTmin:
                 Tcheck += dTcheck
-            model.state = dev.state_get(committed=True)
-            for lg in ana.loggers:
-                lg(model, T)
+            if ana.loggers:
+                sync_nodal()
+                model.state = dev.state_get(committed=True)
+                for lg in ana.loggers:
+                    lg(model, T)
             if autoinc:
                 if dTbk > 0.0:
                     dT = min(dTbk, Tcheck - T)
@@ -199,7 +179,7 @@ def _stage_solver(ana, stage, a):
                         dTbk = 0.0
         else:
             inc -= 1
-            dev.restore()
+            dev.increment_reject()
             if autoinc:
                 q = 1 + math.tanh(math.log10(tol / (res1 + EPS)))
                 q = min(max(q, 0.2), 0.9)
@@ -212,5 +192,6 @@ def _stage_solver(ana, stage, a):
             else:
                 status = "Solver did not converge. Try `autoinc=true`. " + (sysmsg or "")
                 break
+    sync_nodal()
     model.state = dev.state_get(committed=True)
     return status is None, status

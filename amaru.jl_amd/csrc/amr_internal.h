@@ -87,7 +87,28 @@ struct amr_model {
     int nparts = 0;
     hipStream_t stream = nullptr;
     bool assembled = false;
+    // device-resident stage vectors (node order): Uex Fex U F dFex R Rc dUa dUi dFin dUt x
+    double *sv[12] = {};
+    double acr = 0.0;
+    // SpMV timing with events on the model stream
+    bool timing = false;
+    std::vector<hipEvent_t> ev;
+    int ev_used = 0;
+    double t_spmv_ms = 0.0;
+    int64_t n_spmv = 0;
 };
+
+enum { SV_UEX = 0, SV_FEX, SV_U, SV_F, SV_DFEX, SV_R, SV_RC, SV_DUA, SV_DUI, SV_DFIN, SV_DUT, SV_X };
+
+// stage vector kernels (kernels_solve.hip)
+int amr_vec_inc_begin(amr_model *m, double dT, double acr, int add);
+int amr_vec_fill(amr_model *m, double *x, double v);
+int amr_vec_solve_prep(amr_model *m, const double *dUi, const double *R, double *u2, double *ky, double *b);
+int amr_vec_combine(amr_model *m, const double *x, double *dUi);
+int amr_vec_add(amr_model *m, const double *a, const double *b, double *out);
+int amr_vec_nr_post(amr_model *m, double *res, double *dui_max, double *dua_max, int *nan_flag);
+int amr_vec_commit(amr_model *m);
+int amr_timing_flush(amr_model *m);
 
 // kernels (kernels_elem.hip)
 int amr_launch_ke(amr_model *m);

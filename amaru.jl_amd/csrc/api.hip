@@ -185,6 +185,10 @@ static void free_model(amr_model *m)
         if (p) hipFree(p);
     for (float *p : m->f)
         if (p) hipFree(p);
+    for (double *p : m->sv)
+        if (p) hipFree(p);
+    for (hipEvent_t e : m->ev)
+        if (e) hipEventDestroy(e);
     if (m->stream) hipStreamDestroy(m->stream);
     delete m;
 }
@@ -303,8 +307,8 @@ extern "C" int amr_model_create(amr_model **out, int device, int etype, int nips
     TRY(dalloc(&m->eqd, n3));
     TRY(dalloc(&m->dF, (size_t)ne * nn * 3));
     TRY(dalloc(&m->estat, (size_t)ne));
-    TRY(dalloc(&m->err, 1));
-    TRY(dalloc(&m->part, 8192));
+    TRY(dalloc(&m->err, 4));
+    TRY(dalloc(&m->part, 16384));
     TRY(dalloc(&m->scal, 64));
     for (double *&p : m->v) TRY(dalloc(&p, n3));
     for (float *&p : m->f) TRY(dalloc(&p, n3));
@@ -328,7 +332,7 @@ extern "C" int amr_model_create(amr_model **out, int device, int etype, int nips
     AMR_HIP(hipMemsetAsync(m->st_cur, 0, sizeof(double) * AMR_NSTATE * m->nip_total, s));
     AMR_HIP(hipMemsetAsync(m->st_com, 0, sizeof(double) * AMR_NSTATE * m->nip_total, s));
     AMR_HIP(hipMemsetAsync(m->scal, 0, sizeof(double) * 64, s));
-    AMR_HIP(hipMemsetAsync(m->err, 0, sizeof(int32_t), s));
+    AMR_HIP(hipMemsetAsync(m->err, 0, 4 * sizeof(int32_t), s));
     TRY(upload_dofs(m.get(), eq_id, ndofs, nu));
     // host copies of the big maps are no longer needed
     std::vector<int32_t>().swap(m->pat.csrc);
@@ -648,5 +652,131 @@ extern "C" int amr_update_state(amr_model *m, const double *dUt, double *dFin)
             amr_set_error("solve_system!: NaN values in internal forces vector");
             return AMR_ERR_NAN;
         }
+    return AMR_OK;
+}
+
+// ------------------------------------------------ device-resident stage
+
+extern "C" int amr_stage_begin(amr_model *m, const double *Uex, const double *Fex)
+{
+    if (!m || !Uex || !Fex) return AMR_ERR_ARG;
+    AMR_HIP(hipSetDevice(m->device));
+    const size_t n = (size_t)m->nnodes * 3;
+    for (double *&p : m->sv)
+        if (!p) TRY(dalloc(&p, n));
+    std::vector<double> un, fn;
+    eq_to_node(m, Uex, un);
+    eq_to_node(m, Fex, fn);
+    TRY(h2d(m->sv[SV_UEX], un.data(), n, m->stream));
+    TRY(h2d(m->sv[SV_FEX], fn.data(), n, m->stream));
+    for (int k : {SV_U, SV_F, SV_RC, SV_DUA, SV_DUI, SV_DFIN, SV_R, SV_DFEX}) TRY(amr_vec_fill(m, m->sv[k], 0.0));
+    AMR_HIP(hipStreamSynchronize(m->stream));
+    return AMR_OK;
+}
+
+extern "C" int amr_increment_begin(amr_model *m, double dT, double acr, int add_residual)
+{
+    if (!m || !m->sv[SV_U]) return AMR_ERR_ARG;
+    AMR_HIP(hipSetDevice(m->device));
+    m->acr = acr;
+    return amr_vec_inc_begin(m, dT, acr, add_residual);
+}
+
+extern "C" int amr_nr_iteration(amr_model *m, int it, const amr_solve_opts *opts, amr_nr_info *info)
+{
+    if (!m || !m->sv[SV_U] || it < 1) return AMR_ERR_ARG;
+    AMR_HIP(hipSetDevice(m->device));
+    amr_solve_opts o = opts ? *opts : default_opts();
+    amr_nr_info inf{};
+    double **v = m->sv;
+    if (it > 1) TRY(amr_vec_fill(m, v[SV_DUI], 0.0));
+    // mount_K
+    AMR_HIP(hipMemsetAsync(m->err, 0x7f, 3 * sizeof(int32_t), m->stream));
+    TRY(amr_launch_ke(m));
+    TRY(amr_launch_assemble(m, false));
+    m->assembled = true;
+    // solve_system!(K, ΔUitr, Rtr, nu)
+    int rc = AMR_OK;
+    if (m->nu > 0) {
+        TRY(amr_vec_solve_prep(m, v[SV_DUI], v[SV_R], m->v[8], m->v[9], m->v[10]));
+        amr_solve_info si{};
+        rc = amr_pcg_solve(m, m->v[10], v[SV_X], &o, &si);
+        inf.pcg_iterations = si.iterations;
+        inf.relres = si.relres;
+        if (rc == AMR_OK) TRY(amr_vec_combine(m, v[SV_X], v[SV_DUI]));
+    }
+    int32_t herr[3] = {0, 0, 0};
+    if (rc == AMR_OK) {
+        // update_state!(ΔUa + ΔUitr) from the committed state
+        TRY(amr_vec_add(m, v[SV_DUA], v[SV_DUI], v[SV_DUT]));
+        TRY(amr_launch_update(m, v[SV_DUT], v[SV_DFIN]));
+        int nanf = 0;
+        double umax = 0.0, uamax = 0.0;
+        TRY(amr_vec_nr_post(m, &inf.res, &umax, &uamax, &nanf));
+        inf.err = umax / uamax;
+        TRY(d2h(herr, m->err, 3, m->stream));
+        AMR_HIP(hipStreamSynchronize(m->stream));
+        if (herr[0] != 0x7f7f7f7f) {
+            amr_set_error("Negative Jacobian determinant in cell " + std::to_string(herr[0]));
+            rc = AMR_ERR_JACOBIAN;
+        } else if (herr[2] != 0x7f7f7f7f) {
+            amr_set_error("Negative Jacobian determinant in cell " + std::to_string(herr[2]));
+            rc = AMR_ERR_JACOBIAN;
+        } else if (herr[1] != 0x7f7f7f7f) {
+            amr_set_error("VonMisses: Negative value for √J2D (cell " + std::to_string(herr[1]) + ")");
+            rc = AMR_ERR_MATERIAL;
+        } else if (nanf) {
+            amr_set_error("solve_system!: NaN values in internal forces vector");
+            rc = AMR_ERR_NAN;
+        } else if (umax > 1e8) {
+            // max|U1| > 1e8 (src/solver.jl:68-71); ΔUi's prescribed part is the
+            // essential increment, far below this bound in any sane model
+            amr_set_error("\nsolve_system!: Possible syngular matrix");
+            rc = AMR_ERR_SOLVE;
+        }
+    }
+    inf.status = rc;
+    if (info) *info = inf;
+    return rc;
+}
+
+extern "C" int amr_increment_commit(amr_model *m)
+{
+    if (!m || !m->sv[SV_U]) return AMR_ERR_ARG;
+    AMR_HIP(hipSetDevice(m->device));
+    TRY(amr_vec_commit(m));
+    return amr_state_commit(m);
+}
+
+extern "C" int amr_increment_reject(amr_model *m)
+{
+    if (!m) return AMR_ERR_ARG;
+    return amr_state_restore(m);
+}
+
+extern "C" int amr_stage_get(amr_model *m, double *U, double *F, int last)
+{
+    if (!m || !m->sv[SV_U]) return AMR_ERR_ARG;
+    AMR_HIP(hipSetDevice(m->device));
+    const size_t n = (size_t)m->nnodes * 3;
+    std::vector<double> a(n), b(n);
+    TRY(d2h(a.data(), m->sv[last ? SV_DUA : SV_U], n, m->stream));
+    TRY(d2h(b.data(), m->sv[last ? SV_DFIN : SV_F], n, m->stream));
+    AMR_HIP(hipStreamSynchronize(m->stream));
+    if (U) node_to_eq(m, a, U);
+    if (F) node_to_eq(m, b, F);
+    return AMR_OK;
+}
+
+extern "C" int amr_timing(amr_model *m, int enable, double *spmv_ms, int64_t *spmv_launches)
+{
+    if (!m) return AMR_ERR_ARG;
+    if (enable && !m->timing) {
+        m->t_spmv_ms = 0.0;
+        m->n_spmv = 0;
+    }
+    m->timing = enable != 0;
+    if (spmv_ms) *spmv_ms = m->t_spmv_ms;
+    if (spmv_launches) *spmv_launches = m->n_spmv;
     return AMR_OK;
 }

@@ -429,7 +429,7 @@ __global__ void __launch_bounds__(256) update_kernel(int64_t ne, const double *_
                                                      double *__restrict__ st_out, int64_t nip,
                                                      const double *__restrict__ dU,
                                                      double *__restrict__ dF,
-                                                     int32_t *__restrict__ estat)
+                                                     int32_t *__restrict__ estat, int32_t *__restrict__ err)
 {
     constexpr int ND = 3 * NN;
     __shared__ double sC[EPW][NN * 3];
@@ -534,6 +534,8 @@ __global__ void __launch_bounds__(256) update_kernel(int64_t ne, const double *_
                 }
             }
             estat[e] = s;
+            if (s == 1) atomicMin(err + 1, (int32_t)(e + 1));
+            if (s == 2) atomicMin(err + 2, (int32_t)(e + 1));
         }
     }
 }
@@ -570,7 +572,7 @@ int launch_update_t(amr_model *m, const double *dU)
     unsigned grid = (unsigned)((m->ne + EPW - 1) / EPW);
     hipLaunchKernelGGL((update_kernel<NN, NQ, EPW>), dim3(grid), dim3(256), 0, m->stream, m->ne, m->X, m->conn,
                        m->emat, m->mats, m->dNdR, m->wip, m->st_com, m->st_cur, m->nip_total, dU, m->dF,
-                       m->estat);
+                       m->estat, m->err);
     AMR_HIP(hipGetLastError());
     return AMR_OK;
 }

@@ -12,6 +12,7 @@
 // deterministic for a given model and launch shape.
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 
 #include "amr_internal.h"
 
@@ -19,8 +20,8 @@ namespace {
 
 constexpr int WG = 256;
 
-enum { S_RZ = 0, S_ALPHA, S_BETA, S_RR, S_BB, S_THR, S_PQ, S_RZ0, S_NSCAL = 16 };
-enum { C_DONE = 0, C_ITERS, C_BREAK, C_T0, C_T1, C_T2, C_T3, C_NCTL = 16 };
+enum { S_RZ = 0, S_ALPHA, S_BETA, S_RR, S_BB, S_THR, S_PQ, S_RZ0, S_RES, S_ERRN, S_ERRD, S_KX, S_NSCAL = 16 };
+enum { C_DONE = 0, C_ITERS, C_BREAK, C_T0, C_T1, C_T2, C_T3, C_T4, C_NAN, C_NCTL = 16 };
 
 template <typename T>
 __device__ inline T wave_sum(T v)
@@ -90,6 +91,24 @@ __device__ inline bool last_block_sum(double a, double b, double *pa, double *pb
 // stride L, multiplies the 3x3 block by x[col], and the L partial 3-vectors
 // are reduced with cross-lane shuffles.  mode: 0 = plain y = K x,
 // 1 = y = mask .* (K x).
+// |K||x| row sums (for the attainable-accuracy floor of the residual test)
+template <int L>
+__device__ inline void absrow(int64_t row, int sub, const int32_t *__restrict__ rowptr,
+                              const int32_t *__restrict__ col, const double *__restrict__ vals,
+                              const double *__restrict__ x, double *a)
+{
+    a[0] = a[1] = a[2] = 0.0;
+    for (int32_t k = rowptr[row] + sub; k < rowptr[row + 1]; k += L) {
+        const int64_t c = col[k];
+        const double *v = vals + (int64_t)k * 9;
+        for (int r = 0; r < 3; r++)
+            for (int j = 0; j < 3; j++) a[r] += fabs(v[r * 3 + j] * x[c * 3 + j]);
+    }
+#pragma unroll
+    for (int off = L / 2; off > 0; off >>= 1)
+        for (int r = 0; r < 3; r++) a[r] += __shfl_xor(a[r], off, L);
+}
+
 template <int L, typename VT, typename XT, typename AT>
 __device__ inline void spmv_row(int64_t row, int sub, const int32_t *__restrict__ rowptr,
                                 const int32_t *__restrict__ col, const VT *__restrict__ vals,
@@ -152,10 +171,11 @@ __global__ void __launch_bounds__(WG) residual_kernel(int64_t nrows, const int32
 {
     constexpr int RPW = WG / L;
     const int sub = threadIdx.x % L;
-    double rr = 0.0, bb = 0.0;
+    double rr = 0.0, bb = 0.0, kx = 0.0;
     for (int64_t row = (int64_t)blockIdx.x * RPW + threadIdx.x / L; row < nrows; row += (int64_t)gridDim.x * RPW) {
-        double y[3];
+        double y[3], a[3];
         spmv_row<L, double, double, double>(row, sub, rowptr, col, vals, x, y[0], y[1], y[2]);
+        absrow<L>(row, sub, rowptr, col, vals, x, a);
         if (sub == 0) {
             for (int d = 0; d < 3; d++) {
                 int64_t i = row * 3 + d;
@@ -164,14 +184,29 @@ __global__ void __launch_bounds__(WG) residual_kernel(int64_t nrows, const int32
                 r[i] = ri;
                 rr = fma(ri, ri, rr);
                 bb = fma(m * b[i], m * b[i], bb);
+                double ai = m != 0.0 ? a[d] + fabs(b[i]) : 0.0;
+                kx = fma(ai, ai, kx);
             }
         }
     }
     block_sum2(rr, bb);
-    double trr, tbb;
-    if (last_block_sum(rr, bb, pa, pb, ctl + C_T0, trr, tbb)) {
+    double dummy = 0.0;
+    block_sum2(kx, dummy);
+    double trr, tbb, tkx, t2;
+    __shared__ bool last3;
+    if (threadIdx.x == 0) pa[8192 + blockIdx.x] = kx;
+    if (last_block_sum(rr, bb, pa, pb, ctl + C_T0, trr, tbb)) last3 = true;
+    else if (threadIdx.x == 0) last3 = false;
+    __syncthreads();
+    if (!last3) return;
+    // the last block: the kx partials were written before every block's ticket
+    double x3 = 0.0, y3 = 0.0;
+    for (int i = threadIdx.x; i < (int)gridDim.x; i += WG) x3 += pa[8192 + i];
+    block_sum2(x3, y3);
+    if (threadIdx.x == 0) {
         scal[S_RR] = trr;
         scal[S_BB] = tbb;
+        scal[S_KX] = x3;
     }
 }
 
@@ -393,9 +428,17 @@ int run_pcg(amr_model *m, const Launch &L, const VT *vals, const XT *dinv, const
     AMR_HIP(hipGetLastError());
     int32_t hctl[C_NCTL];
     double hscal[S_NSCAL];
+    int prev_iters = 0;
+    if (m->timing && (int)m->ev.size() < 2 * check_every) {
+        for (hipEvent_t e : m->ev) hipEventDestroy(e);
+        m->ev.assign(2 * check_every, nullptr);
+        for (hipEvent_t &e : m->ev) AMR_HIP(hipEventCreate(&e));
+    }
     for (;;) {
         for (int k = 0; k < check_every; k++) {
+            if (m->timing) AMR_HIP(hipEventRecord(m->ev[2 * k], m->stream));
             launch_pcg_spmv<VT, XT>(m, L, vals, p, q, pa, pb, ctl, scal);
+            if (m->timing) AMR_HIP(hipEventRecord(m->ev[2 * k + 1], m->stream));
             hipLaunchKernelGGL((pcg_update_kernel<XT>), dim3(L.grid_vec), dim3(WG), 0, m->stream, n, x, r, z, p, q,
                                dinv, pa, pb, ctl, scal, maxiter);
             hipLaunchKernelGGL((pcg_dir_kernel<XT>), dim3(L.grid_vec), dim3(WG), 0, m->stream, n, p, z, ctl, scal);
@@ -404,6 +447,18 @@ int run_pcg(amr_model *m, const Launch &L, const VT *vals, const XT *dinv, const
         AMR_HIP(hipMemcpyAsync(hctl, ctl, sizeof hctl, hipMemcpyDeviceToHost, m->stream));
         AMR_HIP(hipMemcpyAsync(hscal, scal, sizeof hscal, hipMemcpyDeviceToHost, m->stream));
         AMR_HIP(hipStreamSynchronize(m->stream));
+        if (m->timing) {
+            // the SpMVs of iterations executed in this interval ran in full;
+            // later launches returned early on the done flag
+            int real = std::min(hctl[C_ITERS] - prev_iters, check_every);
+            for (int k = 0; k < real; k++) {
+                float ms = 0.f;
+                AMR_HIP(hipEventElapsedTime(&ms, m->ev[2 * k], m->ev[2 * k + 1]));
+                m->t_spmv_ms += ms;
+                m->n_spmv++;
+            }
+        }
+        prev_iters = hctl[C_ITERS];
         if (hctl[C_DONE]) break;
     }
     *iters = hctl[C_ITERS];
@@ -452,6 +507,8 @@ int amr_pcg_solve(amr_model *m, const double *b, double *x, const amr_solve_opts
     double relres = 0.0;
     const int check = o->check_every > 0 ? o->check_every : 16;
     double *rres = m->v[0];
+    double rr_prev = INFINITY;
+    const double kEps = 2.220446049250313e-16, kFloorC = 64.0;
 
     hipLaunchKernelGGL(fill_kernel, dim3(L.grid_vec), dim3(WG), 0, m->stream, n, x, 0.0);
     AMR_HIP(hipGetLastError());
@@ -463,10 +520,15 @@ int amr_pcg_solve(amr_model *m, const double *b, double *x, const amr_solve_opts
         AMR_HIP(hipStreamSynchronize(m->stream));
         const double bb = hs[S_BB], rr = hs[S_RR];
         relres = bb > 0.0 ? std::sqrt(rr / bb) : 0.0;
-        if (bb == 0.0 || rr <= o->rtol * o->rtol * bb) {
+        // attainable accuracy: a residual at the rounding level of computing
+        // b - K x itself (c * eps * || |K||x| + |b| ||) cannot be improved
+        const double floor2 = kFloorC * kFloorC * kEps * kEps * hs[S_KX];
+        if (bb == 0.0 || rr <= o->rtol * o->rtol * bb || (refinements > 0 && rr <= floor2)) {
             converged = 1;
             break;
         }
+        if (refinements > 0 && rr > 0.25 * rr_prev) break;  // refinement stagnated above both bounds
+        rr_prev = rr;
         if (total >= o->maxiter || refinements >= 64 || !std::isfinite(rr) || broke) break;
         int it = 0;
         double rri, rr0i;
@@ -496,9 +558,220 @@ int amr_pcg_solve(amr_model *m, const double *b, double *x, const amr_solve_opts
         info->relres = relres;
     }
     if (!converged) {
-        amr_set_error("PCG did not reach rtol=" + std::to_string(o->rtol) + " (relres " + std::to_string(relres) +
-                      " after " + std::to_string(total) + " iterations" + (broke ? ", breakdown" : "") + ")");
+        char buf[256];
+        snprintf(buf, sizeof buf, "PCG did not reach rtol=%.3e (relres %.3e after %d iterations, %d refinements%s)",
+                 o->rtol, relres, total, refinements, broke ? ", breakdown" : "");
+        amr_set_error(buf);
         return AMR_ERR_SOLVE;
     }
+    return AMR_OK;
+}
+
+// ----------------------------------------------------------- stage vectors
+namespace {
+
+__global__ void inc_begin_kernel(int64_t n, double dT, double acr, int add, const double *__restrict__ Uex,
+                                 const double *__restrict__ Fex, const double *__restrict__ Rc,
+                                 double *__restrict__ dFex, double *__restrict__ R, double *__restrict__ dUa,
+                                 double *__restrict__ dUi)
+{
+    for (int64_t i = (int64_t)blockIdx.x * WG + threadIdx.x; i < n; i += (int64_t)gridDim.x * WG) {
+        double f = dT * Fex[i];
+        if (add) f = f + acr * Rc[i];
+        dFex[i] = f;
+        R[i] = f;
+        dUa[i] = 0.0;
+        dUi[i] = dT * Uex[i];
+    }
+}
+
+__global__ void prescribed_part_kernel(int64_t n, const double *__restrict__ mask, const double *__restrict__ u,
+                                       double *__restrict__ u2)
+{
+    for (int64_t i = (int64_t)blockIdx.x * WG + threadIdx.x; i < n; i += (int64_t)gridDim.x * WG)
+        u2[i] = mask[i] != 0.0 ? 0.0 : u[i];
+}
+
+__global__ void masked_diff_kernel(int64_t n, const double *__restrict__ mask, const double *__restrict__ a,
+                                   const double *__restrict__ b, double *__restrict__ out)
+{
+    for (int64_t i = (int64_t)blockIdx.x * WG + threadIdx.x; i < n; i += (int64_t)gridDim.x * WG)
+        out[i] = mask[i] != 0.0 ? a[i] - b[i] : 0.0;
+}
+
+__global__ void combine_kernel(int64_t n, const double *__restrict__ mask, const double *__restrict__ x,
+                               double *__restrict__ u)
+{
+    for (int64_t i = (int64_t)blockIdx.x * WG + threadIdx.x; i < n; i += (int64_t)gridDim.x * WG)
+        if (mask[i] != 0.0) u[i] = x[i];
+}
+
+__global__ void vadd_kernel(int64_t n, const double *__restrict__ a, const double *__restrict__ b,
+                            double *__restrict__ out)
+{
+    for (int64_t i = (int64_t)blockIdx.x * WG + threadIdx.x; i < n; i += (int64_t)gridDim.x * WG) out[i] = a[i] + b[i];
+}
+
+__device__ inline double wave_max(double v)
+{
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_down(v, off, 64));
+    return v;
+}
+
+// ΔUa += ΔUi; R = mask .* (ΔFex - ΔFin); max|R|, max|ΔUi|, max|ΔUa|, NaN in ΔFin
+__global__ void __launch_bounds__(WG) nr_post_kernel(int64_t n, const double *__restrict__ mask,
+                                                     const double *__restrict__ dFex,
+                                                     const double *__restrict__ dFin,
+                                                     const double *__restrict__ dUi, double *__restrict__ dUa,
+                                                     double *__restrict__ R, double *__restrict__ part,
+                                                     int32_t *__restrict__ ctl, double *__restrict__ scal)
+{
+    double mr = 0.0, mi = 0.0, ma = 0.0, bad = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * WG + threadIdx.x; i < n; i += (int64_t)gridDim.x * WG) {
+        double ua = dUa[i] + dUi[i];
+        dUa[i] = ua;
+        double fi = dFin[i];
+        double r = mask[i] != 0.0 ? dFex[i] - fi : 0.0;
+        R[i] = r;
+        if (fi != fi) bad = 1.0;
+        mr = fmax(mr, fabs(r));
+        mi = fmax(mi, fabs(dUi[i]));
+        ma = fmax(ma, fabs(ua));
+    }
+    __shared__ double sm[4][WG / 64];
+    __shared__ bool last;
+    mr = wave_max(mr);
+    mi = wave_max(mi);
+    ma = wave_max(ma);
+    bad = wave_max(bad);
+    const int w = threadIdx.x / 64, l = threadIdx.x % 64;
+    if (l == 0) {
+        sm[0][w] = mr;
+        sm[1][w] = mi;
+        sm[2][w] = ma;
+        sm[3][w] = bad;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 0; k < 4; k++) {
+            double v = 0.0;
+            for (int j = 0; j < WG / 64; j++) v = fmax(v, sm[k][j]);
+            part[k * 2048 + blockIdx.x] = v;
+        }
+        __threadfence();
+        last = atomicAdd(ctl + C_T4, 1) == (int)gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int i = threadIdx.x; i < (int)gridDim.x; i += WG)
+        for (int k = 0; k < 4; k++) v[k] = fmax(v[k], part[k * 2048 + i]);
+    for (int k = 0; k < 4; k++) v[k] = wave_max(v[k]);
+    __syncthreads();
+    if (l == 0)
+        for (int k = 0; k < 4; k++) sm[k][w] = v[k];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 0; k < 4; k++) {
+            double t = 0.0;
+            for (int j = 0; j < WG / 64; j++) t = fmax(t, sm[k][j]);
+            v[k] = t;
+        }
+        scal[S_RES] = v[0];
+        scal[S_ERRN] = v[1];
+        scal[S_ERRD] = v[2];
+        ctl[C_NAN] = v[3] != 0.0;
+        ctl[C_T4] = 0;
+    }
+}
+
+__global__ void commit_kernel(int64_t n, double acr, const double *__restrict__ dUa, const double *__restrict__ dFin,
+                              const double *__restrict__ R, double *__restrict__ U, double *__restrict__ F,
+                              double *__restrict__ Rc)
+{
+    for (int64_t i = (int64_t)blockIdx.x * WG + threadIdx.x; i < n; i += (int64_t)gridDim.x * WG) {
+        U[i] += dUa[i];
+        F[i] += dFin[i];
+        Rc[i] = (1.0 - acr) * Rc[i] + R[i];
+    }
+}
+
+unsigned vec_grid(const amr_model *m)
+{
+    int64_t nv = (m->nnodes * 3 + WG - 1) / WG;
+    return (unsigned)std::max<int64_t>(1, std::min<int64_t>(nv, 2048));
+}
+
+}  // namespace
+
+int amr_vec_inc_begin(amr_model *m, double dT, double acr, int add)
+{
+    double **v = m->sv;
+    hipLaunchKernelGGL(inc_begin_kernel, dim3(vec_grid(m)), dim3(WG), 0, m->stream, m->nnodes * 3, dT, acr, add,
+                       v[SV_UEX], v[SV_FEX], v[SV_RC], v[SV_DFEX], v[SV_R], v[SV_DUA], v[SV_DUI]);
+    AMR_HIP(hipGetLastError());
+    return AMR_OK;
+}
+
+int amr_vec_fill(amr_model *m, double *x, double val)
+{
+    hipLaunchKernelGGL(fill_kernel, dim3(vec_grid(m)), dim3(WG), 0, m->stream, m->nnodes * 3, x, val);
+    AMR_HIP(hipGetLastError());
+    return AMR_OK;
+}
+
+// b = mask .* (R - K (prescribed part of dUi)); u2/ky are scratch
+int amr_vec_solve_prep(amr_model *m, const double *dUi, const double *R, double *u2, double *ky, double *b)
+{
+    const int64_t n = m->nnodes * 3;
+    hipLaunchKernelGGL(prescribed_part_kernel, dim3(vec_grid(m)), dim3(WG), 0, m->stream, n, m->mask, dUi, u2);
+    AMR_HIP(hipGetLastError());
+    int rc = amr_spmv_full(m, u2, ky, 0);
+    if (rc) return rc;
+    hipLaunchKernelGGL(masked_diff_kernel, dim3(vec_grid(m)), dim3(WG), 0, m->stream, n, m->mask, R, ky, b);
+    AMR_HIP(hipGetLastError());
+    return AMR_OK;
+}
+
+int amr_vec_combine(amr_model *m, const double *x, double *dUi)
+{
+    hipLaunchKernelGGL(combine_kernel, dim3(vec_grid(m)), dim3(WG), 0, m->stream, m->nnodes * 3, m->mask, x, dUi);
+    AMR_HIP(hipGetLastError());
+    return AMR_OK;
+}
+
+int amr_vec_add(amr_model *m, const double *a, const double *b, double *out)
+{
+    hipLaunchKernelGGL(vadd_kernel, dim3(vec_grid(m)), dim3(WG), 0, m->stream, m->nnodes * 3, a, b, out);
+    AMR_HIP(hipGetLastError());
+    return AMR_OK;
+}
+
+int amr_vec_nr_post(amr_model *m, double *res, double *dui_max, double *dua_max, int *nan_flag)
+{
+    double **v = m->sv;
+    int32_t *ctl = (int32_t *)(m->scal + S_NSCAL);
+    hipLaunchKernelGGL(nr_post_kernel, dim3(vec_grid(m)), dim3(WG), 0, m->stream, m->nnodes * 3, m->mask,
+                       v[SV_DFEX], v[SV_DFIN], v[SV_DUI], v[SV_DUA], v[SV_R], m->part, ctl, m->scal);
+    AMR_HIP(hipGetLastError());
+    double hs[S_NSCAL];
+    int32_t hc[C_NCTL];
+    AMR_HIP(hipMemcpyAsync(hs, m->scal, sizeof hs, hipMemcpyDeviceToHost, m->stream));
+    AMR_HIP(hipMemcpyAsync(hc, ctl, sizeof hc, hipMemcpyDeviceToHost, m->stream));
+    AMR_HIP(hipStreamSynchronize(m->stream));
+    *res = hs[S_RES];
+    *dui_max = hs[S_ERRN];
+    *dua_max = hs[S_ERRD];
+    *nan_flag = hc[C_NAN];
+    return AMR_OK;
+}
+
+int amr_vec_commit(amr_model *m)
+{
+    double **v = m->sv;
+    hipLaunchKernelGGL(commit_kernel, dim3(vec_grid(m)), dim3(WG), 0, m->stream, m->nnodes * 3, m->acr, v[SV_DUA],
+                       v[SV_DFIN], v[SV_R], v[SV_U], v[SV_F], v[SV_RC]);
+    AMR_HIP(hipGetLastError());
     return AMR_OK;
 }

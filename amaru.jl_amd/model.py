@@ -447,6 +447,53 @@ class DeviceModel:
         return {"iterations": info.iterations, "refinements": info.refinements, "relres": info.relres,
                 "converged": bool(info.converged)}
 
+    # ------------------------------------------- device-resident stage driver
+    @staticmethod
+    def opts(precision="fp64", rtol=1e-10, inner_rtol=1e-4, maxiter=200000, check_every=32):
+        return _lib.SolveOpts(_lib.AMR_PREC_MIXED if precision == "mixed" else _lib.AMR_PREC_FP64,
+                              _lib.AMR_PRECOND_JACOBI, int(maxiter), int(check_every), float(rtol), float(inner_rtol))
+
+    def stage_begin(self, Uex, Fex):
+        Uex = np.ascontiguousarray(Uex, dtype=float)
+        Fex = np.ascontiguousarray(Fex, dtype=float)
+        _lib.check(_lib.lib().amr_stage_begin(self._h, _lib.ptr(Uex), _lib.ptr(Fex)))
+
+    def increment_begin(self, dT, acr, add_residual):
+        _lib.check(_lib.lib().amr_increment_begin(self._h, float(dT), float(acr), 1 if add_residual else 0))
+
+    def nr_iteration(self, it, opts):
+        """One NR iteration on the device; returns the amr_nr_info fields.
+        A failed iteration is reported in 'status'/'message', not raised."""
+        import ctypes
+        info = _lib.NRInfo()
+        rc = _lib.lib().amr_nr_iteration(self._h, int(it), ctypes.byref(opts), ctypes.byref(info))
+        out = {"res": info.res, "err": info.err, "relres": info.relres, "pcg_iterations": info.pcg_iterations,
+               "status": rc, "message": None}
+        if rc != _lib.AMR_OK:
+            out["message"] = _lib.lib().amr_last_error().decode(errors="replace")
+            if rc in (-1, -2, -7):   # argument, runtime or memory errors are not solver failures
+                raise _lib.AmaruError(rc, out["message"])
+        return out
+
+    def increment_commit(self):
+        _lib.check(_lib.lib().amr_increment_commit(self._h))
+
+    def increment_reject(self):
+        _lib.check(_lib.lib().amr_increment_reject(self._h))
+
+    def stage_get(self, last=False):
+        U = np.zeros(self.model.ndofs)
+        F = np.zeros(self.model.ndofs)
+        _lib.check(_lib.lib().amr_stage_get(self._h, _lib.ptr(U), _lib.ptr(F), 1 if last else 0))
+        return U, F
+
+    def timing(self, enable):
+        import ctypes
+        ms = ctypes.c_double(0.0)
+        n = ctypes.c_int64(0)
+        _lib.check(_lib.lib().amr_timing(self._h, 1 if enable else 0, ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value
+
     def update_state(self, dUt):
         """update_state!(elems, ΔUt, t) → ΔFin (raises on failure)."""
         dUt = np.ascontiguousarray(dUt, dtype=float)

@@ -139,6 +139,51 @@ int amr_solve_system(amr_model *m, double *U, double *F, const amr_solve_opts *o
  */
 int amr_update_state(amr_model *m, const double *dUt, double *dFin);
 
+/* ------------------------------------------------------------------------
+ * Device-resident stage driver.  The vectors of mech_stage_solver!
+ * (src/mech/mech-solver.jl:258-264: U, F, R, ΔUa, ΔUi, ΔFin, Rc) live on the
+ * GPU; the host keeps the reference's control flow (increments, NR loop,
+ * auto-increment) and reads back two scalars per NR iteration.
+ * ------------------------------------------------------------------------ */
+typedef struct {
+    double res;          /* ||R||_inf after the iteration (R zero at prescribed dofs) */
+    double err;          /* ||ΔUi||_inf / ||ΔUa||_inf */
+    double relres;       /* final relative residual of the linear solve */
+    int pcg_iterations;  /* PCG iterations of this NR iteration */
+    int status;          /* AMR_OK, or the error code of a failed iteration */
+} amr_nr_info;
+
+/* Uex, Fex (ndofs, eq order): essential values and natural loads of the
+ * stage (get_bc_vals, src/bc.jl:237-249).  Zeroes U, F and Rc. */
+int amr_stage_begin(amr_model *m, const double *Uex, const double *Fex);
+
+/* ΔUex = dT Uex; ΔFex = dT Fex (+ acr Rc if add_residual); R = ΔFex;
+ * ΔUa = 0; ΔUi = ΔUex  (src/mech/mech-solver.jl:301-309). */
+int amr_increment_begin(amr_model *m, double dT, double acr, int add_residual);
+
+/* One Newton-Raphson iteration it (1-based) of src/mech/mech-solver.jl:
+ * 319-367 with scheme :FE: ΔUi = 0 if it > 1; mount_K; solve_system!;
+ * update_state! from the committed state with ΔUa + ΔUi; ΔUa += ΔUi;
+ * R = ΔFex - ΔFin with R zero at prescribed dofs.  A failed linear solve or
+ * constitutive update is reported in info->status (and the return code). */
+int amr_nr_iteration(amr_model *m, int it, const amr_solve_opts *opts, amr_nr_info *info);
+
+/* Converged increment: U += ΔUa; F += ΔFin; Rc = (1-acr) Rc + R; state
+ * commit (src/mech/mech-solver.jl:386-391). */
+int amr_increment_commit(amr_model *m);
+
+/* Failed increment: state restore (src/mech/mech-solver.jl:461). */
+int amr_increment_reject(amr_model *m);
+
+/* Stage totals U and F (ndofs, eq order); either pointer may be NULL.
+ * With last != 0 the last increment's ΔUa and ΔFin instead. */
+int amr_stage_get(amr_model *m, double *U, double *F, int last);
+
+/* Kernel timing of the PCG SpMV with HIP events on the model's stream:
+ * enable = 1 starts (and clears) the counters, 0 stops.  Returns the
+ * accumulated milliseconds and launch count so far. */
+int amr_timing(amr_model *m, int enable, double *spmv_ms, int64_t *spmv_launches);
+
 #ifdef __cplusplus
 }
 #endif
