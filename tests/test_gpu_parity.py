@@ -136,7 +136,9 @@ def test_solve_system_vs_direct(amaru, shape, nips, precision):
     assert np.array_equal(Ug[nu:], U[nu:]) and np.array_equal(Fg[:nu], F[:nu])
     rerr = np.abs(Fg[nu:] - Fo[nu:]).max() / np.abs(Fo[nu:]).max()
     assert rerr <= 1e-8
-    assert info["converged"] and info["relres"] <= 1e-12
+    # converged means relres <= rtol or the residual reached its rounding floor
+    # (64 eps || |K||x| + |b| ||); either way far below 1e-10 here
+    assert info["converged"] and info["relres"] <= 1e-10
     dev.close()
 
 
