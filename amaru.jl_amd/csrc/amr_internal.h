@@ -69,7 +69,7 @@ struct amr_model {
     int64_t *cptr = nullptr;
     int32_t *csrc = nullptr;
     int32_t *fptr = nullptr, *fsrc = nullptr;
-    double *Kv = nullptr;         // nblk*9 FP64 values
+    double *Kv = nullptr;         // 9*nblk FP64 values, slot-major (s*nblk + k)
     float *Kv32 = nullptr;        // nblk*9 FP32 values
     uint8_t *Kflag = nullptr;     // nblk*9 survivor flags (exact pattern)
     double *dinv = nullptr;       // 3*nnodes Jacobi inverse (0 at prescribed)
@@ -108,7 +108,6 @@ int amr_vec_combine(amr_model *m, const double *x, double *dUi);
 int amr_vec_add(amr_model *m, const double *a, const double *b, double *out);
 int amr_vec_nr_post(amr_model *m, double *res, double *dui_max, double *dua_max, int *nan_flag);
 int amr_vec_commit(amr_model *m);
-int amr_timing_flush(amr_model *m);
 
 // kernels (kernels_elem.hip)
 int amr_launch_ke(amr_model *m);

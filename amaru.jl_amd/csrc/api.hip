@@ -474,7 +474,7 @@ extern "C" int amr_get_K_csc(amr_model *m, int64_t *nnz, int64_t *colptr, int32_
             const int64_t B = m->pat.col[k];
             for (int r = 0; r < 3; r++)
                 for (int c = 0; c < 3; c++) {
-                    if (!fl[(size_t)k * 9 + r * 3 + c]) continue;
+                    if (!fl[(size_t)(r * 3 + c) * nb + k]) continue;
                     const int32_t i = m->eq[A * 3 + r], j = m->eq[B * 3 + c];
                     if (i < 0 || j < 0) continue;
                     cnt[j + 1]++;
@@ -492,12 +492,12 @@ extern "C" int amr_get_K_csc(amr_model *m, int64_t *nnz, int64_t *colptr, int32_
             const int64_t B = m->pat.col[k];
             for (int r = 0; r < 3; r++)
                 for (int c = 0; c < 3; c++) {
-                    if (!fl[(size_t)k * 9 + r * 3 + c]) continue;
+                    if (!fl[(size_t)(r * 3 + c) * nb + k]) continue;
                     const int32_t i = m->eq[A * 3 + r], j = m->eq[B * 3 + c];
                     if (i < 0 || j < 0) continue;
                     int64_t p = pos[j]++;
                     rowval[p] = i;
-                    nzval[p] = val[(size_t)k * 9 + r * 3 + c];
+                    nzval[p] = val[(size_t)(r * 3 + c) * nb + k];
                 }
         }
     // sort row ids within each column

@@ -358,7 +358,8 @@ __global__ void __launch_bounds__(64) ke_kernel(int64_t ne, const double *__rest
 // One thread per K block: sum the element blocks listed for it in element
 // order, each scalar only if |v| >= eps (mount_K's filter), starting from
 // 0.0 (0.0 + v1 == v1).  Writes FP64 and FP32 values and, when asked, the
-// survivor flags that define the reference's scalar pattern.
+// survivor flags that define the reference's scalar pattern.  Values are
+// stored slot-major: scalar s of block k at s*nblk + k.
 __global__ void __launch_bounds__(256) assemble_kernel(int64_t nblk, const int64_t *__restrict__ cptr,
                                                        const int32_t *__restrict__ csrc,
                                                        const double *__restrict__ Ke,
@@ -384,17 +385,17 @@ __global__ void __launch_bounds__(256) assemble_kernel(int64_t nblk, const int64
     }
 #pragma unroll
     for (int s = 0; s < 9; s++) {
-        Kv[k * 9 + s] = acc[s];
-        Kv32[k * 9 + s] = (float)acc[s];
+        Kv[s * nblk + k] = acc[s];
+        Kv32[s * nblk + k] = (float)acc[s];
     }
     if (flags) {
 #pragma unroll
-        for (int s = 0; s < 9; s++) flags[k * 9 + s] = (fl >> s) & 1u;
+        for (int s = 0; s < 9; s++) flags[s * nblk + k] = (fl >> s) & 1u;
     }
 }
 
 // Jacobi inverse diagonal (0 at prescribed dofs)
-__global__ void jacobi_kernel(int64_t nnodes, const int32_t *__restrict__ diag, const double *__restrict__ Kv,
+__global__ void jacobi_kernel(int64_t nnodes, int64_t nblk, const int32_t *__restrict__ diag, const double *__restrict__ Kv,
                               const double *__restrict__ mask, double *__restrict__ dinv,
                               float *__restrict__ dinv32)
 {
@@ -403,7 +404,7 @@ __global__ void jacobi_kernel(int64_t nnodes, const int32_t *__restrict__ diag, 
     int64_t n = i / 3;
     int d = (int)(i % 3);
     const int32_t db = diag[n];  // -1 for a node in no element (it has no dofs)
-    double kd = db >= 0 ? Kv[(int64_t)db * 9 + d * 4] : 1.0;
+    double kd = db >= 0 ? Kv[(int64_t)(d * 4) * nblk + db] : 1.0;
     bool fr = mask[i] != 0.0 && db >= 0;
     dinv[i] = fr ? 1.0 / kd : 0.0;
     dinv32[i] = fr ? 1.0f / (float)kd : 0.0f;
@@ -598,7 +599,7 @@ int amr_launch_assemble(amr_model *m, bool flags)
                        m->Kv, m->Kv32, flags ? m->Kflag : nullptr);
     AMR_HIP(hipGetLastError());
     unsigned g2 = (unsigned)((m->nnodes * 3 + 255) / 256);
-    hipLaunchKernelGGL(jacobi_kernel, dim3(g2), dim3(256), 0, m->stream, m->nnodes, m->diag, m->Kv, m->mask,
+    hipLaunchKernelGGL(jacobi_kernel, dim3(g2), dim3(256), 0, m->stream, m->nnodes, m->pat.nblk, m->diag, m->Kv, m->mask,
                        m->dinv, m->dinv32);
     AMR_HIP(hipGetLastError());
     return AMR_OK;

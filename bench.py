@@ -52,6 +52,7 @@ def log(msg):
 def run_steps(dev, opts, k):
     lat = []
     its = []
+    r = None
     for _ in range(k):
         t0 = time.perf_counter()
         dev.increment_begin(1.0, 1.0, False)
@@ -138,7 +139,8 @@ def main():
     value = nu * args.steps / total
     plain_value = nu * len(lat_p) / float(np.sum(lat_p))
     nblk, nnodes = info["nblk"], info["nnodes"]
-    spmv_bytes = 40 * nblk + 28 * nnodes + 4    # FP32 3x3 blocks + int32 cols + rowptr + x + q
+    # FP32 3x3 blocks + int32 cols + rowptr + FP32 p gather (once) + FP64 q write + FP64 p read for p.q
+    spmv_bytes = 40 * nblk + 4 * (nnodes + 1) + 12 * nnodes + 24 * nnodes + 24 * nnodes
     avg_s = spmv_ms / 1e3 / max(spmv_n, 1)
     achieved = spmv_bytes / avg_s / 1e9 if spmv_n else None
     out = {
