@@ -19,9 +19,10 @@
 // SpMV reads FP32 values and an FP32 copy of the search direction and
 // accumulates in FP32, and the Jacobi preconditioner is applied in FP32.
 // The FP64 residual r = b - K x is recomputed with the FP64 matrix whenever
-// the recursive residual has dropped 100x since the last recomputation and
+// the recursive residual has dropped 10x since the last recomputation and
 // at the end (residual replacement), which keeps the iteration converging
-// to the FP64 solution without restarting the Krylov process.
+// to the FP64 solution without restarting the Krylov process; a model whose
+// FP32 operator drifts too far from K falls back to the FP64 operator.
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -35,7 +36,7 @@ constexpr int WGS = 512;  // SpMV kernels
 constexpr int MAX_GR = 1024, MAX_GV = 512;
 
 // scalar and control slots
-enum { S_RZ = 0, S_RR, S_BB, S_THR, S_KX, S_RES, S_ERRN, S_ERRD, S_NSCAL = 16 };
+enum { S_RZ = 0, S_RR, S_BB, S_THR, S_KX, S_RES, S_ERRN, S_ERRD, S_GAP, S_NSCAL = 16 };
 enum { C_DONE = 0, C_ITERS, C_BREAK, C_NAN, C_NCTL = 16 };
 // partial buffers inside m->part
 enum { P_PQ = 0, P_RZ0 = 1024, P_RZ1 = 1536, P_RR0 = 2048, P_RR1 = 2560, P_A = 4096, P_B = 5120, P_C = 6144,
@@ -149,17 +150,19 @@ __global__ void __launch_bounds__(WGS) spmv_kernel(int64_t nrows, const int32_t 
 }
 
 // r = mask .* (b - K x) in FP64; partials ||r||^2, ||b||^2, || |K||x| + |b| ||^2
+// With rold != nullptr also the partial ||r - rold||^2 (drift of a recursive
+// residual rold from the true one).
 template <int L>
 __global__ void __launch_bounds__(WGS) residual_kernel(int64_t nrows, const int32_t *__restrict__ rowptr,
                                                        const int32_t *__restrict__ col,
                                                        const double *__restrict__ vals, int64_t nblk,
                                                        const double *__restrict__ x, const double *__restrict__ b,
-                                                       double *__restrict__ r, const double *__restrict__ mask,
-                                                       double *__restrict__ part)
+                                                       double *__restrict__ r, const double *__restrict__ rold,
+                                                       const double *__restrict__ mask, double *__restrict__ part)
 {
     constexpr int RPW = WGS / L;
     const int sub = threadIdx.x % L;
-    double rr = 0.0, bb = 0.0, kx = 0.0;
+    double rr = 0.0, bb = 0.0, kx = 0.0, gap = 0.0;
     for (int64_t row = (int64_t)blockIdx.x * RPW + threadIdx.x / L; row < nrows; row += (int64_t)gridDim.x * RPW) {
         double y[3], a[3];
         spmv_row<L, double, double, double>(row, sub, rowptr, col, vals, nblk, x, y[0], y[1], y[2]);
@@ -175,16 +178,22 @@ __global__ void __launch_bounds__(WGS) residual_kernel(int64_t nrows, const int3
                 rr = fma(ri, ri, rr);
                 bb = fma(bi, bi, bb);
                 kx = fma(ai, ai, kx);
+                if (rold) {
+                    const double g = ri - rold[i];
+                    gap = fma(g, g, gap);
+                }
             }
         }
     }
     rr = block_sum<WGS>(rr);
     bb = block_sum<WGS>(bb);
     kx = block_sum<WGS>(kx);
+    gap = block_sum<WGS>(gap);
     if (threadIdx.x == 0) {
         part[P_A + blockIdx.x] = rr;
         part[P_B + blockIdx.x] = bb;
         part[P_C + blockIdx.x] = kx;
+        part[P_D + blockIdx.x] = gap;
     }
 }
 
@@ -319,16 +328,24 @@ __global__ void __launch_bounds__(WGV) pcg_init_kernel(int64_t n, const double *
 
 // After a residual recomputation: z = M r and the rz partials in the slot
 // the next iteration reads as "previous".
-__global__ void __launch_bounds__(WGV) pcg_rez_kernel(int64_t n, const double *__restrict__ r,
-                                                      double *__restrict__ z, const double *__restrict__ dinv,
+// r = rt; with restart also p = z (the search direction starts afresh).
+__global__ void __launch_bounds__(WGV) pcg_rez_kernel(int64_t n, const double *__restrict__ rt,
+                                                      double *__restrict__ r, double *__restrict__ z,
+                                                      double *__restrict__ p, float *__restrict__ p32, int restart,
+                                                      const double *__restrict__ dinv,
                                                       const float *__restrict__ dinv32, int prec32,
                                                       double *__restrict__ part, int slot_pi)
 {
     double rz = 0.0;
     for (int64_t i = (int64_t)blockIdx.x * WGV + threadIdx.x; i < n; i += (int64_t)gridDim.x * WGV) {
-        const double ri = r[i];
+        const double ri = rt[i];
         const double zi = prec32 ? (double)(dinv32[i] * (float)ri) : dinv[i] * ri;
+        r[i] = ri;
         z[i] = zi;
+        if (restart) {
+            p[i] = zi;
+            if (p32) p32[i] = (float)zi;
+        }
         rz = fma(ri, zi, rz);
     }
     rz = block_sum<WGV>(rz);
@@ -355,20 +372,27 @@ __global__ void __launch_bounds__(WGV) pcg_finalize_kernel(int mode, int nv, int
             ctl[C_BREAK] = 0;
             ctl[C_DONE] = rr == 0.0 ? 1 : 0;
         }
-    } else {
+    } else if (mode == 1) {
         const double rr = sum_partials<WGV>(part + P_A, nr);
         const double bb = sum_partials<WGV>(part + P_B, nr);
         const double kx = sum_partials<WGV>(part + P_C, nr);
-        const double rz = sum_partials<WGV>(part + (slot_pi ? P_RZ1 : P_RZ0), nv);
+        const double gap = sum_partials<WGV>(part + P_D, nr);
         if (threadIdx.x == 0) {
             scal[S_RR] = rr;
             scal[S_BB] = bb;
             scal[S_KX] = kx;
+            scal[S_GAP] = gap;
+        }
+    } else {
+        const double rz = sum_partials<WGV>(part + (slot_pi ? P_RZ1 : P_RZ0), nv);
+        if (threadIdx.x == 0) {
             scal[S_RZ] = rz;
             ctl[C_DONE] = 0;
         }
     }
 }
+
+__global__ void clear_break_kernel(int32_t *__restrict__ ctl) { ctl[C_BREAK] = 0; }
 
 __global__ void fill_kernel(int64_t n, double *__restrict__ x, double v)
 {
@@ -406,11 +430,11 @@ void launch_pcg_spmv(amr_model *m, const Launch &L, const VT *vals, const XT *px
 #undef AMR_PCG_SPMV
 }
 
-void launch_residual(amr_model *m, const Launch &L, const double *x, const double *b, double *r)
+void launch_residual(amr_model *m, const Launch &L, const double *x, const double *b, double *r, const double *rold)
 {
 #define AMR_RES(LN)                                                                                                \
     hipLaunchKernelGGL((residual_kernel<LN>), dim3(L.grid_rows), dim3(WGS), 0, m->stream, m->nnodes, m->rowptr,   \
-                       m->col, m->Kv, m->pat.nblk, x, b, r, m->mask, m->part)
+                       m->col, m->Kv, m->pat.nblk, x, b, r, rold, m->mask, m->part)
     if (L.lanes == 64) AMR_RES(64);
     else if (L.lanes == 32) AMR_RES(32);
     else AMR_RES(16);
@@ -443,18 +467,26 @@ int amr_spmv_full(amr_model *m, const double *x, double *y, int mask_mode)
 }
 
 // Solve K11 x = b (node order, b zero at prescribed dofs), x FP64.
+//
+// Residual replacement: when the recursive residual has dropped by 10x
+// since the last recomputation (mixed path) or met the tolerance (both
+// paths), r = b - K x is recomputed in FP64.  The drift between the two
+// residuals measures how well the FP32 operator represents K for this
+// model: if it exceeds 0.3 ||r||, the rest of the solve runs with the FP64
+// operator (restarting the search direction), so an ill-conditioned model
+// still converges to the FP64 solution.
 int amr_pcg_solve(amr_model *m, const double *b, double *x, const amr_solve_opts *o, amr_solve_info *info)
 {
     const int64_t n = m->nnodes * 3;
     const Launch L = pick_launch(m);
-    const bool mixed = o->precision == AMR_PREC_MIXED;
-    const int prec32 = mixed ? 1 : 0;
+    bool mixed = o->precision == AMR_PREC_MIXED;
     const int check = std::max(1, std::min(o->check_every > 0 ? o->check_every : 32, 256));
     const double kEps = 2.220446049250313e-16, kFloorC = 64.0;
-    const double replace_ratio = mixed ? 1e-2 : 0.0;  // residual drop that triggers a recomputation
+    const double replace_ratio = 0.1;   // residual drop that triggers a recomputation (mixed path)
+    const double drift_limit = 0.3;     // drift/||r|| above which FP32 is abandoned
     int32_t *ctl = (int32_t *)(m->scal + S_NSCAL);
-    double *r = m->v[1], *z = m->v[2], *p = m->v[3], *q = m->v[4];
-    float *p32 = mixed ? m->f[0] : nullptr;
+    double *r = m->v[1], *z = m->v[2], *p = m->v[3], *q = m->v[4], *rt = m->v[5];
+    float *p32 = m->f[0];
     double hs[S_NSCAL];
     int32_t hc[C_NCTL];
 
@@ -464,31 +496,31 @@ int amr_pcg_solve(amr_model *m, const double *b, double *x, const amr_solve_opts
         for (hipEvent_t &e : m->ev) AMR_HIP(hipEventCreate(&e));
     }
     const double tr = 0.5 * o->rtol;
-    hipLaunchKernelGGL(pcg_init_kernel, dim3(L.grid_vec), dim3(WGV), 0, m->stream, n, b, x, r, z, p, p32, m->dinv,
-                       m->dinv32, prec32, m->part);
+    hipLaunchKernelGGL(pcg_init_kernel, dim3(L.grid_vec), dim3(WGV), 0, m->stream, n, b, x, r, z, p,
+                       mixed ? p32 : nullptr, m->dinv, m->dinv32, mixed ? 1 : 0, m->part);
     hipLaunchKernelGGL(pcg_finalize_kernel, dim3(1), dim3(WGV), 0, m->stream, 0, (int)L.grid_vec, 0, 1, tr * tr,
                        m->part, ctl, m->scal);
     AMR_HIP(hipGetLastError());
     if (int rc = read_scalars(m, hs, hc)) return rc;
     const double bb = hs[S_BB];
     double rr_ref = hs[S_RR], rr_true_prev = INFINITY, relres = 0.0;
-    int converged = bb == 0.0, broke = 0, replacements = 0, iters = 0;
+    int converged = bb == 0.0, broke = 0, replacements = 0, iters = 0, fell_back = 0, breakdowns = 0;
     int k = 0;
     while (!converged) {
         for (int j = 0; j < check; j++, k++) {
             const int pi = k & 1;
-            if (m->timing) AMR_HIP(hipEventRecord(m->ev[2 * j], m->stream));
+            if (m->timing && mixed) AMR_HIP(hipEventRecord(m->ev[2 * j], m->stream));
             if (mixed) launch_pcg_spmv<float, float>(m, L, m->Kv32, p32, p, q);
             else launch_pcg_spmv<double, double>(m, L, m->Kv, p, p, q);
-            if (m->timing) AMR_HIP(hipEventRecord(m->ev[2 * j + 1], m->stream));
+            if (m->timing && mixed) AMR_HIP(hipEventRecord(m->ev[2 * j + 1], m->stream));
             hipLaunchKernelGGL(pcg_update_kernel, dim3(L.grid_vec), dim3(WGV), 0, m->stream, n, x, r, z, p, q, m->dinv,
-                               m->dinv32, prec32, m->part, (int)L.grid_rows, (int)L.grid_vec, pi, ctl);
-            hipLaunchKernelGGL(pcg_dir_kernel, dim3(L.grid_vec), dim3(WGV), 0, m->stream, n, p, p32, z, m->part,
-                               (int)L.grid_vec, pi, k + 1, o->maxiter, ctl, m->scal);
+                               m->dinv32, mixed ? 1 : 0, m->part, (int)L.grid_rows, (int)L.grid_vec, pi, ctl);
+            hipLaunchKernelGGL(pcg_dir_kernel, dim3(L.grid_vec), dim3(WGV), 0, m->stream, n, p, mixed ? p32 : nullptr,
+                               z, m->part, (int)L.grid_vec, pi, k + 1, o->maxiter, ctl, m->scal);
         }
         AMR_HIP(hipGetLastError());
         if (int rc = read_scalars(m, hs, hc)) return rc;
-        if (m->timing) {
+        if (m->timing && mixed) {
             // the SpMVs of the iterations run in this batch executed in full;
             // later launches returned at once on the done flag
             const int real = std::min(hc[C_ITERS] - iters, check);
@@ -503,36 +535,52 @@ int amr_pcg_solve(amr_model *m, const double *b, double *x, const amr_solve_opts
         broke = hc[C_BREAK];
         k = iters;
         const bool done = hc[C_DONE] != 0;
-        if (done || (replace_ratio > 0.0 && hs[S_RR] <= replace_ratio * replace_ratio * rr_ref)) {
-            // residual replacement: r = b - K x in FP64, z = M r
-            const int slot = 1 - (iters & 1);
-            launch_residual(m, L, x, b, r);
-            hipLaunchKernelGGL(pcg_rez_kernel, dim3(L.grid_vec), dim3(WGV), 0, m->stream, n, r, z, m->dinv,
-                               m->dinv32, prec32, m->part, slot);
-            hipLaunchKernelGGL(pcg_finalize_kernel, dim3(1), dim3(WGV), 0, m->stream, 1, (int)L.grid_vec,
-                               (int)L.grid_rows, slot, 0.0, m->part, ctl, m->scal);
-            AMR_HIP(hipGetLastError());
-            if (int rc = read_scalars(m, hs, hc)) return rc;
-            replacements++;
-            const double rr = hs[S_RR];
-            relres = bb > 0.0 ? std::sqrt(rr / bb) : 0.0;
-            const double floor2 = kFloorC * kFloorC * kEps * kEps * hs[S_KX];
-            if (rr <= o->rtol * o->rtol * bb || rr <= floor2) {
-                converged = 1;
-                break;
-            }
-            if (!std::isfinite(rr) || broke || iters >= o->maxiter) break;
-            if (done && rr > 0.25 * rr_true_prev) break;  // stagnation above both bounds
-            rr_true_prev = rr;
-            rr_ref = rr;
-        } else if (broke || iters >= o->maxiter) {
+        if (!(done || (mixed && hs[S_RR] <= replace_ratio * replace_ratio * rr_ref))) {
+            if (broke || iters >= o->maxiter) break;
+            continue;
+        }
+        // residual replacement: rt = b - K x in FP64, drift ||rt - r||
+        const double rr_rec = hs[S_RR];
+        launch_residual(m, L, x, b, rt, r);
+        hipLaunchKernelGGL(pcg_finalize_kernel, dim3(1), dim3(WGV), 0, m->stream, 1, (int)L.grid_vec,
+                           (int)L.grid_rows, 0, 0.0, m->part, ctl, m->scal);
+        AMR_HIP(hipGetLastError());
+        if (int rc = read_scalars(m, hs, hc)) return rc;
+        replacements++;
+        const double rr = hs[S_RR];
+        relres = bb > 0.0 ? std::sqrt(rr / bb) : 0.0;
+        const double floor2 = kFloorC * kFloorC * kEps * kEps * hs[S_KX];
+        if (rr <= o->rtol * o->rtol * bb || rr <= floor2) {
+            converged = 1;
             break;
         }
+        if (!std::isfinite(rr) || iters >= o->maxiter) break;
+        if (done && !broke && rr > 0.25 * rr_true_prev) break;  // stagnation above both bounds
+        if (broke && ++breakdowns > 3) break;
+        bool restart = broke != 0;
+        if (mixed && hs[S_GAP] > drift_limit * drift_limit * std::max(rr, rr_rec)) {
+            mixed = false;  // the FP32 operator is too coarse for this model
+            fell_back = 1;
+            restart = true;
+        }
+        const int slot = 1 - (iters & 1);
+        hipLaunchKernelGGL(pcg_rez_kernel, dim3(L.grid_vec), dim3(WGV), 0, m->stream, n, rt, r, z, p,
+                           mixed ? p32 : nullptr, restart ? 1 : 0, m->dinv, m->dinv32, mixed ? 1 : 0, m->part, slot);
+        hipLaunchKernelGGL(pcg_finalize_kernel, dim3(1), dim3(WGV), 0, m->stream, 2, (int)L.grid_vec,
+                           (int)L.grid_rows, slot, 0.0, m->part, ctl, m->scal);
+        AMR_HIP(hipGetLastError());
+        if (broke) {
+            hipLaunchKernelGGL(clear_break_kernel, dim3(1), dim3(1), 0, m->stream, ctl);
+            broke = 0;
+        }
+        rr_true_prev = rr;
+        rr_ref = rr;
     }
     if (info) {
         info->iterations = iters;
         info->refinements = replacements;
         info->converged = converged;
+        info->pad = fell_back;
         info->relres = relres;
     }
     if (!converged) {
