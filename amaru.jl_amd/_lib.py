@@ -27,17 +27,19 @@ I32, I64, D = ctypes.c_int, ctypes.c_int64, ctypes.c_double
 
 class SolveOpts(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int), ("precond", ctypes.c_int), ("maxiter", ctypes.c_int),
-                ("check_every", ctypes.c_int), ("rtol", ctypes.c_double), ("inner_rtol", ctypes.c_double)]
+                ("check_every", ctypes.c_int), ("rtol", ctypes.c_double), ("inner_rtol", ctypes.c_double),
+                ("replace_ratio", ctypes.c_double), ("drift_limit", ctypes.c_double)]
 
 
 class SolveInfo(ctypes.Structure):
     _fields_ = [("iterations", ctypes.c_int), ("refinements", ctypes.c_int), ("converged", ctypes.c_int),
-                ("pad", ctypes.c_int), ("relres", ctypes.c_double)]
+                ("fallback", ctypes.c_int), ("relres", ctypes.c_double)]
 
 
 class NRInfo(ctypes.Structure):
     _fields_ = [("res", ctypes.c_double), ("err", ctypes.c_double), ("relres", ctypes.c_double),
-                ("pcg_iterations", ctypes.c_int), ("status", ctypes.c_int)]
+                ("pcg_iterations", ctypes.c_int), ("status", ctypes.c_int), ("replacements", ctypes.c_int),
+                ("fallback", ctypes.c_int)]
 
 
 SIGNATURES = {
@@ -63,6 +65,7 @@ SIGNATURES = {
     "amr_increment_reject": (I32, [P]),
     "amr_stage_get": (I32, [P, P, P, I32]),
     "amr_timing": (I32, [P, I32, P, P]),
+    "amr_set_spmv_lanes": (I32, [P, I32, I32]),
 }
 
 _lib = None

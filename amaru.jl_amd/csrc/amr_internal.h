@@ -96,6 +96,8 @@ struct amr_model {
     int ev_used = 0;
     double t_spmv_ms = 0.0;
     int64_t n_spmv = 0;
+    // SpMV lanes per block row chosen by the autotuner (0 = not tuned yet)
+    int lanes64 = 0, lanes32 = 0;
 };
 
 enum { SV_UEX = 0, SV_FEX, SV_U, SV_F, SV_DFEX, SV_R, SV_RC, SV_DUA, SV_DUI, SV_DFIN, SV_DUT, SV_X };
@@ -117,3 +119,4 @@ int amr_launch_update(amr_model *m, const double *dU_node /*device, node order*/
 // solver (kernels_solve.hip)
 int amr_pcg_solve(amr_model *m, const double *b, double *x, const amr_solve_opts *o, amr_solve_info *info);
 int amr_spmv_full(amr_model *m, const double *x, double *y, int mask_mode);
+int amr_autotune(amr_model *m);

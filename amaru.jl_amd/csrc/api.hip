@@ -563,6 +563,8 @@ static amr_solve_opts default_opts()
     o.check_every = 32;
     o.rtol = 1e-12;
     o.inner_rtol = 1e-4;
+    o.replace_ratio = 0.0;
+    o.drift_limit = 0.0;
     return o;
 }
 
@@ -703,6 +705,8 @@ extern "C" int amr_nr_iteration(amr_model *m, int it, const amr_solve_opts *opts
         rc = amr_pcg_solve(m, m->v[10], v[SV_X], &o, &si);
         inf.pcg_iterations = si.iterations;
         inf.relres = si.relres;
+        inf.replacements = si.refinements;
+        inf.fallback = si.fallback;
         if (rc == AMR_OK) TRY(amr_vec_combine(m, v[SV_X], v[SV_DUI]));
     }
     int32_t herr[3] = {0, 0, 0};
@@ -778,5 +782,18 @@ extern "C" int amr_timing(amr_model *m, int enable, double *spmv_ms, int64_t *sp
     m->timing = enable != 0;
     if (spmv_ms) *spmv_ms = m->t_spmv_ms;
     if (spmv_launches) *spmv_launches = m->n_spmv;
+    return AMR_OK;
+}
+
+extern "C" int amr_set_spmv_lanes(amr_model *m, int lanes_fp64, int lanes_fp32)
+{
+    auto ok = [](int l) { return l == 0 || l == 8 || l == 16 || l == 32 || l == 64; };
+    if (!m || !ok(lanes_fp64) || !ok(lanes_fp32)) {
+        amr_set_error("amr_set_spmv_lanes: lanes must be 0, 8, 16, 32 or 64");
+        return AMR_ERR_ARG;
+    }
+    m->lanes64 = lanes_fp64;
+    m->lanes32 = lanes_fp32;
+    if (!lanes_fp64 || !lanes_fp32) m->lanes64 = m->lanes32 = 0;
     return AMR_OK;
 }

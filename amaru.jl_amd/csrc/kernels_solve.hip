@@ -78,122 +78,170 @@ __device__ inline double sum_partials(const double *__restrict__ P, int n)
 }
 
 // ------------------------------------------------------------------ SpMV
+//
+// Row tiles: workgroup b owns block rows [b*R, (b+1)*R) (R a power of two
+// fixed per model), stages their row pointers in LDS and walks them with L
+// lanes per row.  Each row's sum is formed over 64 virtual slots (slot j
+// takes blocks rb+j, rb+j+64, ... in order) reduced by one fixed binary
+// tree, whatever L is: a lane holds 64/L slots and performs the tree levels
+// above L itself, the levels below L are cross-lane shuffles.  Per-row dot
+// contributions go to LDS and are reduced by a fixed tree too.  So neither
+// the row sums nor the partials depend on L, and L can be autotuned.
 
-// L lanes per block row (node): each lane walks the row's blocks with
-// stride L and multiplies the 3x3 block by x[col]; the L partial 3-vectors
-// are combined with cross-lane shuffles.  AT is the accumulation type.
-template <int L, typename VT, typename XT, typename AT>
-__device__ inline void spmv_row(int64_t row, int sub, const int32_t *__restrict__ rowptr,
-                                const int32_t *__restrict__ col, const VT *__restrict__ vals, int64_t nblk,
-                                const XT *__restrict__ x, AT &y0, AT &y1, AT &y2)
+template <int L, bool ABS, typename VT, typename XT, typename AT>
+__device__ inline void row_sum(int32_t rb, int32_t re, int sub, const int32_t *__restrict__ col,
+                               const VT *__restrict__ vals, int64_t nblk, const XT *__restrict__ x, AT y[3])
 {
-    y0 = 0;
-    y1 = 0;
-    y2 = 0;
-    const int32_t rb = rowptr[row], re = rowptr[row + 1];
-    for (int32_t k = rb + sub; k < re; k += L) {
-        const int64_t c = col[k];
-        const AT x0 = (AT)x[c * 3], x1 = (AT)x[c * 3 + 1], x2 = (AT)x[c * 3 + 2];
-        const VT *v = vals + k;
-        y0 = fma((AT)v[2 * nblk], x2, fma((AT)v[nblk], x1, fma((AT)v[0], x0, y0)));
-        y1 = fma((AT)v[5 * nblk], x2, fma((AT)v[4 * nblk], x1, fma((AT)v[3 * nblk], x0, y1)));
-        y2 = fma((AT)v[8 * nblk], x2, fma((AT)v[7 * nblk], x1, fma((AT)v[6 * nblk], x0, y2)));
+    constexpr int NS = 64 / L;
+    AT acc[NS][3];
+#pragma unroll
+    for (int t = 0; t < NS; t++) acc[t][0] = acc[t][1] = acc[t][2] = (AT)0;
+    for (int32_t base = rb; base < re; base += 64) {
+#pragma unroll
+        for (int t = 0; t < NS; t++) {
+            const int32_t k = base + sub + L * t;
+            if (k < re) {
+                const int64_t c = col[k];
+                const AT x0 = (AT)x[c * 3], x1 = (AT)x[c * 3 + 1], x2 = (AT)x[c * 3 + 2];
+                const VT *v = vals + k;
+                if (ABS) {
+                    acc[t][0] += fabs((AT)v[0] * x0) + fabs((AT)v[nblk] * x1) + fabs((AT)v[2 * nblk] * x2);
+                    acc[t][1] += fabs((AT)v[3 * nblk] * x0) + fabs((AT)v[4 * nblk] * x1) + fabs((AT)v[5 * nblk] * x2);
+                    acc[t][2] += fabs((AT)v[6 * nblk] * x0) + fabs((AT)v[7 * nblk] * x1) + fabs((AT)v[8 * nblk] * x2);
+                } else {
+                    acc[t][0] = fma((AT)v[2 * nblk], x2, fma((AT)v[nblk], x1, fma((AT)v[0], x0, acc[t][0])));
+                    acc[t][1] = fma((AT)v[5 * nblk], x2, fma((AT)v[4 * nblk], x1, fma((AT)v[3 * nblk], x0, acc[t][1])));
+                    acc[t][2] = fma((AT)v[8 * nblk], x2, fma((AT)v[7 * nblk], x1, fma((AT)v[6 * nblk], x0, acc[t][2])));
+                }
+            }
+        }
+    }
+    // tree levels with offset >= L combine slots held by the same lane
+#pragma unroll
+    for (int off = 32; off >= L; off >>= 1) {
+#pragma unroll
+        for (int t = 0; t < off / L; t++)
+#pragma unroll
+            for (int d = 0; d < 3; d++) acc[t][d] = acc[t][d] + acc[t + off / L][d];
     }
 #pragma unroll
-    for (int off = L / 2; off > 0; off >>= 1) {
-        y0 += __shfl_xor(y0, off, L);
-        y1 += __shfl_xor(y1, off, L);
-        y2 += __shfl_xor(y2, off, L);
+    for (int d = 0; d < 3; d++) {
+        AT v = acc[0][d];
+#pragma unroll
+        for (int off = L / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, L);
+        y[d] = v;
     }
 }
 
-// |K||x| row sums (for the attainable-accuracy floor of the residual test)
-template <int L>
-__device__ inline void absrow(int64_t row, int sub, const int32_t *__restrict__ rowptr,
-                              const int32_t *__restrict__ col, const double *__restrict__ vals, int64_t nblk,
-                              const double *__restrict__ x, double *a)
+// Fixed-order tree sum of R (power of two) LDS values; result in every thread.
+__device__ inline double lds_tree_sum(double *s, int R)
 {
-    a[0] = a[1] = a[2] = 0.0;
-    for (int32_t k = rowptr[row] + sub; k < rowptr[row + 1]; k += L) {
-        const int64_t c = col[k];
-        for (int r = 0; r < 3; r++)
-            for (int j = 0; j < 3; j++) a[r] += fabs(vals[(r * 3 + j) * nblk + k] * x[c * 3 + j]);
+    for (int h = R / 2; h > 0; h >>= 1) {
+        for (int i = threadIdx.x; i < h; i += blockDim.x) s[i] = s[i] + s[i + h];
+        __syncthreads();
     }
-#pragma unroll
-    for (int off = L / 2; off > 0; off >>= 1)
-        for (int r = 0; r < 3; r++) a[r] += __shfl_xor(a[r], off, L);
+    const double t = s[0];
+    __syncthreads();
+    return t;
+}
+
+// Dynamic LDS layout of the row-tile kernels: R+1 row pointers, then up to
+// four R-long double arrays.
+__device__ inline int32_t *tile_rowptr(char *sm) { return (int32_t *)sm; }
+__device__ inline double *tile_dbl(char *sm, int R, int j)
+{
+    const int off = ((R + 1) * 4 + 15) & ~15;
+    return (double *)(sm + off) + (int64_t)j * R;
+}
+
+__device__ inline int load_tile_rows(const int32_t *__restrict__ rowptr, int64_t nrows, int R, int32_t *srp,
+                                     int64_t &r0)
+{
+    r0 = (int64_t)blockIdx.x * R;
+    const int nr = (int)std::min<int64_t>(R, nrows - r0);
+    for (int i = threadIdx.x; i <= nr; i += blockDim.x) srp[i] = rowptr[r0 + i];
+    __syncthreads();
+    return nr;
 }
 
 // y = K x (mode 0) or mask .* (K x) (mode 1), FP64
 template <int L>
-__global__ void __launch_bounds__(WGS) spmv_kernel(int64_t nrows, const int32_t *__restrict__ rowptr,
+__global__ void __launch_bounds__(WGS) spmv_kernel(int64_t nrows, int R, const int32_t *__restrict__ rowptr,
                                                    const int32_t *__restrict__ col, const double *__restrict__ vals,
                                                    int64_t nblk, const double *__restrict__ x, double *__restrict__ y,
                                                    const double *__restrict__ mask, int mode)
 {
+    extern __shared__ __align__(16) char sm[];
+    int32_t *srp = tile_rowptr(sm);
+    int64_t r0;
+    const int nr = load_tile_rows(rowptr, nrows, R, srp, r0);
     constexpr int RPW = WGS / L;
     const int sub = threadIdx.x % L;
-    for (int64_t row = (int64_t)blockIdx.x * RPW + threadIdx.x / L; row < nrows; row += (int64_t)gridDim.x * RPW) {
-        double y0, y1, y2;
-        spmv_row<L, double, double, double>(row, sub, rowptr, col, vals, nblk, x, y0, y1, y2);
+    for (int i = threadIdx.x / L; i < nr; i += RPW) {
+        double yy[3];
+        row_sum<L, false, double, double, double>(srp[i], srp[i + 1], sub, col, vals, nblk, x, yy);
         if (sub == 0) {
-            if (mode == 1) {
-                y0 = mask[row * 3] != 0.0 ? y0 : 0.0;
-                y1 = mask[row * 3 + 1] != 0.0 ? y1 : 0.0;
-                y2 = mask[row * 3 + 2] != 0.0 ? y2 : 0.0;
-            }
-            y[row * 3] = y0;
-            y[row * 3 + 1] = y1;
-            y[row * 3 + 2] = y2;
+            const int64_t row = r0 + i;
+            for (int d = 0; d < 3; d++)
+                y[row * 3 + d] = (mode == 1 && mask[row * 3 + d] == 0.0) ? 0.0 : yy[d];
         }
     }
 }
 
-// r = mask .* (b - K x) in FP64; partials ||r||^2, ||b||^2, || |K||x| + |b| ||^2
-// With rold != nullptr also the partial ||r - rold||^2 (drift of a recursive
-// residual rold from the true one).
+// r = mask .* (b - K x) in FP64; per-tile partials ||r||^2, ||b||^2,
+// || |K||x| + |b| ||^2 and, with rold, the drift ||r - rold||^2
 template <int L>
-__global__ void __launch_bounds__(WGS) residual_kernel(int64_t nrows, const int32_t *__restrict__ rowptr,
+__global__ void __launch_bounds__(WGS) residual_kernel(int64_t nrows, int R, const int32_t *__restrict__ rowptr,
                                                        const int32_t *__restrict__ col,
                                                        const double *__restrict__ vals, int64_t nblk,
                                                        const double *__restrict__ x, const double *__restrict__ b,
                                                        double *__restrict__ r, const double *__restrict__ rold,
                                                        const double *__restrict__ mask, double *__restrict__ part)
 {
+    extern __shared__ __align__(16) char sm[];
+    int32_t *srp = tile_rowptr(sm);
+    double *s_rr = tile_dbl(sm, R, 0), *s_bb = tile_dbl(sm, R, 1), *s_kx = tile_dbl(sm, R, 2),
+           *s_gap = tile_dbl(sm, R, 3);
+    for (int i = threadIdx.x; i < R; i += WGS) s_rr[i] = s_bb[i] = s_kx[i] = s_gap[i] = 0.0;
+    int64_t r0;
+    const int nr = load_tile_rows(rowptr, nrows, R, srp, r0);
     constexpr int RPW = WGS / L;
     const int sub = threadIdx.x % L;
-    double rr = 0.0, bb = 0.0, kx = 0.0, gap = 0.0;
-    for (int64_t row = (int64_t)blockIdx.x * RPW + threadIdx.x / L; row < nrows; row += (int64_t)gridDim.x * RPW) {
+    for (int i = threadIdx.x / L; i < nr; i += RPW) {
         double y[3], a[3];
-        spmv_row<L, double, double, double>(row, sub, rowptr, col, vals, nblk, x, y[0], y[1], y[2]);
-        absrow<L>(row, sub, rowptr, col, vals, nblk, x, a);
+        row_sum<L, false, double, double, double>(srp[i], srp[i + 1], sub, col, vals, nblk, x, y);
+        row_sum<L, true, double, double, double>(srp[i], srp[i + 1], sub, col, vals, nblk, x, a);
         if (sub == 0) {
+            double rr = 0.0, bb = 0.0, kx = 0.0, gap = 0.0;
             for (int d = 0; d < 3; d++) {
-                const int64_t i = row * 3 + d;
-                const bool fr = mask[i] != 0.0;
-                const double ri = fr ? b[i] - y[d] : 0.0;
-                const double bi = fr ? b[i] : 0.0;
-                const double ai = fr ? a[d] + fabs(b[i]) : 0.0;
-                r[i] = ri;
+                const int64_t k = (r0 + i) * 3 + d;
+                const bool fr = mask[k] != 0.0;
+                const double ri = fr ? b[k] - y[d] : 0.0;
+                const double bi = fr ? b[k] : 0.0;
+                const double ai = fr ? a[d] + fabs(b[k]) : 0.0;
+                r[k] = ri;
                 rr = fma(ri, ri, rr);
                 bb = fma(bi, bi, bb);
                 kx = fma(ai, ai, kx);
                 if (rold) {
-                    const double g = ri - rold[i];
+                    const double g = ri - rold[k];
                     gap = fma(g, g, gap);
                 }
             }
+            s_rr[i] = rr;
+            s_bb[i] = bb;
+            s_kx[i] = kx;
+            s_gap[i] = gap;
         }
     }
-    rr = block_sum<WGS>(rr);
-    bb = block_sum<WGS>(bb);
-    kx = block_sum<WGS>(kx);
-    gap = block_sum<WGS>(gap);
+    __syncthreads();
+    const double trr = lds_tree_sum(s_rr, R), tbb = lds_tree_sum(s_bb, R), tkx = lds_tree_sum(s_kx, R),
+                 tgap = lds_tree_sum(s_gap, R);
     if (threadIdx.x == 0) {
-        part[P_A + blockIdx.x] = rr;
-        part[P_B + blockIdx.x] = bb;
-        part[P_C + blockIdx.x] = kx;
-        part[P_D + blockIdx.x] = gap;
+        part[P_A + blockIdx.x] = trr;
+        part[P_B + blockIdx.x] = tbb;
+        part[P_C + blockIdx.x] = tkx;
+        part[P_D + blockIdx.x] = tgap;
     }
 }
 
@@ -208,7 +256,7 @@ __global__ void __launch_bounds__(WGS) residual_kernel(int64_t nrows, const int3
 // initialisation, or by a residual replacement).
 
 template <int L, typename VT, typename XT>
-__global__ void __launch_bounds__(WGS) pcg_spmv_kernel(int64_t nrows, const int32_t *__restrict__ rowptr,
+__global__ void __launch_bounds__(WGS) pcg_spmv_kernel(int64_t nrows, int R, const int32_t *__restrict__ rowptr,
                                                        const int32_t *__restrict__ col,
                                                        const VT *__restrict__ vals, int64_t nblk,
                                                        const XT *__restrict__ px, const double *__restrict__ p,
@@ -216,24 +264,30 @@ __global__ void __launch_bounds__(WGS) pcg_spmv_kernel(int64_t nrows, const int3
                                                        double *__restrict__ part, const int32_t *__restrict__ ctl)
 {
     if (ctl[C_DONE]) return;
-    using AT = VT;
+    extern __shared__ __align__(16) char sm[];
+    int32_t *srp = tile_rowptr(sm);
+    double *sdot = tile_dbl(sm, R, 0);
+    for (int i = threadIdx.x; i < R; i += WGS) sdot[i] = 0.0;
+    int64_t r0;
+    const int nr = load_tile_rows(rowptr, nrows, R, srp, r0);
     constexpr int RPW = WGS / L;
     const int sub = threadIdx.x % L;
-    double pq = 0.0;
-    for (int64_t row = (int64_t)blockIdx.x * RPW + threadIdx.x / L; row < nrows; row += (int64_t)gridDim.x * RPW) {
-        AT y0, y1, y2;
-        spmv_row<L, VT, XT, AT>(row, sub, rowptr, col, vals, nblk, px, y0, y1, y2);
+    for (int i = threadIdx.x / L; i < nr; i += RPW) {
+        VT y[3];
+        row_sum<L, false, VT, XT, VT>(srp[i], srp[i + 1], sub, col, vals, nblk, px, y);
         if (sub == 0) {
-            const AT yy[3] = {y0, y1, y2};
+            double c = 0.0;
             for (int d = 0; d < 3; d++) {
-                const int64_t i = row * 3 + d;
-                const double qi = mask[i] != 0.0 ? (double)yy[d] : 0.0;
-                q[i] = qi;
-                pq = fma(p[i], qi, pq);
+                const int64_t k = (r0 + i) * 3 + d;
+                const double qk = mask[k] != 0.0 ? (double)y[d] : 0.0;
+                q[k] = qk;
+                c = fma(p[k], qk, c);
             }
+            sdot[i] = c;
         }
     }
-    pq = block_sum<WGS>(pq);
+    __syncthreads();
+    const double pq = lds_tree_sum(sdot, R);
     if (threadIdx.x == 0) part[P_PQ + blockIdx.x] = pq;
 }
 
@@ -394,50 +448,102 @@ __global__ void __launch_bounds__(WGV) pcg_finalize_kernel(int mode, int nv, int
 
 __global__ void clear_break_kernel(int32_t *__restrict__ ctl) { ctl[C_BREAK] = 0; }
 
+__global__ void fill_f32_kernel(int64_t n, float *__restrict__ x, float v)
+{
+    for (int64_t i = (int64_t)blockIdx.x * WGV + threadIdx.x; i < n; i += (int64_t)gridDim.x * WGV) x[i] = v;
+}
+
 __global__ void fill_kernel(int64_t n, double *__restrict__ x, double v)
 {
     for (int64_t i = (int64_t)blockIdx.x * WGV + threadIdx.x; i < n; i += (int64_t)gridDim.x * WGV) x[i] = v;
 }
 
 struct Launch {
-    int lanes;  // lanes per block row
+    int R;                     // block rows per SpMV workgroup
     unsigned grid_rows, grid_vec;
 };
 
 Launch pick_launch(const amr_model *m)
 {
     Launch L;
-    const double avg = m->nnodes ? (double)m->pat.nblk / (double)m->nnodes : 0.0;
-    L.lanes = avg > 40 ? 64 : (avg > 20 ? 32 : 16);
-    const int64_t rows_per_wg = WGS / L.lanes;
-    const int64_t need = (m->nnodes + rows_per_wg - 1) / rows_per_wg;
-    L.grid_rows = (unsigned)std::max<int64_t>(1, std::min<int64_t>(need, MAX_GR));
+    int R = 512;
+    while ((m->nnodes + R - 1) / R > MAX_GR) R *= 2;
+    L.R = R;
+    L.grid_rows = (unsigned)std::max<int64_t>(1, (m->nnodes + R - 1) / R);
     const int64_t nv = (m->nnodes * 3 + WGV - 1) / WGV;
     L.grid_vec = (unsigned)std::max<int64_t>(1, std::min<int64_t>(nv, MAX_GV));
     return L;
 }
 
+size_t tile_lds(int R, int ndbl) { return (size_t)(((R + 1) * 4 + 15) & ~15) + (size_t)ndbl * R * 8; }
+
+template <typename VT, typename XT>
+void launch_pcg_spmv_lanes(amr_model *m, const Launch &L, int lanes, const VT *vals, const XT *px, const double *p,
+                           double *q)
+{
+    int32_t *ctl = (int32_t *)(m->scal + S_NSCAL);
+    const size_t lds = tile_lds(L.R, 1);
+#define AMR_PCG_SPMV(LN)                                                                                           \
+    hipLaunchKernelGGL((pcg_spmv_kernel<LN, VT, XT>), dim3(L.grid_rows), dim3(WGS), lds, m->stream, m->nnodes,    \
+                       L.R, m->rowptr, m->col, vals, m->pat.nblk, px, p, q, m->mask, m->part, ctl)
+    switch (lanes) {
+    case 8: AMR_PCG_SPMV(8); break;
+    case 16: AMR_PCG_SPMV(16); break;
+    case 32: AMR_PCG_SPMV(32); break;
+    default: AMR_PCG_SPMV(64); break;
+    }
+#undef AMR_PCG_SPMV
+}
+
+// Autotuner: time each lanes-per-row choice on this model's matrix and keep
+// the fastest.  Every choice gives bit-identical results (see row_sum), so
+// the choice only affects speed.
+template <typename VT, typename XT>
+int tune_lanes(amr_model *m, const Launch &L, const VT *vals, const XT *px, const double *p, double *q, int *out)
+{
+    static const int cand[4] = {8, 16, 32, 64};
+    hipEvent_t e0, e1;
+    AMR_HIP(hipEventCreate(&e0));
+    AMR_HIP(hipEventCreate(&e1));
+    float best = 1e30f;
+    int bestL = 64;
+    for (int c : cand) {
+        launch_pcg_spmv_lanes<VT, XT>(m, L, c, vals, px, p, q);  // warm
+        AMR_HIP(hipEventRecord(e0, m->stream));
+        for (int rep = 0; rep < 3; rep++) launch_pcg_spmv_lanes<VT, XT>(m, L, c, vals, px, p, q);
+        AMR_HIP(hipEventRecord(e1, m->stream));
+        AMR_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        AMR_HIP(hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best) {
+            best = ms;
+            bestL = c;
+        }
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    *out = bestL;
+    return AMR_OK;
+}
+
 template <typename VT, typename XT>
 void launch_pcg_spmv(amr_model *m, const Launch &L, const VT *vals, const XT *px, const double *p, double *q)
 {
-    int32_t *ctl = (int32_t *)(m->scal + S_NSCAL);
-#define AMR_PCG_SPMV(LN)                                                                                           \
-    hipLaunchKernelGGL((pcg_spmv_kernel<LN, VT, XT>), dim3(L.grid_rows), dim3(WGS), 0, m->stream, m->nnodes,      \
-                       m->rowptr, m->col, vals, m->pat.nblk, px, p, q, m->mask, m->part, ctl)
-    if (L.lanes == 64) AMR_PCG_SPMV(64);
-    else if (L.lanes == 32) AMR_PCG_SPMV(32);
-    else AMR_PCG_SPMV(16);
-#undef AMR_PCG_SPMV
+    launch_pcg_spmv_lanes<VT, XT>(m, L, sizeof(VT) == 4 ? m->lanes32 : m->lanes64, vals, px, p, q);
 }
 
 void launch_residual(amr_model *m, const Launch &L, const double *x, const double *b, double *r, const double *rold)
 {
+    const size_t lds = tile_lds(L.R, 4);
 #define AMR_RES(LN)                                                                                                \
-    hipLaunchKernelGGL((residual_kernel<LN>), dim3(L.grid_rows), dim3(WGS), 0, m->stream, m->nnodes, m->rowptr,   \
-                       m->col, m->Kv, m->pat.nblk, x, b, r, rold, m->mask, m->part)
-    if (L.lanes == 64) AMR_RES(64);
-    else if (L.lanes == 32) AMR_RES(32);
-    else AMR_RES(16);
+    hipLaunchKernelGGL((residual_kernel<LN>), dim3(L.grid_rows), dim3(WGS), lds, m->stream, m->nnodes, L.R,       \
+                       m->rowptr, m->col, m->Kv, m->pat.nblk, x, b, r, rold, m->mask, m->part)
+    switch (m->lanes64) {
+    case 8: AMR_RES(8); break;
+    case 16: AMR_RES(16); break;
+    case 32: AMR_RES(32); break;
+    default: AMR_RES(64); break;
+    }
 #undef AMR_RES
 }
 
@@ -454,15 +560,39 @@ int read_scalars(amr_model *m, double *hs, int32_t *hc)
 // y = K x (mask_mode 0) or mask .* (K x) (mask_mode 1), FP64, node order.
 int amr_spmv_full(amr_model *m, const double *x, double *y, int mask_mode)
 {
-    Launch L = pick_launch(m);
+    if (int rc = amr_autotune(m)) return rc;
+    const Launch L = pick_launch(m);
+    const size_t lds = tile_lds(L.R, 0);
 #define AMR_SPMV(LN)                                                                                               \
-    hipLaunchKernelGGL((spmv_kernel<LN>), dim3(L.grid_rows), dim3(WGS), 0, m->stream, m->nnodes, m->rowptr, m->col, \
-                       m->Kv, m->pat.nblk, x, y, m->mask, mask_mode)
-    if (L.lanes == 64) AMR_SPMV(64);
-    else if (L.lanes == 32) AMR_SPMV(32);
-    else AMR_SPMV(16);
+    hipLaunchKernelGGL((spmv_kernel<LN>), dim3(L.grid_rows), dim3(WGS), lds, m->stream, m->nnodes, L.R, m->rowptr, \
+                       m->col, m->Kv, m->pat.nblk, x, y, m->mask, mask_mode)
+    switch (m->lanes64) {
+    case 8: AMR_SPMV(8); break;
+    case 16: AMR_SPMV(16); break;
+    case 32: AMR_SPMV(32); break;
+    default: AMR_SPMV(64); break;
+    }
 #undef AMR_SPMV
     AMR_HIP(hipGetLastError());
+    return AMR_OK;
+}
+
+// Pick the SpMV lanes-per-row for both precisions (once per model).
+int amr_autotune(amr_model *m)
+{
+    if (m->lanes64 && m->lanes32) return AMR_OK;
+    const Launch L = pick_launch(m);
+    int32_t *ctl = (int32_t *)(m->scal + S_NSCAL);
+    AMR_HIP(hipMemsetAsync(ctl, 0, sizeof(int32_t) * C_NCTL, m->stream));
+    const int64_t n = m->nnodes * 3;
+    hipLaunchKernelGGL(fill_kernel, dim3(L.grid_vec), dim3(WGV), 0, m->stream, n, m->v[3], 1.0);
+    hipLaunchKernelGGL(fill_f32_kernel, dim3(L.grid_vec), dim3(WGV), 0, m->stream, n, m->f[0], 1.0f);
+    AMR_HIP(hipGetLastError());
+    int l64 = 64, l32 = 64;
+    if (int rc = tune_lanes<double, double>(m, L, m->Kv, m->v[3], m->v[3], m->v[4], &l64)) return rc;
+    if (int rc = tune_lanes<float, float>(m, L, m->Kv32, m->f[0], m->v[3], m->v[4], &l32)) return rc;
+    m->lanes64 = l64;
+    m->lanes32 = l32;
     return AMR_OK;
 }
 
@@ -482,8 +612,11 @@ int amr_pcg_solve(amr_model *m, const double *b, double *x, const amr_solve_opts
     bool mixed = o->precision == AMR_PREC_MIXED;
     const int check = std::max(1, std::min(o->check_every > 0 ? o->check_every : 32, 256));
     const double kEps = 2.220446049250313e-16, kFloorC = 64.0;
-    const double replace_ratio = 0.1;   // residual drop that triggers a recomputation (mixed path)
-    const double drift_limit = 0.3;     // drift/||r|| above which FP32 is abandoned
+    // residual drop that triggers a recomputation (mixed path), and the
+    // drift/||r|| above which the FP32 operator is abandoned
+    const double replace_ratio = o->replace_ratio > 0.0 ? o->replace_ratio : 0.1;
+    const double drift_limit = o->drift_limit > 0.0 ? o->drift_limit : 0.3;
+    if (int rc = amr_autotune(m)) return rc;
     int32_t *ctl = (int32_t *)(m->scal + S_NSCAL);
     double *r = m->v[1], *z = m->v[2], *p = m->v[3], *q = m->v[4], *rt = m->v[5];
     float *p32 = m->f[0];
@@ -580,7 +713,7 @@ int amr_pcg_solve(amr_model *m, const double *b, double *x, const amr_solve_opts
         info->iterations = iters;
         info->refinements = replacements;
         info->converged = converged;
-        info->pad = fell_back;
+        info->fallback = fell_back;
         info->relres = relres;
     }
     if (!converged) {

@@ -435,23 +435,25 @@ class DeviceModel:
         _lib.check(L.amr_get_K_csc(self._h, ctypes.byref(nnz), _lib.ptr(colptr), _lib.ptr(rowval), _lib.ptr(nzval)))
         return sp.csc_matrix((nzval[:nnz.value], rowval[:nnz.value], colptr), shape=(n, n))
 
-    def solve_system(self, U, F, precision="fp64", rtol=1e-12, inner_rtol=1e-4, maxiter=200000, check_every=32):
+    def solve_system(self, U, F, precision="fp64", rtol=1e-12, maxiter=200000, check_every=32, replace_ratio=0.0,
+                     drift_limit=0.0):
         """solve_system!(K, U, F, nu): mutates U[:nu] and F[nu:]."""
         import ctypes
-        o = _lib.SolveOpts(_lib.AMR_PREC_MIXED if precision == "mixed" else _lib.AMR_PREC_FP64,
-                           _lib.AMR_PRECOND_JACOBI, int(maxiter), int(check_every), float(rtol), float(inner_rtol))
+        o = self.opts(precision, rtol, maxiter=maxiter, check_every=check_every, replace_ratio=replace_ratio,
+                      drift_limit=drift_limit)
         info = _lib.SolveInfo()
         assert U.flags["C_CONTIGUOUS"] and F.flags["C_CONTIGUOUS"] and U.dtype == F.dtype == np.float64
         rc = _lib.lib().amr_solve_system(self._h, _lib.ptr(U), _lib.ptr(F), ctypes.byref(o), ctypes.byref(info))
         _lib.check(rc)
-        return {"iterations": info.iterations, "refinements": info.refinements, "relres": info.relres,
-                "converged": bool(info.converged)}
+        return {"iterations": info.iterations, "replacements": info.refinements, "relres": info.relres,
+                "converged": bool(info.converged), "fallback": bool(info.fallback)}
 
     # ------------------------------------------- device-resident stage driver
     @staticmethod
-    def opts(precision="fp64", rtol=1e-10, inner_rtol=1e-4, maxiter=200000, check_every=32):
+    def opts(precision="fp64", rtol=1e-10, maxiter=200000, check_every=32, replace_ratio=0.0, drift_limit=0.0):
         return _lib.SolveOpts(_lib.AMR_PREC_MIXED if precision == "mixed" else _lib.AMR_PREC_FP64,
-                              _lib.AMR_PRECOND_JACOBI, int(maxiter), int(check_every), float(rtol), float(inner_rtol))
+                              _lib.AMR_PRECOND_JACOBI, int(maxiter), int(check_every), float(rtol), 0.0,
+                              float(replace_ratio), float(drift_limit))
 
     def stage_begin(self, Uex, Fex):
         Uex = np.ascontiguousarray(Uex, dtype=float)
@@ -468,7 +470,7 @@ class DeviceModel:
         info = _lib.NRInfo()
         rc = _lib.lib().amr_nr_iteration(self._h, int(it), ctypes.byref(opts), ctypes.byref(info))
         out = {"res": info.res, "err": info.err, "relres": info.relres, "pcg_iterations": info.pcg_iterations,
-               "status": rc, "message": None}
+               "replacements": info.replacements, "fallback": bool(info.fallback), "status": rc, "message": None}
         if rc != _lib.AMR_OK:
             out["message"] = _lib.lib().amr_last_error().decode(errors="replace")
             if rc in (-1, -2, -7):   # argument, runtime or memory errors are not solver failures
@@ -486,6 +488,9 @@ class DeviceModel:
         F = np.zeros(self.model.ndofs)
         _lib.check(_lib.lib().amr_stage_get(self._h, _lib.ptr(U), _lib.ptr(F), 1 if last else 0))
         return U, F
+
+    def set_spmv_lanes(self, lanes_fp64=0, lanes_fp32=0):
+        _lib.check(_lib.lib().amr_set_spmv_lanes(self._h, int(lanes_fp64), int(lanes_fp32)))
 
     def timing(self, enable):
         import ctypes

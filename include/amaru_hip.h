@@ -50,14 +50,19 @@ typedef struct {
     int maxiter;        /* cap on PCG iterations (all refinement steps together) */
     int check_every;    /* iterations between host convergence checks */
     double rtol;        /* stop when ||b - K11 x||_2 <= rtol ||b||_2 (FP64 residual) */
-    double inner_rtol;  /* mixed: relative reduction asked of each FP32 inner solve */
+    double inner_rtol;  /* reserved */
+    double replace_ratio; /* mixed: recompute the FP64 residual each time the recursive
+                             residual drops by this factor (0 = default 0.1) */
+    double drift_limit;   /* mixed: fall back to the FP64 operator when the recursive
+                             residual drifts from the true one by more than this
+                             fraction (0 = default 0.3) */
 } amr_solve_opts;
 
 typedef struct {
     int iterations;     /* PCG iterations performed (inner ones for mixed) */
-    int refinements;    /* FP64 refinement steps (mixed), 1 for FP64 */
-    int converged;      /* 1 if rtol was met */
-    int pad;
+    int refinements;    /* FP64 residual recomputations (residual replacements) */
+    int converged;      /* 1 if rtol (or the rounding floor of the residual) was met */
+    int fallback;       /* mixed: 1 if the solve finished on the FP64 operator */
     double relres;      /* final ||b - K11 x|| / ||b|| evaluated in FP64 */
 } amr_solve_info;
 
@@ -151,6 +156,8 @@ typedef struct {
     double relres;       /* final relative residual of the linear solve */
     int pcg_iterations;  /* PCG iterations of this NR iteration */
     int status;          /* AMR_OK, or the error code of a failed iteration */
+    int replacements;    /* FP64 residual recomputations in the linear solve */
+    int fallback;        /* mixed: 1 if the solve finished on the FP64 operator */
 } amr_nr_info;
 
 /* Uex, Fex (ndofs, eq order): essential values and natural loads of the
@@ -178,6 +185,11 @@ int amr_increment_reject(amr_model *m);
 /* Stage totals U and F (ndofs, eq order); either pointer may be NULL.
  * With last != 0 the last increment's ΔUa and ΔFin instead. */
 int amr_stage_get(amr_model *m, double *U, double *F, int last);
+
+/* SpMV lanes per block row (8, 16, 32 or 64) for the FP64 and FP32
+ * operators; 0 lets the autotuner choose on the next solve.  Every choice
+ * gives bit-identical results. */
+int amr_set_spmv_lanes(amr_model *m, int lanes_fp64, int lanes_fp32);
 
 /* Kernel timing of the PCG SpMV with HIP events on the model's stream:
  * enable = 1 starts (and clears) the counters, 0 stops.  Returns the

@@ -154,3 +154,26 @@ def test_solve_failure_reports(amaru):
     with pytest.raises(amaru.AmaruError):
         dev.solve_system(U, F, maxiter=2000)
     dev.close()
+
+
+def test_spmv_lanes_do_not_change_results(amaru):
+    """Every lanes-per-row choice of the SpMV autotuner gives bit-identical
+    solutions (fixed 64-slot reduction tree), for both precisions."""
+    mesh = amaru.Mesh(amaru.Block([[0, 0, 0], [1.0, 1.0, 1.0]], nx=4, ny=4, nz=4, cellshape="HEX20"))
+    model = amaru.FEModel(mesh, [("all", amaru.LinearElastic(E=30e9, nu=0.2))], nips=27)
+    bcs = [("z==0", amaru.SurfaceBC(ux=0, uy=0, uz=0)), ("z==1", amaru.SurfaceBC(tz=-1e6))]
+    model.configure_dofs(bcs)
+    U, F = model.bc_vals(bcs)
+    dev = amaru.DeviceModel(model)
+    dev.assemble()
+    for precision in ("fp64", "mixed"):
+        outs = []
+        for lanes in (8, 16, 32, 64):
+            dev.set_spmv_lanes(lanes, lanes)
+            Ug, Fg = U.copy(), F.copy()
+            info = dev.solve_system(Ug, Fg, precision=precision, rtol=1e-10)
+            outs.append((Ug, Fg, info["iterations"]))
+        for Ug, Fg, it in outs[1:]:
+            assert it == outs[0][2]
+            assert np.array_equal(Ug, outs[0][0]) and np.array_equal(Fg, outs[0][1])
+    dev.close()
