@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(HERE, "libamaru_hip.so")
 AMR_OK = 0
 AMR_HEX8, AMR_HEX20, AMR_TET10 = 1, 2, 3
 AMR_MAT_LE, AMR_MAT_VM, AMR_MAT_DP = 0, 1, 2
-AMR_PREC_FP64, AMR_PREC_MIXED = 0, 1
+AMR_PREC_FP64, AMR_PREC_MIXED, AMR_PREC_MIXED_F64ACC = 0, 1, 2
 AMR_PRECOND_JACOBI, AMR_PRECOND_BLOCK_JACOBI = 0, 1
 ERRORS = {-1: "AMR_ERR_ARG", -2: "AMR_ERR_HIP", -3: "AMR_ERR_JACOBIAN", -4: "AMR_ERR_MATERIAL",
           -5: "AMR_ERR_SOLVE", -6: "AMR_ERR_NAN", -7: "AMR_ERR_NOMEM"}
@@ -66,6 +66,7 @@ SIGNATURES = {
     "amr_stage_get": (I32, [P, P, P, I32]),
     "amr_timing": (I32, [P, I32, P, P]),
     "amr_set_spmv_lanes": (I32, [P, I32, I32]),
+    "amr_get_spmv_lanes": (I32, [P, P]),
 }
 
 _lib = None

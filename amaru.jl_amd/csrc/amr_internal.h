@@ -97,7 +97,7 @@ struct amr_model {
     double t_spmv_ms = 0.0;
     int64_t n_spmv = 0;
     // SpMV lanes per block row chosen by the autotuner (0 = not tuned yet)
-    int lanes64 = 0, lanes32 = 0;
+    int lanes64 = 0, lanes32 = 0, lanes32d = 0;
 };
 
 enum { SV_UEX = 0, SV_FEX, SV_U, SV_F, SV_DFEX, SV_R, SV_RC, SV_DUA, SV_DUI, SV_DFIN, SV_DUT, SV_X };

@@ -793,7 +793,16 @@ extern "C" int amr_set_spmv_lanes(amr_model *m, int lanes_fp64, int lanes_fp32)
         return AMR_ERR_ARG;
     }
     m->lanes64 = lanes_fp64;
-    m->lanes32 = lanes_fp32;
-    if (!lanes_fp64 || !lanes_fp32) m->lanes64 = m->lanes32 = 0;
+    m->lanes32 = m->lanes32d = lanes_fp32;
+    if (!lanes_fp64 || !lanes_fp32) m->lanes64 = m->lanes32 = m->lanes32d = 0;
+    return AMR_OK;
+}
+
+extern "C" int amr_get_spmv_lanes(amr_model *m, int *out)
+{
+    if (!m || !out) return AMR_ERR_ARG;
+    out[0] = m->lanes64;
+    out[1] = m->lanes32;
+    out[2] = m->lanes32d;
     return AMR_OK;
 }

@@ -255,7 +255,7 @@ __global__ void __launch_bounds__(WGS) residual_kernel(int64_t nrows, int R, con
 // rz_k is the sum of P_RZ[1-pi] (written by iteration k-1, by the
 // initialisation, or by a residual replacement).
 
-template <int L, typename VT, typename XT>
+template <int L, typename VT, typename XT, typename AT>
 __global__ void __launch_bounds__(WGS) pcg_spmv_kernel(int64_t nrows, int R, const int32_t *__restrict__ rowptr,
                                                        const int32_t *__restrict__ col,
                                                        const VT *__restrict__ vals, int64_t nblk,
@@ -267,14 +267,15 @@ __global__ void __launch_bounds__(WGS) pcg_spmv_kernel(int64_t nrows, int R, con
     extern __shared__ __align__(16) char sm[];
     int32_t *srp = tile_rowptr(sm);
     double *sdot = tile_dbl(sm, R, 0);
+    static_assert(sizeof(AT) >= sizeof(VT), "accumulate at least in the storage precision");
     for (int i = threadIdx.x; i < R; i += WGS) sdot[i] = 0.0;
     int64_t r0;
     const int nr = load_tile_rows(rowptr, nrows, R, srp, r0);
     constexpr int RPW = WGS / L;
     const int sub = threadIdx.x % L;
     for (int i = threadIdx.x / L; i < nr; i += RPW) {
-        VT y[3];
-        row_sum<L, false, VT, XT, VT>(srp[i], srp[i + 1], sub, col, vals, nblk, px, y);
+        AT y[3];
+        row_sum<L, false, VT, XT, AT>(srp[i], srp[i + 1], sub, col, vals, nblk, px, y);
         if (sub == 0) {
             double c = 0.0;
             for (int d = 0; d < 3; d++) {
@@ -477,14 +478,14 @@ Launch pick_launch(const amr_model *m)
 
 size_t tile_lds(int R, int ndbl) { return (size_t)(((R + 1) * 4 + 15) & ~15) + (size_t)ndbl * R * 8; }
 
-template <typename VT, typename XT>
+template <typename VT, typename XT, typename AT = VT>
 void launch_pcg_spmv_lanes(amr_model *m, const Launch &L, int lanes, const VT *vals, const XT *px, const double *p,
                            double *q)
 {
     int32_t *ctl = (int32_t *)(m->scal + S_NSCAL);
     const size_t lds = tile_lds(L.R, 1);
 #define AMR_PCG_SPMV(LN)                                                                                           \
-    hipLaunchKernelGGL((pcg_spmv_kernel<LN, VT, XT>), dim3(L.grid_rows), dim3(WGS), lds, m->stream, m->nnodes,    \
+    hipLaunchKernelGGL((pcg_spmv_kernel<LN, VT, XT, AT>), dim3(L.grid_rows), dim3(WGS), lds, m->stream, m->nnodes, \
                        L.R, m->rowptr, m->col, vals, m->pat.nblk, px, p, q, m->mask, m->part, ctl)
     switch (lanes) {
     case 8: AMR_PCG_SPMV(8); break;
@@ -498,7 +499,7 @@ void launch_pcg_spmv_lanes(amr_model *m, const Launch &L, int lanes, const VT *v
 // Autotuner: time each lanes-per-row choice on this model's matrix and keep
 // the fastest.  Every choice gives bit-identical results (see row_sum), so
 // the choice only affects speed.
-template <typename VT, typename XT>
+template <typename VT, typename XT, typename AT = VT>
 int tune_lanes(amr_model *m, const Launch &L, const VT *vals, const XT *px, const double *p, double *q, int *out)
 {
     static const int cand[4] = {8, 16, 32, 64};
@@ -508,9 +509,9 @@ int tune_lanes(amr_model *m, const Launch &L, const VT *vals, const XT *px, cons
     float best = 1e30f;
     int bestL = 64;
     for (int c : cand) {
-        launch_pcg_spmv_lanes<VT, XT>(m, L, c, vals, px, p, q);  // warm
+        launch_pcg_spmv_lanes<VT, XT, AT>(m, L, c, vals, px, p, q);  // warm
         AMR_HIP(hipEventRecord(e0, m->stream));
-        for (int rep = 0; rep < 3; rep++) launch_pcg_spmv_lanes<VT, XT>(m, L, c, vals, px, p, q);
+        for (int rep = 0; rep < 3; rep++) launch_pcg_spmv_lanes<VT, XT, AT>(m, L, c, vals, px, p, q);
         AMR_HIP(hipEventRecord(e1, m->stream));
         AMR_HIP(hipEventSynchronize(e1));
         float ms = 0.f;
@@ -526,10 +527,11 @@ int tune_lanes(amr_model *m, const Launch &L, const VT *vals, const XT *px, cons
     return AMR_OK;
 }
 
-template <typename VT, typename XT>
+template <typename VT, typename XT, typename AT = VT>
 void launch_pcg_spmv(amr_model *m, const Launch &L, const VT *vals, const XT *px, const double *p, double *q)
 {
-    launch_pcg_spmv_lanes<VT, XT>(m, L, sizeof(VT) == 4 ? m->lanes32 : m->lanes64, vals, px, p, q);
+    const int lanes = sizeof(VT) == 8 ? m->lanes64 : (sizeof(AT) == 8 ? m->lanes32d : m->lanes32);
+    launch_pcg_spmv_lanes<VT, XT, AT>(m, L, lanes, vals, px, p, q);
 }
 
 void launch_residual(amr_model *m, const Launch &L, const double *x, const double *b, double *r, const double *rold)
@@ -580,19 +582,24 @@ int amr_spmv_full(amr_model *m, const double *x, double *y, int mask_mode)
 // Pick the SpMV lanes-per-row for both precisions (once per model).
 int amr_autotune(amr_model *m)
 {
-    if (m->lanes64 && m->lanes32) return AMR_OK;
+    if (m->lanes64 && m->lanes32 && m->lanes32d) return AMR_OK;
     const Launch L = pick_launch(m);
+    int rc_ = AMR_OK;
     int32_t *ctl = (int32_t *)(m->scal + S_NSCAL);
     AMR_HIP(hipMemsetAsync(ctl, 0, sizeof(int32_t) * C_NCTL, m->stream));
     const int64_t n = m->nnodes * 3;
     hipLaunchKernelGGL(fill_kernel, dim3(L.grid_vec), dim3(WGV), 0, m->stream, n, m->v[3], 1.0);
     hipLaunchKernelGGL(fill_f32_kernel, dim3(L.grid_vec), dim3(WGV), 0, m->stream, n, m->f[0], 1.0f);
     AMR_HIP(hipGetLastError());
-    int l64 = 64, l32 = 64;
-    if (int rc = tune_lanes<double, double>(m, L, m->Kv, m->v[3], m->v[3], m->v[4], &l64)) return rc;
-    if (int rc = tune_lanes<float, float>(m, L, m->Kv32, m->f[0], m->v[3], m->v[4], &l32)) return rc;
-    m->lanes64 = l64;
-    m->lanes32 = l32;
+    int l64 = 64, l32 = 64, l32d = 64;
+    if (!m->lanes64 && (rc_ = tune_lanes<double, double>(m, L, m->Kv, m->v[3], m->v[3], m->v[4], &l64))) return rc_;
+    if (!m->lanes32 && (rc_ = tune_lanes<float, float>(m, L, m->Kv32, m->f[0], m->v[3], m->v[4], &l32))) return rc_;
+    if (!m->lanes32d &&
+        (rc_ = tune_lanes<float, double, double>(m, L, m->Kv32, m->v[3], m->v[3], m->v[4], &l32d)))
+        return rc_;
+    if (!m->lanes64) m->lanes64 = l64;
+    if (!m->lanes32) m->lanes32 = l32;
+    if (!m->lanes32d) m->lanes32d = l32d;
     return AMR_OK;
 }
 
@@ -609,7 +616,11 @@ int amr_pcg_solve(amr_model *m, const double *b, double *x, const amr_solve_opts
 {
     const int64_t n = m->nnodes * 3;
     const Launch L = pick_launch(m);
-    bool mixed = o->precision == AMR_PREC_MIXED;
+    // mixed: FP32 values; f32x: FP32 search direction and FP32 accumulation
+    // (AMR_PREC_MIXED), otherwise FP64 gather and accumulation of the FP32
+    // values (AMR_PREC_MIXED_F64ACC)
+    bool mixed = o->precision == AMR_PREC_MIXED || o->precision == AMR_PREC_MIXED_F64ACC;
+    const bool f32x = o->precision == AMR_PREC_MIXED;
     const int check = std::max(1, std::min(o->check_every > 0 ? o->check_every : 32, 256));
     const double kEps = 2.220446049250313e-16, kFloorC = 64.0;
     // residual drop that triggers a recomputation (mixed path), and the
@@ -630,7 +641,7 @@ int amr_pcg_solve(amr_model *m, const double *b, double *x, const amr_solve_opts
     }
     const double tr = 0.5 * o->rtol;
     hipLaunchKernelGGL(pcg_init_kernel, dim3(L.grid_vec), dim3(WGV), 0, m->stream, n, b, x, r, z, p,
-                       mixed ? p32 : nullptr, m->dinv, m->dinv32, mixed ? 1 : 0, m->part);
+                       mixed && f32x ? p32 : nullptr, m->dinv, m->dinv32, mixed ? 1 : 0, m->part);
     hipLaunchKernelGGL(pcg_finalize_kernel, dim3(1), dim3(WGV), 0, m->stream, 0, (int)L.grid_vec, 0, 1, tr * tr,
                        m->part, ctl, m->scal);
     AMR_HIP(hipGetLastError());
@@ -643,13 +654,15 @@ int amr_pcg_solve(amr_model *m, const double *b, double *x, const amr_solve_opts
         for (int j = 0; j < check; j++, k++) {
             const int pi = k & 1;
             if (m->timing && mixed) AMR_HIP(hipEventRecord(m->ev[2 * j], m->stream));
-            if (mixed) launch_pcg_spmv<float, float>(m, L, m->Kv32, p32, p, q);
+            if (mixed && f32x) launch_pcg_spmv<float, float>(m, L, m->Kv32, p32, p, q);
+            else if (mixed) launch_pcg_spmv<float, double, double>(m, L, m->Kv32, p, p, q);
             else launch_pcg_spmv<double, double>(m, L, m->Kv, p, p, q);
             if (m->timing && mixed) AMR_HIP(hipEventRecord(m->ev[2 * j + 1], m->stream));
             hipLaunchKernelGGL(pcg_update_kernel, dim3(L.grid_vec), dim3(WGV), 0, m->stream, n, x, r, z, p, q, m->dinv,
                                m->dinv32, mixed ? 1 : 0, m->part, (int)L.grid_rows, (int)L.grid_vec, pi, ctl);
-            hipLaunchKernelGGL(pcg_dir_kernel, dim3(L.grid_vec), dim3(WGV), 0, m->stream, n, p, mixed ? p32 : nullptr,
-                               z, m->part, (int)L.grid_vec, pi, k + 1, o->maxiter, ctl, m->scal);
+            hipLaunchKernelGGL(pcg_dir_kernel, dim3(L.grid_vec), dim3(WGV), 0, m->stream, n, p,
+                               mixed && f32x ? p32 : nullptr, z, m->part, (int)L.grid_vec, pi, k + 1, o->maxiter, ctl,
+                               m->scal);
         }
         AMR_HIP(hipGetLastError());
         if (int rc = read_scalars(m, hs, hc)) return rc;
@@ -698,7 +711,8 @@ int amr_pcg_solve(amr_model *m, const double *b, double *x, const amr_solve_opts
         }
         const int slot = 1 - (iters & 1);
         hipLaunchKernelGGL(pcg_rez_kernel, dim3(L.grid_vec), dim3(WGV), 0, m->stream, n, rt, r, z, p,
-                           mixed ? p32 : nullptr, restart ? 1 : 0, m->dinv, m->dinv32, mixed ? 1 : 0, m->part, slot);
+                           mixed && f32x ? p32 : nullptr, restart ? 1 : 0, m->dinv, m->dinv32, mixed ? 1 : 0, m->part,
+                           slot);
         hipLaunchKernelGGL(pcg_finalize_kernel, dim3(1), dim3(WGV), 0, m->stream, 2, (int)L.grid_vec,
                            (int)L.grid_rows, slot, 0.0, m->part, ctl, m->scal);
         AMR_HIP(hipGetLastError());

@@ -451,7 +451,8 @@ class DeviceModel:
     # ------------------------------------------- device-resident stage driver
     @staticmethod
     def opts(precision="fp64", rtol=1e-10, maxiter=200000, check_every=32, replace_ratio=0.0, drift_limit=0.0):
-        return _lib.SolveOpts(_lib.AMR_PREC_MIXED if precision == "mixed" else _lib.AMR_PREC_FP64,
+        prec = {"fp64": _lib.AMR_PREC_FP64, "mixed": _lib.AMR_PREC_MIXED, "mixed64acc": _lib.AMR_PREC_MIXED_F64ACC}
+        return _lib.SolveOpts(prec[precision],
                               _lib.AMR_PRECOND_JACOBI, int(maxiter), int(check_every), float(rtol), 0.0,
                               float(replace_ratio), float(drift_limit))
 
@@ -488,6 +489,11 @@ class DeviceModel:
         F = np.zeros(self.model.ndofs)
         _lib.check(_lib.lib().amr_stage_get(self._h, _lib.ptr(U), _lib.ptr(F), 1 if last else 0))
         return U, F
+
+    def spmv_lanes(self):
+        out = np.zeros(3, dtype=np.int32)
+        _lib.check(_lib.lib().amr_get_spmv_lanes(self._h, _lib.ptr(out)))
+        return {"fp64": int(out[0]), "fp32": int(out[1]), "fp32_f64acc": int(out[2])}
 
     def set_spmv_lanes(self, lanes_fp64=0, lanes_fp32=0):
         _lib.check(_lib.lib().amr_set_spmv_lanes(self._h, int(lanes_fp64), int(lanes_fp32)))

@@ -38,14 +38,17 @@ enum {
 
 enum { AMR_HEX8 = 1, AMR_HEX20 = 2, AMR_TET10 = 3 };
 enum { AMR_MAT_LE = 0, AMR_MAT_VM = 1, AMR_MAT_DP = 2 };
-enum { AMR_PREC_FP64 = 0, AMR_PREC_MIXED = 1 };
+enum { AMR_PREC_FP64 = 0, AMR_PREC_MIXED = 1, AMR_PREC_MIXED_F64ACC = 2 };
 enum { AMR_PRECOND_JACOBI = 0, AMR_PRECOND_BLOCK_JACOBI = 1 };
 
 typedef struct amr_model amr_model;
 
 typedef struct {
-    int precision;      /* AMR_PREC_FP64: FP64 Jacobi-PCG.  AMR_PREC_MIXED: FP32
-                           SpMV/preconditioner PCG inside FP64 iterative refinement */
+    int precision;      /* AMR_PREC_FP64: FP64 Jacobi-PCG.  AMR_PREC_MIXED: FP32 matrix,
+                           FP32 search direction and FP32 accumulation in the SpMV,
+                           FP32 preconditioner, FP64 vectors/dots/residual.
+                           AMR_PREC_MIXED_F64ACC: FP32 matrix gathered against the FP64
+                           search direction with FP64 accumulation */
     int precond;        /* AMR_PRECOND_* */
     int maxiter;        /* cap on PCG iterations (all refinement steps together) */
     int check_every;    /* iterations between host convergence checks */
@@ -190,6 +193,9 @@ int amr_stage_get(amr_model *m, double *U, double *F, int last);
  * operators; 0 lets the autotuner choose on the next solve.  Every choice
  * gives bit-identical results. */
 int amr_set_spmv_lanes(amr_model *m, int lanes_fp64, int lanes_fp32);
+/* out[0..2]: lanes chosen for the FP64, FP32 and FP32-storage/FP64-accumulation
+ * SpMVs (0 = not tuned yet). */
+int amr_get_spmv_lanes(amr_model *m, int *out);
 
 /* Kernel timing of the PCG SpMV with HIP events on the model's stream:
  * enable = 1 starts (and clears) the counters, 0 stops.  Returns the

@@ -115,7 +115,7 @@ def test_update_state_and_tangent_bit_exact(amaru, shape, nips, mat):
     dev.close()
 
 
-@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("precision", ["fp64", "mixed", "mixed64acc"])
 @pytest.mark.parametrize("shape,nips", [("HEX8", 0), ("HEX20", 27), ("TET10", 0)])
 def test_solve_system_vs_direct(amaru, shape, nips, precision):
     mesh = amaru.Mesh(amaru.Block([[0, 0, 0], [1.0, 0.2, 0.2]], nx=10, ny=2, nz=2, cellshape=shape))
@@ -166,7 +166,7 @@ def test_spmv_lanes_do_not_change_results(amaru):
     U, F = model.bc_vals(bcs)
     dev = amaru.DeviceModel(model)
     dev.assemble()
-    for precision in ("fp64", "mixed"):
+    for precision in ("fp64", "mixed", "mixed64acc"):
         outs = []
         for lanes in (8, 16, 32, 64):
             dev.set_spmv_lanes(lanes, lanes)
