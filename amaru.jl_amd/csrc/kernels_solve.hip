@@ -37,7 +37,11 @@ constexpr int MAX_GR = 1024, MAX_GV = 512;
 
 // scalar and control slots
 enum { S_RZ = 0, S_RR, S_BB, S_THR, S_KX, S_RES, S_ERRN, S_ERRD, S_GAP, S_NSCAL = 16 };
-enum { C_DONE = 0, C_ITERS, C_BREAK, C_NAN, C_NCTL = 16 };
+// C_DONE is read by every workgroup at kernel entry and written only where
+// all workgroups of the running launch agree on it (breakdown); the
+// convergence test writes C_STOP, which the next SpMV promotes to C_DONE, so
+// no workgroup ever sees a flag raised during its own launch.
+enum { C_DONE = 0, C_ITERS, C_BREAK, C_NAN, C_STOP, C_NCTL = 16 };
 // partial buffers inside m->part
 enum { P_PQ = 0, P_RZ0 = 1024, P_RZ1 = 1536, P_RR0 = 2048, P_RR1 = 2560, P_A = 4096, P_B = 5120, P_C = 6144,
        P_D = 7168 };
@@ -264,6 +268,12 @@ __global__ void __launch_bounds__(WGS) pcg_spmv_kernel(int64_t nrows, int R, con
                                                        double *__restrict__ part, const int32_t *__restrict__ ctl)
 {
     if (ctl[C_DONE]) return;
+    if (ctl[C_STOP]) {
+        // converged in the previous iteration: every workgroup leaves, and
+        // the first one raises C_DONE for the kernels that follow
+        if (blockIdx.x == 0 && threadIdx.x == 0) ctl[C_DONE] = 1;
+        return;
+    }
     extern __shared__ __align__(16) char sm[];
     int32_t *srp = tile_rowptr(sm);
     double *sdot = tile_dbl(sm, R, 0);
@@ -344,7 +354,7 @@ __global__ void __launch_bounds__(WGV) pcg_dir_kernel(int64_t n, double *__restr
         ctl[C_ITERS] = iter;
         scal[S_RR] = rr;
         scal[S_RZ] = rzn;
-        if (rr <= scal[S_THR] || iter >= maxiter || !std::isfinite(rr)) ctl[C_DONE] = 1;
+        if (rr <= scal[S_THR] || iter >= maxiter || !std::isfinite(rr)) ctl[C_STOP] = 1;
     }
     for (int64_t i = (int64_t)blockIdx.x * WGV + threadIdx.x; i < n; i += (int64_t)gridDim.x * WGV) {
         const double pn = fma(beta, p[i], z[i]);
@@ -425,6 +435,7 @@ __global__ void __launch_bounds__(WGV) pcg_finalize_kernel(int mode, int nv, int
             scal[S_THR] = thr_fac * rr;
             ctl[C_ITERS] = 0;
             ctl[C_BREAK] = 0;
+            ctl[C_STOP] = 0;
             ctl[C_DONE] = rr == 0.0 ? 1 : 0;
         }
     } else if (mode == 1) {
@@ -443,6 +454,7 @@ __global__ void __launch_bounds__(WGV) pcg_finalize_kernel(int mode, int nv, int
         if (threadIdx.x == 0) {
             scal[S_RZ] = rz;
             ctl[C_DONE] = 0;
+            ctl[C_STOP] = 0;
         }
     }
 }
@@ -680,7 +692,7 @@ int amr_pcg_solve(amr_model *m, const double *b, double *x, const amr_solve_opts
         iters = hc[C_ITERS];
         broke = hc[C_BREAK];
         k = iters;
-        const bool done = hc[C_DONE] != 0;
+        const bool done = hc[C_DONE] != 0 || hc[C_STOP] != 0;
         if (!(done || (mixed && hs[S_RR] <= replace_ratio * replace_ratio * rr_ref))) {
             if (broke || iters >= o->maxiter) break;
             continue;
