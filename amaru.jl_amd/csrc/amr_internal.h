@@ -74,6 +74,8 @@ struct amr_model {
     uint8_t *Kflag = nullptr;     // nblk*9 survivor flags (exact pattern)
     double *dinv = nullptr;       // 3*nnodes Jacobi inverse (0 at prescribed)
     float *dinv32 = nullptr;
+    double *binv = nullptr;       // 9*nnodes nodal block-Jacobi inverses
+    float *binv32 = nullptr;
     double *mask = nullptr;       // 3*nnodes: 1 free, 0 prescribed
     int32_t *eqd = nullptr;       // 3*nnodes eq ids (device)
     double *dF = nullptr;         // ne*nn*3 element internal forces

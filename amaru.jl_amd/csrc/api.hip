@@ -178,7 +178,7 @@ static void free_model(amr_model *m)
     void *ptrs[] = {m->X,    m->conn, m->emat, m->mats,  m->dNdR, m->wip,  m->st_cur, m->st_com, m->Ke,
                     m->rowptr, m->col, m->diag, m->cptr,  m->csrc, m->fptr, m->fsrc,   m->Kv,     m->Kv32,
                     m->Kflag, m->dinv, m->dinv32, m->mask, m->eqd,  m->dF,   m->estat,  m->err,    m->part,
-                    m->scal};
+                    m->scal, m->binv, m->binv32};
     for (void *p : ptrs)
         if (p) hipFree(p);
     for (double *p : m->v)
@@ -303,6 +303,8 @@ extern "C" int amr_model_create(amr_model **out, int device, int etype, int nips
     TRY(dalloc(&m->Kflag, (size_t)m->pat.nblk * 9));
     TRY(dalloc(&m->dinv, n3));
     TRY(dalloc(&m->dinv32, n3));
+    TRY(dalloc(&m->binv, n3 * 3));
+    TRY(dalloc(&m->binv32, n3 * 3));
     TRY(dalloc(&m->mask, n3));
     TRY(dalloc(&m->eqd, n3));
     TRY(dalloc(&m->dF, (size_t)ne * nn * 3));

@@ -410,6 +410,48 @@ __global__ void jacobi_kernel(int64_t nnodes, int64_t nblk, const int32_t *__res
     dinv32[i] = fr ? 1.0f / (float)kd : 0.0f;
 }
 
+// 3x3 nodal block Jacobi: inverse of each node's (symmetrised) diagonal
+// block restricted to the node's free dofs; rows/columns of prescribed dofs
+// are zero.
+__global__ void block_jacobi_kernel(int64_t nnodes, int64_t nblk, const int32_t *__restrict__ diag,
+                                    const double *__restrict__ Kv, const double *__restrict__ mask,
+                                    double *__restrict__ binv, float *__restrict__ binv32)
+{
+    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= nnodes) return;
+    const int32_t db = diag[n];
+    double A[3][3], I[3][3];
+    int fr[3], nf = 0, idx[3];
+    for (int r = 0; r < 3; r++) {
+        fr[r] = db >= 0 && mask[n * 3 + r] != 0.0;
+        if (fr[r]) idx[nf++] = r;
+        for (int c = 0; c < 3; c++) {
+            A[r][c] = db >= 0 ? 0.5 * (Kv[(int64_t)(r * 3 + c) * nblk + db] + Kv[(int64_t)(c * 3 + r) * nblk + db]) : 0.0;
+            I[r][c] = 0.0;
+        }
+    }
+    if (nf == 1) {
+        const int a = idx[0];
+        I[a][a] = 1.0 / A[a][a];
+    } else if (nf == 2) {
+        const int a = idx[0], b = idx[1];
+        const double det = A[a][a] * A[b][b] - A[a][b] * A[b][a];
+        I[a][a] = A[b][b] / det;
+        I[b][b] = A[a][a] / det;
+        I[a][b] = -A[a][b] / det;
+        I[b][a] = -A[b][a] / det;
+    } else if (nf == 3) {
+        double J[9], inv[9];
+        for (int k = 0; k < 9; k++) J[k] = A[k / 3][k % 3];
+        inv3(J, inv);
+        for (int k = 0; k < 9; k++) I[k / 3][k % 3] = inv[k];
+    }
+    for (int k = 0; k < 9; k++) {
+        binv[n * 9 + k] = I[k / 3][k % 3];
+        binv32[n * 9 + k] = (float)I[k / 3][k % 3];
+    }
+}
+
 // ------------------------------------------------------------- IP update
 //
 // EPW elements per 256-thread workgroup.  Phase A: one thread per IP builds
@@ -601,6 +643,10 @@ int amr_launch_assemble(amr_model *m, bool flags)
     unsigned g2 = (unsigned)((m->nnodes * 3 + 255) / 256);
     hipLaunchKernelGGL(jacobi_kernel, dim3(g2), dim3(256), 0, m->stream, m->nnodes, m->pat.nblk, m->diag, m->Kv, m->mask,
                        m->dinv, m->dinv32);
+    AMR_HIP(hipGetLastError());
+    const unsigned g3 = (unsigned)((m->nnodes + 255) / 256);
+    hipLaunchKernelGGL(block_jacobi_kernel, dim3(g3), dim3(256), 0, m->stream, m->nnodes, m->pat.nblk, m->diag, m->Kv,
+                       m->mask, m->binv, m->binv32);
     AMR_HIP(hipGetLastError());
     return AMR_OK;
 }

@@ -81,6 +81,36 @@ __device__ inline double sum_partials(const double *__restrict__ P, int n)
     return block_sum<NT>(v);
 }
 
+// Preconditioner: Jacobi (inverse diagonal) or 3x3 nodal block Jacobi
+// (inverse of each node's diagonal block restricted to its free dofs), in
+// FP64 or applied in FP32 (mixed paths).
+struct Prec {
+    const double *dinv;
+    const float *dinv32;
+    const double *binv;   // 9 per node, node-major
+    const float *binv32;
+    int f32, block;
+};
+
+__device__ inline void apply_prec(const Prec &P, int64_t node, const double r[3], double z[3])
+{
+    const int64_t i = node * 3;
+    if (P.block) {
+        if (P.f32) {
+            const float *B = P.binv32 + node * 9;
+            const float r0 = (float)r[0], r1 = (float)r[1], r2 = (float)r[2];
+            for (int d = 0; d < 3; d++) z[d] = (double)fmaf(B[d * 3 + 2], r2, fmaf(B[d * 3 + 1], r1, B[d * 3] * r0));
+        } else {
+            const double *B = P.binv + node * 9;
+            for (int d = 0; d < 3; d++) z[d] = fma(B[d * 3 + 2], r[2], fma(B[d * 3 + 1], r[1], B[d * 3] * r[0]));
+        }
+    } else if (P.f32) {
+        for (int d = 0; d < 3; d++) z[d] = (double)(P.dinv32[i + d] * (float)r[d]);
+    } else {
+        for (int d = 0; d < 3; d++) z[d] = P.dinv[i + d] * r[d];
+    }
+}
+
 // ------------------------------------------------------------------ SpMV
 //
 // Row tiles: workgroup b owns block rows [b*R, (b+1)*R) (R a power of two
@@ -265,7 +295,7 @@ __global__ void __launch_bounds__(WGS) pcg_spmv_kernel(int64_t nrows, int R, con
                                                        const VT *__restrict__ vals, int64_t nblk,
                                                        const XT *__restrict__ px, const double *__restrict__ p,
                                                        double *__restrict__ q, const double *__restrict__ mask,
-                                                       double *__restrict__ part, const int32_t *__restrict__ ctl)
+                                                       double *__restrict__ part, int32_t *__restrict__ ctl)
 {
     if (ctl[C_DONE]) return;
     if (ctl[C_STOP]) {
@@ -302,13 +332,11 @@ __global__ void __launch_bounds__(WGS) pcg_spmv_kernel(int64_t nrows, int R, con
     if (threadIdx.x == 0) part[P_PQ + blockIdx.x] = pq;
 }
 
-__global__ void __launch_bounds__(WGV) pcg_update_kernel(int64_t n, double *__restrict__ x, double *__restrict__ r,
-                                                         double *__restrict__ z, const double *__restrict__ p,
-                                                         const double *__restrict__ q,
-                                                         const double *__restrict__ dinv,
-                                                         const float *__restrict__ dinv32, int prec32,
-                                                         double *__restrict__ part, int npq, int nv, int pi,
-                                                         int32_t *__restrict__ ctl)
+__global__ void __launch_bounds__(WGV) pcg_update_kernel(int64_t nnodes, double *__restrict__ x,
+                                                         double *__restrict__ r, double *__restrict__ z,
+                                                         const double *__restrict__ p, const double *__restrict__ q,
+                                                         const Prec P, double *__restrict__ part, int npq, int nv,
+                                                         int pi, int32_t *__restrict__ ctl)
 {
     if (ctl[C_DONE]) return;
     const double pq = sum_partials<WGV>(part + P_PQ, npq);
@@ -322,15 +350,20 @@ __global__ void __launch_bounds__(WGV) pcg_update_kernel(int64_t n, double *__re
     }
     const double a = rz / pq;
     double rzn = 0.0, rr = 0.0;
-    for (int64_t i = (int64_t)blockIdx.x * WGV + threadIdx.x; i < n; i += (int64_t)gridDim.x * WGV) {
-        const double xi = fma(a, p[i], x[i]);
-        const double ri = fma(-a, q[i], r[i]);
-        const double zi = prec32 ? (double)(dinv32[i] * (float)ri) : dinv[i] * ri;
-        x[i] = xi;
-        r[i] = ri;
-        z[i] = zi;
-        rzn = fma(ri, zi, rzn);
-        rr = fma(ri, ri, rr);
+    for (int64_t nd = (int64_t)blockIdx.x * WGV + threadIdx.x; nd < nnodes; nd += (int64_t)gridDim.x * WGV) {
+        double rv[3], zv[3];
+        for (int d = 0; d < 3; d++) {
+            const int64_t i = nd * 3 + d;
+            x[i] = fma(a, p[i], x[i]);
+            rv[d] = fma(-a, q[i], r[i]);
+            r[i] = rv[d];
+        }
+        apply_prec(P, nd, rv, zv);
+        for (int d = 0; d < 3; d++) {
+            z[nd * 3 + d] = zv[d];
+            rzn = fma(rv[d], zv[d], rzn);
+            rr = fma(rv[d], rv[d], rr);
+        }
     }
     rzn = block_sum<WGV>(rzn);
     rr = block_sum<WGV>(rr);
@@ -364,24 +397,27 @@ __global__ void __launch_bounds__(WGV) pcg_dir_kernel(int64_t n, double *__restr
 }
 
 // x = 0, r = b, z = M r, p = z; partials rz -> P_RZ1, rr -> P_RR1
-__global__ void __launch_bounds__(WGV) pcg_init_kernel(int64_t n, const double *__restrict__ b,
+__global__ void __launch_bounds__(WGV) pcg_init_kernel(int64_t nnodes, const double *__restrict__ b,
                                                        double *__restrict__ x, double *__restrict__ r,
                                                        double *__restrict__ z, double *__restrict__ p,
-                                                       float *__restrict__ p32, const double *__restrict__ dinv,
-                                                       const float *__restrict__ dinv32, int prec32,
+                                                       float *__restrict__ p32, const Prec P,
                                                        double *__restrict__ part)
 {
     double rz = 0.0, rr = 0.0;
-    for (int64_t i = (int64_t)blockIdx.x * WGV + threadIdx.x; i < n; i += (int64_t)gridDim.x * WGV) {
-        const double ri = b[i];
-        const double zi = prec32 ? (double)(dinv32[i] * (float)ri) : dinv[i] * ri;
-        x[i] = 0.0;
-        r[i] = ri;
-        z[i] = zi;
-        p[i] = zi;
-        if (p32) p32[i] = (float)zi;
-        rz = fma(ri, zi, rz);
-        rr = fma(ri, ri, rr);
+    for (int64_t nd = (int64_t)blockIdx.x * WGV + threadIdx.x; nd < nnodes; nd += (int64_t)gridDim.x * WGV) {
+        double rv[3], zv[3];
+        for (int d = 0; d < 3; d++) rv[d] = b[nd * 3 + d];
+        apply_prec(P, nd, rv, zv);
+        for (int d = 0; d < 3; d++) {
+            const int64_t i = nd * 3 + d;
+            x[i] = 0.0;
+            r[i] = rv[d];
+            z[i] = zv[d];
+            p[i] = zv[d];
+            if (p32) p32[i] = (float)zv[d];
+            rz = fma(rv[d], zv[d], rz);
+            rr = fma(rv[d], rv[d], rr);
+        }
     }
     rz = block_sum<WGV>(rz);
     rr = block_sum<WGV>(rr);
@@ -391,27 +427,28 @@ __global__ void __launch_bounds__(WGV) pcg_init_kernel(int64_t n, const double *
     }
 }
 
-// After a residual recomputation: z = M r and the rz partials in the slot
-// the next iteration reads as "previous".
-// r = rt; with restart also p = z (the search direction starts afresh).
-__global__ void __launch_bounds__(WGV) pcg_rez_kernel(int64_t n, const double *__restrict__ rt,
+// r = rt; z = M r; with restart also p = z (the search direction starts
+// afresh); rz partials into the slot the next iteration reads as "previous".
+__global__ void __launch_bounds__(WGV) pcg_rez_kernel(int64_t nnodes, const double *__restrict__ rt,
                                                       double *__restrict__ r, double *__restrict__ z,
                                                       double *__restrict__ p, float *__restrict__ p32, int restart,
-                                                      const double *__restrict__ dinv,
-                                                      const float *__restrict__ dinv32, int prec32,
-                                                      double *__restrict__ part, int slot_pi)
+                                                      const Prec P, double *__restrict__ part, int slot_pi)
 {
     double rz = 0.0;
-    for (int64_t i = (int64_t)blockIdx.x * WGV + threadIdx.x; i < n; i += (int64_t)gridDim.x * WGV) {
-        const double ri = rt[i];
-        const double zi = prec32 ? (double)(dinv32[i] * (float)ri) : dinv[i] * ri;
-        r[i] = ri;
-        z[i] = zi;
-        if (restart) {
-            p[i] = zi;
-            if (p32) p32[i] = (float)zi;
+    for (int64_t nd = (int64_t)blockIdx.x * WGV + threadIdx.x; nd < nnodes; nd += (int64_t)gridDim.x * WGV) {
+        double rv[3], zv[3];
+        for (int d = 0; d < 3; d++) rv[d] = rt[nd * 3 + d];
+        apply_prec(P, nd, rv, zv);
+        for (int d = 0; d < 3; d++) {
+            const int64_t i = nd * 3 + d;
+            r[i] = rv[d];
+            z[i] = zv[d];
+            if (restart) {
+                p[i] = zv[d];
+                if (p32) p32[i] = (float)zv[d];
+            }
+            rz = fma(rv[d], zv[d], rz);
         }
-        rz = fma(ri, zi, rz);
     }
     rz = block_sum<WGV>(rz);
     if (threadIdx.x == 0) part[(slot_pi ? P_RZ1 : P_RZ0) + blockIdx.x] = rz;
@@ -533,8 +570,8 @@ int tune_lanes(amr_model *m, const Launch &L, const VT *vals, const XT *px, cons
             bestL = c;
         }
     }
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
     *out = bestL;
     return AMR_OK;
 }
@@ -647,13 +684,14 @@ int amr_pcg_solve(amr_model *m, const double *b, double *x, const amr_solve_opts
     int32_t hc[C_NCTL];
 
     if (m->timing && (int)m->ev.size() < 2 * check) {
-        for (hipEvent_t e : m->ev) hipEventDestroy(e);
+        for (hipEvent_t e : m->ev) (void)hipEventDestroy(e);
         m->ev.assign(2 * check, nullptr);
         for (hipEvent_t &e : m->ev) AMR_HIP(hipEventCreate(&e));
     }
     const double tr = 0.5 * o->rtol;
-    hipLaunchKernelGGL(pcg_init_kernel, dim3(L.grid_vec), dim3(WGV), 0, m->stream, n, b, x, r, z, p,
-                       mixed && f32x ? p32 : nullptr, m->dinv, m->dinv32, mixed ? 1 : 0, m->part);
+    Prec P{m->dinv, m->dinv32, m->binv, m->binv32, mixed ? 1 : 0, o->precond == AMR_PRECOND_BLOCK_JACOBI ? 1 : 0};
+    hipLaunchKernelGGL(pcg_init_kernel, dim3(L.grid_vec), dim3(WGV), 0, m->stream, m->nnodes, b, x, r, z, p,
+                       mixed && f32x ? p32 : nullptr, P, m->part);
     hipLaunchKernelGGL(pcg_finalize_kernel, dim3(1), dim3(WGV), 0, m->stream, 0, (int)L.grid_vec, 0, 1, tr * tr,
                        m->part, ctl, m->scal);
     AMR_HIP(hipGetLastError());
@@ -665,20 +703,20 @@ int amr_pcg_solve(amr_model *m, const double *b, double *x, const amr_solve_opts
     while (!converged) {
         for (int j = 0; j < check; j++, k++) {
             const int pi = k & 1;
-            if (m->timing && mixed) AMR_HIP(hipEventRecord(m->ev[2 * j], m->stream));
+            if (m->timing) AMR_HIP(hipEventRecord(m->ev[2 * j], m->stream));
             if (mixed && f32x) launch_pcg_spmv<float, float>(m, L, m->Kv32, p32, p, q);
             else if (mixed) launch_pcg_spmv<float, double, double>(m, L, m->Kv32, p, p, q);
             else launch_pcg_spmv<double, double>(m, L, m->Kv, p, p, q);
-            if (m->timing && mixed) AMR_HIP(hipEventRecord(m->ev[2 * j + 1], m->stream));
-            hipLaunchKernelGGL(pcg_update_kernel, dim3(L.grid_vec), dim3(WGV), 0, m->stream, n, x, r, z, p, q, m->dinv,
-                               m->dinv32, mixed ? 1 : 0, m->part, (int)L.grid_rows, (int)L.grid_vec, pi, ctl);
+            if (m->timing) AMR_HIP(hipEventRecord(m->ev[2 * j + 1], m->stream));
+            hipLaunchKernelGGL(pcg_update_kernel, dim3(L.grid_vec), dim3(WGV), 0, m->stream, m->nnodes, x, r, z, p, q,
+                               P, m->part, (int)L.grid_rows, (int)L.grid_vec, pi, ctl);
             hipLaunchKernelGGL(pcg_dir_kernel, dim3(L.grid_vec), dim3(WGV), 0, m->stream, n, p,
                                mixed && f32x ? p32 : nullptr, z, m->part, (int)L.grid_vec, pi, k + 1, o->maxiter, ctl,
                                m->scal);
         }
         AMR_HIP(hipGetLastError());
         if (int rc = read_scalars(m, hs, hc)) return rc;
-        if (m->timing && mixed) {
+        if (m->timing) {
             // the SpMVs of the iterations run in this batch executed in full;
             // later launches returned at once on the done flag
             const int real = std::min(hc[C_ITERS] - iters, check);
@@ -722,9 +760,9 @@ int amr_pcg_solve(amr_model *m, const double *b, double *x, const amr_solve_opts
             restart = true;
         }
         const int slot = 1 - (iters & 1);
-        hipLaunchKernelGGL(pcg_rez_kernel, dim3(L.grid_vec), dim3(WGV), 0, m->stream, n, rt, r, z, p,
-                           mixed && f32x ? p32 : nullptr, restart ? 1 : 0, m->dinv, m->dinv32, mixed ? 1 : 0, m->part,
-                           slot);
+        P.f32 = mixed ? 1 : 0;
+        hipLaunchKernelGGL(pcg_rez_kernel, dim3(L.grid_vec), dim3(WGV), 0, m->stream, m->nnodes, rt, r, z, p,
+                           mixed && f32x ? p32 : nullptr, restart ? 1 : 0, P, m->part, slot);
         hipLaunchKernelGGL(pcg_finalize_kernel, dim3(1), dim3(WGV), 0, m->stream, 2, (int)L.grid_vec,
                            (int)L.grid_rows, slot, 0.0, m->part, ctl, m->scal);
         AMR_HIP(hipGetLastError());

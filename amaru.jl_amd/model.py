@@ -436,11 +436,11 @@ class DeviceModel:
         return sp.csc_matrix((nzval[:nnz.value], rowval[:nnz.value], colptr), shape=(n, n))
 
     def solve_system(self, U, F, precision="fp64", rtol=1e-12, maxiter=200000, check_every=32, replace_ratio=0.0,
-                     drift_limit=0.0):
+                     drift_limit=0.0, precond="jacobi"):
         """solve_system!(K, U, F, nu): mutates U[:nu] and F[nu:]."""
         import ctypes
         o = self.opts(precision, rtol, maxiter=maxiter, check_every=check_every, replace_ratio=replace_ratio,
-                      drift_limit=drift_limit)
+                      drift_limit=drift_limit, precond=precond)
         info = _lib.SolveInfo()
         assert U.flags["C_CONTIGUOUS"] and F.flags["C_CONTIGUOUS"] and U.dtype == F.dtype == np.float64
         rc = _lib.lib().amr_solve_system(self._h, _lib.ptr(U), _lib.ptr(F), ctypes.byref(o), ctypes.byref(info))
@@ -450,10 +450,11 @@ class DeviceModel:
 
     # ------------------------------------------- device-resident stage driver
     @staticmethod
-    def opts(precision="fp64", rtol=1e-10, maxiter=200000, check_every=32, replace_ratio=0.0, drift_limit=0.0):
+    def opts(precision="fp64", rtol=1e-10, maxiter=200000, check_every=32, replace_ratio=0.0, drift_limit=0.0,
+             precond="jacobi"):
         prec = {"fp64": _lib.AMR_PREC_FP64, "mixed": _lib.AMR_PREC_MIXED, "mixed64acc": _lib.AMR_PREC_MIXED_F64ACC}
-        return _lib.SolveOpts(prec[precision],
-                              _lib.AMR_PRECOND_JACOBI, int(maxiter), int(check_every), float(rtol), 0.0,
+        pc = {"jacobi": _lib.AMR_PRECOND_JACOBI, "block_jacobi": _lib.AMR_PRECOND_BLOCK_JACOBI}
+        return _lib.SolveOpts(prec[precision], pc[precond], int(maxiter), int(check_every), float(rtol), 0.0,
                               float(replace_ratio), float(drift_limit))
 
     def stage_begin(self, Uex, Fex):

@@ -20,7 +20,9 @@ import time
 import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
-SPMV_KERNEL = "pcg_spmv_kernel<64, float, float>"
+SPMV_KERNEL = {"mixed": "pcg_spmv_kernel<L, float, float, float>",
+               "mixed64acc": "pcg_spmv_kernel<L, float, double, double>",
+               "fp64": "pcg_spmv_kernel<L, double, double, double>"}
 
 
 def load_pkg():
@@ -94,6 +96,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=40, help="elements per edge (40 = the 811,923-DOF block)")
     ap.add_argument("--plain-steps", type=int, default=2)
+    ap.add_argument("--precision", default="mixed64acc", choices=["mixed", "mixed64acc", "fp64"])
+    ap.add_argument("--precond", default="block_jacobi", choices=["jacobi", "block_jacobi"])
+    ap.add_argument("--variant-steps", type=int, default=1)
     ap.add_argument("--rtol", type=float, default=1e-10)
     ap.add_argument("--cpu-n", type=int, default=12, help="HEX20 elements per edge of the CPU sample")
     ap.add_argument("--no-cpu", action="store_true")
@@ -111,8 +116,14 @@ def main():
     dev.stage_begin(Uex, Fex)
     log(f"model: {info['nnodes']} nodes, {info['ne']} elements, ndofs {model.ndofs}, nu {model.nu}, "
         f"{info['nblk']} blocks; setup {time.perf_counter() - t0:.1f}s")
-    mixed = dev.opts(precision="mixed", rtol=args.rtol)
-    plain = dev.opts(precision="fp64", rtol=args.rtol)
+    # headline: FP32-stored K with FP64 gather/accumulation, nodal block-Jacobi
+    # in FP32, FP64 residual replacement; the literal FP32-accumulation variant
+    # and Jacobi variants are measured beside it
+    mixed = dev.opts(precision=args.precision, rtol=args.rtol, precond=args.precond)
+    plain = dev.opts(precision="fp64", rtol=args.rtol, precond="jacobi")
+    variants = {"mixed_f32acc_jacobi": dev.opts(precision="mixed", rtol=args.rtol, precond="jacobi"),
+                "mixed_f64acc_jacobi": dev.opts(precision="mixed64acc", rtol=args.rtol, precond="jacobi"),
+                "fp64_block_jacobi": dev.opts(precision="fp64", rtol=args.rtol, precond="block_jacobi")}
 
     # accuracy reference: FP64 PCG to 1e-13 (a direct factorisation of this
     # K11 does not fit a CPU run)
@@ -134,13 +145,27 @@ def main():
     err_plain = float(np.linalg.norm(Upl - Uref) / np.linalg.norm(Uref))
     log(f"plain fp64: {np.mean(lat_p):.3f}s/step, PCG its {its_p}, err {err_plain:.2e}")
 
+    extra = {}
+    for name, o in variants.items():
+        if args.variant_steps <= 0:
+            break
+        lv, iv, rv = run_steps(dev, o, args.variant_steps)
+        Uv, _ = dev.stage_get(last=True)
+        extra[name] = {"value": model.nu * len(lv) / float(np.sum(lv)), "ms_per_step": 1e3 * float(np.mean(lv)),
+                       "pcg": iv, "rel_err": float(np.linalg.norm(Uv - Uref) / np.linalg.norm(Uref))}
+        log(f"{name}: {np.mean(lv):.3f}s/step, {iv}")
+    lanes = dev.spmv_lanes()
+
     nu = model.nu
     total = float(np.sum(lat))
     value = nu * args.steps / total
     plain_value = nu * len(lat_p) / float(np.sum(lat_p))
     nblk, nnodes = info["nblk"], info["nnodes"]
-    # FP32 3x3 blocks + int32 cols + rowptr + FP32 p gather (once) + FP64 q write + FP64 p read for p.q
-    spmv_bytes = 40 * nblk + 4 * (nnodes + 1) + 12 * nnodes + 24 * nnodes + 24 * nnodes
+    # 3x3 blocks (FP32 or FP64) + int32 cols + rowptr + p gather (once, FP32 or
+    # FP64) + FP64 q write + FP64 p read for p.q
+    vb = 8 if args.precision == "fp64" else 4
+    xb = 4 if args.precision == "mixed" else 8
+    spmv_bytes = (9 * vb + 4) * nblk + 4 * (nnodes + 1) + 3 * xb * nnodes + 24 * nnodes + 24 * nnodes
     avg_s = spmv_ms / 1e3 / max(spmv_n, 1)
     achieved = spmv_bytes / avg_s / 1e9 if spmv_n else None
     out = {
@@ -154,17 +179,23 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 SpMV/precond in f64 refinement; f64 assembly and state",
+        "dtype": {"mixed": "f32 SpMV (f32 accumulation), f32 precond, f64 vectors/dots/residual",
+                  "mixed64acc": "f32-stored K gathered/accumulated in f64, f32 precond, f64 vectors/dots/residual",
+                  "fp64": "f64"}[args.precision] + "; f64 assembly and IP state",
         "data": "synthetic",
         "config": {"workload": f"HEX20 block {args.n}^3, 27 IP, E=30GPa nu=0.2, base clamped, top pressure 1MPa",
-                   "ndofs": model.ndofs, "unknowns": nu, "rtol": args.rtol, "step": "mount_K+solve_system!+update_state!"},
+                   "ndofs": model.ndofs, "unknowns": nu, "rtol": args.rtol, "precond": args.precond,
+                   "step": "mount_K+solve_system!+update_state!", "spmv_lanes": lanes},
+        "variants": extra,
         "latency_s": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
         "plain_fp64": {"value": plain_value, "ms_per_step": 1e3 * float(np.mean(lat_p)),
                        "pcg_iterations": its_p, "rel_err": err_plain},
         "speedup_vs_plain_fp64": value / plain_value,
         "pcg_iterations": its,
         "rel_err_vs_fp64_pcg_1e-13": err_mixed,
-        "roofline": {"bound": "hbm", "kernel": SPMV_KERNEL, "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
+        "roofline": {"bound": "hbm", "kernel": SPMV_KERNEL[args.precision].replace("L", str(
+                         lanes["fp64"] if args.precision == "fp64" else
+                         lanes["fp32"] if args.precision == "mixed" else lanes["fp32_f64acc"])), "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
                      "frac": achieved / 8000.0 if achieved else None, "traffic": None,
                      "bytes_per_launch": spmv_bytes, "avg_launch_us": avg_s * 1e6, "launches": spmv_n},
     }

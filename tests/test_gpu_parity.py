@@ -115,9 +115,10 @@ def test_update_state_and_tangent_bit_exact(amaru, shape, nips, mat):
     dev.close()
 
 
-@pytest.mark.parametrize("precision", ["fp64", "mixed", "mixed64acc"])
+@pytest.mark.parametrize("precision,precond", [("fp64", "jacobi"), ("mixed", "jacobi"), ("mixed64acc", "jacobi"),
+                                               ("fp64", "block_jacobi"), ("mixed64acc", "block_jacobi")])
 @pytest.mark.parametrize("shape,nips", [("HEX8", 0), ("HEX20", 27), ("TET10", 0)])
-def test_solve_system_vs_direct(amaru, shape, nips, precision):
+def test_solve_system_vs_direct(amaru, shape, nips, precision, precond):
     mesh = amaru.Mesh(amaru.Block([[0, 0, 0], [1.0, 0.2, 0.2]], nx=10, ny=2, nz=2, cellshape=shape))
     model = amaru.FEModel(mesh, [("all", amaru.LinearElastic(E=200e9, nu=0.3))], nips=nips)
     bcs = [("x==0", amaru.SurfaceBC(ux=0, uy=0, uz=0)), ("x==1", amaru.SurfaceBC(tz=-1e5)),
@@ -128,7 +129,7 @@ def test_solve_system_vs_direct(amaru, shape, nips, precision):
     dev.assemble()
     K = dev.K_csc().tocsr()
     Ug, Fg = U.copy(), F.copy()
-    info = dev.solve_system(Ug, Fg, precision=precision, rtol=1e-12)
+    info = dev.solve_system(Ug, Fg, precision=precision, rtol=1e-12, precond=precond)
     Uo, Fo = U.copy(), F.copy()
     assert fem.solve_system(K, Uo, Fo, nu) is None
     err = np.linalg.norm(Ug - Uo) / np.linalg.norm(Uo)
