@@ -21,6 +21,13 @@ struct AmrMat {
 
 void amr_set_error(const std::string &msg);
 
+// K block values in chunks of 64 consecutive blocks: inside a chunk, slot s
+// (row-major 3x3 position) of all 64 blocks is contiguous, so the lanes of a
+// wave reading consecutive blocks touch one contiguous 64-value run per slot
+// and a chunk's 9 slots are one contiguous region.
+__host__ __device__ inline int64_t kv_at(int64_t k, int s) { return (k >> 6) * 576 + (int64_t)s * 64 + (k & 63); }
+inline int64_t kv_size(int64_t nblk) { return ((nblk + 63) >> 6) * 576; }
+
 #define AMR_HIP(call)                                                                  \
     do {                                                                               \
         hipError_t _e = (call);                                                        \
@@ -69,7 +76,7 @@ struct amr_model {
     int64_t *cptr = nullptr;
     int32_t *csrc = nullptr;
     int32_t *fptr = nullptr, *fsrc = nullptr;
-    double *Kv = nullptr;         // 9*nblk FP64 values, slot-major (s*nblk + k)
+    double *Kv = nullptr;         // FP64 block values, kv_at(k, s) layout
     float *Kv32 = nullptr;        // nblk*9 FP32 values
     uint8_t *Kflag = nullptr;     // nblk*9 survivor flags (exact pattern)
     double *dinv = nullptr;       // 3*nnodes Jacobi inverse (0 at prescribed)

@@ -298,9 +298,9 @@ extern "C" int amr_model_create(amr_model **out, int device, int etype, int nips
     TRY(dalloc(&m->csrc, (size_t)m->pat.ncontrib));
     TRY(dalloc(&m->fptr, (size_t)nnodes + 1));
     TRY(dalloc(&m->fsrc, m->pat.fsrc.size()));
-    TRY(dalloc(&m->Kv, (size_t)m->pat.nblk * 9));
-    TRY(dalloc(&m->Kv32, (size_t)m->pat.nblk * 9));
-    TRY(dalloc(&m->Kflag, (size_t)m->pat.nblk * 9));
+    TRY(dalloc(&m->Kv, (size_t)kv_size(m->pat.nblk)));
+    TRY(dalloc(&m->Kv32, (size_t)kv_size(m->pat.nblk)));
+    TRY(dalloc(&m->Kflag, (size_t)kv_size(m->pat.nblk)));
     TRY(dalloc(&m->dinv, n3));
     TRY(dalloc(&m->dinv32, n3));
     TRY(dalloc(&m->binv, n3 * 3));
@@ -310,7 +310,7 @@ extern "C" int amr_model_create(amr_model **out, int device, int etype, int nips
     TRY(dalloc(&m->dF, (size_t)ne * nn * 3));
     TRY(dalloc(&m->estat, (size_t)ne));
     TRY(dalloc(&m->err, 4));
-    TRY(dalloc(&m->part, 16384));
+    TRY(dalloc(&m->part, 24576));
     TRY(dalloc(&m->scal, 64));
     for (double *&p : m->v) TRY(dalloc(&p, n3));
     for (float *&p : m->f) TRY(dalloc(&p, n3));
@@ -464,8 +464,8 @@ extern "C" int amr_get_K_csc(amr_model *m, int64_t *nnz, int64_t *colptr, int32_
     }
     AMR_HIP(hipSetDevice(m->device));
     const int64_t nb = m->pat.nblk;
-    std::vector<double> val((size_t)nb * 9);
-    std::vector<uint8_t> fl((size_t)nb * 9);
+    std::vector<double> val((size_t)kv_size(nb));
+    std::vector<uint8_t> fl((size_t)kv_size(nb));
     TRY(d2h(val.data(), m->Kv, val.size(), m->stream));
     TRY(d2h(fl.data(), m->Kflag, fl.size(), m->stream));
     AMR_HIP(hipStreamSynchronize(m->stream));
@@ -476,7 +476,7 @@ extern "C" int amr_get_K_csc(amr_model *m, int64_t *nnz, int64_t *colptr, int32_
             const int64_t B = m->pat.col[k];
             for (int r = 0; r < 3; r++)
                 for (int c = 0; c < 3; c++) {
-                    if (!fl[(size_t)(r * 3 + c) * nb + k]) continue;
+                    if (!fl[kv_at(k, r * 3 + c)]) continue;
                     const int32_t i = m->eq[A * 3 + r], j = m->eq[B * 3 + c];
                     if (i < 0 || j < 0) continue;
                     cnt[j + 1]++;
@@ -494,12 +494,12 @@ extern "C" int amr_get_K_csc(amr_model *m, int64_t *nnz, int64_t *colptr, int32_
             const int64_t B = m->pat.col[k];
             for (int r = 0; r < 3; r++)
                 for (int c = 0; c < 3; c++) {
-                    if (!fl[(size_t)(r * 3 + c) * nb + k]) continue;
+                    if (!fl[kv_at(k, r * 3 + c)]) continue;
                     const int32_t i = m->eq[A * 3 + r], j = m->eq[B * 3 + c];
                     if (i < 0 || j < 0) continue;
                     int64_t p = pos[j]++;
                     rowval[p] = i;
-                    nzval[p] = val[(size_t)(r * 3 + c) * nb + k];
+                    nzval[p] = val[kv_at(k, r * 3 + c)];
                 }
         }
     // sort row ids within each column

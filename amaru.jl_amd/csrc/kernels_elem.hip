@@ -359,7 +359,7 @@ __global__ void __launch_bounds__(64) ke_kernel(int64_t ne, const double *__rest
 // order, each scalar only if |v| >= eps (mount_K's filter), starting from
 // 0.0 (0.0 + v1 == v1).  Writes FP64 and FP32 values and, when asked, the
 // survivor flags that define the reference's scalar pattern.  Values are
-// stored slot-major: scalar s of block k at s*nblk + k.
+// stored in the kv_at(k, s) chunked layout.
 __global__ void __launch_bounds__(256) assemble_kernel(int64_t nblk, const int64_t *__restrict__ cptr,
                                                        const int32_t *__restrict__ csrc,
                                                        const double *__restrict__ Ke,
@@ -385,12 +385,12 @@ __global__ void __launch_bounds__(256) assemble_kernel(int64_t nblk, const int64
     }
 #pragma unroll
     for (int s = 0; s < 9; s++) {
-        Kv[s * nblk + k] = acc[s];
-        Kv32[s * nblk + k] = (float)acc[s];
+        Kv[kv_at(k, s)] = acc[s];
+        Kv32[kv_at(k, s)] = (float)acc[s];
     }
     if (flags) {
 #pragma unroll
-        for (int s = 0; s < 9; s++) flags[s * nblk + k] = (fl >> s) & 1u;
+        for (int s = 0; s < 9; s++) flags[kv_at(k, s)] = (fl >> s) & 1u;
     }
 }
 
@@ -404,7 +404,7 @@ __global__ void jacobi_kernel(int64_t nnodes, int64_t nblk, const int32_t *__res
     int64_t n = i / 3;
     int d = (int)(i % 3);
     const int32_t db = diag[n];  // -1 for a node in no element (it has no dofs)
-    double kd = db >= 0 ? Kv[(int64_t)(d * 4) * nblk + db] : 1.0;
+    double kd = db >= 0 ? Kv[kv_at(db, d * 4)] : 1.0;
     bool fr = mask[i] != 0.0 && db >= 0;
     dinv[i] = fr ? 1.0 / kd : 0.0;
     dinv32[i] = fr ? 1.0f / (float)kd : 0.0f;
@@ -426,7 +426,7 @@ __global__ void block_jacobi_kernel(int64_t nnodes, int64_t nblk, const int32_t 
         fr[r] = db >= 0 && mask[n * 3 + r] != 0.0;
         if (fr[r]) idx[nf++] = r;
         for (int c = 0; c < 3; c++) {
-            A[r][c] = db >= 0 ? 0.5 * (Kv[(int64_t)(r * 3 + c) * nblk + db] + Kv[(int64_t)(c * 3 + r) * nblk + db]) : 0.0;
+            A[r][c] = db >= 0 ? 0.5 * (Kv[kv_at(db, r * 3 + c)] + Kv[kv_at(db, c * 3 + r)]) : 0.0;
             I[r][c] = 0.0;
         }
     }

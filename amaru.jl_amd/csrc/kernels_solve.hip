@@ -33,7 +33,7 @@ namespace {
 
 constexpr int WGV = 256;  // vector kernels
 constexpr int WGS = 512;  // SpMV kernels
-constexpr int MAX_GR = 1024, MAX_GV = 512;
+constexpr int MAX_GR = 4096, MAX_GV = 512;
 
 // scalar and control slots
 enum { S_RZ = 0, S_RR, S_BB, S_THR, S_KX, S_RES, S_ERRN, S_ERRD, S_GAP, S_NSCAL = 16 };
@@ -43,8 +43,8 @@ enum { S_RZ = 0, S_RR, S_BB, S_THR, S_KX, S_RES, S_ERRN, S_ERRD, S_GAP, S_NSCAL 
 // no workgroup ever sees a flag raised during its own launch.
 enum { C_DONE = 0, C_ITERS, C_BREAK, C_NAN, C_STOP, C_NCTL = 16 };
 // partial buffers inside m->part
-enum { P_PQ = 0, P_RZ0 = 1024, P_RZ1 = 1536, P_RR0 = 2048, P_RR1 = 2560, P_A = 4096, P_B = 5120, P_C = 6144,
-       P_D = 7168 };
+enum { P_PQ = 0, P_RZ0 = 4096, P_RZ1 = 4608, P_RR0 = 5120, P_RR1 = 5632, P_A = 8192, P_B = 12288, P_C = 16384,
+       P_D = 20480, P_END = 24576 };
 
 template <int NT>
 __device__ inline double block_sum(double v)
@@ -135,17 +135,20 @@ __device__ inline void row_sum(int32_t rb, int32_t re, int sub, const int32_t *_
         for (int t = 0; t < NS; t++) {
             const int32_t k = base + sub + L * t;
             if (k < re) {
-                const int64_t c = col[k];
+                const int64_t c = __builtin_nontemporal_load(col + k);
                 const AT x0 = (AT)x[c * 3], x1 = (AT)x[c * 3 + 1], x2 = (AT)x[c * 3 + 2];
-                const VT *v = vals + k;
+                const VT *vp = vals + kv_at(k, 0);
+                AT v[9];
+#pragma unroll
+                for (int q = 0; q < 9; q++) v[q] = (AT)__builtin_nontemporal_load(vp + q * 64);
                 if (ABS) {
-                    acc[t][0] += fabs((AT)v[0] * x0) + fabs((AT)v[nblk] * x1) + fabs((AT)v[2 * nblk] * x2);
-                    acc[t][1] += fabs((AT)v[3 * nblk] * x0) + fabs((AT)v[4 * nblk] * x1) + fabs((AT)v[5 * nblk] * x2);
-                    acc[t][2] += fabs((AT)v[6 * nblk] * x0) + fabs((AT)v[7 * nblk] * x1) + fabs((AT)v[8 * nblk] * x2);
+                    acc[t][0] += fabs(v[0] * x0) + fabs(v[1] * x1) + fabs(v[2] * x2);
+                    acc[t][1] += fabs(v[3] * x0) + fabs(v[4] * x1) + fabs(v[5] * x2);
+                    acc[t][2] += fabs(v[6] * x0) + fabs(v[7] * x1) + fabs(v[8] * x2);
                 } else {
-                    acc[t][0] = fma((AT)v[2 * nblk], x2, fma((AT)v[nblk], x1, fma((AT)v[0], x0, acc[t][0])));
-                    acc[t][1] = fma((AT)v[5 * nblk], x2, fma((AT)v[4 * nblk], x1, fma((AT)v[3 * nblk], x0, acc[t][1])));
-                    acc[t][2] = fma((AT)v[8 * nblk], x2, fma((AT)v[7 * nblk], x1, fma((AT)v[6 * nblk], x0, acc[t][2])));
+                    acc[t][0] = fma(v[2], x2, fma(v[1], x1, fma(v[0], x0, acc[t][0])));
+                    acc[t][1] = fma(v[5], x2, fma(v[4], x1, fma(v[3], x0, acc[t][1])));
+                    acc[t][2] = fma(v[8], x2, fma(v[7], x1, fma(v[6], x0, acc[t][2])));
                 }
             }
         }
@@ -188,10 +191,19 @@ __device__ inline double *tile_dbl(char *sm, int R, int j)
     return (double *)(sm + off) + (int64_t)j * R;
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs (workgroup b runs on
+// XCD b % 8).  Tile t(b) gives each XCD one contiguous run of row tiles, so
+// the slice of x its rows gather stays in that XCD's L2.
+__device__ inline int64_t xcd_tile(int64_t b, int64_t G)
+{
+    const int64_t x = b & 7, i = b >> 3, q = G >> 3, rm = G & 7;
+    return x * q + (x < rm ? x : rm) + i;
+}
+
 __device__ inline int load_tile_rows(const int32_t *__restrict__ rowptr, int64_t nrows, int R, int32_t *srp,
                                      int64_t &r0)
 {
-    r0 = (int64_t)blockIdx.x * R;
+    r0 = xcd_tile(blockIdx.x, gridDim.x) * R;
     const int nr = (int)std::min<int64_t>(R, nrows - r0);
     for (int i = threadIdx.x; i <= nr; i += blockDim.x) srp[i] = rowptr[r0 + i];
     __syncthreads();
@@ -272,10 +284,11 @@ __global__ void __launch_bounds__(WGS) residual_kernel(int64_t nrows, int R, con
     const double trr = lds_tree_sum(s_rr, R), tbb = lds_tree_sum(s_bb, R), tkx = lds_tree_sum(s_kx, R),
                  tgap = lds_tree_sum(s_gap, R);
     if (threadIdx.x == 0) {
-        part[P_A + blockIdx.x] = trr;
-        part[P_B + blockIdx.x] = tbb;
-        part[P_C + blockIdx.x] = tkx;
-        part[P_D + blockIdx.x] = tgap;
+        const int64_t t = xcd_tile(blockIdx.x, gridDim.x);
+        part[P_A + t] = trr;
+        part[P_B + t] = tbb;
+        part[P_C + t] = tkx;
+        part[P_D + t] = tgap;
     }
 }
 
@@ -329,7 +342,7 @@ __global__ void __launch_bounds__(WGS) pcg_spmv_kernel(int64_t nrows, int R, con
     }
     __syncthreads();
     const double pq = lds_tree_sum(sdot, R);
-    if (threadIdx.x == 0) part[P_PQ + blockIdx.x] = pq;
+    if (threadIdx.x == 0) part[P_PQ + xcd_tile(blockIdx.x, gridDim.x)] = pq;
 }
 
 __global__ void __launch_bounds__(WGV) pcg_update_kernel(int64_t nnodes, double *__restrict__ x,
@@ -516,7 +529,7 @@ struct Launch {
 Launch pick_launch(const amr_model *m)
 {
     Launch L;
-    int R = 512;
+    int R = 128;
     while ((m->nnodes + R - 1) / R > MAX_GR) R *= 2;
     L.R = R;
     L.grid_rows = (unsigned)std::max<int64_t>(1, (m->nnodes + R - 1) / R);
@@ -872,7 +885,7 @@ __global__ void __launch_bounds__(WGV) nr_finalize_kernel(int nv, const double *
 {
     double v[4] = {0.0, 0.0, 0.0, 0.0};
     for (int i = threadIdx.x; i < nv; i += WGV)
-        for (int k = 0; k < 4; k++) v[k] = fmax(v[k], part[P_A + k * 1024 + i]);
+        for (int k = 0; k < 4; k++) v[k] = fmax(v[k], part[P_A + k * 4096 + i]);
     for (int k = 0; k < 4; k++) v[k] = block_max<WGV>(v[k]);
     if (threadIdx.x == 0) {
         scal[S_RES] = v[0];
