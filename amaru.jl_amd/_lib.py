@@ -2,7 +2,7 @@
 
 There is no fallback: if the library is missing or does not load, every
 entry point raises.  The signatures below are the ones a Julia ``ccall``
-glue would use (see INTEGRATION.md).
+glue would use.
 """
 import ctypes
 import os

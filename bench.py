@@ -104,8 +104,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (each GPU would hold whole models; "
-              "see DESIGN.md for how the path would shard). Run with --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU: there is no multi-GPU path (models would be sharded whole, one GPU "
+              "per model, or a single model by node ranges with halo exchange). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
     am = load_pkg()
@@ -126,7 +126,7 @@ def main():
                 "fp64_block_jacobi": dev.opts(precision="fp64", rtol=args.rtol, precond="block_jacobi")}
 
     # accuracy reference: FP64 PCG to 1e-13 (a direct factorisation of this
-    # K11 does not fit a CPU run)
+    # K11, 134M nonzeros, is out of reach for a CPU run)
     r = run_steps(dev, dev.opts(precision="fp64", rtol=1e-13), 1)[2]
     Uref, _ = dev.stage_get(last=True)
     log(f"reference FP64 PCG(1e-13): {r['pcg_iterations']} iterations, relres {r['relres']:.2e}")
