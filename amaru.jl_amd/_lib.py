@@ -16,7 +16,7 @@ AMR_OK = 0
 AMR_HEX8, AMR_HEX20, AMR_TET10 = 1, 2, 3
 AMR_MAT_LE, AMR_MAT_VM, AMR_MAT_DP = 0, 1, 2
 AMR_PREC_FP64, AMR_PREC_MIXED, AMR_PREC_MIXED_F64ACC = 0, 1, 2
-AMR_PRECOND_JACOBI, AMR_PRECOND_BLOCK_JACOBI = 0, 1
+AMR_PRECOND_JACOBI, AMR_PRECOND_BLOCK_JACOBI, AMR_PRECOND_TWO_LEVEL = 0, 1, 2
 ERRORS = {-1: "AMR_ERR_ARG", -2: "AMR_ERR_HIP", -3: "AMR_ERR_JACOBIAN", -4: "AMR_ERR_MATERIAL",
           -5: "AMR_ERR_SOLVE", -6: "AMR_ERR_NAN", -7: "AMR_ERR_NOMEM"}
 
@@ -67,6 +67,21 @@ SIGNATURES = {
     "amr_timing": (I32, [P, I32, P, P]),
     "amr_set_spmv_lanes": (I32, [P, I32, I32]),
     "amr_get_spmv_lanes": (I32, [P, P]),
+    "amr_nr_prepare": (I32, [P, I32]),
+    "amr_nr_finish": (I32, [P, ctypes.POINTER(SolveInfo), ctypes.POINTER(NRInfo)]),
+    "amr_pool_create": (I32, [P, I32, I32, I32]),
+    "amr_pool_destroy": (I32, [P]),
+    "amr_pool_attach": (I32, [P, P]),
+    "amr_pool_detach": (I32, [P]),
+    "amr_pool_add": (I32, [P, P, ctypes.POINTER(SolveOpts)]),
+    "amr_pool_step": (I32, [P, I32, P, P, ctypes.POINTER(SolveInfo)]),
+    "amr_pool_size": (I32, [P, P]),
+    "amr_pool_timing": (I32, [P, I32, P, P]),
+    "amr_model_device_bytes": (I32, [P, P]),
+    "amr_mem_info": (I32, [I32, P, P]),
+    "amr_set_coarse": (I32, [P, I32, I32]),
+    "amr_get_coarse": (I32, [P, P]),
+    "amr_get_coarse_space": (I32, [P, P, P]),
 }
 
 _lib = None

@@ -48,6 +48,7 @@ class MechAnalysis:
         self.log = []          # (stage, inc, it, residue) like solve.log
         self.stats = dict(nr_iterations=0, pcg_iterations=0, increments=0, dof_solves=0)
         self.dev = None
+        self.opts = None       # amr_solve_opts of the running stage
 
 
 def addstage(ana, bcs, nincs=1, nouts=0):
@@ -59,6 +60,19 @@ def addstage(ana, bcs, nincs=1, nouts=0):
 
 def solve(ana, **kwargs):
     """solve!(ana; args...) → (success, message)."""
+    args = solver_args(**kwargs)
+    status = None
+    gen = analysis_machine(ana, args)
+    try:
+        it = next(gen)
+        while True:
+            it = gen.send(ana.dev.nr_iteration(it, ana.opts))
+    except StopIteration as stop:
+        status = stop.value
+    return status
+
+
+def solver_args(**kwargs):
     unknown = set(kwargs) - set(SOLVER_DEFAULTS)
     if unknown:
         raise TypeError(f"solve!: unknown arguments {sorted(unknown)}")
@@ -67,6 +81,18 @@ def solve(ana, **kwargs):
         raise NotImplementedError("only the :FE predictor-corrector scheme is implemented on the device")
     if not 1 <= args["maxits"] <= 10:
         raise ValueError("maxits must be in 1..10")
+    return args
+
+
+def analysis_machine(ana, args):
+    """stage_iterator! (src/solver.jl:82-198) as a generator.
+
+    It yields the number ``it`` of each Newton-Raphson iteration it needs
+    and expects the ``DeviceModel.nr_iteration``-style result of that
+    iteration to be sent back; it returns (success, message).  ``solve``
+    answers each request at once; a pool runner answers them when the
+    model's linear solve, advanced together with other models', is done.
+    """
     todo = [s for s in ana.stages if s.status != "done"]
     if not todo:
         raise RuntimeError("stage_iterator!: No stages have been set")
@@ -77,7 +103,7 @@ def solve(ana, **kwargs):
         if stage.nouts > 0 and not args["autoinc"] and stage.nouts > nincs:
             nincs = stage.nouts
         stage.nincs = nincs
-        ok, msg = _stage_solver(ana, stage, args)
+        ok, msg = yield from _stage_solver(ana, stage, args)
         stage.status = "done" if ok else "failed"
         if not ok:
             break
@@ -97,8 +123,8 @@ def _stage_solver(ana, stage, a):
     dev = ana.dev
     Uex, Fex = model.bc_vals(bcs)
     dev.stage_begin(Uex, Fex)
-    opts = DeviceModel.opts(precision=a["precision"], rtol=a["pcg_rtol"], maxiter=a["pcg_maxiter"],
-                            precond=a["precond"])
+    ana.opts = DeviceModel.opts(precision=a["precision"], rtol=a["pcg_rtol"], maxiter=a["pcg_maxiter"],
+                                precond=a["precond"])
     # eq id -> flat (node, dir) index; dof.vals start values of this stage
     flat = np.zeros(ndofs, dtype=np.int64)
     live = model.eq.ravel() >= 0
@@ -132,7 +158,7 @@ def _stage_solver(ana, stage, a):
         sysmsg = None
         for it in range(1, maxits + 1):
             lastres = res
-            r = dev.nr_iteration(it, opts)   # mount_K, solve_system!, update_state!, R, norms
+            r = yield it   # mount_K, solve_system!, update_state!, R, norms
             ana.stats["pcg_iterations"] += r["pcg_iterations"]
             if r["status"] != 0:
                 syserror = True

@@ -52,6 +52,25 @@ struct AmrPattern {
 
 int amr_build_pattern(int nn, int64_t nnodes, int64_t ne, const int32_t *conn, AmrPattern &P);
 
+// Host-built coarse space of the two-level preconditioner (model_build.cpp).
+struct AmrCoarse {
+    int32_t nagg = 0;
+    std::vector<int32_t> agg;       // nnodes: aggregate of each node
+    std::vector<int32_t> aggptr;    // nagg+1
+    std::vector<int32_t> aggnodes;  // nnodes: nodes of each aggregate, ascending
+    std::vector<double> cent;       // 3*nagg: aggregate centroids
+    std::vector<int32_t> pptr;      // nnodes+1: (row, aggregate) pairs of each block row
+    std::vector<int32_t> pagg;      // npairs: the pair's aggregate
+    std::vector<int32_t> prow;      // npairs: the pair's block row
+    std::vector<int32_t> cba, cbb;  // coarse blocks (a, b), ascending
+    std::vector<int32_t> cbptr;     // ncb+1
+    std::vector<int32_t> cbslot;    // npairs: pair slots of each coarse block, ascending row
+};
+int amr_build_coarse(int64_t nnodes, const double *X, const int32_t *rowptr, const int32_t *col, int target,
+                     AmrCoarse &C);
+
+struct amr_pool;
+
 // Per-model device resources.
 struct amr_model {
     int etype = 0, nn = 0, nips = 0, device = 0;
@@ -94,7 +113,24 @@ struct amr_model {
     double *part = nullptr;       // reduction partials
     double *scal = nullptr;       // device scalars
     int nparts = 0;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;      // where the model's work runs (a pool's stream while attached)
+    hipStream_t own_stream = nullptr;  // the stream created with the model
+    amr_pool *self_pool = nullptr;     // pool of one for amr_pcg_solve and the autotuner
+    amr_pool *pool = nullptr;          // shared pool the model is attached to
+    int64_t dev_bytes = 0;             // device memory allocated for the model
+    std::vector<double> Xh;            // host copy of the coordinates
+    int64_t k_version = 0;             // bumped by every assembly
+    // two-level preconditioner: aggregates of about coarse_target nodes (more
+    // when the model would need over coarse_max aggregates), rigid-body modes
+    // per aggregate, Kc = Phi^T K Phi inverted on the device
+    int coarse_target = 256, coarse_max = 170;
+    bool coarse_built = false;
+    int64_t coarse_version = -1;
+    int32_t nagg = 0, ncoarse = 0, ncb = 0;
+    int64_t npairs = 0;
+    int32_t *c_agg = nullptr, *c_aggptr = nullptr, *c_aggnodes = nullptr, *c_prow = nullptr, *c_pagg = nullptr;
+    int32_t *c_cba = nullptr, *c_cbb = nullptr, *c_cbptr = nullptr, *c_cbslot = nullptr, *c_info = nullptr;
+    double *c_cent = nullptr, *c_pair = nullptr, *Kinv = nullptr, *c_vec = nullptr, *y_vec = nullptr;
     bool assembled = false;
     // device-resident stage vectors (node order): Uex Fex U F dFex R Rc dUa dUi dFin dUt x
     double *sv[12] = {};
@@ -125,7 +161,27 @@ int amr_launch_ke(amr_model *m);
 int amr_launch_assemble(amr_model *m, bool flags);
 int amr_launch_update(amr_model *m, const double *dU_node /*device, node order*/, double *dFin_node);
 
+// two-level coarse space (coarse.hip): build on first use, refresh Kc^-1
+// after every assembly; queued on the model's stream
+int amr_coarse_setup(amr_model *m);
+void amr_coarse_free(amr_model *m);
+
 // solver (kernels_solve.hip)
 int amr_pcg_solve(amr_model *m, const double *b, double *x, const amr_solve_opts *o, amr_solve_info *info);
+
+// PCG pools (kernels_solve.hip)
+struct amr_pool_result {
+    amr_model *m;
+    amr_solve_info info;
+};
+int amr_pool_new(amr_pool **out, int device, int cap, int check, hipStream_t stream /* null: own stream */);
+void amr_pool_free(amr_pool *p);
+int amr_pool_attach_model(amr_pool *p, amr_model *m);
+int amr_pool_detach_model(amr_model *m);
+int amr_pool_count(const amr_pool *p);
+bool amr_pool_is_member(const amr_model *m);  // solving in its shared pool right now
+int amr_pool_join(amr_pool *p, amr_model *m, const double *b, double *x, const amr_solve_opts *o);
+int amr_pool_run(amr_pool *p, std::vector<amr_pool_result> &out);
+int amr_pool_set_timing(amr_pool *p, int enable, double *ms, int64_t *n);
 int amr_spmv_full(amr_model *m, const double *x, double *y, int mask_mode);
 int amr_autotune(amr_model *m);

@@ -135,8 +135,9 @@ static int nn_of(int etype)
 
 // ------------------------------------------------------------- helpers
 
+// Device allocation charged to the model's byte count.
 template <typename T>
-static int dalloc(T **p, size_t n)
+static int dalloc(amr_model *m, T **p, size_t n)
 {
     if (n == 0) n = 1;
     hipError_t e = hipMalloc((void **)p, n * sizeof(T));
@@ -146,6 +147,7 @@ static int dalloc(T **p, size_t n)
                       hipGetErrorString(e));
         return e == hipErrorOutOfMemory ? AMR_ERR_NOMEM : AMR_ERR_HIP;
     }
+    m->dev_bytes += (int64_t)(n * sizeof(T));
     return AMR_OK;
 }
 
@@ -174,22 +176,27 @@ static int d2h(T *h, const T *d, size_t n, hipStream_t s)
 static void free_model(amr_model *m)
 {
     if (!m) return;
-    hipSetDevice(m->device);
+    (void)hipSetDevice(m->device);
+    if (m->pool) amr_pool_detach_model(m);
+    if (m->self_pool) amr_pool_free(m->self_pool);
+    m->self_pool = nullptr;
+    amr_coarse_free(m);
+    if (m->own_stream) (void)hipStreamSynchronize(m->own_stream);
     void *ptrs[] = {m->X,    m->conn, m->emat, m->mats,  m->dNdR, m->wip,  m->st_cur, m->st_com, m->Ke,
                     m->rowptr, m->col, m->diag, m->cptr,  m->csrc, m->fptr, m->fsrc,   m->Kv,     m->Kv32,
                     m->Kflag, m->dinv, m->dinv32, m->mask, m->eqd,  m->dF,   m->estat,  m->err,    m->part,
                     m->scal, m->binv, m->binv32};
     for (void *p : ptrs)
-        if (p) hipFree(p);
+        if (p) (void)hipFree(p);
     for (double *p : m->v)
-        if (p) hipFree(p);
+        if (p) (void)hipFree(p);
     for (float *p : m->f)
-        if (p) hipFree(p);
+        if (p) (void)hipFree(p);
     for (double *p : m->sv)
-        if (p) hipFree(p);
+        if (p) (void)hipFree(p);
     for (hipEvent_t e : m->ev)
-        if (e) hipEventDestroy(e);
-    if (m->stream) hipStreamDestroy(m->stream);
+        if (e) (void)hipEventDestroy(e);
+    if (m->own_stream) (void)hipStreamDestroy(m->own_stream);
     delete m;
 }
 
@@ -220,6 +227,7 @@ static int upload_dofs(amr_model *m, const int32_t *eq_id, int64_t ndofs, int64_
         }
     }
     TRY(h2d(m->mask, mask.data(), mask.size(), m->stream));
+    m->k_version++;
     TRY(h2d(m->eqd, eq_id, (size_t)m->nnodes * 3, m->stream));
     AMR_HIP(hipStreamSynchronize(m->stream));
     m->assembled = false;
@@ -280,45 +288,47 @@ extern "C" int amr_model_create(amr_model **out, int device, int etype, int nips
     TRY(amr_build_pattern(nn, nnodes, ne, conn, m->pat));
 
     AMR_HIP(hipSetDevice(device));
-    AMR_HIP(hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking));
+    AMR_HIP(hipStreamCreateWithFlags(&m->own_stream, hipStreamNonBlocking));
+    m->stream = m->own_stream;
     const size_t n3 = (size_t)nnodes * 3;
-    TRY(dalloc(&m->X, n3));
-    TRY(dalloc(&m->conn, (size_t)ne * nn));
-    TRY(dalloc(&m->emat, (size_t)ne));
-    TRY(dalloc(&m->mats, (size_t)nmats));
-    TRY(dalloc(&m->dNdR, (size_t)nq * nn * 3));
-    TRY(dalloc(&m->wip, (size_t)nq));
-    TRY(dalloc(&m->st_cur, (size_t)AMR_NSTATE * m->nip_total));
-    TRY(dalloc(&m->st_com, (size_t)AMR_NSTATE * m->nip_total));
-    TRY(dalloc(&m->Ke, (size_t)ne * nn * nn * 9));
-    TRY(dalloc(&m->rowptr, (size_t)nnodes + 1));
-    TRY(dalloc(&m->col, (size_t)m->pat.nblk));
-    TRY(dalloc(&m->diag, (size_t)nnodes));
-    TRY(dalloc(&m->cptr, (size_t)m->pat.nblk + 1));
-    TRY(dalloc(&m->csrc, (size_t)m->pat.ncontrib));
-    TRY(dalloc(&m->fptr, (size_t)nnodes + 1));
-    TRY(dalloc(&m->fsrc, m->pat.fsrc.size()));
-    TRY(dalloc(&m->Kv, (size_t)kv_size(m->pat.nblk)));
-    TRY(dalloc(&m->Kv32, (size_t)kv_size(m->pat.nblk)));
-    TRY(dalloc(&m->Kflag, (size_t)kv_size(m->pat.nblk)));
-    TRY(dalloc(&m->dinv, n3));
-    TRY(dalloc(&m->dinv32, n3));
-    TRY(dalloc(&m->binv, n3 * 3));
-    TRY(dalloc(&m->binv32, n3 * 3));
-    TRY(dalloc(&m->mask, n3));
-    TRY(dalloc(&m->eqd, n3));
-    TRY(dalloc(&m->dF, (size_t)ne * nn * 3));
-    TRY(dalloc(&m->estat, (size_t)ne));
-    TRY(dalloc(&m->err, 4));
-    TRY(dalloc(&m->part, 24576));
-    TRY(dalloc(&m->scal, 64));
-    for (double *&p : m->v) TRY(dalloc(&p, n3));
-    for (float *&p : m->f) TRY(dalloc(&p, n3));
+    TRY(dalloc(m.get(), &m->X, n3));
+    TRY(dalloc(m.get(), &m->conn, (size_t)ne * nn));
+    TRY(dalloc(m.get(), &m->emat, (size_t)ne));
+    TRY(dalloc(m.get(), &m->mats, (size_t)nmats));
+    TRY(dalloc(m.get(), &m->dNdR, (size_t)nq * nn * 3));
+    TRY(dalloc(m.get(), &m->wip, (size_t)nq));
+    TRY(dalloc(m.get(), &m->st_cur, (size_t)AMR_NSTATE * m->nip_total));
+    TRY(dalloc(m.get(), &m->st_com, (size_t)AMR_NSTATE * m->nip_total));
+    TRY(dalloc(m.get(), &m->Ke, (size_t)ne * nn * nn * 9));
+    TRY(dalloc(m.get(), &m->rowptr, (size_t)nnodes + 1));
+    TRY(dalloc(m.get(), &m->col, (size_t)m->pat.nblk));
+    TRY(dalloc(m.get(), &m->diag, (size_t)nnodes));
+    TRY(dalloc(m.get(), &m->cptr, (size_t)m->pat.nblk + 1));
+    TRY(dalloc(m.get(), &m->csrc, (size_t)m->pat.ncontrib));
+    TRY(dalloc(m.get(), &m->fptr, (size_t)nnodes + 1));
+    TRY(dalloc(m.get(), &m->fsrc, m->pat.fsrc.size()));
+    TRY(dalloc(m.get(), &m->Kv, (size_t)kv_size(m->pat.nblk)));
+    TRY(dalloc(m.get(), &m->Kv32, (size_t)kv_size(m->pat.nblk)));
+    TRY(dalloc(m.get(), &m->Kflag, (size_t)kv_size(m->pat.nblk)));
+    TRY(dalloc(m.get(), &m->dinv, n3));
+    TRY(dalloc(m.get(), &m->dinv32, n3));
+    TRY(dalloc(m.get(), &m->binv, n3 * 3));
+    TRY(dalloc(m.get(), &m->binv32, n3 * 3));
+    TRY(dalloc(m.get(), &m->mask, n3));
+    TRY(dalloc(m.get(), &m->eqd, n3));
+    TRY(dalloc(m.get(), &m->dF, (size_t)ne * nn * 3));
+    TRY(dalloc(m.get(), &m->estat, (size_t)ne));
+    TRY(dalloc(m.get(), &m->err, 4));
+    TRY(dalloc(m.get(), &m->part, 24576));
+    TRY(dalloc(m.get(), &m->scal, 64));
+    for (double *&p : m->v) TRY(dalloc(m.get(), &p, n3));
+    for (float *&p : m->f) TRY(dalloc(m.get(), &p, n3));
 
     std::vector<double> dNdR((size_t)nq * nn * 3);
     for (int q = 0; q < nq; q++) shape_deriv(etype, R + 3 * q, dNdR.data() + (size_t)q * nn * 3);
     hipStream_t s = m->stream;
     TRY(h2d(m->X, coords, n3, s));
+    m->Xh.assign(coords, coords + n3);
     TRY(h2d(m->conn, conn, (size_t)ne * nn, s));
     TRY(h2d(m->emat, emat, (size_t)ne, s));
     TRY(h2d(m->mats, mats.data(), mats.size(), s));
@@ -353,6 +363,10 @@ extern "C" int amr_model_destroy(amr_model *m)
 extern "C" int amr_model_set_dofs(amr_model *m, const int32_t *eq_id, int64_t ndofs, int64_t nu)
 {
     if (!m || !eq_id) return AMR_ERR_ARG;
+    if (amr_pool_is_member(m)) {
+        amr_set_error("the model is solving in a pool; its matrix and solver vectors are in use");
+        return AMR_ERR_ARG;
+    }
     AMR_HIP(hipSetDevice(m->device));
     return upload_dofs(m, eq_id, ndofs, nu);
 }
@@ -423,6 +437,10 @@ extern "C" int amr_state_restore(amr_model *m)
 extern "C" int amr_assemble(amr_model *m)
 {
     if (!m) return AMR_ERR_ARG;
+    if (amr_pool_is_member(m)) {
+        amr_set_error("the model is solving in a pool; its matrix and solver vectors are in use");
+        return AMR_ERR_ARG;
+    }
     AMR_HIP(hipSetDevice(m->device));
     AMR_HIP(hipMemsetAsync(m->err, 0x7f, sizeof(int32_t), m->stream));  // "no bad element"
     TRY(amr_launch_ke(m));
@@ -574,6 +592,10 @@ extern "C" int amr_solve_system(amr_model *m, double *U, double *F, const amr_so
                                 amr_solve_info *info)
 {
     if (!m || !U || !F) return AMR_ERR_ARG;
+    if (amr_pool_is_member(m)) {
+        amr_set_error("the model is solving in a pool; its matrix and solver vectors are in use");
+        return AMR_ERR_ARG;
+    }
     if (!m->assembled) {
         amr_set_error("amr_solve_system: K has not been assembled");
         return AMR_ERR_ARG;
@@ -667,7 +689,7 @@ extern "C" int amr_stage_begin(amr_model *m, const double *Uex, const double *Fe
     AMR_HIP(hipSetDevice(m->device));
     const size_t n = (size_t)m->nnodes * 3;
     for (double *&p : m->sv)
-        if (!p) TRY(dalloc(&p, n));
+        if (!p) TRY(dalloc(m, &p, n));
     std::vector<double> un, fn;
     eq_to_node(m, Uex, un);
     eq_to_node(m, Fex, fn);
@@ -686,12 +708,14 @@ extern "C" int amr_increment_begin(amr_model *m, double dT, double acr, int add_
     return amr_vec_inc_begin(m, dT, acr, add_residual);
 }
 
-extern "C" int amr_nr_iteration(amr_model *m, int it, const amr_solve_opts *opts, amr_nr_info *info)
+extern "C" int amr_nr_prepare(amr_model *m, int it)
 {
     if (!m || !m->sv[SV_U] || it < 1) return AMR_ERR_ARG;
+    if (amr_pool_is_member(m)) {
+        amr_set_error("the model is solving in a pool; its matrix and solver vectors are in use");
+        return AMR_ERR_ARG;
+    }
     AMR_HIP(hipSetDevice(m->device));
-    amr_solve_opts o = opts ? *opts : default_opts();
-    amr_nr_info inf{};
     double **v = m->sv;
     if (it > 1) TRY(amr_vec_fill(m, v[SV_DUI], 0.0));
     // mount_K
@@ -699,17 +723,40 @@ extern "C" int amr_nr_iteration(amr_model *m, int it, const amr_solve_opts *opts
     TRY(amr_launch_ke(m));
     TRY(amr_launch_assemble(m, false));
     m->assembled = true;
-    // solve_system!(K, ΔUitr, Rtr, nu)
+    // right-hand side of solve_system!(K, ΔUitr, Rtr, nu): R - K12 ΔUi2
+    if (m->nu > 0) TRY(amr_vec_solve_prep(m, v[SV_DUI], v[SV_R], m->v[8], m->v[9], m->v[10]));
+    return AMR_OK;
+}
+
+extern "C" int amr_nr_finish(amr_model *m, const amr_solve_info *si, amr_nr_info *info)
+{
+    if (!m || !m->sv[SV_U]) return AMR_ERR_ARG;
+    if (amr_pool_is_member(m)) {
+        amr_set_error("the model is solving in a pool; its matrix and solver vectors are in use");
+        return AMR_ERR_ARG;
+    }
+    AMR_HIP(hipSetDevice(m->device));
+    amr_nr_info inf{};
+    double **v = m->sv;
     int rc = AMR_OK;
     if (m->nu > 0) {
-        TRY(amr_vec_solve_prep(m, v[SV_DUI], v[SV_R], m->v[8], m->v[9], m->v[10]));
-        amr_solve_info si{};
-        rc = amr_pcg_solve(m, m->v[10], v[SV_X], &o, &si);
-        inf.pcg_iterations = si.iterations;
-        inf.relres = si.relres;
-        inf.replacements = si.refinements;
-        inf.fallback = si.fallback;
-        if (rc == AMR_OK) TRY(amr_vec_combine(m, v[SV_X], v[SV_DUI]));
+        if (!si) {
+            amr_set_error("amr_nr_finish: the model has unknowns; pass the outcome of its linear solve");
+            return AMR_ERR_ARG;
+        }
+        inf.pcg_iterations = si->iterations;
+        inf.relres = si->relres;
+        inf.replacements = si->refinements;
+        inf.fallback = si->fallback;
+        if (si->converged) {
+            TRY(amr_vec_combine(m, v[SV_X], v[SV_DUI]));
+        } else {
+            char buf[200];
+            snprintf(buf, sizeof buf, "PCG did not converge (relres %.3e after %d iterations, %d replacements)",
+                     si->relres, si->iterations, si->refinements);
+            amr_set_error(buf);
+            rc = AMR_ERR_SOLVE;
+        }
     }
     int32_t herr[3] = {0, 0, 0};
     if (rc == AMR_OK) {
@@ -744,6 +791,18 @@ extern "C" int amr_nr_iteration(amr_model *m, int it, const amr_solve_opts *opts
     inf.status = rc;
     if (info) *info = inf;
     return rc;
+}
+
+extern "C" int amr_nr_iteration(amr_model *m, int it, const amr_solve_opts *opts, amr_nr_info *info)
+{
+    TRY(amr_nr_prepare(m, it));
+    amr_solve_opts o = opts ? *opts : default_opts();
+    amr_solve_info si{};
+    if (m->nu > 0) {
+        const int rc = amr_pcg_solve(m, m->v[10], m->sv[SV_X], &o, &si);
+        if (rc != AMR_OK && rc != AMR_ERR_SOLVE) return rc;
+    }
+    return amr_nr_finish(m, m->nu > 0 ? &si : nullptr, info);
 }
 
 extern "C" int amr_increment_commit(amr_model *m)
@@ -806,5 +865,129 @@ extern "C" int amr_get_spmv_lanes(amr_model *m, int *out)
     out[0] = m->lanes64;
     out[1] = m->lanes32;
     out[2] = m->lanes32d;
+    return AMR_OK;
+}
+
+// ---------------------------------------------------------------- pools
+
+extern "C" int amr_pool_create(amr_pool **out, int device, int max_members, int check_every)
+{
+    if (!out) return AMR_ERR_ARG;
+    return amr_pool_new(out, device, max_members, check_every, nullptr);
+}
+
+extern "C" int amr_pool_destroy(amr_pool *p)
+{
+    amr_pool_free(p);
+    return AMR_OK;
+}
+
+extern "C" int amr_pool_attach(amr_pool *p, amr_model *m)
+{
+    if (!p || !m) return AMR_ERR_ARG;
+    AMR_HIP(hipSetDevice(m->device));
+    return amr_pool_attach_model(p, m);
+}
+
+extern "C" int amr_pool_detach(amr_model *m)
+{
+    if (!m) return AMR_ERR_ARG;
+    AMR_HIP(hipSetDevice(m->device));
+    return amr_pool_detach_model(m);
+}
+
+extern "C" int amr_pool_add(amr_pool *p, amr_model *m, const amr_solve_opts *opts)
+{
+    if (!p || !m || !m->sv[SV_U]) return AMR_ERR_ARG;
+    if (m->nu == 0) {
+        amr_set_error("amr_pool_add: the model has no unknowns; call amr_nr_finish with si = NULL");
+        return AMR_ERR_ARG;
+    }
+    AMR_HIP(hipSetDevice(m->device));
+    const amr_solve_opts o = opts ? *opts : default_opts();
+    return amr_pool_join(p, m, m->v[10], m->sv[SV_X], &o);
+}
+
+extern "C" int amr_pool_step(amr_pool *p, int capacity, int *nfinished, amr_model **models, amr_solve_info *infos)
+{
+    if (!p || !nfinished) return AMR_ERR_ARG;
+    if (capacity < amr_pool_count(p) || (capacity > 0 && (!models || !infos))) {
+        amr_set_error("amr_pool_step: the output arrays must hold every member of the pool");
+        return AMR_ERR_ARG;
+    }
+    std::vector<amr_pool_result> res;
+    const int rc = amr_pool_run(p, res);
+    *nfinished = (int)res.size();
+    for (size_t i = 0; i < res.size(); i++) {
+        models[i] = res[i].m;
+        infos[i] = res[i].info;
+    }
+    return rc;
+}
+
+extern "C" int amr_pool_size(amr_pool *p, int *members)
+{
+    if (!p || !members) return AMR_ERR_ARG;
+    *members = amr_pool_count(p);
+    return AMR_OK;
+}
+
+extern "C" int amr_pool_timing(amr_pool *p, int enable, double *spmv_ms, int64_t *spmv_launches)
+{
+    if (!p) return AMR_ERR_ARG;
+    return amr_pool_set_timing(p, enable, spmv_ms, spmv_launches);
+}
+
+extern "C" int amr_model_device_bytes(amr_model *m, int64_t *bytes)
+{
+    if (!m || !bytes) return AMR_ERR_ARG;
+    *bytes = m->dev_bytes;
+    return AMR_OK;
+}
+
+extern "C" int amr_mem_info(int device, int64_t *free_bytes, int64_t *total_bytes)
+{
+    AMR_HIP(hipSetDevice(device));
+    size_t f = 0, t = 0;
+    AMR_HIP(hipMemGetInfo(&f, &t));
+    if (free_bytes) *free_bytes = (int64_t)f;
+    if (total_bytes) *total_bytes = (int64_t)t;
+    return AMR_OK;
+}
+
+extern "C" int amr_set_coarse(amr_model *m, int target_nodes, int max_aggregates)
+{
+    if (!m || target_nodes < 1 || max_aggregates < 1 || max_aggregates > 340) {
+        amr_set_error("amr_set_coarse: target_nodes >= 1 and 1 <= max_aggregates <= 340");
+        return AMR_ERR_ARG;
+    }
+    if (m->coarse_built) {
+        amr_set_error("amr_set_coarse: the coarse space is already built");
+        return AMR_ERR_ARG;
+    }
+    m->coarse_target = target_nodes;
+    m->coarse_max = max_aggregates;
+    return AMR_OK;
+}
+
+extern "C" int amr_get_coarse(amr_model *m, int64_t *out)
+{
+    if (!m || !out) return AMR_ERR_ARG;
+    out[0] = m->coarse_built ? m->nagg : 0;
+    out[1] = m->coarse_built ? m->ncoarse : 0;
+    return AMR_OK;
+}
+
+extern "C" int amr_get_coarse_space(amr_model *m, int32_t *agg, double *kc_inv)
+{
+    if (!m) return AMR_ERR_ARG;
+    if (!m->coarse_built) {
+        amr_set_error("amr_get_coarse_space: no two-level solve has built the coarse space yet");
+        return AMR_ERR_ARG;
+    }
+    AMR_HIP(hipSetDevice(m->device));
+    if (agg) TRY(d2h(agg, m->c_agg, (size_t)m->nnodes, m->stream));
+    if (kc_inv) TRY(d2h(kc_inv, m->Kinv, (size_t)m->ncoarse * m->ncoarse, m->stream));
+    AMR_HIP(hipStreamSynchronize(m->stream));
     return AMR_OK;
 }

@@ -1,0 +1,266 @@
+// Coarse space of the two-level preconditioner (AMR_PRECOND_TWO_LEVEL).
+//
+// The nodes are split into aggregates (recursive coordinate bisection,
+// model_build.cpp).  The coarse basis Phi holds, per aggregate, the six
+// rigid-body modes of its nodes about the aggregate centroid (restricted to
+// the free dofs: rows of prescribed dofs are zero).  The preconditioner is
+// additive:  M^-1 r = D^-1 r + Phi Kc^-1 Phi^T r  with  Kc = Phi^T K Phi.
+//
+// Kc is formed on the device without atomics, in a fixed order:
+//   pass 1, one thread per (block row i, aggregate b) pair:
+//       U_ib = R_i^T  sum_{j in row i, agg(j) = b}  K_ij R_j       (6x6)
+//     (R_j: 3x6 rigid-mode matrix of node j, masked), blocks in column order;
+//   pass 2, one thread per (coarse block (a,b), entry): Kc_ab = sum over the
+//     pairs (i in a, b) in ascending i.
+// Kc (dense, n = 6 nagg) is then factored and inverted by rocSOLVER
+// (potrf + potri, rocBLAS with atomics disallowed, so the inverse is
+// reproducible) and symmetrised; each PCG iteration applies Kc^-1 as a dense
+// product.  A coarse matrix that is not positive definite (degenerate
+// aggregates) is replaced by zero, which leaves plain Jacobi.
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+#include <algorithm>
+#include <cmath>
+#include <mutex>
+
+#include "amr_internal.h"
+
+namespace {
+
+// 3x6 rigid-body-mode matrix of a node at offset d from the centroid, rows
+// scaled by the node's free-dof mask
+__device__ inline void rigid_modes(const double d[3], const double mk[3], double R[3][6])
+{
+    for (int r = 0; r < 3; r++)
+        for (int q = 0; q < 6; q++) R[r][q] = 0.0;
+    R[0][0] = R[1][1] = R[2][2] = 1.0;
+    R[1][3] = -d[2];
+    R[2][3] = d[1];
+    R[0][4] = d[2];
+    R[2][4] = -d[0];
+    R[0][5] = -d[1];
+    R[1][5] = d[0];
+    for (int r = 0; r < 3; r++)
+        for (int q = 0; q < 6; q++) R[r][q] *= mk[r];
+}
+
+__global__ void __launch_bounds__(256) galerkin_pairs_kernel(int64_t npairs, const int32_t *__restrict__ prow,
+                                                             const int32_t *__restrict__ pagg,
+                                                             const int32_t *__restrict__ agg,
+                                                             const double *__restrict__ cent,
+                                                             const double *__restrict__ X,
+                                                             const double *__restrict__ mask,
+                                                             const int32_t *__restrict__ rowptr,
+                                                             const int32_t *__restrict__ col,
+                                                             const double *__restrict__ Kv, double *__restrict__ out)
+{
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= npairs) return;
+    const int32_t i = prow[s], b = pagg[s];
+    double W[3][6];
+    for (int r = 0; r < 3; r++)
+        for (int q = 0; q < 6; q++) W[r][q] = 0.0;
+    for (int32_t k = rowptr[i]; k < rowptr[i + 1]; k++) {
+        const int32_t j = col[k];
+        if (agg[j] != b) continue;
+        double d[3], mk[3], R[3][6], Kb[9];
+        for (int c = 0; c < 3; c++) {
+            d[c] = X[(int64_t)j * 3 + c] - cent[(int64_t)b * 3 + c];
+            mk[c] = mask[(int64_t)j * 3 + c];
+        }
+        rigid_modes(d, mk, R);
+        for (int e = 0; e < 9; e++) Kb[e] = Kv[kv_at(k, e)];
+        for (int r = 0; r < 3; r++)
+            for (int q = 0; q < 6; q++)
+                W[r][q] = fma(Kb[r * 3 + 2], R[2][q], fma(Kb[r * 3 + 1], R[1][q], fma(Kb[r * 3], R[0][q], W[r][q])));
+    }
+    const int32_t a = agg[i];
+    double d[3], mk[3], R[3][6];
+    for (int c = 0; c < 3; c++) {
+        d[c] = X[(int64_t)i * 3 + c] - cent[(int64_t)a * 3 + c];
+        mk[c] = mask[(int64_t)i * 3 + c];
+    }
+    rigid_modes(d, mk, R);
+    for (int p = 0; p < 6; p++)
+        for (int q = 0; q < 6; q++)
+            out[s * 36 + p * 6 + q] = fma(R[2][p], W[2][q], fma(R[1][p], W[1][q], R[0][p] * W[0][q]));
+}
+
+__global__ void __launch_bounds__(256) galerkin_blocks_kernel(int32_t ncb, int32_t nc, const int32_t *__restrict__ cba,
+                                                              const int32_t *__restrict__ cbb,
+                                                              const int32_t *__restrict__ cbptr,
+                                                              const int32_t *__restrict__ cbslot,
+                                                              const double *__restrict__ pair, double *__restrict__ Kc)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)ncb * 36) return;
+    const int32_t cb = (int32_t)(t / 36), e = (int32_t)(t % 36);
+    double acc = 0.0;
+    for (int32_t k = cbptr[cb]; k < cbptr[cb + 1]; k++) acc += pair[(int64_t)cbslot[k] * 36 + e];
+    Kc[(int64_t)(6 * cba[cb] + e / 6) * nc + 6 * cbb[cb] + e % 6] = acc;
+}
+
+__global__ void zero_kernel(int64_t n, double *__restrict__ x)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        x[i] = 0.0;
+}
+
+// coarse dofs whose basis vector vanishes (all dofs of the aggregate
+// prescribed) get a unit diagonal; their right-hand side is always zero
+__global__ void fix_diag_kernel(int32_t nc, double *__restrict__ Kc, int32_t *__restrict__ info)
+{
+    const int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k == 0) info[0] = info[1] = 0;
+    if (k < nc && Kc[(int64_t)k * nc + k] == 0.0) Kc[(int64_t)k * nc + k] = 1.0;
+}
+
+// potri leaves the inverse in one triangle (column-major lower); mirror it,
+// or clear everything if the factorisation or inversion failed
+__global__ void symmetrize_kernel(int32_t nc, double *__restrict__ A, const int32_t *__restrict__ info)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)nc * nc) return;
+    // A[t], t = i*nc + j, is the column-major element (j, i): in the lower
+    // triangle, which potri filled, when i < j
+    const int32_t i = (int32_t)(t / nc), j = (int32_t)(t % nc);
+    if (info[0] != 0 || info[1] != 0) {
+        A[t] = 0.0;
+        return;
+    }
+    if (i < j) A[(int64_t)j * nc + i] = A[(int64_t)i * nc + j];
+}
+
+std::mutex g_rb_mu;
+rocblas_handle g_rb[64] = {};
+
+int rb_handle(int device, rocblas_handle *h)
+{
+    std::lock_guard<std::mutex> lk(g_rb_mu);
+    if (device < 0 || device >= 64) {
+        amr_set_error("device index out of range");
+        return AMR_ERR_ARG;
+    }
+    if (!g_rb[device]) {
+        if (rocblas_create_handle(&g_rb[device]) != rocblas_status_success) {
+            amr_set_error("rocblas_create_handle failed");
+            return AMR_ERR_HIP;
+        }
+        rocblas_set_atomics_mode(g_rb[device], rocblas_atomics_not_allowed);
+    }
+    *h = g_rb[device];
+    return AMR_OK;
+}
+
+template <typename T>
+int dev_upload(amr_model *m, T **d, const std::vector<T> &h)
+{
+    const size_t n = std::max<size_t>(h.size(), 1);
+    hipError_t e = hipMalloc((void **)d, n * sizeof(T));
+    if (e != hipSuccess) {
+        *d = nullptr;
+        amr_set_error(std::string("hipMalloc (coarse space) failed: ") + hipGetErrorString(e));
+        return e == hipErrorOutOfMemory ? AMR_ERR_NOMEM : AMR_ERR_HIP;
+    }
+    m->dev_bytes += (int64_t)(n * sizeof(T));
+    if (!h.empty()) AMR_HIP(hipMemcpyAsync(*d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, m->stream));
+    return AMR_OK;
+}
+
+template <typename T>
+int dev_alloc(amr_model *m, T **d, size_t n)
+{
+    n = std::max<size_t>(n, 1);
+    hipError_t e = hipMalloc((void **)d, n * sizeof(T));
+    if (e != hipSuccess) {
+        *d = nullptr;
+        amr_set_error(std::string("hipMalloc (coarse space) failed: ") + hipGetErrorString(e));
+        return e == hipErrorOutOfMemory ? AMR_ERR_NOMEM : AMR_ERR_HIP;
+    }
+    m->dev_bytes += (int64_t)(n * sizeof(T));
+    return AMR_OK;
+}
+
+int build(amr_model *m)
+{
+    const int64_t n = m->nnodes;
+    const int target = (int)std::max<int64_t>(m->coarse_target, (n + m->coarse_max - 1) / m->coarse_max);
+    AmrCoarse C;
+    if (int rc = amr_build_coarse(n, m->Xh.data(), m->pat.rowptr.data(), m->pat.col.data(), target, C)) return rc;
+    m->nagg = C.nagg;
+    m->ncoarse = 6 * C.nagg;
+    m->npairs = (int64_t)C.pagg.size();
+    m->ncb = (int32_t)C.cba.size();
+    const size_t nc = (size_t)m->ncoarse;
+    if (int rc = dev_upload(m, &m->c_agg, C.agg)) return rc;
+    if (int rc = dev_upload(m, &m->c_aggptr, C.aggptr)) return rc;
+    if (int rc = dev_upload(m, &m->c_aggnodes, C.aggnodes)) return rc;
+    if (int rc = dev_upload(m, &m->c_prow, C.prow)) return rc;
+    if (int rc = dev_upload(m, &m->c_pagg, C.pagg)) return rc;
+    if (int rc = dev_upload(m, &m->c_cba, C.cba)) return rc;
+    if (int rc = dev_upload(m, &m->c_cbb, C.cbb)) return rc;
+    if (int rc = dev_upload(m, &m->c_cbptr, C.cbptr)) return rc;
+    if (int rc = dev_upload(m, &m->c_cbslot, C.cbslot)) return rc;
+    if (int rc = dev_upload(m, &m->c_cent, C.cent)) return rc;
+    // written in full by the setup and solver kernels before any read
+    if (int rc = dev_alloc(m, &m->c_pair, (size_t)m->npairs * 36)) return rc;
+    if (int rc = dev_alloc(m, &m->Kinv, nc * nc)) return rc;
+    if (int rc = dev_alloc(m, &m->c_vec, nc)) return rc;
+    if (int rc = dev_alloc(m, &m->y_vec, nc)) return rc;
+    if (int rc = dev_alloc(m, &m->c_info, 2)) return rc;
+    // C's vectors are released on return; the copies read them
+    // asynchronously, so wait here
+    AMR_HIP(hipStreamSynchronize(m->stream));
+    m->coarse_built = true;
+    return AMR_OK;
+}
+
+}  // namespace
+
+int amr_coarse_setup(amr_model *m)
+{
+    if (!m->coarse_built)
+        if (int rc = build(m)) return rc;
+    if (m->coarse_version == m->k_version) return AMR_OK;
+    hipStream_t s = m->stream;
+    const int32_t nc = m->ncoarse;
+    const int64_t nn = (int64_t)nc * nc;
+    hipLaunchKernelGGL(zero_kernel, dim3((unsigned)std::min<int64_t>((nn + 255) / 256, 4096)), dim3(256), 0, s, nn,
+                       m->Kinv);
+    if (m->npairs > 0)
+        hipLaunchKernelGGL(galerkin_pairs_kernel, dim3((unsigned)((m->npairs + 255) / 256)), dim3(256), 0, s,
+                           m->npairs, m->c_prow, m->c_pagg, m->c_agg, m->c_cent, m->X, m->mask, m->rowptr, m->col,
+                           m->Kv, m->c_pair);
+    if (m->ncb > 0)
+        hipLaunchKernelGGL(galerkin_blocks_kernel, dim3((unsigned)(((int64_t)m->ncb * 36 + 255) / 256)), dim3(256),
+                           0, s, m->ncb, nc, m->c_cba, m->c_cbb, m->c_cbptr, m->c_cbslot, m->c_pair, m->Kinv);
+    hipLaunchKernelGGL(fix_diag_kernel, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, s, nc, m->Kinv, m->c_info);
+    AMR_HIP(hipGetLastError());
+    rocblas_handle h;
+    if (int rc = rb_handle(m->device, &h)) return rc;
+    {
+        std::lock_guard<std::mutex> lk(g_rb_mu);
+        if (rocblas_set_stream(h, s) != rocblas_status_success ||
+            rocsolver_dpotrf(h, rocblas_fill_lower, nc, m->Kinv, nc, m->c_info) != rocblas_status_success ||
+            rocsolver_dpotri(h, rocblas_fill_lower, nc, m->Kinv, nc, m->c_info + 1) != rocblas_status_success) {
+            amr_set_error("rocSOLVER factorisation of the coarse matrix failed to launch");
+            return AMR_ERR_HIP;
+        }
+    }
+    hipLaunchKernelGGL(symmetrize_kernel, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, s, nc, m->Kinv,
+                       m->c_info);
+    AMR_HIP(hipGetLastError());
+    m->coarse_version = m->k_version;
+    return AMR_OK;
+}
+
+void amr_coarse_free(amr_model *m)
+{
+    void *ptrs[] = {m->c_agg, m->c_aggptr, m->c_aggnodes, m->c_prow, m->c_pagg, m->c_cba,  m->c_cbb, m->c_cbptr,
+                    m->c_cbslot, m->c_info, m->c_cent, m->c_pair, m->Kinv, m->c_vec, m->y_vec};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    m->coarse_built = false;
+    m->coarse_version = -1;
+}

@@ -635,6 +635,7 @@ int amr_launch_ke(amr_model *m)
 
 int amr_launch_assemble(amr_model *m, bool flags)
 {
+    m->k_version++;
     if (m->pat.nblk == 0) return AMR_OK;
     unsigned grid = (unsigned)((m->pat.nblk + 255) / 256);
     hipLaunchKernelGGL(assemble_kernel, dim3(grid), dim3(256), 0, m->stream, m->pat.nblk, m->cptr, m->csrc, m->Ke,

@@ -5,27 +5,39 @@
 // Vectors live in node order (3 dofs per node).  The operator is K with the
 // prescribed dofs masked out: y = mask .* (K x).  With x zero at prescribed
 // dofs this is K11 x on the unknowns, so one resident copy of K serves K11,
-// K12 U2 (right-hand side) and K21 U1 + K22 U2 (reactions).  Block values
-// are stored slot-major (value s of block k at s*nblk + k) so that the lanes
-// of a wave, which walk consecutive blocks of a row, read contiguous memory.
+// K12 U2 (right-hand side) and K21 U1 + K22 U2 (reactions).
+//
+// Pools.  The PCG kernels work on a pool of independent systems (models) at
+// once: a descriptor per model holds its matrix, vectors, partials and
+// scalars, and every launch walks a work list of (model, row tile) or
+// (model, vector block) entries, so one launch per PCG phase advances every
+// model of the pool.  Each model's tile and block decomposition depends on
+// that model alone, and each model's scalars are reduced from its own
+// partials in a fixed order, so a model's iterates are bit-identical whether
+// it is solved alone (a pool of one, which is what amr_pcg_solve does) or
+// together with any others.  Models join and leave the pool between steps
+// (continuous batching at the granularity of one step of check_every
+// iterations).
 //
 // Reductions need no atomics and no fences: a reducing kernel writes one
 // partial per workgroup, and every workgroup of the next kernel that needs
 // the scalar sums those partials itself in index order (kernel boundaries
-// make them visible).  Results are deterministic for a given model.
+// make them visible).
 //
-// Precision.  Plain path: FP64 PCG.  Mixed path: the same CG recurrences
+// Precision.  Plain path: FP64 PCG.  Mixed paths: the same CG recurrences
 // with FP64 vectors, FP64 dot products and FP64 solution update, but the
-// SpMV reads FP32 values and an FP32 copy of the search direction and
-// accumulates in FP32, and the Jacobi preconditioner is applied in FP32.
-// The FP64 residual r = b - K x is recomputed with the FP64 matrix whenever
-// the recursive residual has dropped 10x since the last recomputation and
-// at the end (residual replacement), which keeps the iteration converging
-// to the FP64 solution without restarting the Krylov process; a model whose
-// FP32 operator drifts too far from K falls back to the FP64 operator.
+// SpMV reads FP32 values (with an FP32 copy of the search direction and FP32
+// accumulation for AMR_PREC_MIXED, FP64 gather and accumulation for
+// AMR_PREC_MIXED_F64ACC) and the preconditioner is applied in FP32.  The
+// FP64 residual r = b - K x is recomputed with the FP64 matrix whenever the
+// recursive residual has dropped 10x since the last recomputation and at the
+// end (residual replacement), which keeps the iteration converging to the
+// FP64 solution without restarting the Krylov process; a model whose FP32
+// operator drifts too far from K falls back to the FP64 operator.
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstring>
 
 #include "amr_internal.h"
 
@@ -35,16 +47,21 @@ constexpr int WGV = 256;  // vector kernels
 constexpr int WGS = 512;  // SpMV kernels
 constexpr int MAX_GR = 4096, MAX_GV = 512;
 
-// scalar and control slots
+// scalar and control slots of one model: S_NSCAL doubles, then C_NCTL ints
 enum { S_RZ = 0, S_RR, S_BB, S_THR, S_KX, S_RES, S_ERRN, S_ERRD, S_GAP, S_NSCAL = 16 };
 // C_DONE is read by every workgroup at kernel entry and written only where
-// all workgroups of the running launch agree on it (breakdown); the
-// convergence test writes C_STOP, which the next SpMV promotes to C_DONE, so
-// no workgroup ever sees a flag raised during its own launch.
+// all workgroups of the model in the running launch agree on it
+// (breakdown); the convergence test writes C_STOP, which the next SpMV
+// promotes to C_DONE, so no workgroup ever sees a flag raised during its
+// own launch.  C_ITERS is advanced by the SpMV's first tile and read by the
+// kernels after it.
 enum { C_DONE = 0, C_ITERS, C_BREAK, C_NAN, C_STOP, C_NCTL = 16 };
+constexpr int SEG_DBL = 32;  // doubles per model in a pool's scalar array
+static_assert(S_NSCAL + C_NCTL / 2 <= SEG_DBL, "scalar block");
 // partial buffers inside m->part
-enum { P_PQ = 0, P_RZ0 = 4096, P_RZ1 = 4608, P_RR0 = 5120, P_RR1 = 5632, P_A = 8192, P_B = 12288, P_C = 16384,
-       P_D = 20480, P_END = 24576 };
+enum { P_PQ = 0, P_RZ0 = 4096, P_RZ1 = 4608, P_RR0 = 5120, P_RR1 = 5632, P_CY0 = 6144, P_CY1 = 6656, P_A = 8192,
+       P_B = 12288, P_C = 16384, P_D = 20480, P_END = 24576 };
+constexpr int MAX_CW = 512;  // coarse-product workgroups per model (4 coarse rows each)
 
 template <int NT>
 __device__ inline double block_sum(double v)
@@ -81,18 +98,60 @@ __device__ inline double sum_partials(const double *__restrict__ P, int n)
     return block_sum<NT>(v);
 }
 
+// One model of a pool, as the kernels see it.
+struct SegDesc {
+    const int32_t *rowptr, *col;
+    const double *Kv;
+    const float *Kv32;
+    const double *mask;
+    const double *dinv, *binv;    // Jacobi / nodal block-Jacobi inverses
+    const float *dinv32, *binv32;
+    const double *b;              // right-hand side (node order, 0 at prescribed dofs)
+    double *x, *r, *z, *p, *q, *rt;
+    float *p32;                   // FP32 copy of p (AMR_PREC_MIXED)
+    double *part, *scal;
+    int32_t *ctl;
+    int64_t nnodes, nblk;
+    double thr_fac;               // recursive-residual stop: rr <= thr_fac * bb
+    int R, ntiles, nvblk, maxiter;
+    int f32;                      // preconditioner applied in FP32
+    int block;                    // nodal block Jacobi
+    int wp32;                     // keep p32 up to date
+    int tl;                       // two-level: add the coarse correction Phi Kc^-1 Phi^T r
+    // two-level coarse space
+    int nagg, nc, ncw;
+    const int32_t *agg, *aggptr, *aggnodes;
+    const double *cent, *X, *Kinv;
+    double *cvec, *yvec;
+};
+
+// r.z with z = M r: the Jacobi part's partials plus, two-level, the coarse
+// part c.y = (Phi^T r).(Kc^-1 Phi^T r)
+__device__ inline double sum_rz(const SegDesc &d, int slot)
+{
+    double v = sum_partials<WGV>(d.part + (slot ? P_RZ1 : P_RZ0), d.nvblk);
+    if (d.tl) v += sum_partials<WGV>(d.part + (slot ? P_CY1 : P_CY0), d.ncw);
+    return v;
+}
+
+// coarse correction at dof c of node nd: (R_nd y_agg(nd))_c, masked
+__device__ inline double coarse_term(const SegDesc &d, int64_t nd, int c)
+{
+    const int32_t a = d.agg[nd];
+    const double *y = d.yvec + (int64_t)a * 6;
+    const double dx = d.X[nd * 3] - d.cent[a * 3], dy = d.X[nd * 3 + 1] - d.cent[a * 3 + 1],
+                 dz = d.X[nd * 3 + 2] - d.cent[a * 3 + 2];
+    double v;
+    if (c == 0) v = fma(-dy, y[5], fma(dz, y[4], y[0]));
+    else if (c == 1) v = fma(dx, y[5], fma(-dz, y[3], y[1]));
+    else v = fma(-dx, y[4], fma(dy, y[3], y[2]));
+    return d.mask[nd * 3 + c] != 0.0 ? v : 0.0;
+}
+
 // Preconditioner: Jacobi (inverse diagonal) or 3x3 nodal block Jacobi
 // (inverse of each node's diagonal block restricted to its free dofs), in
 // FP64 or applied in FP32 (mixed paths).
-struct Prec {
-    const double *dinv;
-    const float *dinv32;
-    const double *binv;   // 9 per node, node-major
-    const float *binv32;
-    int f32, block;
-};
-
-__device__ inline void apply_prec(const Prec &P, int64_t node, const double r[3], double z[3])
+__device__ inline void apply_prec(const SegDesc &P, int64_t node, const double r[3], double z[3])
 {
     const int64_t i = node * 3;
     if (P.block) {
@@ -113,18 +172,19 @@ __device__ inline void apply_prec(const Prec &P, int64_t node, const double r[3]
 
 // ------------------------------------------------------------------ SpMV
 //
-// Row tiles: workgroup b owns block rows [b*R, (b+1)*R) (R a power of two
-// fixed per model), stages their row pointers in LDS and walks them with L
-// lanes per row.  Each row's sum is formed over 64 virtual slots (slot j
-// takes blocks rb+j, rb+j+64, ... in order) reduced by one fixed binary
-// tree, whatever L is: a lane holds 64/L slots and performs the tree levels
-// above L itself, the levels below L are cross-lane shuffles.  Per-row dot
-// contributions go to LDS and are reduced by a fixed tree too.  So neither
-// the row sums nor the partials depend on L, and L can be autotuned.
+// Row tiles: a workgroup owns block rows [t*R, (t+1)*R) of one model (R a
+// power of two fixed per model), stages their row pointers in LDS and walks
+// them with L lanes per row.  Each row's sum is formed over 64 virtual slots
+// (slot j takes blocks rb+j, rb+j+64, ... in order) reduced by one fixed
+// binary tree, whatever L is: a lane holds 64/L slots and performs the tree
+// levels above L itself, the levels below L are cross-lane shuffles.
+// Per-row dot contributions go to LDS and are reduced by a fixed tree too.
+// So neither the row sums nor the partials depend on L, and L can be
+// autotuned per model.
 
 template <int L, bool ABS, typename VT, typename XT, typename AT>
 __device__ inline void row_sum(int32_t rb, int32_t re, int sub, const int32_t *__restrict__ col,
-                               const VT *__restrict__ vals, int64_t nblk, const XT *__restrict__ x, AT y[3])
+                               const VT *__restrict__ vals, const XT *__restrict__ x, AT y[3])
 {
     constexpr int NS = 64 / L;
     AT acc[NS][3];
@@ -183,7 +243,7 @@ __device__ inline double lds_tree_sum(double *s, int R)
 }
 
 // Dynamic LDS layout of the row-tile kernels: R+1 row pointers, then up to
-// four R-long double arrays.
+// four R-long double arrays (sized for the largest R of the launch).
 __device__ inline int32_t *tile_rowptr(char *sm) { return (int32_t *)sm; }
 __device__ inline double *tile_dbl(char *sm, int R, int j)
 {
@@ -192,40 +252,40 @@ __device__ inline double *tile_dbl(char *sm, int R, int j)
 }
 
 // Workgroups are dispatched round-robin over the 8 XCDs (workgroup b runs on
-// XCD b % 8).  Tile t(b) gives each XCD one contiguous run of row tiles, so
-// the slice of x its rows gather stays in that XCD's L2.
+// XCD b % 8).  Entry t(b) gives each XCD one contiguous run of the work
+// list, so the slice of x a run of row tiles gathers stays in that XCD's L2.
 __device__ inline int64_t xcd_tile(int64_t b, int64_t G)
 {
     const int64_t x = b & 7, i = b >> 3, q = G >> 3, rm = G & 7;
     return x * q + (x < rm ? x : rm) + i;
 }
 
-__device__ inline int load_tile_rows(const int32_t *__restrict__ rowptr, int64_t nrows, int R, int32_t *srp,
-                                     int64_t &r0)
+__device__ inline int load_rows(const int32_t *__restrict__ rowptr, int64_t nrows, int R, int64_t tile,
+                                int32_t *srp, int64_t &r0)
 {
-    r0 = xcd_tile(blockIdx.x, gridDim.x) * R;
+    r0 = tile * R;
     const int nr = (int)std::min<int64_t>(R, nrows - r0);
     for (int i = threadIdx.x; i <= nr; i += blockDim.x) srp[i] = rowptr[r0 + i];
     __syncthreads();
     return nr;
 }
 
-// y = K x (mode 0) or mask .* (K x) (mode 1), FP64
+// y = K x (mode 0) or mask .* (K x) (mode 1), FP64, one model
 template <int L>
 __global__ void __launch_bounds__(WGS) spmv_kernel(int64_t nrows, int R, const int32_t *__restrict__ rowptr,
                                                    const int32_t *__restrict__ col, const double *__restrict__ vals,
-                                                   int64_t nblk, const double *__restrict__ x, double *__restrict__ y,
+                                                   const double *__restrict__ x, double *__restrict__ y,
                                                    const double *__restrict__ mask, int mode)
 {
     extern __shared__ __align__(16) char sm[];
     int32_t *srp = tile_rowptr(sm);
     int64_t r0;
-    const int nr = load_tile_rows(rowptr, nrows, R, srp, r0);
+    const int nr = load_rows(rowptr, nrows, R, xcd_tile(blockIdx.x, gridDim.x), srp, r0);
     constexpr int RPW = WGS / L;
     const int sub = threadIdx.x % L;
     for (int i = threadIdx.x / L; i < nr; i += RPW) {
         double yy[3];
-        row_sum<L, false, double, double, double>(srp[i], srp[i + 1], sub, col, vals, nblk, x, yy);
+        row_sum<L, false, double, double, double>(srp[i], srp[i + 1], sub, col, vals, x, yy);
         if (sub == 0) {
             const int64_t row = r0 + i;
             for (int d = 0; d < 3; d++)
@@ -234,45 +294,43 @@ __global__ void __launch_bounds__(WGS) spmv_kernel(int64_t nrows, int R, const i
     }
 }
 
-// r = mask .* (b - K x) in FP64; per-tile partials ||r||^2, ||b||^2,
-// || |K||x| + |b| ||^2 and, with rold, the drift ||r - rold||^2
+// rt = mask .* (b - K x) in FP64; per-tile partials ||rt||^2, ||b||^2,
+// || |K||x| + |b| ||^2 and the drift ||rt - r||^2 (r: recursive residual)
 template <int L>
-__global__ void __launch_bounds__(WGS) residual_kernel(int64_t nrows, int R, const int32_t *__restrict__ rowptr,
-                                                       const int32_t *__restrict__ col,
-                                                       const double *__restrict__ vals, int64_t nblk,
-                                                       const double *__restrict__ x, const double *__restrict__ b,
-                                                       double *__restrict__ r, const double *__restrict__ rold,
-                                                       const double *__restrict__ mask, double *__restrict__ part)
+__global__ void __launch_bounds__(WGS) residual_kernel(const SegDesc *__restrict__ D, const int2 *__restrict__ tl)
 {
+    const int2 te = tl[xcd_tile(blockIdx.x, gridDim.x)];
+    const SegDesc &d = D[te.x];
+    const int R = d.R;
     extern __shared__ __align__(16) char sm[];
     int32_t *srp = tile_rowptr(sm);
     double *s_rr = tile_dbl(sm, R, 0), *s_bb = tile_dbl(sm, R, 1), *s_kx = tile_dbl(sm, R, 2),
            *s_gap = tile_dbl(sm, R, 3);
     for (int i = threadIdx.x; i < R; i += WGS) s_rr[i] = s_bb[i] = s_kx[i] = s_gap[i] = 0.0;
     int64_t r0;
-    const int nr = load_tile_rows(rowptr, nrows, R, srp, r0);
+    const int nr = load_rows(d.rowptr, d.nnodes, R, te.y, srp, r0);
     constexpr int RPW = WGS / L;
     const int sub = threadIdx.x % L;
+    const double *__restrict__ b = d.b;
+    const double *__restrict__ mask = d.mask;
     for (int i = threadIdx.x / L; i < nr; i += RPW) {
         double y[3], a[3];
-        row_sum<L, false, double, double, double>(srp[i], srp[i + 1], sub, col, vals, nblk, x, y);
-        row_sum<L, true, double, double, double>(srp[i], srp[i + 1], sub, col, vals, nblk, x, a);
+        row_sum<L, false, double, double, double>(srp[i], srp[i + 1], sub, d.col, d.Kv, d.x, y);
+        row_sum<L, true, double, double, double>(srp[i], srp[i + 1], sub, d.col, d.Kv, d.x, a);
         if (sub == 0) {
             double rr = 0.0, bb = 0.0, kx = 0.0, gap = 0.0;
-            for (int d = 0; d < 3; d++) {
-                const int64_t k = (r0 + i) * 3 + d;
+            for (int c = 0; c < 3; c++) {
+                const int64_t k = (r0 + i) * 3 + c;
                 const bool fr = mask[k] != 0.0;
-                const double ri = fr ? b[k] - y[d] : 0.0;
+                const double ri = fr ? b[k] - y[c] : 0.0;
                 const double bi = fr ? b[k] : 0.0;
-                const double ai = fr ? a[d] + fabs(b[k]) : 0.0;
-                r[k] = ri;
+                const double ai = fr ? a[c] + fabs(b[k]) : 0.0;
+                d.rt[k] = ri;
                 rr = fma(ri, ri, rr);
                 bb = fma(bi, bi, bb);
                 kx = fma(ai, ai, kx);
-                if (rold) {
-                    const double g = ri - rold[k];
-                    gap = fma(g, g, gap);
-                }
+                const double g = ri - d.r[k];
+                gap = fma(g, g, gap);
             }
             s_rr[i] = rr;
             s_bb[i] = bb;
@@ -284,17 +342,16 @@ __global__ void __launch_bounds__(WGS) residual_kernel(int64_t nrows, int R, con
     const double trr = lds_tree_sum(s_rr, R), tbb = lds_tree_sum(s_bb, R), tkx = lds_tree_sum(s_kx, R),
                  tgap = lds_tree_sum(s_gap, R);
     if (threadIdx.x == 0) {
-        const int64_t t = xcd_tile(blockIdx.x, gridDim.x);
-        part[P_A + t] = trr;
-        part[P_B + t] = tbb;
-        part[P_C + t] = tkx;
-        part[P_D + t] = tgap;
+        d.part[P_A + te.y] = trr;
+        d.part[P_B + te.y] = tbb;
+        d.part[P_C + te.y] = tkx;
+        d.part[P_D + te.y] = tgap;
     }
 }
 
 // ------------------------------------------------------------------- PCG
 //
-// Iteration k (parity pi = k % 2), three kernels:
+// Iteration k of a model (parity pi = k % 2), three kernels:
 //   pcg_spmv:   q = A p, partial p.q                          -> P_PQ
 //   pcg_update: a = rz_k / pq; x += a p; r -= a q; z = M r;
 //               partials rz_{k+1}, rr                         -> P_RZ[pi], P_RR[pi]
@@ -302,60 +359,75 @@ __global__ void __launch_bounds__(WGS) residual_kernel(int64_t nrows, int R, con
 // rz_k is the sum of P_RZ[1-pi] (written by iteration k-1, by the
 // initialisation, or by a residual replacement).
 
+template <typename VT>
+__device__ inline const VT *seg_vals(const SegDesc &d);
+template <>
+__device__ inline const double *seg_vals<double>(const SegDesc &d) { return d.Kv; }
+template <>
+__device__ inline const float *seg_vals<float>(const SegDesc &d) { return d.Kv32; }
+template <typename XT>
+__device__ inline const XT *seg_px(const SegDesc &d);
+template <>
+__device__ inline const double *seg_px<double>(const SegDesc &d) { return d.p; }
+template <>
+__device__ inline const float *seg_px<float>(const SegDesc &d) { return d.p32; }
+
 template <int L, typename VT, typename XT, typename AT>
-__global__ void __launch_bounds__(WGS) pcg_spmv_kernel(int64_t nrows, int R, const int32_t *__restrict__ rowptr,
-                                                       const int32_t *__restrict__ col,
-                                                       const VT *__restrict__ vals, int64_t nblk,
-                                                       const XT *__restrict__ px, const double *__restrict__ p,
-                                                       double *__restrict__ q, const double *__restrict__ mask,
-                                                       double *__restrict__ part, int32_t *__restrict__ ctl)
+__global__ void __launch_bounds__(WGS) pcg_spmv_kernel(const SegDesc *__restrict__ D, const int2 *__restrict__ tl)
 {
+    const int2 te = tl[xcd_tile(blockIdx.x, gridDim.x)];
+    const SegDesc &d = D[te.x];
+    int32_t *ctl = d.ctl;
     if (ctl[C_DONE]) return;
     if (ctl[C_STOP]) {
         // converged in the previous iteration: every workgroup leaves, and
-        // the first one raises C_DONE for the kernels that follow
-        if (blockIdx.x == 0 && threadIdx.x == 0) ctl[C_DONE] = 1;
+        // the first tile raises C_DONE for the kernels that follow
+        if (te.y == 0 && threadIdx.x == 0) ctl[C_DONE] = 1;
         return;
     }
+    if (te.y == 0 && threadIdx.x == 0) ctl[C_ITERS] = ctl[C_ITERS] + 1;
+    const int R = d.R;
     extern __shared__ __align__(16) char sm[];
     int32_t *srp = tile_rowptr(sm);
     double *sdot = tile_dbl(sm, R, 0);
     static_assert(sizeof(AT) >= sizeof(VT), "accumulate at least in the storage precision");
     for (int i = threadIdx.x; i < R; i += WGS) sdot[i] = 0.0;
     int64_t r0;
-    const int nr = load_tile_rows(rowptr, nrows, R, srp, r0);
+    const int nr = load_rows(d.rowptr, d.nnodes, R, te.y, srp, r0);
     constexpr int RPW = WGS / L;
     const int sub = threadIdx.x % L;
+    const VT *vals = seg_vals<VT>(d);
+    const XT *px = seg_px<XT>(d);
     for (int i = threadIdx.x / L; i < nr; i += RPW) {
         AT y[3];
-        row_sum<L, false, VT, XT, AT>(srp[i], srp[i + 1], sub, col, vals, nblk, px, y);
+        row_sum<L, false, VT, XT, AT>(srp[i], srp[i + 1], sub, d.col, vals, px, y);
         if (sub == 0) {
             double c = 0.0;
-            for (int d = 0; d < 3; d++) {
-                const int64_t k = (r0 + i) * 3 + d;
-                const double qk = mask[k] != 0.0 ? (double)y[d] : 0.0;
-                q[k] = qk;
-                c = fma(p[k], qk, c);
+            for (int e = 0; e < 3; e++) {
+                const int64_t k = (r0 + i) * 3 + e;
+                const double qk = d.mask[k] != 0.0 ? (double)y[e] : 0.0;
+                d.q[k] = qk;
+                c = fma(d.p[k], qk, c);
             }
             sdot[i] = c;
         }
     }
     __syncthreads();
     const double pq = lds_tree_sum(sdot, R);
-    if (threadIdx.x == 0) part[P_PQ + xcd_tile(blockIdx.x, gridDim.x)] = pq;
+    if (threadIdx.x == 0) d.part[P_PQ + te.y] = pq;
 }
 
-__global__ void __launch_bounds__(WGV) pcg_update_kernel(int64_t nnodes, double *__restrict__ x,
-                                                         double *__restrict__ r, double *__restrict__ z,
-                                                         const double *__restrict__ p, const double *__restrict__ q,
-                                                         const Prec P, double *__restrict__ part, int npq, int nv,
-                                                         int pi, int32_t *__restrict__ ctl)
+__global__ void __launch_bounds__(WGV) pcg_update_kernel(const SegDesc *__restrict__ D, const int2 *__restrict__ vl)
 {
+    const int2 ve = vl[blockIdx.x];
+    const SegDesc &d = D[ve.x];
+    int32_t *ctl = d.ctl;
     if (ctl[C_DONE]) return;
-    const double pq = sum_partials<WGV>(part + P_PQ, npq);
-    const double rz = sum_partials<WGV>(part + (pi ? P_RZ0 : P_RZ1), nv);
+    const int pi = (ctl[C_ITERS] - 1) & 1;
+    const double pq = sum_partials<WGV>(d.part + P_PQ, d.ntiles);
+    const double rz = sum_rz(d, 1 - pi);
     if (!(pq > 0.0) || !std::isfinite(pq)) {
-        if (blockIdx.x == 0 && threadIdx.x == 0) {
+        if (ve.y == 0 && threadIdx.x == 0) {
             ctl[C_BREAK] = 1;
             ctl[C_DONE] = 1;
         }
@@ -363,136 +435,221 @@ __global__ void __launch_bounds__(WGV) pcg_update_kernel(int64_t nnodes, double 
     }
     const double a = rz / pq;
     double rzn = 0.0, rr = 0.0;
-    for (int64_t nd = (int64_t)blockIdx.x * WGV + threadIdx.x; nd < nnodes; nd += (int64_t)gridDim.x * WGV) {
+    double *__restrict__ x = d.x, *__restrict__ r = d.r, *__restrict__ z = d.z;
+    const double *__restrict__ p = d.p, *__restrict__ q = d.q;
+    const int64_t stride = (int64_t)d.nvblk * WGV;
+    for (int64_t nd = (int64_t)ve.y * WGV + threadIdx.x; nd < d.nnodes; nd += stride) {
         double rv[3], zv[3];
-        for (int d = 0; d < 3; d++) {
-            const int64_t i = nd * 3 + d;
+        for (int c = 0; c < 3; c++) {
+            const int64_t i = nd * 3 + c;
             x[i] = fma(a, p[i], x[i]);
-            rv[d] = fma(-a, q[i], r[i]);
-            r[i] = rv[d];
+            rv[c] = fma(-a, q[i], r[i]);
+            r[i] = rv[c];
         }
-        apply_prec(P, nd, rv, zv);
-        for (int d = 0; d < 3; d++) {
-            z[nd * 3 + d] = zv[d];
-            rzn = fma(rv[d], zv[d], rzn);
-            rr = fma(rv[d], rv[d], rr);
+        apply_prec(d, nd, rv, zv);
+        for (int c = 0; c < 3; c++) {
+            z[nd * 3 + c] = zv[c];
+            rzn = fma(rv[c], zv[c], rzn);
+            rr = fma(rv[c], rv[c], rr);
         }
     }
     rzn = block_sum<WGV>(rzn);
     rr = block_sum<WGV>(rr);
     if (threadIdx.x == 0) {
-        part[(pi ? P_RZ1 : P_RZ0) + blockIdx.x] = rzn;
-        part[(pi ? P_RR1 : P_RR0) + blockIdx.x] = rr;
+        d.part[(pi ? P_RZ1 : P_RZ0) + ve.y] = rzn;
+        d.part[(pi ? P_RR1 : P_RR0) + ve.y] = rr;
     }
 }
 
-__global__ void __launch_bounds__(WGV) pcg_dir_kernel(int64_t n, double *__restrict__ p, float *__restrict__ p32,
-                                                      const double *__restrict__ z, const double *__restrict__ part,
-                                                      int nv, int pi, int iter, int maxiter,
-                                                      int32_t *__restrict__ ctl, double *__restrict__ scal)
+__global__ void __launch_bounds__(WGV) pcg_dir_kernel(const SegDesc *__restrict__ D, const int2 *__restrict__ vl)
 {
+    const int2 ve = vl[blockIdx.x];
+    const SegDesc &d = D[ve.x];
+    int32_t *ctl = d.ctl;
     if (ctl[C_DONE]) return;
-    const double rzn = sum_partials<WGV>(part + (pi ? P_RZ1 : P_RZ0), nv);
-    const double rzo = sum_partials<WGV>(part + (pi ? P_RZ0 : P_RZ1), nv);
-    const double rr = sum_partials<WGV>(part + (pi ? P_RR1 : P_RR0), nv);
+    const int it = ctl[C_ITERS], pi = (it - 1) & 1;
+    const double rzn = sum_rz(d, pi);
+    const double rzo = sum_rz(d, 1 - pi);
+    const double rr = sum_partials<WGV>(d.part + (pi ? P_RR1 : P_RR0), d.nvblk);
     const double beta = rzn / rzo;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        ctl[C_ITERS] = iter;
-        scal[S_RR] = rr;
-        scal[S_RZ] = rzn;
-        if (rr <= scal[S_THR] || iter >= maxiter || !std::isfinite(rr)) ctl[C_STOP] = 1;
+    if (ve.y == 0 && threadIdx.x == 0) {
+        d.scal[S_RR] = rr;
+        d.scal[S_RZ] = rzn;
+        if (rr <= d.scal[S_THR] || it >= d.maxiter || !std::isfinite(rr)) ctl[C_STOP] = 1;
     }
-    for (int64_t i = (int64_t)blockIdx.x * WGV + threadIdx.x; i < n; i += (int64_t)gridDim.x * WGV) {
-        const double pn = fma(beta, p[i], z[i]);
+    double *__restrict__ p = d.p;
+    float *__restrict__ p32 = d.wp32 ? d.p32 : nullptr;
+    const double *__restrict__ z = d.z;
+    const int64_t n = d.nnodes * 3, stride = (int64_t)d.nvblk * WGV;
+    for (int64_t i = (int64_t)ve.y * WGV + threadIdx.x; i < n; i += stride) {
+        const double zi = d.tl ? z[i] + coarse_term(d, i / 3, (int)(i % 3)) : z[i];
+        const double pn = fma(beta, p[i], zi);
         p[i] = pn;
         if (p32) p32[i] = (float)pn;
     }
 }
 
-// x = 0, r = b, z = M r, p = z; partials rz -> P_RZ1, rr -> P_RR1
-__global__ void __launch_bounds__(WGV) pcg_init_kernel(int64_t nnodes, const double *__restrict__ b,
-                                                       double *__restrict__ x, double *__restrict__ r,
-                                                       double *__restrict__ z, double *__restrict__ p,
-                                                       float *__restrict__ p32, const Prec P,
-                                                       double *__restrict__ part)
+// Two-level pieces, for the members with d.tl (mode 0: inside the iteration,
+// skipped once the member is done; mode 1: initialisation and resume)
+//   restrict: c_a = sum_{i in a} R_i^T r_i, one wave per aggregate
+//   coarse:   y = Kc^-1 c, one wave per coarse row, and partials of c.y
+//   prolong:  p = z + Phi y (start of a search direction)
+__global__ void __launch_bounds__(64) tl_restrict_kernel(const SegDesc *__restrict__ D, const int2 *__restrict__ al,
+                                                         int mode)
 {
+    const int2 e = al[blockIdx.x];
+    const SegDesc &d = D[e.x];
+    if (mode == 0 && d.ctl[C_DONE]) return;
+    const int32_t a = e.y;
+    const double cx = d.cent[a * 3], cy = d.cent[a * 3 + 1], cz = d.cent[a * 3 + 2];
+    double v[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int32_t k = d.aggptr[a] + threadIdx.x; k < d.aggptr[a + 1]; k += 64) {
+        const int64_t nd = d.aggnodes[k];
+        const double r0 = d.r[nd * 3], r1 = d.r[nd * 3 + 1], r2 = d.r[nd * 3 + 2];
+        const double dx = d.X[nd * 3] - cx, dy = d.X[nd * 3 + 1] - cy, dz = d.X[nd * 3 + 2] - cz;
+        v[0] += r0;
+        v[1] += r1;
+        v[2] += r2;
+        v[3] += fma(dy, r2, -dz * r1);
+        v[4] += fma(dz, r0, -dx * r2);
+        v[5] += fma(dx, r1, -dy * r0);
+    }
+    for (int q = 0; q < 6; q++)
+        for (int off = 32; off > 0; off >>= 1) v[q] += __shfl_down(v[q], off, 64);
+    if (threadIdx.x == 0)
+        for (int q = 0; q < 6; q++) d.cvec[a * 6 + q] = v[q];
+}
+
+__global__ void __launch_bounds__(256) tl_coarse_kernel(const SegDesc *__restrict__ D, const int2 *__restrict__ wl,
+                                                        int mode)
+{
+    const int2 e = wl[blockIdx.x];
+    const SegDesc &d = D[e.x];
+    if (mode == 0 && d.ctl[C_DONE]) return;
+    const int pi = (d.ctl[C_ITERS] - 1) & 1;
+    const int row = e.y * 4 + (int)(threadIdx.x / 64), lane = threadIdx.x % 64;
+    double acc = 0.0;
+    if (row < d.nc) {
+        const double *Kr = d.Kinv + (int64_t)row * d.nc;
+        for (int c = lane; c < d.nc; c += 64) acc = fma(Kr[c], d.cvec[c], acc);
+    }
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+    __shared__ double cyw[4];
+    if (lane == 0) {
+        if (row < d.nc) d.yvec[row] = acc;
+        cyw[threadIdx.x / 64] = row < d.nc ? d.cvec[row] * acc : 0.0;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) d.part[(pi ? P_CY1 : P_CY0) + e.y] = ((cyw[0] + cyw[1]) + cyw[2]) + cyw[3];
+}
+
+__global__ void __launch_bounds__(WGV) tl_prolong_kernel(const SegDesc *__restrict__ D, const int2 *__restrict__ vl)
+{
+    const int2 ve = vl[blockIdx.x];
+    const SegDesc &d = D[ve.x];
+    float *__restrict__ p32 = d.wp32 ? d.p32 : nullptr;
+    const int64_t n = d.nnodes * 3, stride = (int64_t)d.nvblk * WGV;
+    for (int64_t i = (int64_t)ve.y * WGV + threadIdx.x; i < n; i += stride) {
+        const double pn = d.z[i] + coarse_term(d, i / 3, (int)(i % 3));
+        d.p[i] = pn;
+        if (p32) p32[i] = (float)pn;
+    }
+}
+
+// x = 0, r = b, z = M r, p = z; partials rz -> P_RZ1, rr -> P_RR1
+__global__ void __launch_bounds__(WGV) pcg_init_kernel(const SegDesc *__restrict__ D, const int2 *__restrict__ vl)
+{
+    const int2 ve = vl[blockIdx.x];
+    const SegDesc &d = D[ve.x];
+    // iteration 0: the two-level kernels after this one write the parity
+    // slot of C_ITERS - 1, i.e. slot 1, like the Jacobi partials below
+    if (ve.y == 0 && threadIdx.x == 0) d.ctl[C_ITERS] = 0;
     double rz = 0.0, rr = 0.0;
-    for (int64_t nd = (int64_t)blockIdx.x * WGV + threadIdx.x; nd < nnodes; nd += (int64_t)gridDim.x * WGV) {
+    float *__restrict__ p32 = d.wp32 ? d.p32 : nullptr;
+    const int64_t stride = (int64_t)d.nvblk * WGV;
+    for (int64_t nd = (int64_t)ve.y * WGV + threadIdx.x; nd < d.nnodes; nd += stride) {
         double rv[3], zv[3];
-        for (int d = 0; d < 3; d++) rv[d] = b[nd * 3 + d];
-        apply_prec(P, nd, rv, zv);
-        for (int d = 0; d < 3; d++) {
-            const int64_t i = nd * 3 + d;
-            x[i] = 0.0;
-            r[i] = rv[d];
-            z[i] = zv[d];
-            p[i] = zv[d];
-            if (p32) p32[i] = (float)zv[d];
-            rz = fma(rv[d], zv[d], rz);
-            rr = fma(rv[d], rv[d], rr);
+        for (int c = 0; c < 3; c++) rv[c] = d.b[nd * 3 + c];
+        apply_prec(d, nd, rv, zv);
+        for (int c = 0; c < 3; c++) {
+            const int64_t i = nd * 3 + c;
+            d.x[i] = 0.0;
+            d.r[i] = rv[c];
+            d.z[i] = zv[c];
+            d.p[i] = zv[c];
+            if (p32) p32[i] = (float)zv[c];
+            rz = fma(rv[c], zv[c], rz);
+            rr = fma(rv[c], rv[c], rr);
         }
     }
     rz = block_sum<WGV>(rz);
     rr = block_sum<WGV>(rr);
     if (threadIdx.x == 0) {
-        part[P_RZ1 + blockIdx.x] = rz;
-        part[P_RR1 + blockIdx.x] = rr;
+        d.part[P_RZ1 + ve.y] = rz;
+        d.part[P_RR1 + ve.y] = rr;
     }
 }
 
 // r = rt; z = M r; with restart also p = z (the search direction starts
 // afresh); rz partials into the slot the next iteration reads as "previous".
-__global__ void __launch_bounds__(WGV) pcg_rez_kernel(int64_t nnodes, const double *__restrict__ rt,
-                                                      double *__restrict__ r, double *__restrict__ z,
-                                                      double *__restrict__ p, float *__restrict__ p32, int restart,
-                                                      const Prec P, double *__restrict__ part, int slot_pi)
+__global__ void __launch_bounds__(WGV) pcg_rez_kernel(const SegDesc *__restrict__ D, const int2 *__restrict__ vl,
+                                                      int restart)
 {
+    const int2 ve = vl[blockIdx.x];
+    const SegDesc &d = D[ve.x];
+    const int slot_pi = 1 - (d.ctl[C_ITERS] & 1);
+    float *__restrict__ p32 = d.wp32 ? d.p32 : nullptr;
     double rz = 0.0;
-    for (int64_t nd = (int64_t)blockIdx.x * WGV + threadIdx.x; nd < nnodes; nd += (int64_t)gridDim.x * WGV) {
+    const int64_t stride = (int64_t)d.nvblk * WGV;
+    for (int64_t nd = (int64_t)ve.y * WGV + threadIdx.x; nd < d.nnodes; nd += stride) {
         double rv[3], zv[3];
-        for (int d = 0; d < 3; d++) rv[d] = rt[nd * 3 + d];
-        apply_prec(P, nd, rv, zv);
-        for (int d = 0; d < 3; d++) {
-            const int64_t i = nd * 3 + d;
-            r[i] = rv[d];
-            z[i] = zv[d];
+        for (int c = 0; c < 3; c++) rv[c] = d.rt[nd * 3 + c];
+        apply_prec(d, nd, rv, zv);
+        for (int c = 0; c < 3; c++) {
+            const int64_t i = nd * 3 + c;
+            d.r[i] = rv[c];
+            d.z[i] = zv[c];
             if (restart) {
-                p[i] = zv[d];
-                if (p32) p32[i] = (float)zv[d];
+                d.p[i] = zv[c];
+                if (p32) p32[i] = (float)zv[c];
             }
-            rz = fma(rv[d], zv[d], rz);
+            rz = fma(rv[c], zv[c], rz);
         }
     }
     rz = block_sum<WGV>(rz);
-    if (threadIdx.x == 0) part[(slot_pi ? P_RZ1 : P_RZ0) + blockIdx.x] = rz;
+    if (threadIdx.x == 0) d.part[(slot_pi ? P_RZ1 : P_RZ0) + ve.y] = rz;
 }
 
-// One workgroup publishes scalars from partials.
+// One workgroup per listed model publishes scalars from partials.
 //   mode 0 (after init):        rz, rr = bb from slot 1; recursive threshold; reset control
-//   mode 1 (after replacement): rr, bb, kx from the residual partials, rz from the slot
-__global__ void __launch_bounds__(WGV) pcg_finalize_kernel(int mode, int nv, int nr, int slot_pi, double thr_fac,
-                                                           const double *__restrict__ part,
-                                                           int32_t *__restrict__ ctl, double *__restrict__ scal)
+//   mode 1 (after replacement): rr, bb, kx, drift from the residual partials
+//   mode 2 (resume):            rz from the slot written by pcg_rez; clear the stop flags
+__global__ void __launch_bounds__(WGV) pcg_finalize_kernel(const SegDesc *__restrict__ D,
+                                                           const int32_t *__restrict__ sl, int mode)
 {
+    const SegDesc &d = D[sl[blockIdx.x]];
+    int32_t *ctl = d.ctl;
+    double *scal = d.scal;
     if (mode == 0) {
-        const double rz = sum_partials<WGV>(part + P_RZ1, nv);
-        const double rr = sum_partials<WGV>(part + P_RR1, nv);
+        const double rz = sum_rz(d, 1);
+        const double rr = sum_partials<WGV>(d.part + P_RR1, d.nvblk);
         if (threadIdx.x == 0) {
             scal[S_RZ] = rz;
             scal[S_RR] = rr;
             scal[S_BB] = rr;
             scal[S_KX] = 0.0;
-            scal[S_THR] = thr_fac * rr;
+            scal[S_GAP] = 0.0;
+            scal[S_THR] = d.thr_fac * rr;
             ctl[C_ITERS] = 0;
             ctl[C_BREAK] = 0;
             ctl[C_STOP] = 0;
             ctl[C_DONE] = rr == 0.0 ? 1 : 0;
         }
     } else if (mode == 1) {
-        const double rr = sum_partials<WGV>(part + P_A, nr);
-        const double bb = sum_partials<WGV>(part + P_B, nr);
-        const double kx = sum_partials<WGV>(part + P_C, nr);
-        const double gap = sum_partials<WGV>(part + P_D, nr);
+        const double rr = sum_partials<WGV>(d.part + P_A, d.ntiles);
+        const double bb = sum_partials<WGV>(d.part + P_B, d.ntiles);
+        const double kx = sum_partials<WGV>(d.part + P_C, d.ntiles);
+        const double gap = sum_partials<WGV>(d.part + P_D, d.ntiles);
         if (threadIdx.x == 0) {
             scal[S_RR] = rr;
             scal[S_BB] = bb;
@@ -500,16 +657,16 @@ __global__ void __launch_bounds__(WGV) pcg_finalize_kernel(int mode, int nv, int
             scal[S_GAP] = gap;
         }
     } else {
-        const double rz = sum_partials<WGV>(part + (slot_pi ? P_RZ1 : P_RZ0), nv);
+        const int slot_pi = 1 - (ctl[C_ITERS] & 1);
+        const double rz = sum_rz(d, slot_pi);
         if (threadIdx.x == 0) {
             scal[S_RZ] = rz;
             ctl[C_DONE] = 0;
             ctl[C_STOP] = 0;
+            ctl[C_BREAK] = 0;
         }
     }
 }
-
-__global__ void clear_break_kernel(int32_t *__restrict__ ctl) { ctl[C_BREAK] = 0; }
 
 __global__ void fill_f32_kernel(int64_t n, float *__restrict__ x, float v)
 {
@@ -521,88 +678,64 @@ __global__ void fill_kernel(int64_t n, double *__restrict__ x, double v)
     for (int64_t i = (int64_t)blockIdx.x * WGV + threadIdx.x; i < n; i += (int64_t)gridDim.x * WGV) x[i] = v;
 }
 
-struct Launch {
-    int R;                     // block rows per SpMV workgroup
-    unsigned grid_rows, grid_vec;
+// Rows per tile R (a power of two, at least 128, grown until the model has
+// at most MAX_GR tiles) and vector workgroups (at most MAX_GV) of a model.
+struct Geom {
+    int R, ntiles, nvblk;
 };
 
-Launch pick_launch(const amr_model *m)
+Geom model_geom(const amr_model *m)
 {
-    Launch L;
+    Geom g;
     int R = 128;
     while ((m->nnodes + R - 1) / R > MAX_GR) R *= 2;
-    L.R = R;
-    L.grid_rows = (unsigned)std::max<int64_t>(1, (m->nnodes + R - 1) / R);
+    g.R = R;
+    g.ntiles = (int)std::max<int64_t>(1, (m->nnodes + R - 1) / R);
     const int64_t nv = (m->nnodes * 3 + WGV - 1) / WGV;
-    L.grid_vec = (unsigned)std::max<int64_t>(1, std::min<int64_t>(nv, MAX_GV));
-    return L;
+    g.nvblk = (int)std::max<int64_t>(1, std::min<int64_t>(nv, MAX_GV));
+    return g;
 }
 
 size_t tile_lds(int R, int ndbl) { return (size_t)(((R + 1) * 4 + 15) & ~15) + (size_t)ndbl * R * 8; }
 
-template <typename VT, typename XT, typename AT = VT>
-void launch_pcg_spmv_lanes(amr_model *m, const Launch &L, int lanes, const VT *vals, const XT *px, const double *p,
-                           double *q)
+// SpMV operand kinds: FP64 values; FP32 values with FP32 p and FP32
+// accumulation; FP32 values with FP64 gather and accumulation.
+enum { K_F64 = 0, K_F32, K_F32D, K_NKIND };
+
+int lanes_index(int lanes) { return lanes == 8 ? 0 : lanes == 16 ? 1 : lanes == 32 ? 2 : 3; }
+constexpr int LANES[4] = {8, 16, 32, 64};
+
+int kind_lanes(const amr_model *m, int kind)
 {
-    int32_t *ctl = (int32_t *)(m->scal + S_NSCAL);
-    const size_t lds = tile_lds(L.R, 1);
-#define AMR_PCG_SPMV(LN)                                                                                           \
-    hipLaunchKernelGGL((pcg_spmv_kernel<LN, VT, XT, AT>), dim3(L.grid_rows), dim3(WGS), lds, m->stream, m->nnodes, \
-                       L.R, m->rowptr, m->col, vals, m->pat.nblk, px, p, q, m->mask, m->part, ctl)
+    return kind == K_F64 ? m->lanes64 : kind == K_F32 ? m->lanes32 : m->lanes32d;
+}
+
+void launch_pcg_spmv(int kind, int lanes, unsigned grid, int Rmax, hipStream_t s, const SegDesc *D, const int2 *tl)
+{
+    const size_t lds = tile_lds(Rmax, 1);
+#define AMR_PCG_SPMV(LN, VT, XT, AT) \
+    hipLaunchKernelGGL((pcg_spmv_kernel<LN, VT, XT, AT>), dim3(grid), dim3(WGS), lds, s, D, tl)
+#define AMR_PCG_KIND(LN)                                          \
+    do {                                                          \
+        if (kind == K_F64) AMR_PCG_SPMV(LN, double, double, double); \
+        else if (kind == K_F32) AMR_PCG_SPMV(LN, float, float, float); \
+        else AMR_PCG_SPMV(LN, float, double, double);             \
+    } while (0)
     switch (lanes) {
-    case 8: AMR_PCG_SPMV(8); break;
-    case 16: AMR_PCG_SPMV(16); break;
-    case 32: AMR_PCG_SPMV(32); break;
-    default: AMR_PCG_SPMV(64); break;
+    case 8: AMR_PCG_KIND(8); break;
+    case 16: AMR_PCG_KIND(16); break;
+    case 32: AMR_PCG_KIND(32); break;
+    default: AMR_PCG_KIND(64); break;
     }
+#undef AMR_PCG_KIND
 #undef AMR_PCG_SPMV
 }
 
-// Autotuner: time each lanes-per-row choice on this model's matrix and keep
-// the fastest.  Every choice gives bit-identical results (see row_sum), so
-// the choice only affects speed.
-template <typename VT, typename XT, typename AT = VT>
-int tune_lanes(amr_model *m, const Launch &L, const VT *vals, const XT *px, const double *p, double *q, int *out)
+void launch_residual(int lanes, unsigned grid, int Rmax, hipStream_t s, const SegDesc *D, const int2 *tl)
 {
-    static const int cand[4] = {8, 16, 32, 64};
-    hipEvent_t e0, e1;
-    AMR_HIP(hipEventCreate(&e0));
-    AMR_HIP(hipEventCreate(&e1));
-    float best = 1e30f;
-    int bestL = 64;
-    for (int c : cand) {
-        launch_pcg_spmv_lanes<VT, XT, AT>(m, L, c, vals, px, p, q);  // warm
-        AMR_HIP(hipEventRecord(e0, m->stream));
-        for (int rep = 0; rep < 3; rep++) launch_pcg_spmv_lanes<VT, XT, AT>(m, L, c, vals, px, p, q);
-        AMR_HIP(hipEventRecord(e1, m->stream));
-        AMR_HIP(hipEventSynchronize(e1));
-        float ms = 0.f;
-        AMR_HIP(hipEventElapsedTime(&ms, e0, e1));
-        if (ms < best) {
-            best = ms;
-            bestL = c;
-        }
-    }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    *out = bestL;
-    return AMR_OK;
-}
-
-template <typename VT, typename XT, typename AT = VT>
-void launch_pcg_spmv(amr_model *m, const Launch &L, const VT *vals, const XT *px, const double *p, double *q)
-{
-    const int lanes = sizeof(VT) == 8 ? m->lanes64 : (sizeof(AT) == 8 ? m->lanes32d : m->lanes32);
-    launch_pcg_spmv_lanes<VT, XT, AT>(m, L, lanes, vals, px, p, q);
-}
-
-void launch_residual(amr_model *m, const Launch &L, const double *x, const double *b, double *r, const double *rold)
-{
-    const size_t lds = tile_lds(L.R, 4);
-#define AMR_RES(LN)                                                                                                \
-    hipLaunchKernelGGL((residual_kernel<LN>), dim3(L.grid_rows), dim3(WGS), lds, m->stream, m->nnodes, L.R,       \
-                       m->rowptr, m->col, m->Kv, m->pat.nblk, x, b, r, rold, m->mask, m->part)
-    switch (m->lanes64) {
+    const size_t lds = tile_lds(Rmax, 4);
+#define AMR_RES(LN) hipLaunchKernelGGL((residual_kernel<LN>), dim3(grid), dim3(WGS), lds, s, D, tl)
+    switch (lanes) {
     case 8: AMR_RES(8); break;
     case 16: AMR_RES(16); break;
     case 32: AMR_RES(32); break;
@@ -611,13 +744,660 @@ void launch_residual(amr_model *m, const Launch &L, const double *x, const doubl
 #undef AMR_RES
 }
 
-int read_scalars(amr_model *m, double *hs, int32_t *hc)
+}  // namespace
+
+// ------------------------------------------------------------- the pool
+
+namespace {
+
+struct Member {
+    amr_model *m = nullptr;
+    amr_solve_opts o{};
+    const double *b = nullptr;
+    double *x = nullptr;
+    int state = 0;  // 0 free, 1 joining, 2 active
+    bool mixed = false, f32x = false;
+    double bb = 0.0, rr_ref = 0.0, rr_true_prev = INFINITY, relres = 0.0;
+    int converged = 0, broke = 0, replacements = 0, iters = 0, fell_back = 0, breakdowns = 0;
+    int last_rep = 0;  // iteration of the last residual replacement
+    // per-step scratch
+    bool rep = false, done = false;
+    double rr_rec = 0.0;
+};
+
+enum { M_FREE = 0, M_JOIN, M_ACTIVE };
+
+}  // namespace
+
+struct amr_pool {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    int cap = 0, check = 32;
+    std::vector<Member> mem;
+    std::vector<amr_model *> attached;
+    // device descriptors and scalars, with pinned host mirrors
+    SegDesc *dD = nullptr, *hD = nullptr;
+    double *dS = nullptr, *hS = nullptr;
+    // work lists: staged in pinned memory, uploaded in one copy per use
+    char *dL = nullptr, *hL = nullptr;
+    size_t lcap = 0, lused = 0;
+    bool pending = false;  // an upload from the pinned mirrors may still be in flight
+    bool desc_dirty = false;
+    // SpMV timing
+    bool timing = false;
+    std::vector<hipEvent_t> ev;
+    double t_spmv_ms = 0.0;
+    int64_t n_spmv = 0;
+};
+
+namespace {
+
+int pool_sync(amr_pool *p)
 {
-    AMR_HIP(hipMemcpyAsync(hs, m->scal, sizeof(double) * S_NSCAL, hipMemcpyDeviceToHost, m->stream));
-    AMR_HIP(hipMemcpyAsync(hc, m->scal + S_NSCAL, sizeof(int32_t) * C_NCTL, hipMemcpyDeviceToHost, m->stream));
-    AMR_HIP(hipStreamSynchronize(m->stream));
+    AMR_HIP(hipStreamSynchronize(p->stream));
+    p->pending = false;
     return AMR_OK;
 }
+
+// Make the pinned mirrors writable: wait for any upload still reading them.
+int pool_begin_host_writes(amr_pool *p)
+{
+    if (p->pending) return pool_sync(p);
+    return AMR_OK;
+}
+
+int pool_reserve_lists(amr_pool *p, size_t bytes)
+{
+    if (bytes <= p->lcap) return AMR_OK;
+    // kernels queued earlier may still read the device list buffer
+    if (int rc = pool_sync(p)) return rc;
+    size_t ncap = std::max<size_t>(bytes, 2 * p->lcap);
+    ncap = std::max<size_t>(ncap, 1 << 16);
+    if (p->dL) AMR_HIP(hipFree(p->dL));
+    if (p->hL) AMR_HIP(hipHostFree(p->hL));
+    p->dL = p->hL = nullptr;
+    p->lcap = 0;
+    AMR_HIP(hipMalloc(&p->dL, ncap));
+    AMR_HIP(hipHostMalloc(&p->hL, ncap, hipHostMallocDefault));
+    p->lcap = ncap;
+    return AMR_OK;
+}
+
+// List staging: reset, append, upload.  Offsets are 16-byte aligned.
+struct Lists {
+    amr_pool *p;
+    size_t off = 0;
+    template <typename T>
+    T *dev_at(size_t o) const { return (T *)(p->dL + o); }
+    template <typename T>
+    size_t push(const std::vector<T> &v)
+    {
+        const size_t o = off;
+        if (!v.empty()) std::memcpy(p->hL + o, v.data(), v.size() * sizeof(T));
+        off = (o + v.size() * sizeof(T) + 15) & ~(size_t)15;
+        return o;
+    }
+};
+
+template <typename T>
+size_t padded(const std::vector<T> &v) { return (v.size() * sizeof(T) + 15) & ~(size_t)15; }
+
+int upload(amr_pool *p, size_t bytes, bool desc)
+{
+    if (bytes) AMR_HIP(hipMemcpyAsync(p->dL, p->hL, bytes, hipMemcpyHostToDevice, p->stream));
+    if (desc) {
+        AMR_HIP(hipMemcpyAsync(p->dD, p->hD, sizeof(SegDesc) * p->cap, hipMemcpyHostToDevice, p->stream));
+        p->desc_dirty = false;
+    }
+    if (bytes || desc) p->pending = true;
+    return AMR_OK;
+}
+
+void fill_desc(amr_pool *p, int slot)
+{
+    Member &M = p->mem[slot];
+    amr_model *m = M.m;
+    const Geom g = model_geom(m);
+    SegDesc &d = p->hD[slot];
+    std::memset(&d, 0, sizeof d);
+    d.rowptr = m->rowptr;
+    d.col = m->col;
+    d.Kv = m->Kv;
+    d.Kv32 = m->Kv32;
+    d.mask = m->mask;
+    d.dinv = m->dinv;
+    d.binv = m->binv;
+    d.dinv32 = m->dinv32;
+    d.binv32 = m->binv32;
+    d.b = M.b;
+    d.x = M.x;
+    d.r = m->v[1];
+    d.z = m->v[2];
+    d.p = m->v[3];
+    d.q = m->v[4];
+    d.rt = m->v[5];
+    d.p32 = m->f[0];
+    d.part = m->part;
+    d.scal = p->dS + (size_t)slot * SEG_DBL;
+    d.ctl = (int32_t *)(d.scal + S_NSCAL);
+    d.nnodes = m->nnodes;
+    d.nblk = m->pat.nblk;
+    const double tr = 0.5 * M.o.rtol;
+    d.thr_fac = tr * tr;
+    d.R = g.R;
+    d.ntiles = g.ntiles;
+    d.nvblk = g.nvblk;
+    d.maxiter = M.o.maxiter;
+    d.f32 = M.mixed ? 1 : 0;
+    d.block = M.o.precond == AMR_PRECOND_BLOCK_JACOBI ? 1 : 0;
+    d.wp32 = (M.mixed && M.f32x) ? 1 : 0;
+    d.tl = M.o.precond == AMR_PRECOND_TWO_LEVEL ? 1 : 0;
+    if (d.tl) {
+        d.nagg = m->nagg;
+        d.nc = m->ncoarse;
+        d.ncw = (m->ncoarse + 3) / 4;
+        d.agg = m->c_agg;
+        d.aggptr = m->c_aggptr;
+        d.aggnodes = m->c_aggnodes;
+        d.cent = m->c_cent;
+        d.X = m->X;
+        d.Kinv = m->Kinv;
+        d.cvec = m->c_vec;
+        d.yvec = m->y_vec;
+    }
+}
+
+std::vector<int2> agg_list(amr_pool *p, const std::vector<int> &slots)
+{
+    std::vector<int2> v;
+    for (int s : slots)
+        if (p->hD[s].tl)
+            for (int a = 0; a < p->hD[s].nagg; a++) v.push_back(make_int2(s, a));
+    return v;
+}
+
+std::vector<int2> cw_list(amr_pool *p, const std::vector<int> &slots)
+{
+    std::vector<int2> v;
+    for (int s : slots)
+        if (p->hD[s].tl)
+            for (int w = 0; w < p->hD[s].ncw; w++) v.push_back(make_int2(s, w));
+    return v;
+}
+
+std::vector<int> tl_only(amr_pool *p, const std::vector<int> &slots)
+{
+    std::vector<int> v;
+    for (int s : slots)
+        if (p->hD[s].tl) v.push_back(s);
+    return v;
+}
+
+// the two-level steps after an initialisation or a resume: c, y, and with
+// `prolong` the search direction p = z + Phi y
+void launch_tl_fresh(amr_pool *p, const Lists &Ls, size_t o_al, unsigned nal, size_t o_cw, unsigned ncw,
+                     size_t o_pv, unsigned npv)
+{
+    hipStream_t st = p->stream;
+    if (nal) hipLaunchKernelGGL(tl_restrict_kernel, dim3(nal), dim3(64), 0, st, p->dD, Ls.dev_at<int2>(o_al), 1);
+    if (ncw) hipLaunchKernelGGL(tl_coarse_kernel, dim3(ncw), dim3(256), 0, st, p->dD, Ls.dev_at<int2>(o_cw), 1);
+    if (npv) hipLaunchKernelGGL(tl_prolong_kernel, dim3(npv), dim3(WGV), 0, st, p->dD, Ls.dev_at<int2>(o_pv));
+}
+
+int member_kind(const Member &M) { return !M.mixed ? K_F64 : (M.f32x ? K_F32 : K_F32D); }
+
+// Tile lists of the given members grouped by (kind, lanes); for residual
+// launches kind is ignored and the FP64 lanes are used.
+struct Group {
+    int kind, lanes, Rmax;
+    size_t off;     // device offset of the group's entries
+    unsigned n;
+};
+
+void build_groups(amr_pool *p, const std::vector<int> &slots, bool residual, std::vector<std::vector<int2>> &ent,
+                  std::vector<Group> &gr)
+{
+    ent.assign(K_NKIND * 4, {});
+    std::vector<int> rmax(K_NKIND * 4, 0);
+    for (int s : slots) {
+        const Member &M = p->mem[s];
+        const int kind = residual ? K_F64 : member_kind(M);
+        const int key = kind * 4 + lanes_index(kind_lanes(M.m, kind));
+        const SegDesc &d = p->hD[s];
+        for (int t = 0; t < d.ntiles; t++) ent[key].push_back(make_int2(s, t));
+        rmax[key] = std::max(rmax[key], d.R);
+    }
+    gr.clear();
+    for (int key = 0; key < K_NKIND * 4; key++)
+        if (!ent[key].empty()) gr.push_back({key / 4, LANES[key % 4], rmax[key], 0, (unsigned)ent[key].size()});
+}
+
+std::vector<int2> vec_list(amr_pool *p, const std::vector<int> &slots)
+{
+    std::vector<int2> v;
+    for (int s : slots)
+        for (int j = 0; j < p->hD[s].nvblk; j++) v.push_back(make_int2(s, j));
+    return v;
+}
+
+int ensure_events(amr_pool *p)
+{
+    if ((int)p->ev.size() >= 2 * p->check) return AMR_OK;
+    for (hipEvent_t e : p->ev) (void)hipEventDestroy(e);
+    p->ev.assign(2 * p->check, nullptr);
+    for (hipEvent_t &e : p->ev) AMR_HIP(hipEventCreate(&e));
+    return AMR_OK;
+}
+
+void finish(amr_pool *p, int slot, std::vector<amr_pool_result> &out)
+{
+    Member &M = p->mem[slot];
+    amr_pool_result r;
+    r.m = M.m;
+    r.info.iterations = M.iters;
+    r.info.refinements = M.replacements;
+    r.info.converged = M.converged;
+    r.info.fallback = M.fell_back;
+    r.info.relres = M.relres;
+    out.push_back(r);
+    M = Member();
+}
+
+}  // namespace
+
+int amr_pool_new(amr_pool **out, int device, int cap, int check, hipStream_t stream)
+{
+    if (cap < 1 || cap > (1 << 16)) {
+        amr_set_error("amr_pool_create: max_members must be in [1, 65536]");
+        return AMR_ERR_ARG;
+    }
+    AMR_HIP(hipSetDevice(device));
+    amr_pool *p = new amr_pool();
+    p->device = device;
+    p->cap = cap;
+    p->check = std::max(1, std::min(check > 0 ? check : 32, 256));
+    p->mem.assign(cap, Member());
+    int rc = AMR_OK;
+    auto fail = [&](hipError_t e) {
+        amr_set_error(std::string("HIP error: ") + hipGetErrorString(e) + " creating a pool");
+        amr_pool_free(p);
+        return AMR_ERR_HIP;
+    };
+    hipError_t e;
+    if (stream) p->stream = stream;
+    else {
+        if ((e = hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking)) != hipSuccess) return fail(e);
+        p->own_stream = true;
+    }
+    if ((e = hipMalloc(&p->dD, sizeof(SegDesc) * cap)) != hipSuccess) return fail(e);
+    if ((e = hipHostMalloc(&p->hD, sizeof(SegDesc) * cap, hipHostMallocDefault)) != hipSuccess) return fail(e);
+    if ((e = hipMalloc(&p->dS, sizeof(double) * SEG_DBL * cap)) != hipSuccess) return fail(e);
+    if ((e = hipHostMalloc(&p->hS, sizeof(double) * SEG_DBL * cap, hipHostMallocDefault)) != hipSuccess)
+        return fail(e);
+    std::memset(p->hD, 0, sizeof(SegDesc) * cap);
+    if ((e = hipMemsetAsync(p->dS, 0, sizeof(double) * SEG_DBL * cap, p->stream)) != hipSuccess) return fail(e);
+    if ((e = hipStreamSynchronize(p->stream)) != hipSuccess) return fail(e);
+    (void)rc;
+    *out = p;
+    return AMR_OK;
+}
+
+void amr_pool_free(amr_pool *p)
+{
+    if (!p) return;
+    (void)hipSetDevice(p->device);
+    if (p->stream) (void)hipStreamSynchronize(p->stream);
+    for (amr_model *m : p->attached) {
+        m->stream = m->own_stream;
+        m->pool = nullptr;
+    }
+    if (p->dD) (void)hipFree(p->dD);
+    if (p->hD) (void)hipHostFree(p->hD);
+    if (p->dS) (void)hipFree(p->dS);
+    if (p->hS) (void)hipHostFree(p->hS);
+    if (p->dL) (void)hipFree(p->dL);
+    if (p->hL) (void)hipHostFree(p->hL);
+    for (hipEvent_t e : p->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (p->own_stream && p->stream) (void)hipStreamDestroy(p->stream);
+    delete p;
+}
+
+int amr_pool_attach_model(amr_pool *p, amr_model *m)
+{
+    if (m->pool == p) return AMR_OK;
+    if (m->pool) {
+        amr_set_error("the model is attached to another pool");
+        return AMR_ERR_ARG;
+    }
+    if (m->device != p->device) {
+        amr_set_error("the model and the pool are on different devices");
+        return AMR_ERR_ARG;
+    }
+    AMR_HIP(hipStreamSynchronize(m->stream));
+    m->stream = p->stream;
+    m->pool = p;
+    if (m->self_pool) m->self_pool->stream = p->stream;
+    p->attached.push_back(m);
+    return AMR_OK;
+}
+
+int amr_pool_detach_model(amr_model *m)
+{
+    amr_pool *p = m->pool;
+    if (!p) return AMR_OK;
+    for (int s = 0; s < p->cap; s++)
+        if (p->mem[s].m == m) p->mem[s] = Member();
+    AMR_HIP(hipStreamSynchronize(p->stream));
+    p->attached.erase(std::remove(p->attached.begin(), p->attached.end(), m), p->attached.end());
+    m->stream = m->own_stream;
+    m->pool = nullptr;
+    if (m->self_pool) m->self_pool->stream = m->own_stream;
+    return AMR_OK;
+}
+
+bool amr_pool_is_member(const amr_model *m)
+{
+    if (!m->pool) return false;
+    for (const Member &M : m->pool->mem)
+        if (M.m == m) return true;
+    return false;
+}
+
+int amr_pool_count(const amr_pool *p)
+{
+    int n = 0;
+    for (const Member &M : p->mem) n += M.state != M_FREE;
+    return n;
+}
+
+// Join: the model's system K11 x = b starts at the next step.
+int amr_pool_join(amr_pool *p, amr_model *m, const double *b, double *x, const amr_solve_opts *o)
+{
+    if (m->stream != p->stream) {
+        amr_set_error("amr_pool_add: the model is not attached to this pool");
+        return AMR_ERR_ARG;
+    }
+    for (const Member &M : p->mem)
+        if (M.m == m) {
+            amr_set_error("amr_pool_add: the model is already solving in this pool");
+            return AMR_ERR_ARG;
+        }
+    int slot = -1;
+    for (int s = 0; s < p->cap && slot < 0; s++)
+        if (p->mem[s].state == M_FREE) slot = s;
+    if (slot < 0) {
+        amr_set_error("amr_pool_add: the pool is full");
+        return AMR_ERR_ARG;
+    }
+    if (int rc = amr_autotune(m)) return rc;
+    if (o->precond == AMR_PRECOND_TWO_LEVEL) {
+        if (int rc = amr_coarse_setup(m)) return rc;
+        if ((m->ncoarse + 3) / 4 > MAX_CW) {
+            amr_set_error("two-level preconditioner: too many aggregates for the coarse product");
+            return AMR_ERR_ARG;
+        }
+    }
+    Member M;
+    M.m = m;
+    M.o = *o;
+    M.b = b;
+    M.x = x;
+    M.state = M_JOIN;
+    M.mixed = o->precision == AMR_PREC_MIXED || o->precision == AMR_PREC_MIXED_F64ACC;
+    M.f32x = o->precision == AMR_PREC_MIXED;
+    if (int rc = pool_begin_host_writes(p)) return rc;
+    p->mem[slot] = M;
+    fill_desc(p, slot);
+    p->desc_dirty = true;
+    return AMR_OK;
+}
+
+// One step: initialise joining members, run check_every PCG iterations for
+// every member, then residual replacements and convergence decisions.
+// Members that finished (converged or failed) are appended to `out` and
+// leave the pool.
+int amr_pool_run(amr_pool *p, std::vector<amr_pool_result> &out)
+{
+    AMR_HIP(hipSetDevice(p->device));
+    std::vector<int> join, act;
+    for (int s = 0; s < p->cap; s++) {
+        if (p->mem[s].state == M_JOIN) join.push_back(s);
+        if (p->mem[s].state != M_FREE) act.push_back(s);
+    }
+    if (act.empty()) return AMR_OK;
+    const int check = p->check;
+    const double kEps = 2.220446049250313e-16, kFloorC = 64.0, kGrow = 1e4;
+    const int kForceEvery = 256;
+    hipStream_t st = p->stream;
+
+    // (a) lists for the initialisation and the iterations
+    std::vector<std::vector<int2>> ent;
+    std::vector<Group> gr;
+    if (int rc = pool_begin_host_writes(p)) return rc;
+    build_groups(p, act, false, ent, gr);
+    std::vector<int2> vj = vec_list(p, join), va = vec_list(p, act);
+    std::vector<int32_t> sj(join.begin(), join.end());
+    std::vector<int2> aj = agg_list(p, join), cj = cw_list(p, join), pj = vec_list(p, tl_only(p, join));
+    std::vector<int2> aa = agg_list(p, act), ca = cw_list(p, act);
+    size_t need = padded(vj) + padded(va) + padded(sj) + padded(aj) + padded(cj) + padded(pj) + padded(aa) +
+                  padded(ca);
+    for (const auto &e : ent) need += padded(e);
+    if (int rc = pool_reserve_lists(p, need)) return rc;
+    Lists Ls{p};
+    const size_t o_vj = Ls.push(vj), o_va = Ls.push(va), o_sj = Ls.push(sj);
+    const size_t o_aj = Ls.push(aj), o_cj = Ls.push(cj), o_pj = Ls.push(pj), o_aa = Ls.push(aa), o_ca = Ls.push(ca);
+    {
+        int gi = 0;
+        for (const auto &e : ent)
+            if (!e.empty()) gr[gi++].off = Ls.push(e);
+    }
+    if (int rc = upload(p, Ls.off, p->desc_dirty)) return rc;
+    if (!join.empty()) {
+        hipLaunchKernelGGL(pcg_init_kernel, dim3((unsigned)vj.size()), dim3(WGV), 0, st, p->dD,
+                           Ls.dev_at<int2>(o_vj));
+        launch_tl_fresh(p, Ls, o_aj, (unsigned)aj.size(), o_cj, (unsigned)cj.size(), o_pj, (unsigned)pj.size());
+        hipLaunchKernelGGL(pcg_finalize_kernel, dim3((unsigned)sj.size()), dim3(WGV), 0, st, p->dD,
+                           Ls.dev_at<int32_t>(o_sj), 0);
+    }
+    if (p->timing)
+        if (int rc = ensure_events(p)) return rc;
+    for (int j = 0; j < check; j++) {
+        if (p->timing) AMR_HIP(hipEventRecord(p->ev[2 * j], st));
+        for (const Group &g : gr) launch_pcg_spmv(g.kind, g.lanes, g.n, g.Rmax, st, p->dD, Ls.dev_at<int2>(g.off));
+        if (p->timing) AMR_HIP(hipEventRecord(p->ev[2 * j + 1], st));
+        hipLaunchKernelGGL(pcg_update_kernel, dim3((unsigned)va.size()), dim3(WGV), 0, st, p->dD,
+                           Ls.dev_at<int2>(o_va));
+        if (!aa.empty()) {
+            hipLaunchKernelGGL(tl_restrict_kernel, dim3((unsigned)aa.size()), dim3(64), 0, st, p->dD,
+                               Ls.dev_at<int2>(o_aa), 0);
+            hipLaunchKernelGGL(tl_coarse_kernel, dim3((unsigned)ca.size()), dim3(256), 0, st, p->dD,
+                               Ls.dev_at<int2>(o_ca), 0);
+        }
+        hipLaunchKernelGGL(pcg_dir_kernel, dim3((unsigned)va.size()), dim3(WGV), 0, st, p->dD,
+                           Ls.dev_at<int2>(o_va));
+    }
+    AMR_HIP(hipGetLastError());
+    const size_t sbytes = sizeof(double) * SEG_DBL * p->cap;
+    AMR_HIP(hipMemcpyAsync(p->hS, p->dS, sbytes, hipMemcpyDeviceToHost, st));
+    if (int rc = pool_sync(p)) return rc;
+
+    // (b) decisions; residual replacement for members that stopped or whose
+    // recursive residual dropped enough (mixed paths)
+    int real_max = 0;
+    std::vector<int> rep;
+    for (int s : act) {
+        Member &M = p->mem[s];
+        const double *hs = p->hS + (size_t)s * SEG_DBL;
+        const int32_t *hc = (const int32_t *)(hs + S_NSCAL);
+        const int prev = M.state == M_JOIN ? 0 : M.iters;
+        if (M.state == M_JOIN) {
+            M.state = M_ACTIVE;
+            M.bb = hs[S_BB];
+            M.rr_ref = M.bb;
+            if (M.bb == 0.0) {
+                M.converged = 1;
+                finish(p, s, out);
+                continue;
+            }
+        }
+        real_max = std::max(real_max, std::min(hc[C_ITERS] - prev, check));
+        M.iters = hc[C_ITERS];
+        M.broke = hc[C_BREAK];
+        M.done = hc[C_DONE] != 0 || hc[C_STOP] != 0;
+        const double rr = hs[S_RR];
+        const double ratio = M.o.replace_ratio > 0.0 ? M.o.replace_ratio : 0.1;
+        // mixed: replace when the recursive residual fell by `ratio`, and
+        // also when it rose far above the last true residual or no
+        // replacement happened for a while, so that a refinement that does
+        // not contract is seen (and abandoned) early
+        const bool mixed_due = M.mixed && (rr <= ratio * ratio * M.rr_ref || rr > kGrow * M.rr_ref ||
+                                           M.iters - M.last_rep >= kForceEvery);
+        if (!(M.done || mixed_due)) {
+            if (M.broke || M.iters >= M.o.maxiter) finish(p, s, out);
+            continue;
+        }
+        M.rr_rec = rr;
+        rep.push_back(s);
+    }
+    if (p->timing) {
+        // the SpMVs of iterations in which some member was active executed
+        for (int j = 0; j < real_max; j++) {
+            float ms = 0.f;
+            AMR_HIP(hipEventElapsedTime(&ms, p->ev[2 * j], p->ev[2 * j + 1]));
+            p->t_spmv_ms += ms;
+            p->n_spmv++;
+        }
+    }
+    if (rep.empty()) return AMR_OK;
+
+    // rt = b - K x in FP64 for the replacing members
+    build_groups(p, rep, true, ent, gr);
+    std::vector<int32_t> sr(rep.begin(), rep.end());
+    need = padded(sr);
+    for (const auto &e : ent) need += padded(e);
+    if (int rc = pool_reserve_lists(p, need)) return rc;
+    Ls.off = 0;
+    const size_t o_sr = Ls.push(sr);
+    {
+        int gi = 0;
+        for (const auto &e : ent)
+            if (!e.empty()) gr[gi++].off = Ls.push(e);
+    }
+    if (int rc = upload(p, Ls.off, false)) return rc;
+    for (const Group &g : gr) launch_residual(g.lanes, g.n, g.Rmax, st, p->dD, Ls.dev_at<int2>(g.off));
+    hipLaunchKernelGGL(pcg_finalize_kernel, dim3((unsigned)sr.size()), dim3(WGV), 0, st, p->dD,
+                       Ls.dev_at<int32_t>(o_sr), 1);
+    AMR_HIP(hipGetLastError());
+    AMR_HIP(hipMemcpyAsync(p->hS, p->dS, sbytes, hipMemcpyDeviceToHost, st));
+    if (int rc = pool_sync(p)) return rc;
+
+    std::vector<int> cont[2];  // resume as is / with a restarted search direction
+    bool desc_changed = false;
+    for (int s : rep) {
+        Member &M = p->mem[s];
+        const double *hs = p->hS + (size_t)s * SEG_DBL;
+        M.replacements++;
+        M.last_rep = M.iters;
+        const double rr = hs[S_RR];
+        M.relres = M.bb > 0.0 ? std::sqrt(rr / M.bb) : 0.0;
+        const double floor2 = kFloorC * kFloorC * kEps * kEps * hs[S_KX];
+        if (rr <= M.o.rtol * M.o.rtol * M.bb || rr <= floor2) {
+            M.converged = 1;
+            finish(p, s, out);
+            continue;
+        }
+        if (!std::isfinite(rr) || M.iters >= M.o.maxiter) {
+            finish(p, s, out);
+            continue;
+        }
+        bool restart = M.broke != 0;
+        const double drift = M.o.drift_limit > 0.0 ? M.o.drift_limit : 0.3;
+        if (M.mixed && (hs[S_GAP] > drift * drift * std::max(rr, M.rr_rec) || rr > M.rr_true_prev)) {
+            // the FP32 operator is too coarse for this model: the recursive
+            // residual drifted from the true one, or the refinement stopped
+            // contracting; finish on the FP64 operator
+            M.mixed = false;
+            M.fell_back = 1;
+            restart = true;
+        } else if ((M.done && !M.broke && rr > 0.25 * M.rr_true_prev) || (M.broke && ++M.breakdowns > 3)) {
+            finish(p, s, out);  // stagnation above both bounds, or repeated breakdown
+            continue;
+        }
+        SegDesc &d = p->hD[s];
+        const int f32 = M.mixed ? 1 : 0, wp32 = (M.mixed && M.f32x) ? 1 : 0;
+        if (d.f32 != f32 || d.wp32 != wp32) {
+            d.f32 = f32;
+            d.wp32 = wp32;
+            desc_changed = true;
+        }
+        M.broke = 0;
+        M.rr_true_prev = rr;
+        M.rr_ref = rr;
+        cont[restart ? 1 : 0].push_back(s);
+    }
+    if (cont[0].empty() && cont[1].empty()) return AMR_OK;
+    // resume: r = rt, z = M r (p = z on restart), rz, clear the stop flags
+    std::vector<int2> vc0 = vec_list(p, cont[0]), vc1 = vec_list(p, cont[1]);
+    std::vector<int32_t> sc(cont[0].begin(), cont[0].end());
+    sc.insert(sc.end(), cont[1].begin(), cont[1].end());
+    std::vector<int> call(sc.begin(), sc.end());
+    std::vector<int2> ac = agg_list(p, call), cc = cw_list(p, call), pc = vec_list(p, tl_only(p, cont[1]));
+    if (int rc = pool_reserve_lists(p, padded(vc0) + padded(vc1) + padded(sc) + padded(ac) + padded(cc) + padded(pc)))
+        return rc;
+    Ls.off = 0;
+    const size_t o_vc0 = Ls.push(vc0), o_vc1 = Ls.push(vc1), o_sc = Ls.push(sc);
+    const size_t o_ac = Ls.push(ac), o_cc = Ls.push(cc), o_pc = Ls.push(pc);
+    if (int rc = upload(p, Ls.off, desc_changed || p->desc_dirty)) return rc;
+    if (!vc0.empty())
+        hipLaunchKernelGGL(pcg_rez_kernel, dim3((unsigned)vc0.size()), dim3(WGV), 0, st, p->dD,
+                           Ls.dev_at<int2>(o_vc0), 0);
+    if (!vc1.empty())
+        hipLaunchKernelGGL(pcg_rez_kernel, dim3((unsigned)vc1.size()), dim3(WGV), 0, st, p->dD,
+                           Ls.dev_at<int2>(o_vc1), 1);
+    launch_tl_fresh(p, Ls, o_ac, (unsigned)ac.size(), o_cc, (unsigned)cc.size(), o_pc, (unsigned)pc.size());
+    hipLaunchKernelGGL(pcg_finalize_kernel, dim3((unsigned)sc.size()), dim3(WGV), 0, st, p->dD,
+                       Ls.dev_at<int32_t>(o_sc), 2);
+    AMR_HIP(hipGetLastError());
+    return AMR_OK;
+}
+
+int amr_pool_set_timing(amr_pool *p, int enable, double *ms, int64_t *n)
+{
+    if (enable && !p->timing) {
+        p->t_spmv_ms = 0.0;
+        p->n_spmv = 0;
+    }
+    p->timing = enable != 0;
+    if (ms) *ms = p->t_spmv_ms;
+    if (n) *n = p->n_spmv;
+    return AMR_OK;
+}
+
+// ------------------------------------------------- single-model wrappers
+
+namespace {
+
+int self_pool(amr_model *m, amr_pool **out)
+{
+    if (!m->self_pool) {
+        if (int rc = amr_pool_new(&m->self_pool, m->device, 1, 32, m->stream)) return rc;
+    }
+    m->self_pool->stream = m->stream;
+    *out = m->self_pool;
+    return AMR_OK;
+}
+
+// Heuristic lanes for small matrices, where timing the choices would cost
+// more than it saves: lanes about half the average blocks per row.
+int small_lanes(const amr_model *m)
+{
+    const double avg = (double)m->pat.nblk / (double)std::max<int64_t>(1, m->nnodes);
+    return avg <= 12.0 ? 8 : avg <= 40.0 ? 16 : 32;
+}
+
+constexpr int64_t kTuneMinBlocks = 1 << 20;
 
 }  // namespace
 
@@ -625,11 +1405,11 @@ int read_scalars(amr_model *m, double *hs, int32_t *hc)
 int amr_spmv_full(amr_model *m, const double *x, double *y, int mask_mode)
 {
     if (int rc = amr_autotune(m)) return rc;
-    const Launch L = pick_launch(m);
-    const size_t lds = tile_lds(L.R, 0);
+    const Geom g = model_geom(m);
+    const size_t lds = tile_lds(g.R, 0);
 #define AMR_SPMV(LN)                                                                                               \
-    hipLaunchKernelGGL((spmv_kernel<LN>), dim3(L.grid_rows), dim3(WGS), lds, m->stream, m->nnodes, L.R, m->rowptr, \
-                       m->col, m->Kv, m->pat.nblk, x, y, m->mask, mask_mode)
+    hipLaunchKernelGGL((spmv_kernel<LN>), dim3((unsigned)g.ntiles), dim3(WGS), lds, m->stream, m->nnodes, g.R,     \
+                       m->rowptr, m->col, m->Kv, x, y, m->mask, mask_mode)
     switch (m->lanes64) {
     case 8: AMR_SPMV(8); break;
     case 16: AMR_SPMV(16); break;
@@ -641,162 +1421,109 @@ int amr_spmv_full(amr_model *m, const double *x, double *y, int mask_mode)
     return AMR_OK;
 }
 
-// Pick the SpMV lanes-per-row for both precisions (once per model).
+// Pick the SpMV lanes-per-row for the three operand kinds (once per model):
+// time each choice on this model's matrix for large models, a heuristic for
+// small ones.  Every choice gives bit-identical results (see row_sum).
 int amr_autotune(amr_model *m)
 {
     if (m->lanes64 && m->lanes32 && m->lanes32d) return AMR_OK;
-    const Launch L = pick_launch(m);
-    int rc_ = AMR_OK;
-    int32_t *ctl = (int32_t *)(m->scal + S_NSCAL);
-    AMR_HIP(hipMemsetAsync(ctl, 0, sizeof(int32_t) * C_NCTL, m->stream));
+    if (m->pat.nblk < kTuneMinBlocks) {
+        const int l = small_lanes(m);
+        if (!m->lanes64) m->lanes64 = l;
+        if (!m->lanes32) m->lanes32 = l;
+        if (!m->lanes32d) m->lanes32d = l;
+        return AMR_OK;
+    }
+    amr_pool *p;
+    if (int rc = self_pool(m, &p)) return rc;
+    if (int rc = pool_begin_host_writes(p)) return rc;
+    // a descriptor whose vectors are scratch: p = 1, q, x, r written freely
+    Member &M = p->mem[0];
+    M = Member();
+    M.m = m;
+    M.o.maxiter = 1 << 30;
+    M.o.rtol = 1.0;
+    M.b = m->v[5];
+    M.x = m->v[6];
+    fill_desc(p, 0);
+    M = Member();  // not a solving member
+    const Geom g = model_geom(m);
+    std::vector<int2> tl;
+    for (int t = 0; t < g.ntiles; t++) tl.push_back(make_int2(0, t));
+    if (int rc = pool_reserve_lists(p, padded(tl))) return rc;
+    Lists Ls{p};
+    const size_t o = Ls.push(tl);
+    if (int rc = upload(p, Ls.off, true)) return rc;
+    p->desc_dirty = true;  // the next real member rewrites it
     const int64_t n = m->nnodes * 3;
-    hipLaunchKernelGGL(fill_kernel, dim3(L.grid_vec), dim3(WGV), 0, m->stream, n, m->v[3], 1.0);
-    hipLaunchKernelGGL(fill_f32_kernel, dim3(L.grid_vec), dim3(WGV), 0, m->stream, n, m->f[0], 1.0f);
+    const unsigned gv = (unsigned)g.nvblk;
+    AMR_HIP(hipMemsetAsync(p->dS, 0, sizeof(double) * SEG_DBL, p->stream));
+    hipLaunchKernelGGL(fill_kernel, dim3(gv), dim3(WGV), 0, p->stream, n, m->v[3], 1.0);
+    hipLaunchKernelGGL(fill_f32_kernel, dim3(gv), dim3(WGV), 0, p->stream, n, m->f[0], 1.0f);
     AMR_HIP(hipGetLastError());
-    int l64 = 64, l32 = 64, l32d = 64;
-    if (!m->lanes64 && (rc_ = tune_lanes<double, double>(m, L, m->Kv, m->v[3], m->v[3], m->v[4], &l64))) return rc_;
-    if (!m->lanes32 && (rc_ = tune_lanes<float, float>(m, L, m->Kv32, m->f[0], m->v[3], m->v[4], &l32))) return rc_;
-    if (!m->lanes32d &&
-        (rc_ = tune_lanes<float, double, double>(m, L, m->Kv32, m->v[3], m->v[3], m->v[4], &l32d)))
-        return rc_;
-    if (!m->lanes64) m->lanes64 = l64;
-    if (!m->lanes32) m->lanes32 = l32;
-    if (!m->lanes32d) m->lanes32d = l32d;
+    hipEvent_t e0, e1;
+    AMR_HIP(hipEventCreate(&e0));
+    AMR_HIP(hipEventCreate(&e1));
+    int *dst[K_NKIND] = {&m->lanes64, &m->lanes32, &m->lanes32d};
+    for (int kind = 0; kind < K_NKIND; kind++) {
+        if (*dst[kind]) continue;
+        float best = 1e30f;
+        int bestL = 64;
+        for (int L : LANES) {
+            launch_pcg_spmv(kind, L, (unsigned)g.ntiles, g.R, p->stream, p->dD, Ls.dev_at<int2>(o));  // warm
+            AMR_HIP(hipEventRecord(e0, p->stream));
+            for (int rep = 0; rep < 3; rep++)
+                launch_pcg_spmv(kind, L, (unsigned)g.ntiles, g.R, p->stream, p->dD, Ls.dev_at<int2>(o));
+            AMR_HIP(hipEventRecord(e1, p->stream));
+            AMR_HIP(hipEventSynchronize(e1));
+            float ms = 0.f;
+            AMR_HIP(hipEventElapsedTime(&ms, e0, e1));
+            if (ms < best) {
+                best = ms;
+                bestL = L;
+            }
+        }
+        *dst[kind] = bestL;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    p->pending = false;  // the event waits above drained the stream
     return AMR_OK;
 }
 
-// Solve K11 x = b (node order, b zero at prescribed dofs), x FP64.
-//
-// Residual replacement: when the recursive residual has dropped by 10x
-// since the last recomputation (mixed path) or met the tolerance (both
-// paths), r = b - K x is recomputed in FP64.  The drift between the two
-// residuals measures how well the FP32 operator represents K for this
-// model: if it exceeds 0.3 ||r||, the rest of the solve runs with the FP64
-// operator (restarting the search direction), so an ill-conditioned model
-// still converges to the FP64 solution.
+// Solve K11 x = b (node order, b zero at prescribed dofs), x FP64: a pool of
+// one, stepped until the model leaves it.
 int amr_pcg_solve(amr_model *m, const double *b, double *x, const amr_solve_opts *o, amr_solve_info *info)
 {
-    const int64_t n = m->nnodes * 3;
-    const Launch L = pick_launch(m);
-    // mixed: FP32 values; f32x: FP32 search direction and FP32 accumulation
-    // (AMR_PREC_MIXED), otherwise FP64 gather and accumulation of the FP32
-    // values (AMR_PREC_MIXED_F64ACC)
-    bool mixed = o->precision == AMR_PREC_MIXED || o->precision == AMR_PREC_MIXED_F64ACC;
-    const bool f32x = o->precision == AMR_PREC_MIXED;
-    const int check = std::max(1, std::min(o->check_every > 0 ? o->check_every : 32, 256));
-    const double kEps = 2.220446049250313e-16, kFloorC = 64.0;
-    // residual drop that triggers a recomputation (mixed path), and the
-    // drift/||r|| above which the FP32 operator is abandoned
-    const double replace_ratio = o->replace_ratio > 0.0 ? o->replace_ratio : 0.1;
-    const double drift_limit = o->drift_limit > 0.0 ? o->drift_limit : 0.3;
+    if (amr_pool_is_member(m)) {
+        amr_set_error("the model is solving in a pool; wait for it to leave before solving it alone");
+        return AMR_ERR_ARG;
+    }
+    amr_pool *p;
     if (int rc = amr_autotune(m)) return rc;
-    int32_t *ctl = (int32_t *)(m->scal + S_NSCAL);
-    double *r = m->v[1], *z = m->v[2], *p = m->v[3], *q = m->v[4], *rt = m->v[5];
-    float *p32 = m->f[0];
-    double hs[S_NSCAL];
-    int32_t hc[C_NCTL];
-
-    if (m->timing && (int)m->ev.size() < 2 * check) {
-        for (hipEvent_t e : m->ev) (void)hipEventDestroy(e);
-        m->ev.assign(2 * check, nullptr);
-        for (hipEvent_t &e : m->ev) AMR_HIP(hipEventCreate(&e));
+    if (int rc = self_pool(m, &p)) return rc;
+    p->check = std::max(1, std::min(o->check_every > 0 ? o->check_every : 32, 256));
+    p->timing = m->timing;
+    p->t_spmv_ms = 0.0;
+    p->n_spmv = 0;
+    if (int rc = amr_pool_join(p, m, b, x, o)) return rc;
+    std::vector<amr_pool_result> res;
+    while (res.empty()) {
+        if (int rc = amr_pool_run(p, res)) {
+            p->mem[0] = Member();
+            return rc;
+        }
     }
-    const double tr = 0.5 * o->rtol;
-    Prec P{m->dinv, m->dinv32, m->binv, m->binv32, mixed ? 1 : 0, o->precond == AMR_PRECOND_BLOCK_JACOBI ? 1 : 0};
-    hipLaunchKernelGGL(pcg_init_kernel, dim3(L.grid_vec), dim3(WGV), 0, m->stream, m->nnodes, b, x, r, z, p,
-                       mixed && f32x ? p32 : nullptr, P, m->part);
-    hipLaunchKernelGGL(pcg_finalize_kernel, dim3(1), dim3(WGV), 0, m->stream, 0, (int)L.grid_vec, 0, 1, tr * tr,
-                       m->part, ctl, m->scal);
-    AMR_HIP(hipGetLastError());
-    if (int rc = read_scalars(m, hs, hc)) return rc;
-    const double bb = hs[S_BB];
-    double rr_ref = hs[S_RR], rr_true_prev = INFINITY, relres = 0.0;
-    int converged = bb == 0.0, broke = 0, replacements = 0, iters = 0, fell_back = 0, breakdowns = 0;
-    int k = 0;
-    while (!converged) {
-        for (int j = 0; j < check; j++, k++) {
-            const int pi = k & 1;
-            if (m->timing) AMR_HIP(hipEventRecord(m->ev[2 * j], m->stream));
-            if (mixed && f32x) launch_pcg_spmv<float, float>(m, L, m->Kv32, p32, p, q);
-            else if (mixed) launch_pcg_spmv<float, double, double>(m, L, m->Kv32, p, p, q);
-            else launch_pcg_spmv<double, double>(m, L, m->Kv, p, p, q);
-            if (m->timing) AMR_HIP(hipEventRecord(m->ev[2 * j + 1], m->stream));
-            hipLaunchKernelGGL(pcg_update_kernel, dim3(L.grid_vec), dim3(WGV), 0, m->stream, m->nnodes, x, r, z, p, q,
-                               P, m->part, (int)L.grid_rows, (int)L.grid_vec, pi, ctl);
-            hipLaunchKernelGGL(pcg_dir_kernel, dim3(L.grid_vec), dim3(WGV), 0, m->stream, n, p,
-                               mixed && f32x ? p32 : nullptr, z, m->part, (int)L.grid_vec, pi, k + 1, o->maxiter, ctl,
-                               m->scal);
-        }
-        AMR_HIP(hipGetLastError());
-        if (int rc = read_scalars(m, hs, hc)) return rc;
-        if (m->timing) {
-            // the SpMVs of the iterations run in this batch executed in full;
-            // later launches returned at once on the done flag
-            const int real = std::min(hc[C_ITERS] - iters, check);
-            for (int j = 0; j < real; j++) {
-                float ms = 0.f;
-                AMR_HIP(hipEventElapsedTime(&ms, m->ev[2 * j], m->ev[2 * j + 1]));
-                m->t_spmv_ms += ms;
-                m->n_spmv++;
-            }
-        }
-        iters = hc[C_ITERS];
-        broke = hc[C_BREAK];
-        k = iters;
-        const bool done = hc[C_DONE] != 0 || hc[C_STOP] != 0;
-        if (!(done || (mixed && hs[S_RR] <= replace_ratio * replace_ratio * rr_ref))) {
-            if (broke || iters >= o->maxiter) break;
-            continue;
-        }
-        // residual replacement: rt = b - K x in FP64, drift ||rt - r||
-        const double rr_rec = hs[S_RR];
-        launch_residual(m, L, x, b, rt, r);
-        hipLaunchKernelGGL(pcg_finalize_kernel, dim3(1), dim3(WGV), 0, m->stream, 1, (int)L.grid_vec,
-                           (int)L.grid_rows, 0, 0.0, m->part, ctl, m->scal);
-        AMR_HIP(hipGetLastError());
-        if (int rc = read_scalars(m, hs, hc)) return rc;
-        replacements++;
-        const double rr = hs[S_RR];
-        relres = bb > 0.0 ? std::sqrt(rr / bb) : 0.0;
-        const double floor2 = kFloorC * kFloorC * kEps * kEps * hs[S_KX];
-        if (rr <= o->rtol * o->rtol * bb || rr <= floor2) {
-            converged = 1;
-            break;
-        }
-        if (!std::isfinite(rr) || iters >= o->maxiter) break;
-        if (done && !broke && rr > 0.25 * rr_true_prev) break;  // stagnation above both bounds
-        if (broke && ++breakdowns > 3) break;
-        bool restart = broke != 0;
-        if (mixed && hs[S_GAP] > drift_limit * drift_limit * std::max(rr, rr_rec)) {
-            mixed = false;  // the FP32 operator is too coarse for this model
-            fell_back = 1;
-            restart = true;
-        }
-        const int slot = 1 - (iters & 1);
-        P.f32 = mixed ? 1 : 0;
-        hipLaunchKernelGGL(pcg_rez_kernel, dim3(L.grid_vec), dim3(WGV), 0, m->stream, m->nnodes, rt, r, z, p,
-                           mixed && f32x ? p32 : nullptr, restart ? 1 : 0, P, m->part, slot);
-        hipLaunchKernelGGL(pcg_finalize_kernel, dim3(1), dim3(WGV), 0, m->stream, 2, (int)L.grid_vec,
-                           (int)L.grid_rows, slot, 0.0, m->part, ctl, m->scal);
-        AMR_HIP(hipGetLastError());
-        if (broke) {
-            hipLaunchKernelGGL(clear_break_kernel, dim3(1), dim3(1), 0, m->stream, ctl);
-            broke = 0;
-        }
-        rr_true_prev = rr;
-        rr_ref = rr;
+    if (m->timing) {
+        m->t_spmv_ms += p->t_spmv_ms;
+        m->n_spmv += p->n_spmv;
     }
-    if (info) {
-        info->iterations = iters;
-        info->refinements = replacements;
-        info->converged = converged;
-        info->fallback = fell_back;
-        info->relres = relres;
-    }
-    if (!converged) {
+    const amr_solve_info &r = res[0].info;
+    if (info) *info = r;
+    if (!r.converged) {
         char buf[256];
-        snprintf(buf, sizeof buf, "PCG did not reach rtol=%.3e (relres %.3e after %d iterations, %d replacements%s)",
-                 o->rtol, relres, iters, replacements, broke ? ", breakdown" : "");
+        snprintf(buf, sizeof buf, "PCG did not reach rtol=%.3e (relres %.3e after %d iterations, %d replacements)",
+                 o->rtol, r.relres, r.iterations, r.refinements);
         amr_set_error(buf);
         return AMR_ERR_SOLVE;
     }
@@ -904,6 +1631,14 @@ __global__ void commit_kernel(int64_t n, double acr, const double *__restrict__ 
         F[i] += dFin[i];
         Rc[i] = (1.0 - acr) * Rc[i] + R[i];
     }
+}
+
+int read_scalars(amr_model *m, double *hs, int32_t *hc)
+{
+    AMR_HIP(hipMemcpyAsync(hs, m->scal, sizeof(double) * S_NSCAL, hipMemcpyDeviceToHost, m->stream));
+    AMR_HIP(hipMemcpyAsync(hc, m->scal + S_NSCAL, sizeof(int32_t) * C_NCTL, hipMemcpyDeviceToHost, m->stream));
+    AMR_HIP(hipStreamSynchronize(m->stream));
+    return AMR_OK;
 }
 
 unsigned vec_grid(const amr_model *m)

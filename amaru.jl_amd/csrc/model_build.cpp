@@ -14,6 +14,7 @@
 // each list in order reproduces the reference's element-order sums without
 // atomics.
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 
 #include "amr_internal.h"
@@ -98,5 +99,103 @@ int amr_build_pattern(int nn, int64_t nnodes, int64_t ne, const int32_t *conn, A
     P.csrc.resize(P.ncontrib);
     std::vector<int64_t> pos(P.cptr.begin(), P.cptr.end() - 1);
     for (int64_t i = 0; i < (int64_t)slot.size(); i++) P.csrc[pos[slot[i]]++] = (int32_t)i;
+    return AMR_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Coarse space of the two-level preconditioner: aggregates of nodes by
+// recursive coordinate bisection (split the longest extent at the median,
+// stable on node id, until every part has at most `target` nodes), and the
+// index lists that let the device form Kc = Phi^T K Phi deterministically:
+//   pairs:  for each block row i, the distinct aggregates b of its columns
+//           (ascending), one output slot per (i, b);
+//   cblocks: for each coarse block (a, b), the slots of rows i in a with
+//           aggregate b, ascending i.
+int amr_build_coarse(int64_t nnodes, const double *X, const int32_t *rowptr, const int32_t *col, int target,
+                     AmrCoarse &C)
+{
+    if (target < 1) target = 1;
+    C = AmrCoarse();
+    C.agg.assign(nnodes, 0);
+    std::vector<int32_t> idx(nnodes);
+    for (int64_t i = 0; i < nnodes; i++) idx[i] = (int32_t)i;
+    // explicit stack of [lo, hi) ranges of idx; parts are numbered in the
+    // order they are finished (depth first, lower half first)
+    std::vector<std::pair<int64_t, int64_t>> st{{0, nnodes}};
+    int32_t nagg = 0;
+    while (!st.empty()) {
+        auto [lo, hi] = st.back();
+        st.pop_back();
+        if (hi - lo <= target) {
+            for (int64_t k = lo; k < hi; k++) C.agg[idx[k]] = nagg;
+            nagg++;
+            continue;
+        }
+        double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+        for (int64_t k = lo; k < hi; k++)
+            for (int d = 0; d < 3; d++) {
+                mn[d] = std::min(mn[d], X[idx[k] * 3 + d]);
+                mx[d] = std::max(mx[d], X[idx[k] * 3 + d]);
+            }
+        int ax = 0;
+        for (int d = 1; d < 3; d++)
+            if (mx[d] - mn[d] > mx[ax] - mn[ax]) ax = d;
+        std::stable_sort(idx.begin() + lo, idx.begin() + hi,
+                         [&](int32_t a, int32_t b) { return X[a * 3 + ax] < X[b * 3 + ax]; });
+        const int64_t mid = lo + (hi - lo) / 2;
+        st.push_back({mid, hi});
+        st.push_back({lo, mid});
+    }
+    C.nagg = nagg;
+    // nodes of each aggregate, ascending; centroids
+    C.aggptr.assign(nagg + 1, 0);
+    for (int64_t i = 0; i < nnodes; i++) C.aggptr[C.agg[i] + 1]++;
+    for (int a = 0; a < nagg; a++) C.aggptr[a + 1] += C.aggptr[a];
+    C.aggnodes.assign(nnodes, 0);
+    {
+        std::vector<int32_t> fill(C.aggptr.begin(), C.aggptr.end() - 1);
+        for (int64_t i = 0; i < nnodes; i++) C.aggnodes[fill[C.agg[i]]++] = (int32_t)i;
+    }
+    C.cent.assign((size_t)nagg * 3, 0.0);
+    for (int a = 0; a < nagg; a++) {
+        double s[3] = {0.0, 0.0, 0.0};
+        for (int32_t k = C.aggptr[a]; k < C.aggptr[a + 1]; k++)
+            for (int d = 0; d < 3; d++) s[d] += X[(int64_t)C.aggnodes[k] * 3 + d];
+        const double w = 1.0 / (double)(C.aggptr[a + 1] - C.aggptr[a]);
+        for (int d = 0; d < 3; d++) C.cent[a * 3 + d] = s[d] * w;
+    }
+    // (row, aggregate) pairs
+    C.pptr.assign(nnodes + 1, 0);
+    C.prow.clear();
+    std::vector<int32_t> seen;
+    for (int64_t i = 0; i < nnodes; i++) {
+        seen.clear();
+        for (int32_t k = rowptr[i]; k < rowptr[i + 1]; k++) seen.push_back(C.agg[col[k]]);
+        std::sort(seen.begin(), seen.end());
+        seen.erase(std::unique(seen.begin(), seen.end()), seen.end());
+        for (int32_t b : seen) {
+            C.pagg.push_back(b);
+            C.prow.push_back((int32_t)i);
+        }
+        C.pptr[i + 1] = (int32_t)C.pagg.size();
+    }
+    // coarse blocks (a, b) ascending, slots ascending in row
+    const int64_t np = (int64_t)C.pagg.size();
+    std::vector<int64_t> key(np);
+    for (int64_t i = 0; i < nnodes; i++)
+        for (int32_t s = C.pptr[i]; s < C.pptr[i + 1]; s++) key[s] = (int64_t)C.agg[i] * nagg + C.pagg[s];
+    std::vector<int32_t> ord(np);
+    for (int64_t s = 0; s < np; s++) ord[s] = (int32_t)s;
+    std::stable_sort(ord.begin(), ord.end(), [&](int32_t x, int32_t y) { return key[x] < key[y]; });
+    C.cbptr.push_back(0);
+    for (int64_t t = 0; t < np; t++) {
+        if (t > 0 && key[ord[t]] != key[ord[t - 1]]) C.cbptr.push_back((int32_t)t);
+        if (t == 0 || key[ord[t]] != key[ord[t - 1]]) {
+            C.cba.push_back((int32_t)(key[ord[t]] / nagg));
+            C.cbb.push_back((int32_t)(key[ord[t]] % nagg));
+        }
+        C.cbslot.push_back(ord[t]);
+    }
+    C.cbptr.push_back((int32_t)np);
     return AMR_OK;
 }

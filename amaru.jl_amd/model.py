@@ -370,6 +370,8 @@ class DeviceModel:
         _lib.check(L.amr_model_create(ctypes.byref(self._h), device, model.etype, model.nips, model.nnodes,
                                       _lib.ptr(X), model.ne, _lib.ptr(C), len(model.mats), _lib.ptr(K),
                                       _lib.ptr(Pm), _lib.ptr(E), _lib.ptr(Q), model.ndofs, model.nu))
+        self.device = device
+        self._eq = Q.copy()
         if model.state.any():
             self.state_set(model.state)
 
@@ -394,9 +396,13 @@ class DeviceModel:
         return dict(zip(["nnodes", "ne", "nips", "nblk", "nnz_max", "ncontrib", "ndofs", "nu"], out.tolist()))
 
     def set_dofs(self):
+        """Upload the model's DOF numbering if it changed since the last upload."""
         m = self.model
         eq = np.ascontiguousarray(m.eq.ravel(), dtype=np.int32)
+        if np.array_equal(eq, self._eq):
+            return
         _lib.check(_lib.lib().amr_model_set_dofs(self._h, _lib.ptr(eq), m.ndofs, m.nu))
+        self._eq = eq
 
     def state_set(self, st):
         st = np.ascontiguousarray(st, dtype=float)
@@ -453,7 +459,8 @@ class DeviceModel:
     def opts(precision="fp64", rtol=1e-10, maxiter=200000, check_every=32, replace_ratio=0.0, drift_limit=0.0,
              precond="jacobi"):
         prec = {"fp64": _lib.AMR_PREC_FP64, "mixed": _lib.AMR_PREC_MIXED, "mixed64acc": _lib.AMR_PREC_MIXED_F64ACC}
-        pc = {"jacobi": _lib.AMR_PRECOND_JACOBI, "block_jacobi": _lib.AMR_PRECOND_BLOCK_JACOBI}
+        pc = {"jacobi": _lib.AMR_PRECOND_JACOBI, "block_jacobi": _lib.AMR_PRECOND_BLOCK_JACOBI,
+              "two_level": _lib.AMR_PRECOND_TWO_LEVEL}
         return _lib.SolveOpts(prec[precision], pc[precond], int(maxiter), int(check_every), float(rtol), 0.0,
                               float(replace_ratio), float(drift_limit))
 
@@ -471,6 +478,22 @@ class DeviceModel:
         import ctypes
         info = _lib.NRInfo()
         rc = _lib.lib().amr_nr_iteration(self._h, int(it), ctypes.byref(opts), ctypes.byref(info))
+        return self._nr_result(rc, info)
+
+    def nr_prepare(self, it):
+        """First half of nr_iteration: mount_K and the solve's right-hand side, queued without waiting."""
+        _lib.check(_lib.lib().amr_nr_prepare(self._h, int(it)))
+
+    def nr_finish(self, solve_info):
+        """Second half of nr_iteration, given the linear solve's outcome (a SolveInfo, or None
+        for a model without unknowns); returns what nr_iteration returns."""
+        import ctypes
+        info = _lib.NRInfo()
+        si = ctypes.byref(solve_info) if solve_info is not None else None
+        rc = _lib.lib().amr_nr_finish(self._h, si, ctypes.byref(info))
+        return self._nr_result(rc, info)
+
+    def _nr_result(self, rc, info):
         out = {"res": info.res, "err": info.err, "relres": info.relres, "pcg_iterations": info.pcg_iterations,
                "replacements": info.replacements, "fallback": bool(info.fallback), "status": rc, "message": None}
         if rc != _lib.AMR_OK:
@@ -478,6 +501,28 @@ class DeviceModel:
             if rc in (-1, -2, -7):   # argument, runtime or memory errors are not solver failures
                 raise _lib.AmaruError(rc, out["message"])
         return out
+
+    def set_coarse(self, target_nodes=256, max_aggregates=170):
+        _lib.check(_lib.lib().amr_set_coarse(self._h, int(target_nodes), int(max_aggregates)))
+
+    def coarse_info(self):
+        out = np.zeros(2, dtype=np.int64)
+        _lib.check(_lib.lib().amr_get_coarse(self._h, _lib.ptr(out)))
+        return {"aggregates": int(out[0]), "coarse_dofs": int(out[1])}
+
+    def coarse_space(self):
+        """(aggregate of each node, Kc^-1) of the two-level preconditioner."""
+        info = self.coarse_info()
+        agg = np.zeros(self.model.nnodes, dtype=np.int32)
+        kinv = np.zeros((info["coarse_dofs"], info["coarse_dofs"]))
+        _lib.check(_lib.lib().amr_get_coarse_space(self._h, _lib.ptr(agg), _lib.ptr(kinv)))
+        return agg, kinv
+
+    def device_bytes(self):
+        import ctypes
+        b = ctypes.c_int64(0)
+        _lib.check(_lib.lib().amr_model_device_bytes(self._h, ctypes.byref(b)))
+        return b.value
 
     def increment_commit(self):
         _lib.check(_lib.lib().amr_increment_commit(self._h))

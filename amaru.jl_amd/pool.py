@@ -1,0 +1,323 @@
+"""Batched and served analyses: many models' linear solves advanced together.
+
+A ``Pool`` (``amr_pool_*`` in the C ABI) runs the PCG of every member model
+with one launch per PCG phase for all of them.  ``run_analyses`` runs a set
+of ``MechAnalysis`` objects to completion with their linear solves pooled
+(batching); ``Server`` admits requests as they arrive, by free device
+memory, and keeps each model resident across its load increments while
+models join and leave the pool between steps (continuous batching).
+
+The per-model control flow is the reference's ``mech_stage_solver!``
+(src/mech/mech-solver.jl:186-492), unchanged: ``analysis_machine`` yields
+at each Newton-Raphson iteration and is resumed with the iteration's result.
+A model's PCG iterates do not depend on what else is in the pool, so every
+result is bit-identical to solving that model alone with ``solve``.
+"""
+import collections
+import concurrent.futures
+import ctypes
+import threading
+import time
+
+import numpy as np
+
+from . import _lib
+from .analysis import analysis_machine, solver_args
+from .model import DeviceModel
+
+
+class Pool:
+    """amr_pool: up to ``max_members`` models solving at once on one device."""
+
+    def __init__(self, device=0, max_members=256, check_every=32):
+        self._h = ctypes.c_void_p()
+        _lib.check(_lib.lib().amr_pool_create(ctypes.byref(self._h), int(device), int(max_members),
+                                              int(check_every)))
+        self.device = device
+        self.cap = int(max_members)
+        self._out_m = (ctypes.c_void_p * self.cap)()
+        self._out_i = (_lib.SolveInfo * self.cap)()
+        self._attached = set()
+
+    def close(self):
+        if self._h:
+            _lib.lib().amr_pool_destroy(self._h)
+            self._h = None
+            self._attached.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def attach(self, dev):
+        if dev.handle.value not in self._attached:
+            _lib.check(_lib.lib().amr_pool_attach(self._h, dev.handle))
+            self._attached.add(dev.handle.value)
+
+    def detach(self, dev):
+        if dev.handle and dev.handle.value in self._attached:
+            _lib.check(_lib.lib().amr_pool_detach(dev.handle))
+            self._attached.discard(dev.handle.value)
+
+    def forget(self, dev):
+        """The model is about to be destroyed (which detaches it)."""
+        if dev.handle:
+            self._attached.discard(dev.handle.value)
+
+    def add(self, dev, opts):
+        _lib.check(_lib.lib().amr_pool_add(self._h, dev.handle, ctypes.byref(opts)))
+
+    def size(self):
+        n = ctypes.c_int(0)
+        _lib.check(_lib.lib().amr_pool_size(self._h, ctypes.byref(n)))
+        return n.value
+
+    def step(self):
+        """One step; returns [(model handle value, SolveInfo)] of members that left."""
+        n = ctypes.c_int(0)
+        _lib.check(_lib.lib().amr_pool_step(self._h, self.cap, ctypes.byref(n), self._out_m, self._out_i))
+        out = []
+        for i in range(n.value):
+            out.append((self._out_m[i], _lib.SolveInfo.from_buffer_copy(self._out_i[i])))
+        return out
+
+    def timing(self, enable):
+        ms = ctypes.c_double(0.0)
+        n = ctypes.c_int64(0)
+        _lib.check(_lib.lib().amr_pool_timing(self._h, 1 if enable else 0, ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value
+
+
+class Job:
+    """One analysis driven through a pool: its machine is advanced until it
+    needs a pooled linear solve, then parked until the pool returns it."""
+
+    def __init__(self, ana, args, pool):
+        self.ana = ana
+        self.pool = pool
+        self.gen = analysis_machine(ana, args)
+        self.result = None
+        self.done = False
+
+    def start(self):
+        return self._advance(None, first=True)
+
+    def resume(self, solve_info):
+        return self._advance(self.ana.dev.nr_finish(solve_info))
+
+    def _advance(self, r, first=False):
+        try:
+            it = next(self.gen) if first else self.gen.send(r)
+            while True:
+                dev = self.ana.dev
+                self.pool.attach(dev)
+                dev.nr_prepare(it)
+                if dev.model.nu == 0:
+                    it = self.gen.send(dev.nr_finish(None))
+                    continue
+                self.pool.add(dev, self.ana.opts)
+                return False
+        except StopIteration as stop:
+            self.result = stop.value
+            self.done = True
+            return True
+
+
+def run_analyses(analyses, pool=None, device=0, check_every=32, **kwargs):
+    """solve!(ana; kwargs...) for every analysis, their linear solves pooled.
+    Returns the list of (success, message)."""
+    args = solver_args(device=device, **kwargs)
+    own = pool is None
+    if own:
+        pool = Pool(device, max_members=max(1, len(analyses)), check_every=check_every)
+    try:
+        jobs = {}
+        results = [None] * len(analyses)
+        for i, ana in enumerate(analyses):
+            job = Job(ana, args, pool)
+            if job.start():
+                results[i] = job.result
+            else:
+                jobs[ana.dev.handle.value] = (i, job)
+        while jobs:
+            for h, si in pool.step():
+                i, job = jobs.pop(h)
+                if job.resume(si):
+                    results[i] = job.result
+                else:
+                    jobs[job.ana.dev.handle.value] = (i, job)
+        return results
+    finally:
+        if own:
+            for ana in analyses:
+                if ana.dev is not None:
+                    pool.detach(ana.dev)
+            pool.close()
+
+
+# ------------------------------------------------------------------ serving
+
+def estimate_device_bytes(model):
+    """Upper estimate of the device memory amr_model_create and the stage
+    driver allocate for ``model`` (admission before the pattern is built):
+    node-block count bounded by the element type's neighbour count."""
+    nn = model.nen
+    nnodes, ne, nips = model.nnodes, model.ne, model.nips
+    maxnb = {8: 27, 20: 81, 10: 65}[nn]
+    nblk = min(nnodes * maxnb, ne * nn * nn)
+    per_node = 888
+    per_elem = 4 * nn + 4 + 72 * nn * nn + 24 * nn + 4 + 4 * nn + 224 * nips + 4 * nn * nn
+    per_blk = 4 + 8 + 72 + 36 + 9
+    return int(per_node * nnodes + per_elem * ne + per_blk * (nblk + 64) + 24576 * 8 + 4096)
+
+
+class Request:
+    """A served request: ``build()`` returns (MechAnalysis, solver kwargs)
+    on the host; the analysis runs on the device when admitted."""
+
+    __slots__ = ("rid", "t_arrival", "build", "ana", "kwargs", "reserve", "bytes", "t_admit", "t_start", "t_done",
+                 "ok", "msg", "dof_solves", "job")
+
+    def __init__(self, rid, t_arrival, build):
+        self.rid = rid
+        self.t_arrival = float(t_arrival)
+        self.build = build
+        self.ana = self.kwargs = self.job = None
+        self.reserve = self.bytes = 0
+        self.t_admit = self.t_start = self.t_done = None
+        self.ok = None
+        self.msg = None
+        self.dof_solves = 0
+
+    @property
+    def latency(self):
+        return None if self.t_done is None else self.t_done - self.t_arrival
+
+
+class Server:
+    """Continuous-batching FE solve server on one device.
+
+    Requests arrive at their ``t_arrival`` (seconds after ``serve`` starts).
+    Host-side model construction and device upload run on worker threads; a
+    request is admitted when its estimated device memory fits the HBM budget
+    and the pool has a free member slot; its model stays resident until its
+    analysis ends; its linear solves join the shared pool between steps.
+    """
+
+    def __init__(self, device=0, max_members=256, check_every=32, hbm_budget=None, workers=4):
+        self.device = device
+        self.pool = Pool(device, max_members=max_members, check_every=check_every)
+        if hbm_budget is None:
+            free = ctypes.c_int64(0)
+            total = ctypes.c_int64(0)
+            _lib.check(_lib.lib().amr_mem_info(int(device), ctypes.byref(free), ctypes.byref(total)))
+            hbm_budget = int(0.8 * free.value)
+        self.budget = int(hbm_budget)
+        self.workers = int(workers)
+        self.in_use = 0
+        self.peak_in_use = 0
+        self.steps = 0
+
+    def close(self):
+        self.pool.close()
+
+    def _finish(self, req, now):
+        req.t_done = now
+        req.ok, req.msg = req.job.result
+        req.dof_solves = req.ana.stats["dof_solves"]
+        dev = req.ana.dev
+        self.pool.forget(dev)
+        dev.close()
+        self.in_use -= req.bytes
+
+    def serve(self, requests, keep_models=False):
+        """Runs every request; returns them with their timings filled in."""
+        reqs = sorted(requests, key=lambda r: r.t_arrival)
+        N = len(reqs)
+        ex = concurrent.futures.ThreadPoolExecutor(max_workers=self.workers)
+        lock = threading.Lock()
+        t0 = time.perf_counter()
+        clock = lambda: time.perf_counter() - t0  # noqa: E731
+
+        def host_build(req):
+            ana, kw = req.build()
+            ana.model.configure_dofs(ana.stages[0].bcs)
+            return req, ana, kw, estimate_device_bytes(ana.model)
+
+        def device_build(req):
+            req.ana.dev = DeviceModel(req.ana.model, device=self.device)
+            return req
+
+        nxt = 0
+        building, creating = set(), set()
+        admit_q = collections.deque()
+        running = {}
+        ndone = 0
+        try:
+            while ndone < N:
+                now = clock()
+                while nxt < N and reqs[nxt].t_arrival <= now:
+                    building.add(ex.submit(host_build, reqs[nxt]))
+                    nxt += 1
+                for f in [f for f in building if f.done()]:
+                    building.discard(f)
+                    req, ana, kw, est = f.result()
+                    req.ana, req.kwargs, req.reserve = ana, kw, est
+                    admit_q.append(req)
+                # admission: FIFO by free HBM budget and member slots
+                while admit_q:
+                    req = admit_q[0]
+                    slots = self.pool.cap - len(running) - len(creating)
+                    fits = self.in_use + req.reserve <= self.budget or (not running and not creating)
+                    if slots <= 0 or not fits:
+                        break
+                    admit_q.popleft()
+                    with lock:
+                        self.in_use += req.reserve
+                        self.peak_in_use = max(self.peak_in_use, self.in_use)
+                    req.t_admit = clock()
+                    creating.add(ex.submit(device_build, req))
+                for f in [f for f in creating if f.done()]:
+                    creating.discard(f)
+                    req = f.result()
+                    req.bytes = req.ana.dev.device_bytes()
+                    self.in_use += req.bytes - req.reserve
+                    self.peak_in_use = max(self.peak_in_use, self.in_use)
+                    req.t_start = clock()
+                    args = solver_args(device=self.device, **req.kwargs)
+                    req.job = Job(req.ana, args, self.pool)
+                    if req.job.start():
+                        self._finish(req, clock())
+                        ndone += 1
+                    else:
+                        running[req.ana.dev.handle.value] = req
+                if self.pool.size() > 0:
+                    self.steps += 1
+                    for h, si in self.pool.step():
+                        req = running.pop(h)
+                        if req.job.resume(si):
+                            self._finish(req, clock())
+                            ndone += 1
+                        else:
+                            running[req.ana.dev.handle.value] = req
+                else:
+                    pend = building | creating
+                    wait = (reqs[nxt].t_arrival - clock()) if nxt < N else None
+                    if pend:
+                        concurrent.futures.wait(pend, timeout=wait if wait is not None and wait > 0 else None,
+                                                return_when=concurrent.futures.FIRST_COMPLETED)
+                    elif wait is not None and wait > 0:
+                        time.sleep(wait)
+        finally:
+            ex.shutdown(wait=True)
+        return reqs
+
+
+def latency_stats(reqs):
+    lat = np.array([r.latency for r in reqs if r.latency is not None])
+    if lat.size == 0:
+        return {}
+    return {"p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
+            "mean_s": float(lat.mean()), "max_s": float(lat.max())}

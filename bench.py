@@ -10,6 +10,19 @@ value = unknown DOFs solved per second (DOF-solves/s).  The plain FP64
 path (FP64 Jacobi-PCG, same tolerance) is timed beside it on the same
 model, and the oracle (CPU restatement: mount_K rule + SuperLU) on a
 bounded sample of the same problem family.
+
+Two more legs are reported beside the headline:
+* "batched": independent Tet10 cantilevers (seeded; element counts
+  log-uniform 500-20000), each one linear static analysis; the plain FP64
+  path solves them one after another, the batched path solves all of them
+  with their PCGs pooled (one launch per PCG phase for every model) in
+  mixed precision.  Models are resident on the device before timing.
+* "served": a seeded request stream (Hex8/Tet10 boxes of 1e3-3e5 DOFs, 70 %
+  elastic, 30 % von Mises in 5 increments) with Poisson arrivals; the plain
+  path serves one request at a time (its throughput fixes the arrival
+  rate, a multiple of it); the server admits requests by HBM budget and
+  batches their solves continuously.  Latency = arrival to result on host,
+  host-side model construction included.
 """
 import argparse
 import json
@@ -89,6 +102,162 @@ def cpu_baseline(n_cpu, threads):
     return dict(nu=nu, ndofs=ndofs, t_assemble=t1 - t0, t_solve=t2 - t1, t_update=t3 - t2, total=t3 - t0)
 
 
+def dof_solves(anas):
+    return sum(a.stats["dof_solves"] for a in anas)
+
+
+def rel_err(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def batched_leg(am, args):
+    """Tet10 cantilever batch: plain FP64 one at a time vs pooled mixed precision."""
+    from amaru_amd import pool as P, workloads as W
+    params = W.cantilever_params(args.batch_seed, args.batch)
+    t0 = time.perf_counter()
+
+    def build():
+        anas = []
+        for p in params:
+            ana = W.cantilever(p)
+            ana.model.configure_dofs(ana.stages[0].bcs)
+            ana.dev = am.DeviceModel(ana.model)
+            anas.append(ana)
+        return anas
+
+    plain_set = build()
+    log(f"batched: {len(params)} models built in {time.perf_counter() - t0:.1f}s, "
+        f"{sum(a.model.nu for a in plain_set)} unknowns in total")
+    nplain = min(args.batch_plain, len(plain_set)) if args.batch_plain > 0 else len(plain_set)
+    t1 = time.perf_counter()
+    for ana in plain_set[:nplain]:
+        ok, msg = am.solve(ana, precision="fp64", precond="jacobi", pcg_rtol=args.rtol)
+        if not ok:
+            raise RuntimeError(msg)
+    t_plain = time.perf_counter() - t1
+    plain_rate = dof_solves(plain_set[:nplain]) / t_plain
+    Uplain = [a.model.uvals.copy() for a in plain_set[:nplain]]
+    for a in plain_set:
+        a.dev.close()
+    log(f"batched: plain fp64, {nplain} models one at a time: {t_plain:.2f}s, {plain_rate:.4g} DOF-solves/s")
+    res = {}
+    for prec, pc in (("fp64", "jacobi"), ("mixed64acc", "jacobi"), ("mixed64acc", "two_level")):
+        key = f"{prec}_{pc}"
+        warm = [W.cantilever(params[0])]
+        P.run_analyses(warm, precision=prec, precond=pc, pcg_rtol=args.rtol)  # load the kernels
+        warm[0].dev.close()
+        anas = build()
+        pool = P.Pool(max_members=len(anas), check_every=32)
+        pool.timing(True)
+        t2 = time.perf_counter()
+        out = P.run_analyses(anas, pool=pool, precision=prec, precond=pc, pcg_rtol=args.rtol)
+        t_b = time.perf_counter() - t2
+        ms, nsp = pool.timing(False)
+        if not all(ok for ok, _ in out):
+            raise RuntimeError(f"batched {key}: {[m for ok, m in out if not ok][:3]}")
+        errs = [rel_err(a.model.uvals, u) for a, u in zip(anas, Uplain)]
+        res[key] = {"value": dof_solves(anas) / t_b, "seconds": t_b, "max_rel_err_vs_plain": max(errs),
+                    "pcg_iterations_total": int(sum(a.stats["pcg_iterations"] for a in anas)),
+                    "pcg_iterations_max": max(a.stats["pcg_iterations"] for a in anas),
+                    "spmv_ms": ms, "spmv_launches": nsp}
+        log(f"batched {key}: {t_b:.2f}s, {res[key]['value']:.4g} DOF-solves/s, max err vs plain {max(errs):.2e}, "
+            f"PCG its total {res[key]['pcg_iterations_total']}")
+        # error against the FP64 direct solve on a bounded sample of the models
+        if prec == "mixed64acc":
+            import scipy.sparse.linalg as spla
+            derr = []
+            order = np.argsort([a.model.nu for a in anas])
+            for i in order[:: max(1, len(order) // args.batch_direct)][:args.batch_direct]:
+                a = anas[int(i)]
+                m = a.model
+                ndofs, nu = m.configure_dofs(a.stages[0].bcs)
+                U, F = m.bc_vals(a.stages[0].bcs)
+                pool.detach(a.dev)
+                a.dev.assemble()
+                K = a.dev.K_csc().tocsc()
+                x = spla.splu(K[:nu, :nu].tocsc()).solve(F[:nu] - K[:nu, nu:] @ U[nu:])
+                eq = m.eq.ravel()
+                Ug = np.zeros(ndofs)
+                Ug[eq[eq >= 0]] = m.uvals.ravel()[eq >= 0]
+                derr.append(rel_err(Ug[:nu], x))
+            res[key]["max_rel_err_vs_fp64_direct"] = max(derr)
+            res[key]["direct_sample"] = len(derr)
+        for a in anas:
+            pool.forget(a.dev)
+            a.dev.close()
+        pool.close()
+    best = res["mixed64acc_two_level"]
+    return {"workload": f"{len(params)} Tet10 cantilevers 1x0.1x0.1 m (seed {args.batch_seed}), 500-20000 "
+                        "elements log-uniform, E 10-200 GPa, nu 0.2-0.35, tip load +-1 kN, 4 IP",
+            "unknowns_total": int(sum(a.model.nu for a in plain_set)),
+            "plain_fp64": {"value": plain_rate, "seconds": t_plain, "models": nplain,
+                           "pcg_iterations_total": int(sum(a.stats["pcg_iterations"] for a in plain_set[:nplain]))},
+            "pooled": res, "value": best["value"],
+            "speedup_vs_plain_fp64": best["value"] / plain_rate,
+            "speedup_breakdown": {k: v["value"] / plain_rate for k, v in res.items()}}
+
+
+def served_leg(am, args):
+    """Request stream: plain one-at-a-time capacity, then the server at a
+    Poisson rate that is a multiple of it."""
+    from amaru_amd import pool as P, workloads as W
+    params = W.request_params(args.serve_seed, args.serve)
+
+    def builder(p, prec, pc="jacobi"):
+        def build():
+            ana, kw = W.box_request(p)
+            return ana, dict(kw, precision=prec, precond=pc, pcg_rtol=args.rtol)
+        return build
+
+    # plain: one request at a time, arrival to result (host construction included)
+    t0 = time.perf_counter()
+    lat_plain, ds_plain = [], 0
+    for p in params:
+        t1 = time.perf_counter()
+        ana, kw = builder(p, "fp64")()
+        ok, msg = am.solve(ana, **kw)
+        if not ok:
+            raise RuntimeError(msg)
+        lat_plain.append(time.perf_counter() - t1)
+        ds_plain += ana.stats["dof_solves"]
+        ana.dev.close()
+    t_plain = time.perf_counter() - t0
+    plain_rate = ds_plain / t_plain
+    req_rate = len(params) / t_plain
+    rate = args.serve_load * req_rate
+    log(f"served: plain {t_plain:.2f}s for {len(params)} requests ({plain_rate:.4g} DOF-solves/s); "
+        f"serving at {rate:.3g} requests/s")
+    arr = W.arrival_times(args.serve_seed, len(params), rate)
+    reqs = [P.Request(i, arr[i], builder(p, "mixed64acc", args.serve_precond)) for i, p in enumerate(params)]
+    srv = P.Server(max_members=args.serve_members, hbm_budget=int(args.serve_hbm_gb * 2 ** 30), workers=8)
+    t2 = time.perf_counter()
+    try:
+        done = srv.serve(reqs)
+    finally:
+        srv.close()
+    t_srv = time.perf_counter() - t2
+    bad = [r.msg for r in done if not r.ok]
+    if bad:
+        raise RuntimeError(f"served: {bad[:3]}")
+    ds = sum(r.dof_solves for r in done)
+    st = P.latency_stats(done)
+    makespan = max(r.t_done for r in done)
+    out = {"workload": f"{len(params)} requests (seed {args.serve_seed}): Hex8/Tet10 boxes 1e3-3e5 DOFs, 70% elastic, "
+                       "30% von Mises 5 increments; Poisson arrivals",
+           "arrival_rate_per_s": rate, "load_vs_plain_capacity": args.serve_load,
+           "value": ds / makespan, "makespan_s": makespan, "dof_solves": int(ds),
+           "latency_s": st, "hbm_budget_gb": args.serve_hbm_gb, "peak_admitted_gb": srv.peak_in_use / 2 ** 30,
+           "pool_steps": srv.steps,
+           "plain_fp64": {"value": plain_rate, "seconds": t_plain,
+                          "latency_s": {"p50_s": float(np.percentile(lat_plain, 50)),
+                                        "p99_s": float(np.percentile(lat_plain, 99))}},
+           "speedup_vs_plain_fp64": (ds / makespan) / plain_rate}
+    log(f"served: {out['value']:.4g} DOF-solves/s, p50 {st['p50_s']:.3f}s p99 {st['p99_s']:.3f}s, "
+        f"{out['speedup_vs_plain_fp64']:.2f}x plain")
+    del t_srv
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -102,6 +271,16 @@ def main():
     ap.add_argument("--rtol", type=float, default=1e-10)
     ap.add_argument("--cpu-n", type=int, default=12, help="HEX20 elements per edge of the CPU sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batch", type=int, default=256, help="cantilevers in the batched leg (0 = skip)")
+    ap.add_argument("--batch-seed", type=int, default=0)
+    ap.add_argument("--batch-plain", type=int, default=0, help="models timed on the plain path (0 = all)")
+    ap.add_argument("--batch-direct", type=int, default=6, help="models checked against the FP64 direct solve")
+    ap.add_argument("--serve", type=int, default=48, help="requests in the served leg (0 = skip)")
+    ap.add_argument("--serve-seed", type=int, default=1)
+    ap.add_argument("--serve-load", type=float, default=12.0, help="arrival rate / plain capacity")
+    ap.add_argument("--serve-members", type=int, default=64)
+    ap.add_argument("--serve-hbm-gb", type=float, default=64.0)
+    ap.add_argument("--serve-precond", default="two_level", choices=["jacobi", "block_jacobi", "two_level"])
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: there is no multi-GPU path (models would be sharded whole, one GPU "
@@ -209,6 +388,14 @@ def main():
                                           f"({threads} OpenMP threads), SuperLU factor+solve "
                                           f"{cb['t_solve']:.2f}s (1 thread), update {cb['t_update']:.2f}s")}
     dev.close()
+    for name, leg, n in (("batched", batched_leg, args.batch), ("served", served_leg, args.serve)):
+        if n <= 0:
+            continue
+        try:
+            out[name] = leg(am, args)
+        except Exception as e:  # report, keep the headline line
+            log(f"{name} leg failed: {e!r}")
+            out[name] = {"error": repr(e)}
     print(json.dumps(out), flush=True)
 
 

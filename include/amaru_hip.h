@@ -39,7 +39,11 @@ enum {
 enum { AMR_HEX8 = 1, AMR_HEX20 = 2, AMR_TET10 = 3 };
 enum { AMR_MAT_LE = 0, AMR_MAT_VM = 1, AMR_MAT_DP = 2 };
 enum { AMR_PREC_FP64 = 0, AMR_PREC_MIXED = 1, AMR_PREC_MIXED_F64ACC = 2 };
-enum { AMR_PRECOND_JACOBI = 0, AMR_PRECOND_BLOCK_JACOBI = 1 };
+/* Preconditioners: Jacobi; 3x3 nodal block Jacobi; two-level = Jacobi plus
+ * a coarse correction on rigid-body modes of node aggregates (additive,
+ * Phi Kc^-1 Phi^T with Kc = Phi^T K Phi), which removes most of the
+ * iterations slender models need. */
+enum { AMR_PRECOND_JACOBI = 0, AMR_PRECOND_BLOCK_JACOBI = 1, AMR_PRECOND_TWO_LEVEL = 2 };
 
 typedef struct amr_model amr_model;
 
@@ -201,6 +205,73 @@ int amr_get_spmv_lanes(amr_model *m, int *out);
  * enable = 1 starts (and clears) the counters, 0 stops.  Returns the
  * accumulated milliseconds and launch count so far. */
 int amr_timing(amr_model *m, int enable, double *spmv_ms, int64_t *spmv_launches);
+
+/* The two halves of amr_nr_iteration, for callers that solve many models'
+ * linear systems together in a pool.  amr_nr_prepare: ΔUi = 0 if it > 1;
+ * mount_K; the right-hand side of solve_system! (src/mech/mech-solver.jl:
+ * 319-331), all queued on the model's stream without waiting.
+ * amr_nr_finish: given the outcome of the linear solve (si, from
+ * amr_pool_step; NULL means the model has no unknowns), combines ΔUi,
+ * runs update_state! and the residual as amr_nr_iteration does
+ * (src/mech/mech-solver.jl:331-367). */
+int amr_nr_prepare(amr_model *m, int it);
+int amr_nr_finish(amr_model *m, const amr_solve_info *si, amr_nr_info *info);
+
+/* ------------------------------------------------------------------------
+ * Pools: the linear solves of many models advanced together, one launch
+ * per PCG phase for all of them (batching), with models joining and leaving
+ * between steps (continuous batching).  Each model's PCG iterates are
+ * bit-identical to those of solving it alone with amr_solve_system or
+ * amr_nr_iteration (same kernels, per-model reductions in a fixed order).
+ * No reference counterpart: the reference solves one model at a time
+ * (src/solver.jl:5); these entry points serve the batched and served
+ * forms of solve_system!.
+ * ------------------------------------------------------------------------ */
+typedef struct amr_pool amr_pool;
+
+/* A pool on device for up to max_members models solving at once; every
+ * step runs check_every PCG iterations (0: 32) before the host reads the
+ * members' residuals. */
+int amr_pool_create(amr_pool **out, int device, int max_members, int check_every);
+/* Detaches every attached model (they keep working on their own streams). */
+int amr_pool_destroy(amr_pool *p);
+
+/* Moves the model's work onto the pool's stream (waiting for its own
+ * first); amr_model_destroy detaches a model by itself. */
+int amr_pool_attach(amr_pool *p, amr_model *m);
+int amr_pool_detach(amr_model *m);
+
+/* After amr_nr_prepare: the model's system K11 ΔUi = R - K12 ΔUi2 joins
+ * the pool and starts at the next step.  opts may be NULL. */
+int amr_pool_add(amr_pool *p, amr_model *m, const amr_solve_opts *opts);
+
+/* One step for every member.  Members whose solve ended (converged or
+ * failed, see info->converged) leave the pool and are written to
+ * models[0..*nfinished) and infos (arrays of at least capacity entries;
+ * capacity >= the pool's member count). */
+int amr_pool_step(amr_pool *p, int capacity, int *nfinished, amr_model **models, amr_solve_info *infos);
+
+/* Members currently solving. */
+int amr_pool_size(amr_pool *p, int *members);
+
+/* SpMV kernel timing over the pool's steps, as amr_timing. */
+int amr_pool_timing(amr_pool *p, int enable, double *spmv_ms, int64_t *spmv_launches);
+
+/* Aggregate size of the two-level preconditioner: about target_nodes nodes
+ * per aggregate, fewer aggregates than max_aggregates (defaults 256, 170);
+ * takes effect before the model's first two-level solve. */
+int amr_set_coarse(amr_model *m, int target_nodes, int max_aggregates);
+/* out[0..1]: aggregates and coarse dofs of the built coarse space (0 if not built). */
+int amr_get_coarse(amr_model *m, int64_t *out);
+/* The built coarse space: each node's aggregate (nnodes) and Kc^-1 as of
+ * the last two-level solve (coarse dofs squared, row-major); either pointer
+ * may be NULL. */
+int amr_get_coarse_space(amr_model *m, int32_t *agg, double *kc_inv);
+
+/* Device memory held by the model (bytes), and the device's free and total
+ * memory, for admission by HBM budget. */
+int amr_model_device_bytes(amr_model *m, int64_t *bytes);
+int amr_mem_info(int device, int64_t *free_bytes, int64_t *total_bytes);
 
 #ifdef __cplusplus
 }
