@@ -74,6 +74,7 @@ struct amr_pool;
 // Per-model device resources.
 struct amr_model {
     int etype = 0, nn = 0, nips = 0, device = 0;
+    int ncu = 256;  // compute units of the device (row-tile sizing)
     int64_t nnodes = 0, ne = 0, ndofs = 0, nu = 0, nip_total = 0;
     AmrPattern pat;
     std::vector<int32_t> eq;      // nnodes*3 eq ids (-1: no dof)
@@ -120,17 +121,33 @@ struct amr_model {
     int64_t dev_bytes = 0;             // device memory allocated for the model
     std::vector<double> Xh;            // host copy of the coordinates
     int64_t k_version = 0;             // bumped by every assembly
+    int64_t dof_version = 0;           // bumped by every DOF map upload
+    int64_t inc_version = 0;           // bumped by every amr_increment_begin
     // two-level preconditioner: aggregates of about coarse_target nodes (more
     // when the model would need over coarse_max aggregates), rigid-body modes
     // per aggregate, Kc = Phi^T K Phi inverted on the device
     int coarse_target = 256, coarse_max = 170;
     bool coarse_built = false;
-    int64_t coarse_version = -1;
+    int64_t coarse_version = -1;       // k_version the coarse operator was formed from
+    int64_t coarse_dofs = -1, coarse_inc = -1;
     int32_t nagg = 0, ncoarse = 0, ncb = 0;
     int64_t npairs = 0;
     int32_t *c_agg = nullptr, *c_aggptr = nullptr, *c_aggnodes = nullptr, *c_prow = nullptr, *c_pagg = nullptr;
     int32_t *c_cba = nullptr, *c_cbb = nullptr, *c_cbptr = nullptr, *c_cbslot = nullptr, *c_info = nullptr;
     double *c_cent = nullptr, *c_pair = nullptr, *Kinv = nullptr, *c_vec = nullptr, *y_vec = nullptr;
+    // split NR finish: pinned landing area of the residual scalars and error
+    // words, and the state between the two halves
+    struct FinHost {
+        double s[16];
+        int32_t c[16];
+        int32_t err[4];
+    } *fin_h = nullptr;
+    bool fin_pending = false;
+    int fin_rc = 0;
+    amr_nr_info fin_info{};
+    // stage_begin's host vectors stay alive until their copies are done
+    std::vector<double> sb_host[2];
+    hipEvent_t sb_event = nullptr;
     bool assembled = false;
     // device-resident stage vectors (node order): Uex Fex U F dFex R Rc dUa dUi dFin dUt x
     double *sv[12] = {};
@@ -155,6 +172,8 @@ int amr_vec_combine(amr_model *m, const double *x, double *dUi);
 int amr_vec_add(amr_model *m, const double *a, const double *b, double *out);
 int amr_vec_nr_post(amr_model *m, double *res, double *dui_max, double *dua_max, int *nan_flag);
 int amr_vec_commit(amr_model *m);
+int amr_vec_nr_post_async(amr_model *m, double *hs, int32_t *hc);
+int amr_nr_post_slots(int *res, int *errn, int *errd, int *nan);
 
 // kernels (kernels_elem.hip)
 int amr_launch_ke(amr_model *m);
@@ -164,6 +183,7 @@ int amr_launch_update(amr_model *m, const double *dU_node /*device, node order*/
 // two-level coarse space (coarse.hip): build on first use, refresh Kc^-1
 // after every assembly; queued on the model's stream
 int amr_coarse_setup(amr_model *m);
+int amr_coarse_build(amr_model *m);  // the index structures only (host work, one wait)
 void amr_coarse_free(amr_model *m);
 
 // solver (kernels_solve.hip)

@@ -196,6 +196,8 @@ static void free_model(amr_model *m)
         if (p) (void)hipFree(p);
     for (hipEvent_t e : m->ev)
         if (e) (void)hipEventDestroy(e);
+    if (m->fin_h) (void)hipHostFree(m->fin_h);
+    if (m->sb_event) (void)hipEventDestroy(m->sb_event);
     if (m->own_stream) (void)hipStreamDestroy(m->own_stream);
     delete m;
 }
@@ -228,6 +230,7 @@ static int upload_dofs(amr_model *m, const int32_t *eq_id, int64_t ndofs, int64_
     }
     TRY(h2d(m->mask, mask.data(), mask.size(), m->stream));
     m->k_version++;
+    m->dof_version++;
     TRY(h2d(m->eqd, eq_id, (size_t)m->nnodes * 3, m->stream));
     AMR_HIP(hipStreamSynchronize(m->stream));
     m->assembled = false;
@@ -288,6 +291,7 @@ extern "C" int amr_model_create(amr_model **out, int device, int etype, int nips
     TRY(amr_build_pattern(nn, nnodes, ne, conn, m->pat));
 
     AMR_HIP(hipSetDevice(device));
+    AMR_HIP(hipDeviceGetAttribute(&m->ncu, hipDeviceAttributeMultiprocessorCount, device));
     AMR_HIP(hipStreamCreateWithFlags(&m->own_stream, hipStreamNonBlocking));
     m->stream = m->own_stream;
     const size_t n3 = (size_t)nnodes * 3;
@@ -690,13 +694,16 @@ extern "C" int amr_stage_begin(amr_model *m, const double *Uex, const double *Fe
     const size_t n = (size_t)m->nnodes * 3;
     for (double *&p : m->sv)
         if (!p) TRY(dalloc(m, &p, n));
-    std::vector<double> un, fn;
-    eq_to_node(m, Uex, un);
-    eq_to_node(m, Fex, fn);
-    TRY(h2d(m->sv[SV_UEX], un.data(), n, m->stream));
-    TRY(h2d(m->sv[SV_FEX], fn.data(), n, m->stream));
+    // the host vectors live in the model until the copies from them are done
+    // (waited for here before they are overwritten by the next stage)
+    if (!m->sb_event) AMR_HIP(hipEventCreateWithFlags(&m->sb_event, hipEventDisableTiming));
+    else AMR_HIP(hipEventSynchronize(m->sb_event));
+    eq_to_node(m, Uex, m->sb_host[0]);
+    eq_to_node(m, Fex, m->sb_host[1]);
+    TRY(h2d(m->sv[SV_UEX], m->sb_host[0].data(), n, m->stream));
+    TRY(h2d(m->sv[SV_FEX], m->sb_host[1].data(), n, m->stream));
+    AMR_HIP(hipEventRecord(m->sb_event, m->stream));
     for (int k : {SV_U, SV_F, SV_RC, SV_DUA, SV_DUI, SV_DFIN, SV_R, SV_DFEX}) TRY(amr_vec_fill(m, m->sv[k], 0.0));
-    AMR_HIP(hipStreamSynchronize(m->stream));
     return AMR_OK;
 }
 
@@ -705,6 +712,7 @@ extern "C" int amr_increment_begin(amr_model *m, double dT, double acr, int add_
     if (!m || !m->sv[SV_U]) return AMR_ERR_ARG;
     AMR_HIP(hipSetDevice(m->device));
     m->acr = acr;
+    m->inc_version++;
     return amr_vec_inc_begin(m, dT, acr, add_residual);
 }
 
@@ -728,14 +736,19 @@ extern "C" int amr_nr_prepare(amr_model *m, int it)
     return AMR_OK;
 }
 
-extern "C" int amr_nr_finish(amr_model *m, const amr_solve_info *si, amr_nr_info *info)
+extern "C" int amr_nr_finish_begin(amr_model *m, const amr_solve_info *si)
 {
     if (!m || !m->sv[SV_U]) return AMR_ERR_ARG;
     if (amr_pool_is_member(m)) {
         amr_set_error("the model is solving in a pool; its matrix and solver vectors are in use");
         return AMR_ERR_ARG;
     }
+    if (m->fin_pending) {
+        amr_set_error("amr_nr_finish_begin: the previous finish has not been completed");
+        return AMR_ERR_ARG;
+    }
     AMR_HIP(hipSetDevice(m->device));
+    if (!m->fin_h) AMR_HIP(hipHostMalloc((void **)&m->fin_h, sizeof(amr_model::FinHost), hipHostMallocDefault));
     amr_nr_info inf{};
     double **v = m->sv;
     int rc = AMR_OK;
@@ -748,27 +761,42 @@ extern "C" int amr_nr_finish(amr_model *m, const amr_solve_info *si, amr_nr_info
         inf.relres = si->relres;
         inf.replacements = si->refinements;
         inf.fallback = si->fallback;
-        if (si->converged) {
-            TRY(amr_vec_combine(m, v[SV_X], v[SV_DUI]));
-        } else {
-            char buf[200];
-            snprintf(buf, sizeof buf, "PCG did not converge (relres %.3e after %d iterations, %d replacements)",
-                     si->relres, si->iterations, si->refinements);
-            amr_set_error(buf);
-            rc = AMR_ERR_SOLVE;
-        }
+        if (!si->converged) rc = AMR_ERR_SOLVE;
+        else TRY(amr_vec_combine(m, v[SV_X], v[SV_DUI]));
     }
-    int32_t herr[3] = {0, 0, 0};
     if (rc == AMR_OK) {
         // update_state!(ΔUa + ΔUitr) from the committed state
         TRY(amr_vec_add(m, v[SV_DUA], v[SV_DUI], v[SV_DUT]));
         TRY(amr_launch_update(m, v[SV_DUT], v[SV_DFIN]));
-        int nanf = 0;
-        double umax = 0.0, uamax = 0.0;
-        TRY(amr_vec_nr_post(m, &inf.res, &umax, &uamax, &nanf));
-        inf.err = umax / uamax;
-        TRY(d2h(herr, m->err, 3, m->stream));
+        TRY(amr_vec_nr_post_async(m, m->fin_h->s, m->fin_h->c));
+        AMR_HIP(hipMemcpyAsync(m->fin_h->err, m->err, 3 * sizeof(int32_t), hipMemcpyDeviceToHost, m->stream));
+    }
+    m->fin_info = inf;
+    m->fin_rc = rc;
+    m->fin_pending = true;
+    return AMR_OK;
+}
+
+extern "C" int amr_nr_finish_end(amr_model *m, amr_nr_info *info)
+{
+    if (!m || !m->fin_pending) return AMR_ERR_ARG;
+    amr_nr_info inf = m->fin_info;
+    int rc = m->fin_rc;
+    m->fin_pending = false;
+    if (rc == AMR_ERR_SOLVE) {
+        char buf[200];
+        snprintf(buf, sizeof buf, "PCG did not converge (relres %.3e after %d iterations, %d replacements)",
+                 inf.relres, inf.pcg_iterations, inf.replacements);
+        amr_set_error(buf);
+    } else if (rc == AMR_OK) {
+        AMR_HIP(hipSetDevice(m->device));
         AMR_HIP(hipStreamSynchronize(m->stream));
+        int i_res, i_errn, i_errd, i_nan;
+        amr_nr_post_slots(&i_res, &i_errn, &i_errd, &i_nan);
+        const double umax = m->fin_h->s[i_errn], uamax = m->fin_h->s[i_errd];
+        const int32_t *herr = m->fin_h->err;
+        inf.res = m->fin_h->s[i_res];
+        inf.err = umax / uamax;
         if (herr[0] != 0x7f7f7f7f) {
             amr_set_error("Negative Jacobian determinant in cell " + std::to_string(herr[0]));
             rc = AMR_ERR_JACOBIAN;
@@ -778,7 +806,7 @@ extern "C" int amr_nr_finish(amr_model *m, const amr_solve_info *si, amr_nr_info
         } else if (herr[1] != 0x7f7f7f7f) {
             amr_set_error("VonMisses: Negative value for √J2D (cell " + std::to_string(herr[1]) + ")");
             rc = AMR_ERR_MATERIAL;
-        } else if (nanf) {
+        } else if (m->fin_h->c[i_nan]) {
             amr_set_error("solve_system!: NaN values in internal forces vector");
             rc = AMR_ERR_NAN;
         } else if (umax > 1e8) {
@@ -791,6 +819,12 @@ extern "C" int amr_nr_finish(amr_model *m, const amr_solve_info *si, amr_nr_info
     inf.status = rc;
     if (info) *info = inf;
     return rc;
+}
+
+extern "C" int amr_nr_finish(amr_model *m, const amr_solve_info *si, amr_nr_info *info)
+{
+    TRY(amr_nr_finish_begin(m, si));
+    return amr_nr_finish_end(m, info);
 }
 
 extern "C" int amr_nr_iteration(amr_model *m, int it, const amr_solve_opts *opts, amr_nr_info *info)
@@ -967,7 +1001,8 @@ extern "C" int amr_set_coarse(amr_model *m, int target_nodes, int max_aggregates
     }
     m->coarse_target = target_nodes;
     m->coarse_max = max_aggregates;
-    return AMR_OK;
+    AMR_HIP(hipSetDevice(m->device));
+    return amr_coarse_build(m);
 }
 
 extern "C" int amr_get_coarse(amr_model *m, int64_t *out)

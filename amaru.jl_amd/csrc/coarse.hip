@@ -218,11 +218,24 @@ int build(amr_model *m)
 
 }  // namespace
 
+int amr_coarse_build(amr_model *m)
+{
+    if (m->coarse_built) return AMR_OK;
+    return build(m);
+}
+
 int amr_coarse_setup(amr_model *m)
 {
     if (!m->coarse_built)
         if (int rc = build(m)) return rc;
+    // The coarse operator only preconditions: it is formed from the K of the
+    // first assembly of each DOF map, and for models with plastic materials
+    // again at the first solve of each load increment (a linear elastic K
+    // does not change between assemblies)
     if (m->coarse_version == m->k_version) return AMR_OK;
+    if (m->coarse_version >= 0 && m->coarse_dofs == m->dof_version &&
+        (m->all_elastic || m->coarse_inc == m->inc_version))
+        return AMR_OK;
     hipStream_t s = m->stream;
     const int32_t nc = m->ncoarse;
     const int64_t nn = (int64_t)nc * nc;
@@ -252,6 +265,8 @@ int amr_coarse_setup(amr_model *m)
                        m->c_info);
     AMR_HIP(hipGetLastError());
     m->coarse_version = m->k_version;
+    m->coarse_dofs = m->dof_version;
+    m->coarse_inc = m->inc_version;
     return AMR_OK;
 }
 

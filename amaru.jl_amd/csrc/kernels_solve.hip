@@ -230,6 +230,15 @@ __device__ inline void row_sum(int32_t rb, int32_t re, int sub, const int32_t *_
     }
 }
 
+// Smallest power of two >= R: LDS reduction arrays are that long, zero
+// padded, so the tree is the same fixed binary tree for any R.
+__host__ __device__ inline int pow2ceil(int R)
+{
+    int p = 1;
+    while (p < R) p <<= 1;
+    return p;
+}
+
 // Fixed-order tree sum of R (power of two) LDS values; result in every thread.
 __device__ inline double lds_tree_sum(double *s, int R)
 {
@@ -243,12 +252,13 @@ __device__ inline double lds_tree_sum(double *s, int R)
 }
 
 // Dynamic LDS layout of the row-tile kernels: R+1 row pointers, then up to
-// four R-long double arrays (sized for the largest R of the launch).
+// four double arrays of pow2ceil(R) entries (sized for the largest R of the
+// launch).
 __device__ inline int32_t *tile_rowptr(char *sm) { return (int32_t *)sm; }
 __device__ inline double *tile_dbl(char *sm, int R, int j)
 {
     const int off = ((R + 1) * 4 + 15) & ~15;
-    return (double *)(sm + off) + (int64_t)j * R;
+    return (double *)(sm + off) + (int64_t)j * pow2ceil(R);
 }
 
 // Workgroups are dispatched round-robin over the 8 XCDs (workgroup b runs on
@@ -306,7 +316,8 @@ __global__ void __launch_bounds__(WGS) residual_kernel(const SegDesc *__restrict
     int32_t *srp = tile_rowptr(sm);
     double *s_rr = tile_dbl(sm, R, 0), *s_bb = tile_dbl(sm, R, 1), *s_kx = tile_dbl(sm, R, 2),
            *s_gap = tile_dbl(sm, R, 3);
-    for (int i = threadIdx.x; i < R; i += WGS) s_rr[i] = s_bb[i] = s_kx[i] = s_gap[i] = 0.0;
+    const int RP = pow2ceil(R);
+    for (int i = threadIdx.x; i < RP; i += WGS) s_rr[i] = s_bb[i] = s_kx[i] = s_gap[i] = 0.0;
     int64_t r0;
     const int nr = load_rows(d.rowptr, d.nnodes, R, te.y, srp, r0);
     constexpr int RPW = WGS / L;
@@ -339,8 +350,8 @@ __global__ void __launch_bounds__(WGS) residual_kernel(const SegDesc *__restrict
         }
     }
     __syncthreads();
-    const double trr = lds_tree_sum(s_rr, R), tbb = lds_tree_sum(s_bb, R), tkx = lds_tree_sum(s_kx, R),
-                 tgap = lds_tree_sum(s_gap, R);
+    const double trr = lds_tree_sum(s_rr, RP), tbb = lds_tree_sum(s_bb, RP), tkx = lds_tree_sum(s_kx, RP),
+                 tgap = lds_tree_sum(s_gap, RP);
     if (threadIdx.x == 0) {
         d.part[P_A + te.y] = trr;
         d.part[P_B + te.y] = tbb;
@@ -391,7 +402,8 @@ __global__ void __launch_bounds__(WGS) pcg_spmv_kernel(const SegDesc *__restrict
     int32_t *srp = tile_rowptr(sm);
     double *sdot = tile_dbl(sm, R, 0);
     static_assert(sizeof(AT) >= sizeof(VT), "accumulate at least in the storage precision");
-    for (int i = threadIdx.x; i < R; i += WGS) sdot[i] = 0.0;
+    const int RP = pow2ceil(R);
+    for (int i = threadIdx.x; i < RP; i += WGS) sdot[i] = 0.0;
     int64_t r0;
     const int nr = load_rows(d.rowptr, d.nnodes, R, te.y, srp, r0);
     constexpr int RPW = WGS / L;
@@ -413,7 +425,7 @@ __global__ void __launch_bounds__(WGS) pcg_spmv_kernel(const SegDesc *__restrict
         }
     }
     __syncthreads();
-    const double pq = lds_tree_sum(sdot, R);
+    const double pq = lds_tree_sum(sdot, RP);
     if (threadIdx.x == 0) d.part[P_PQ + te.y] = pq;
 }
 
@@ -678,8 +690,11 @@ __global__ void fill_kernel(int64_t n, double *__restrict__ x, double v)
     for (int64_t i = (int64_t)blockIdx.x * WGV + threadIdx.x; i < n; i += (int64_t)gridDim.x * WGV) x[i] = v;
 }
 
-// Rows per tile R (a power of two, at least 128, grown until the model has
-// at most MAX_GR tiles) and vector workgroups (at most MAX_GV) of a model.
+// Row tiles of a model.  The SpMV kernels run 4 workgroups per CU (8 waves
+// per SIMD), so ncu*4 tiles are resident at once: R is chosen so that the
+// tiles fill a whole number k of such rounds (k ~ rows / (128 * resident)),
+// avoiding a nearly empty last round, with R a multiple of 8, at least 64,
+// and at most MAX_GR tiles.  Vector kernels: at most MAX_GV workgroups.
 struct Geom {
     int R, ntiles, nvblk;
 };
@@ -687,16 +702,20 @@ struct Geom {
 Geom model_geom(const amr_model *m)
 {
     Geom g;
-    int R = 128;
-    while ((m->nnodes + R - 1) / R > MAX_GR) R *= 2;
-    g.R = R;
-    g.ntiles = (int)std::max<int64_t>(1, (m->nnodes + R - 1) / R);
+    const int64_t n = std::max<int64_t>(1, m->nnodes);
+    const int64_t res = (int64_t)std::max(1, m->ncu) * 4;
+    const int64_t k = std::max<int64_t>(1, (n + 64 * res) / (128 * res));
+    int64_t R = (n + k * res - 1) / (k * res);
+    R = std::max<int64_t>(64, (R + 7) / 8 * 8);
+    if ((n + R - 1) / R > MAX_GR) R = ((n + MAX_GR - 1) / MAX_GR + 7) / 8 * 8;
+    g.R = (int)R;
+    g.ntiles = (int)((n + R - 1) / R);
     const int64_t nv = (m->nnodes * 3 + WGV - 1) / WGV;
     g.nvblk = (int)std::max<int64_t>(1, std::min<int64_t>(nv, MAX_GV));
     return g;
 }
 
-size_t tile_lds(int R, int ndbl) { return (size_t)(((R + 1) * 4 + 15) & ~15) + (size_t)ndbl * R * 8; }
+size_t tile_lds(int R, int ndbl) { return (size_t)(((R + 1) * 4 + 15) & ~15) + (size_t)ndbl * pow2ceil(R) * 8; }
 
 // SpMV operand kinds: FP64 values; FP32 values with FP32 p and FP32
 // accumulation; FP32 values with FP64 gather and accumulation.
@@ -1708,6 +1727,31 @@ int amr_vec_nr_post(amr_model *m, double *res, double *dui_max, double *dua_max,
     *dui_max = hs[S_ERRN];
     *dua_max = hs[S_ERRD];
     *nan_flag = hc[C_NAN];
+    return AMR_OK;
+}
+
+// The same as amr_vec_nr_post without waiting: the scalars are copied into
+// hs/hc (pinned host memory) in stream order
+int amr_vec_nr_post_async(amr_model *m, double *hs, int32_t *hc)
+{
+    double **v = m->sv;
+    int32_t *ctl = (int32_t *)(m->scal + S_NSCAL);
+    const unsigned g = vec_grid(m);
+    hipLaunchKernelGGL(nr_post_kernel, dim3(g), dim3(WGV), 0, m->stream, m->nnodes * 3, m->mask, v[SV_DFEX],
+                       v[SV_DFIN], v[SV_DUI], v[SV_DUA], v[SV_R], m->part);
+    hipLaunchKernelGGL(nr_finalize_kernel, dim3(1), dim3(WGV), 0, m->stream, (int)g, m->part, ctl, m->scal);
+    AMR_HIP(hipGetLastError());
+    AMR_HIP(hipMemcpyAsync(hs, m->scal, sizeof(double) * S_NSCAL, hipMemcpyDeviceToHost, m->stream));
+    AMR_HIP(hipMemcpyAsync(hc, m->scal + S_NSCAL, sizeof(int32_t) * C_NCTL, hipMemcpyDeviceToHost, m->stream));
+    return AMR_OK;
+}
+
+int amr_nr_post_slots(int *res, int *errn, int *errd, int *nan)
+{
+    *res = S_RES;
+    *errn = S_ERRN;
+    *errd = S_ERRD;
+    *nan = C_NAN;
     return AMR_OK;
 }
 

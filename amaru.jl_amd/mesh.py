@@ -169,15 +169,22 @@ class Mesh:
 
     def facets(self):
         """Outer facets (get_outer_facets, src/mesh/mesh.jl:69-85): facets seen
-        once, first-appearance order.  Returns (owner element, node ids)."""
+        once, first-appearance order.  Returns (owner element, node ids).
+
+        In a conforming mesh two facets that share their three smallest
+        corner ids are the same facet, so those three ids, packed in one
+        int64, identify it (a 1-D sort instead of a sort of rows)."""
         if self._facets is None:
             F = FACETS[self.cellshape]
             nf, nfn = F.shape
             all_f = self.conn[:, F].reshape(-1, nfn)
-            key = np.sort(all_f, axis=1)
-            _, first, inv, cnt = np.unique(key, axis=0, return_index=True, return_inverse=True, return_counts=True)
-            inv = inv.ravel()
-            outer = np.nonzero(cnt[inv] == 1)[0]
+            if self.nnodes < (1 << 21):
+                c = np.sort(all_f[:, :3 if nfn == 6 else 4], axis=1)[:, :3].astype(np.int64)
+                _, inv, cnt = np.unique((c[:, 0] << 42) | (c[:, 1] << 21) | c[:, 2], return_inverse=True,
+                                        return_counts=True)
+            else:
+                _, inv, cnt = np.unique(np.sort(all_f, axis=1), axis=0, return_inverse=True, return_counts=True)
+            outer = np.nonzero(cnt[inv.ravel()] == 1)[0]
             self._facets = (outer // nf, all_f[outer])
         return self._facets
 
@@ -188,8 +195,9 @@ class Mesh:
             E = FACET_EDGES[fn.shape[1]]
             all_e = fn[:, E].reshape(-1, E.shape[1])
             own = np.repeat(owners, E.shape[0])
-            key = np.sort(all_e, axis=1)
-            _, first = np.unique(key, axis=0, return_index=True)
+            # an edge is identified by its two end nodes
+            a, b = all_e[:, 0].astype(np.int64), all_e[:, 1].astype(np.int64)
+            _, first = np.unique((np.minimum(a, b) << 32) | np.maximum(a, b), return_index=True)
             first = np.sort(first)
             self._edges = (own[first], all_e[first])
         return self._edges

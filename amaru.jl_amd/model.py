@@ -270,7 +270,7 @@ class FEModel:
     def configure_dofs(self, bcs):
         """configure_dofs! (src/bc.jl:198-233): DOFs of the active nodes in node
         order (ux,uy,uz), stable partition unknowns-first, eq ids = position."""
-        active = np.unique(self.conn)
+        active = np.nonzero(np.bincount(self.conn.ravel(), minlength=self.nnodes))[0]
         presc = np.zeros((self.nnodes, 3), dtype=bool)
         self._bcsel = []
         for flt, bc in bcs:
@@ -491,6 +491,19 @@ class DeviceModel:
         info = _lib.NRInfo()
         si = ctypes.byref(solve_info) if solve_info is not None else None
         rc = _lib.lib().amr_nr_finish(self._h, si, ctypes.byref(info))
+        return self._nr_result(rc, info)
+
+    def nr_finish_begin(self, solve_info):
+        """First half of nr_finish: queue the update and the residual read-back."""
+        import ctypes
+        si = ctypes.byref(solve_info) if solve_info is not None else None
+        _lib.check(_lib.lib().amr_nr_finish_begin(self._h, si))
+
+    def nr_finish_end(self):
+        """Second half of nr_finish: wait and return what nr_iteration returns."""
+        import ctypes
+        info = _lib.NRInfo()
+        rc = _lib.lib().amr_nr_finish_end(self._h, ctypes.byref(info))
         return self._nr_result(rc, info)
 
     def _nr_result(self, rc, info):

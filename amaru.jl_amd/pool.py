@@ -107,6 +107,14 @@ class Job:
     def resume(self, solve_info):
         return self._advance(self.ana.dev.nr_finish(solve_info))
 
+    def resume_begin(self, solve_info):
+        """Queue this member's NR finish; resume_end completes it."""
+        self.ana.dev.nr_finish_begin(solve_info)
+
+    def resume_end(self):
+        return self._advance(self.ana.dev.nr_finish_end())
+
+
     def _advance(self, r, first=False):
         try:
             it = next(self.gen) if first else self.gen.send(r)
@@ -123,6 +131,17 @@ class Job:
             self.result = stop.value
             self.done = True
             return True
+
+
+def resume_all(finished, jobs_of):
+    """Resume the jobs of the members a step finished: all NR finishes are
+    queued first and then completed, so they share one wait.  Yields
+    (handle, job, done)."""
+    jobs = [(h, jobs_of(h)) for h, _ in finished]
+    for (h, job), (_, si) in zip(jobs, finished):
+        job.resume_begin(si)
+    for h, job in jobs:
+        yield h, job, job.resume_end()
 
 
 def run_analyses(analyses, pool=None, device=0, check_every=32, **kwargs):
@@ -142,9 +161,11 @@ def run_analyses(analyses, pool=None, device=0, check_every=32, **kwargs):
             else:
                 jobs[ana.dev.handle.value] = (i, job)
         while jobs:
-            for h, si in pool.step():
-                i, job = jobs.pop(h)
-                if job.resume(si):
+            fin = pool.step()
+            idx = {h: jobs.pop(h) for h, _ in fin}
+            for h, job, done in resume_all(fin, lambda h: idx[h][1]):
+                i = idx[h][0]
+                if done:
                     results[i] = job.result
                 else:
                     jobs[job.ana.dev.handle.value] = (i, job)
@@ -248,6 +269,8 @@ class Server:
 
         def device_build(req):
             req.ana.dev = DeviceModel(req.ana.model, device=self.device)
+            if req.kwargs.get("precond") == "two_level":
+                req.ana.dev.set_coarse()   # aggregates built here, off the serving loop
             return req
 
         nxt = 0
@@ -295,9 +318,11 @@ class Server:
                         running[req.ana.dev.handle.value] = req
                 if self.pool.size() > 0:
                     self.steps += 1
-                    for h, si in self.pool.step():
-                        req = running.pop(h)
-                        if req.job.resume(si):
+                    fin = self.pool.step()
+                    reqs_of = {h: running.pop(h) for h, _ in fin}
+                    for h, job, done in resume_all(fin, lambda h: reqs_of[h].job):
+                        req = reqs_of[h]
+                        if done:
                             self._finish(req, clock())
                             ndone += 1
                         else:

@@ -116,12 +116,16 @@ def batched_leg(am, args):
     params = W.cantilever_params(args.batch_seed, args.batch)
     t0 = time.perf_counter()
 
-    def build():
+    def build(two_level=False):
+        """Models resident on the device: mesh, DOFs, sparsity pattern and, for
+        the two-level preconditioner, its aggregates (all mesh-only data)."""
         anas = []
         for p in params:
             ana = W.cantilever(p)
             ana.model.configure_dofs(ana.stages[0].bcs)
             ana.dev = am.DeviceModel(ana.model)
+            if two_level:
+                ana.dev.set_coarse()
             anas.append(ana)
         return anas
 
@@ -146,7 +150,7 @@ def batched_leg(am, args):
         warm = [W.cantilever(params[0])]
         P.run_analyses(warm, precision=prec, precond=pc, pcg_rtol=args.rtol)  # load the kernels
         warm[0].dev.close()
-        anas = build()
+        anas = build(two_level=pc == "two_level")
         pool = P.Pool(max_members=len(anas), check_every=32)
         pool.timing(True)
         t2 = time.perf_counter()
@@ -211,50 +215,63 @@ def served_leg(am, args):
 
     # plain: one request at a time, arrival to result (host construction included)
     t0 = time.perf_counter()
-    lat_plain, ds_plain = [], 0
+    lat_plain, ds_plain, t_build_plain, U_plain = [], 0, 0.0, []
     for p in params:
         t1 = time.perf_counter()
         ana, kw = builder(p, "fp64")()
+        ana.model.configure_dofs(ana.stages[0].bcs)
+        ana.dev = am.DeviceModel(ana.model)
+        t_build_plain += time.perf_counter() - t1
         ok, msg = am.solve(ana, **kw)
         if not ok:
             raise RuntimeError(msg)
         lat_plain.append(time.perf_counter() - t1)
         ds_plain += ana.stats["dof_solves"]
+        U_plain.append(ana.model.uvals.copy())
         ana.dev.close()
     t_plain = time.perf_counter() - t0
     plain_rate = ds_plain / t_plain
     req_rate = len(params) / t_plain
+    def serve(arr):
+        reqs = [P.Request(i, arr[i], builder(p, "mixed64acc", args.serve_precond)) for i, p in enumerate(params)]
+        srv = P.Server(max_members=args.serve_members, hbm_budget=int(args.serve_hbm_gb * 2 ** 30), workers=8)
+        try:
+            done = srv.serve(reqs)
+        finally:
+            srv.close()
+        bad = [r.msg for r in done if not r.ok]
+        if bad:
+            raise RuntimeError(f"served: {bad[:3]}")
+        ds = sum(r.dof_solves for r in done)
+        makespan = max(r.t_done for r in done) - min(r.t_arrival for r in done)
+        err = {"elastic": 0.0, "von_mises": 0.0}
+        for r in done:
+            k = "von_mises" if params[r.rid]["vm"] else "elastic"
+            err[k] = max(err[k], rel_err(r.ana.model.uvals, U_plain[r.rid]))
+        return {"value": ds / makespan, "makespan_s": makespan, "dof_solves": int(ds),
+                "max_rel_err_vs_plain": err,
+                "latency_s": P.latency_stats(done), "peak_admitted_gb": srv.peak_in_use / 2 ** 30,
+                "pool_steps": srv.steps,
+                "phases_mean_s": {"arrival_to_admit": float(np.mean([r.t_admit - r.t_arrival for r in done])),
+                                  "admit_to_device_ready": float(np.mean([r.t_start - r.t_admit for r in done])),
+                                  "device_ready_to_done": float(np.mean([r.t_done - r.t_start for r in done]))}}
+
+    cap = serve(np.zeros(len(params)))
+    log(f"served capacity (all requests at t=0): {cap['value']:.4g} DOF-solves/s, "
+        f"{cap['value'] / plain_rate:.2f}x plain")
     rate = args.serve_load * req_rate
-    log(f"served: plain {t_plain:.2f}s for {len(params)} requests ({plain_rate:.4g} DOF-solves/s); "
-        f"serving at {rate:.3g} requests/s")
-    arr = W.arrival_times(args.serve_seed, len(params), rate)
-    reqs = [P.Request(i, arr[i], builder(p, "mixed64acc", args.serve_precond)) for i, p in enumerate(params)]
-    srv = P.Server(max_members=args.serve_members, hbm_budget=int(args.serve_hbm_gb * 2 ** 30), workers=8)
-    t2 = time.perf_counter()
-    try:
-        done = srv.serve(reqs)
-    finally:
-        srv.close()
-    t_srv = time.perf_counter() - t2
-    bad = [r.msg for r in done if not r.ok]
-    if bad:
-        raise RuntimeError(f"served: {bad[:3]}")
-    ds = sum(r.dof_solves for r in done)
-    st = P.latency_stats(done)
-    makespan = max(r.t_done for r in done)
-    out = {"workload": f"{len(params)} requests (seed {args.serve_seed}): Hex8/Tet10 boxes 1e3-3e5 DOFs, 70% elastic, "
-                       "30% von Mises 5 increments; Poisson arrivals",
-           "arrival_rate_per_s": rate, "load_vs_plain_capacity": args.serve_load,
-           "value": ds / makespan, "makespan_s": makespan, "dof_solves": int(ds),
-           "latency_s": st, "hbm_budget_gb": args.serve_hbm_gb, "peak_admitted_gb": srv.peak_in_use / 2 ** 30,
-           "pool_steps": srv.steps,
-           "plain_fp64": {"value": plain_rate, "seconds": t_plain,
+    lat = serve(W.arrival_times(args.serve_seed, len(params), rate))
+    log(f"served at {rate:.3g} requests/s: {lat['value']:.4g} DOF-solves/s, p50 {lat['latency_s']['p50_s']:.3f}s "
+        f"p99 {lat['latency_s']['p99_s']:.3f}s")
+    out = {"workload": f"{len(params)} requests (seed {args.serve_seed}): Hex8/Tet10 boxes 1e3-3e5 DOFs, 70% "
+                       "elastic, 30% von Mises 5 increments",
+           "precision": "mixed64acc", "precond": args.serve_precond, "hbm_budget_gb": args.serve_hbm_gb,
+           "value": cap["value"], "speedup_vs_plain_fp64": cap["value"] / plain_rate,
+           "capacity_all_at_t0": cap,
+           "poisson": dict(lat, arrival_rate_per_s=rate, load_vs_plain_request_rate=args.serve_load),
+           "plain_fp64": {"value": plain_rate, "seconds": t_plain, "host_build_s": t_build_plain,
                           "latency_s": {"p50_s": float(np.percentile(lat_plain, 50)),
-                                        "p99_s": float(np.percentile(lat_plain, 99))}},
-           "speedup_vs_plain_fp64": (ds / makespan) / plain_rate}
-    log(f"served: {out['value']:.4g} DOF-solves/s, p50 {st['p50_s']:.3f}s p99 {st['p99_s']:.3f}s, "
-        f"{out['speedup_vs_plain_fp64']:.2f}x plain")
-    del t_srv
+                                        "p99_s": float(np.percentile(lat_plain, 99))}}}
     return out
 
 
@@ -277,7 +294,8 @@ def main():
     ap.add_argument("--batch-direct", type=int, default=6, help="models checked against the FP64 direct solve")
     ap.add_argument("--serve", type=int, default=48, help="requests in the served leg (0 = skip)")
     ap.add_argument("--serve-seed", type=int, default=1)
-    ap.add_argument("--serve-load", type=float, default=12.0, help="arrival rate / plain capacity")
+    ap.add_argument("--serve-load", type=float, default=4.0,
+                    help="Poisson arrival rate as a multiple of the plain path's request rate")
     ap.add_argument("--serve-members", type=int, default=64)
     ap.add_argument("--serve-hbm-gb", type=float, default=64.0)
     ap.add_argument("--serve-precond", default="two_level", choices=["jacobi", "block_jacobi", "two_level"])

@@ -216,6 +216,12 @@ int amr_timing(amr_model *m, int enable, double *spmv_ms, int64_t *spmv_launches
  * (src/mech/mech-solver.jl:331-367). */
 int amr_nr_prepare(amr_model *m, int it);
 int amr_nr_finish(amr_model *m, const amr_solve_info *si, amr_nr_info *info);
+/* amr_nr_finish in two halves: _begin queues the work and the read-back of
+ * the residual norms, _end waits for them and fills info (the return code as
+ * amr_nr_finish's).  Finishing many models of one pool: begin all, then end
+ * all, so they share one wait. */
+int amr_nr_finish_begin(amr_model *m, const amr_solve_info *si);
+int amr_nr_finish_end(amr_model *m, amr_nr_info *info);
 
 /* ------------------------------------------------------------------------
  * Pools: the linear solves of many models advanced together, one launch
@@ -258,8 +264,11 @@ int amr_pool_size(amr_pool *p, int *members);
 int amr_pool_timing(amr_pool *p, int enable, double *spmv_ms, int64_t *spmv_launches);
 
 /* Aggregate size of the two-level preconditioner: about target_nodes nodes
- * per aggregate, fewer aggregates than max_aggregates (defaults 256, 170);
- * takes effect before the model's first two-level solve. */
+ * per aggregate, fewer aggregates than max_aggregates (defaults 256, 170).
+ * Builds the aggregates and their index structures now (host work and one
+ * wait on the model's stream), so that the first two-level solve does not;
+ * without this call the first two-level solve builds them with the
+ * defaults. */
 int amr_set_coarse(amr_model *m, int target_nodes, int max_aggregates);
 /* out[0..1]: aggregates and coarse dofs of the built coarse space (0 if not built). */
 int amr_get_coarse(amr_model *m, int64_t *out);

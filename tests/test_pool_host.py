@@ -59,6 +59,12 @@ class FakeDev:
     def nr_finish(self, si):
         return self._result(self._pending)
 
+    def nr_finish_begin(self, si):
+        self.calls.append(("finish_begin",))
+
+    def nr_finish_end(self):
+        return self._result(self._pending)
+
     def increment_commit(self):
         self.calls.append(("commit",))
 
@@ -82,6 +88,9 @@ class FakePool:
 
     def attach(self, dev):
         self.attached.add(dev.handle.value)
+
+    def detach(self, dev):
+        self.attached.discard(dev.handle.value)
 
     def add(self, dev, opts):
         assert dev.handle.value in self.attached
@@ -143,6 +152,21 @@ def test_pooled_driver_makes_the_alone_calls(amaru, monkeypatch, autoinc):
         assert [s.status for s in a.stages] == [s.status for s in b.stages]
     # the script really exercises rejections
     assert any(("reject",) in a.dev.calls for a in alone)
+
+
+def test_run_analyses_interleaves_like_alone(amaru, monkeypatch):
+    """run_analyses resumes the members a step returns with queued-then-
+    completed finishes; per model the calls are those of solving alone."""
+    from amaru_amd import analysis, pool as P
+    monkeypatch.setattr(analysis, "DeviceModel", FakeDev)
+    alone = analyses(amaru)
+    res_alone = [amaru.solve(a, autoinc=True) for a in alone]
+    pooled = analyses(amaru)
+    res_pool = P.run_analyses(pooled, pool=FakePool(), autoinc=True)
+    assert res_pool == res_alone
+    for a, b in zip(alone, pooled):
+        assert a.dev.calls == [c for c in b.dev.calls if c != ("finish_begin",)]
+        assert a.stats == b.stats
 
 
 def test_cantilever_batch_sizes(amaru):
