@@ -195,12 +195,12 @@ __device__ inline void row_sum(int32_t rb, int32_t re, int sub, const int32_t *_
         for (int t = 0; t < NS; t++) {
             const int32_t k = base + sub + L * t;
             if (k < re) {
-                const int64_t c = __builtin_nontemporal_load(col + k);
+                const int64_t c = col[k];
                 const AT x0 = (AT)x[c * 3], x1 = (AT)x[c * 3 + 1], x2 = (AT)x[c * 3 + 2];
                 const VT *vp = vals + kv_at(k, 0);
                 AT v[9];
 #pragma unroll
-                for (int q = 0; q < 9; q++) v[q] = (AT)__builtin_nontemporal_load(vp + q * 64);
+                for (int q = 0; q < 9; q++) v[q] = (AT)vp[q * 64];
                 if (ABS) {
                     acc[t][0] += fabs(v[0] * x0) + fabs(v[1] * x1) + fabs(v[2] * x2);
                     acc[t][1] += fabs(v[3] * x0) + fabs(v[4] * x1) + fabs(v[5] * x2);

@@ -60,6 +60,19 @@ def block_problem(am, n, shape="HEX20", nips=27):
     return mesh, model, bcs, U, F
 
 
+def measured_traffic(precision, n):
+    """HBM bytes per SpMV launch from the committed rocprofv3 --pmc summary of
+    the same kernel and workload (FETCH_SIZE + WRITE_SIZE, calibrated on a
+    copy kernel), or None."""
+    path = os.path.join(ROOT, "profiles", "pcg_spmv_hex20_40cube_pmc_traffic.md")
+    if precision != "mixed64acc" or n != 40 or not os.path.exists(path):
+        return None, None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return int(line.split(":")[1]), os.path.relpath(path, ROOT)
+    return None, None
+
+
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -275,6 +288,23 @@ def served_leg(am, args):
     return out
 
 
+def pmc_probe(am, args):
+    """One mixed-precision step on the block (its SpMV launches are what a
+    counter pass reads) and a plain device copy of 1 GiB, the calibration
+    kernel of known traffic (1 GiB read, 1 GiB written)."""
+    import torch
+    mesh, model, bcs, Uex, Fex = block_problem(am, args.n)
+    dev = am.DeviceModel(model, device=0)
+    dev.stage_begin(Uex, Fex)
+    run_steps(dev, dev.opts(precision=args.precision, rtol=args.rtol, precond=args.precond), 1)
+    dev.close()
+    a = torch.ones(2 ** 27, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    log("pmc probe done: one step, one 1 GiB copy")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -299,6 +329,8 @@ def main():
     ap.add_argument("--serve-members", type=int, default=64)
     ap.add_argument("--serve-hbm-gb", type=float, default=64.0)
     ap.add_argument("--serve-precond", default="two_level", choices=["jacobi", "block_jacobi", "two_level"])
+    ap.add_argument("--pmc-probe", action="store_true",
+                    help="run one step and a 1 GiB copy only (for rocprofv3 --pmc passes), print nothing else")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: there is no multi-GPU path (models would be sharded whole, one GPU "
@@ -306,6 +338,9 @@ def main():
         sys.exit(2)
 
     am = load_pkg()
+    if args.pmc_probe:
+        pmc_probe(am, args)
+        return
     t0 = time.perf_counter()
     mesh, model, bcs, Uex, Fex = block_problem(am, args.n)
     dev = am.DeviceModel(model, device=0)
@@ -365,6 +400,7 @@ def main():
     spmv_bytes = (9 * vb + 4) * nblk + 4 * (nnodes + 1) + 3 * xb * nnodes + 24 * nnodes + 24 * nnodes
     avg_s = spmv_ms / 1e3 / max(spmv_n, 1)
     achieved = spmv_bytes / avg_s / 1e9 if spmv_n else None
+    traffic, traffic_src = measured_traffic(args.precision, args.n)
     out = {
         "metric": "DOF-solves/s + p50/p99 latency vs plain FP64 GPU path at 1e-8 rel error",
         "value": value,
@@ -393,7 +429,8 @@ def main():
         "roofline": {"bound": "hbm", "kernel": SPMV_KERNEL[args.precision].replace("L", str(
                          lanes["fp64"] if args.precision == "fp64" else
                          lanes["fp32"] if args.precision == "mixed" else lanes["fp32_f64acc"])), "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
-                     "frac": achieved / 8000.0 if achieved else None, "traffic": None,
+                     "frac": achieved / 8000.0 if achieved else None, "traffic": traffic,
+                     "traffic_source": traffic_src,
                      "bytes_per_launch": spmv_bytes, "avg_launch_us": avg_s * 1e6, "launches": spmv_n},
     }
     if not args.no_cpu:
