@@ -21,11 +21,18 @@ struct AmrMat {
 
 void amr_set_error(const std::string &msg);
 
-// K block values in chunks of 64 consecutive blocks: inside a chunk, slot s
-// (row-major 3x3 position) of all 64 blocks is contiguous, so the lanes of a
-// wave reading consecutive blocks touch one contiguous 64-value run per slot
-// and a chunk's 9 slots are one contiguous region.
-__host__ __device__ inline int64_t kv_at(int64_t k, int s) { return (k >> 6) * 576 + (int64_t)s * 64 + (k & 63); }
+// K block values in chunks of 64 consecutive blocks.  Inside a chunk the
+// nine values (row-major 3x3 position s) of a block are stored as four pairs
+// (s = 0,1 | 2,3 | 4,5 | 6,7) and a single (s = 8): pair p of all 64 blocks is
+// one contiguous run of 64 two-element vectors, then the run of s = 8.  A
+// lane reading block k fetches its values with four 2-wide loads and one
+// scalar load, and the lanes of a wave reading consecutive blocks touch one
+// contiguous run per load.
+__host__ __device__ inline int64_t kv_at(int64_t k, int s)
+{
+    const int64_t base = (k >> 6) * 576;
+    return s < 8 ? base + (int64_t)(s >> 1) * 128 + (k & 63) * 2 + (s & 1) : base + 512 + (k & 63);
+}
 inline int64_t kv_size(int64_t nblk) { return ((nblk + 63) >> 6) * 576; }
 
 #define AMR_HIP(call)                                                                  \

@@ -182,6 +182,43 @@ __device__ inline void apply_prec(const SegDesc &P, int64_t node, const double r
 // So neither the row sums nor the partials depend on L, and L can be
 // autotuned per model.
 
+// Pointers taken from a descriptor are generic; device buffers are all in
+// the global address space, and saying so lets the compiler use global
+// (not flat) memory instructions.
+template <typename T>
+__device__ inline const __attribute__((address_space(1))) T *gl(const T *p)
+{
+    return (const __attribute__((address_space(1))) T *)p;
+}
+template <typename T>
+__device__ inline __attribute__((address_space(1))) T *glw(T *p)
+{
+    return (__attribute__((address_space(1))) T *)p;
+}
+
+// The nine values of block k (kv_at layout): four 2-wide loads and one
+// scalar load, converted to the accumulation type.
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+template <typename VT> struct Vec2;
+template <> struct Vec2<float> { using T = f32x2; };
+template <> struct Vec2<double> { using T = f64x2; };
+
+template <typename VT, typename AT>
+__device__ inline void load_block(const VT *__restrict__ vals, int64_t k, AT v[9])
+{
+    using V2 = typename Vec2<VT>::T;
+    const auto *base = gl(vals) + (k >> 6) * 576;
+    const auto *pr = gl(reinterpret_cast<const V2 *>(vals + (k >> 6) * 576)) + (k & 63);
+#pragma unroll
+    for (int p = 0; p < 4; p++) {
+        const V2 w = pr[p * 64];
+        v[2 * p] = (AT)w.x;
+        v[2 * p + 1] = (AT)w.y;
+    }
+    v[8] = (AT)base[512 + (k & 63)];
+}
+
 template <int L, bool ABS, typename VT, typename XT, typename AT>
 __device__ inline void row_sum(int32_t rb, int32_t re, int sub, const int32_t *__restrict__ col,
                                const VT *__restrict__ vals, const XT *__restrict__ x, AT y[3])
@@ -195,12 +232,11 @@ __device__ inline void row_sum(int32_t rb, int32_t re, int sub, const int32_t *_
         for (int t = 0; t < NS; t++) {
             const int32_t k = base + sub + L * t;
             if (k < re) {
-                const int64_t c = col[k];
-                const AT x0 = (AT)x[c * 3], x1 = (AT)x[c * 3 + 1], x2 = (AT)x[c * 3 + 2];
-                const VT *vp = vals + kv_at(k, 0);
+                const int64_t c = gl(col)[k];
+                const auto *xc = gl(x) + c * 3;
+                const AT x0 = (AT)xc[0], x1 = (AT)xc[1], x2 = (AT)xc[2];
                 AT v[9];
-#pragma unroll
-                for (int q = 0; q < 9; q++) v[q] = (AT)vp[q * 64];
+                load_block<VT, AT>(vals, k, v);
                 if (ABS) {
                     acc[t][0] += fabs(v[0] * x0) + fabs(v[1] * x1) + fabs(v[2] * x2);
                     acc[t][1] += fabs(v[3] * x0) + fabs(v[4] * x1) + fabs(v[5] * x2);
@@ -275,7 +311,7 @@ __device__ inline int load_rows(const int32_t *__restrict__ rowptr, int64_t nrow
 {
     r0 = tile * R;
     const int nr = (int)std::min<int64_t>(R, nrows - r0);
-    for (int i = threadIdx.x; i <= nr; i += blockDim.x) srp[i] = rowptr[r0 + i];
+    for (int i = threadIdx.x; i <= nr; i += blockDim.x) srp[i] = gl(rowptr)[r0 + i];
     __syncthreads();
     return nr;
 }
@@ -417,9 +453,9 @@ __global__ void __launch_bounds__(WGS) pcg_spmv_kernel(const SegDesc *__restrict
             double c = 0.0;
             for (int e = 0; e < 3; e++) {
                 const int64_t k = (r0 + i) * 3 + e;
-                const double qk = d.mask[k] != 0.0 ? (double)y[e] : 0.0;
-                d.q[k] = qk;
-                c = fma(d.p[k], qk, c);
+                const double qk = gl(d.mask)[k] != 0.0 ? (double)y[e] : 0.0;
+                glw(d.q)[k] = qk;
+                c = fma(gl(d.p)[k], qk, c);
             }
             sdot[i] = c;
         }
