@@ -4,6 +4,11 @@
   element counts log-uniform in [500, 20000] (m = round((N/60)^(1/3)),
   mesh 10m x m x m cells of 6 tets), E ~ U[10, 200] GPa, nu ~ U[0.2, 0.35],
   x = 0 clamped, tip load resultant ~ U[-1, 1] kN in z on x = 1 (4 IPs).
+* ``footing(n)``: elastoplastic soil block 10 x 10 x 5 m, (2n x 2n x n) Hex8,
+  Drucker-Prager (E = 50 MPa, nu = 0.3, c = 20 kPa, phi = 30 deg as the
+  compression cone: alpha = 0.23094, kappa = 24 kPa, H = 0), base clamped,
+  lateral faces on rollers, a 2 x 2 m centred rigid footing pushed down
+  0.04 m in 20 increments (n = 30 is the 60 x 60 x 30 block, 346,053 DOFs).
 * ``served_stream(seed, n, rate)``: Poisson arrivals; each request a box
   4a x a x a (a = 0.25 m) meshed with Hex8 or Tet10 (50/50) at a DOF count
   log-uniform in [1e3, 3e5]; 70 % linear elastic under a tip traction, 30 %
@@ -16,7 +21,7 @@ import numpy as np
 
 from .analysis import MechAnalysis, addstage
 from .mesh import HEX8, TET10, Block, Mesh
-from .model import FEModel, LinearElastic, MechSolid, SurfaceBC, VonMises
+from .model import DruckerPrager, FEModel, LinearElastic, MechSolid, NodeBC, SurfaceBC, VonMises
 
 
 def cantilever_params(seed, n):
@@ -40,6 +45,17 @@ def cantilever(p):
 
 def cantilever_batch(seed=0, n=256):
     return [cantilever(p) for p in cantilever_params(seed, n)]
+
+
+def footing(n=30, settlement=0.04, nincs=20):
+    """(MechAnalysis, solver kwargs) of the footing problem."""
+    mesh = Mesh(Block([[0, 0, 0], [10.0, 10.0, 5.0]], nx=2 * n, ny=2 * n, nz=n, cellshape=HEX8))
+    model = FEModel(mesh, [("all", MechSolid, DruckerPrager(E=50e6, nu=0.3, alpha=0.23094, kappa=24000.0, H=0.0))])
+    ana = MechAnalysis(model)
+    bcs = [("z==0", SurfaceBC(ux=0, uy=0, uz=0)), ("x==0 || x==10", SurfaceBC(ux=0)),
+           ("y==0 || y==10", SurfaceBC(uy=0)), ("z==5 && x>=4 && x<=6 && y>=4 && y<=6", NodeBC(uz=-settlement))]
+    addstage(ana, bcs, nincs=nincs)
+    return ana, {"maxits": 5}
 
 
 def request_params(seed, n):

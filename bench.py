@@ -214,6 +214,41 @@ def batched_leg(am, args):
             "speedup_breakdown": {k: v["value"] / plain_rate for k, v in res.items()}}
 
 
+def footing_leg(am, args):
+    """Elastoplastic footing: the full Newton-Raphson analysis (20 increments
+    of prescribed settlement, tangent assembly and return mapping every NR
+    iteration) on the plain FP64 path and on the mixed two-level path."""
+    from amaru_amd import workloads as W
+    out = {}
+    for name, prec, pc in (("plain_fp64", "fp64", "jacobi"), ("mixed64acc_two_level", "mixed64acc", "two_level")):
+        ana, kw = W.footing(n=args.footing, settlement=args.footing_settlement)
+        if args.footing_autoinc:
+            kw = dict(kw, autoinc=True)
+        ana.model.configure_dofs(ana.stages[0].bcs)
+        ana.dev = am.DeviceModel(ana.model)
+        if pc == "two_level":
+            ana.dev.set_coarse()
+        t0 = time.perf_counter()
+        ok, msg = am.solve(ana, precision=prec, precond=pc, pcg_rtol=args.rtol, **kw)
+        dt = time.perf_counter() - t0
+        if not ok:
+            raise RuntimeError(f"footing {name}: {msg}")
+        out[name] = {"value": ana.stats["dof_solves"] / dt, "seconds": dt, "nr_iterations": ana.stats["nr_iterations"],
+                     "pcg_iterations": ana.stats["pcg_iterations"], "increments": ana.stats["increments"],
+                     "plastic_ips": int((ana.model.state[:, 13] != 0).sum())}
+        out[name + "_U"] = ana.model.uvals.copy()
+        log(f"footing {name}: {dt:.2f}s, {out[name]['value']:.4g} DOF-solves/s, NR {ana.stats['nr_iterations']}, "
+            f"PCG {ana.stats['pcg_iterations']}")
+        ana.dev.close()
+    err = rel_err(out.pop("mixed64acc_two_level_U"), out.pop("plain_fp64_U"))
+    nu = ana.model.nu
+    return {"workload": f"Drucker-Prager footing, soil 10x10x5 m, {2 * args.footing}x{2 * args.footing}x"
+                        f"{args.footing} Hex8, 2x2 m footing settled {args.footing_settlement} m, 20 increments"
+                        + (" (auto-increment)" if args.footing_autoinc else ""),
+            "unknowns": nu, **out, "rel_err_mixed_vs_plain": err,
+            "speedup_vs_plain_fp64": out["mixed64acc_two_level"]["value"] / out["plain_fp64"]["value"]}
+
+
 def served_leg(am, args):
     """Request stream: plain one-at-a-time capacity, then the server at a
     Poisson rate that is a multiple of it."""
@@ -323,6 +358,10 @@ def main():
     ap.add_argument("--batch-plain", type=int, default=0, help="models timed on the plain path (0 = all)")
     ap.add_argument("--batch-direct", type=int, default=6, help="models checked against the FP64 direct solve")
     ap.add_argument("--serve", type=int, default=48, help="requests in the served leg (0 = skip)")
+    ap.add_argument("--footing", type=int, default=30, help="footing leg: n for the (2n x 2n x n) block (0 = skip)")
+    ap.add_argument("--footing-settlement", type=float, default=0.02,
+                    help="0.1 m (the nominal value) collapses the perfectly plastic soil before the end")
+    ap.add_argument("--footing-autoinc", action="store_true")
     ap.add_argument("--serve-seed", type=int, default=1)
     ap.add_argument("--serve-load", type=float, default=4.0,
                     help="Poisson arrival rate as a multiple of the plain path's request rate")
@@ -443,7 +482,8 @@ def main():
                                           f"({threads} OpenMP threads), SuperLU factor+solve "
                                           f"{cb['t_solve']:.2f}s (1 thread), update {cb['t_update']:.2f}s")}
     dev.close()
-    for name, leg, n in (("batched", batched_leg, args.batch), ("served", served_leg, args.serve)):
+    for name, leg, n in (("batched", batched_leg, args.batch), ("served", served_leg, args.serve),
+                         ("footing", footing_leg, args.footing)):
         if n <= 0:
             continue
         try:
