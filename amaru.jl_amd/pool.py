@@ -133,15 +133,32 @@ class Job:
             return True
 
 
-def resume_all(finished, jobs_of):
+def resume_all(finished, jobs_of, catch=False):
     """Resume the jobs of the members a step finished: all NR finishes are
     queued first and then completed, so they share one wait.  Yields
-    (handle, job, done)."""
+    (handle, job, done), or with catch=True (handle, job, done, exception)
+    so that one failing job does not stop the others."""
     jobs = [(h, jobs_of(h)) for h, _ in finished]
+    errs = {}
     for (h, job), (_, si) in zip(jobs, finished):
-        job.resume_begin(si)
+        try:
+            job.resume_begin(si)
+        except Exception as e:
+            if not catch:
+                raise
+            errs[h] = e
     for h, job in jobs:
-        yield h, job, job.resume_end()
+        if h in errs:
+            yield h, job, True, errs[h]
+            continue
+        try:
+            done = job.resume_end()
+        except Exception as e:
+            if not catch:
+                raise
+            yield h, job, True, e
+            continue
+        yield (h, job, done, None) if catch else (h, job, done)
 
 
 def run_analyses(analyses, pool=None, device=0, check_every=32, **kwargs):
@@ -244,17 +261,25 @@ class Server:
     def close(self):
         self.pool.close()
 
-    def _finish(self, req, now):
+    def _finish(self, req, now, exc=None):
+        """Results are on the host (the analysis read them back); free the
+        model and its budget.  With exc the request failed."""
         req.t_done = now
-        req.ok, req.msg = req.job.result
-        req.dof_solves = req.ana.stats["dof_solves"]
-        dev = req.ana.dev
-        self.pool.forget(dev)
-        dev.close()
-        self.in_use -= req.bytes
+        if exc is not None or req.job is None or req.job.result is None:
+            req.ok, req.msg = False, repr(exc) if exc is not None else "request failed"
+        else:
+            req.ok, req.msg = req.job.result
+        req.dof_solves = req.ana.stats["dof_solves"] if req.ana is not None else 0
+        dev = req.ana.dev if req.ana is not None else None
+        if dev is not None and dev.handle:
+            self.pool.forget(dev)
+            dev.close()
+        self.in_use -= req.bytes if req.bytes else req.reserve
 
-    def serve(self, requests, keep_models=False):
-        """Runs every request; returns them with their timings filled in."""
+    def serve(self, requests):
+        """Runs every request; returns them with their timings filled in.
+        A request whose construction or analysis raises fails alone
+        (ok False, msg the exception)."""
         reqs = sorted(requests, key=lambda r: r.t_arrival)
         N = len(reqs)
         ex = concurrent.futures.ThreadPoolExecutor(max_workers=self.workers)
@@ -263,15 +288,21 @@ class Server:
         clock = lambda: time.perf_counter() - t0  # noqa: E731
 
         def host_build(req):
-            ana, kw = req.build()
-            ana.model.configure_dofs(ana.stages[0].bcs)
-            return req, ana, kw, estimate_device_bytes(ana.model)
+            try:
+                ana, kw = req.build()
+                ana.model.configure_dofs(ana.stages[0].bcs)
+                return req, ana, kw, estimate_device_bytes(ana.model), None
+            except Exception as e:
+                return req, None, None, 0, e
 
         def device_build(req):
-            req.ana.dev = DeviceModel(req.ana.model, device=self.device)
-            if req.kwargs.get("precond") == "two_level":
-                req.ana.dev.set_coarse()   # aggregates built here, off the serving loop
-            return req
+            try:
+                req.ana.dev = DeviceModel(req.ana.model, device=self.device)
+                if req.kwargs.get("precond") == "two_level":
+                    req.ana.dev.set_coarse()   # aggregates built here, off the serving loop
+                return req, None
+            except Exception as e:
+                return req, e
 
         nxt = 0
         building, creating = set(), set()
@@ -286,7 +317,11 @@ class Server:
                     nxt += 1
                 for f in [f for f in building if f.done()]:
                     building.discard(f)
-                    req, ana, kw, est = f.result()
+                    req, ana, kw, est, exc = f.result()
+                    if exc is not None:
+                        self._finish(req, clock(), exc)
+                        ndone += 1
+                        continue
                     req.ana, req.kwargs, req.reserve = ana, kw, est
                     admit_q.append(req)
                 # admission: FIFO by free HBM budget and member slots
@@ -304,15 +339,21 @@ class Server:
                     creating.add(ex.submit(device_build, req))
                 for f in [f for f in creating if f.done()]:
                     creating.discard(f)
-                    req = f.result()
-                    req.bytes = req.ana.dev.device_bytes()
-                    self.in_use += req.bytes - req.reserve
-                    self.peak_in_use = max(self.peak_in_use, self.in_use)
-                    req.t_start = clock()
-                    args = solver_args(device=self.device, **req.kwargs)
-                    req.job = Job(req.ana, args, self.pool)
-                    if req.job.start():
-                        self._finish(req, clock())
+                    req, exc = f.result()
+                    if exc is None:
+                        req.bytes = req.ana.dev.device_bytes()
+                        self.in_use += req.bytes - req.reserve
+                        self.peak_in_use = max(self.peak_in_use, self.in_use)
+                        req.t_start = clock()
+                        try:
+                            req.job = Job(req.ana, solver_args(device=self.device, **req.kwargs), self.pool)
+                            done = req.job.start()
+                        except Exception as e:
+                            exc, done = e, True
+                    else:
+                        done = True
+                    if done:
+                        self._finish(req, clock(), exc)
                         ndone += 1
                     else:
                         running[req.ana.dev.handle.value] = req
@@ -320,10 +361,10 @@ class Server:
                     self.steps += 1
                     fin = self.pool.step()
                     reqs_of = {h: running.pop(h) for h, _ in fin}
-                    for h, job, done in resume_all(fin, lambda h: reqs_of[h].job):
+                    for h, job, done, exc in resume_all(fin, lambda h: reqs_of[h].job, catch=True):
                         req = reqs_of[h]
                         if done:
-                            self._finish(req, clock())
+                            self._finish(req, clock(), exc)
                             ndone += 1
                         else:
                             running[req.ana.dev.handle.value] = req

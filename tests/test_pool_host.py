@@ -71,6 +71,15 @@ class FakeDev:
     def increment_reject(self):
         self.calls.append(("reject",))
 
+    def device_bytes(self):
+        return 1000 + self.model.nnodes
+
+    def set_coarse(self, *a):
+        pass
+
+    def close(self):
+        self.closed = True
+
     def stage_get(self, last=False):
         return np.zeros(self.model.ndofs), np.zeros(self.model.ndofs)
 
@@ -81,7 +90,9 @@ class FakeDev:
 class FakePool:
     """Returns members after a member-dependent number of steps."""
 
-    def __init__(self):
+    cap = 64
+
+    def __init__(self, *a, **k):
         self.members = {}
         self.t = 0
         self.attached = set()
@@ -91,6 +102,12 @@ class FakePool:
 
     def detach(self, dev):
         self.attached.discard(dev.handle.value)
+
+    def forget(self, dev):
+        self.attached.discard(dev.handle.value)
+
+    def close(self):
+        pass
 
     def add(self, dev, opts):
         assert dev.handle.value in self.attached
@@ -167,6 +184,42 @@ def test_run_analyses_interleaves_like_alone(amaru, monkeypatch):
     for a, b in zip(alone, pooled):
         assert a.dev.calls == [c for c in b.dev.calls if c != ("finish_begin",)]
         assert a.stats == b.stats
+
+
+def test_server_isolates_failures_and_returns_budget(amaru, monkeypatch):
+    """A request whose construction raises fails alone; the others finish
+    with the results of solving alone; the admitted bytes return to zero."""
+    from amaru_amd import analysis, pool as P
+    monkeypatch.setattr(analysis, "DeviceModel", FakeDev)
+    monkeypatch.setattr(P, "DeviceModel", FakeDev)
+    monkeypatch.setattr(P, "Pool", FakePool)
+    alone = analyses(amaru, 4)
+    ref = [amaru.solve(a) for a in alone]
+
+    def builder(i):
+        def build():
+            if i == 2:
+                raise ValueError("bad request")
+            return analyses(amaru, 4)[i], {}
+        return build
+    reqs = [P.Request(i, 0.001 * i, builder(i)) for i in range(4)]
+    est = max(P.estimate_device_bytes(a.model) for a in alone)
+    budget = int(2.5 * est)   # room for two models at once
+    srv = P.Server(hbm_budget=budget, workers=2)
+    done = srv.serve(reqs)
+    bad = [r for r in done if r.rid == 2][0]
+    assert not bad.ok and "bad request" in bad.msg
+    for r in done:
+        if r.rid != 2:
+            assert (r.ok, r.msg) == ref[r.rid]
+            # the server creates the device model before the analysis starts, so
+            # stage 1 re-checks the DOF map once (a no-op on the device)
+            calls = [c for c in r.ana.dev.calls if c != ("finish_begin",)]
+            assert calls[0] == ("set_dofs",) and calls[1:] == alone[r.rid].dev.calls
+            assert r.ana.dev.closed
+            assert r.t_arrival <= r.t_admit <= r.t_start <= r.t_done
+    assert srv.in_use == 0
+    assert est < srv.peak_in_use <= budget
 
 
 def test_cantilever_batch_sizes(amaru):
