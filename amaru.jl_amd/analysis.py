@@ -25,7 +25,8 @@ EPS = 2.220446049250313e-16
 SOLVER_DEFAULTS = dict(tol=0.01, rspan=0.01, rtol=0.01, dT0=0.01, dTmin=1e-7, dTmax=0.1, scheme="FE", maxits=5,
                        autoinc=False, quiet=True,
                        # device options (additions of this implementation)
-                       device=0, precision="fp64", precond="jacobi", pcg_rtol=1e-10, pcg_maxiter=200000)
+                       device=0, precision="fp64", precond="jacobi", pcg_rtol=1e-10, pcg_maxiter=200000,
+                       pcg_replace_ratio=0.0)
 
 
 class Stage:
@@ -124,7 +125,7 @@ def _stage_solver(ana, stage, a):
     Uex, Fex = model.bc_vals(bcs)
     dev.stage_begin(Uex, Fex)
     ana.opts = DeviceModel.opts(precision=a["precision"], rtol=a["pcg_rtol"], maxiter=a["pcg_maxiter"],
-                                precond=a["precond"])
+                                precond=a["precond"], replace_ratio=a["pcg_replace_ratio"])
     # eq id -> flat (node, dir) index; dof.vals start values of this stage
     flat = np.zeros(ndofs, dtype=np.int64)
     live = model.eq.ravel() >= 0

@@ -155,6 +155,8 @@ struct amr_model {
     // stage_begin's host vectors stay alive until their copies are done
     std::vector<double> sb_host[2];
     hipEvent_t sb_event = nullptr;
+    hipEvent_t ev_handoff = nullptr;   // orders the switch between its own and the pool's stream
+    hipEvent_t fin_event = nullptr;    // end of the queued NR finish (its read-back)
     bool assembled = false;
     // device-resident stage vectors (node order): Uex Fex U F dFex R Rc dUa dUi dFin dUt x
     double *sv[12] = {};
@@ -209,6 +211,9 @@ int amr_pool_count(const amr_pool *p);
 bool amr_pool_is_member(const amr_model *m);  // solving in its shared pool right now
 int amr_pool_join(amr_pool *p, amr_model *m, const double *b, double *x, const amr_solve_opts *o);
 int amr_pool_run(amr_pool *p, std::vector<amr_pool_result> &out);
+int amr_pool_step_launch(amr_pool *p);
+int amr_pool_step_collect(amr_pool *p, std::vector<amr_pool_result> &out);
+int amr_pool_step_members(const amr_pool *p);  // members of the step in flight (0: none)
 int amr_pool_set_timing(amr_pool *p, int enable, double *ms, int64_t *n);
 int amr_spmv_full(amr_model *m, const double *x, double *y, int mask_mode);
 int amr_autotune(amr_model *m);

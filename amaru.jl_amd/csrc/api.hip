@@ -3,6 +3,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <mutex>
 
 #include "amr_internal.h"
 
@@ -198,6 +199,8 @@ static void free_model(amr_model *m)
         if (e) (void)hipEventDestroy(e);
     if (m->fin_h) (void)hipHostFree(m->fin_h);
     if (m->sb_event) (void)hipEventDestroy(m->sb_event);
+    if (m->ev_handoff) (void)hipEventDestroy(m->ev_handoff);
+    if (m->fin_event) (void)hipEventDestroy(m->fin_event);
     if (m->own_stream) (void)hipStreamDestroy(m->own_stream);
     delete m;
 }
@@ -405,16 +408,47 @@ extern "C" int amr_state_set(amr_model *m, const double *st)
     return AMR_OK;
 }
 
+// IP states live field-major on the device (field f of IP i at f * n + i);
+// the host layout is IP-major (i * AMR_NSTATE + f).  The read-back
+// transposes on the device into a staging buffer, one per device, grown on
+// demand and kept until the process ends, and copies it out in one piece.
+__global__ void state_aos_kernel(int64_t n, const double *__restrict__ soa, double *__restrict__ aos)
+{
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n * AMR_NSTATE) return;
+    const int64_t i = k / AMR_NSTATE;
+    const int f = (int)(k - i * AMR_NSTATE);
+    aos[k] = soa[(int64_t)f * n + i];
+}
+
+static std::mutex g_stage_mu;
+static double *g_stage[64] = {};
+static size_t g_stage_cap[64] = {};
+
 extern "C" int amr_state_get(amr_model *m, double *st, int committed)
 {
     if (!m || !st) return AMR_ERR_ARG;
+    if (m->device < 0 || m->device >= 64) return AMR_ERR_ARG;
     AMR_HIP(hipSetDevice(m->device));
     const int64_t n = m->nip_total;
-    std::vector<double> soa((size_t)n * AMR_NSTATE);
-    TRY(d2h(soa.data(), committed ? m->st_com : m->st_cur, soa.size(), m->stream));
+    const size_t cnt = (size_t)n * AMR_NSTATE;
+    if (cnt == 0) return AMR_OK;
+    // the staging buffer is shared by the device's models: one read-back at a time
+    std::lock_guard<std::mutex> lk(g_stage_mu);
+    double *&buf = g_stage[m->device];
+    if (g_stage_cap[m->device] < cnt) {
+        if (buf) AMR_HIP(hipFree(buf));
+        buf = nullptr;
+        g_stage_cap[m->device] = 0;
+        AMR_HIP(hipMalloc((void **)&buf, cnt * sizeof(double)));
+        g_stage_cap[m->device] = cnt;
+    }
+    const unsigned g = (unsigned)((cnt + 255) / 256);
+    hipLaunchKernelGGL(state_aos_kernel, dim3(g), dim3(256), 0, m->stream, n, committed ? m->st_com : m->st_cur,
+                       buf);
+    AMR_HIP(hipGetLastError());
+    TRY(d2h(st, buf, cnt, m->stream));
     AMR_HIP(hipStreamSynchronize(m->stream));
-    for (int64_t i = 0; i < n; i++)
-        for (int f = 0; f < AMR_NSTATE; f++) st[i * AMR_NSTATE + f] = soa[f * n + i];
     return AMR_OK;
 }
 
@@ -770,6 +804,10 @@ extern "C" int amr_nr_finish_begin(amr_model *m, const amr_solve_info *si)
         TRY(amr_launch_update(m, v[SV_DUT], v[SV_DFIN]));
         TRY(amr_vec_nr_post_async(m, m->fin_h->s, m->fin_h->c));
         AMR_HIP(hipMemcpyAsync(m->fin_h->err, m->err, 3 * sizeof(int32_t), hipMemcpyDeviceToHost, m->stream));
+        // waited for by amr_nr_finish_end: whatever is queued on the stream
+        // after this point (another pool step) is not waited for
+        if (!m->fin_event) AMR_HIP(hipEventCreateWithFlags(&m->fin_event, hipEventDisableTiming));
+        AMR_HIP(hipEventRecord(m->fin_event, m->stream));
     }
     m->fin_info = inf;
     m->fin_rc = rc;
@@ -790,7 +828,7 @@ extern "C" int amr_nr_finish_end(amr_model *m, amr_nr_info *info)
         amr_set_error(buf);
     } else if (rc == AMR_OK) {
         AMR_HIP(hipSetDevice(m->device));
-        AMR_HIP(hipStreamSynchronize(m->stream));
+        AMR_HIP(hipEventSynchronize(m->fin_event));
         int i_res, i_errn, i_errd, i_nan;
         amr_nr_post_slots(&i_res, &i_errn, &i_errd, &i_nan);
         const double umax = m->fin_h->s[i_errn], uamax = m->fin_h->s[i_errd];
@@ -942,6 +980,17 @@ extern "C" int amr_pool_add(amr_pool *p, amr_model *m, const amr_solve_opts *opt
     return amr_pool_join(p, m, m->v[10], m->sv[SV_X], &o);
 }
 
+static int copy_results(const std::vector<amr_pool_result> &res, int *nfinished, amr_model **models,
+                        amr_solve_info *infos)
+{
+    *nfinished = (int)res.size();
+    for (size_t i = 0; i < res.size(); i++) {
+        models[i] = res[i].m;
+        infos[i] = res[i].info;
+    }
+    return AMR_OK;
+}
+
 extern "C" int amr_pool_step(amr_pool *p, int capacity, int *nfinished, amr_model **models, amr_solve_info *infos)
 {
     if (!p || !nfinished) return AMR_ERR_ARG;
@@ -951,11 +1000,27 @@ extern "C" int amr_pool_step(amr_pool *p, int capacity, int *nfinished, amr_mode
     }
     std::vector<amr_pool_result> res;
     const int rc = amr_pool_run(p, res);
-    *nfinished = (int)res.size();
-    for (size_t i = 0; i < res.size(); i++) {
-        models[i] = res[i].m;
-        infos[i] = res[i].info;
+    copy_results(res, nfinished, models, infos);
+    return rc;
+}
+
+extern "C" int amr_pool_step_begin(amr_pool *p)
+{
+    if (!p) return AMR_ERR_ARG;
+    return amr_pool_step_launch(p);
+}
+
+extern "C" int amr_pool_step_end(amr_pool *p, int capacity, int *nfinished, amr_model **models,
+                                 amr_solve_info *infos)
+{
+    if (!p || !nfinished) return AMR_ERR_ARG;
+    if (capacity < amr_pool_step_members(p) || (capacity > 0 && (!models || !infos))) {
+        amr_set_error("amr_pool_step_end: the output arrays must hold every member of the step");
+        return AMR_ERR_ARG;
     }
+    std::vector<amr_pool_result> res;
+    const int rc = amr_pool_step_collect(p, res);
+    copy_results(res, nfinished, models, infos);
     return rc;
 }
 

@@ -132,8 +132,14 @@ __global__ void symmetrize_kernel(int32_t nc, double *__restrict__ A, const int3
     if (i < j) A[(int64_t)j * nc + i] = A[(int64_t)i * nc + j];
 }
 
+// One rocBLAS handle per device, shared by every model.  The handle's
+// device workspace is shared too, so its uses are chained on the device
+// with an event: each factorisation waits for the previous one, whatever
+// stream either ran on.
 std::mutex g_rb_mu;
 rocblas_handle g_rb[64] = {};
+hipEvent_t g_rb_ev[64] = {};
+bool g_rb_used[64] = {};
 
 int rb_handle(int device, rocblas_handle *h)
 {
@@ -148,6 +154,10 @@ int rb_handle(int device, rocblas_handle *h)
             return AMR_ERR_HIP;
         }
         rocblas_set_atomics_mode(g_rb[device], rocblas_atomics_not_allowed);
+        if (hipEventCreateWithFlags(&g_rb_ev[device], hipEventDisableTiming) != hipSuccess) {
+            amr_set_error("hipEventCreate failed for the rocBLAS handle");
+            return AMR_ERR_HIP;
+        }
     }
     *h = g_rb[device];
     return AMR_OK;
@@ -254,12 +264,15 @@ int amr_coarse_setup(amr_model *m)
     if (int rc = rb_handle(m->device, &h)) return rc;
     {
         std::lock_guard<std::mutex> lk(g_rb_mu);
+        if (g_rb_used[m->device]) AMR_HIP(hipStreamWaitEvent(s, g_rb_ev[m->device], 0));
         if (rocblas_set_stream(h, s) != rocblas_status_success ||
             rocsolver_dpotrf(h, rocblas_fill_lower, nc, m->Kinv, nc, m->c_info) != rocblas_status_success ||
             rocsolver_dpotri(h, rocblas_fill_lower, nc, m->Kinv, nc, m->c_info + 1) != rocblas_status_success) {
             amr_set_error("rocSOLVER factorisation of the coarse matrix failed to launch");
             return AMR_ERR_HIP;
         }
+        AMR_HIP(hipEventRecord(g_rb_ev[m->device], s));
+        g_rb_used[m->device] = true;
     }
     hipLaunchKernelGGL(symmetrize_kernel, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, s, nc, m->Kinv,
                        m->c_info);

@@ -820,7 +820,11 @@ struct Member {
     double rr_rec = 0.0;
 };
 
-enum { M_FREE = 0, M_JOIN, M_ACTIVE };
+// M_WAIT: joined, but work the model queued on its own stream before
+// joining (assembly, coarse operator) may still be running; it starts at the
+// first step that finds that work done, so that a slow joiner does not hold
+// up the members already solving
+enum { M_FREE = 0, M_JOIN, M_ACTIVE, M_WAIT };
 
 }  // namespace
 
@@ -838,7 +842,17 @@ struct amr_pool {
     char *dL = nullptr, *hL = nullptr;
     size_t lcap = 0, lused = 0;
     bool pending = false;  // an upload from the pinned mirrors may still be in flight
+    hipEvent_t ev_upload = nullptr;  // recorded after each upload
     bool desc_dirty = false;
+    // a step launched by amr_pool_step_begin and not yet collected
+    bool in_flight = false;
+    hipEvent_t ev_step = nullptr;    // recorded after the step's scalar read-back
+    std::vector<int> fl_act;         // members of the in-flight step
+    // models that left the pool at the last step and still run on its
+    // stream; they return to their own streams when the next step is queued,
+    // so that the work queued for them in between (the NR finish) runs ahead
+    // of that step
+    std::vector<amr_model *> leaving;
     // SpMV timing
     bool timing = false;
     std::vector<hipEvent_t> ev;
@@ -855,10 +869,13 @@ int pool_sync(amr_pool *p)
     return AMR_OK;
 }
 
-// Make the pinned mirrors writable: wait for any upload still reading them.
+// Make the pinned mirrors writable: wait for any upload still reading them
+// (only for the copy, not for the kernels queued after it).
 int pool_begin_host_writes(amr_pool *p)
 {
-    if (p->pending) return pool_sync(p);
+    if (!p->pending) return AMR_OK;
+    AMR_HIP(hipEventSynchronize(p->ev_upload));
+    p->pending = false;
     return AMR_OK;
 }
 
@@ -905,7 +922,22 @@ int upload(amr_pool *p, size_t bytes, bool desc)
         AMR_HIP(hipMemcpyAsync(p->dD, p->hD, sizeof(SegDesc) * p->cap, hipMemcpyHostToDevice, p->stream));
         p->desc_dirty = false;
     }
-    if (bytes || desc) p->pending = true;
+    if (bytes || desc) {
+        AMR_HIP(hipEventRecord(p->ev_upload, p->stream));
+        p->pending = true;
+    }
+    return AMR_OK;
+}
+
+// A model's work runs on its own stream, and on the pool's stream while it
+// is a member; the switches are ordered with events, without waiting.
+int handoff(amr_model *m, hipStream_t from, hipStream_t to)
+{
+    if (from == to) return AMR_OK;
+    if (!m->ev_handoff) AMR_HIP(hipEventCreateWithFlags(&m->ev_handoff, hipEventDisableTiming));
+    AMR_HIP(hipEventRecord(m->ev_handoff, from));
+    AMR_HIP(hipStreamWaitEvent(to, m->ev_handoff, 0));
+    m->stream = to;
     return AMR_OK;
 }
 
@@ -1085,6 +1117,8 @@ int amr_pool_new(amr_pool **out, int device, int cap, int check, hipStream_t str
         if ((e = hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking)) != hipSuccess) return fail(e);
         p->own_stream = true;
     }
+    if ((e = hipEventCreateWithFlags(&p->ev_upload, hipEventDisableTiming)) != hipSuccess) return fail(e);
+    if ((e = hipEventCreateWithFlags(&p->ev_step, hipEventDisableTiming)) != hipSuccess) return fail(e);
     if ((e = hipMalloc(&p->dD, sizeof(SegDesc) * cap)) != hipSuccess) return fail(e);
     if ((e = hipHostMalloc(&p->hD, sizeof(SegDesc) * cap, hipHostMallocDefault)) != hipSuccess) return fail(e);
     if ((e = hipMalloc(&p->dS, sizeof(double) * SEG_DBL * cap)) != hipSuccess) return fail(e);
@@ -1115,6 +1149,8 @@ void amr_pool_free(amr_pool *p)
     if (p->hL) (void)hipHostFree(p->hL);
     for (hipEvent_t e : p->ev)
         if (e) (void)hipEventDestroy(e);
+    if (p->ev_upload) (void)hipEventDestroy(p->ev_upload);
+    if (p->ev_step) (void)hipEventDestroy(p->ev_step);
     if (p->own_stream && p->stream) (void)hipStreamDestroy(p->stream);
     delete p;
 }
@@ -1130,10 +1166,7 @@ int amr_pool_attach_model(amr_pool *p, amr_model *m)
         amr_set_error("the model and the pool are on different devices");
         return AMR_ERR_ARG;
     }
-    AMR_HIP(hipStreamSynchronize(m->stream));
-    m->stream = p->stream;
     m->pool = p;
-    if (m->self_pool) m->self_pool->stream = p->stream;
     p->attached.push_back(m);
     return AMR_OK;
 }
@@ -1142,13 +1175,23 @@ int amr_pool_detach_model(amr_model *m)
 {
     amr_pool *p = m->pool;
     if (!p) return AMR_OK;
+    bool member = false;
     for (int s = 0; s < p->cap; s++)
-        if (p->mem[s].m == m) p->mem[s] = Member();
-    AMR_HIP(hipStreamSynchronize(p->stream));
+        if (p->mem[s].m == m) {
+            p->mem[s] = Member();
+            member = true;
+        }
+    if (member || m->stream != m->own_stream) {
+        // its kernels may still be queued on the pool's stream
+        AMR_HIP(hipStreamSynchronize(p->stream));
+        p->fl_act.erase(std::remove_if(p->fl_act.begin(), p->fl_act.end(),
+                                       [&](int s) { return p->mem[s].state == M_FREE; }),
+                        p->fl_act.end());
+        m->stream = m->own_stream;
+    }
     p->attached.erase(std::remove(p->attached.begin(), p->attached.end(), m), p->attached.end());
-    m->stream = m->own_stream;
+    p->leaving.erase(std::remove(p->leaving.begin(), p->leaving.end(), m), p->leaving.end());
     m->pool = nullptr;
-    if (m->self_pool) m->self_pool->stream = m->own_stream;
     return AMR_OK;
 }
 
@@ -1170,7 +1213,7 @@ int amr_pool_count(const amr_pool *p)
 // Join: the model's system K11 x = b starts at the next step.
 int amr_pool_join(amr_pool *p, amr_model *m, const double *b, double *x, const amr_solve_opts *o)
 {
-    if (m->stream != p->stream) {
+    if (m->pool != p && m->self_pool != p) {
         amr_set_error("amr_pool_add: the model is not attached to this pool");
         return AMR_ERR_ARG;
     }
@@ -1203,28 +1246,69 @@ int amr_pool_join(amr_pool *p, amr_model *m, const double *b, double *x, const a
     M.mixed = o->precision == AMR_PREC_MIXED || o->precision == AMR_PREC_MIXED_F64ACC;
     M.f32x = o->precision == AMR_PREC_MIXED;
     if (int rc = pool_begin_host_writes(p)) return rc;
+    // the model's queued work (assembly, right-hand side) precedes its solve
+    if (m->stream != p->stream) {
+        if (!m->ev_handoff) AMR_HIP(hipEventCreateWithFlags(&m->ev_handoff, hipEventDisableTiming));
+        AMR_HIP(hipEventRecord(m->ev_handoff, m->stream));
+        m->stream = p->stream;
+        M.state = M_WAIT;
+    }
     p->mem[slot] = M;
     fill_desc(p, slot);
     p->desc_dirty = true;
     return AMR_OK;
 }
 
-// One step: initialise joining members, run check_every PCG iterations for
+// A step: initialise joining members, run check_every PCG iterations for
 // every member, then residual replacements and convergence decisions.
-// Members that finished (converged or failed) are appended to `out` and
-// leave the pool.
-int amr_pool_run(amr_pool *p, std::vector<amr_pool_result> &out)
+// amr_pool_step_launch queues the iterations and the read-back of the
+// members' scalars without waiting; amr_pool_step_collect waits for them,
+// decides, queues the replacements and resumes, and appends the members
+// that finished (converged or failed) to `out`; they leave the pool, and
+// their models' work returns to their own streams when the next step is
+// queued.  Members that join while a step is in flight start at the next
+// one.
+int amr_pool_step_launch(amr_pool *p)
 {
+    if (p->in_flight) {
+        amr_set_error("amr_pool_step_begin: the previous step has not been collected");
+        return AMR_ERR_ARG;
+    }
     AMR_HIP(hipSetDevice(p->device));
+    // joiners whose own work is done start now; if nobody else is solving,
+    // the waiting ones start too, the pool's stream waiting for them
+    std::vector<int> waiting;
+    bool ready = false;
+    for (int s = 0; s < p->cap; s++) {
+        Member &M = p->mem[s];
+        if (M.state == M_WAIT) {
+            const hipError_t q = hipEventQuery(M.m->ev_handoff);
+            if (q == hipSuccess) M.state = M_JOIN;
+            else if (q == hipErrorNotReady) {
+                (void)hipGetLastError();  // not an error: clear it
+                waiting.push_back(s);
+            } else {
+                AMR_HIP(q);
+            }
+        }
+        ready = ready || M.state == M_JOIN || M.state == M_ACTIVE;
+    }
+    if (!ready)
+        for (int s : waiting) {
+            AMR_HIP(hipStreamWaitEvent(p->stream, p->mem[s].m->ev_handoff, 0));
+            p->mem[s].state = M_JOIN;
+        }
     std::vector<int> join, act;
     for (int s = 0; s < p->cap; s++) {
         if (p->mem[s].state == M_JOIN) join.push_back(s);
-        if (p->mem[s].state != M_FREE) act.push_back(s);
+        if (p->mem[s].state == M_JOIN || p->mem[s].state == M_ACTIVE) act.push_back(s);
     }
+    for (amr_model *m : p->leaving)
+        if (m->stream == p->stream && m->pool == p && !amr_pool_is_member(m))
+            if (int rc = handoff(m, p->stream, m->own_stream)) return rc;
+    p->leaving.clear();
     if (act.empty()) return AMR_OK;
     const int check = p->check;
-    const double kEps = 2.220446049250313e-16, kFloorC = 64.0, kGrow = 1e4;
-    const int kForceEvery = 256;
     hipStream_t st = p->stream;
 
     // (a) lists for the initialisation and the iterations
@@ -1276,7 +1360,26 @@ int amr_pool_run(amr_pool *p, std::vector<amr_pool_result> &out)
     AMR_HIP(hipGetLastError());
     const size_t sbytes = sizeof(double) * SEG_DBL * p->cap;
     AMR_HIP(hipMemcpyAsync(p->hS, p->dS, sbytes, hipMemcpyDeviceToHost, st));
-    if (int rc = pool_sync(p)) return rc;
+    AMR_HIP(hipEventRecord(p->ev_step, st));
+    p->fl_act = act;
+    p->in_flight = true;
+    return AMR_OK;
+}
+
+namespace {
+
+int step_decide(amr_pool *p, std::vector<amr_pool_result> &out)
+{
+    const std::vector<int> act = p->fl_act;
+    const int check = p->check;
+    const double kEps = 2.220446049250313e-16, kFloorC = 64.0, kGrow = 1e4;
+    const int kForceEvery = 256;
+    hipStream_t st = p->stream;
+    const size_t sbytes = sizeof(double) * SEG_DBL * p->cap;
+    std::vector<std::vector<int2>> ent;
+    std::vector<Group> gr;
+    Lists Ls{p};
+    size_t need = 0;
 
     // (b) decisions; residual replacement for members that stopped or whose
     // recursive residual dropped enough (mixed paths)
@@ -1328,6 +1431,7 @@ int amr_pool_run(amr_pool *p, std::vector<amr_pool_result> &out)
     if (rep.empty()) return AMR_OK;
 
     // rt = b - K x in FP64 for the replacing members
+    if (int rc = pool_begin_host_writes(p)) return rc;
     build_groups(p, rep, true, ent, gr);
     std::vector<int32_t> sr(rep.begin(), rep.end());
     need = padded(sr);
@@ -1416,6 +1520,29 @@ int amr_pool_run(amr_pool *p, std::vector<amr_pool_result> &out)
                        Ls.dev_at<int32_t>(o_sc), 2);
     AMR_HIP(hipGetLastError());
     return AMR_OK;
+}
+
+}  // namespace
+
+int amr_pool_step_collect(amr_pool *p, std::vector<amr_pool_result> &out)
+{
+    if (!p->in_flight) return AMR_OK;
+    AMR_HIP(hipSetDevice(p->device));
+    AMR_HIP(hipEventSynchronize(p->ev_step));
+    p->in_flight = false;
+    const size_t first = out.size();
+    const int rc = step_decide(p, out);
+    for (size_t i = first; i < out.size(); i++)
+        if (out[i].m->stream != out[i].m->own_stream) p->leaving.push_back(out[i].m);
+    return rc;
+}
+
+int amr_pool_step_members(const amr_pool *p) { return p->in_flight ? (int)p->fl_act.size() : 0; }
+
+int amr_pool_run(amr_pool *p, std::vector<amr_pool_result> &out)
+{
+    if (int rc = amr_pool_step_launch(p)) return rc;
+    return amr_pool_step_collect(p, out);
 }
 
 int amr_pool_set_timing(amr_pool *p, int enable, double *ms, int64_t *n)

@@ -409,7 +409,7 @@ class DeviceModel:
         _lib.check(_lib.lib().amr_state_set(self._h, _lib.ptr(st)))
 
     def state_get(self, committed=False):
-        st = np.zeros((self.model.ne * self.model.nips, NSTATE))
+        st = np.empty((self.model.ne * self.model.nips, NSTATE))
         _lib.check(_lib.lib().amr_state_get(self._h, _lib.ptr(st), 1 if committed else 0))
         return st
 

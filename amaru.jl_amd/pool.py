@@ -74,14 +74,25 @@ class Pool:
         _lib.check(_lib.lib().amr_pool_size(self._h, ctypes.byref(n)))
         return n.value
 
+    def _results(self, n):
+        return [(self._out_m[i], _lib.SolveInfo.from_buffer_copy(self._out_i[i])) for i in range(n)]
+
     def step(self):
         """One step; returns [(model handle value, SolveInfo)] of members that left."""
         n = ctypes.c_int(0)
         _lib.check(_lib.lib().amr_pool_step(self._h, self.cap, ctypes.byref(n), self._out_m, self._out_i))
-        out = []
-        for i in range(n.value):
-            out.append((self._out_m[i], _lib.SolveInfo.from_buffer_copy(self._out_i[i])))
-        return out
+        return self._results(n.value)
+
+    def step_begin(self):
+        """Queues a step and returns at once; members added before
+        step_end start at the next step."""
+        _lib.check(_lib.lib().amr_pool_step_begin(self._h))
+
+    def step_end(self):
+        """Waits for the step step_begin queued; as step()."""
+        n = ctypes.c_int(0)
+        _lib.check(_lib.lib().amr_pool_step_end(self._h, self.cap, ctypes.byref(n), self._out_m, self._out_i))
+        return self._results(n.value)
 
     def timing(self, enable):
         ms = ctypes.c_double(0.0)
@@ -133,9 +144,11 @@ class Job:
             return True
 
 
-def resume_all(finished, jobs_of, catch=False):
+def resume_all(finished, jobs_of, catch=False, between=None):
     """Resume the jobs of the members a step finished: all NR finishes are
-    queued first and then completed, so they share one wait.  Yields
+    queued first and then completed, so they share one wait; ``between()``
+    runs after the finishes are queued (the next pool step is launched
+    there, so that it overlaps the host work of completing them).  Yields
     (handle, job, done), or with catch=True (handle, job, done, exception)
     so that one failing job does not stop the others."""
     jobs = [(h, jobs_of(h)) for h, _ in finished]
@@ -147,6 +160,8 @@ def resume_all(finished, jobs_of, catch=False):
             if not catch:
                 raise
             errs[h] = e
+    if between is not None:
+        between()
     for h, job in jobs:
         if h in errs:
             yield h, job, True, errs[h]
@@ -161,6 +176,31 @@ def resume_all(finished, jobs_of, catch=False):
         yield (h, job, done, None) if catch else (h, job, done)
 
 
+class _Pipeline:
+    """Keeps one pool step in flight: ``end()`` collects it, and
+    ``relaunch()`` (called by resume_all between queuing and completing the
+    finished members' NR finishes) queues the next one for the members
+    still solving.  Members added while a step is in flight start at the
+    next step."""
+
+    def __init__(self, pool):
+        self.pool = pool
+        self.in_flight = False
+
+    def launch(self):
+        if not self.in_flight and self.pool.size() > 0:
+            self.pool.step_begin()
+            self.in_flight = True
+
+    relaunch = launch
+
+    def end(self):
+        if not self.in_flight:
+            return []
+        self.in_flight = False
+        return self.pool.step_end()
+
+
 def run_analyses(analyses, pool=None, device=0, check_every=32, **kwargs):
     """solve!(ana; kwargs...) for every analysis, their linear solves pooled.
     Returns the list of (success, message)."""
@@ -168,6 +208,7 @@ def run_analyses(analyses, pool=None, device=0, check_every=32, **kwargs):
     own = pool is None
     if own:
         pool = Pool(device, max_members=max(1, len(analyses)), check_every=check_every)
+    pipe = _Pipeline(pool)
     try:
         jobs = {}
         results = [None] * len(analyses)
@@ -178,9 +219,10 @@ def run_analyses(analyses, pool=None, device=0, check_every=32, **kwargs):
             else:
                 jobs[ana.dev.handle.value] = (i, job)
         while jobs:
-            fin = pool.step()
+            pipe.launch()
+            fin = pipe.end()
             idx = {h: jobs.pop(h) for h, _ in fin}
-            for h, job, done in resume_all(fin, lambda h: idx[h][1]):
+            for h, job, done in resume_all(fin, lambda h: idx[h][1], between=pipe.relaunch):
                 i = idx[h][0]
                 if done:
                     results[i] = job.result
@@ -189,6 +231,7 @@ def run_analyses(analyses, pool=None, device=0, check_every=32, **kwargs):
         return results
     finally:
         if own:
+            pipe.end()
             for ana in analyses:
                 if ana.dev is not None:
                     pool.detach(ana.dev)
@@ -257,6 +300,14 @@ class Server:
         self.in_use = 0
         self.peak_in_use = 0
         self.steps = 0
+        # main-loop time accounting (seconds): pool steps, resuming finished
+        # members, starting admitted ones, idle waits
+        self.t_loop = {"step": 0.0, "resume": 0.0, "start": 0.0, "wait": 0.0, "free": 0.0}
+        # finished models whose device memory is released later (a release
+        # can wait for the device), at idle points or when too much waits
+        self._graveyard = []
+        self._graveyard_bytes = 0
+        self.graveyard_limit = 16 << 30
 
     def close(self):
         self.pool.close()
@@ -272,9 +323,20 @@ class Server:
         req.dof_solves = req.ana.stats["dof_solves"] if req.ana is not None else 0
         dev = req.ana.dev if req.ana is not None else None
         if dev is not None and dev.handle:
+            self._graveyard.append(dev)
+            self._graveyard_bytes += req.bytes
+            if self._graveyard_bytes > self.graveyard_limit:
+                self._release()
+        self.in_use -= req.bytes if req.bytes else req.reserve
+
+    def _release(self):
+        ts = time.perf_counter()
+        for dev in self._graveyard:
             self.pool.forget(dev)
             dev.close()
-        self.in_use -= req.bytes if req.bytes else req.reserve
+        self._graveyard.clear()
+        self._graveyard_bytes = 0
+        self.t_loop["free"] += time.perf_counter() - ts
 
     def serve(self, requests):
         """Runs every request; returns them with their timings filled in.
@@ -303,6 +365,13 @@ class Server:
                 return req, None
             except Exception as e:
                 return req, e
+
+        pipe = _Pipeline(self.pool)
+
+        def relaunch():
+            if not pipe.in_flight and self.pool.size() > 0:
+                self.steps += 1
+                pipe.launch()
 
         nxt = 0
         building, creating = set(), set()
@@ -340,6 +409,7 @@ class Server:
                 for f in [f for f in creating if f.done()]:
                     creating.discard(f)
                     req, exc = f.result()
+                    ts = time.perf_counter()
                     if exc is None:
                         req.bytes = req.ana.dev.device_bytes()
                         self.in_use += req.bytes - req.reserve
@@ -357,18 +427,29 @@ class Server:
                         ndone += 1
                     else:
                         running[req.ana.dev.handle.value] = req
-                if self.pool.size() > 0:
-                    self.steps += 1
-                    fin = self.pool.step()
+                    self.t_loop["start"] += time.perf_counter() - ts
+                if pipe.in_flight:
+                    # the step queued last time has run beside the host work above
+                    ts = time.perf_counter()
+                    fin = pipe.end()
+                    self.t_loop["step"] += time.perf_counter() - ts
+                    ts = time.perf_counter()
                     reqs_of = {h: running.pop(h) for h, _ in fin}
-                    for h, job, done, exc in resume_all(fin, lambda h: reqs_of[h].job, catch=True):
+                    for h, job, done, exc in resume_all(fin, lambda h: reqs_of[h].job, catch=True,
+                                                        between=relaunch):
                         req = reqs_of[h]
                         if done:
                             self._finish(req, clock(), exc)
                             ndone += 1
                         else:
                             running[req.ana.dev.handle.value] = req
+                    self.t_loop["resume"] += time.perf_counter() - ts
+                elif self.pool.size() > 0:
+                    relaunch()
                 else:
+                    if self._graveyard:
+                        self._release()
+                    ts = time.perf_counter()
                     pend = building | creating
                     wait = (reqs[nxt].t_arrival - clock()) if nxt < N else None
                     if pend:
@@ -376,8 +457,13 @@ class Server:
                                                 return_when=concurrent.futures.FIRST_COMPLETED)
                     elif wait is not None and wait > 0:
                         time.sleep(wait)
+                    self.t_loop["wait"] += time.perf_counter() - ts
         finally:
-            ex.shutdown(wait=True)
+            try:
+                pipe.end()
+            finally:
+                ex.shutdown(wait=True)
+                self._release()
         return reqs
 
 

@@ -242,8 +242,11 @@ int amr_pool_create(amr_pool **out, int device, int max_members, int check_every
 /* Detaches every attached model (they keep working on their own streams). */
 int amr_pool_destroy(amr_pool *p);
 
-/* Moves the model's work onto the pool's stream (waiting for its own
- * first); amr_model_destroy detaches a model by itself. */
+/* Registers the model with the pool.  While it is a member (from
+ * amr_pool_add until a step returns it) its device work runs on the pool's
+ * stream, otherwise on its own; both switches are ordered with events, so
+ * neither waits on the host.  amr_model_destroy detaches a model by
+ * itself. */
 int amr_pool_attach(amr_pool *p, amr_model *m);
 int amr_pool_detach(amr_model *m);
 
@@ -256,6 +259,16 @@ int amr_pool_add(amr_pool *p, amr_model *m, const amr_solve_opts *opts);
  * models[0..*nfinished) and infos (arrays of at least capacity entries;
  * capacity >= the pool's member count). */
 int amr_pool_step(amr_pool *p, int capacity, int *nfinished, amr_model **models, amr_solve_info *infos);
+
+/* amr_pool_step in two halves, so that the host can work on other models
+ * while the device runs a step: step_begin queues the step's iterations
+ * and the read-back of the members' residuals and returns; step_end waits
+ * for them, decides, and returns the members that left, as amr_pool_step
+ * (capacity >= the members of that step).  Models added in between start
+ * at the next step; the finished ones' work continues on their own streams
+ * after the step, without waiting for the next one. */
+int amr_pool_step_begin(amr_pool *p);
+int amr_pool_step_end(amr_pool *p, int capacity, int *nfinished, amr_model **models, amr_solve_info *infos);
 
 /* Members currently solving. */
 int amr_pool_size(amr_pool *p, int *members);

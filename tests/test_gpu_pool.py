@@ -121,6 +121,30 @@ def test_run_analyses_vs_direct(amaru):
         ana.dev.close()
 
 
+def test_run_analyses_two_level_bit_identical(amaru):
+    """Pooled two-level solves: every model's coarse operator is factorised
+    on its own stream while other models' solves and factorisations run, and
+    the results are bit-identical to solving each model alone."""
+    from amaru_amd import pool as P, workloads as W
+    params = W.cantilever_params(5, 10)
+    for p in params:
+        p["nelem"] = min(max(p["nelem"], 4000.0), 12000.0)
+    kw = dict(precision="mixed64acc", precond="two_level", pcg_rtol=1e-10)
+    alone = []
+    for p in params:
+        ana = W.cantilever(p)
+        ok, msg = amaru.solve(ana, **kw)
+        assert ok, msg
+        alone.append(ana.model.uvals.copy())
+        ana.dev.close()
+    anas = [W.cantilever(p) for p in params]
+    res = P.run_analyses(anas, **kw)
+    assert all(ok for ok, _ in res), res
+    for ana, ref in zip(anas, alone):
+        assert np.array_equal(ana.model.uvals, ref)
+        ana.dev.close()
+
+
 def test_join_between_steps_is_bit_identical(amaru):
     """A model that joins a running pool later gets the same iterates."""
     from amaru_amd import pool as P, workloads as W

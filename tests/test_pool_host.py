@@ -118,10 +118,19 @@ class FakePool:
         return len(self.members)
 
     def step(self):
+        self.step_begin()
+        return self.step_end()
+
+    def step_begin(self):
+        # members added before step_end start at the next step, as on the device
         self.t += 1
-        out = [(h, object()) for h, (d, t) in self.members.items() if t <= self.t]
+        self.flight = list(self.members.items())
+
+    def step_end(self):
+        out = [(h, object()) for h, (d, t) in self.flight if t <= self.t]
         for h, _ in out:
             del self.members[h]
+        self.flight = []
         return out
 
 
