@@ -67,6 +67,7 @@ SIGNATURES = {
     "amr_timing": (I32, [P, I32, P, P]),
     "amr_set_spmv_lanes": (I32, [P, I32, I32]),
     "amr_get_spmv_lanes": (I32, [P, P]),
+    "amr_tune_spmv_lanes": (I32, [P]),
     "amr_nr_prepare": (I32, [P, I32]),
     "amr_nr_finish": (I32, [P, ctypes.POINTER(SolveInfo), ctypes.POINTER(NRInfo)]),
     "amr_nr_finish_begin": (I32, [P, ctypes.POINTER(SolveInfo)]),

@@ -123,7 +123,7 @@ struct amr_model {
     int nparts = 0;
     hipStream_t stream = nullptr;      // where the model's work runs (a pool's stream while attached)
     hipStream_t own_stream = nullptr;  // the stream created with the model
-    amr_pool *self_pool = nullptr;     // pool of one for amr_pcg_solve and the autotuner
+    amr_pool *self_pool = nullptr;     // pool of one for amr_pcg_solve and amr_tune_lanes
     amr_pool *pool = nullptr;          // shared pool the model is attached to
     int64_t dev_bytes = 0;             // device memory allocated for the model
     std::vector<double> Xh;            // host copy of the coordinates
@@ -167,7 +167,7 @@ struct amr_model {
     int ev_used = 0;
     double t_spmv_ms = 0.0;
     int64_t n_spmv = 0;
-    // SpMV lanes per block row chosen by the autotuner (0 = not tuned yet)
+    // SpMV lanes per block row (0 = not chosen yet)
     int lanes64 = 0, lanes32 = 0, lanes32d = 0;
 };
 
@@ -216,4 +216,5 @@ int amr_pool_step_collect(amr_pool *p, std::vector<amr_pool_result> &out);
 int amr_pool_step_members(const amr_pool *p);  // members of the step in flight (0: none)
 int amr_pool_set_timing(amr_pool *p, int enable, double *ms, int64_t *n);
 int amr_spmv_full(amr_model *m, const double *x, double *y, int mask_mode);
-int amr_autotune(amr_model *m);
+int amr_choose_lanes(amr_model *m);
+int amr_tune_lanes(amr_model *m);

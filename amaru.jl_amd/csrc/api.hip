@@ -931,6 +931,13 @@ extern "C" int amr_set_spmv_lanes(amr_model *m, int lanes_fp64, int lanes_fp32)
     return AMR_OK;
 }
 
+extern "C" int amr_tune_spmv_lanes(amr_model *m)
+{
+    if (!m) return AMR_ERR_ARG;
+    AMR_HIP(hipSetDevice(m->device));
+    return amr_tune_lanes(m);
+}
+
 extern "C" int amr_get_spmv_lanes(amr_model *m, int *out)
 {
     if (!m || !out) return AMR_ERR_ARG;

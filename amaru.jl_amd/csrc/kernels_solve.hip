@@ -180,7 +180,7 @@ __device__ inline void apply_prec(const SegDesc &P, int64_t node, const double r
 // levels above L itself, the levels below L are cross-lane shuffles.
 // Per-row dot contributions go to LDS and are reduced by a fixed tree too.
 // So neither the row sums nor the partials depend on L, and L can be
-// autotuned per model.
+// chosen per model for speed alone.
 
 // Pointers taken from a descriptor are generic; device buffers are all in
 // the global address space, and saying so lets the compiler use global
@@ -1229,7 +1229,7 @@ int amr_pool_join(amr_pool *p, amr_model *m, const double *b, double *x, const a
         amr_set_error("amr_pool_add: the pool is full");
         return AMR_ERR_ARG;
     }
-    if (int rc = amr_autotune(m)) return rc;
+    if (int rc = amr_choose_lanes(m)) return rc;
     if (o->precond == AMR_PRECOND_TWO_LEVEL) {
         if (int rc = amr_coarse_setup(m)) return rc;
         if ((m->ncoarse + 3) / 4 > MAX_CW) {
@@ -1571,22 +1571,27 @@ int self_pool(amr_model *m, amr_pool **out)
     return AMR_OK;
 }
 
-// Heuristic lanes for small matrices, where timing the choices would cost
-// more than it saves: lanes about half the average blocks per row.
-int small_lanes(const amr_model *m)
+// SpMV lanes per block row.  Measured on MI355X: alone, a model's SpMV is
+// fastest with 8 lanes below about 0.75M blocks and with 32 above 4M blocks,
+// the choices in between within 10% of each other (64 lanes never best); in
+// a pool, members with the same lanes and operand kind share one launch per
+// iteration, and one lanes value for all (16) beat the per-model best by
+// 8-10% on both pooled workloads (256 Tet10 cantilevers, 48 Hex8/Tet10
+// boxes).  So: 16 lanes, 32 for the FP64 operator from 0.75M blocks (16
+// lanes cost 20% more there) and for every operator from 4M blocks (the
+// 811,923-DOF Hex20 block: 10% less time than with 16).
+int choose_lanes(const amr_model *m, int kind)
 {
-    const double avg = (double)m->pat.nblk / (double)std::max<int64_t>(1, m->nnodes);
-    return avg <= 12.0 ? 8 : avg <= 40.0 ? 16 : 32;
+    const int64_t nb = m->pat.nblk;
+    return (nb >= 4 * 1024 * 1024 || (kind == K_F64 && nb >= 768 * 1024)) ? 32 : 16;
 }
-
-constexpr int64_t kTuneMinBlocks = 1 << 20;
 
 }  // namespace
 
 // y = K x (mask_mode 0) or mask .* (K x) (mask_mode 1), FP64, node order.
 int amr_spmv_full(amr_model *m, const double *x, double *y, int mask_mode)
 {
-    if (int rc = amr_autotune(m)) return rc;
+    if (int rc = amr_choose_lanes(m)) return rc;
     const Geom g = model_geom(m);
     const size_t lds = tile_lds(g.R, 0);
 #define AMR_SPMV(LN)                                                                                               \
@@ -1603,18 +1608,24 @@ int amr_spmv_full(amr_model *m, const double *x, double *y, int mask_mode)
     return AMR_OK;
 }
 
-// Pick the SpMV lanes-per-row for the three operand kinds (once per model):
-// time each choice on this model's matrix for large models, a heuristic for
-// small ones.  Every choice gives bit-identical results (see row_sum).
-int amr_autotune(amr_model *m)
+// Lanes per row for the three operand kinds, unless set explicitly.  Every
+// choice gives bit-identical results (see row_sum); the choice depends only
+// on the matrix, so it needs no device work.
+int amr_choose_lanes(amr_model *m)
 {
-    if (m->lanes64 && m->lanes32 && m->lanes32d) return AMR_OK;
-    if (m->pat.nblk < kTuneMinBlocks) {
-        const int l = small_lanes(m);
-        if (!m->lanes64) m->lanes64 = l;
-        if (!m->lanes32) m->lanes32 = l;
-        if (!m->lanes32d) m->lanes32d = l;
-        return AMR_OK;
+    if (!m->lanes64) m->lanes64 = choose_lanes(m, K_F64);
+    if (!m->lanes32) m->lanes32 = choose_lanes(m, K_F32);
+    if (!m->lanes32d) m->lanes32d = choose_lanes(m, K_F32D);
+    return AMR_OK;
+}
+
+// Time every lanes choice for the three operand kinds on this model's matrix
+// and keep the fastest (explicit tuning; amr_choose_lanes is the default).
+int amr_tune_lanes(amr_model *m)
+{
+    if (amr_pool_is_member(m)) {
+        amr_set_error("the model is solving in a pool; tune it before adding it");
+        return AMR_ERR_ARG;
     }
     amr_pool *p;
     if (int rc = self_pool(m, &p)) return rc;
@@ -1648,7 +1659,6 @@ int amr_autotune(amr_model *m)
     AMR_HIP(hipEventCreate(&e1));
     int *dst[K_NKIND] = {&m->lanes64, &m->lanes32, &m->lanes32d};
     for (int kind = 0; kind < K_NKIND; kind++) {
-        if (*dst[kind]) continue;
         float best = 1e30f;
         int bestL = 64;
         for (int L : LANES) {
@@ -1682,7 +1692,7 @@ int amr_pcg_solve(amr_model *m, const double *b, double *x, const amr_solve_opts
         return AMR_ERR_ARG;
     }
     amr_pool *p;
-    if (int rc = amr_autotune(m)) return rc;
+    if (int rc = amr_choose_lanes(m)) return rc;
     if (int rc = self_pool(m, &p)) return rc;
     p->check = std::max(1, std::min(o->check_every > 0 ? o->check_every : 32, 256));
     p->timing = m->timing;

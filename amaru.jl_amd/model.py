@@ -557,6 +557,10 @@ class DeviceModel:
     def set_spmv_lanes(self, lanes_fp64=0, lanes_fp32=0):
         _lib.check(_lib.lib().amr_set_spmv_lanes(self._h, int(lanes_fp64), int(lanes_fp32)))
 
+    def tune_spmv_lanes(self):
+        """Time every lanes choice on the assembled matrix; keep the fastest."""
+        _lib.check(_lib.lib().amr_tune_spmv_lanes(self._h))
+
     def timing(self, enable):
         import ctypes
         ms = ctypes.c_double(0.0)

@@ -194,11 +194,15 @@ int amr_increment_reject(amr_model *m);
 int amr_stage_get(amr_model *m, double *U, double *F, int last);
 
 /* SpMV lanes per block row (8, 16, 32 or 64) for the FP64 and FP32
- * operators; 0 lets the autotuner choose on the next solve.  Every choice
- * gives bit-identical results. */
+ * operators; 0 restores the default choice, made on the next solve from the
+ * matrix size and blocks per row.  Every choice gives bit-identical
+ * results. */
 int amr_set_spmv_lanes(amr_model *m, int lanes_fp64, int lanes_fp32);
+/* Times every lanes choice on the model's assembled matrix (a few dozen SpMV
+ * launches and waits) and keeps the fastest for each operand kind. */
+int amr_tune_spmv_lanes(amr_model *m);
 /* out[0..2]: lanes chosen for the FP64, FP32 and FP32-storage/FP64-accumulation
- * SpMVs (0 = not tuned yet). */
+ * SpMVs (0 = not chosen yet). */
 int amr_get_spmv_lanes(amr_model *m, int *out);
 
 /* Kernel timing of the PCG SpMV with HIP events on the model's stream:

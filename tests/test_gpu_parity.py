@@ -158,8 +158,9 @@ def test_solve_failure_reports(amaru):
 
 
 def test_spmv_lanes_do_not_change_results(amaru):
-    """Every lanes-per-row choice of the SpMV autotuner gives bit-identical
-    solutions (fixed 64-slot reduction tree), for both precisions."""
+    """Every lanes-per-row choice of the SpMV (set explicitly, the default
+    rule's, or the timed one) gives bit-identical solutions (fixed 64-slot
+    reduction tree), for both precisions."""
     mesh = amaru.Mesh(amaru.Block([[0, 0, 0], [1.0, 1.0, 1.0]], nx=4, ny=4, nz=4, cellshape="HEX20"))
     model = amaru.FEModel(mesh, [("all", amaru.LinearElastic(E=30e9, nu=0.2))], nips=27)
     bcs = [("z==0", amaru.SurfaceBC(ux=0, uy=0, uz=0)), ("z==1", amaru.SurfaceBC(tz=-1e6))]
@@ -169,8 +170,14 @@ def test_spmv_lanes_do_not_change_results(amaru):
     dev.assemble()
     for precision in ("fp64", "mixed", "mixed64acc"):
         outs = []
-        for lanes in (8, 16, 32, 64):
-            dev.set_spmv_lanes(lanes, lanes)
+        for lanes in (8, 16, 32, 64, "default", "tuned"):
+            if lanes == "default":
+                dev.set_spmv_lanes(0, 0)
+            elif lanes == "tuned":
+                dev.tune_spmv_lanes()
+                assert set(dev.spmv_lanes().values()) <= {8, 16, 32, 64}
+            else:
+                dev.set_spmv_lanes(lanes, lanes)
             Ug, Fg = U.copy(), F.copy()
             info = dev.solve_system(Ug, Fg, precision=precision, rtol=1e-10)
             outs.append((Ug, Fg, info["iterations"]))
