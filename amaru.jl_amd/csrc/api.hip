@@ -1077,6 +1077,16 @@ extern "C" int amr_set_coarse(amr_model *m, int target_nodes, int max_aggregates
     return amr_coarse_build(m);
 }
 
+extern "C" int amr_set_coarse_refresh(amr_model *m, int policy)
+{
+    if (!m || policy < AMR_COARSE_PER_DOF_MAP || policy > AMR_COARSE_PER_ASSEMBLY) {
+        amr_set_error("amr_set_coarse_refresh: policy must be AMR_COARSE_PER_DOF_MAP, _PER_INCREMENT or _PER_ASSEMBLY");
+        return AMR_ERR_ARG;
+    }
+    m->coarse_refresh = policy;
+    return AMR_OK;
+}
+
 extern "C" int amr_get_coarse(amr_model *m, int64_t *out)
 {
     if (!m || !out) return AMR_ERR_ARG;

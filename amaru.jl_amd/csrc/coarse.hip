@@ -12,17 +12,16 @@
 //     (R_j: 3x6 rigid-mode matrix of node j, masked), blocks in column order;
 //   pass 2, one thread per (coarse block (a,b), entry): Kc_ab = sum over the
 //     pairs (i in a, b) in ascending i.
-// Kc (dense, n = 6 nagg) is then factored and inverted by rocSOLVER
-// (potrf + potri, rocBLAS with atomics disallowed, so the inverse is
-// reproducible) and symmetrised; each PCG iteration applies Kc^-1 as a dense
-// product.  A coarse matrix that is not positive definite (degenerate
-// aggregates) is replaced by zero, which leaves plain Jacobi.
-#include <rocblas/rocblas.h>
-#include <rocsolver/rocsolver.h>
-
+// Kc (dense, n = 6 nagg <= 2048) is then inverted in place by block
+// Gauss-Jordan elimination without pivoting (Kc is symmetric positive
+// definite, so every pivot block is), 64x64 pivot blocks, three kernels per
+// block step, every sum in a fixed order: the inverse is bit-reproducible
+// whatever else runs on the device.  It is symmetrised by mirroring its upper
+// triangle; each PCG iteration applies Kc^-1 as a dense product.  A coarse
+// matrix with a pivot that is not positive (degenerate aggregates) is
+// replaced by zero, which leaves plain Jacobi.
 #include <algorithm>
 #include <cmath>
-#include <mutex>
 
 #include "amr_internal.h"
 
@@ -116,53 +115,134 @@ __global__ void fix_diag_kernel(int32_t nc, double *__restrict__ Kc, int32_t *__
     if (k < nc && Kc[(int64_t)k * nc + k] == 0.0) Kc[(int64_t)k * nc + k] = 1.0;
 }
 
-// potri leaves the inverse in one triangle (column-major lower); mirror it,
-// or clear everything if the factorisation or inversion failed
+// Mirror the upper triangle of the inverse onto the lower one (the
+// elimination leaves them equal up to rounding), or clear everything if a
+// pivot failed
 __global__ void symmetrize_kernel(int32_t nc, double *__restrict__ A, const int32_t *__restrict__ info)
 {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (int64_t)nc * nc) return;
-    // A[t], t = i*nc + j, is the column-major element (j, i): in the lower
-    // triangle, which potri filled, when i < j
     const int32_t i = (int32_t)(t / nc), j = (int32_t)(t % nc);
-    if (info[0] != 0 || info[1] != 0) {
+    if (info[0] != 0) {
         A[t] = 0.0;
         return;
     }
     if (i < j) A[(int64_t)j * nc + i] = A[(int64_t)i * nc + j];
 }
 
+// ---- block Gauss-Jordan inversion (in place, row-major nc x nc) ----------
+// Step k, pivot block K = [k0, k0 + b):
+//   P    = A_KK^-1                              (gj_pivot, one workgroup)
+//   R    = P A_Kj for j outside K; C = A_iK     (gj_panel, old column panel)
+//   A_ij -= C_i R_j,  A_Kj = R_j,  A_iK = -C_i P,  A_KK = P      (gj_update)
+constexpr int GJB = 64;
+
+__global__ void __launch_bounds__(256) gj_pivot_kernel(int32_t nc, int32_t k0, int32_t b,
+                                                       const double *__restrict__ A, double *__restrict__ P,
+                                                       int32_t *__restrict__ info)
+{
+    __shared__ double S[GJB][GJB + 1];
+    __shared__ double colp[GJB], rowp[GJB];
+    for (int t = threadIdx.x; t < b * b; t += 256) S[t / b][t % b] = A[(int64_t)(k0 + t / b) * nc + k0 + t % b];
+    __syncthreads();
+    for (int p = 0; p < b; p++) {
+        if (threadIdx.x < b) {
+            colp[threadIdx.x] = S[threadIdx.x][p];
+            rowp[threadIdx.x] = S[p][threadIdx.x];
+        }
+        __syncthreads();
+        const double piv = rowp[p];
+        const bool ok = piv > 0.0 && std::isfinite(piv);
+        if (!ok && threadIdx.x == 0) info[0] = 1;
+        const double ip = ok ? 1.0 / piv : 0.0;
+        for (int t = threadIdx.x; t < b * b; t += 256) {
+            const int i = t / b, j = t % b;
+            double v;
+            if (i == p && j == p) v = ip;
+            else if (i == p) v = rowp[j] * ip;
+            else if (j == p) v = -(colp[i] * ip);
+            else v = S[i][j] - colp[i] * (rowp[j] * ip);
+            S[i][j] = v;
+        }
+        __syncthreads();
+    }
+    for (int t = threadIdx.x; t < b * b; t += 256) P[(t / b) * GJB + t % b] = S[t / b][t % b];
+}
+
+// one workgroup per 64-column tile t: R[:, tile] = P A_K,tile (t != k), and
+// C[tile rows, :] = A_tile,K
+__global__ void __launch_bounds__(256) gj_panel_kernel(int32_t nc, int32_t k, int32_t b,
+                                                       const double *__restrict__ A, const double *__restrict__ P,
+                                                       double *__restrict__ R, double *__restrict__ C)
+{
+    __shared__ double Ps[GJB][GJB + 1];
+    __shared__ double As[GJB][GJB + 1];
+    const int t = blockIdx.x, k0 = k * GJB, c0 = t * GJB;
+    const int w = std::min(GJB, nc - c0);  // columns (and rows) of this tile
+    for (int e = threadIdx.x; e < w * b; e += 256) {
+        const int r = e / b, c = e % b;
+        C[(int64_t)(c0 + r) * GJB + c] = A[(int64_t)(c0 + r) * nc + k0 + c];
+    }
+    if (t == k) return;
+    for (int e = threadIdx.x; e < b * b; e += 256) Ps[e / b][e % b] = P[(e / b) * GJB + e % b];
+    for (int e = threadIdx.x; e < b * w; e += 256) As[e / w][e % w] = A[(int64_t)(k0 + e / w) * nc + c0 + e % w];
+    __syncthreads();
+    for (int e = threadIdx.x; e < b * w; e += 256) {
+        const int r = e / w, c = e % w;
+        double acc = 0.0;
+        for (int l = 0; l < b; l++) acc = fma(Ps[r][l], As[l][c], acc);
+        R[(int64_t)r * nc + c0 + c] = acc;
+    }
+}
+
+// one workgroup per 64x64 tile (ti, tj) of A, 4x4 entries per thread
+__global__ void __launch_bounds__(256) gj_update_kernel(int32_t nc, int32_t k, int32_t b, double *__restrict__ A,
+                                                        const double *__restrict__ P, const double *__restrict__ R,
+                                                        const double *__restrict__ C)
+{
+    __shared__ double Cs[GJB][GJB + 1];
+    __shared__ double Bs[GJB][GJB + 1];
+    const int tj = blockIdx.x, ti = blockIdx.y, k0 = k * GJB;
+    const int r0 = ti * GJB, c0 = tj * GJB;
+    const int hr = std::min(GJB, nc - r0), hc = std::min(GJB, nc - c0);
+    const int rr = 4 * (threadIdx.x / 16), cc = 4 * (threadIdx.x % 16);
+    if (ti == k) {
+        // pivot block row: A_KK = P, A_Kj = R_j
+        for (int e = threadIdx.x; e < b * hc; e += 256) {
+            const int r = e / hc, c = e % hc;
+            A[(int64_t)(k0 + r) * nc + c0 + c] = tj == k ? P[r * GJB + c] : R[(int64_t)r * nc + c0 + c];
+        }
+        return;
+    }
+    for (int e = threadIdx.x; e < hr * b; e += 256) Cs[e / b][e % b] = C[(int64_t)(r0 + e / b) * GJB + e % b];
+    if (tj == k)
+        for (int e = threadIdx.x; e < b * b; e += 256) Bs[e / b][e % b] = P[(e / b) * GJB + e % b];
+    else
+        for (int e = threadIdx.x; e < b * hc; e += 256) Bs[e / hc][e % hc] = R[(int64_t)(e / hc) * nc + c0 + e % hc];
+    __syncthreads();
+    double acc[4][4] = {};
+    for (int l = 0; l < b; l++) {
+        double cv[4], bv[4];
+        for (int q = 0; q < 4; q++) {
+            cv[q] = Cs[rr + q][l];
+            bv[q] = Bs[l][cc + q];
+        }
+        for (int x = 0; x < 4; x++)
+            for (int y = 0; y < 4; y++) acc[x][y] = fma(cv[x], bv[y], acc[x][y]);
+    }
+    for (int x = 0; x < 4; x++)
+        for (int y = 0; y < 4; y++) {
+            const int r = rr + x, c = cc + y;
+            if (r >= hr || c >= hc) continue;
+            double *a = A + (int64_t)(r0 + r) * nc + c0 + c;
+            *a = tj == k ? -acc[x][y] : *a - acc[x][y];
+        }
+}
+
 // One rocBLAS handle per device, shared by every model.  The handle's
 // device workspace is shared too, so its uses are chained on the device
 // with an event: each factorisation waits for the previous one, whatever
 // stream either ran on.
-std::mutex g_rb_mu;
-rocblas_handle g_rb[64] = {};
-hipEvent_t g_rb_ev[64] = {};
-bool g_rb_used[64] = {};
-
-int rb_handle(int device, rocblas_handle *h)
-{
-    std::lock_guard<std::mutex> lk(g_rb_mu);
-    if (device < 0 || device >= 64) {
-        amr_set_error("device index out of range");
-        return AMR_ERR_ARG;
-    }
-    if (!g_rb[device]) {
-        if (rocblas_create_handle(&g_rb[device]) != rocblas_status_success) {
-            amr_set_error("rocblas_create_handle failed");
-            return AMR_ERR_HIP;
-        }
-        rocblas_set_atomics_mode(g_rb[device], rocblas_atomics_not_allowed);
-        if (hipEventCreateWithFlags(&g_rb_ev[device], hipEventDisableTiming) != hipSuccess) {
-            amr_set_error("hipEventCreate failed for the rocBLAS handle");
-            return AMR_ERR_HIP;
-        }
-    }
-    *h = g_rb[device];
-    return AMR_OK;
-}
-
 template <typename T>
 int dev_upload(amr_model *m, T **d, const std::vector<T> &h)
 {
@@ -219,6 +299,8 @@ int build(amr_model *m)
     if (int rc = dev_alloc(m, &m->c_vec, nc)) return rc;
     if (int rc = dev_alloc(m, &m->y_vec, nc)) return rc;
     if (int rc = dev_alloc(m, &m->c_info, 2)) return rc;
+    // Gauss-Jordan workspace: P (64 x 64), R (64 x nc), C (nc x 64)
+    if (int rc = dev_alloc(m, &m->c_work, (size_t)GJB * GJB + 2 * (size_t)GJB * nc)) return rc;
     // C's vectors are released on return; the copies read them
     // asynchronously, so wait here
     AMR_HIP(hipStreamSynchronize(m->stream));
@@ -243,8 +325,10 @@ int amr_coarse_setup(amr_model *m)
     // again at the first solve of each load increment (a linear elastic K
     // does not change between assemblies)
     if (m->coarse_version == m->k_version) return AMR_OK;
+    const bool same_inc = m->all_elastic || m->coarse_inc == m->inc_version;
     if (m->coarse_version >= 0 && m->coarse_dofs == m->dof_version &&
-        (m->all_elastic || m->coarse_inc == m->inc_version))
+        (m->coarse_refresh == AMR_COARSE_PER_DOF_MAP ||
+         (m->coarse_refresh == AMR_COARSE_PER_INCREMENT && same_inc)))
         return AMR_OK;
     hipStream_t s = m->stream;
     const int32_t nc = m->ncoarse;
@@ -260,19 +344,17 @@ int amr_coarse_setup(amr_model *m)
                            0, s, m->ncb, nc, m->c_cba, m->c_cbb, m->c_cbptr, m->c_cbslot, m->c_pair, m->Kinv);
     hipLaunchKernelGGL(fix_diag_kernel, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, s, nc, m->Kinv, m->c_info);
     AMR_HIP(hipGetLastError());
-    rocblas_handle h;
-    if (int rc = rb_handle(m->device, &h)) return rc;
     {
-        std::lock_guard<std::mutex> lk(g_rb_mu);
-        if (g_rb_used[m->device]) AMR_HIP(hipStreamWaitEvent(s, g_rb_ev[m->device], 0));
-        if (rocblas_set_stream(h, s) != rocblas_status_success ||
-            rocsolver_dpotrf(h, rocblas_fill_lower, nc, m->Kinv, nc, m->c_info) != rocblas_status_success ||
-            rocsolver_dpotri(h, rocblas_fill_lower, nc, m->Kinv, nc, m->c_info + 1) != rocblas_status_success) {
-            amr_set_error("rocSOLVER factorisation of the coarse matrix failed to launch");
-            return AMR_ERR_HIP;
+        double *P = m->c_work, *R = P + GJB * GJB, *C = R + (size_t)GJB * nc;
+        const int nt = (nc + GJB - 1) / GJB;
+        for (int k = 0; k < nt; k++) {
+            const int b = std::min(GJB, nc - k * GJB);
+            hipLaunchKernelGGL(gj_pivot_kernel, dim3(1), dim3(256), 0, s, nc, k * GJB, b, m->Kinv, P, m->c_info);
+            hipLaunchKernelGGL(gj_panel_kernel, dim3((unsigned)nt), dim3(256), 0, s, nc, k, b, m->Kinv, P, R, C);
+            hipLaunchKernelGGL(gj_update_kernel, dim3((unsigned)nt, (unsigned)nt), dim3(256), 0, s, nc, k, b, m->Kinv,
+                               P, R, C);
         }
-        AMR_HIP(hipEventRecord(g_rb_ev[m->device], s));
-        g_rb_used[m->device] = true;
+        AMR_HIP(hipGetLastError());
     }
     hipLaunchKernelGGL(symmetrize_kernel, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, s, nc, m->Kinv,
                        m->c_info);
@@ -286,7 +368,7 @@ int amr_coarse_setup(amr_model *m)
 void amr_coarse_free(amr_model *m)
 {
     void *ptrs[] = {m->c_agg, m->c_aggptr, m->c_aggnodes, m->c_prow, m->c_pagg, m->c_cba,  m->c_cbb, m->c_cbptr,
-                    m->c_cbslot, m->c_info, m->c_cent, m->c_pair, m->Kinv, m->c_vec, m->y_vec};
+                    m->c_cbslot, m->c_info, m->c_cent, m->c_pair, m->Kinv, m->c_vec, m->y_vec, m->c_work};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     m->coarse_built = false;

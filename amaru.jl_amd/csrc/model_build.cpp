@@ -51,54 +51,63 @@ int amr_build_pattern(int nn, int64_t nnodes, int64_t ne, const int32_t *conn, A
     P.fsrc.resize(ninc[nnodes]);
     for (size_t i = 0; i < P.fsrc.size(); i++) P.fsrc[i] = inc_e[i] * nn + inc_a[i];
 
-    // block rows: sorted unique neighbours of each node
+    // block rows: sorted unique neighbours of each node (a stamp per node
+    // drops repeats before the sort)
     P.rowptr.assign(nnodes + 1, 0);
-    std::vector<std::vector<int32_t>> nbr(nnodes);
+    P.col.clear();
+    P.col.reserve((size_t)nnodes * 32);
+    P.diag.assign(nnodes, -1);
+    std::vector<int32_t> stamp(nnodes, -1), buf;
     for (int64_t n = 0; n < nnodes; n++) {
-        std::vector<int32_t> &v = nbr[n];
-        v.reserve((ninc[n + 1] - ninc[n]) * nn);
+        buf.clear();
         for (int32_t i = ninc[n]; i < ninc[n + 1]; i++) {
             const int32_t *el = conn + (int64_t)inc_e[i] * nn;
-            v.insert(v.end(), el, el + nn);
+            for (int b = 0; b < nn; b++)
+                if (stamp[el[b]] != (int32_t)n) {
+                    stamp[el[b]] = (int32_t)n;
+                    buf.push_back(el[b]);
+                }
         }
-        std::sort(v.begin(), v.end());
-        v.erase(std::unique(v.begin(), v.end()), v.end());
-        P.rowptr[n + 1] = P.rowptr[n] + (int32_t)v.size();
+        std::sort(buf.begin(), buf.end());
+        auto it = std::lower_bound(buf.begin(), buf.end(), (int32_t)n);
+        if (it != buf.end() && *it == n) P.diag[n] = (int32_t)(P.col.size() + (it - buf.begin()));
+        P.col.insert(P.col.end(), buf.begin(), buf.end());
+        P.rowptr[n + 1] = (int32_t)P.col.size();
     }
     P.nblk = P.rowptr[nnodes];
-    P.col.resize(P.nblk);
-    P.diag.assign(nnodes, -1);
-    for (int64_t n = 0; n < nnodes; n++) {
-        std::copy(nbr[n].begin(), nbr[n].end(), P.col.begin() + P.rowptr[n]);
-        auto it = std::lower_bound(nbr[n].begin(), nbr[n].end(), (int32_t)n);
-        if (it != nbr[n].end() && *it == n) P.diag[n] = P.rowptr[n] + (int32_t)(it - nbr[n].begin());
-        std::vector<int32_t>().swap(nbr[n]);
-    }
-
-    // block contribution lists: pass 1 counts, pass 2 fills, both in
-    // ascending element order so each list is sorted by element
-    auto find_blk = [&](int32_t A, int32_t B) -> int64_t {
-        const int32_t *b = P.col.data() + P.rowptr[A], *e = P.col.data() + P.rowptr[A + 1];
-        return (int64_t)(std::lower_bound(b, e, B) - P.col.data());
-    };
-    P.cptr.assign(P.nblk + 1, 0);
-    std::vector<int64_t> slot((size_t)ne * nn * nn);
-    for (int64_t e = 0; e < ne; e++)
-        for (int a = 0; a < nn; a++)
-            for (int b = 0; b < nn; b++) {
-                int64_t k = find_blk(conn[e * nn + a], conn[e * nn + b]);
-                slot[(e * nn + a) * nn + b] = k;
-                P.cptr[k + 1]++;
-            }
-    for (int64_t k = 0; k < P.nblk; k++) P.cptr[k + 1] += P.cptr[k];
-    P.ncontrib = P.cptr[P.nblk];
     if ((int64_t)ne * nn * nn > INT32_MAX) {
         amr_set_error("model too large for 32-bit element-block ids");
         return AMR_ERR_ARG;
     }
+
+    // block contribution lists, one block row at a time: the element blocks
+    // (e, a, b) of row A come from A's incident elements, visited in
+    // ascending e, and each element adds at most one entry to a block, so
+    // every list comes out sorted by element.  Pass 1 counts, pass 2 fills.
+    std::vector<int32_t> &where = stamp;  // column -> block index within the current row
+    P.cptr.assign(P.nblk + 1, 0);
+    for (int64_t A = 0; A < nnodes; A++) {
+        for (int32_t k = P.rowptr[A]; k < P.rowptr[A + 1]; k++) where[P.col[k]] = k;
+        for (int32_t i = ninc[A]; i < ninc[A + 1]; i++) {
+            const int32_t *el = conn + (int64_t)inc_e[i] * nn;
+            for (int b = 0; b < nn; b++) P.cptr[where[el[b]] + 1]++;
+        }
+    }
+    for (int64_t k = 0; k < P.nblk; k++) P.cptr[k + 1] += P.cptr[k];
+    P.ncontrib = P.cptr[P.nblk];
     P.csrc.resize(P.ncontrib);
-    std::vector<int64_t> pos(P.cptr.begin(), P.cptr.end() - 1);
-    for (int64_t i = 0; i < (int64_t)slot.size(); i++) P.csrc[pos[slot[i]]++] = (int32_t)i;
+    std::vector<int64_t> fill;
+    for (int64_t A = 0; A < nnodes; A++) {
+        const int32_t r0 = P.rowptr[A];
+        fill.assign(P.cptr.begin() + r0, P.cptr.begin() + P.rowptr[A + 1]);
+        for (int32_t k = r0; k < P.rowptr[A + 1]; k++) where[P.col[k]] = k - r0;
+        for (int32_t i = ninc[A]; i < ninc[A + 1]; i++) {
+            const int64_t e = inc_e[i];
+            const int32_t *el = conn + e * nn;
+            const int32_t base = (int32_t)((e * nn + inc_a[i]) * nn);
+            for (int b = 0; b < nn; b++) P.csrc[fill[where[el[b]]]++] = base + b;
+        }
+    }
     return AMR_OK;
 }
 

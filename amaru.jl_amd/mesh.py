@@ -169,24 +169,51 @@ class Mesh:
 
     def facets(self):
         """Outer facets (get_outer_facets, src/mesh/mesh.jl:69-85): facets seen
-        once, first-appearance order.  Returns (owner element, node ids).
-
-        In a conforming mesh two facets that share their three smallest
-        corner ids are the same facet, so those three ids, packed in one
-        int64, identify it (a 1-D sort instead of a sort of rows)."""
+        once, first-appearance order.  Returns (owner element, node ids)."""
         if self._facets is None:
-            F = FACETS[self.cellshape]
-            nf, nfn = F.shape
-            all_f = self.conn[:, F].reshape(-1, nfn)
-            if self.nnodes < (1 << 21):
-                c = np.sort(all_f[:, :3 if nfn == 6 else 4], axis=1)[:, :3].astype(np.int64)
-                _, inv, cnt = np.unique((c[:, 0] << 42) | (c[:, 1] << 21) | c[:, 2], return_inverse=True,
-                                        return_counts=True)
-            else:
-                _, inv, cnt = np.unique(np.sort(all_f, axis=1), axis=0, return_inverse=True, return_counts=True)
-            outer = np.nonzero(cnt[inv.ravel()] == 1)[0]
-            self._facets = (outer // nf, all_f[outer])
+            self._facets = self._outer(None)
         return self._facets
+
+    def facets_where(self, node_mask):
+        """The outer facets whose nodes all satisfy ``node_mask``, in the order
+        of ``facets()``.  A facet and its twin have the same nodes, so testing
+        the mask before the seen-once test keeps exactly the outer facets
+        among those that pass it, without sorting every facet of the mesh."""
+        if self._facets is not None:
+            owners, nodes = self._facets
+            keep = node_mask[nodes].all(axis=1)
+            return owners[keep], nodes[keep]
+        return self._outer(node_mask)
+
+    def _outer(self, node_mask):
+        """Facets seen once among all (node_mask None) or among those whose
+        nodes all pass node_mask.  In a conforming mesh two facets that share
+        their three smallest corner ids are the same facet, so those three
+        ids, packed in one int64, identify it (a 1-D sort instead of a sort
+        of rows)."""
+        F = FACETS[self.cellshape]
+        nf, nfn = F.shape
+        idx = None
+        if node_mask is None:
+            all_f = self.conn[:, F].reshape(-1, nfn)
+        else:
+            # only elements with at least a facet's worth of nodes in the mask
+            # can own such a facet
+            cm = node_mask[self.conn]
+            el = np.nonzero(np.count_nonzero(cm, axis=1) >= nfn)[0]
+            fm = cm[el][:, F].all(axis=2)
+            e_i, f_i = np.nonzero(fm)
+            idx = el[e_i] * nf + f_i   # ascending: el ascending, facets ascending within
+            all_f = self.conn[el[e_i][:, None], F[f_i]]
+        if self.nnodes < (1 << 21):
+            c = np.sort(all_f[:, :3 if nfn == 6 else 4], axis=1)[:, :3].astype(np.int64)
+            _, inv, cnt = np.unique((c[:, 0] << 42) | (c[:, 1] << 21) | c[:, 2], return_inverse=True,
+                                    return_counts=True)
+        else:
+            _, inv, cnt = np.unique(np.sort(all_f, axis=1), axis=0, return_inverse=True, return_counts=True)
+        outer = np.nonzero(cnt[inv.ravel()] == 1)[0]
+        gidx = outer if idx is None else idx[outer]
+        return gidx // nf, all_f[outer]
 
     def edges(self):
         """Unique edges of the outer facets, first-appearance order."""

@@ -261,8 +261,11 @@ class FEModel:
         if bc.kind == "body":
             e = self.elem_filter(flt)
             return e, self.conn[e].ravel()
-        owners, nodes = self.mesh.facets() if bc.kind == "surface" else self.mesh.edges()
         m = self._node_mask(flt)
+        if bc.kind == "surface":
+            owners, nodes = self.mesh.facets_where(m)
+            return (owners, nodes), nodes.ravel()
+        owners, nodes = self.mesh.edges()
         keep = m[nodes].all(axis=1)
         return (owners[keep], nodes[keep]), nodes[keep].ravel()
 
@@ -517,6 +520,13 @@ class DeviceModel:
 
     def set_coarse(self, target_nodes=256, max_aggregates=170):
         _lib.check(_lib.lib().amr_set_coarse(self._h, int(target_nodes), int(max_aggregates)))
+
+    COARSE_REFRESH = {"dof_map": 0, "increment": 1, "assembly": 2}
+
+    def set_coarse_refresh(self, policy="increment"):
+        """When the two-level coarse operator is formed again: "dof_map",
+        "increment" (non-elastic models; the default) or "assembly"."""
+        _lib.check(_lib.lib().amr_set_coarse_refresh(self._h, self.COARSE_REFRESH[policy]))
 
     def coarse_info(self):
         out = np.zeros(2, dtype=np.int64)

@@ -344,7 +344,10 @@ class Server:
         (ok False, msg the exception)."""
         reqs = sorted(requests, key=lambda r: r.t_arrival)
         N = len(reqs)
+        # host construction and device set-up have workers of their own, so
+        # that admitted requests are not queued behind arrivals' host builds
         ex = concurrent.futures.ThreadPoolExecutor(max_workers=self.workers)
+        ex_dev = concurrent.futures.ThreadPoolExecutor(max_workers=max(1, self.workers // 2))
         lock = threading.Lock()
         t0 = time.perf_counter()
         clock = lambda: time.perf_counter() - t0  # noqa: E731
@@ -405,7 +408,7 @@ class Server:
                         self.in_use += req.reserve
                         self.peak_in_use = max(self.peak_in_use, self.in_use)
                     req.t_admit = clock()
-                    creating.add(ex.submit(device_build, req))
+                    creating.add(ex_dev.submit(device_build, req))
                 for f in [f for f in creating if f.done()]:
                     creating.discard(f)
                     req, exc = f.result()
@@ -463,6 +466,7 @@ class Server:
                 pipe.end()
             finally:
                 ex.shutdown(wait=True)
+                ex_dev.shutdown(wait=True)
                 self._release()
         return reqs
 

@@ -287,6 +287,13 @@ int amr_pool_timing(amr_pool *p, int enable, double *spmv_ms, int64_t *spmv_laun
  * without this call the first two-level solve builds them with the
  * defaults. */
 int amr_set_coarse(amr_model *m, int target_nodes, int max_aggregates);
+/* When the coarse operator Kc = Phi^T K Phi and its inverse are formed
+ * again (at the first two-level solve after an assembly): once per DOF map,
+ * at the first solve of every load increment for models with non-elastic
+ * materials (the default; a linear elastic K does not change), or after
+ * every assembly. */
+enum { AMR_COARSE_PER_DOF_MAP = 0, AMR_COARSE_PER_INCREMENT = 1, AMR_COARSE_PER_ASSEMBLY = 2 };
+int amr_set_coarse_refresh(amr_model *m, int policy);
 /* out[0..1]: aggregates and coarse dofs of the built coarse space (0 if not built). */
 int amr_get_coarse(amr_model *m, int64_t *out);
 /* The built coarse space: each node's aggregate (nnodes) and Kc^-1 as of

@@ -145,6 +145,32 @@ def test_run_analyses_two_level_bit_identical(amaru):
         ana.dev.close()
 
 
+def test_pooled_two_level_is_reproducible(amaru):
+    """The same pooled run three times gives bit-identical solutions and
+    iteration counts, although models assemble, form their coarse operators
+    and finish their NR iterations on their own streams while pool steps
+    run (served-like mix: Hex8/Tet10 boxes, some von Mises)."""
+    from amaru_amd import pool as P, workloads as W
+    params = [p for p in W.request_params(1, 48) if p["dofs"] < 1.2e5][:16]
+    runs = []
+    for rep in range(3):
+        anas = []
+        for p in params:
+            ana, _ = W.box_request(p)
+            ana.model.configure_dofs(ana.stages[0].bcs)
+            ana.dev = amaru.DeviceModel(ana.model)
+            ana.dev.set_coarse()
+            anas.append(ana)
+        res = P.run_analyses(anas, precision="mixed64acc", precond="two_level", pcg_rtol=1e-10)
+        assert all(ok for ok, _ in res), res
+        runs.append([(a.model.uvals.copy(), a.stats["pcg_iterations"]) for a in anas])
+        for a in anas:
+            a.dev.close()
+    for other in runs[1:]:
+        for (u0, i0), (u1, i1) in zip(runs[0], other):
+            assert i0 == i1 and np.array_equal(u0, u1)
+
+
 def test_join_between_steps_is_bit_identical(amaru):
     """A model that joins a running pool later gets the same iterates."""
     from amaru_amd import pool as P, workloads as W
