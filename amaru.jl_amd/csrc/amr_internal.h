@@ -137,7 +137,7 @@ struct amr_model {
     bool coarse_built = false;
     int64_t coarse_version = -1;       // k_version the coarse operator was formed from
     int64_t coarse_dofs = -1, coarse_inc = -1;
-    int coarse_refresh = AMR_COARSE_PER_INCREMENT;  // when the coarse operator is formed again
+    int coarse_refresh = AMR_COARSE_PER_DOF_MAP;  // when the coarse operator is formed again
     int32_t nagg = 0, ncoarse = 0, ncb = 0;
     int64_t npairs = 0;
     int32_t *c_agg = nullptr, *c_aggptr = nullptr, *c_aggnodes = nullptr, *c_prow = nullptr, *c_pagg = nullptr;

@@ -273,6 +273,12 @@ class FEModel:
     def configure_dofs(self, bcs):
         """configure_dofs! (src/bc.jl:198-233): DOFs of the active nodes in node
         order (ux,uy,uz), stable partition unknowns-first, eq ids = position."""
+        # the same boundary conditions again (a stage re-configures the DOFs it
+        # was set up with): the numbering cannot have changed
+        dkey = tuple((str(flt), bc.kind, tuple(sorted((k, repr(v)) for k, v in bc.conds.items())))
+                     for flt, bc in bcs)
+        if self.eq is not None and getattr(self, "_dof_key", None) == dkey:
+            return self.ndofs, self.nu
         active = np.nonzero(np.bincount(self.conn.ravel(), minlength=self.nnodes))[0]
         presc = np.zeros((self.nnodes, 3), dtype=bool)
         self._bcsel = []
@@ -290,6 +296,7 @@ class FEModel:
         self.eq = eq.reshape(self.nnodes, 3)
         self.ndofs = int(order.size)
         self.nu = int((~p).sum())
+        self._dof_key = dkey
         return self.ndofs, self.nu
 
     def bc_vals(self, bcs, t=0.0):
@@ -523,9 +530,9 @@ class DeviceModel:
 
     COARSE_REFRESH = {"dof_map": 0, "increment": 1, "assembly": 2}
 
-    def set_coarse_refresh(self, policy="increment"):
-        """When the two-level coarse operator is formed again: "dof_map",
-        "increment" (non-elastic models; the default) or "assembly"."""
+    def set_coarse_refresh(self, policy="dof_map"):
+        """When the two-level coarse operator is formed again: "dof_map" (the
+        default), "increment" (non-elastic models) or "assembly"."""
         _lib.check(_lib.lib().amr_set_coarse_refresh(self._h, self.COARSE_REFRESH[policy]))
 
     def coarse_info(self):

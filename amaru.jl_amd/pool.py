@@ -287,7 +287,7 @@ class Server:
     analysis ends; its linear solves join the shared pool between steps.
     """
 
-    def __init__(self, device=0, max_members=256, check_every=32, hbm_budget=None, workers=4):
+    def __init__(self, device=0, max_members=256, check_every=16, hbm_budget=None, workers=4):
         self.device = device
         self.pool = Pool(device, max_members=max_members, check_every=check_every)
         if hbm_budget is None:

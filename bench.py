@@ -282,7 +282,7 @@ def served_leg(am, args):
     req_rate = len(params) / t_plain
     def serve(arr):
         reqs = [P.Request(i, arr[i], builder(p, "mixed64acc", args.serve_precond)) for i, p in enumerate(params)]
-        srv = P.Server(max_members=args.serve_members, hbm_budget=int(args.serve_hbm_gb * 2 ** 30), workers=8)
+        srv = P.Server(max_members=args.serve_members, hbm_budget=int(args.serve_hbm_gb * 2 ** 30), workers=4)
         try:
             done = srv.serve(reqs)
         finally:
@@ -307,7 +307,7 @@ def served_leg(am, args):
     # a server runs warm: serve the 4 smallest requests once (untimed) so that
     # one-time library initialisation (rocSOLVER's kernels) is not in the run
     small = sorted(range(len(params)), key=lambda i: params[i]["dofs"])[:4]
-    srv = P.Server(max_members=args.serve_members, hbm_budget=int(args.serve_hbm_gb * 2 ** 30), workers=8)
+    srv = P.Server(max_members=args.serve_members, hbm_budget=int(args.serve_hbm_gb * 2 ** 30), workers=4)
     try:
         srv.serve([P.Request(k, 0.0, builder(params[i], "mixed64acc", args.serve_precond))
                    for k, i in enumerate(small)])

@@ -288,9 +288,10 @@ int amr_pool_timing(amr_pool *p, int enable, double *spmv_ms, int64_t *spmv_laun
  * defaults. */
 int amr_set_coarse(amr_model *m, int target_nodes, int max_aggregates);
 /* When the coarse operator Kc = Phi^T K Phi and its inverse are formed
- * again (at the first two-level solve after an assembly): once per DOF map,
- * at the first solve of every load increment for models with non-elastic
- * materials (the default; a linear elastic K does not change), or after
+ * again (at the first two-level solve after an assembly): once per DOF map
+ * (the default: it only preconditions, and the elastic one served plastic
+ * increments with the same or 1% more iterations), at the first solve of
+ * every load increment for models with non-elastic materials, or after
  * every assembly. */
 enum { AMR_COARSE_PER_DOF_MAP = 0, AMR_COARSE_PER_INCREMENT = 1, AMR_COARSE_PER_ASSEMBLY = 2 };
 int amr_set_coarse_refresh(amr_model *m, int policy);
