@@ -26,7 +26,7 @@ SOLVER_DEFAULTS = dict(tol=0.01, rspan=0.01, rtol=0.01, dT0=0.01, dTmin=1e-7, dT
                        autoinc=False, quiet=True,
                        # device options (additions of this implementation)
                        device=0, precision="fp64", precond="jacobi", pcg_rtol=1e-10, pcg_maxiter=200000,
-                       pcg_replace_ratio=0.0)
+                       pcg_replace_ratio=0.0, pcg_check_every=32)
 
 
 class Stage:
@@ -125,7 +125,8 @@ def _stage_solver(ana, stage, a):
     Uex, Fex = model.bc_vals(bcs)
     dev.stage_begin(Uex, Fex)
     ana.opts = DeviceModel.opts(precision=a["precision"], rtol=a["pcg_rtol"], maxiter=a["pcg_maxiter"],
-                                precond=a["precond"], replace_ratio=a["pcg_replace_ratio"])
+                                precond=a["precond"], replace_ratio=a["pcg_replace_ratio"],
+                                check_every=a["pcg_check_every"])
     # eq id -> flat (node, dir) index; dof.vals start values of this stage
     flat = np.zeros(ndofs, dtype=np.int64)
     live = model.eq.ravel() >= 0

@@ -141,32 +141,38 @@ __global__ void __launch_bounds__(256) gj_pivot_kernel(int32_t nc, int32_t k0, i
                                                        const double *__restrict__ A, double *__restrict__ P,
                                                        int32_t *__restrict__ info)
 {
-    __shared__ double S[GJB][GJB + 1];
-    __shared__ double colp[GJB], rowp[GJB];
-    for (int t = threadIdx.x; t < b * b; t += 256) S[t / b][t % b] = A[(int64_t)(k0 + t / b) * nc + k0 + t % b];
-    __syncthreads();
+    // thread: row i, columns j0 .. j0 + 15 of the pivot block, held in
+    // registers; row p and column p are published through LDS each step
+    __shared__ double rowp[GJB], colp[GJB];
+    const int i = threadIdx.x >> 2, j0 = (threadIdx.x & 3) * 16;
+    double v[16];
+    for (int q = 0; q < 16; q++)
+        v[q] = (i < b && j0 + q < b) ? A[(int64_t)(k0 + i) * nc + k0 + j0 + q] : 0.0;
     for (int p = 0; p < b; p++) {
-        if (threadIdx.x < b) {
-            colp[threadIdx.x] = S[threadIdx.x][p];
-            rowp[threadIdx.x] = S[p][threadIdx.x];
-        }
+        if (i == p)
+            for (int q = 0; q < 16; q++) rowp[j0 + q] = v[q];
+        for (int q = 0; q < 16; q++)
+            if (j0 + q == p) colp[i] = v[q];
         __syncthreads();
         const double piv = rowp[p];
         const bool ok = piv > 0.0 && std::isfinite(piv);
         if (!ok && threadIdx.x == 0) info[0] = 1;
         const double ip = ok ? 1.0 / piv : 0.0;
-        for (int t = threadIdx.x; t < b * b; t += 256) {
-            const int i = t / b, j = t % b;
-            double v;
-            if (i == p && j == p) v = ip;
-            else if (i == p) v = rowp[j] * ip;
-            else if (j == p) v = -(colp[i] * ip);
-            else v = S[i][j] - colp[i] * (rowp[j] * ip);
-            S[i][j] = v;
+        const double ci = colp[i];
+        for (int q = 0; q < 16; q++) {
+            const int j = j0 + q;
+            const double rj = rowp[j];
+            double nv;
+            if (i == p && j == p) nv = ip;
+            else if (i == p) nv = rj * ip;
+            else if (j == p) nv = -(ci * ip);
+            else nv = v[q] - ci * (rj * ip);
+            v[q] = nv;
         }
         __syncthreads();
     }
-    for (int t = threadIdx.x; t < b * b; t += 256) P[(t / b) * GJB + t % b] = S[t / b][t % b];
+    for (int q = 0; q < 16; q++)
+        if (i < b && j0 + q < b) P[i * GJB + j0 + q] = v[q];
 }
 
 // one workgroup per 64-column tile t: R[:, tile] = P A_K,tile (t != k), and
@@ -179,20 +185,22 @@ __global__ void __launch_bounds__(256) gj_panel_kernel(int32_t nc, int32_t k, in
     __shared__ double As[GJB][GJB + 1];
     const int t = blockIdx.x, k0 = k * GJB, c0 = t * GJB;
     const int w = std::min(GJB, nc - c0);  // columns (and rows) of this tile
-    for (int e = threadIdx.x; e < w * b; e += 256) {
-        const int r = e / b, c = e % b;
-        C[(int64_t)(c0 + r) * GJB + c] = A[(int64_t)(c0 + r) * nc + k0 + c];
-    }
+    const int r = threadIdx.x >> 2, q0 = (threadIdx.x & 3) * 16;  // 64 rows x 4 quarters
+    if (r < w)
+        for (int q = q0; q < q0 + 16 && q < b; q++) C[(int64_t)(c0 + r) * GJB + q] = A[(int64_t)(c0 + r) * nc + k0 + q];
     if (t == k) return;
-    for (int e = threadIdx.x; e < b * b; e += 256) Ps[e / b][e % b] = P[(e / b) * GJB + e % b];
-    for (int e = threadIdx.x; e < b * w; e += 256) As[e / w][e % w] = A[(int64_t)(k0 + e / w) * nc + c0 + e % w];
+    if (r < b)
+        for (int q = q0; q < q0 + 16; q++) {
+            if (q < b) Ps[r][q] = P[r * GJB + q];
+            if (q < w) As[r][q] = A[(int64_t)(k0 + r) * nc + c0 + q];
+        }
     __syncthreads();
-    for (int e = threadIdx.x; e < b * w; e += 256) {
-        const int r = e / w, c = e % w;
-        double acc = 0.0;
-        for (int l = 0; l < b; l++) acc = fma(Ps[r][l], As[l][c], acc);
-        R[(int64_t)r * nc + c0 + c] = acc;
-    }
+    if (r < b)
+        for (int q = q0; q < q0 + 16 && q < w; q++) {
+            double acc = 0.0;
+            for (int l = 0; l < b; l++) acc = fma(Ps[r][l], As[l][q], acc);
+            R[(int64_t)r * nc + c0 + q] = acc;
+        }
 }
 
 // one workgroup per 64x64 tile (ti, tj) of A, 4x4 entries per thread
@@ -206,19 +214,19 @@ __global__ void __launch_bounds__(256) gj_update_kernel(int32_t nc, int32_t k, i
     const int r0 = ti * GJB, c0 = tj * GJB;
     const int hr = std::min(GJB, nc - r0), hc = std::min(GJB, nc - c0);
     const int rr = 4 * (threadIdx.x / 16), cc = 4 * (threadIdx.x % 16);
+    const int lr = threadIdx.x >> 2, lq = (threadIdx.x & 3) * 16;  // load mapping: 64 rows x 4 quarters
     if (ti == k) {
         // pivot block row: A_KK = P, A_Kj = R_j
-        for (int e = threadIdx.x; e < b * hc; e += 256) {
-            const int r = e / hc, c = e % hc;
-            A[(int64_t)(k0 + r) * nc + c0 + c] = tj == k ? P[r * GJB + c] : R[(int64_t)r * nc + c0 + c];
-        }
+        if (lr < b)
+            for (int c = lq; c < lq + 16 && c < hc; c++)
+                A[(int64_t)(k0 + lr) * nc + c0 + c] = tj == k ? P[lr * GJB + c] : R[(int64_t)lr * nc + c0 + c];
         return;
     }
-    for (int e = threadIdx.x; e < hr * b; e += 256) Cs[e / b][e % b] = C[(int64_t)(r0 + e / b) * GJB + e % b];
-    if (tj == k)
-        for (int e = threadIdx.x; e < b * b; e += 256) Bs[e / b][e % b] = P[(e / b) * GJB + e % b];
-    else
-        for (int e = threadIdx.x; e < b * hc; e += 256) Bs[e / hc][e % hc] = R[(int64_t)(e / hc) * nc + c0 + e % hc];
+    if (lr < hr)
+        for (int c = lq; c < lq + 16 && c < b; c++) Cs[lr][c] = C[(int64_t)(r0 + lr) * GJB + c];
+    if (lr < b)
+        for (int c = lq; c < lq + 16 && c < hc; c++)
+            Bs[lr][c] = tj == k ? P[lr * GJB + c] : R[(int64_t)lr * nc + c0 + c];
     __syncthreads();
     double acc[4][4] = {};
     for (int l = 0; l < b; l++) {

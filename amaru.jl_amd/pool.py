@@ -11,7 +11,9 @@ The per-model control flow is the reference's ``mech_stage_solver!``
 (src/mech/mech-solver.jl:186-492), unchanged: ``analysis_machine`` yields
 at each Newton-Raphson iteration and is resumed with the iteration's result.
 A model's PCG iterates do not depend on what else is in the pool, so every
-result is bit-identical to solving that model alone with ``solve``.
+result is bit-identical to solving that model alone with ``solve`` with the
+same ``pcg_check_every`` as the pool's ``check_every`` (the iterations
+between convergence and precision decisions).
 """
 import collections
 import concurrent.futures

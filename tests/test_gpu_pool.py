@@ -224,7 +224,8 @@ def test_server_same_alone_and_in_mix(amaru):
     alone = []
     for p in params:
         ana, kw = W.box_request(p)
-        ok, msg = amaru.solve(ana, precision="mixed64acc", pcg_rtol=1e-10, **kw)
+        # the pool decides every check_every iterations; alone with the same value
+        ok, msg = amaru.solve(ana, precision="mixed64acc", pcg_rtol=1e-10, pcg_check_every=16, **kw)
         assert ok, msg
         alone.append((ana.model.uvals.copy(), ana.model.fvals.copy(), ana.stats["dof_solves"]))
         ana.dev.close()
@@ -232,7 +233,7 @@ def test_server_same_alone_and_in_mix(amaru):
     reqs = [P.Request(i, arr[i], (lambda p=p: (lambda a, k: (a, dict(k, precision="mixed64acc", pcg_rtol=1e-10)))(
         *W.box_request(p)))) for i, p in enumerate(params)]
     budget = 3 * max(P.estimate_device_bytes(W.box_request(p)[0].model) for p in params)
-    srv = P.Server(max_members=4, hbm_budget=budget, workers=2)
+    srv = P.Server(max_members=4, hbm_budget=budget, workers=2, check_every=16)
     try:
         done = srv.serve(reqs)
     finally:
