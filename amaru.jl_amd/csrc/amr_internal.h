@@ -216,6 +216,7 @@ int amr_pool_run(amr_pool *p, std::vector<amr_pool_result> &out);
 int amr_pool_step_launch(amr_pool *p);
 int amr_pool_step_collect(amr_pool *p, std::vector<amr_pool_result> &out);
 int amr_pool_step_members(const amr_pool *p);  // members of the step in flight (0: none)
+int amr_pool_step_query(amr_pool *p, int *done);
 int amr_pool_set_timing(amr_pool *p, int enable, double *ms, int64_t *n);
 int amr_spmv_full(amr_model *m, const double *x, double *y, int mask_mode);
 int amr_choose_lanes(amr_model *m);

@@ -1017,6 +1017,12 @@ extern "C" int amr_pool_step_begin(amr_pool *p)
     return amr_pool_step_launch(p);
 }
 
+extern "C" int amr_pool_step_done(amr_pool *p, int *done)
+{
+    if (!p || !done) return AMR_ERR_ARG;
+    return amr_pool_step_query(p, done);
+}
+
 extern "C" int amr_pool_step_end(amr_pool *p, int capacity, int *nfinished, amr_model **models,
                                  amr_solve_info *infos)
 {

@@ -1539,6 +1539,21 @@ int amr_pool_step_collect(amr_pool *p, std::vector<amr_pool_result> &out)
 
 int amr_pool_step_members(const amr_pool *p) { return p->in_flight ? (int)p->fl_act.size() : 0; }
 
+int amr_pool_step_query(amr_pool *p, int *done)
+{
+    *done = 1;
+    if (!p->in_flight) return AMR_OK;
+    AMR_HIP(hipSetDevice(p->device));
+    const hipError_t q = hipEventQuery(p->ev_step);
+    if (q == hipErrorNotReady) {
+        (void)hipGetLastError();  // not an error: clear it
+        *done = 0;
+        return AMR_OK;
+    }
+    AMR_HIP(q);
+    return AMR_OK;
+}
+
 int amr_pool_run(amr_pool *p, std::vector<amr_pool_result> &out)
 {
     if (int rc = amr_pool_step_launch(p)) return rc;

@@ -90,6 +90,12 @@ class Pool:
         step_end start at the next step."""
         _lib.check(_lib.lib().amr_pool_step_begin(self._h))
 
+    def step_done(self):
+        """True when step_end would not wait."""
+        d = ctypes.c_int(1)
+        _lib.check(_lib.lib().amr_pool_step_done(self._h, ctypes.byref(d)))
+        return bool(d.value)
+
     def step_end(self):
         """Waits for the step step_begin queued; as step()."""
         n = ctypes.c_int(0)
@@ -203,41 +209,143 @@ class _Pipeline:
         return self.pool.step_end()
 
 
-def run_analyses(analyses, pool=None, device=0, check_every=32, **kwargs):
-    """solve!(ana; kwargs...) for every analysis, their linear solves pooled.
-    Returns the list of (success, message)."""
+class PoolSet:
+    """Several pools on one device, each with its own stream, driven from one
+    host thread.  A model belongs to one pool (the least loaded one, by a
+    work weight, when its analysis starts) for its whole analysis.  Each
+    pool's next step is queued as soon as its previous one is collected, so
+    the pools' steps overlap on the device and a model's NR iterations are
+    not held to the cadence of every other model.  A model's results do not
+    depend on its pool or on the number of pools."""
+
+    def __init__(self, npools=1, device=0, max_members=256, check_every=32, pools=None):
+        self.pools = list(pools) if pools is not None else [
+            Pool(device, max_members=max_members, check_every=check_every) for _ in range(max(1, int(npools)))]
+        self.pipes = [_Pipeline(p) for p in self.pools]
+        self.load = [0.0] * len(self.pools)
+        self.count = [0] * len(self.pools)   # analyses assigned (each has at most one member at a time)
+        self.t_launch = [0.0] * len(self.pools)
+        self.launches = 0
+
+    @property
+    def cap(self):
+        return sum(p.cap for p in self.pools)
+
+    def assign(self, weight):
+        free = [i for i in range(len(self.pools)) if self.count[i] < self.pools[i].cap]
+        if not free:
+            raise RuntimeError("every pool is full")
+        k = min(free, key=lambda i: self.load[i])
+        self.load[k] += weight
+        self.count[k] += 1
+        return k
+
+    def release(self, k, weight):
+        self.load[k] -= weight
+        self.count[k] -= 1
+
+    def size(self):
+        return sum(p.size() for p in self.pools)
+
+    def in_flight(self):
+        return any(p.in_flight for p in self.pipes)
+
+    def _launch(self, k):
+        if not self.pipes[k].in_flight and self.pools[k].size() > 0:
+            self.pipes[k].launch()
+            self.t_launch[k] = time.perf_counter()
+            self.launches += 1
+
+    def launch_idle(self):
+        for k in range(len(self.pools)):
+            self._launch(k)
+
+    def relauncher(self, k):
+        return lambda: self._launch(k)
+
+    def collect(self, block=True):
+        """[(pool index, finished members)] of the pools whose step has
+        finished; with block and none finished, waits for the step queued
+        first."""
+        out = [(k, pipe.end()) for k, pipe in enumerate(self.pipes) if pipe.in_flight and self.pools[k].step_done()]
+        if not out and block:
+            fl = [k for k, pipe in enumerate(self.pipes) if pipe.in_flight]
+            if fl:
+                k = min(fl, key=lambda i: self.t_launch[i])
+                out.append((k, self.pipes[k].end()))
+        return out
+
+    def end_all(self):
+        for pipe in self.pipes:
+            pipe.end()
+
+    def forget(self, dev):
+        for p in self.pools:
+            p.forget(dev)
+
+    def detach(self, dev):
+        for p in self.pools:
+            p.detach(dev)
+
+    def timing(self, enable):
+        """SpMV timing summed over the pools (as Pool.timing)."""
+        ms = n = 0
+        for p in self.pools:
+            a, b = p.timing(enable)
+            ms += a
+            n += b
+        return ms, n
+
+    def close(self):
+        for p in self.pools:
+            p.close()
+
+
+def work_weight(ana):
+    """Relative device work of an analysis, for balancing pools: nodes times
+    load increments."""
+    return float(ana.model.nnodes) * max(1, sum(st.nincs for st in ana.stages))
+
+
+def run_analyses(analyses, pool=None, device=0, check_every=32, npools=1, **kwargs):
+    """solve!(ana; kwargs...) for every analysis, their linear solves pooled
+    (in ``pool``: a Pool or a PoolSet; by default in ``npools`` pools of the
+    library's own).  Returns the list of (success, message)."""
     args = solver_args(device=device, **kwargs)
     own = pool is None
     if own:
-        pool = Pool(device, max_members=max(1, len(analyses)), check_every=check_every)
-    pipe = _Pipeline(pool)
+        ps = PoolSet(npools, device, max_members=max(1, len(analyses)), check_every=check_every)
+    else:
+        ps = pool if isinstance(pool, PoolSet) else PoolSet(pools=[pool])
     try:
         jobs = {}
         results = [None] * len(analyses)
+        order = sorted(range(len(analyses)), key=lambda i: -work_weight(analyses[i]))
+        home = {i: ps.assign(work_weight(analyses[i])) for i in order}
         for i, ana in enumerate(analyses):
-            job = Job(ana, args, pool)
+            job = Job(ana, args, ps.pools[home[i]])
             if job.start():
                 results[i] = job.result
             else:
                 jobs[ana.dev.handle.value] = (i, job)
         while jobs:
-            pipe.launch()
-            fin = pipe.end()
-            idx = {h: jobs.pop(h) for h, _ in fin}
-            for h, job, done in resume_all(fin, lambda h: idx[h][1], between=pipe.relaunch):
-                i = idx[h][0]
-                if done:
-                    results[i] = job.result
-                else:
-                    jobs[job.ana.dev.handle.value] = (i, job)
+            ps.launch_idle()
+            for k, fin in ps.collect(block=True):
+                idx = {h: jobs.pop(h) for h, _ in fin}
+                for h, job, done in resume_all(fin, lambda h: idx[h][1], between=ps.relauncher(k)):
+                    i = idx[h][0]
+                    if done:
+                        results[i] = job.result
+                    else:
+                        jobs[job.ana.dev.handle.value] = (i, job)
         return results
     finally:
+        ps.end_all()
         if own:
-            pipe.end()
             for ana in analyses:
                 if ana.dev is not None:
-                    pool.detach(ana.dev)
-            pool.close()
+                    ps.detach(ana.dev)
+            ps.close()
 
 
 # ------------------------------------------------------------------ serving
@@ -261,14 +369,15 @@ class Request:
     on the host; the analysis runs on the device when admitted."""
 
     __slots__ = ("rid", "t_arrival", "build", "ana", "kwargs", "reserve", "bytes", "t_admit", "t_start", "t_done",
-                 "ok", "msg", "dof_solves", "job")
+                 "ok", "msg", "dof_solves", "job", "home", "weight")
 
     def __init__(self, rid, t_arrival, build):
         self.rid = rid
         self.t_arrival = float(t_arrival)
         self.build = build
-        self.ana = self.kwargs = self.job = None
+        self.ana = self.kwargs = self.job = self.home = None
         self.reserve = self.bytes = 0
+        self.weight = 0.0
         self.t_admit = self.t_start = self.t_done = None
         self.ok = None
         self.msg = None
@@ -289,9 +398,10 @@ class Server:
     analysis ends; its linear solves join the shared pool between steps.
     """
 
-    def __init__(self, device=0, max_members=256, check_every=16, hbm_budget=None, workers=4):
+    def __init__(self, device=0, max_members=256, check_every=16, hbm_budget=None, workers=4, npools=3):
         self.device = device
-        self.pool = Pool(device, max_members=max_members, check_every=check_every)
+        npools = max(1, int(npools))
+        self.pools = PoolSet(npools, device, max_members=-(-int(max_members) // npools), check_every=check_every)
         if hbm_budget is None:
             free = ctypes.c_int64(0)
             total = ctypes.c_int64(0)
@@ -301,7 +411,6 @@ class Server:
         self.workers = int(workers)
         self.in_use = 0
         self.peak_in_use = 0
-        self.steps = 0
         # main-loop time accounting (seconds): pool steps, resuming finished
         # members, starting admitted ones, idle waits
         self.t_loop = {"step": 0.0, "resume": 0.0, "start": 0.0, "wait": 0.0, "free": 0.0}
@@ -312,12 +421,19 @@ class Server:
         self.graveyard_limit = 16 << 30
 
     def close(self):
-        self.pool.close()
+        self.pools.close()
+
+    @property
+    def steps(self):
+        return self.pools.launches
 
     def _finish(self, req, now, exc=None):
         """Results are on the host (the analysis read them back); free the
         model and its budget.  With exc the request failed."""
         req.t_done = now
+        if req.home is not None:
+            self.pools.release(req.home, req.weight)
+            req.home = None
         if exc is not None or req.job is None or req.job.result is None:
             req.ok, req.msg = False, repr(exc) if exc is not None else "request failed"
         else:
@@ -334,7 +450,7 @@ class Server:
     def _release(self):
         ts = time.perf_counter()
         for dev in self._graveyard:
-            self.pool.forget(dev)
+            self.pools.forget(dev)
             dev.close()
         self._graveyard.clear()
         self._graveyard_bytes = 0
@@ -371,13 +487,6 @@ class Server:
             except Exception as e:
                 return req, e
 
-        pipe = _Pipeline(self.pool)
-
-        def relaunch():
-            if not pipe.in_flight and self.pool.size() > 0:
-                self.steps += 1
-                pipe.launch()
-
         nxt = 0
         building, creating = set(), set()
         admit_q = collections.deque()
@@ -401,7 +510,7 @@ class Server:
                 # admission: FIFO by free HBM budget and member slots
                 while admit_q:
                     req = admit_q[0]
-                    slots = self.pool.cap - len(running) - len(creating)
+                    slots = self.pools.cap - len(running) - len(creating)
                     fits = self.in_use + req.reserve <= self.budget or (not running and not creating)
                     if slots <= 0 or not fits:
                         break
@@ -421,7 +530,10 @@ class Server:
                         self.peak_in_use = max(self.peak_in_use, self.in_use)
                         req.t_start = clock()
                         try:
-                            req.job = Job(req.ana, solver_args(device=self.device, **req.kwargs), self.pool)
+                            req.weight = work_weight(req.ana)
+                            req.home = self.pools.assign(req.weight)
+                            req.job = Job(req.ana, solver_args(device=self.device, **req.kwargs),
+                                          self.pools.pools[req.home])
                             done = req.job.start()
                         except Exception as e:
                             exc, done = e, True
@@ -433,24 +545,25 @@ class Server:
                     else:
                         running[req.ana.dev.handle.value] = req
                     self.t_loop["start"] += time.perf_counter() - ts
-                if pipe.in_flight:
-                    # the step queued last time has run beside the host work above
+                if self.pools.size() > 0 or self.pools.in_flight():
+                    # each pool's step has run beside the host work above;
+                    # collect the finished ones (waiting for the oldest if none)
+                    self.pools.launch_idle()
                     ts = time.perf_counter()
-                    fin = pipe.end()
+                    fins = self.pools.collect(block=True)
                     self.t_loop["step"] += time.perf_counter() - ts
                     ts = time.perf_counter()
-                    reqs_of = {h: running.pop(h) for h, _ in fin}
-                    for h, job, done, exc in resume_all(fin, lambda h: reqs_of[h].job, catch=True,
-                                                        between=relaunch):
-                        req = reqs_of[h]
-                        if done:
-                            self._finish(req, clock(), exc)
-                            ndone += 1
-                        else:
-                            running[req.ana.dev.handle.value] = req
+                    for k, fin in fins:
+                        reqs_of = {h: running.pop(h) for h, _ in fin}
+                        for h, job, done, exc in resume_all(fin, lambda h: reqs_of[h].job, catch=True,
+                                                            between=self.pools.relauncher(k)):
+                            req = reqs_of[h]
+                            if done:
+                                self._finish(req, clock(), exc)
+                                ndone += 1
+                            else:
+                                running[req.ana.dev.handle.value] = req
                     self.t_loop["resume"] += time.perf_counter() - ts
-                elif self.pool.size() > 0:
-                    relaunch()
                 else:
                     if self._graveyard:
                         self._release()
@@ -465,7 +578,7 @@ class Server:
                     self.t_loop["wait"] += time.perf_counter() - ts
         finally:
             try:
-                pipe.end()
+                self.pools.end_all()
             finally:
                 ex.shutdown(wait=True)
                 ex_dev.shutdown(wait=True)

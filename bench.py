@@ -164,7 +164,7 @@ def batched_leg(am, args):
         P.run_analyses(warm, precision=prec, precond=pc, pcg_rtol=args.rtol)  # load the kernels
         warm[0].dev.close()
         anas = build(two_level=pc == "two_level")
-        pool = P.Pool(max_members=len(anas), check_every=32)
+        pool = P.PoolSet(args.batch_pools, 0, max_members=len(anas), check_every=32)
         pool.timing(True)
         t2 = time.perf_counter()
         out = P.run_analyses(anas, pool=pool, precision=prec, precond=pc, pcg_rtol=args.rtol)
@@ -282,7 +282,8 @@ def served_leg(am, args):
     req_rate = len(params) / t_plain
     def serve(arr):
         reqs = [P.Request(i, arr[i], builder(p, "mixed64acc", args.serve_precond)) for i, p in enumerate(params)]
-        srv = P.Server(max_members=args.serve_members, hbm_budget=int(args.serve_hbm_gb * 2 ** 30), workers=4)
+        srv = P.Server(max_members=args.serve_members, hbm_budget=int(args.serve_hbm_gb * 2 ** 30), workers=4,
+                       npools=args.serve_pools)
         try:
             done = srv.serve(reqs)
         finally:
@@ -307,7 +308,8 @@ def served_leg(am, args):
     # a server runs warm: serve the 4 smallest requests once (untimed) so that
     # one-time library initialisation (rocSOLVER's kernels) is not in the run
     small = sorted(range(len(params)), key=lambda i: params[i]["dofs"])[:4]
-    srv = P.Server(max_members=args.serve_members, hbm_budget=int(args.serve_hbm_gb * 2 ** 30), workers=4)
+    srv = P.Server(max_members=args.serve_members, hbm_budget=int(args.serve_hbm_gb * 2 ** 30), workers=4,
+                   npools=args.serve_pools)
     try:
         srv.serve([P.Request(k, 0.0, builder(params[i], "mixed64acc", args.serve_precond))
                    for k, i in enumerate(small)])
@@ -367,6 +369,8 @@ def main():
     ap.add_argument("--batch-seed", type=int, default=0)
     ap.add_argument("--batch-plain", type=int, default=0, help="models timed on the plain path (0 = all)")
     ap.add_argument("--batch-direct", type=int, default=6, help="models checked against the FP64 direct solve")
+    ap.add_argument("--batch-pools", type=int, default=3, help="pools stepping concurrently in the batched leg")
+    ap.add_argument("--serve-pools", type=int, default=3, help="pools stepping concurrently in the server")
     ap.add_argument("--serve", type=int, default=48, help="requests in the served leg (0 = skip)")
     ap.add_argument("--footing", type=int, default=30, help="footing leg: n for the (2n x 2n x n) block (0 = skip)")
     ap.add_argument("--footing-settlement", type=float, default=0.02,

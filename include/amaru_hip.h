@@ -273,6 +273,10 @@ int amr_pool_step(amr_pool *p, int capacity, int *nfinished, amr_model **models,
  * after the step, without waiting for the next one. */
 int amr_pool_step_begin(amr_pool *p);
 int amr_pool_step_end(amr_pool *p, int capacity, int *nfinished, amr_model **models, amr_solve_info *infos);
+/* *done = 1 when no step is in flight or the one in flight has finished on
+ * the device (amr_pool_step_end would not wait), else 0.  For hosts that
+ * drive several pools from one thread. */
+int amr_pool_step_done(amr_pool *p, int *done);
 
 /* Members currently solving. */
 int amr_pool_size(amr_pool *p, int *members);

@@ -171,6 +171,23 @@ def test_pooled_two_level_is_reproducible(amaru):
             assert i0 == i1 and np.array_equal(u0, u1)
 
 
+def test_pool_count_does_not_change_results(amaru):
+    """The same analyses pooled in one pool and spread over three pools that
+    step concurrently: bit-identical results and statistics."""
+    from amaru_amd import pool as P
+    runs = []
+    for npools in (1, 3):
+        anas = [a for a, _ in small_set(amaru)]
+        res = P.run_analyses(anas, npools=npools, precision="mixed64acc", precond="two_level", pcg_rtol=1e-10)
+        assert all(ok for ok, _ in res), res
+        runs.append([results(a) for a in anas])
+        for a in anas:
+            a.dev.close()
+    for a, b in zip(*runs):
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2])
+        assert a[3] == b[3]
+
+
 def test_join_between_steps_is_bit_identical(amaru):
     """A model that joins a running pool later gets the same iterates."""
     from amaru_amd import pool as P, workloads as W

@@ -121,6 +121,9 @@ class FakePool:
         self.step_begin()
         return self.step_end()
 
+    def step_done(self):
+        return True
+
     def step_begin(self):
         # members added before step_end start at the next step, as on the device
         self.t += 1
