@@ -368,13 +368,16 @@ class Request:
     """A served request: ``build()`` returns (MechAnalysis, solver kwargs)
     on the host; the analysis runs on the device when admitted."""
 
-    __slots__ = ("rid", "t_arrival", "build", "ana", "kwargs", "reserve", "bytes", "t_admit", "t_start", "t_done",
-                 "ok", "msg", "dof_solves", "job", "home", "weight")
+    __slots__ = ("rid", "t_arrival", "build", "work", "ana", "kwargs", "reserve", "bytes", "t_admit", "t_start",
+                 "t_done", "ok", "msg", "dof_solves", "job", "home", "weight")
 
-    def __init__(self, rid, t_arrival, build):
+    def __init__(self, rid, t_arrival, build, work=None):
         self.rid = rid
         self.t_arrival = float(t_arrival)
         self.build = build
+        # optional client estimate of the request's device work (for example
+        # DOFs x load increments), used by the largest-first order
+        self.work = work
         self.ana = self.kwargs = self.job = self.home = None
         self.reserve = self.bytes = 0
         self.weight = 0.0
@@ -398,7 +401,11 @@ class Server:
     analysis ends; its linear solves join the shared pool between steps.
     """
 
-    def __init__(self, device=0, max_members=256, check_every=16, hbm_budget=None, workers=4, npools=3):
+    def __init__(self, device=0, max_members=256, check_every=16, hbm_budget=None, workers=4, npools=3,
+                 order="fifo"):
+        if order not in ("fifo", "largest_first"):
+            raise ValueError("order must be 'fifo' or 'largest_first'")
+        self.order = order
         self.device = device
         npools = max(1, int(npools))
         self.pools = PoolSet(npools, device, max_members=-(-int(max_members) // npools), check_every=check_every)
@@ -490,14 +497,26 @@ class Server:
         nxt = 0
         building, creating = set(), set()
         admit_q = collections.deque()
+        arrived = []   # arrived, host build not started (largest_first only)
         running = {}
         ndone = 0
+        lpt = self.order == "largest_first"
+        key = lambda r: (-(r.work or 0.0), r.t_arrival, r.rid)  # noqa: E731
         try:
             while ndone < N:
                 now = clock()
                 while nxt < N and reqs[nxt].t_arrival <= now:
-                    building.add(ex.submit(host_build, reqs[nxt]))
+                    if lpt:
+                        arrived.append(reqs[nxt])
+                    else:
+                        building.add(ex.submit(host_build, reqs[nxt]))
                     nxt += 1
+                if lpt and arrived:
+                    # largest first among the arrived ones, one build per worker
+                    # at a time so that a later, larger arrival can still pass
+                    arrived.sort(key=key)
+                    while arrived and len(building) < self.workers:
+                        building.add(ex.submit(host_build, arrived.pop(0)))
                 for f in [f for f in building if f.done()]:
                     building.discard(f)
                     req, ana, kw, est, exc = f.result()
@@ -507,6 +526,8 @@ class Server:
                         continue
                     req.ana, req.kwargs, req.reserve = ana, kw, est
                     admit_q.append(req)
+                if lpt and len(admit_q) > 1:
+                    admit_q = collections.deque(sorted(admit_q, key=key))
                 # admission: FIFO by free HBM budget and member slots
                 while admit_q:
                     req = admit_q[0]
@@ -570,6 +591,8 @@ class Server:
                     ts = time.perf_counter()
                     pend = building | creating
                     wait = (reqs[nxt].t_arrival - clock()) if nxt < N else None
+                    if arrived:
+                        wait = 0.0
                     if pend:
                         concurrent.futures.wait(pend, timeout=wait if wait is not None and wait > 0 else None,
                                                 return_when=concurrent.futures.FIRST_COMPLETED)

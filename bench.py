@@ -280,10 +280,12 @@ def served_leg(am, args):
     t_plain = time.perf_counter() - t0
     plain_rate = ds_plain / t_plain
     req_rate = len(params) / t_plain
-    def serve(arr):
-        reqs = [P.Request(i, arr[i], builder(p, "mixed64acc", args.serve_precond)) for i, p in enumerate(params)]
+    def serve(arr, order):
+        # work hint: DOFs x load increments, as a client would state it
+        reqs = [P.Request(i, arr[i], builder(p, "mixed64acc", args.serve_precond),
+                          work=p["dofs"] * (5 if p["vm"] else 1)) for i, p in enumerate(params)]
         srv = P.Server(max_members=args.serve_members, hbm_budget=int(args.serve_hbm_gb * 2 ** 30), workers=4,
-                       npools=args.serve_pools)
+                       npools=args.serve_pools, order=order)
         try:
             done = srv.serve(reqs)
         finally:
@@ -315,17 +317,20 @@ def served_leg(am, args):
                    for k, i in enumerate(small)])
     finally:
         srv.close()
-    cap = serve(np.zeros(len(params)))
+    # a batch submitted at once is scheduled largest first; a stream in arrival order
+    cap = serve(np.zeros(len(params)), "largest_first")
     log(f"served capacity (all requests at t=0): {cap['value']:.4g} DOF-solves/s, "
         f"{cap['value'] / plain_rate:.2f}x plain")
     rate = args.serve_load * req_rate
-    lat = serve(W.arrival_times(args.serve_seed, len(params), rate))
+    lat = serve(W.arrival_times(args.serve_seed, len(params), rate), "fifo")
     log(f"served at {rate:.3g} requests/s: {lat['value']:.4g} DOF-solves/s, p50 {lat['latency_s']['p50_s']:.3f}s "
         f"p99 {lat['latency_s']['p99_s']:.3f}s")
     out = {"workload": f"{len(params)} requests (seed {args.serve_seed}): Hex8/Tet10 boxes 1e3-3e5 DOFs, 70% "
                        "elastic, 30% von Mises 5 increments",
            "precision": "mixed64acc", "precond": args.serve_precond, "hbm_budget_gb": args.serve_hbm_gb,
            "warmup": "the 4 smallest requests served once, untimed, before the runs",
+           "server": {"pools": args.serve_pools, "check_every": 16, "host_workers": 4,
+                      "order": {"capacity_all_at_t0": "largest_first", "poisson": "fifo"}},
            "value": cap["value"], "speedup_vs_plain_fp64": cap["value"] / plain_rate,
            "capacity_all_at_t0": cap,
            "poisson": dict(lat, arrival_rate_per_s=rate, load_vs_plain_request_rate=args.serve_load),

@@ -235,6 +235,32 @@ def test_server_isolates_failures_and_returns_budget(amaru, monkeypatch):
     assert est <= srv.peak_in_use <= budget
 
 
+def test_server_largest_first_order(amaru, monkeypatch):
+    """order="largest_first": requests waiting together are built (and
+    admitted) in decreasing work hint; results are those of solving alone."""
+    from amaru_amd import analysis, pool as P
+    monkeypatch.setattr(analysis, "DeviceModel", FakeDev)
+    monkeypatch.setattr(P, "DeviceModel", FakeDev)
+    monkeypatch.setattr(P, "Pool", FakePool)
+    built = []
+
+    def builder(i):
+        def build():
+            built.append(i)
+            return analyses(amaru, 4)[i], {}
+        return build
+    works = [1.0, 5.0, 3.0, 4.0]
+    reqs = [P.Request(i, 0.0, builder(i), work=works[i]) for i in range(4)]
+    srv = P.Server(hbm_budget=1 << 40, workers=1, order="largest_first")
+    done = srv.serve(reqs)
+    assert built == [1, 3, 2, 0]
+    ref = [amaru.solve(a) for a in analyses(amaru, 4)]
+    for r in done:
+        assert (r.ok, r.msg) == ref[r.rid]
+    with pytest.raises(ValueError):
+        P.Server(hbm_budget=1, order="random")
+
+
 def test_cantilever_batch_sizes(amaru):
     from amaru_amd import workloads as W
     ps = W.cantilever_params(0, 256)
