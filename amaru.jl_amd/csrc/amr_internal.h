@@ -34,6 +34,14 @@ __host__ __device__ inline int64_t kv_at(int64_t k, int s)
     return s < 8 ? base + (int64_t)(s >> 1) * 128 + (k & 63) * 2 + (s & 1) : base + 512 + (k & 63);
 }
 inline int64_t kv_size(int64_t nblk) { return ((nblk + 63) >> 6) * 576; }
+// The FP32 copy uses the same 64-block chunks with four-value groups
+// (s = 0..3 | 4..7, then s = 8): a lane reads a block with two 16-byte
+// loads and one scalar load.
+__host__ __device__ inline int64_t kv32_at(int64_t k, int s)
+{
+    const int64_t base = (k >> 6) * 576;
+    return s < 8 ? base + (int64_t)(s >> 2) * 256 + (k & 63) * 4 + (s & 3) : base + 512 + (k & 63);
+}
 
 #define AMR_HIP(call)                                                                  \
     do {                                                                               \

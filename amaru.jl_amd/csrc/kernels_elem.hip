@@ -386,7 +386,7 @@ __global__ void __launch_bounds__(256) assemble_kernel(int64_t nblk, const int64
 #pragma unroll
     for (int s = 0; s < 9; s++) {
         Kv[kv_at(k, s)] = acc[s];
-        Kv32[kv_at(k, s)] = (float)acc[s];
+        Kv32[kv32_at(k, s)] = (float)acc[s];
     }
     if (flags) {
 #pragma unroll

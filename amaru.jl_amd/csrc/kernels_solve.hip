@@ -196,27 +196,50 @@ __device__ inline __attribute__((address_space(1))) T *glw(T *p)
     return (__attribute__((address_space(1))) T *)p;
 }
 
-// The nine values of block k (kv_at layout): four 2-wide loads and one
-// scalar load, converted to the accumulation type.
-typedef float f32x2 __attribute__((ext_vector_type(2)));
+// The nine values of block k, converted to the accumulation type.
 typedef double f64x2 __attribute__((ext_vector_type(2)));
-template <typename VT> struct Vec2;
-template <> struct Vec2<float> { using T = f32x2; };
-template <> struct Vec2<double> { using T = f64x2; };
+
+template <typename VT, typename AT> struct BlockLoad;
+
+// FP64 values (kv_at layout): four 2-wide loads and one scalar load
+template <typename AT> struct BlockLoad<double, AT> {
+    __device__ static inline void load(const double *__restrict__ vals, int64_t k, AT v[9])
+    {
+        const auto *base = gl(vals) + (k >> 6) * 576;
+        const auto *pr = gl(reinterpret_cast<const f64x2 *>(vals + (k >> 6) * 576)) + (k & 63);
+#pragma unroll
+        for (int p = 0; p < 4; p++) {
+            const f64x2 w = pr[p * 64];
+            v[2 * p] = (AT)w.x;
+            v[2 * p + 1] = (AT)w.y;
+        }
+        v[8] = (AT)base[512 + (k & 63)];
+    }
+};
+
+// FP32 values (kv32_at layout): two 4-wide loads and one scalar load
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+template <typename AT> struct BlockLoad<float, AT> {
+    __device__ static inline void load(const float *__restrict__ vals, int64_t k, AT v[9])
+    {
+        const auto *base = gl(vals) + (k >> 6) * 576;
+        const auto *pq = gl(reinterpret_cast<const f32x4 *>(vals + (k >> 6) * 576)) + (k & 63);
+#pragma unroll
+        for (int q = 0; q < 2; q++) {
+            const f32x4 w = pq[q * 64];
+            v[4 * q] = (AT)w.x;
+            v[4 * q + 1] = (AT)w.y;
+            v[4 * q + 2] = (AT)w.z;
+            v[4 * q + 3] = (AT)w.w;
+        }
+        v[8] = (AT)base[512 + (k & 63)];
+    }
+};
 
 template <typename VT, typename AT>
 __device__ inline void load_block(const VT *__restrict__ vals, int64_t k, AT v[9])
 {
-    using V2 = typename Vec2<VT>::T;
-    const auto *base = gl(vals) + (k >> 6) * 576;
-    const auto *pr = gl(reinterpret_cast<const V2 *>(vals + (k >> 6) * 576)) + (k & 63);
-#pragma unroll
-    for (int p = 0; p < 4; p++) {
-        const V2 w = pr[p * 64];
-        v[2 * p] = (AT)w.x;
-        v[2 * p + 1] = (AT)w.y;
-    }
-    v[8] = (AT)base[512 + (k & 63)];
+    BlockLoad<VT, AT>::load(vals, k, v);
 }
 
 template <int L, bool ABS, typename VT, typename XT, typename AT>
