@@ -63,6 +63,20 @@ enum { P_PQ = 0, P_RZ0 = 4096, P_RZ1 = 4608, P_RR0 = 5120, P_RR1 = 5632, P_CY0 =
        P_B = 12288, P_C = 16384, P_D = 20480, P_END = 24576 };
 constexpr int MAX_CW = 512;  // coarse-product workgroups per model (4 coarse rows each)
 
+// Pointers taken from a descriptor are generic; device buffers are all in
+// the global address space, and saying so lets the compiler use global
+// (not flat) memory instructions.
+template <typename T>
+__device__ inline const __attribute__((address_space(1))) T *gl(const T *p)
+{
+    return (const __attribute__((address_space(1))) T *)p;
+}
+template <typename T>
+__device__ inline __attribute__((address_space(1))) T *glw(T *p)
+{
+    return (__attribute__((address_space(1))) T *)p;
+}
+
 template <int NT>
 __device__ inline double block_sum(double v)
 {
@@ -94,7 +108,17 @@ template <int NT>
 __device__ inline double sum_partials(const double *__restrict__ P, int n)
 {
     double v = 0.0;
-    for (int i = threadIdx.x; i < n; i += NT) v += P[i];
+    // four loads in flight per pass; the additions keep the order i, i+NT, ...
+    const auto *Pg = gl(P);
+    int i = threadIdx.x;
+    for (; i + 3 * NT < n; i += 4 * NT) {
+        const double a0 = Pg[i], a1 = Pg[i + NT], a2 = Pg[i + 2 * NT], a3 = Pg[i + 3 * NT];
+        v += a0;
+        v += a1;
+        v += a2;
+        v += a3;
+    }
+    for (; i < n; i += NT) v += Pg[i];
     return block_sum<NT>(v);
 }
 
@@ -137,15 +161,16 @@ __device__ inline double sum_rz(const SegDesc &d, int slot)
 // coarse correction at dof c of node nd: (R_nd y_agg(nd))_c, masked
 __device__ inline double coarse_term(const SegDesc &d, int64_t nd, int c)
 {
-    const int32_t a = d.agg[nd];
-    const double *y = d.yvec + (int64_t)a * 6;
-    const double dx = d.X[nd * 3] - d.cent[a * 3], dy = d.X[nd * 3 + 1] - d.cent[a * 3 + 1],
-                 dz = d.X[nd * 3 + 2] - d.cent[a * 3 + 2];
+    const int32_t a = gl(d.agg)[nd];
+    const auto *y = gl(d.yvec) + (int64_t)a * 6;
+    const auto *X = gl(d.X), *cent = gl(d.cent);
+    const double dx = X[nd * 3] - cent[a * 3], dy = X[nd * 3 + 1] - cent[a * 3 + 1],
+                 dz = X[nd * 3 + 2] - cent[a * 3 + 2];
     double v;
     if (c == 0) v = fma(-dy, y[5], fma(dz, y[4], y[0]));
     else if (c == 1) v = fma(dx, y[5], fma(-dz, y[3], y[1]));
     else v = fma(-dx, y[4], fma(dy, y[3], y[2]));
-    return d.mask[nd * 3 + c] != 0.0 ? v : 0.0;
+    return gl(d.mask)[nd * 3 + c] != 0.0 ? v : 0.0;
 }
 
 // Preconditioner: Jacobi (inverse diagonal) or 3x3 nodal block Jacobi
@@ -156,17 +181,17 @@ __device__ inline void apply_prec(const SegDesc &P, int64_t node, const double r
     const int64_t i = node * 3;
     if (P.block) {
         if (P.f32) {
-            const float *B = P.binv32 + node * 9;
+            const auto *B = gl(P.binv32) + node * 9;
             const float r0 = (float)r[0], r1 = (float)r[1], r2 = (float)r[2];
             for (int d = 0; d < 3; d++) z[d] = (double)fmaf(B[d * 3 + 2], r2, fmaf(B[d * 3 + 1], r1, B[d * 3] * r0));
         } else {
-            const double *B = P.binv + node * 9;
+            const auto *B = gl(P.binv) + node * 9;
             for (int d = 0; d < 3; d++) z[d] = fma(B[d * 3 + 2], r[2], fma(B[d * 3 + 1], r[1], B[d * 3] * r[0]));
         }
     } else if (P.f32) {
-        for (int d = 0; d < 3; d++) z[d] = (double)(P.dinv32[i + d] * (float)r[d]);
+        for (int d = 0; d < 3; d++) z[d] = (double)(gl(P.dinv32)[i + d] * (float)r[d]);
     } else {
-        for (int d = 0; d < 3; d++) z[d] = P.dinv[i + d] * r[d];
+        for (int d = 0; d < 3; d++) z[d] = gl(P.dinv)[i + d] * r[d];
     }
 }
 
@@ -182,19 +207,6 @@ __device__ inline void apply_prec(const SegDesc &P, int64_t node, const double r
 // So neither the row sums nor the partials depend on L, and L can be
 // chosen per model for speed alone.
 
-// Pointers taken from a descriptor are generic; device buffers are all in
-// the global address space, and saying so lets the compiler use global
-// (not flat) memory instructions.
-template <typename T>
-__device__ inline const __attribute__((address_space(1))) T *gl(const T *p)
-{
-    return (const __attribute__((address_space(1))) T *)p;
-}
-template <typename T>
-__device__ inline __attribute__((address_space(1))) T *glw(T *p)
-{
-    return (__attribute__((address_space(1))) T *)p;
-}
 
 // The nine values of block k, converted to the accumulation type.
 typedef double f64x2 __attribute__((ext_vector_type(2)));
@@ -506,15 +518,20 @@ __global__ void __launch_bounds__(WGV) pcg_update_kernel(const SegDesc *__restri
     }
     const double a = rz / pq;
     double rzn = 0.0, rr = 0.0;
-    double *__restrict__ x = d.x, *__restrict__ r = d.r, *__restrict__ z = d.z;
-    const double *__restrict__ p = d.p, *__restrict__ q = d.q;
+    auto *__restrict__ x = glw(d.x);
+    auto *__restrict__ r = glw(d.r);
+    auto *__restrict__ z = glw(d.z);
+    const auto *__restrict__ p = gl(d.p);
+    const auto *__restrict__ q = gl(d.q);
     const int64_t stride = (int64_t)d.nvblk * WGV;
-    for (int64_t nd = (int64_t)ve.y * WGV + threadIdx.x; nd < d.nnodes; nd += stride) {
+    // two nodes per pass (nd, nd + stride), all loads issued before any
+    // store; the dot contributions keep the one-node-at-a-time order
+    auto node = [&](int64_t nd, const double pv[3], const double qv[3], const double xv[3], const double rv0[3]) {
         double rv[3], zv[3];
         for (int c = 0; c < 3; c++) {
             const int64_t i = nd * 3 + c;
-            x[i] = fma(a, p[i], x[i]);
-            rv[c] = fma(-a, q[i], r[i]);
+            x[i] = fma(a, pv[c], xv[c]);
+            rv[c] = fma(-a, qv[c], rv0[c]);
             r[i] = rv[c];
         }
         apply_prec(d, nd, rv, zv);
@@ -523,6 +540,33 @@ __global__ void __launch_bounds__(WGV) pcg_update_kernel(const SegDesc *__restri
             rzn = fma(rv[c], zv[c], rzn);
             rr = fma(rv[c], rv[c], rr);
         }
+    };
+    int64_t nd = (int64_t)ve.y * WGV + threadIdx.x;
+    for (; nd + stride < d.nnodes; nd += 2 * stride) {
+        const int64_t n1 = nd + stride;
+        double pa[3], qa[3], xa[3], ra[3], pb[3], qb[3], xb[3], rb[3];
+        for (int c = 0; c < 3; c++) {
+            pa[c] = p[nd * 3 + c];
+            qa[c] = q[nd * 3 + c];
+            xa[c] = x[nd * 3 + c];
+            ra[c] = r[nd * 3 + c];
+            pb[c] = p[n1 * 3 + c];
+            qb[c] = q[n1 * 3 + c];
+            xb[c] = x[n1 * 3 + c];
+            rb[c] = r[n1 * 3 + c];
+        }
+        node(nd, pa, qa, xa, ra);
+        node(n1, pb, qb, xb, rb);
+    }
+    if (nd < d.nnodes) {
+        double pa[3], qa[3], xa[3], ra[3];
+        for (int c = 0; c < 3; c++) {
+            pa[c] = p[nd * 3 + c];
+            qa[c] = q[nd * 3 + c];
+            xa[c] = x[nd * 3 + c];
+            ra[c] = r[nd * 3 + c];
+        }
+        node(nd, pa, qa, xa, ra);
     }
     rzn = block_sum<WGV>(rzn);
     rr = block_sum<WGV>(rr);
@@ -548,11 +592,25 @@ __global__ void __launch_bounds__(WGV) pcg_dir_kernel(const SegDesc *__restrict_
         d.scal[S_RZ] = rzn;
         if (rr <= d.scal[S_THR] || it >= d.maxiter || !std::isfinite(rr)) ctl[C_STOP] = 1;
     }
-    double *__restrict__ p = d.p;
-    float *__restrict__ p32 = d.wp32 ? d.p32 : nullptr;
-    const double *__restrict__ z = d.z;
+    auto *__restrict__ p = glw(d.p);
+    auto *__restrict__ p32 = d.wp32 ? glw(d.p32) : nullptr;
+    const auto *__restrict__ z = gl(d.z);
     const int64_t n = d.nnodes * 3, stride = (int64_t)d.nvblk * WGV;
-    for (int64_t i = (int64_t)ve.y * WGV + threadIdx.x; i < n; i += stride) {
+    int64_t i = (int64_t)ve.y * WGV + threadIdx.x;
+    for (; i + stride < n; i += 2 * stride) {
+        const int64_t j = i + stride;
+        const double pi0 = p[i], pj0 = p[j];
+        const double zi = d.tl ? z[i] + coarse_term(d, i / 3, (int)(i % 3)) : z[i];
+        const double zj = d.tl ? z[j] + coarse_term(d, j / 3, (int)(j % 3)) : z[j];
+        const double pn = fma(beta, pi0, zi), pm = fma(beta, pj0, zj);
+        p[i] = pn;
+        p[j] = pm;
+        if (p32) {
+            p32[i] = (float)pn;
+            p32[j] = (float)pm;
+        }
+    }
+    if (i < n) {
         const double zi = d.tl ? z[i] + coarse_term(d, i / 3, (int)(i % 3)) : z[i];
         const double pn = fma(beta, p[i], zi);
         p[i] = pn;
@@ -572,12 +630,14 @@ __global__ void __launch_bounds__(64) tl_restrict_kernel(const SegDesc *__restri
     const SegDesc &d = D[e.x];
     if (mode == 0 && d.ctl[C_DONE]) return;
     const int32_t a = e.y;
-    const double cx = d.cent[a * 3], cy = d.cent[a * 3 + 1], cz = d.cent[a * 3 + 2];
+    const auto *cent = gl(d.cent), *X = gl(d.X), *rv = gl(d.r);
+    const auto *aggptr = gl(d.aggptr), *aggnodes = gl(d.aggnodes);
+    const double cx = cent[a * 3], cy = cent[a * 3 + 1], cz = cent[a * 3 + 2];
     double v[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    for (int32_t k = d.aggptr[a] + threadIdx.x; k < d.aggptr[a + 1]; k += 64) {
-        const int64_t nd = d.aggnodes[k];
-        const double r0 = d.r[nd * 3], r1 = d.r[nd * 3 + 1], r2 = d.r[nd * 3 + 2];
-        const double dx = d.X[nd * 3] - cx, dy = d.X[nd * 3 + 1] - cy, dz = d.X[nd * 3 + 2] - cz;
+    for (int32_t k = aggptr[a] + threadIdx.x; k < aggptr[a + 1]; k += 64) {
+        const int64_t nd = aggnodes[k];
+        const double r0 = rv[nd * 3], r1 = rv[nd * 3 + 1], r2 = rv[nd * 3 + 2];
+        const double dx = X[nd * 3] - cx, dy = X[nd * 3 + 1] - cy, dz = X[nd * 3 + 2] - cz;
         v[0] += r0;
         v[1] += r1;
         v[2] += r2;
@@ -588,7 +648,7 @@ __global__ void __launch_bounds__(64) tl_restrict_kernel(const SegDesc *__restri
     for (int q = 0; q < 6; q++)
         for (int off = 32; off > 0; off >>= 1) v[q] += __shfl_down(v[q], off, 64);
     if (threadIdx.x == 0)
-        for (int q = 0; q < 6; q++) d.cvec[a * 6 + q] = v[q];
+        for (int q = 0; q < 6; q++) glw(d.cvec)[a * 6 + q] = v[q];
 }
 
 __global__ void __launch_bounds__(256) tl_coarse_kernel(const SegDesc *__restrict__ D, const int2 *__restrict__ wl,
@@ -600,15 +660,16 @@ __global__ void __launch_bounds__(256) tl_coarse_kernel(const SegDesc *__restric
     const int pi = (d.ctl[C_ITERS] - 1) & 1;
     const int row = e.y * 4 + (int)(threadIdx.x / 64), lane = threadIdx.x % 64;
     double acc = 0.0;
+    const auto *cvec = gl(d.cvec);
     if (row < d.nc) {
-        const double *Kr = d.Kinv + (int64_t)row * d.nc;
-        for (int c = lane; c < d.nc; c += 64) acc = fma(Kr[c], d.cvec[c], acc);
+        const auto *Kr = gl(d.Kinv) + (int64_t)row * d.nc;
+        for (int c = lane; c < d.nc; c += 64) acc = fma(Kr[c], cvec[c], acc);
     }
     for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
     __shared__ double cyw[4];
     if (lane == 0) {
-        if (row < d.nc) d.yvec[row] = acc;
-        cyw[threadIdx.x / 64] = row < d.nc ? d.cvec[row] * acc : 0.0;
+        if (row < d.nc) glw(d.yvec)[row] = acc;
+        cyw[threadIdx.x / 64] = row < d.nc ? cvec[row] * acc : 0.0;
     }
     __syncthreads();
     if (threadIdx.x == 0) d.part[(pi ? P_CY1 : P_CY0) + e.y] = ((cyw[0] + cyw[1]) + cyw[2]) + cyw[3];
