@@ -499,6 +499,7 @@ class Server:
         admit_q = collections.deque()
         arrived = []   # arrived, host build not started (largest_first only)
         running = {}
+        pending = collections.deque()   # (request, error): NR update queued, not yet completed
         ndone = 0
         lpt = self.order == "largest_first"
         key = lambda r: (-(r.work or 0.0), r.t_arrival, r.rid)  # noqa: E731
@@ -566,24 +567,40 @@ class Server:
                     else:
                         running[req.ana.dev.handle.value] = req
                     self.t_loop["start"] += time.perf_counter() - ts
-                if self.pools.size() > 0 or self.pools.in_flight():
-                    # each pool's step has run beside the host work above;
-                    # collect the finished ones (waiting for the oldest if none)
+                if pending or self.pools.size() > 0 or self.pools.in_flight():
+                    # collect every pool whose step has finished: queue its
+                    # finished members' NR updates, queue its next step, and
+                    # leave completing the updates to the passes below, one
+                    # member per pass, so that a pool is relaunched as soon as
+                    # its step ends, not after the host has completed another
+                    # pool's members
                     self.pools.launch_idle()
                     ts = time.perf_counter()
-                    fins = self.pools.collect(block=True)
+                    fins = self.pools.collect(block=not pending)
                     self.t_loop["step"] += time.perf_counter() - ts
                     ts = time.perf_counter()
                     for k, fin in fins:
-                        reqs_of = {h: running.pop(h) for h, _ in fin}
-                        for h, job, done, exc in resume_all(fin, lambda h: reqs_of[h].job, catch=True,
-                                                            between=self.pools.relauncher(k)):
-                            req = reqs_of[h]
-                            if done:
-                                self._finish(req, clock(), exc)
-                                ndone += 1
-                            else:
-                                running[req.ana.dev.handle.value] = req
+                        for h, si in fin:
+                            req = running.pop(h)
+                            try:
+                                req.job.resume_begin(si)
+                                pending.append((req, None))
+                            except Exception as e:
+                                pending.append((req, e))
+                        self.pools.relauncher(k)()
+                    if pending:
+                        req, exc = pending.popleft()
+                        done = True
+                        if exc is None:
+                            try:
+                                done = req.job.resume_end()
+                            except Exception as e:
+                                exc = e
+                        if done:
+                            self._finish(req, clock(), exc)
+                            ndone += 1
+                        else:
+                            running[req.ana.dev.handle.value] = req
                     self.t_loop["resume"] += time.perf_counter() - ts
                 else:
                     if self._graveyard:
