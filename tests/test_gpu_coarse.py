@@ -125,3 +125,45 @@ def test_two_level_cuts_iterations(amaru):
     assert np.array_equal(k1, k2) and info["iterations"] == its["two_level"]
     assert np.array_equal(Ug, sols["two_level"])
     dev.close()
+
+
+def test_coarse_refresh_policies(amaru):
+    """A von Mises cantilever over several increments: with the coarse
+    operator formed once per DOF map its inverse stays the one of the first
+    (elastic) assembly; per increment and per assembly it is formed again from
+    the tangent. All three converge to the same displacements (the coarse
+    operator only preconditions)."""
+    vm = amaru.VonMises(E=200e9, nu=0.3, fy=250e6, H=2e9)
+    out = {}
+    for policy in ("dof_map", "increment", "assembly"):
+        mesh = amaru.Mesh(amaru.Block([[0, 0, 0], [1, 0.2, 0.2]], nx=12, ny=2, nz=2, cellshape="HEX8"))
+        model = amaru.FEModel(mesh, [("all", amaru.MechSolid, vm)])
+        ana = amaru.MechAnalysis(model)
+        amaru.addstage(ana, [("x==0", amaru.SurfaceBC(ux=0, uy=0, uz=0)), ("x==1", amaru.SurfaceBC(uz=-0.02))], nincs=4)
+        model.configure_dofs(ana.stages[0].bcs)
+        ana.dev = amaru.DeviceModel(model)
+        ana.dev.set_coarse(target_nodes=16)
+        ana.dev.set_coarse_refresh(policy)
+        ok, msg = amaru.solve(ana, precision="fp64", precond="two_level", pcg_rtol=1e-12)
+        assert ok, msg
+        assert (model.state[:, 12] > 0).any()   # it yielded (accumulated plastic strain)
+        out[policy] = (model.uvals.copy(), ana.dev.coarse_space()[1].copy())
+        ana.dev.close()
+    u0 = out["dof_map"][0]
+    for policy in ("increment", "assembly"):
+        assert np.linalg.norm(out[policy][0] - u0) <= 1e-8 * np.linalg.norm(u0)
+    # the plastic tangent changed the coarse operator for the refreshing policies
+    assert not np.array_equal(out["dof_map"][1], out["increment"][1])
+    assert not np.array_equal(out["dof_map"][1], out["assembly"][1])
+    # per DOF map: the inverse is the one of the elastic K of the first assembly
+    mesh = amaru.Mesh(amaru.Block([[0, 0, 0], [1, 0.2, 0.2]], nx=12, ny=2, nz=2, cellshape="HEX8"))
+    model = amaru.FEModel(mesh, [("all", amaru.MechSolid, vm)])
+    bcs = [("x==0", amaru.SurfaceBC(ux=0, uy=0, uz=0)), ("x==1", amaru.SurfaceBC(uz=-0.02))]
+    model.configure_dofs(bcs)
+    U, F = model.bc_vals(bcs)
+    dev = amaru.DeviceModel(model)
+    dev.set_coarse(target_nodes=16)
+    dev.assemble()
+    dev.solve_system(U * 0.0, F, precision="fp64", precond="two_level", rtol=1e-10)
+    assert np.array_equal(dev.coarse_space()[1], out["dof_map"][1])
+    dev.close()
