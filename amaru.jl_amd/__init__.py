@@ -11,9 +11,9 @@ the reference's user API:
 from ._lib import AmaruError, device_count, lib
 from .analysis import MechAnalysis, addstage, solve
 from .mesh import HEX8, HEX20, TET10, Block, Mesh
-from .model import (BodyC, DeviceModel, DruckerPrager, EdgeBC, FaceBC, FEModel, LinearElastic, MechSolid, NodeBC,
-                    SurfaceBC, VonMises)
+from .model import (OUTPUT_FIELDS, BodyC, DeviceModel, DruckerPrager, EdgeBC, FaceBC, FEModel, LinearElastic,
+                    MechSolid, NodeBC, SurfaceBC, VonMises, output_fields)
 
 __all__ = ["AmaruError", "device_count", "lib", "MechAnalysis", "addstage", "solve", "HEX8", "HEX20", "TET10",
            "Block", "Mesh", "BodyC", "DeviceModel", "DruckerPrager", "EdgeBC", "FaceBC", "FEModel", "LinearElastic",
-           "MechSolid", "NodeBC", "SurfaceBC", "VonMises"]
+           "MechSolid", "NodeBC", "SurfaceBC", "VonMises", "OUTPUT_FIELDS", "output_fields"]

@@ -89,7 +89,11 @@ SIGNATURES = {
     "amr_set_coarse": (I32, [P, I32, I32]),
     "amr_get_coarse": (I32, [P, P]),
     "amr_get_coarse_space": (I32, [P, P, P]),
+    "amr_ip_values": (I32, [P, I32, P]),
+    "amr_nodal_recovery": (I32, [P, I32, P, P]),
 }
+
+AMR_NFIELDS = 19
 
 _lib = None
 

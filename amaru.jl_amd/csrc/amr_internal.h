@@ -179,6 +179,10 @@ struct amr_model {
     int64_t n_spmv = 0;
     // SpMV lanes per block row (0 = not chosen yet)
     int lanes64 = 0, lanes32 = 0, lanes32d = 0;
+    // output post-processing (recovery.hip)
+    std::vector<double> Nip;        // nips*nn shape functions at the IPs
+    int elem_kind = -1;             // material kind of every element, -2 if they differ
+    struct AmrRecovery *rec = nullptr;
 };
 
 enum { SV_UEX = 0, SV_FEX, SV_U, SV_F, SV_DFEX, SV_R, SV_RC, SV_DUA, SV_DUI, SV_DFIN, SV_DUT, SV_X };
@@ -204,6 +208,9 @@ int amr_launch_update(amr_model *m, const double *dU_node /*device, node order*/
 int amr_coarse_setup(amr_model *m);
 int amr_coarse_build(amr_model *m);  // the index structures only (host work, one wait)
 void amr_coarse_free(amr_model *m);
+
+// output post-processing (recovery.hip)
+void amr_recovery_free(amr_model *m);
 
 // solver (kernels_solve.hip)
 int amr_pcg_solve(amr_model *m, const double *b, double *x, const amr_solve_opts *o, amr_solve_info *info);

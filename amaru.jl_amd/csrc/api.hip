@@ -83,6 +83,34 @@ static void shape_deriv(int etype, const double *R, double *D)
     }
 }
 
+// N at R (nn): the functions whose derivatives shape_deriv returns
+// (src/shape/solids3d.jl:124-146, 472-484, 584-613).
+static void shape_func(int etype, const double *R, double *N)
+{
+    const double r = R[0], s = R[1], t = R[2];
+    if (etype == AMR_HEX8 || etype == AMR_HEX20) {
+        for (int a = 0; a < 8; a++) {
+            const double ra = kHexNodes[a][0], sa = kHexNodes[a][1], ta = kHexNodes[a][2];
+            const double f = 0.125 * (1.0 + ra * r) * (1.0 + sa * s) * (1.0 + ta * t);
+            N[a] = etype == AMR_HEX8 ? f : f * (ra * r + sa * s + ta * t - 2.0);
+        }
+        if (etype == AMR_HEX8) return;
+        for (int a = 8; a < 20; a++) {
+            const double ra = kHexNodes[a][0], sa = kHexNodes[a][1], ta = kHexNodes[a][2];
+            if (ra == 0.0)
+                N[a] = 0.25 * (1.0 - r * r) * (1.0 + sa * s) * (1.0 + ta * t);
+            else if (sa == 0.0)
+                N[a] = 0.25 * (1.0 + ra * r) * (1.0 - s * s) * (1.0 + ta * t);
+            else
+                N[a] = 0.25 * (1.0 + ra * r) * (1.0 + sa * s) * (1.0 - t * t);
+        }
+    } else {
+        const double L[4] = {1.0 - r - s - t, r, s, t};
+        for (int i = 0; i < 4; i++) N[i] = L[i] * (2.0 * L[i] - 1.0);
+        for (int e = 0; e < 6; e++) N[4 + e] = 4.0 * L[kTetEdges[e][0]] * L[kTetEdges[e][1]];
+    }
+}
+
 // IP rules, as tabulated by the reference (src/shape/quadrature.jl:110-114,
 // 167-206): HEX 2x2x2 and 3x3x3 (r fastest), TET 4-point.
 static int ip_rule(int etype, int nips, double *R, double *w)
@@ -182,6 +210,7 @@ static void free_model(amr_model *m)
     if (m->self_pool) amr_pool_free(m->self_pool);
     m->self_pool = nullptr;
     amr_coarse_free(m);
+    amr_recovery_free(m);
     if (m->own_stream) (void)hipStreamSynchronize(m->own_stream);
     void *ptrs[] = {m->X,    m->conn, m->emat, m->mats,  m->dNdR, m->wip,  m->st_cur, m->st_com, m->Ke,
                     m->rowptr, m->col, m->diag, m->cptr,  m->csrc, m->fptr, m->fsrc,   m->Kv,     m->Kv32,
@@ -333,6 +362,14 @@ extern "C" int amr_model_create(amr_model **out, int device, int etype, int nips
 
     std::vector<double> dNdR((size_t)nq * nn * 3);
     for (int q = 0; q < nq; q++) shape_deriv(etype, R + 3 * q, dNdR.data() + (size_t)q * nn * 3);
+    m->Nip.resize((size_t)nq * nn);
+    for (int q = 0; q < nq; q++) shape_func(etype, R + 3 * q, m->Nip.data() + (size_t)q * nn);
+    m->elem_kind = -1;
+    for (int64_t e = 0; e < ne; e++) {
+        const int k = mat_kind[emat[e]];
+        if (e == 0) m->elem_kind = k;
+        else if (k != m->elem_kind) { m->elem_kind = -2; break; }
+    }
     hipStream_t s = m->stream;
     TRY(h2d(m->X, coords, n3, s));
     m->Xh.assign(coords, coords + n3);

@@ -361,6 +361,22 @@ class FEModel:
         return {"ux": u[:, 0], "uy": u[:, 1], "uz": u[:, 2], "fx": f[:, 0], "fy": f[:, 1], "fz": f[:, 2]}
 
 
+OUTPUT_FIELDS = ("σxx", "σyy", "σzz", "σyz", "σxz", "σxy", "σvm", "σ1", "σ2", "σ3",
+                 "εxx", "εyy", "εzz", "εyz", "εxz", "εxy")
+_EXTRA_FIELDS = {"LinearElastic": (), "VonMises": ((16, "ep"),),
+                 "DruckerPrager": ((16, "epa"), (17, "j1"), (18, "srj2d"))}
+
+
+def output_fields(model):
+    """(column, name) of the output table for the model's material
+    (stress_strain_dict, src/tools/tensors.jl:190-213, plus the material's
+    own entries)."""
+    kinds = {type(m).__name__ for m in model.mats}
+    if len(kinds) != 1:
+        raise ValueError("output fields: the model mixes material kinds")
+    return list(enumerate(OUTPUT_FIELDS)) + list(_EXTRA_FIELDS[kinds.pop()])
+
+
 class DeviceModel:
     """An FEModel resident on one GPU behind the C ABI."""
 
@@ -584,6 +600,27 @@ class DeviceModel:
         n = ctypes.c_int64(0)
         _lib.check(_lib.lib().amr_timing(self._h, 1 if enable else 0, ctypes.byref(ms), ctypes.byref(n)))
         return ms.value, n.value
+
+    # -------------------------------------------------------- outputs
+    def ip_values(self, committed=False):
+        """ip_state_vals of every IP (nip_total x 19; ``OUTPUT_FIELDS``)."""
+        out = np.empty((self.model.ne * self.model.nips, _lib.AMR_NFIELDS))
+        _lib.check(_lib.lib().amr_ip_values(self._h, int(committed), _lib.ptr(out)))
+        return out
+
+    def nodal_values(self, committed=False):
+        """nodal_patch_recovery (src/fe-model.jl:506-692) of the IP values:
+        (nnodes x 19 values, patches reaching each node)."""
+        out = np.empty((self.model.nnodes, _lib.AMR_NFIELDS))
+        reps = np.empty(self.model.nnodes, dtype=np.int32)
+        _lib.check(_lib.lib().amr_nodal_recovery(self._h, int(committed), _lib.ptr(out), _lib.ptr(reps)))
+        return out, reps
+
+    def node_fields(self, committed=False):
+        """The recovered nodal fields by the reference's names for the
+        model's material (σxx … εxy, plus ep or epa, j1, srj2d)."""
+        vals, _ = self.nodal_values(committed)
+        return {name: vals[:, i] for i, name in output_fields(self.model)}
 
     def update_state(self, dUt):
         """update_state!(elems, ΔUt, t) → ΔFin (raises on failure)."""

@@ -249,6 +249,40 @@ def footing_leg(am, args):
             "speedup_vs_plain_fp64": out["mixed64acc_two_level"]["value"] / out["plain_fp64"]["value"]}
 
 
+def post_leg(am, dev, model, args):
+    """Output post-processing on the solved block: ip_state_vals of every IP
+    and nodal_patch_recovery of the 19 output fields.  The first call also
+    builds the patches (host, once per mesh); the timed calls are the
+    per-output cost.  The oracle's restatement is timed on a smaller block."""
+    t0 = time.perf_counter()
+    vals, reps = dev.nodal_values()
+    first = time.perf_counter() - t0
+    ts = []
+    for _ in range(args.post):
+        t0 = time.perf_counter()
+        dev.nodal_values()
+        ts.append(time.perf_counter() - t0)
+    t = float(np.median(ts))
+    out = {"workload": f"nodal_patch_recovery of 19 IP output fields, HEX20 {args.n}^3 block (27 IP): "
+                       f"{model.ne * model.nips} IPs, {model.nnodes} nodes",
+           "first_call_s": first, "seconds": t, "nodes_per_s": model.nnodes / t,
+           "patch_reach_max": int(reps.max()), "calls": len(ts)}
+    log(f"post-processing: first call {first:.3f}s, then {t * 1e3:.2f} ms per call")
+    if not args.no_cpu:
+        from oracle import HEX20 as O20, recovery as R
+        n = args.cpu_n
+        mesh_c, model_c, _, _, _ = block_problem(am, n)
+        st = np.random.default_rng(0).standard_normal((model_c.ne * model_c.nips, 14))
+        t0 = time.perf_counter()
+        V = R.ip_values(st, np.zeros(len(st), dtype=int))
+        R.nodal_patch_recovery(model_c.coords, model_c.conn, O20, V, 27)
+        tc = time.perf_counter() - t0
+        out["cpu_oracle"] = {"sample": f"HEX20 {n}^3 (27 IP), {model_c.nnodes} nodes, numpy, 1 process",
+                             "seconds": tc, "nodes_per_s": model_c.nnodes / tc}
+        log(f"post-processing oracle on {n}^3: {tc:.2f}s")
+    return out
+
+
 def served_leg(am, args):
     """Request stream: plain one-at-a-time capacity, then the server at a
     Poisson rate that is a multiple of it."""
@@ -381,6 +415,7 @@ def main():
     ap.add_argument("--footing-settlement", type=float, default=0.02,
                     help="0.1 m (the nominal value) collapses the perfectly plastic soil before the end")
     ap.add_argument("--footing-autoinc", action="store_true")
+    ap.add_argument("--post", type=int, default=3, help="timed nodal-recovery calls on the block (0 = skip)")
     ap.add_argument("--serve-seed", type=int, default=1)
     ap.add_argument("--serve-load", type=float, default=4.0,
                     help="Poisson arrival rate as a multiple of the plain path's request rate")
@@ -507,6 +542,8 @@ def main():
                                           f"{cb['nu']} unknowns: mount_K {cb['t_assemble']:.2f}s "
                                           f"({threads} OpenMP threads), SuperLU factor+solve "
                                           f"{cb['t_solve']:.2f}s (1 thread), update {cb['t_update']:.2f}s")}
+    if args.post:
+        out["post_processing"] = post_leg(am, dev, model, args)
     dev.close()
     for name, leg, n in (("batched", batched_leg, args.batch), ("served", served_leg, args.serve),
                          ("footing", footing_leg, args.footing)):

@@ -311,6 +311,33 @@ int amr_get_coarse_space(amr_model *m, int32_t *agg, double *kc_inv);
 int amr_model_device_bytes(amr_model *m, int64_t *bytes);
 int amr_mem_info(int device, int64_t *free_bytes, int64_t *total_bytes);
 
+/*
+ * Output post-processing.  Values per IP are ip_state_vals of the solid
+ * materials (src/mech/mat/linear-elastic.jl:140-142, von-mises.jl:159-168,
+ * drucker-prager.jl:152-163) over stress_strain_dict
+ * (src/tools/tensors.jl:162-215, 3D), in a fixed table of AMR_NFIELDS
+ * columns: sxx syy szz syz sxz sxy svm s1 s2 s3 exx eyy ezz eyz exz exy
+ * (shear = Mandel / sqrt 2; s1 >= s2 >= s3 the principal stresses), then
+ * column 16 = ep (von Mises) or epa (Drucker-Prager), 17 = j1 and
+ * 18 = srj2d (Drucker-Prager); unused columns hold 0.  committed selects the
+ * committed (1) or current (0) IP states.
+ */
+#define AMR_NFIELDS 19
+/* out: nip_total x AMR_NFIELDS, IP-major (element e, IP q at row e*nips+q). */
+int amr_ip_values(amr_model *m, int committed, double *out);
+/*
+ * nodal_patch_recovery (src/fe-model.jl:506-692): least-squares polynomial
+ * fits over the patches of elements around corner nodes, evaluated at the
+ * patches' nodes and averaged per node (patch contributions summed in
+ * patch order, as one reference thread does); nodes no patch reaches get 0.
+ * out: nnodes x AMR_NFIELDS; reps (optional, nnodes): patches reaching each
+ * node.  The patches are built on the first call (host work, kept with the
+ * model).  Errors: AMR_ERR_ARG when the elements use different material
+ * kinds or a patch has over 256 integration points; AMR_ERR_SOLVE when a
+ * patch's least-squares basis is rank-deficient.
+ */
+int amr_nodal_recovery(amr_model *m, int committed, double *out, int32_t *reps);
+
 #ifdef __cplusplus
 }
 #endif
