@@ -12,6 +12,8 @@ Contents
 ``fem.py``          Python restatement of the structured mesher, DOF
                     numbering, boundary conditions, solve_system! (SuperLU
                     direct solve of K11) and mech_stage_solver!.
+``recovery.py``     ip_state_vals of the solid materials and
+                    nodal_patch_recovery (numpy, SVD pseudo-inverse).
 
 Every function cites the reference file:line it restates.  Pins: see
 ``tests/test_oracle_*.py`` (reference golden values, closed forms and
