@@ -12,11 +12,13 @@
 // built once per model on the host.  Per call:
 //   ipvals_kernel   one thread per IP: 19 output values (AoS, IP-major)
 //   patch_fit_kernel one wave per patch: Householder QR of the patch's basis
-//                   matrix with the IP values appended, back substitution;
-//                   the basis is centred on the patch's node and scaled by the
-//                   patch size (the same polynomial space, better conditioned)
-//   node_eval_kernel one thread per node: its patches' polynomials at the
-//                   node, summed in patch order, divided by their count.
+//                   matrix with the IP values appended (rows in registers,
+//                   column products reduced across the wave), back
+//                   substitution; the basis is centred on the patch's node and
+//                   scaled by the patch size (the same polynomial space,
+//                   better conditioned)
+//   node_eval_kernel one thread per (node, field): the patches' polynomials at
+//                   the node, summed in patch order, divided by their count.
 // Every sum has a fixed order: the results do not depend on the launch.
 #include <algorithm>
 #include <cmath>
@@ -128,8 +130,20 @@ __device__ inline void reg_terms(double x, double y, double z, double *t)
     t[6] = x * z;
 }
 
-// One 64-lane wave per patch.  A (m x NCOL, row-major, LDS): basis columns
-// 0..nt-1, IP values in columns NT..NT+NF-1.  Lane j owns column j.
+// Sum over the 64 lanes by an xor butterfly: every lane ends with the same
+// bits (each step adds the same two operands, in either order).
+__device__ inline double wave_sum(double x)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) x += __shfl_xor(x, o, 64);
+    return x;
+}
+
+// One 64-lane wave per patch.  Lane l holds rows l, l+64, l+128, l+192 of
+// the patch's matrix [basis | IP values] (m x NCOL) in registers; each
+// Householder step reduces the reflector's norm and its products with the
+// remaining columns across the wave.  R and Q^T W (rows 0..nt-1) go through
+// LDS to the back substitution, one lane per field.
 __global__ void __launch_bounds__(64) patch_fit_kernel(int nq, const int32_t *__restrict__ pptr,
                                                        const int32_t *__restrict__ pelem,
                                                        const double *__restrict__ pcen,
@@ -137,62 +151,92 @@ __global__ void __launch_bounds__(64) patch_fit_kernel(int nq, const int32_t *__
                                                        const double *__restrict__ vals, double *__restrict__ coef,
                                                        int32_t *__restrict__ bad)
 {
-    extern __shared__ double A[];
+    constexpr int RPL = MAXM / 64;
+    __shared__ double Rs[NT][NCOL];
     const int p = blockIdx.x, lane = threadIdx.x;
     const int e0 = pptr[p];
     const int m = (pptr[p + 1] - e0) * nq;
     const int nt = nterms_of(m);
     const double cx = pcen[p * 4], cy = pcen[p * 4 + 1], cz = pcen[p * 4 + 2], ih = 1.0 / pcen[p * 4 + 3];
-    for (int i = lane; i < m; i += 64) {
-        const int64_t ip = (int64_t)pelem[e0 + i / nq] * nq + i % nq;
-        double t[NT];
-        reg_terms((ipx[ip * 3] - cx) * ih, (ipx[ip * 3 + 1] - cy) * ih, (ipx[ip * 3 + 2] - cz) * ih, t);
-        double *row = A + (size_t)i * NCOL;
-        for (int c = 0; c < NT; c++) row[c] = t[c];
-        for (int f = 0; f < NF; f++) row[NT + f] = vals[ip * NF + f];
+    double a[RPL][NCOL];
+#pragma unroll
+    for (int r = 0; r < RPL; r++) {
+        const int i = lane + 64 * r;
+#pragma unroll
+        for (int c = 0; c < NCOL; c++) a[r][c] = 0.0;
+        if (i < m) {
+            const int64_t ip = (int64_t)pelem[e0 + i / nq] * nq + i % nq;
+            reg_terms((ipx[ip * 3] - cx) * ih, (ipx[ip * 3 + 1] - cy) * ih, (ipx[ip * 3 + 2] - cz) * ih, a[r]);
+#pragma unroll
+            for (int c = 1; c < NT; c++)
+                if (c >= nt) a[r][c] = 0.0;
+#pragma unroll
+            for (int f = 0; f < NF; f++) a[r][NT + f] = vals[ip * NF + f];
+        }
     }
-    __syncthreads();
-    const bool mine = lane < NCOL && (lane < nt || lane >= NT);
     double rmax = 0.0;
     bool deficient = false;
-    for (int k = 0; k < nt; k++) {
-        // every lane forms the reflector of column k in the same order
-        double sigma = 0.0;
-        for (int i = k; i < m; i++) sigma += A[i * NCOL + k] * A[i * NCOL + k];
-        const double akk = A[k * NCOL + k];
+#pragma unroll
+    for (int k = 0; k < NT; k++) {
+        if (k >= nt) break;
+        // reflector of column k over rows k..m-1 (rows past m hold zeros)
+        double v[RPL];
+#pragma unroll
+        for (int r = 0; r < RPL; r++) v[r] = (r > 0 || lane > k) ? a[r][k] : 0.0;
+        double part = 0.0;
+#pragma unroll
+        for (int r = 0; r < RPL; r++) part += v[r] * v[r];
+        const double akk = __shfl(a[0][k], k, 64);
+        const double sigma = wave_sum(part) + akk * akk;
         const double alpha = -copysign(sqrt(sigma), akk);
         const double vk = akk - alpha;
         const double vv = sigma - akk * akk + vk * vk;
         rmax = fmax(rmax, fabs(alpha));
         if (!(fabs(alpha) > 1e-13 * rmax) || vv == 0.0) deficient = true;
-        __syncthreads();
-        if (mine && lane > k && vv != 0.0) {
-            double d = vk * A[k * NCOL + lane];
-            for (int i = k + 1; i < m; i++) d += A[i * NCOL + k] * A[i * NCOL + lane];
-            const double g = 2.0 * d / vv;
-            A[k * NCOL + lane] -= g * vk;
-            for (int i = k + 1; i < m; i++) A[i * NCOL + lane] -= g * A[i * NCOL + k];
+        if (lane == k) v[0] = vk;
+#pragma unroll
+        for (int c = k + 1; c < NCOL; c++) {
+            if (c >= nt && c < NT) continue;
+            double d = 0.0;
+#pragma unroll
+            for (int r = 0; r < RPL; r++) d += v[r] * a[r][c];
+            const double g = vv != 0.0 ? 2.0 * wave_sum(d) / vv : 0.0;
+#pragma unroll
+            for (int r = 0; r < RPL; r++) a[r][c] -= g * v[r];
         }
-        __syncthreads();
-        if (lane == k) A[k * NCOL + k] = alpha;
-        __syncthreads();
+        if (lane == k) a[0][k] = alpha;
     }
+    if (lane < nt) {
+#pragma unroll
+        for (int c = 0; c < NCOL; c++) Rs[lane][c] = a[0][c];
+    }
+    __syncthreads();
     if (deficient) {
         if (lane == 0) bad[p] = 1;
         return;
     }
-    if (lane >= NT && lane < NCOL) {
-        double a[NT];
-        for (int t = nt - 1; t >= 0; t--) {
-            double b = A[t * NCOL + lane];
-            for (int u = t + 1; u < nt; u++) b -= A[t * NCOL + u] * a[u];
-            a[t] = b / A[t * NCOL + t];
+    if (lane < NF) {
+        double x[NT];
+#pragma unroll
+        for (int t = NT - 1; t >= 0; t--) {
+            if (t >= nt) {
+                x[t] = 0.0;
+                continue;
+            }
+            double b = Rs[t][NT + lane];
+#pragma unroll
+            for (int u = t + 1; u < NT; u++)
+                if (u < nt) b -= Rs[t][u] * x[u];
+            x[t] = b / Rs[t][t];
         }
-        double *c = coef + (size_t)p * NT * NF + (lane - NT);
-        for (int t = 0; t < NT; t++) c[t * NF] = t < nt ? a[t] : 0.0;
+        double *c = coef + (size_t)p * NT * NF + lane;
+#pragma unroll
+        for (int t = 0; t < NT; t++) c[t * NF] = x[t];
     }
 }
 
+// One thread per (node, field): the field's value from each patch reaching
+// the node, summed in patch order, divided by the number of patches.
 __global__ void __launch_bounds__(256) node_eval_kernel(int64_t nnodes, int nq, const double *__restrict__ X,
                                                          const int32_t *__restrict__ nptr,
                                                          const int32_t *__restrict__ npat,
@@ -200,10 +244,11 @@ __global__ void __launch_bounds__(256) node_eval_kernel(int64_t nnodes, int nq, 
                                                          const double *__restrict__ pcen,
                                                          const double *__restrict__ coef, double *__restrict__ out)
 {
-    const int64_t n = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (n >= nnodes) return;
-    double acc[NF];
-    for (int f = 0; f < NF; f++) acc[f] = 0.0;
+    const int64_t id = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (id >= nnodes * NF) return;
+    const int64_t n = id / NF;
+    const int f = (int)(id - n * NF);
+    double acc = 0.0;
     const int j0 = nptr[n], j1 = nptr[n + 1];
     for (int j = j0; j < j1; j++) {
         const int p = npat[j];
@@ -212,18 +257,13 @@ __global__ void __launch_bounds__(256) node_eval_kernel(int64_t nnodes, int nq, 
         double t[NT];
         reg_terms((X[n * 3] - pcen[p * 4]) * ih, (X[n * 3 + 1] - pcen[p * 4 + 1]) * ih,
                   (X[n * 3 + 2] - pcen[p * 4 + 2]) * ih, t);
-        const double *c = coef + (size_t)p * NT * NF;
-        for (int f = 0; f < NF; f++) {
-            double v = t[0] * c[f];
-            for (int u = 1; u < nt; u++) v += t[u] * c[u * NF + f];
-            acc[f] += v;
-        }
+        const double *c = coef + (size_t)p * NT * NF + f;
+        double v = t[0] * c[0];
+        for (int u = 1; u < nt; u++) v += t[u] * c[u * NF];
+        acc += v;
     }
-    const double cnt = (double)(j1 - j0);
-    for (int f = 0; f < NF; f++) {
-        const double v = acc[f] / cnt;
-        out[n * NF + f] = v == v ? v : 0.0;   // no patch (0/0) or NaN values: 0, as the reference
-    }
+    const double r = acc / (double)(j1 - j0);
+    out[id] = r == r ? r : 0.0;   // no patch (0/0) or NaN values: 0, as the reference
 }
 
 template <typename T>
@@ -471,12 +511,11 @@ extern "C" int amr_nodal_recovery(amr_model *m, int committed, double *out, int3
     TRY(launch_ipvals(m, r, committed));
     if (r->npatch > 0) {
         AMR_HIP(hipMemsetAsync(r->bad, 0, sizeof(int32_t) * r->npatch, s));
-        const size_t lds = sizeof(double) * (size_t)r->maxm * NCOL;
-        hipLaunchKernelGGL(patch_fit_kernel, dim3(r->npatch), dim3(64), lds, s, m->nips, r->pptr, r->pelem,
+        hipLaunchKernelGGL(patch_fit_kernel, dim3(r->npatch), dim3(64), 0, s, m->nips, r->pptr, r->pelem,
                            r->pcen, r->ipx, r->vals, r->coef, r->bad);
         AMR_HIP(hipGetLastError());
     }
-    const unsigned g = (unsigned)((m->nnodes + 255) / 256);
+    const unsigned g = (unsigned)((m->nnodes * NF + 255) / 256);
     hipLaunchKernelGGL(node_eval_kernel, dim3(g), dim3(256), 0, s, m->nnodes, m->nips, m->X, r->nptr, r->npat,
                        r->pptr, r->pcen, r->coef, r->nout);
     AMR_HIP(hipGetLastError());
