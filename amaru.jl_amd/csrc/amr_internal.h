@@ -183,6 +183,11 @@ struct amr_model {
     std::vector<double> Nip;        // nips*nn shape functions at the IPs
     int elem_kind = -1;             // material kind of every element, -2 if they differ
     struct AmrRecovery *rec = nullptr;
+    // mass matrix (mount_M): element blocks, assembled values and survivor
+    // flags in the kv_at layout of K, shape functions at the IPs, densities
+    double *Me = nullptr, *Mv = nullptr, *Nd = nullptr, *rho_d = nullptr;
+    uint8_t *Mflag = nullptr;
+    bool m_assembled = false;
 };
 
 enum { SV_UEX = 0, SV_FEX, SV_U, SV_F, SV_DFEX, SV_R, SV_RC, SV_DUA, SV_DUI, SV_DFIN, SV_DUT, SV_X };
@@ -202,6 +207,7 @@ int amr_nr_post_slots(int *res, int *errn, int *errd, int *nan);
 int amr_launch_ke(amr_model *m);
 int amr_launch_assemble(amr_model *m, bool flags);
 int amr_launch_update(amr_model *m, const double *dU_node /*device, node order*/, double *dFin_node);
+int amr_launch_mass(amr_model *m);  // Me from Nd, rho_d; then assembly into Mv, Mflag
 
 // two-level coarse space (coarse.hip): build on first use, refresh Kc^-1
 // after every assembly; queued on the model's stream

@@ -215,7 +215,7 @@ static void free_model(amr_model *m)
     void *ptrs[] = {m->X,    m->conn, m->emat, m->mats,  m->dNdR, m->wip,  m->st_cur, m->st_com, m->Ke,
                     m->rowptr, m->col, m->diag, m->cptr,  m->csrc, m->fptr, m->fsrc,   m->Kv,     m->Kv32,
                     m->Kflag, m->dinv, m->dinv32, m->mask, m->eqd,  m->dF,   m->estat,  m->err,    m->part,
-                    m->scal, m->binv, m->binv32};
+                    m->scal, m->binv, m->binv32, m->Me, m->Mv, m->Mflag, m->Nd, m->rho_d};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (double *p : m->v)
@@ -548,19 +548,17 @@ extern "C" int amr_get_Ke(amr_model *m, double *Ke)
     return AMR_OK;
 }
 
-extern "C" int amr_get_K_csc(amr_model *m, int64_t *nnz, int64_t *colptr, int32_t *rowval, double *nzval)
+// CSC over the DOF numbering of a matrix held in the node-block layout
+// (values and survivor flags in the kv_at layout).
+static int bsr_to_csc(amr_model *m, const double *dv, const uint8_t *dflag, int64_t *nnz, int64_t *colptr,
+                      int32_t *rowval, double *nzval)
 {
-    if (!m || !nnz) return AMR_ERR_ARG;
-    if (!m->assembled) {
-        amr_set_error("amr_get_K_csc: K has not been assembled");
-        return AMR_ERR_ARG;
-    }
     AMR_HIP(hipSetDevice(m->device));
     const int64_t nb = m->pat.nblk;
     std::vector<double> val((size_t)kv_size(nb));
     std::vector<uint8_t> fl((size_t)kv_size(nb));
-    TRY(d2h(val.data(), m->Kv, val.size(), m->stream));
-    TRY(d2h(fl.data(), m->Kflag, fl.size(), m->stream));
+    TRY(d2h(val.data(), dv, val.size(), m->stream));
+    TRY(d2h(fl.data(), dflag, fl.size(), m->stream));
     AMR_HIP(hipStreamSynchronize(m->stream));
     std::vector<int64_t> cnt(m->ndofs + 1, 0);
     int64_t total = 0;
@@ -611,6 +609,56 @@ extern "C" int amr_get_K_csc(amr_model *m, int64_t *nnz, int64_t *colptr, int32_
         }
     }
     return AMR_OK;
+}
+
+extern "C" int amr_get_K_csc(amr_model *m, int64_t *nnz, int64_t *colptr, int32_t *rowval, double *nzval)
+{
+    if (!m || !nnz) return AMR_ERR_ARG;
+    if (!m->assembled) {
+        amr_set_error("amr_get_K_csc: K has not been assembled");
+        return AMR_ERR_ARG;
+    }
+    return bsr_to_csc(m, m->Kv, m->Kflag, nnz, colptr, rowval, nzval);
+}
+
+extern "C" int amr_assemble_M(amr_model *m, const double *rho)
+{
+    if (!m || !rho) return AMR_ERR_ARG;
+    for (int64_t e = 0; e < m->ne; e++)
+        if (!(rho[e] >= 0.0)) {
+            amr_set_error("amr_assemble_M: rho must be >= 0 (element " + std::to_string(e + 1) + ")");
+            return AMR_ERR_ARG;
+        }
+    AMR_HIP(hipSetDevice(m->device));
+    const size_t nq = (size_t)m->nips, nn = (size_t)m->nn;
+    if (!m->Me) TRY(dalloc(m, &m->Me, (size_t)m->ne * nn * nn * 9));
+    if (!m->Mv) TRY(dalloc(m, &m->Mv, (size_t)kv_size(m->pat.nblk)));
+    if (!m->Mflag) TRY(dalloc(m, &m->Mflag, (size_t)kv_size(m->pat.nblk)));
+    if (!m->Nd) TRY(dalloc(m, &m->Nd, nq * nn));
+    if (!m->rho_d) TRY(dalloc(m, &m->rho_d, (size_t)m->ne));
+    TRY(h2d(m->Nd, m->Nip.data(), m->Nip.size(), m->stream));
+    TRY(h2d(m->rho_d, rho, (size_t)m->ne, m->stream));
+    AMR_HIP(hipMemsetAsync(m->err, 0x7f, sizeof(int32_t), m->stream));  // "no bad element"
+    TRY(amr_launch_mass(m));
+    int32_t err = 0;
+    TRY(d2h(&err, m->err, 1, m->stream));
+    AMR_HIP(hipStreamSynchronize(m->stream));
+    if (err != 0x7f7f7f7f) {
+        amr_set_error("Negative Jacobian determinant in cell " + std::to_string(err));
+        return AMR_ERR_JACOBIAN;
+    }
+    m->m_assembled = true;
+    return AMR_OK;
+}
+
+extern "C" int amr_get_M_csc(amr_model *m, int64_t *nnz, int64_t *colptr, int32_t *rowval, double *nzval)
+{
+    if (!m || !nnz) return AMR_ERR_ARG;
+    if (!m->m_assembled) {
+        amr_set_error("amr_get_M_csc: M has not been assembled");
+        return AMR_ERR_ARG;
+    }
+    return bsr_to_csc(m, m->Mv, m->Mflag, nnz, colptr, rowval, nzval);
 }
 
 // ------------------------------------------------------- solve_system!

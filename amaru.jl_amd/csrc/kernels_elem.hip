@@ -353,6 +353,65 @@ __global__ void __launch_bounds__(64) ke_kernel(int64_t ne, const double *__rest
     if (lane == 0 && sBad) atomicMin(err, (int32_t)(e + 1));
 }
 
+// ----------------------------------------------------------- element mass
+//
+// One 64-lane workgroup per element.  Lanes q < NQ build J (the stiffness
+// kernel's fma chains) and coef = (rho * detJ) * w; then lane pairs (a, b)
+// accumulate m_ab = m_ab + coef_q * (N_a N_b) over the IPs in order and
+// write the node block diag(m_ab, m_ab, m_ab) (N'N couples equal directions
+// only; the zeros fall under mount_M's eps filter).
+
+template <int NN, int NQ>
+__global__ void __launch_bounds__(64) mass_kernel(int64_t ne, const double *__restrict__ X,
+                                                  const int32_t *__restrict__ conn,
+                                                  const double *__restrict__ rho,
+                                                  const double *__restrict__ dNdR,
+                                                  const double *__restrict__ Nip,
+                                                  const double *__restrict__ wip, double *__restrict__ Me,
+                                                  int32_t *__restrict__ err)
+{
+    const int64_t e = blockIdx.x;
+    const int lane = threadIdx.x;
+    __shared__ double sC[NN * 3];
+    __shared__ double sN[NQ * NN];
+    __shared__ double sCoef[NQ];
+    __shared__ int sBad;
+    for (int i = lane; i < NN * 3; i += 64) sC[i] = X[(int64_t)conn[e * NN + i / 3] * 3 + i % 3];
+    for (int i = lane; i < NQ * NN; i += 64) sN[i] = Nip[i];
+    if (lane == 0) sBad = 0;
+    __syncthreads();
+    if (lane < NQ) {
+        const double *dq = dNdR + lane * NN * 3;
+        double J[9], inv[9];
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) {
+                double a = 0.0;
+                for (int n = 0; n < NN; n++) a = fma(sC[n * 3 + i], dq[n * 3 + j], a);
+                J[i * 3 + j] = a;
+            }
+        const double det = inv3(J, inv);
+        if (!(det > 0.0)) sBad = 1;
+        sCoef[lane] = rho[e] * det * wip[lane];
+    }
+    __syncthreads();
+    for (int pr = lane; pr < NN * NN; pr += 64) {
+        const int a = pr / NN, b = pr % NN;
+        double mab = 0.0;
+        for (int q = 0; q < NQ; q++) mab += sCoef[q] * (sN[q * NN + a] * sN[q * NN + b]);
+        double *o = Me + ((e * NN + a) * NN + b) * 9;
+        o[0] = mab;
+        o[1] = 0.0;
+        o[2] = 0.0;
+        o[3] = 0.0;
+        o[4] = mab;
+        o[5] = 0.0;
+        o[6] = 0.0;
+        o[7] = 0.0;
+        o[8] = mab;
+    }
+    if (lane == 0 && sBad) atomicMin(err, (int32_t)(e + 1));
+}
+
 // --------------------------------------------------------------- assembly
 //
 // One thread per K block: sum the element blocks listed for it in element
@@ -384,9 +443,10 @@ __global__ void __launch_bounds__(256) assemble_kernel(int64_t nblk, const int64
         }
     }
 #pragma unroll
-    for (int s = 0; s < 9; s++) {
-        Kv[kv_at(k, s)] = acc[s];
-        Kv32[kv32_at(k, s)] = (float)acc[s];
+    for (int s = 0; s < 9; s++) Kv[kv_at(k, s)] = acc[s];
+    if (Kv32) {
+#pragma unroll
+        for (int s = 0; s < 9; s++) Kv32[kv32_at(k, s)] = (float)acc[s];
     }
     if (flags) {
 #pragma unroll
@@ -648,6 +708,35 @@ int amr_launch_assemble(amr_model *m, bool flags)
     const unsigned g3 = (unsigned)((m->nnodes + 255) / 256);
     hipLaunchKernelGGL(block_jacobi_kernel, dim3(g3), dim3(256), 0, m->stream, m->nnodes, m->pat.nblk, m->diag, m->Kv,
                        m->mask, m->binv, m->binv32);
+    AMR_HIP(hipGetLastError());
+    return AMR_OK;
+}
+
+template <int NN, int NQ>
+int launch_mass_t(amr_model *m)
+{
+    hipLaunchKernelGGL((mass_kernel<NN, NQ>), dim3((unsigned)m->ne), dim3(64), 0, m->stream, m->ne, m->X, m->conn,
+                       m->rho_d, m->dNdR, m->Nd, m->wip, m->Me, m->err);
+    AMR_HIP(hipGetLastError());
+    return AMR_OK;
+}
+
+int amr_launch_mass(amr_model *m)
+{
+    if (m->ne > 0) {
+        int rc;
+        switch (m->etype) {
+        case AMR_HEX8: rc = launch_mass_t<8, 8>(m); break;
+        case AMR_HEX20: rc = m->nips == 27 ? launch_mass_t<20, 27>(m) : launch_mass_t<20, 8>(m); break;
+        case AMR_TET10: rc = launch_mass_t<10, 4>(m); break;
+        default: amr_set_error("unknown element type"); return AMR_ERR_ARG;
+        }
+        if (rc) return rc;
+    }
+    if (m->pat.nblk == 0) return AMR_OK;
+    const unsigned grid = (unsigned)((m->pat.nblk + 255) / 256);
+    hipLaunchKernelGGL(assemble_kernel, dim3(grid), dim3(256), 0, m->stream, m->pat.nblk, m->cptr, m->csrc, m->Me,
+                       m->Mv, nullptr, m->Mflag);
     AMR_HIP(hipGetLastError());
     return AMR_OK;
 }

@@ -8,6 +8,8 @@ SurfaceBC / EdgeBC / BodyC (src/bc.jl:10-153), ``configure_dofs!``
 (src/bc.jl:198-233) and ``get_bc_vals`` (src/bc.jl:237-249) with the
 distributed loads of src/mech/elem/distributed.jl:75-210.
 """
+import ctypes
+
 import numpy as np
 
 from . import _lib
@@ -67,7 +69,16 @@ class DruckerPrager(Material):
 
 
 class MechSolid:
-    """Marker for the isoparametric 3D bulk element (src/mech/elem/mech-solid.jl)."""
+    """The isoparametric 3D bulk element (src/mech/elem/mech-solid.jl).  In a
+    material binding it may appear as the class or as an instance carrying the
+    element properties of MechSolidProps (src/mech/elem/mech-solid.jl:13-27):
+    ``MechSolid(rho=7850.0)``."""
+
+    def __init__(self, rho=0.0, gamma=0.0):
+        if rho < 0:
+            raise ValueError("MechSolid: rho must be >= 0")
+        self.rho = float(rho)
+        self.gamma = float(gamma)
 
 
 # ------------------------------------------------------- boundary conditions
@@ -207,6 +218,7 @@ class FEModel:
         self.nnodes, self.ne = mesh.nnodes, mesh.ne
         self.nips = self._rule(nips)
         mats, emat = [], -np.ones(self.ne, dtype=np.int32)
+        self.rho = np.zeros(self.ne)   # MechSolidProps.rho per element
         for bind in matbinds:
             flt, mat = bind[0], bind[-1]
             if not isinstance(mat, Material):
@@ -214,6 +226,8 @@ class FEModel:
             sel = self.elem_filter(flt)
             mats.append(mat)
             emat[sel] = len(mats) - 1
+            props = bind[1] if len(bind) == 3 else None
+            self.rho[sel] = props.rho if isinstance(props, MechSolid) else 0.0
         if (emat < 0).any():
             raise ValueError(f"FEModel: missing material definition for {int((emat < 0).sum())} elements")
         self.mats = mats
@@ -600,6 +614,27 @@ class DeviceModel:
         n = ctypes.c_int64(0)
         _lib.check(_lib.lib().amr_timing(self._h, 1 if enable else 0, ctypes.byref(ms), ctypes.byref(n)))
         return ms.value, n.value
+
+    def assemble_M(self, rho=None):
+        """mount_M (src/mech/dyn-solver.jl:72-104) on the device: element
+        mass matrices (elem_mass, src/mech/elem/mech-solid.jl:169-205) and the
+        eps-filtered assembly; rho per element (default: the model's
+        MechSolid densities)."""
+        r = self.model.rho if rho is None else rho
+        r = np.ascontiguousarray(np.broadcast_to(np.asarray(r, dtype=float), (self.model.ne,)))
+        _lib.check(_lib.lib().amr_assemble_M(self._h, _lib.ptr(r)))
+
+    def M_csc(self):
+        """The assembled M as scipy CSC over the current DOF numbering."""
+        import scipy.sparse as sp
+        nnz = ctypes.c_int64(0)
+        L = _lib.lib()
+        _lib.check(L.amr_get_M_csc(self._h, ctypes.byref(nnz), None, None, None))
+        colptr = np.zeros(self.model.ndofs + 1, dtype=np.int64)
+        rowval = np.zeros(max(nnz.value, 1), dtype=np.int32)
+        nzval = np.zeros(max(nnz.value, 1))
+        _lib.check(L.amr_get_M_csc(self._h, ctypes.byref(nnz), _lib.ptr(colptr), _lib.ptr(rowval), _lib.ptr(nzval)))
+        return sp.csc_matrix((nzval[:nnz.value], rowval[:nnz.value], colptr), shape=(self.model.ndofs,) * 2)
 
     # -------------------------------------------------------- outputs
     def ip_values(self, committed=False):

@@ -283,6 +283,35 @@ def post_leg(am, dev, model, args):
     return out
 
 
+def mass_leg(am, dev, model, args):
+    """mount_M on the block (rho = 2400): element mass matrices and the
+    eps-filtered assembly into the device matrix, timed per call; the
+    oracle's restatement is timed on a smaller block."""
+    rho = np.full(model.ne, 2400.0)
+    dev.assemble_M(rho)           # first call also allocates the mass buffers
+    ts = []
+    for _ in range(args.mass):
+        t0 = time.perf_counter()
+        dev.assemble_M(rho)
+        ts.append(time.perf_counter() - t0)
+    t = float(np.median(ts))
+    out = {"workload": f"mount_M, HEX20 {args.n}^3 block (27 IP), rho 2400: {model.ne} elements",
+           "seconds": t, "elements_per_s": model.ne / t, "calls": len(ts)}
+    log(f"mass matrix: {t * 1e3:.2f} ms per call")
+    if not args.no_cpu:
+        from oracle import HEX20 as O20, fem as OF
+        _, model_c, bcs_c, _, _ = block_problem(am, args.cpu_n)
+        om = OF.Model(model_c.coords, model_c.conn, O20, [("LinearElastic", (30e9, 0.2))], nips=27)
+        om.configure_dofs([("z==0", OF.SurfaceBC(ux=0, uy=0, uz=0))])
+        t0 = time.perf_counter()
+        om.mount_M(2400.0)
+        tc = time.perf_counter() - t0
+        out["cpu_oracle"] = {"sample": f"HEX20 {args.cpu_n}^3 (27 IP), {model_c.ne} elements, C (1 thread) + "
+                                       "triplet assembly", "seconds": tc, "elements_per_s": model_c.ne / tc}
+        log(f"mass matrix oracle on {args.cpu_n}^3: {tc:.2f}s")
+    return out
+
+
 def served_leg(am, args):
     """Request stream: plain one-at-a-time capacity, then the server at a
     Poisson rate that is a multiple of it."""
@@ -416,6 +445,7 @@ def main():
                     help="0.1 m (the nominal value) collapses the perfectly plastic soil before the end")
     ap.add_argument("--footing-autoinc", action="store_true")
     ap.add_argument("--post", type=int, default=3, help="timed nodal-recovery calls on the block (0 = skip)")
+    ap.add_argument("--mass", type=int, default=3, help="timed mount_M calls on the block (0 = skip)")
     ap.add_argument("--serve-seed", type=int, default=1)
     ap.add_argument("--serve-load", type=float, default=4.0,
                     help="Poisson arrival rate as a multiple of the plain path's request rate")
@@ -544,6 +574,8 @@ def main():
                                           f"{cb['t_solve']:.2f}s (1 thread), update {cb['t_update']:.2f}s")}
     if args.post:
         out["post_processing"] = post_leg(am, dev, model, args)
+    if args.mass:
+        out["mass_matrix"] = mass_leg(am, dev, model, args)
     dev.close()
     for name, leg, n in (("batched", batched_leg, args.batch), ("served", served_leg, args.serve),
                          ("footing", footing_leg, args.footing)):

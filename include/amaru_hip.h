@@ -124,6 +124,16 @@ int amr_assemble(amr_model *m);
 
 /* Element matrices of the last amr_assemble, ne × nd × nd row-major. */
 int amr_get_Ke(amr_model *m, double *Ke);
+/*
+ * mount_M (src/mech/dyn-solver.jl:72-104): element mass matrices (elem_mass,
+ * src/mech/elem/mech-solid.jl:169-205, thickness 1) with rho[e] per element
+ * (MechSolidProps.rho), assembled with mount_K's rule (|Me_ij| < eps
+ * dropped, survivors summed in element order) into a device matrix of its
+ * own, kept beside K.  Errors as amr_assemble.
+ */
+int amr_assemble_M(amr_model *m, const double *rho);
+/* CSC of the last amr_assemble_M, as amr_get_K_csc. */
+int amr_get_M_csc(amr_model *m, int64_t *nnz, int64_t *colptr, int32_t *rowval, double *nzval);
 
 /*
  * The assembled K as the reference's SparseMatrixCSC (0-based): column

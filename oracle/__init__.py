@@ -6,8 +6,8 @@ checker.  The product (``amaru.jl_amd``) never imports it.
 
 Contents
 --------
-``amaru_oracle.c``  C restatement of the element/material arithmetic and the
-                    mount_K triplet rule (built by ``Makefile`` into
+``amaru_oracle.c``  C restatement of the element/material arithmetic
+                    (stiffness, mass, IP updates) and the mount_K triplet rule (built by ``Makefile`` into
                     ``libamaru_oracle.so``).
 ``fem.py``          Python restatement of the structured mesher, DOF
                     numbering, boundary conditions, solve_system! (SuperLU
@@ -59,6 +59,8 @@ def lib():
     L.ora_update_elems.argtypes = [i32, i32, i64, P, P, P, P, P, P, i32, P, P, P, P, i32]
     L.ora_gather_forces.argtypes = [i64, i32, P, P, i64, P]
     L.ora_count_triplets.argtypes = [i64, i32, P]
+    L.ora_elem_mass.argtypes = [i32, i32, i64, P, P, P, P]
+    L.ora_elem_mass.restype = i64
     L.ora_count_triplets.restype = i64
     L.ora_mount_K.argtypes = [i64, i32, P, P, i64, P, P, P]
     L.ora_mount_K.restype = i64

@@ -641,6 +641,41 @@ void ora_gather_forces(int64_t ne, int nd, const double *dF, const int32_t *eqma
         for (int c = 0; c < nd; c++) out[eqmap[e * nd + c]] += dF[e * nd + c];
 }
 
+/*
+ * Element mass matrices (elem_mass, src/mech/elem/mech-solid.jl:169-205) for
+ * ne elements of one type, rho per element, thickness 1.  For each IP:
+ *   coef = (rho * detJ) * w            (detJ as in ip_geometry)
+ *   M[(a,i)][(b,i)] = M + coef * (N_a * N_b)   (N'N has one nonzero term)
+ * entries with i != j stay 0.  Me is ne × nd × nd row-major (dof 3a+i).
+ * Returns 0, or 1 + (index of the first element with a non-positive
+ * Jacobian).
+ */
+int64_t ora_elem_mass(int etype, int nips, int64_t ne, const double *X, const int32_t *conn, const double *rho,
+                      double *Me)
+{
+    rule_t r;
+    if (make_rule(etype, nips, &r)) return -1;
+    const int nn = r.nn, nd = 3 * nn, nq = r.nips;
+    double N[MAXIP][MAXNN];
+    for (int q = 0; q < nq; q++) ora_shape_func(etype, r.R + 3 * q, N[q]);
+    for (int64_t e = 0; e < ne; e++) {
+        double C[MAXNN * 3], dNdX[MAXNN * 3], dcoef;
+        gather_coords(nn, X, conn + e * nn, C);
+        double *M = Me + e * nd * nd;
+        memset(M, 0, sizeof(double) * nd * nd);
+        for (int q = 0; q < nq; q++) {
+            if (ip_geometry(nn, r.dNdR[q], 1.0, C, dNdX, &dcoef)) return e + 1;
+            const double coef = rho[e] * dcoef * r.w[q];
+            for (int a = 0; a < nn; a++)
+                for (int b = 0; b < nn; b++) {
+                    const double t = N[q][a] * N[q][b];
+                    for (int i = 0; i < 3; i++) M[(a * 3 + i) * nd + b * 3 + i] += coef * t;
+                }
+        }
+    }
+    return 0;
+}
+
 /* ------------------------------------------------------------- mount_K */
 
 /* Number of triplets that survive the |v| < eps filter. */

@@ -442,6 +442,22 @@ class Model:
         Ke = self.elem_stiffness(state, nthreads)
         return csr_from_Ke(Ke, self.eqmap, self.ndofs)
 
+    def elem_mass(self, rho):
+        """Me for all elements (C restatement of elem_mass,
+        src/mech/elem/mech-solid.jl:169-205); rho per element."""
+        nd = 3 * self.nen
+        Me = np.zeros((self.ne, nd, nd))
+        rho = np.ascontiguousarray(np.broadcast_to(np.asarray(rho, dtype=float), (self.ne,)))
+        bad = lib().ora_elem_mass(self.etype, self.nips, self.ne, ptr(self.X), ptr(self.conn), ptr(rho), ptr(Me))
+        if bad:
+            raise RuntimeError(f"Negative Jacobian determinant in cell {bad}")
+        return Me
+
+    def mount_M(self, rho):
+        """mount_M (src/mech/dyn-solver.jl:72-104): the mount_K triplet rule
+        over the element mass matrices → scipy CSR (ndofs²)."""
+        return csr_from_Ke(self.elem_mass(rho), self.eqmap, self.ndofs)
+
     def update_state(self, dUt, state_in):
         """update_state!(elems, ΔUt, t) (src/mech/mech-solver.jl:124-144).
         Returns (ΔFin, new_state, status_message_or_None)."""
