@@ -183,8 +183,9 @@ struct amr_model {
     std::vector<double> Nip;        // nips*nn shape functions at the IPs
     int elem_kind = -1;             // material kind of every element, -2 if they differ
     struct AmrRecovery *rec = nullptr;
-    // mass matrix (mount_M): element blocks, assembled values and survivor
-    // flags in the kv_at layout of K, shape functions at the IPs, densities
+    // mass matrix (mount_M): one value per element node pair (the block is
+    // that value times I3), assembled values and survivor flags in the kv_at
+    // layout of K, shape functions at the IPs, densities
     double *Me = nullptr, *Mv = nullptr, *Nd = nullptr, *rho_d = nullptr;
     uint8_t *Mflag = nullptr;
     bool m_assembled = false;

@@ -631,7 +631,7 @@ extern "C" int amr_assemble_M(amr_model *m, const double *rho)
         }
     AMR_HIP(hipSetDevice(m->device));
     const size_t nq = (size_t)m->nips, nn = (size_t)m->nn;
-    if (!m->Me) TRY(dalloc(m, &m->Me, (size_t)m->ne * nn * nn * 9));
+    if (!m->Me) TRY(dalloc(m, &m->Me, (size_t)m->ne * nn * nn));
     if (!m->Mv) TRY(dalloc(m, &m->Mv, (size_t)kv_size(m->pat.nblk)));
     if (!m->Mflag) TRY(dalloc(m, &m->Mflag, (size_t)kv_size(m->pat.nblk)));
     if (!m->Nd) TRY(dalloc(m, &m->Nd, nq * nn));

@@ -357,9 +357,10 @@ __global__ void __launch_bounds__(64) ke_kernel(int64_t ne, const double *__rest
 //
 // One 64-lane workgroup per element.  Lanes q < NQ build J (the stiffness
 // kernel's fma chains) and coef = (rho * detJ) * w; then lane pairs (a, b)
-// accumulate m_ab = m_ab + coef_q * (N_a N_b) over the IPs in order and
-// write the node block diag(m_ab, m_ab, m_ab) (N'N couples equal directions
-// only; the zeros fall under mount_M's eps filter).
+// accumulate m_ab = m_ab + coef_q * (N_a N_b) over the IPs in order.  N'N
+// couples equal directions only, so the node block of Me is diag(m_ab) and
+// one value per node pair is stored (Me[(e*NN + a)*NN + b]); the six zeros
+// of each block fall under mount_M's eps filter.
 
 template <int NN, int NQ>
 __global__ void __launch_bounds__(64) mass_kernel(int64_t ne, const double *__restrict__ X,
@@ -398,18 +399,37 @@ __global__ void __launch_bounds__(64) mass_kernel(int64_t ne, const double *__re
         const int a = pr / NN, b = pr % NN;
         double mab = 0.0;
         for (int q = 0; q < NQ; q++) mab += sCoef[q] * (sN[q * NN + a] * sN[q * NN + b]);
-        double *o = Me + ((e * NN + a) * NN + b) * 9;
-        o[0] = mab;
-        o[1] = 0.0;
-        o[2] = 0.0;
-        o[3] = 0.0;
-        o[4] = mab;
-        o[5] = 0.0;
-        o[6] = 0.0;
-        o[7] = 0.0;
-        o[8] = mab;
+        Me[(e * NN + a) * NN + b] = mab;
     }
     if (lane == 0 && sBad) atomicMin(err, (int32_t)(e + 1));
+}
+
+// One thread per block: the block's element values m_ab (|m| >= eps) summed
+// in element order from 0.0, on the three diagonal positions; the others are
+// 0 and not in the pattern.  The same sums and flags as the generic block
+// assembly over diag(m_ab) blocks.
+__global__ void __launch_bounds__(256) mass_assemble_kernel(int64_t nblk, const int64_t *__restrict__ cptr,
+                                                            const int32_t *__restrict__ csrc,
+                                                            const double *__restrict__ Me, double *__restrict__ Mv,
+                                                            uint8_t *__restrict__ flags)
+{
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nblk) return;
+    double acc = 0.0;
+    bool any = false;
+    for (int64_t p = cptr[k]; p < cptr[k + 1]; p++) {
+        const double x = Me[csrc[p]];
+        if (!(fabs(x) < DEPS)) {
+            acc += x;
+            any = true;
+        }
+    }
+#pragma unroll
+    for (int s = 0; s < 9; s++) {
+        const bool d = s == 0 || s == 4 || s == 8;
+        Mv[kv_at(k, s)] = d ? acc : 0.0;
+        flags[kv_at(k, s)] = d && any;
+    }
 }
 
 // --------------------------------------------------------------- assembly
@@ -735,8 +755,8 @@ int amr_launch_mass(amr_model *m)
     }
     if (m->pat.nblk == 0) return AMR_OK;
     const unsigned grid = (unsigned)((m->pat.nblk + 255) / 256);
-    hipLaunchKernelGGL(assemble_kernel, dim3(grid), dim3(256), 0, m->stream, m->pat.nblk, m->cptr, m->csrc, m->Me,
-                       m->Mv, nullptr, m->Mflag);
+    hipLaunchKernelGGL(mass_assemble_kernel, dim3(grid), dim3(256), 0, m->stream, m->pat.nblk, m->cptr, m->csrc,
+                       m->Me, m->Mv, m->Mflag);
     AMR_HIP(hipGetLastError());
     return AMR_OK;
 }
