@@ -90,9 +90,11 @@ def _grid_points(bl, sub):
 
 
 class Mesh:
-    """Mesh(blocks...) for solid blocks of one cell shape."""
+    """Mesh(blocks...) for solid blocks of one cell shape.  With ``device``
+    (a GPU index) a single block is generated there (``amr_block_mesh``):
+    the same coordinates and connectivity, bit for bit."""
 
-    def __init__(self, *blocks):
+    def __init__(self, *blocks, device=None):
         blocks = [b for bl in blocks for b in (bl if isinstance(bl, (list, tuple)) else [bl])]
         if not blocks:
             raise ValueError("Mesh: no blocks")
@@ -100,6 +102,13 @@ class Mesh:
         if len(shapes) != 1:
             raise ValueError("Mesh: all blocks must use one cell shape")
         self.cellshape = blocks[0].cellshape
+        self._facets = None
+        self._edges = None
+        if device is not None:
+            if len(blocks) != 1:
+                raise ValueError("Mesh: device generation takes one block")
+            self._device_block(blocks[0], int(device))
+            return
         coords, conns, tags = [], [], []
         index = {}
         nnodes = 0
@@ -140,6 +149,21 @@ class Mesh:
         self.tags = np.array(tags)
         self._facets = None
         self._edges = None
+
+    def _device_block(self, bl, device):
+        import ctypes
+        from . import _lib
+        et = {HEX8: _lib.AMR_HEX8, HEX20: _lib.AMR_HEX20, TET10: _lib.AMR_TET10}[self.cellshape]
+        corners = np.ascontiguousarray(bl.corners, dtype=float)
+        nn, ne = ctypes.c_int64(0), ctypes.c_int64(0)
+        L = _lib.lib()
+        _lib.check(L.amr_block_mesh(device, et, _lib.ptr(corners), bl.nx, bl.ny, bl.nz, ctypes.byref(nn),
+                                    ctypes.byref(ne), None, None))
+        self.coords = np.empty((nn.value, 3))
+        self.conn = np.empty((ne.value, NNODES[self.cellshape]), dtype=np.int32)
+        _lib.check(L.amr_block_mesh(device, et, _lib.ptr(corners), bl.nx, bl.ny, bl.nz, ctypes.byref(nn),
+                                    ctypes.byref(ne), _lib.ptr(self.coords), _lib.ptr(self.conn)))
+        self.tags = np.array([bl.tag] * ne.value)
 
     def _cells(self, P, bl, sub):
         nx, ny, nz = bl.nx, bl.ny, bl.nz

@@ -284,9 +284,11 @@ class FEModel:
         return (owners[keep], nodes[keep]), nodes[keep].ravel()
 
     # ----------------------------------------------------------- dofs
-    def configure_dofs(self, bcs):
+    def configure_dofs(self, bcs, device=None):
         """configure_dofs! (src/bc.jl:198-233): DOFs of the active nodes in node
-        order (ux,uy,uz), stable partition unknowns-first, eq ids = position."""
+        order (ux,uy,uz), stable partition unknowns-first, eq ids = position.
+        With ``device`` (a GPU index) the numbering is computed there
+        (``amr_dof_map``) from the prescribed flags selected here."""
         # the same boundary conditions again (a stage re-configures the DOFs it
         # was set up with): the numbering cannot have changed
         dkey = tuple((str(flt), bc.kind, tuple(sorted((k, repr(v)) for k, v in bc.conds.items())))
@@ -302,6 +304,18 @@ class FEModel:
                 if key in ESSENTIAL:
                     presc[nodes, DIR[key[1]]] = True
             self._bcsel.append(sel)
+        if device is not None:
+            act = np.zeros(self.nnodes, dtype=np.uint8)
+            act[active] = 1
+            pre = np.ascontiguousarray(presc.astype(np.uint8).ravel())
+            eq = np.empty(self.nnodes * 3, dtype=np.int32)
+            nd, nu = ctypes.c_int64(0), ctypes.c_int64(0)
+            _lib.check(_lib.lib().amr_dof_map(int(device), self.nnodes, _lib.ptr(act), _lib.ptr(pre), _lib.ptr(eq),
+                                              ctypes.byref(nd), ctypes.byref(nu)))
+            self.eq = eq.reshape(self.nnodes, 3)
+            self.ndofs, self.nu = int(nd.value), int(nu.value)
+            self._dof_key = dkey
+            return self.ndofs, self.nu
         dof = (active[:, None] * 3 + np.arange(3)[None, :]).ravel()
         p = presc.ravel()[dof]
         order = np.concatenate([dof[~p], dof[p]])

@@ -312,6 +312,39 @@ def mass_leg(am, dev, model, args):
     return out
 
 
+def meshgen_leg(am, args):
+    """The block's mesh (split_block!) and DOF map (configure_dofs!) built on
+    the device against the host restatement, timed per call (arrays returned
+    to the host in both)."""
+    bl = am.Block([[0, 0, 0], [1, 1, 1]], nx=args.n, ny=args.n, nz=args.n, cellshape="HEX20")
+    bcs = [("z==0", am.SurfaceBC(ux=0, uy=0, uz=0)), ("z==1", am.SurfaceBC(tz=-1e6))]
+    am.Mesh(bl, device=0)
+    td, th = [], []
+    for _ in range(args.meshgen):
+        t0 = time.perf_counter()
+        md = am.Mesh(bl, device=0)
+        td.append(time.perf_counter() - t0)
+        t0 = time.perf_counter()
+        mh = am.Mesh(bl)
+        th.append(time.perf_counter() - t0)
+    same = bool(np.array_equal(md.coords, mh.coords) and np.array_equal(md.conn, mh.conn))
+    a = am.FEModel(mh, [("all", am.LinearElastic(E=30e9, nu=0.2))], nips=27)
+    b = am.FEModel(md, [("all", am.LinearElastic(E=30e9, nu=0.2))], nips=27)
+    t0 = time.perf_counter()
+    a.configure_dofs(bcs)
+    t_dof_h = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    b.configure_dofs(bcs, device=0)
+    t_dof_d = time.perf_counter() - t0
+    out = {"workload": f"HEX20 {args.n}^3 block: {mh.nnodes} nodes, {mh.ne} elements",
+           "mesh_device_s": float(np.median(td)), "mesh_host_s": float(np.median(th)),
+           "dofs_device_s": t_dof_d, "dofs_host_s": t_dof_h,
+           "bit_identical": same and bool(np.array_equal(a.eq, b.eq))}
+    log(f"mesh generation: device {out['mesh_device_s'] * 1e3:.1f} ms, host {out['mesh_host_s'] * 1e3:.1f} ms; "
+        f"dof map device {t_dof_d * 1e3:.1f} ms, host {t_dof_h * 1e3:.1f} ms; identical {out['bit_identical']}")
+    return out
+
+
 def served_leg(am, args):
     """Request stream: plain one-at-a-time capacity, then the server at a
     Poisson rate that is a multiple of it."""
@@ -446,6 +479,7 @@ def main():
     ap.add_argument("--footing-autoinc", action="store_true")
     ap.add_argument("--post", type=int, default=3, help="timed nodal-recovery calls on the block (0 = skip)")
     ap.add_argument("--mass", type=int, default=3, help="timed mount_M calls on the block (0 = skip)")
+    ap.add_argument("--meshgen", type=int, default=3, help="timed device mesh generations of the block (0 = skip)")
     ap.add_argument("--serve-seed", type=int, default=1)
     ap.add_argument("--serve-load", type=float, default=4.0,
                     help="Poisson arrival rate as a multiple of the plain path's request rate")
@@ -576,6 +610,8 @@ def main():
         out["post_processing"] = post_leg(am, dev, model, args)
     if args.mass:
         out["mass_matrix"] = mass_leg(am, dev, model, args)
+    if args.meshgen:
+        out["mesh_generation"] = meshgen_leg(am, args)
     dev.close()
     for name, leg, n in (("batched", batched_leg, args.batch), ("served", served_leg, args.serve),
                          ("footing", footing_leg, args.footing)):

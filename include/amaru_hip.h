@@ -322,6 +322,27 @@ int amr_model_device_bytes(amr_model *m, int64_t *bytes);
 int amr_mem_info(int device, int64_t *free_bytes, int64_t *total_bytes);
 
 /*
+ * Structured block mesh on the device: split_block! of one Block
+ * (src/mesh/structured.jl:384-552) with the synchronize! numbering
+ * (src/mesh/mesh.jl:348-350).  corners: the block's 8 corners (8x3, HEX8
+ * order); nx, ny, nz cells per axis.  Outputs: node coordinates (nnodes x 3,
+ * rounded to 8 decimals as src/node.jl:58-60) and connectivity (ne x nen,
+ * 0-based; TET10: six tetrahedra per cell).  With coords and conn NULL only
+ * nnodes and ne are returned.
+ */
+int amr_block_mesh(int device, int etype, const double *corners, int64_t nx, int64_t ny, int64_t nz,
+                   int64_t *nnodes, int64_t *ne, double *coords, int32_t *conn);
+/*
+ * DOF numbering of configure_dofs! (src/bc.jl:198-233) on the device: the
+ * DOFs (node, ux/uy/uz) of active nodes in node order, unknowns first, then
+ * prescribed, each in DOF order.  active (nnodes, or NULL: all) and presc
+ * (nnodes x 3) are 0/1 flags; eq (nnodes x 3) receives the 0-based eq ids,
+ * -1 for DOFs of inactive nodes.
+ */
+int amr_dof_map(int device, int64_t nnodes, const uint8_t *active, const uint8_t *presc, int32_t *eq,
+                int64_t *ndofs, int64_t *nu);
+
+/*
  * Output post-processing.  Values per IP are ip_state_vals of the solid
  * materials (src/mech/mat/linear-elastic.jl:140-142, von-mises.jl:159-168,
  * drucker-prager.jl:152-163) over stress_strain_dict
