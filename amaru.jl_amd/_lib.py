@@ -89,6 +89,8 @@ SIGNATURES = {
     "amr_set_coarse": (I32, [P, I32, I32]),
     "amr_get_coarse": (I32, [P, P]),
     "amr_get_coarse_space": (I32, [P, P, P]),
+    "amr_check_expr": (I32, [P, I32, I32]),
+    "amr_cell_loads": (I32, [I32, ctypes.c_int64, I32, I32, I32, P, P, P, P, I32, P, P, I32, I32, D, P]),
     "amr_block_mesh": (I32, [I32, I32, P, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, P, P, P, P]),
     "amr_dof_map": (I32, [I32, ctypes.c_int64, P, P, P, P, P]),
     "amr_assemble_M": (I32, [P, P]),

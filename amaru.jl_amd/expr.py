@@ -80,3 +80,52 @@ def compile_expr(src):
         return np.broadcast_to(np.asarray(ev(tree, env)), x.shape)
 
     return f
+
+
+# postfix programs for the device (the operation codes of AMR_OP_* in
+# include/amaru_hip.h)
+OP = {"const": 0, "x": 1, "y": 2, "z": 3, "t": 4, "neg": 5, "sqrt": 6, "abs": 7, "sin": 8, "cos": 9, "exp": 10,
+      "log": 11, ast.Add: 12, ast.Sub: 13, ast.Mult: 14, ast.Div: 15, ast.Pow: 16, ast.Mod: 17}
+
+
+def compile_program(src):
+    """A numeric value expression as a postfix program: (ops (n x 2 int32:
+    operation, argument), consts (float64)).  Comparisons and boolean logic
+    (selection filters) are not load values and are refused."""
+    if callable(src):
+        raise ValueError("a Python callable cannot run on the device; give the value as an expression")
+    ops, consts = [], []
+
+    def const(v):
+        consts.append(float(v))
+        ops.append((OP["const"], len(consts) - 1))
+
+    if isinstance(src, (int, float, np.floating, np.integer)):
+        const(src)
+    else:
+        tree = ast.parse(_to_python(str(src)), mode="eval").body
+
+        def emit(node):
+            if isinstance(node, ast.Constant) and isinstance(node.value, (int, float)) \
+                    and not isinstance(node.value, bool):
+                const(node.value)
+            elif isinstance(node, ast.Name) and node.id in ("x", "y", "z", "t"):
+                ops.append((OP[node.id], 0))
+            elif isinstance(node, ast.Name) and node.id in _CONST:
+                const(_CONST[node.id])
+            elif isinstance(node, ast.UnaryOp) and isinstance(node.op, (ast.USub, ast.UAdd)):
+                emit(node.operand)
+                if isinstance(node.op, ast.USub):
+                    ops.append((OP["neg"], 0))
+            elif isinstance(node, ast.BinOp) and type(node.op) in OP:
+                emit(node.left)
+                emit(node.right)
+                ops.append((OP[type(node.op)], 0))
+            elif isinstance(node, ast.Call) and isinstance(node.func, ast.Name) and node.func.id in _FUNCS \
+                    and len(node.args) == 1:
+                emit(node.args[0])
+                ops.append((OP[node.func.id], 0))
+            else:
+                raise ValueError(f"not a numeric load expression: {ast.dump(node)} in {src!r}")
+        emit(tree)
+    return np.ascontiguousarray(np.array(ops, dtype=np.int32).reshape(-1, 2)), np.array(consts, dtype=float)

@@ -196,6 +196,36 @@ def distributed_forces(C, key, val, t=0.0):
     return F
 
 
+def cell_loads(device, C, N, D, w, d, val, t=0.0):
+    """amr_cell_loads: force vectors (ncells, npts, 3) of a distributed load
+    over cells with node coordinates C (ncells, npts, 3), computed on the
+    device; N, D, w the cell shape's tables, d a direction (0..2) or -1 for
+    the facet normal, val a number or expression over x, y, z, t."""
+    from .expr import compile_program
+    ops, consts = compile_program(val)
+    C = np.ascontiguousarray(C, dtype=float)
+    nc, npts, _ = C.shape
+    N = np.ascontiguousarray(N, dtype=float)
+    D = np.ascontiguousarray(D, dtype=float)
+    w = np.ascontiguousarray(w, dtype=float)
+    nq = len(w)
+    ndim = D.shape[-1] if D.ndim == 3 else 1
+    D = np.ascontiguousarray(D.reshape(nq, npts, ndim))
+    F = np.empty_like(C)
+    _lib.check(_lib.lib().amr_cell_loads(int(device), nc, npts, ndim, nq, _lib.ptr(C), _lib.ptr(N),
+                                         _lib.ptr(D), _lib.ptr(w), int(d),
+                                         _lib.ptr(ops), _lib.ptr(consts) if consts.size else None, len(ops),
+                                         len(consts), float(t), _lib.ptr(F)))
+    return F
+
+
+def distributed_forces_device(device, C, key, val, t=0.0):
+    """distributed_forces on the device: the same (nf, npts, 3) result."""
+    npts = C.shape[1]
+    N, D, w = _facet_tables(npts)
+    return cell_loads(device, C, N, D, w, DIR[key[1]], val, t)
+
+
 # ------------------------------------------------------------------ model
 
 class FEModel:
@@ -327,9 +357,10 @@ class FEModel:
         self._dof_key = dkey
         return self.ndofs, self.nu
 
-    def bc_vals(self, bcs, t=0.0):
+    def bc_vals(self, bcs, t=0.0, device=None):
         """get_bc_vals (src/bc.jl:237-249): essential values U and natural
-        loads F, in eq order."""
+        loads F, in eq order.  With ``device`` (a GPU index) the distributed
+        loads (tractions, line loads, body forces) are integrated there."""
         U = np.zeros(self.ndofs)
         F = np.zeros(self.ndofs)
         X = self.coords
@@ -353,20 +384,24 @@ class FEModel:
                     U[self.eq[nn, DIR[key[1]]]] = f(X[nn, 0], X[nn, 1], X[nn, 2], t)
                     continue
                 if bc.kind == "body":
-                    Fd = self._body_forces(sel, key, f)
+                    Fd = self._body_forces(sel, key, f, val, device)
+                elif device is not None:
+                    Fd = distributed_forces_device(device, X[nodes], key, val, t)
                 else:
                     Fd = distributed_forces(X[nodes], key, val, t)
                 np.add.at(F, self.eq[nodes].reshape(-1), Fd.reshape(-1))
         return U, F
 
-    def _body_forces(self, elems, key, f):
+    def _body_forces(self, elems, key, f, val=None, device=None):
         """mech_solid_body_forces (src/mech/elem/distributed.jl:155-210) with
-        the shape's default rule."""
+        the shape's default rule (values at t = 0)."""
         from .shapes import rule_tables
         N, D, w = rule_tables(self.shape, 0)
         C = self.coords[self.conn[elems]]
-        F = np.zeros(C.shape)
         d = DIR[key[1]]
+        if device is not None:
+            return cell_loads(device, C, N, D, w, d, val, 0.0)
+        F = np.zeros(C.shape)
         for q in range(len(w)):
             J = np.einsum("eak,ai->eki", C, D[q])
             X = np.einsum("eak,a->ek", C, N[q])

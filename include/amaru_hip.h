@@ -322,6 +322,30 @@ int amr_model_device_bytes(amr_model *m, int64_t *bytes);
 int amr_mem_info(int device, int64_t *free_bytes, int64_t *total_bytes);
 
 /*
+ * Distributed loads on the device, for many cells at once: the per-cell
+ * force vectors of mech_boundary_forces (tractions tx/ty/tz/tn on facets,
+ * ndim 2; line loads on edges, ndim 1; src/mech/elem/distributed.jl:75-150)
+ * and of the solid body forces (wx/wy/wz on elements, ndim 3; :155-210).
+ * C: ncells x npts x 3 node coordinates; N (nq x npts), D (nq x npts x
+ * ndim) and w (nq): the cell shape's functions, derivatives and weights at
+ * its integration points; dir 0..2 a global direction, -1 the unit normal
+ * (facets).  The load value is a postfix program over x, y, z, t: nops
+ * (operation, argument) pairs, the argument indexing consts for
+ * AMR_OP_CONST; at most AMR_EXPR_STACK values deep (checked before any
+ * launch).  F: ncells x npts x 3.
+ */
+enum {
+    AMR_OP_CONST = 0, AMR_OP_X, AMR_OP_Y, AMR_OP_Z, AMR_OP_T,                             /* push */
+    AMR_OP_NEG, AMR_OP_SQRT, AMR_OP_ABS, AMR_OP_SIN, AMR_OP_COS, AMR_OP_EXP, AMR_OP_LOG,  /* unary */
+    AMR_OP_ADD, AMR_OP_SUB, AMR_OP_MUL, AMR_OP_DIV, AMR_OP_POW, AMR_OP_MOD                /* binary (MOD floored) */
+};
+#define AMR_EXPR_STACK 16
+int amr_check_expr(const int32_t *ops, int nops, int nconsts);
+int amr_cell_loads(int device, int64_t ncells, int npts, int ndim, int nq, const double *C, const double *N,
+                   const double *D, const double *w, int dir, const int32_t *ops, const double *consts, int nops,
+                   int nconsts, double t, double *F);
+
+/*
  * Structured block mesh on the device: split_block! of one Block
  * (src/mesh/structured.jl:384-552) with the synchronize! numbering
  * (src/mesh/mesh.jl:348-350).  corners: the block's 8 corners (8x3, HEX8
