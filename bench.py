@@ -345,6 +345,43 @@ def meshgen_leg(am, args):
     return out
 
 
+def loads_leg(am, model, args):
+    """Body force with an expression value over every element of the block
+    and a traction on its top facets, on the device against the host
+    restatement (force vectors returned to the host in both)."""
+    from amaru_amd.expr import compile_expr
+    val = "-2400*9.81*(1 + 0.1*sin(3*x)*cos(2*y))"
+    elems = np.arange(model.ne)
+    f = compile_expr(val)
+    model._body_forces(elems, "wz", f, val, device=0)
+    td = []
+    for _ in range(args.loads):
+        t0 = time.perf_counter()
+        Fd = model._body_forces(elems, "wz", f, val, device=0)
+        td.append(time.perf_counter() - t0)
+    t0 = time.perf_counter()
+    Fh = model._body_forces(elems, "wz", f)
+    th = time.perf_counter() - t0
+    err = float(np.abs(Fd - Fh).max() / np.abs(Fh).max())
+    _, fn = model.mesh.facets_where(model.coords[:, 2] == 1.0)
+    from amaru_amd import model as M
+    C = model.coords[fn]
+    t0 = time.perf_counter()
+    Td = M.distributed_forces_device(0, C, "tn", "-1e6*(1 + x*y)")
+    t_tr_d = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    Th = M.distributed_forces(C, "tn", "-1e6*(1 + x*y)")
+    t_tr_h = time.perf_counter() - t0
+    out = {"workload": f"body force wz = {val} on {model.ne} HEX20 elements (27 IP); traction tn on {len(fn)} "
+                       "QUAD8 top facets",
+           "body_device_s": float(np.median(td)), "body_host_s": th, "body_rel_diff": err,
+           "traction_device_s": t_tr_d, "traction_host_s": t_tr_h,
+           "traction_rel_diff": float(np.abs(Td - Th).max() / np.abs(Th).max())}
+    log(f"loads: body device {out['body_device_s'] * 1e3:.1f} ms, host {th * 1e3:.0f} ms (rel diff {err:.1e}); "
+        f"traction device {t_tr_d * 1e3:.1f} ms, host {t_tr_h * 1e3:.1f} ms")
+    return out
+
+
 def served_leg(am, args):
     """Request stream: plain one-at-a-time capacity, then the server at a
     Poisson rate that is a multiple of it."""
@@ -480,6 +517,7 @@ def main():
     ap.add_argument("--post", type=int, default=3, help="timed nodal-recovery calls on the block (0 = skip)")
     ap.add_argument("--mass", type=int, default=3, help="timed mount_M calls on the block (0 = skip)")
     ap.add_argument("--meshgen", type=int, default=3, help="timed device mesh generations of the block (0 = skip)")
+    ap.add_argument("--loads", type=int, default=3, help="timed device body-force integrations on the block (0 = skip)")
     ap.add_argument("--serve-seed", type=int, default=1)
     ap.add_argument("--serve-load", type=float, default=4.0,
                     help="Poisson arrival rate as a multiple of the plain path's request rate")
@@ -612,6 +650,8 @@ def main():
         out["mass_matrix"] = mass_leg(am, dev, model, args)
     if args.meshgen:
         out["mesh_generation"] = meshgen_leg(am, args)
+    if args.loads:
+        out["distributed_loads"] = loads_leg(am, model, args)
     dev.close()
     for name, leg, n in (("batched", batched_leg, args.batch), ("served", served_leg, args.serve),
                          ("footing", footing_leg, args.footing)):
