@@ -435,8 +435,7 @@ static int build_recovery(amr_model *m)
         pn_patch.resize(pn_node.size(), p);
     }
     // node -> patches, ascending patch (the reference's summation order)
-    auto r = new AmrRecovery;
-    m->rec = r;
+    AmrRecovery *r = m->rec ? m->rec : (m->rec = new AmrRecovery);   // amr_ip_values may have made it
     r->npatch = (int32_t)owner.size();
     r->maxm = maxm;
     r->reps.assign(nnodes, 0);
@@ -455,7 +454,7 @@ static int build_recovery(amr_model *m)
     if (!rc) rc = alloc(m, &r->bad, (size_t)np, r->bytes);
     if (!rc) rc = alloc(m, &r->pcen, pcen.size(), r->bytes);
     if (!rc) rc = alloc(m, &r->ipx, ipx.size(), r->bytes);
-    if (!rc) rc = alloc(m, &r->vals, (size_t)m->nip_total * NF, r->bytes);
+    if (!rc && !r->vals) rc = alloc(m, &r->vals, (size_t)m->nip_total * NF, r->bytes);
     if (!rc) rc = alloc(m, &r->coef, (size_t)np * NT * NF, r->bytes);
     if (!rc) rc = alloc(m, &r->nout, (size_t)nnodes * NF, r->bytes);
     if (rc) {
@@ -487,8 +486,10 @@ extern "C" int amr_ip_values(amr_model *m, int committed, double *out)
     if (!m || !out) return AMR_ERR_ARG;
     AMR_HIP(hipSetDevice(m->device));
     if (m->nip_total == 0) return AMR_OK;
-    if (!m->rec) TRY(build_recovery(m));
+    // the IP values need no patches: only their buffer
+    if (!m->rec) m->rec = new AmrRecovery;
     AmrRecovery *r = m->rec;
+    if (!r->vals) TRY(alloc(m, &r->vals, (size_t)m->nip_total * NF, r->bytes));
     TRY(launch_ipvals(m, r, committed));
     AMR_HIP(hipMemcpyAsync(out, r->vals, sizeof(double) * NF * m->nip_total, hipMemcpyDeviceToHost, m->stream));
     AMR_HIP(hipStreamSynchronize(m->stream));
@@ -504,7 +505,7 @@ extern "C" int amr_nodal_recovery(amr_model *m, int committed, double *out, int3
         return AMR_ERR_ARG;
     }
     AMR_HIP(hipSetDevice(m->device));
-    if (!m->rec) TRY(build_recovery(m));
+    if (!m->rec || !m->rec->pptr) TRY(build_recovery(m));
     AmrRecovery *r = m->rec;
     if (reps) std::memcpy(reps, r->reps.data(), sizeof(int32_t) * m->nnodes);
     hipStream_t s = m->stream;
