@@ -94,6 +94,8 @@ SIGNATURES = {
     "amr_block_mesh": (I32, [I32, I32, P, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, P, P, P, P]),
     "amr_dof_map": (I32, [I32, ctypes.c_int64, P, P, P, P, P]),
     "amr_assemble_M": (I32, [P, P]),
+    "amr_combine_KM": (I32, [P, D, D]),
+    "amr_matvec": (I32, [P, I32, P, P]),
     "amr_get_M_csc": (I32, [P, P, P, P, P]),
     "amr_ip_values": (I32, [P, I32, P]),
     "amr_nodal_recovery": (I32, [P, I32, P, P]),

@@ -189,6 +189,7 @@ struct amr_model {
     double *Me = nullptr, *Mv = nullptr, *Nd = nullptr, *rho_d = nullptr;
     uint8_t *Mflag = nullptr;
     bool m_assembled = false;
+    double *mvx = nullptr, *mvy = nullptr;   // 3*nnodes operands of amr_matvec
 };
 
 enum { SV_UEX = 0, SV_FEX, SV_U, SV_F, SV_DFEX, SV_R, SV_RC, SV_DUA, SV_DUI, SV_DFIN, SV_DUT, SV_X };
@@ -209,6 +210,7 @@ int amr_launch_ke(amr_model *m);
 int amr_launch_assemble(amr_model *m, bool flags);
 int amr_launch_update(amr_model *m, const double *dU_node /*device, node order*/, double *dFin_node);
 int amr_launch_mass(amr_model *m);  // Me from Nd, rho_d; then assembly into Mv, Mflag
+int amr_launch_combine_km(amr_model *m, double a, double b);  // K := a K + b M, preconditioners refreshed
 
 // two-level coarse space (coarse.hip): build on first use, refresh Kc^-1
 // after every assembly; queued on the model's stream
@@ -241,5 +243,6 @@ int amr_pool_step_members(const amr_pool *p);  // members of the step in flight 
 int amr_pool_step_query(amr_pool *p, int *done);
 int amr_pool_set_timing(amr_pool *p, int enable, double *ms, int64_t *n);
 int amr_spmv_full(amr_model *m, const double *x, double *y, int mask_mode);
+int amr_spmv_vals(amr_model *m, const double *vals /* kv_at layout */, const double *x, double *y, int mask_mode);
 int amr_choose_lanes(amr_model *m);
 int amr_tune_lanes(amr_model *m);

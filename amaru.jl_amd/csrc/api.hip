@@ -215,7 +215,7 @@ static void free_model(amr_model *m)
     void *ptrs[] = {m->X,    m->conn, m->emat, m->mats,  m->dNdR, m->wip,  m->st_cur, m->st_com, m->Ke,
                     m->rowptr, m->col, m->diag, m->cptr,  m->csrc, m->fptr, m->fsrc,   m->Kv,     m->Kv32,
                     m->Kflag, m->dinv, m->dinv32, m->mask, m->eqd,  m->dF,   m->estat,  m->err,    m->part,
-                    m->scal, m->binv, m->binv32, m->Me, m->Mv, m->Mflag, m->Nd, m->rho_d};
+                    m->scal, m->binv, m->binv32, m->Me, m->Mv, m->Mflag, m->Nd, m->rho_d, m->mvx, m->mvy};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (double *p : m->v)
@@ -648,6 +648,55 @@ extern "C" int amr_assemble_M(amr_model *m, const double *rho)
         return AMR_ERR_JACOBIAN;
     }
     m->m_assembled = true;
+    return AMR_OK;
+}
+
+extern "C" int amr_combine_KM(amr_model *m, double a, double b)
+{
+    if (!m) return AMR_ERR_ARG;
+    if (!m->assembled || !m->m_assembled) {
+        amr_set_error("amr_combine_KM: assemble K (amr_assemble) and M (amr_assemble_M) first");
+        return AMR_ERR_ARG;
+    }
+    if (amr_pool_is_member(m)) {
+        amr_set_error("the model is solving in a pool; its matrix and solver vectors are in use");
+        return AMR_ERR_ARG;
+    }
+    if (!std::isfinite(a) || !std::isfinite(b)) {
+        amr_set_error("amr_combine_KM: non-finite coefficient");
+        return AMR_ERR_ARG;
+    }
+    AMR_HIP(hipSetDevice(m->device));
+    TRY(amr_launch_combine_km(m, a, b));
+    AMR_HIP(hipStreamSynchronize(m->stream));
+    return AMR_OK;
+}
+
+extern "C" int amr_matvec(amr_model *m, int which, const double *x, double *y)
+{
+    if (!m || !x || !y || (which != 0 && which != 1)) return AMR_ERR_ARG;
+    if (which == 0 ? !m->assembled : !m->m_assembled) {
+        amr_set_error(which == 0 ? "amr_matvec: K has not been assembled" : "amr_matvec: M has not been assembled");
+        return AMR_ERR_ARG;
+    }
+    if (amr_pool_is_member(m)) {
+        amr_set_error("the model is solving in a pool; its matrix and solver vectors are in use");
+        return AMR_ERR_ARG;
+    }
+    AMR_HIP(hipSetDevice(m->device));
+    const size_t n3 = (size_t)m->nnodes * 3;
+    if (!m->mvx) TRY(dalloc(m, &m->mvx, n3));
+    if (!m->mvy) TRY(dalloc(m, &m->mvy, n3));
+    // eq order -> node order (DOFs without an eq id hold 0)
+    std::vector<double> xn(n3, 0.0), yn(n3);
+    for (size_t i = 0; i < n3; i++)
+        if (m->eq[i] >= 0) xn[i] = x[m->eq[i]];
+    TRY(h2d(m->mvx, xn.data(), n3, m->stream));
+    TRY(amr_spmv_vals(m, which == 0 ? m->Kv : m->Mv, m->mvx, m->mvy, 0));
+    TRY(d2h(yn.data(), m->mvy, n3, m->stream));
+    AMR_HIP(hipStreamSynchronize(m->stream));
+    for (size_t i = 0; i < n3; i++)
+        if (m->eq[i] >= 0) y[m->eq[i]] = yn[i];
     return AMR_OK;
 }
 

@@ -474,6 +474,26 @@ __global__ void __launch_bounds__(256) assemble_kernel(int64_t nblk, const int64
     }
 }
 
+// Kp = a K + b M in K's storage (the pseudo-stiffness of the dynamic
+// solver, src/mech/dyn-solver.jl:330-331): one thread per block; an entry is
+// in the pattern when it is in K's or in M's; FP64 and FP32 values written.
+__global__ void __launch_bounds__(256) combine_km_kernel(int64_t nblk, double a, double b, double *__restrict__ Kv,
+                                                         float *__restrict__ Kv32, uint8_t *__restrict__ Kflag,
+                                                         const double *__restrict__ Mv,
+                                                         const uint8_t *__restrict__ Mflag)
+{
+    const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (k >= nblk) return;
+#pragma unroll
+    for (int s = 0; s < 9; s++) {
+        const int64_t i = kv_at(k, s);
+        const double v = a * Kv[i] + b * Mv[i];
+        Kv[i] = v;
+        Kv32[kv32_at(k, s)] = (float)v;
+        Kflag[i] = Kflag[i] | Mflag[i];
+    }
+}
+
 // Jacobi inverse diagonal (0 at prescribed dofs)
 __global__ void jacobi_kernel(int64_t nnodes, int64_t nblk, const int32_t *__restrict__ diag, const double *__restrict__ Kv,
                               const double *__restrict__ mask, double *__restrict__ dinv,
@@ -722,6 +742,25 @@ int amr_launch_assemble(amr_model *m, bool flags)
                        m->Kv, m->Kv32, flags ? m->Kflag : nullptr);
     AMR_HIP(hipGetLastError());
     unsigned g2 = (unsigned)((m->nnodes * 3 + 255) / 256);
+    hipLaunchKernelGGL(jacobi_kernel, dim3(g2), dim3(256), 0, m->stream, m->nnodes, m->pat.nblk, m->diag, m->Kv, m->mask,
+                       m->dinv, m->dinv32);
+    AMR_HIP(hipGetLastError());
+    const unsigned g3 = (unsigned)((m->nnodes + 255) / 256);
+    hipLaunchKernelGGL(block_jacobi_kernel, dim3(g3), dim3(256), 0, m->stream, m->nnodes, m->pat.nblk, m->diag, m->Kv,
+                       m->mask, m->binv, m->binv32);
+    AMR_HIP(hipGetLastError());
+    return AMR_OK;
+}
+
+int amr_launch_combine_km(amr_model *m, double a, double b)
+{
+    m->k_version++;
+    if (m->pat.nblk == 0) return AMR_OK;
+    const unsigned grid = (unsigned)((m->pat.nblk + 255) / 256);
+    hipLaunchKernelGGL(combine_km_kernel, dim3(grid), dim3(256), 0, m->stream, m->pat.nblk, a, b, m->Kv, m->Kv32,
+                       m->Kflag, m->Mv, m->Mflag);
+    AMR_HIP(hipGetLastError());
+    const unsigned g2 = (unsigned)((m->nnodes * 3 + 255) / 256);
     hipLaunchKernelGGL(jacobi_kernel, dim3(g2), dim3(256), 0, m->stream, m->nnodes, m->pat.nblk, m->diag, m->Kv, m->mask,
                        m->dinv, m->dinv32);
     AMR_HIP(hipGetLastError());

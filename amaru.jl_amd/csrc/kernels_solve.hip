@@ -1690,12 +1690,17 @@ int choose_lanes(const amr_model *m, int kind)
 // y = K x (mask_mode 0) or mask .* (K x) (mask_mode 1), FP64, node order.
 int amr_spmv_full(amr_model *m, const double *x, double *y, int mask_mode)
 {
+    return amr_spmv_vals(m, m->Kv, x, y, mask_mode);
+}
+
+int amr_spmv_vals(amr_model *m, const double *vals, const double *x, double *y, int mask_mode)
+{
     if (int rc = amr_choose_lanes(m)) return rc;
     const Geom g = model_geom(m);
     const size_t lds = tile_lds(g.R, 0);
 #define AMR_SPMV(LN)                                                                                               \
     hipLaunchKernelGGL((spmv_kernel<LN>), dim3((unsigned)g.ntiles), dim3(WGS), lds, m->stream, m->nnodes, g.R,     \
-                       m->rowptr, m->col, m->Kv, x, y, m->mask, mask_mode)
+                       m->rowptr, m->col, vals, x, y, m->mask, mask_mode)
     switch (m->lanes64) {
     case 8: AMR_SPMV(8); break;
     case 16: AMR_SPMV(16); break;

@@ -685,6 +685,18 @@ class DeviceModel:
         _lib.check(L.amr_get_M_csc(self._h, ctypes.byref(nnz), _lib.ptr(colptr), _lib.ptr(rowval), _lib.ptr(nzval)))
         return sp.csc_matrix((nzval[:nnz.value], rowval[:nnz.value], colptr), shape=(self.model.ndofs,) * 2)
 
+    def combine_KM(self, a, b):
+        """The device K becomes a K + b M (the dynamic solver's pseudo-stiffness)."""
+        _lib.check(_lib.lib().amr_combine_KM(self._h, float(a), float(b)))
+
+    def matvec(self, which, x):
+        """K x ("K") or M x ("M"), x and the result in eq order."""
+        x = np.ascontiguousarray(x, dtype=float)
+        assert x.shape == (self.model.ndofs,)
+        y = np.zeros(self.model.ndofs)
+        _lib.check(_lib.lib().amr_matvec(self._h, {"K": 0, "M": 1}[which], _lib.ptr(x), _lib.ptr(y)))
+        return y
+
     # -------------------------------------------------------- outputs
     def ip_values(self, committed=False):
         """ip_state_vals of every IP (nip_total x 19; ``OUTPUT_FIELDS``)."""

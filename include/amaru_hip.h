@@ -134,6 +134,19 @@ int amr_get_Ke(amr_model *m, double *Ke);
 int amr_assemble_M(amr_model *m, const double *rho);
 /* CSC of the last amr_assemble_M, as amr_get_K_csc. */
 int amr_get_M_csc(amr_model *m, int64_t *nnz, int64_t *colptr, int32_t *rowval, double *nzval);
+/*
+ * The dynamic solver's use of K and M (src/mech/dyn-solver.jl:326-345):
+ * amr_combine_KM replaces the device K by a K + b M (the pseudo-stiffness
+ * Kp = K + (4/dt^2) M + (2/dt) C with C = alpha M + beta K is
+ * a = 1 + 2 beta/dt, b = 4/dt^2 + 2 alpha/dt), pattern = union of K's and
+ * M's, Jacobi preconditioners refreshed, so that amr_solve_system and
+ * amr_get_K_csc work on it; a two-level coarse operator is refreshed by its
+ * own policy.  amr_matvec: y = K x (which 0) or M x (which 1), x and y in
+ * eq order (ndofs), for the products M*(...) and C*(...) of the Newmark
+ * right-hand side.
+ */
+int amr_combine_KM(amr_model *m, double a, double b);
+int amr_matvec(amr_model *m, int which, const double *x, double *y);
 
 /*
  * The assembled K as the reference's SparseMatrixCSC (0-based): column
