@@ -190,6 +190,8 @@ struct amr_model {
     uint8_t *Mflag = nullptr;
     bool m_assembled = false;
     double *mvx = nullptr, *mvy = nullptr;   // 3*nnodes operands of amr_matvec
+    double *Ksave = nullptr;                 // K as it was before the last amr_combine_KM
+    int64_t ksave_version = -1;
 };
 
 enum { SV_UEX = 0, SV_FEX, SV_U, SV_F, SV_DFEX, SV_R, SV_RC, SV_DUA, SV_DUI, SV_DFIN, SV_DUT, SV_X };

@@ -215,7 +215,7 @@ static void free_model(amr_model *m)
     void *ptrs[] = {m->X,    m->conn, m->emat, m->mats,  m->dNdR, m->wip,  m->st_cur, m->st_com, m->Ke,
                     m->rowptr, m->col, m->diag, m->cptr,  m->csrc, m->fptr, m->fsrc,   m->Kv,     m->Kv32,
                     m->Kflag, m->dinv, m->dinv32, m->mask, m->eqd,  m->dF,   m->estat,  m->err,    m->part,
-                    m->scal, m->binv, m->binv32, m->Me, m->Mv, m->Mflag, m->Nd, m->rho_d, m->mvx, m->mvy};
+                    m->scal, m->binv, m->binv32, m->Me, m->Mv, m->Mflag, m->Nd, m->rho_d, m->mvx, m->mvy, m->Ksave};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (double *p : m->v)
@@ -667,6 +667,12 @@ extern "C" int amr_combine_KM(amr_model *m, double a, double b)
         return AMR_ERR_ARG;
     }
     AMR_HIP(hipSetDevice(m->device));
+    // K as assembled stays available to amr_matvec (which 2): the Newmark
+    // iteration multiplies the damping matrix alpha M + beta K after the solve
+    if (!m->Ksave) TRY(dalloc(m, &m->Ksave, (size_t)kv_size(m->pat.nblk)));
+    AMR_HIP(hipMemcpyAsync(m->Ksave, m->Kv, sizeof(double) * kv_size(m->pat.nblk), hipMemcpyDeviceToDevice,
+                           m->stream));
+    m->ksave_version = m->k_version;
     TRY(amr_launch_combine_km(m, a, b));
     AMR_HIP(hipStreamSynchronize(m->stream));
     return AMR_OK;
@@ -674,9 +680,11 @@ extern "C" int amr_combine_KM(amr_model *m, double a, double b)
 
 extern "C" int amr_matvec(amr_model *m, int which, const double *x, double *y)
 {
-    if (!m || !x || !y || (which != 0 && which != 1)) return AMR_ERR_ARG;
-    if (which == 0 ? !m->assembled : !m->m_assembled) {
-        amr_set_error(which == 0 ? "amr_matvec: K has not been assembled" : "amr_matvec: M has not been assembled");
+    if (!m || !x || !y || which < 0 || which > 2) return AMR_ERR_ARG;
+    if (which == 0 ? !m->assembled : which == 1 ? !m->m_assembled : m->ksave_version < 0) {
+        amr_set_error(which == 0   ? "amr_matvec: K has not been assembled"
+                      : which == 1 ? "amr_matvec: M has not been assembled"
+                                   : "amr_matvec: no K was combined with M");
         return AMR_ERR_ARG;
     }
     if (amr_pool_is_member(m)) {
@@ -692,7 +700,7 @@ extern "C" int amr_matvec(amr_model *m, int which, const double *x, double *y)
     for (size_t i = 0; i < n3; i++)
         if (m->eq[i] >= 0) xn[i] = x[m->eq[i]];
     TRY(h2d(m->mvx, xn.data(), n3, m->stream));
-    TRY(amr_spmv_vals(m, which == 0 ? m->Kv : m->Mv, m->mvx, m->mvy, 0));
+    TRY(amr_spmv_vals(m, which == 0 ? m->Kv : which == 1 ? m->Mv : m->Ksave, m->mvx, m->mvy, 0));
     TRY(d2h(yn.data(), m->mvy, n3, m->stream));
     AMR_HIP(hipStreamSynchronize(m->stream));
     for (size_t i = 0; i < n3; i++)

@@ -690,11 +690,12 @@ class DeviceModel:
         _lib.check(_lib.lib().amr_combine_KM(self._h, float(a), float(b)))
 
     def matvec(self, which, x):
-        """K x ("K") or M x ("M"), x and the result in eq order."""
+        """K x ("K": the device K), M x ("M") or K x with K as assembled
+        before the last combine_KM ("K0"); x and the result in eq order."""
         x = np.ascontiguousarray(x, dtype=float)
         assert x.shape == (self.model.ndofs,)
         y = np.zeros(self.model.ndofs)
-        _lib.check(_lib.lib().amr_matvec(self._h, {"K": 0, "M": 1}[which], _lib.ptr(x), _lib.ptr(y)))
+        _lib.check(_lib.lib().amr_matvec(self._h, {"K": 0, "M": 1, "K0": 2}[which], _lib.ptr(x), _lib.ptr(y)))
         return y
 
     # -------------------------------------------------------- outputs

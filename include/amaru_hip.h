@@ -141,9 +141,10 @@ int amr_get_M_csc(amr_model *m, int64_t *nnz, int64_t *colptr, int32_t *rowval, 
  * a = 1 + 2 beta/dt, b = 4/dt^2 + 2 alpha/dt), pattern = union of K's and
  * M's, Jacobi preconditioners refreshed, so that amr_solve_system and
  * amr_get_K_csc work on it; a two-level coarse operator is refreshed by its
- * own policy.  amr_matvec: y = K x (which 0) or M x (which 1), x and y in
- * eq order (ndofs), for the products M*(...) and C*(...) of the Newmark
- * right-hand side.
+ * own policy.  amr_matvec: y = K x (which 0: the device K, i.e. Kp after a
+ * combination), M x (which 1) or K x with K as it was before the last
+ * amr_combine_KM (which 2), x and y in eq order (ndofs), for the products
+ * M*(...) and C*(...) of the Newmark right-hand side and internal force.
  */
 int amr_combine_KM(amr_model *m, double a, double b);
 int amr_matvec(amr_model *m, int which, const double *x, double *y);

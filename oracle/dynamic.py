@@ -1,0 +1,101 @@
+"""Python restatement of Amaru.jl's dynamic stage solver.
+
+TEST INFRASTRUCTURE ONLY (see ``oracle/__init__.py``).
+
+``dyn_stage_solver`` follows src/mech/dyn-solver.jl:203-539 for the first
+stage with fixed increments, no seismic input: initial accelerations from
+``solve_system!(M, A, Fex, nu)``, then per increment Newton-Raphson
+iterations on the pseudo-stiffness ``Kp = K + (4/dt^2) M + (2/dt) C`` with
+``C = alpha M + beta K``, velocities and accelerations by the average
+acceleration rule, and the residual ``Fex - (Fin + C V + M A)`` over the
+unknowns.  K and M come from the oracle's mount_K / mount_M, solves from
+SuperLU.
+"""
+import numpy as np
+
+from .fem import solve_system
+
+
+def dyn_stage_solver(model, bcs, rho, nincs, tspan, tol=0.01, maxits=5, alpha=0.0, beta=0.0, dTmin=1e-7,
+                     log=None):
+    """Returns (ok, message); fills model.uvals, fvals (TFin at the dofs),
+    vvals, avals and model.state like the reference's dof values."""
+    ndofs, nu = model.configure_dofs(bcs)
+    state_bk = model.state.copy()
+    state = model.state.copy()
+    K = model.mount_K(state).tocsc()
+    M = model.mount_M(rho).tocsc()
+    A = np.zeros(ndofs)
+    V = np.zeros(ndofs)
+    Uex, Fex = model.bc_vals(bcs, 0.0)
+    msg = solve_system(M, A, Fex, nu)
+    if msg:
+        return False, msg
+    T, dT, t = 0.0, 1.0 / nincs, 0.0
+    U = np.zeros(ndofs)
+    Fin = np.zeros(ndofs)
+    Fina = np.zeros(ndofs)
+    TFin = np.zeros(ndofs)
+    while T < 1.0 - dTmin:
+        dt = tspan * dT
+        Uex, Fex = model.bc_vals(bcs, t + dt)
+        Fex_Fin = Fex - Fina
+        dUa = np.zeros(ndofs)
+        dUi = Uex.copy()
+        residue, nfails, maxfails, converged = 0.0, 0, 3, False
+        for it in range(1, maxits + 1):
+            if it > 1:
+                dUi[:] = 0.0
+            lastres = residue
+            K = model.mount_K(state).tocsc()
+            M = model.mount_M(rho).tocsc()
+            C = alpha * M + beta * K
+            Kp = K + (4 / dt ** 2) * M + (2 / dt) * C
+            dFp = Fex_Fin + M @ (A + 4 * V / dt - 4 * dUa / dt ** 2) + C @ (V - 2 * dUa / dt)
+            msg = solve_system(Kp.tocsc(), dUi, dFp, nu)
+            if msg:
+                return False, msg
+            dUt = dUa + dUi
+            dFin, state, msg = model.update_state(dUt, state_bk)
+            if msg:
+                return False, msg
+            Fina = Fin + dFin
+            Va = -V + 2 * dUt / dt
+            Aa = -A + 4 * (dUt - V * dt) / dt ** 2
+            TFin = Fina + C @ Va + M @ Aa
+            residue = float(np.max(np.abs((Fex - TFin)[:nu]))) if nu else 0.0
+            dUa += dUi
+            Fex_Fin = Fex - Fina
+            Fex_Fin[nu:] = 0.0
+            if log is not None:
+                log.append((it, residue))
+            if residue > tol:
+                Fina = Fina - dFin
+            if residue < tol:
+                converged = True
+                break
+            if np.isnan(residue):
+                break
+            if it > 1 and residue > lastres:
+                break
+            if residue > 0.9 * lastres:
+                nfails += 1
+            if nfails == maxfails:
+                break
+        if not converged:
+            model.state = state_bk.copy()
+            return False, "solver did not converge"
+        Fin = Fina
+        U += dUa
+        state_bk = state.copy()
+        A, V = Aa, Va
+        T += dT
+        t += dt
+    model.state = state_bk.copy()
+    eq = model.eq
+    for name, vec in (("uvals", U), ("fvals", TFin), ("vvals", V), ("avals", A)):
+        out = np.zeros((model.nnodes, 3))
+        has = eq >= 0
+        out[has] = vec[eq[has]]
+        setattr(model, name, out)
+    return True, None

@@ -1,0 +1,75 @@
+"""CPU: host logic of the dynamic stage driver (amaru_amd.dynamic).
+
+A stand-in device backed by the oracle's mount_K / mount_M / update_state
+and SuperLU runs the product driver; it must take the oracle driver's
+Newton-Raphson path and reach its displacements, velocities and
+accelerations (the same arithmetic but for the damping products, which the
+driver forms as alpha (M x) + beta (K x) and the oracle as (alpha M + beta K) x).
+"""
+import numpy as np
+import pytest
+
+from oracle import HEX8 as O8, TET10 as O10, fem
+from oracle.dynamic import dyn_stage_solver
+
+OSHAPE = {"HEX8": O8, "TET10": O10}
+
+
+class OracleDev:
+    def __init__(self, model, device=0):
+        self.model = model
+        self.om = fem.Model(model.coords, model.conn, OSHAPE[model.shape], [("LinearElastic", (200e9, 0.3))])
+        self.om.eq = model.eq
+        self.om.eqmap = np.ascontiguousarray(model.eq[model.conn].reshape(model.ne, -1))
+        self.om.ndofs = model.ndofs
+        self.st_com = np.zeros_like(self.om.state)
+        self.st_cur = self.st_com.copy()
+
+    def set_dofs(self):
+        pass
+
+    def assemble(self):
+        self.K = self.om.mount_K(self.st_cur).tocsc()
+
+    def assemble_M(self):
+        self.M = self.om.mount_M(self.model.rho).tocsc()
+
+    def combine_KM(self, a, b):
+        self.K0 = self.K
+        self.K = (a * self.K + b * self.M).tocsc()
+
+    def matvec(self, which, x):
+        return {"K": self.K, "M": self.M, "K0": self.K0}[which] @ x
+
+    def solve_system(self, U, F, **kw):
+        assert fem.solve_system(self.K, U, F, self.model.nu) is None
+
+    def update_state(self, dUt):
+        dFin, self.st_cur, msg = self.om.update_state(dUt, self.st_com)
+        assert msg is None
+        return dFin
+
+    def commit(self):
+        self.st_com = self.st_cur.copy()
+
+
+@pytest.mark.parametrize("shape,alpha,beta", [("HEX8", 0.0, 0.0), ("TET10", 40.0, 2e-5)])
+def test_driver_follows_the_oracle(amaru, monkeypatch, shape, alpha, beta):
+    from amaru_amd import dynamic as D
+    monkeypatch.setattr(D, "DeviceModel", OracleDev)
+    mesh = amaru.Mesh(amaru.Block([[0, 0, 0], [1.0, 0.1, 0.1]], nx=8, ny=2, nz=2, cellshape=shape))
+    model = amaru.FEModel(mesh, [("all", amaru.MechSolid(rho=7850.0), amaru.LinearElastic(E=200e9, nu=0.3))])
+    ana = D.DynamicAnalysis(model)
+    bcs = [("x==0", amaru.SurfaceBC(ux=0, uy=0, uz=0)), ("x==1", amaru.SurfaceBC(tz="-1e6*(1 + y)"))]
+    D.addstage(ana, bcs, nincs=6, tspan=3e-3)
+    ok, msg = D.dyn_solve(ana, alpha=alpha, beta=beta)
+    assert ok, msg
+    om = fem.Model(mesh.coords, mesh.conn, OSHAPE[shape], [("LinearElastic", (200e9, 0.3))])
+    obcs = [("x==0", fem.SurfaceBC(ux=0, uy=0, uz=0)), ("x==1", fem.SurfaceBC(tz="-1e6*(1 + y)"))]
+    log = []
+    ok, msg = dyn_stage_solver(om, obcs, 7850.0, nincs=6, tspan=3e-3, alpha=alpha, beta=beta, log=log)
+    assert ok, msg
+    assert [it for _, it, _ in ana.log] == [it for it, _ in log]
+    for name in ("uvals", "vvals", "avals"):
+        a, b = getattr(model, name), getattr(om, name)
+        assert np.linalg.norm(a - b) <= 1e-10 * np.linalg.norm(b), name
