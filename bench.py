@@ -382,6 +382,49 @@ def loads_leg(am, model, args):
     return out
 
 
+def dynamic_leg(am, args):
+    """The dynamic stage solver (Newmark, Rayleigh damping) on a HEX20
+    cantilever, on the device; the oracle's restatement (scipy, SuperLU) on a
+    smaller one beside it."""
+    from amaru_amd import dynamic as D
+
+    def build(n):
+        mesh = am.Mesh(am.Block([[0, 0, 0], [2.0, 0.25, 0.25]], nx=8 * n, ny=n, nz=n, cellshape="HEX20"))
+        model = am.FEModel(mesh, [("all", am.MechSolid(rho=7850.0), am.LinearElastic(E=200e9, nu=0.3))])
+        bcs = [("x==0", am.SurfaceBC(ux=0, uy=0, uz=0)), ("x==2", am.SurfaceBC(tz=-1e4))]
+        return mesh, model, bcs
+    n, nincs = args.dynamic, 10
+    mesh, model, bcs = build(n)
+    ana = D.DynamicAnalysis(model)
+    D.addstage(ana, bcs, nincs=nincs, tspan=5e-3)
+    t0 = time.perf_counter()
+    ok, msg = D.dyn_solve(ana, alpha=20.0, beta=1e-5, pcg_rtol=1e-10)
+    dt = time.perf_counter() - t0
+    if not ok:
+        raise RuntimeError(f"dynamic: {msg}")
+    out = {"workload": f"HEX20 cantilever 2 x 0.25 x 0.25 m, {8 * n}x{n}x{n} cells (8 IP), {model.nu} unknowns, "
+                       f"{nincs} Newmark increments, Rayleigh damping", "seconds": dt,
+           "dof_solves_per_s": model.nu * nincs / dt, "nr_iterations": len(ana.log)}
+    ana.dev.close()
+    log(f"dynamic: {dt:.2f}s for {nincs} increments, {model.nu} unknowns")
+    if not args.no_cpu:
+        from oracle import HEX20 as O20, fem as OF
+        from oracle.dynamic import dyn_stage_solver
+        nc = max(1, n // 4)
+        mesh_c, model_c, _ = build(nc)
+        om = OF.Model(mesh_c.coords, mesh_c.conn, O20, [("LinearElastic", (200e9, 0.3))])
+        obcs = [("x==0", OF.SurfaceBC(ux=0, uy=0, uz=0)), ("x==2", OF.SurfaceBC(tz=-1e4))]
+        t0 = time.perf_counter()
+        ok, msg = dyn_stage_solver(om, obcs, 7850.0, nincs=nincs, tspan=5e-3, alpha=20.0, beta=1e-5)
+        tc = time.perf_counter() - t0
+        nu_c = model_c.configure_dofs([("x==0", am.SurfaceBC(ux=0, uy=0, uz=0))])[1]
+        out["cpu_oracle"] = {"sample": f"HEX20 {8 * nc}x{nc}x{nc} cells, {nu_c} unknowns, {nincs} increments "
+                                       "(scipy + SuperLU, 1 thread)", "seconds": tc,
+                             "dof_solves_per_s": nu_c * nincs / tc}
+        log(f"dynamic oracle: {tc:.2f}s")
+    return out
+
+
 def served_leg(am, args):
     """Request stream: plain one-at-a-time capacity, then the server at a
     Poisson rate that is a multiple of it."""
@@ -518,6 +561,7 @@ def main():
     ap.add_argument("--mass", type=int, default=3, help="timed mount_M calls on the block (0 = skip)")
     ap.add_argument("--meshgen", type=int, default=3, help="timed device mesh generations of the block (0 = skip)")
     ap.add_argument("--loads", type=int, default=3, help="timed device body-force integrations on the block (0 = skip)")
+    ap.add_argument("--dynamic", type=int, default=8, help="dynamic leg: cross-section cells of the HEX20 cantilever (0 = skip)")
     ap.add_argument("--serve-seed", type=int, default=1)
     ap.add_argument("--serve-load", type=float, default=4.0,
                     help="Poisson arrival rate as a multiple of the plain path's request rate")
@@ -652,6 +696,8 @@ def main():
         out["mesh_generation"] = meshgen_leg(am, args)
     if args.loads:
         out["distributed_loads"] = loads_leg(am, model, args)
+    if args.dynamic:
+        out["dynamic"] = dynamic_leg(am, args)
     dev.close()
     for name, leg, n in (("batched", batched_leg, args.batch), ("served", served_leg, args.serve),
                          ("footing", footing_leg, args.footing)):

@@ -14,6 +14,8 @@ Contents
                     direct solve of K11) and mech_stage_solver!.
 ``recovery.py``     ip_state_vals of the solid materials and
                     nodal_patch_recovery (numpy, SVD pseudo-inverse).
+``dynamic.py``      dyn_stage_solver! (Newmark, Rayleigh damping) over the
+                    restated mount_K / mount_M / update_state and SuperLU.
 
 Every function cites the reference file:line it restates.  Pins: see
 ``tests/test_oracle_*.py`` (reference golden values, closed forms and
