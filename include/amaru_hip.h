@@ -313,7 +313,9 @@ int amr_pool_timing(amr_pool *p, int enable, double *spmv_ms, int64_t *spmv_laun
  * Builds the aggregates and their index structures now (host work and one
  * wait on the model's stream), so that the first two-level solve does not;
  * without this call the first two-level solve builds them with the
- * defaults. */
+ * defaults.  A two-level solve takes at most 2,048 coarse dofs (341
+ * aggregates); a larger coarse space is refused with AMR_ERR_ARG when the
+ * model joins a pool. */
 int amr_set_coarse(amr_model *m, int target_nodes, int max_aggregates);
 /* When the coarse operator Kc = Phi^T K Phi and its inverse are formed
  * again (at the first two-level solve after an assembly): once per DOF map
