@@ -3,7 +3,8 @@
 ``dyn_solve`` is the reference's ``solve!(ana::DynamicAnalysis; args...)``
 (src/mech/dyn-solver.jl:191-200) → ``dyn_stage_solver!`` (:203-539) with
 fixed increments (Newmark's average acceleration with Rayleigh damping
-C = alpha M + beta K), the hot calls through the C ABI:
+C = alpha M + beta K) and, optionally, auto-increment; the hot calls
+through the C ABI:
 
     mount_K(elems, ndofs)            -> DeviceModel.assemble
     mount_M(elems, ndofs)            -> DeviceModel.assemble_M
@@ -13,15 +14,23 @@ C = alpha M + beta K), the hot calls through the C ABI:
     update_state!(elems, dUt, dt)    -> DeviceModel.update_state
 
 Supported as the reference runs them: the first stage (the reference sets
-up velocities and accelerations only there), no seismic input, no
-auto-increment.
+up velocities and accelerations only there), no seismic input.  With
+auto-increment a failed increment is retried from the last converged state
+with dT = round(clamp(1 + tanh(log10(tol/res1)), 0.2, 0.9) dT, 3 digits);
+after a converged one the reference's growth factor is 1 (its iteration
+counter ``nits`` is never advanced), so dT stays, capped by 1/nincs and the
+remaining span.
 """
+import math
+
 import numpy as np
 
 from .analysis import Stage
 from .model import DeviceModel
 
-DYN_DEFAULTS = dict(tol=0.01, dTmin=1e-7, maxits=5, alpha=0.0, beta=0.0, autoinc=False, sism=False,
+EPS = 2.220446049250313e-16
+
+DYN_DEFAULTS = dict(tol=0.01, dTmin=1e-7, dTmax=0.1, maxits=5, alpha=0.0, beta=0.0, autoinc=False, sism=False,
                     # device options (additions of this implementation)
                     device=0, precision="fp64", precond="jacobi", pcg_rtol=1e-12)
 
@@ -54,8 +63,8 @@ def dyn_solve(ana, **kwargs):
     if unknown:
         raise TypeError(f"dyn_solve: unknown arguments {sorted(unknown)}")
     a.update(kwargs)
-    if a["autoinc"] or a["sism"]:
-        raise NotImplementedError("dyn_solve: auto-increment and seismic input are not implemented")
+    if a["sism"]:
+        raise NotImplementedError("dyn_solve: seismic input is not implemented")
     if len(ana.stages) != 1:
         raise NotImplementedError("dyn_solve: one stage (the reference initialises velocities only in stage 1)")
     st = ana.stages[0]
@@ -85,7 +94,12 @@ def _stage(ana, stage, a):
     dev.combine_KM(0.0, 1.0)                     # K's storage holds M for this solve
     dev.solve_system(A, Fex, **sopt)
 
+    autoinc = a["autoinc"]
     T, dT, t = 0.0, 1.0 / stage.nincs, ana.t
+    dTcheck = 1.0 / stage.nouts if stage.nouts > 0 else 1.0
+    Tcheck, dTbk = dTcheck, 0.0
+    if autoinc:
+        dT = min(dT, a["dTmax"], dTcheck)
     U = np.zeros(ndofs)
     Fin = np.zeros(ndofs)
     Fina = np.zeros(ndofs)
@@ -98,7 +112,7 @@ def _stage(ana, stage, a):
         Fex_Fin = Fex - Fina
         dUa = np.zeros(ndofs)
         dUi = Uex.copy()
-        residue, nfails, maxfails, converged = 0.0, 0, 3, False
+        residue, nfails, maxfails, converged, res1 = 0.0, 0, 3, False, 0.0
         for it in range(1, maxits + 1):
             if it > 1:
                 dUi[:] = 0.0
@@ -122,6 +136,8 @@ def _stage(ana, stage, a):
             Fex_Fin = Fex - Fina
             Fex_Fin[nu:] = 0.0
             ana.log.append((inc, it, residue))
+            if it == 1:
+                res1 = residue
             if residue > tol:
                 Fina = Fina - dFin
             if residue < tol:
@@ -136,6 +152,14 @@ def _stage(ana, stage, a):
             if nfails == maxfails:
                 break
         if not converged:
+            inc -= 1
+            dev.restore()                            # copyto!.(State, StateBk)
+            if autoinc:
+                q = min(max(1 + math.tanh(math.log10(tol / (res1 + EPS))), 0.2), 0.9)
+                dT = float(f"{q * dT:.3g}")
+                if dT < a["dTmin"]:
+                    return False, "solver did not converge"
+                continue
             return False, "solver did not converge"
         Fin = Fina
         U += dUa
@@ -144,6 +168,20 @@ def _stage(ana, stage, a):
         T += dT
         t += dt
         _store(model, U, TFin, V, A)
+        if T > Tcheck - a["dTmin"]:
+            Tcheck += dTcheck
+        if autoinc:
+            if dTbk > 0.0:
+                dT = min(dTbk, Tcheck - T)
+                dTbk = 0.0
+            else:
+                dTtr = min(dT, 1.0 / stage.nincs, 1.0 - T)
+                if T + dTtr > Tcheck - a["dTmin"]:
+                    dTbk = dT
+                    dT = Tcheck - T
+                else:
+                    dT = dTtr
+                    dTbk = 0.0
     ana.t = t
     return True, None
 

@@ -3,7 +3,12 @@
 TEST INFRASTRUCTURE ONLY (see ``oracle/__init__.py``).
 
 ``dyn_stage_solver`` follows src/mech/dyn-solver.jl:203-539 for the first
-stage with fixed increments, no seismic input: initial accelerations from
+stage, no seismic input, with fixed increments or auto-increment (a failed
+increment is retried from the last converged state with
+dT = round(clamp(1 + tanh(log10(tol/res1)), 0.2, 0.9) dT, 3 digits); after a
+converged one the reference's growth factor is 1, because its iteration
+counter ``nits`` is never advanced, so dT stays, capped by 1/nincs and the
+remaining span): initial accelerations from
 ``solve_system!(M, A, Fex, nu)``, then per increment Newton-Raphson
 iterations on the pseudo-stiffness ``Kp = K + (4/dt^2) M + (2/dt) C`` with
 ``C = alpha M + beta K``, velocities and accelerations by the average
@@ -11,13 +16,15 @@ acceleration rule, and the residual ``Fex - (Fin + C V + M A)`` over the
 unknowns.  K and M come from the oracle's mount_K / mount_M, solves from
 SuperLU.
 """
+import math
+
 import numpy as np
 
-from .fem import solve_system
+from .fem import EPS, solve_system
 
 
 def dyn_stage_solver(model, bcs, rho, nincs, tspan, tol=0.01, maxits=5, alpha=0.0, beta=0.0, dTmin=1e-7,
-                     log=None):
+                     dTmax=0.1, autoinc=False, log=None):
     """Returns (ok, message); fills model.uvals, fvals (TFin at the dofs),
     vvals, avals and model.state like the reference's dof values."""
     ndofs, nu = model.configure_dofs(bcs)
@@ -32,6 +39,9 @@ def dyn_stage_solver(model, bcs, rho, nincs, tspan, tol=0.01, maxits=5, alpha=0.
     if msg:
         return False, msg
     T, dT, t = 0.0, 1.0 / nincs, 0.0
+    Tcheck, dTcheck, dTbk = 1.0, 1.0, 0.0
+    if autoinc:
+        dT = min(dT, dTmax, dTcheck)
     U = np.zeros(ndofs)
     Fin = np.zeros(ndofs)
     Fina = np.zeros(ndofs)
@@ -42,7 +52,7 @@ def dyn_stage_solver(model, bcs, rho, nincs, tspan, tol=0.01, maxits=5, alpha=0.
         Fex_Fin = Fex - Fina
         dUa = np.zeros(ndofs)
         dUi = Uex.copy()
-        residue, nfails, maxfails, converged = 0.0, 0, 3, False
+        residue, nfails, maxfails, converged, res1 = 0.0, 0, 3, False, 0.0
         for it in range(1, maxits + 1):
             if it > 1:
                 dUi[:] = 0.0
@@ -69,6 +79,8 @@ def dyn_stage_solver(model, bcs, rho, nincs, tspan, tol=0.01, maxits=5, alpha=0.
             Fex_Fin[nu:] = 0.0
             if log is not None:
                 log.append((it, residue))
+            if it == 1:
+                res1 = residue
             if residue > tol:
                 Fina = Fina - dFin
             if residue < tol:
@@ -83,6 +95,14 @@ def dyn_stage_solver(model, bcs, rho, nincs, tspan, tol=0.01, maxits=5, alpha=0.
             if nfails == maxfails:
                 break
         if not converged:
+            state = state_bk.copy()
+            if autoinc:
+                q = min(max(1 + math.tanh(math.log10(tol / (res1 + EPS))), 0.2), 0.9)
+                dT = float(f"{q * dT:.3g}")
+                if dT < dTmin:
+                    model.state = state_bk.copy()
+                    return False, "solver did not converge"
+                continue
             model.state = state_bk.copy()
             return False, "solver did not converge"
         Fin = Fina
@@ -91,6 +111,20 @@ def dyn_stage_solver(model, bcs, rho, nincs, tspan, tol=0.01, maxits=5, alpha=0.
         A, V = Aa, Va
         T += dT
         t += dt
+        if T > Tcheck - dTmin:
+            Tcheck += dTcheck
+        if autoinc:
+            if dTbk > 0.0:
+                dT = min(dTbk, Tcheck - T)
+                dTbk = 0.0
+            else:
+                dTtr = min(dT, 1.0 / nincs, 1.0 - T)
+                if T + dTtr > Tcheck - dTmin:
+                    dTbk = dT
+                    dT = Tcheck - T
+                else:
+                    dT = dTtr
+                    dTbk = 0.0
     model.state = state_bk.copy()
     eq = model.eq
     for name, vec in (("uvals", U), ("fvals", TFin), ("vvals", V), ("avals", A)):

@@ -15,10 +15,16 @@ from oracle.dynamic import dyn_stage_solver
 OSHAPE = {"HEX8": O8, "TET10": O10}
 
 
+def oracle_mat(mat):
+    if type(mat).__name__ == "VonMises":
+        return ("VonMises", (mat.E, mat.nu, mat.fy, mat.H))
+    return ("LinearElastic", (mat.E, mat.nu))
+
+
 class OracleDev:
     def __init__(self, model, device=0):
         self.model = model
-        self.om = fem.Model(model.coords, model.conn, OSHAPE[model.shape], [("LinearElastic", (200e9, 0.3))])
+        self.om = fem.Model(model.coords, model.conn, OSHAPE[model.shape], [oracle_mat(model.mats[0])])
         self.om.eq = model.eq
         self.om.eqmap = np.ascontiguousarray(model.eq[model.conn].reshape(model.ne, -1))
         self.om.ndofs = model.ndofs
@@ -52,6 +58,9 @@ class OracleDev:
     def commit(self):
         self.st_com = self.st_cur.copy()
 
+    def restore(self):
+        self.st_cur = self.st_com.copy()
+
 
 @pytest.mark.parametrize("shape,alpha,beta", [("HEX8", 0.0, 0.0), ("TET10", 40.0, 2e-5)])
 def test_driver_follows_the_oracle(amaru, monkeypatch, shape, alpha, beta):
@@ -73,3 +82,30 @@ def test_driver_follows_the_oracle(amaru, monkeypatch, shape, alpha, beta):
     for name in ("uvals", "vvals", "avals"):
         a, b = getattr(model, name), getattr(om, name)
         assert np.linalg.norm(a - b) <= 1e-10 * np.linalg.norm(b), name
+
+
+def test_auto_increment_follows_the_oracle(amaru, monkeypatch):
+    """A von Mises cantilever the fixed increments cannot carry: the driver's
+    cuts and retries follow the oracle's."""
+    from amaru_amd import dynamic as D
+    monkeypatch.setattr(D, "DeviceModel", OracleDev)
+    mesh = amaru.Mesh(amaru.Block([[0, 0, 0], [1.0, 0.1, 0.1]], nx=8, ny=2, nz=2, cellshape="HEX8"))
+    model = amaru.FEModel(mesh, [("all", amaru.MechSolid(rho=7850.0), amaru.VonMises(E=200e9, nu=0.3, fy=250e6))])
+    ana = D.DynamicAnalysis(model)
+    bcs = [("x==0", amaru.SurfaceBC(ux=0, uy=0, uz=0)), ("x==1", amaru.SurfaceBC(tz=-1.2e7))]
+    D.addstage(ana, bcs, nincs=8, tspan=4e-3)
+    ok, msg = D.dyn_solve(ana, alpha=20.0, maxits=10, autoinc=True)
+    assert ok, msg
+    om = fem.Model(mesh.coords, mesh.conn, O8, [("VonMises", (200e9, 0.3, 250e6, 0.0))])
+    log = []
+    ok, msg = dyn_stage_solver(om, [("x==0", fem.SurfaceBC(ux=0, uy=0, uz=0)), ("x==1", fem.SurfaceBC(tz=-1.2e7))],
+                               7850.0, nincs=8, tspan=4e-3, alpha=20.0, maxits=10, autoinc=True, log=log)
+    assert ok, msg
+    assert [it for _, it, _ in ana.log] == [it for it, _ in log]
+    assert np.linalg.norm(model.uvals - om.uvals) <= 1e-10 * np.linalg.norm(om.uvals)
+    # without auto-increment the same problem fails, as the reference's does
+    model2 = amaru.FEModel(mesh, [("all", amaru.MechSolid(rho=7850.0), amaru.VonMises(E=200e9, nu=0.3, fy=250e6))])
+    ana2 = D.DynamicAnalysis(model2)
+    D.addstage(ana2, bcs, nincs=8, tspan=4e-3)
+    ok, msg = D.dyn_solve(ana2, alpha=20.0, maxits=10)
+    assert not ok and "did not converge" in msg

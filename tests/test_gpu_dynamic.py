@@ -49,4 +49,31 @@ def test_unsupported_forms_are_refused(amaru):
     ana = D.DynamicAnalysis(model)
     D.addstage(ana, [("x==0", amaru.SurfaceBC(ux=0, uy=0, uz=0))], nincs=2, tspan=1e-3)
     with pytest.raises(NotImplementedError):
-        D.dyn_solve(ana, autoinc=True)
+        D.dyn_solve(ana, sism=True)
+
+
+def test_plastic_auto_increment_matches_oracle(amaru):
+    """Von Mises cantilever under a step load that the fixed increments cannot
+    carry: auto-increment cuts failed increments; the device takes the
+    oracle's path (every NR iteration, every cut) and ends in its state."""
+    from amaru_amd import dynamic as D
+    mesh = amaru.Mesh(amaru.Block([[0, 0, 0], [1.0, 0.1, 0.1]], nx=8, ny=2, nz=2, cellshape="HEX8"))
+    model = amaru.FEModel(mesh, [("all", amaru.MechSolid(rho=7850.0), amaru.VonMises(E=200e9, nu=0.3, fy=250e6))])
+    ana = D.DynamicAnalysis(model)
+    D.addstage(ana, [("x==0", amaru.SurfaceBC(ux=0, uy=0, uz=0)), ("x==1", amaru.SurfaceBC(tz=-1.2e7))], nincs=8,
+               tspan=4e-3)
+    ok, msg = D.dyn_solve(ana, alpha=20.0, maxits=10, autoinc=True)
+    assert ok, msg
+    om = fem.Model(mesh.coords, mesh.conn, O8, [("VonMises", (200e9, 0.3, 250e6, 0.0))])
+    log = []
+    ok, msg = dyn_stage_solver(om, [("x==0", fem.SurfaceBC(ux=0, uy=0, uz=0)), ("x==1", fem.SurfaceBC(tz=-1.2e7))],
+                               7850.0, nincs=8, tspan=4e-3, alpha=20.0, maxits=10, autoinc=True, log=log)
+    assert ok, msg
+    assert [it for _, it, _ in ana.log] == [it for it, _ in log]
+    assert len(log) > 8 * 3   # cuts happened
+    st = ana.dev.state_get(committed=True)
+    assert np.array_equal(st[:, 12] > 0, om.state[:, 12] > 0) and (st[:, 12] > 0).any()
+    for name in ("uvals", "vvals", "avals"):
+        a, b = getattr(model, name), getattr(om, name)
+        assert np.linalg.norm(a - b) <= 1e-8 * np.linalg.norm(b), name
+    ana.dev.close()
