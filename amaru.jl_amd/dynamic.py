@@ -64,6 +64,9 @@ def dyn_solve(ana, **kwargs):
         raise TypeError(f"dyn_solve: unknown arguments {sorted(unknown)}")
     a.update(kwargs)
     if a["sism"]:
+        # the reference's dyn_stage_solver! reads sism_file, sism_dir, tss and
+        # tds without binding them (src/mech/dyn-solver.jl:272-275, 286-287, 349-351):
+        # its seismic path stops with an undefined-variable error
         raise NotImplementedError("dyn_solve: seismic input is not implemented")
     if len(ana.stages) != 1:
         raise NotImplementedError("dyn_solve: one stage (the reference initialises velocities only in stage 1)")
