@@ -108,6 +108,7 @@ struct amr_model {
     double *st_com = nullptr;     // 14*nip_total SoA (committed)
     double *Ke = nullptr;         // ne*nn*nn*9 (block layout)
     int32_t *rowptr = nullptr, *col = nullptr, *diag = nullptr;
+    int16_t *col16 = nullptr;     // nblk column - row offsets (null when one does not fit 16 bits)
     int64_t *cptr = nullptr;
     int32_t *csrc = nullptr;
     int32_t *fptr = nullptr, *fsrc = nullptr;

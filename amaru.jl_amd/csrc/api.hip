@@ -215,7 +215,7 @@ static void free_model(amr_model *m)
     void *ptrs[] = {m->X,    m->conn, m->emat, m->mats,  m->dNdR, m->wip,  m->st_cur, m->st_com, m->Ke,
                     m->rowptr, m->col, m->diag, m->cptr,  m->csrc, m->fptr, m->fsrc,   m->Kv,     m->Kv32,
                     m->Kflag, m->dinv, m->dinv32, m->mask, m->eqd,  m->dF,   m->estat,  m->err,    m->part,
-                    m->scal, m->binv, m->binv32, m->Me, m->Mv, m->Mflag, m->Nd, m->rho_d, m->mvx, m->mvy, m->Ksave};
+                    m->scal, m->binv, m->binv32, m->Me, m->Mv, m->Mflag, m->Nd, m->rho_d, m->mvx, m->mvy, m->Ksave, m->col16};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (double *p : m->v)
@@ -380,6 +380,25 @@ extern "C" int amr_model_create(amr_model **out, int device, int etype, int nips
     TRY(h2d(m->wip, w, (size_t)nq, s));
     TRY(h2d(m->rowptr, m->pat.rowptr.data(), m->pat.rowptr.size(), s));
     TRY(h2d(m->col, m->pat.col.data(), m->pat.col.size(), s));
+    {
+        // 16-bit column offsets for the PCG SpMV when every block's fits
+        std::vector<int16_t> c16(m->pat.col.size());
+        bool fits = true;
+        for (int64_t r = 0; r < nnodes && fits; r++)
+            for (int32_t k = m->pat.rowptr[r]; k < m->pat.rowptr[r + 1]; k++) {
+                const int64_t d = (int64_t)m->pat.col[k] - r;
+                if (d < -32768 || d > 32767) {
+                    fits = false;
+                    break;
+                }
+                c16[k] = (int16_t)d;
+            }
+        if (fits && !c16.empty()) {
+            TRY(dalloc(m.get(), &m->col16, c16.size()));
+            TRY(h2d(m->col16, c16.data(), c16.size(), s));
+            AMR_HIP(hipStreamSynchronize(s));   // c16 goes out of scope
+        }
+    }
     TRY(h2d(m->diag, m->pat.diag.data(), m->pat.diag.size(), s));
     TRY(h2d(m->cptr, m->pat.cptr.data(), m->pat.cptr.size(), s));
     TRY(h2d(m->csrc, m->pat.csrc.data(), m->pat.csrc.size(), s));
@@ -648,6 +667,13 @@ extern "C" int amr_assemble_M(amr_model *m, const double *rho)
         return AMR_ERR_JACOBIAN;
     }
     m->m_assembled = true;
+    return AMR_OK;
+}
+
+extern "C" int amr_spmv_col_bytes(amr_model *m, int *bytes)
+{
+    if (!m || !bytes) return AMR_ERR_ARG;
+    *bytes = m->col16 ? 2 : 4;
     return AMR_OK;
 }
 

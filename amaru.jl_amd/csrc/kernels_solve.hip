@@ -125,6 +125,7 @@ __device__ inline double sum_partials(const double *__restrict__ P, int n)
 // One model of a pool, as the kernels see it.
 struct SegDesc {
     const int32_t *rowptr, *col;
+    const int16_t *col16;         // column - row per block when every offset fits 16 bits, else null
     const double *Kv;
     const float *Kv32;
     const double *mask;
@@ -254,9 +255,23 @@ __device__ inline void load_block(const VT *__restrict__ vals, int64_t k, AT v[9
     BlockLoad<VT, AT>::load(vals, k, v);
 }
 
-template <int L, bool ABS, typename VT, typename XT, typename AT>
-__device__ inline void row_sum(int32_t rb, int32_t re, int sub, const int32_t *__restrict__ col,
-                               const VT *__restrict__ vals, const XT *__restrict__ x, AT y[3])
+// Column of block k: col[k], or with 16-bit column offsets row + col16[k].
+template <typename CT>
+__device__ inline int64_t block_col(const CT *__restrict__ col, int64_t k, int64_t row);
+template <>
+__device__ inline int64_t block_col<int32_t>(const int32_t *__restrict__ col, int64_t k, int64_t)
+{
+    return gl(col)[k];
+}
+template <>
+__device__ inline int64_t block_col<int16_t>(const int16_t *__restrict__ col, int64_t k, int64_t row)
+{
+    return row + gl(col)[k];
+}
+
+template <int L, bool ABS, typename VT, typename XT, typename AT, typename CT = int32_t>
+__device__ inline void row_sum(int32_t rb, int32_t re, int sub, const CT *__restrict__ col,
+                               const VT *__restrict__ vals, const XT *__restrict__ x, AT y[3], int64_t row = 0)
 {
     constexpr int NS = 64 / L;
     AT acc[NS][3];
@@ -267,7 +282,7 @@ __device__ inline void row_sum(int32_t rb, int32_t re, int sub, const int32_t *_
         for (int t = 0; t < NS; t++) {
             const int32_t k = base + sub + L * t;
             if (k < re) {
-                const int64_t c = gl(col)[k];
+                const int64_t c = block_col<CT>(col, k, row);
                 const auto *xc = gl(x) + c * 3;
                 const AT x0 = (AT)xc[0], x1 = (AT)xc[1], x2 = (AT)xc[2];
                 AT v[9];
@@ -483,7 +498,10 @@ __global__ void __launch_bounds__(WGS) pcg_spmv_kernel(const SegDesc *__restrict
     const XT *px = seg_px<XT>(d);
     for (int i = threadIdx.x / L; i < nr; i += RPW) {
         AT y[3];
-        row_sum<L, false, VT, XT, AT>(srp[i], srp[i + 1], sub, d.col, vals, px, y);
+        if (d.col16)
+            row_sum<L, false, VT, XT, AT, int16_t>(srp[i], srp[i + 1], sub, d.col16, vals, px, y, r0 + i);
+        else
+            row_sum<L, false, VT, XT, AT>(srp[i], srp[i + 1], sub, d.col, vals, px, y);
         if (sub == 0) {
             double c = 0.0;
             for (int e = 0; e < 3; e++) {
@@ -1034,6 +1052,7 @@ void fill_desc(amr_pool *p, int slot)
     std::memset(&d, 0, sizeof d);
     d.rowptr = m->rowptr;
     d.col = m->col;
+    d.col16 = m->col16;
     d.Kv = m->Kv;
     d.Kv32 = m->Kv32;
     d.mask = m->mask;

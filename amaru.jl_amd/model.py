@@ -685,6 +685,12 @@ class DeviceModel:
         _lib.check(L.amr_get_M_csc(self._h, ctypes.byref(nnz), _lib.ptr(colptr), _lib.ptr(rowval), _lib.ptr(nzval)))
         return sp.csc_matrix((nzval[:nnz.value], rowval[:nnz.value], colptr), shape=(self.model.ndofs,) * 2)
 
+    def spmv_col_bytes(self):
+        """Bytes per block of the column indices the PCG SpMV reads (2 or 4)."""
+        b = ctypes.c_int(0)
+        _lib.check(_lib.lib().amr_spmv_col_bytes(self._h, ctypes.byref(b)))
+        return b.value
+
     def combine_KM(self, a, b):
         """The device K becomes a K + b M (the dynamic solver's pseudo-stiffness)."""
         _lib.check(_lib.lib().amr_combine_KM(self._h, float(a), float(b)))

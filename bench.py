@@ -632,17 +632,19 @@ def main():
                        "pcg": iv, "rel_err": float(np.linalg.norm(Uv - Uref) / np.linalg.norm(Uref))}
         log(f"{name}: {np.mean(lv):.3f}s/step, {iv}")
     lanes = dev.spmv_lanes()
+    spmv_col_bytes = dev.spmv_col_bytes()
 
     nu = model.nu
     total = float(np.sum(lat))
     value = nu * args.steps / total
     plain_value = nu * len(lat_p) / float(np.sum(lat_p))
     nblk, nnodes = info["nblk"], info["nnodes"]
-    # 3x3 blocks (FP32 or FP64) + int32 cols + rowptr + p gather (once, FP32 or
-    # FP64) + FP64 q write + FP64 p read for p.q
+    # 3x3 blocks (FP32 or FP64) + column indices (16-bit offsets or int32) +
+    # rowptr + p gather (once, FP32 or FP64) + FP64 q write + FP64 p read for p.q
     vb = 8 if args.precision == "fp64" else 4
     xb = 4 if args.precision == "mixed" else 8
-    spmv_bytes = (9 * vb + 4) * nblk + 4 * (nnodes + 1) + 3 * xb * nnodes + 24 * nnodes + 24 * nnodes
+    cb = spmv_col_bytes
+    spmv_bytes = (9 * vb + cb) * nblk + 4 * (nnodes + 1) + 3 * xb * nnodes + 24 * nnodes + 24 * nnodes
     avg_s = spmv_ms / 1e3 / max(spmv_n, 1)
     achieved = spmv_bytes / avg_s / 1e9 if spmv_n else None
     traffic, traffic_src = measured_traffic(args.precision, args.n)
@@ -677,7 +679,8 @@ def main():
                          lanes["fp32"] if args.precision == "mixed" else lanes["fp32_f64acc"])), "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
                      "frac": achieved / 8000.0 if achieved else None, "traffic": traffic,
                      "traffic_source": traffic_src,
-                     "bytes_per_launch": spmv_bytes, "avg_launch_us": avg_s * 1e6, "launches": spmv_n},
+                     "bytes_per_launch": spmv_bytes, "col_index_bytes": cb, "avg_launch_us": avg_s * 1e6,
+                     "launches": spmv_n},
     }
     if not args.no_cpu:
         threads = min(os.cpu_count() or 1, 16)

@@ -332,6 +332,11 @@ int amr_get_coarse(amr_model *m, int64_t *out);
  * may be NULL. */
 int amr_get_coarse_space(amr_model *m, int32_t *agg, double *kc_inv);
 
+/* Bytes per block of the column indices the PCG SpMV reads: 2 when every
+ * block's column minus its row fits 16 bits (offsets stored beside the
+ * 32-bit columns at model creation), else 4. */
+int amr_spmv_col_bytes(amr_model *m, int *bytes);
+
 /* Device memory held by the model (bytes), and the device's free and total
  * memory, for admission by HBM budget. */
 int amr_model_device_bytes(amr_model *m, int64_t *bytes);
