@@ -95,9 +95,10 @@ def test_recovery_reproduces_polynomials_at_size(amaru):
     dev.close()
 
 
-def test_node_fields_after_a_solve(amaru):
+def test_node_fields_after_a_solve(amaru, tmp_path):
     """A plastic Drucker-Prager footing: the recovered fields carry the
-    reference's names, and match the oracle on the solved states."""
+    reference's names, match the oracle on the solved states, and go to a
+    VTU file with the displacements."""
     from amaru_amd import workloads as W
     ana, kw = W.footing(n=3, settlement=0.04)
     ok, msg = amaru.solve(ana, **kw)
@@ -113,6 +114,14 @@ def test_node_fields_after_a_solve(amaru):
     got = np.stack([fields[name] for _, name in amaru.output_fields(model)], 1)
     cols = [c for c, _ in amaru.output_fields(model)]
     assert (np.abs(got - No[:, cols]) / scale[cols]).max() <= 1e-9
+    from amaru_amd.vtu import save_vtu
+    import xml.etree.ElementTree as ET
+    path = tmp_path / "footing.vtu"
+    save_vtu(model, str(path), node_data={"U": model.uvals, **fields})
+    arrays = {da.get("Name"): da for da in ET.parse(path).getroot().iter("DataArray")}
+    assert set(fields) | {"U", "Points", "connectivity", "offsets", "types"} == set(arrays)
+    epa = np.array(arrays["epa"].text.split(), dtype=float)
+    assert np.allclose(epa, fields["epa"].astype(np.float32), rtol=1e-9, atol=0)
     dev.close()
 
 
