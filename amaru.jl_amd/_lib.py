@@ -96,6 +96,7 @@ SIGNATURES = {
     "amr_assemble_M": (I32, [P, P]),
     "amr_combine_KM": (I32, [P, D, D]),
     "amr_spmv_col_bytes": (I32, [P, P]),
+    "amr_set_spmv_col_bytes": (I32, [P, I32]),
     "amr_matvec": (I32, [P, I32, P, P]),
     "amr_get_M_csc": (I32, [P, P, P, P, P]),
     "amr_ip_values": (I32, [P, I32, P]),

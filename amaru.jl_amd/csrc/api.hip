@@ -234,6 +234,24 @@ static void free_model(amr_model *m)
     delete m;
 }
 
+// 16-bit column offsets (column - row) for the PCG SpMV, when every block's
+// offset fits; otherwise the SpMV reads the 32-bit columns.
+static int build_col16(amr_model *m, hipStream_t s)
+{
+    std::vector<int16_t> c16(m->pat.col.size());
+    for (int64_t r = 0; r < m->nnodes; r++)
+        for (int32_t k = m->pat.rowptr[r]; k < m->pat.rowptr[r + 1]; k++) {
+            const int64_t d = (int64_t)m->pat.col[k] - r;
+            if (d < -32768 || d > 32767) return AMR_OK;
+            c16[k] = (int16_t)d;
+        }
+    if (c16.empty()) return AMR_OK;
+    TRY(dalloc(m, &m->col16, c16.size()));
+    TRY(h2d(m->col16, c16.data(), c16.size(), s));
+    AMR_HIP(hipStreamSynchronize(s));   // c16 goes out of scope
+    return AMR_OK;
+}
+
 static int upload_dofs(amr_model *m, const int32_t *eq_id, int64_t ndofs, int64_t nu)
 {
     if (ndofs < 0 || nu < 0 || nu > ndofs) {
@@ -380,25 +398,7 @@ extern "C" int amr_model_create(amr_model **out, int device, int etype, int nips
     TRY(h2d(m->wip, w, (size_t)nq, s));
     TRY(h2d(m->rowptr, m->pat.rowptr.data(), m->pat.rowptr.size(), s));
     TRY(h2d(m->col, m->pat.col.data(), m->pat.col.size(), s));
-    {
-        // 16-bit column offsets for the PCG SpMV when every block's fits
-        std::vector<int16_t> c16(m->pat.col.size());
-        bool fits = true;
-        for (int64_t r = 0; r < nnodes && fits; r++)
-            for (int32_t k = m->pat.rowptr[r]; k < m->pat.rowptr[r + 1]; k++) {
-                const int64_t d = (int64_t)m->pat.col[k] - r;
-                if (d < -32768 || d > 32767) {
-                    fits = false;
-                    break;
-                }
-                c16[k] = (int16_t)d;
-            }
-        if (fits && !c16.empty()) {
-            TRY(dalloc(m.get(), &m->col16, c16.size()));
-            TRY(h2d(m->col16, c16.data(), c16.size(), s));
-            AMR_HIP(hipStreamSynchronize(s));   // c16 goes out of scope
-        }
-    }
+    TRY(build_col16(m.get(), s));
     TRY(h2d(m->diag, m->pat.diag.data(), m->pat.diag.size(), s));
     TRY(h2d(m->cptr, m->pat.cptr.data(), m->pat.cptr.size(), s));
     TRY(h2d(m->csrc, m->pat.csrc.data(), m->pat.csrc.size(), s));
@@ -674,6 +674,32 @@ extern "C" int amr_spmv_col_bytes(amr_model *m, int *bytes)
 {
     if (!m || !bytes) return AMR_ERR_ARG;
     *bytes = m->col16 ? 2 : 4;
+    return AMR_OK;
+}
+
+extern "C" int amr_set_spmv_col_bytes(amr_model *m, int bytes)
+{
+    if (!m || (bytes != 2 && bytes != 4)) {
+        amr_set_error("amr_set_spmv_col_bytes: bytes must be 2 or 4");
+        return AMR_ERR_ARG;
+    }
+    if (amr_pool_is_member(m)) {
+        amr_set_error("the model is solving in a pool; its matrix is in use");
+        return AMR_ERR_ARG;
+    }
+    AMR_HIP(hipSetDevice(m->device));
+    if (bytes == 4 && m->col16) {
+        AMR_HIP(hipStreamSynchronize(m->stream));
+        AMR_HIP(hipFree(m->col16));
+        m->col16 = nullptr;
+        m->dev_bytes -= (int64_t)(m->pat.col.size() * sizeof(int16_t));
+    } else if (bytes == 2 && !m->col16) {
+        TRY(build_col16(m, m->stream));
+        if (!m->col16) {
+            amr_set_error("amr_set_spmv_col_bytes: a block's column offset does not fit 16 bits");
+            return AMR_ERR_ARG;
+        }
+    }
     return AMR_OK;
 }
 

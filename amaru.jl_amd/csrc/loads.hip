@@ -43,6 +43,8 @@ __device__ double eval_expr(const int32_t *ops, const double *consts, int nops, 
         case AMR_OP_COS: st[sp - 1] = cos(st[sp - 1]); break;
         case AMR_OP_EXP: st[sp - 1] = exp(st[sp - 1]); break;
         case AMR_OP_LOG: st[sp - 1] = log(st[sp - 1]); break;
+        case AMR_OP_TAN: st[sp - 1] = tan(st[sp - 1]); break;
+        case AMR_OP_LOG10: st[sp - 1] = log10(st[sp - 1]); break;
         default: {
             const double b = st[--sp], a = st[sp - 1];
             double r;
@@ -52,10 +54,9 @@ __device__ double eval_expr(const int32_t *ops, const double *consts, int nops, 
             case AMR_OP_MUL: r = a * b; break;
             case AMR_OP_DIV: r = a / b; break;
             case AMR_OP_POW: r = pow(a, b); break;
-            default: {   // floored modulo: the sign of the divisor
-                r = fmod(a, b);
-                if (r != 0.0 && ((r < 0.0) != (b < 0.0))) r += b;
-            }
+            case AMR_OP_MAX: r = fmax(a, b); break;
+            case AMR_OP_MIN: r = fmin(a, b); break;
+            default: r = trunc((a - fmod(a, b)) / b);  // div: exact remainder, truncated quotient
             }
             st[sp - 1] = r;
         }
@@ -160,12 +161,12 @@ extern "C" int amr_check_expr(const int32_t *ops, int nops, int nconsts)
             sp++;
         } else if (op >= AMR_OP_X && op <= AMR_OP_T) {
             sp++;
-        } else if (op >= AMR_OP_NEG && op <= AMR_OP_LOG) {
+        } else if (op >= AMR_OP_NEG && op <= AMR_OP_LOG10) {
             if (sp < 1) {
                 amr_set_error("load expression: stack underflow");
                 return AMR_ERR_ARG;
             }
-        } else if (op >= AMR_OP_ADD && op <= AMR_OP_MOD) {
+        } else if (op >= AMR_OP_ADD && op <= AMR_OP_IDIV) {
             if (sp < 2) {
                 amr_set_error("load expression: stack underflow");
                 return AMR_ERR_ARG;

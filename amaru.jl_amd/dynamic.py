@@ -20,11 +20,21 @@ with dT = round(clamp(1 + tanh(log10(tol/res1)), 0.2, 0.9) dT, 3 digits);
 after a converged one the reference's growth factor is 1 (its iteration
 counter ``nits`` is never advanced), so dT stays, capped by 1/nincs and the
 remaining span.
+
+A failed constitutive update (a material failure or NaN internal forces)
+does not stop the stage: the reference ignores update_state!'s status here
+(src/mech/dyn-solver.jl:389) and goes on with the internal forces its
+threads had gathered before the failing element, which depends on the
+thread count.  Here such an iteration counts as an unbounded residue: the
+iteration loop ends, the increment is not converged, the committed states
+are restored and, with auto-increment, the step is cut (by the smallest
+factor, 0.2, when it happens at the first iteration).
 """
 import math
 
 import numpy as np
 
+from ._lib import AmaruError
 from .analysis import Stage
 from .model import DeviceModel
 
@@ -129,7 +139,13 @@ def _stage(ana, stage, a):
             dev.combine_KM(1.0 + 2.0 * beta / dt, 4.0 / dt ** 2 + 2.0 * alpha / dt)
             dev.solve_system(dUi, dFp, **sopt)
             dUt = dUa + dUi
-            dFin = dev.update_state(dUt)
+            try:
+                dFin = dev.update_state(dUt)
+            except AmaruError:
+                ana.log.append((inc, it, math.inf))
+                if it == 1:
+                    res1 = math.inf
+                break
             Fina = Fin + dFin
             Va = -V + 2 * dUt / dt
             Aa = -A + 4 * (dUt - V * dt) / dt ** 2
@@ -158,8 +174,7 @@ def _stage(ana, stage, a):
             inc -= 1
             dev.restore()                            # copyto!.(State, StateBk)
             if autoinc:
-                q = min(max(1 + math.tanh(math.log10(tol / (res1 + EPS))), 0.2), 0.9)
-                dT = float(f"{q * dT:.3g}")
+                dT = float(f"{cut_factor(tol, res1) * dT:.3g}")
                 if dT < a["dTmin"]:
                     return False, "solver did not converge"
                 continue
@@ -187,6 +202,14 @@ def _stage(ana, stage, a):
                     dTbk = 0.0
     ana.t = t
     return True, None
+
+
+def cut_factor(tol, res1):
+    """clamp(1 + tanh(log10(tol/res1)), 0.2, 0.9) (src/mech/dyn-solver.jl:
+    515-516); an unbounded first residue gives the smallest factor."""
+    if math.isinf(res1):
+        return 0.2
+    return min(max(1 + math.tanh(math.log10(tol / (res1 + EPS))), 0.2), 0.9)
 
 
 def _cmul(dev, alpha, beta, x, kname):

@@ -1,29 +1,58 @@
 """Vectorised evaluation of the reference's filter and value expressions.
 
 Amaru writes selections and boundary values as Julia expressions over the
-point coordinates and time, e.g. ``:(x==0 && y==0)`` or ``:(3*z)``
-(evaluated by src/tools/expr.jl).  Here the same text is parsed once with
-Python's ``ast`` (``&&``/``||``/``!`` mapped to ``and``/``or``/``not``)
-and evaluated over numpy arrays of coordinates.  Only arithmetic,
-comparisons, boolean logic, x/y/z/t and a few math functions are allowed.
+point coordinates and time, e.g. ``:(x==0 && y==0)`` or ``:(3*z)``,
+evaluated by ``evaluate`` (src/tools/expr.jl:130-187) with the operations
+of its ``op_dict`` (src/tools/expr.jl:98-125).  Here the same text is
+parsed once with Python's ``ast`` (``&&``/``||`` mapped to ``and``/``or``,
+``^`` to ``**``) and evaluated over numpy arrays of coordinates with the
+same operation set:
+- arithmetic ``+ - * / ^`` and ``div`` (truncated quotient);
+- comparisons with the reference's absolute tolerance ``arith_tol = 1e-6``:
+  ``a == b`` is ``|a - b| < tol``, ``a != b`` is ``|a - b| >= tol``,
+  ``a > b`` is ``a > b + tol``, ``a < b`` is ``a < b - tol``,
+  ``a >= b`` is ``a > b - tol``, ``a <= b`` is ``a < b + tol``;
+- ``&&`` and ``||``;
+- ``abs sin cos tan exp log log10 sqrt max min`` and the constant ``pi``.
+Anything else (``%``, a unary ``!``, other functions) is refused, as the
+reference refuses an operation missing from ``op_dict``.
 """
 import ast
 import math
 
 import numpy as np
 
-_FUNCS = {"sqrt": np.sqrt, "abs": np.abs, "sin": np.sin, "cos": np.cos, "exp": np.exp, "log": np.log}
+ARITH_TOL = 1e-6  # src/tools/expr.jl:96
+
+
+def _div(a, b):
+    """Julia's div on floats: the quotient truncated toward zero, from the
+    exact remainder (div(a, b) = (a - rem(a, b)) / b)."""
+    a = np.asarray(a, dtype=float)
+    b = np.asarray(b, dtype=float)
+    return np.trunc((a - np.fmod(a, b)) / b)
+
+
+_FUNCS = {"sqrt": np.sqrt, "abs": np.abs, "sin": np.sin, "cos": np.cos, "tan": np.tan, "exp": np.exp,
+          "log": np.log, "log10": np.log10, "max": np.maximum, "min": np.minimum, "div": _div}
+_NARGS = {"max": 2, "min": 2, "div": 2}
 _CONST = {"pi": math.pi}
-_CMP = {ast.Eq: np.equal, ast.NotEq: np.not_equal, ast.Lt: np.less, ast.LtE: np.less_equal,
-        ast.Gt: np.greater, ast.GtE: np.greater_equal}
+_CMP = {ast.Eq: lambda a, b: np.abs(np.subtract(a, b)) < ARITH_TOL,
+        ast.NotEq: lambda a, b: np.abs(np.subtract(a, b)) >= ARITH_TOL,
+        ast.Lt: lambda a, b: np.less(a, np.subtract(b, ARITH_TOL)),
+        ast.LtE: lambda a, b: np.less(a, np.add(b, ARITH_TOL)),
+        ast.Gt: lambda a, b: np.greater(a, np.add(b, ARITH_TOL)),
+        ast.GtE: lambda a, b: np.greater(a, np.subtract(b, ARITH_TOL))}
 _BIN = {ast.Add: np.add, ast.Sub: np.subtract, ast.Mult: np.multiply, ast.Div: np.true_divide,
-        ast.Pow: np.power, ast.Mod: np.mod}
+        ast.Pow: np.power}
 
 
 def _to_python(src):
     s = src.replace("&&", " and ").replace("||", " or ")
-    s = s.replace("!=", "\x00").replace("!", " not ").replace("\x00", "!=").replace("^", "**")
-    return s
+    s = s.replace("!=", "\x00")
+    if "!" in s:
+        raise ValueError(f"operation ! not allowed in {src!r} (src/tools/expr.jl op_dict)")
+    return s.replace("\x00", "!=").replace("^", "**")
 
 
 def compile_expr(src):
@@ -53,8 +82,6 @@ def compile_expr(src):
             return out
         if isinstance(node, ast.UnaryOp):
             v = ev(node.operand, env)
-            if isinstance(node.op, ast.Not):
-                return np.logical_not(v)
             if isinstance(node.op, ast.USub):
                 return np.negative(v)
             if isinstance(node.op, ast.UAdd):
@@ -70,7 +97,8 @@ def compile_expr(src):
                 out = r if out is None else np.logical_and(out, r)
                 left = right
             return out
-        if isinstance(node, ast.Call) and isinstance(node.func, ast.Name) and node.func.id in _FUNCS:
+        if isinstance(node, ast.Call) and isinstance(node.func, ast.Name) and node.func.id in _FUNCS \
+                and len(node.args) == _NARGS.get(node.func.id, 1) and not node.keywords:
             return _FUNCS[node.func.id](*[ev(a, env) for a in node.args])
         raise ValueError(f"unsupported expression element {ast.dump(node)} in {src!r}")
 
@@ -85,7 +113,8 @@ def compile_expr(src):
 # postfix programs for the device (the operation codes of AMR_OP_* in
 # include/amaru_hip.h)
 OP = {"const": 0, "x": 1, "y": 2, "z": 3, "t": 4, "neg": 5, "sqrt": 6, "abs": 7, "sin": 8, "cos": 9, "exp": 10,
-      "log": 11, ast.Add: 12, ast.Sub: 13, ast.Mult: 14, ast.Div: 15, ast.Pow: 16, ast.Mod: 17}
+      "log": 11, "tan": 12, "log10": 13, ast.Add: 14, ast.Sub: 15, ast.Mult: 16, ast.Div: 17, ast.Pow: 18,
+      "max": 19, "min": 20, "div": 21}
 
 
 def compile_program(src):
@@ -122,8 +151,9 @@ def compile_program(src):
                 emit(node.right)
                 ops.append((OP[type(node.op)], 0))
             elif isinstance(node, ast.Call) and isinstance(node.func, ast.Name) and node.func.id in _FUNCS \
-                    and len(node.args) == 1:
-                emit(node.args[0])
+                    and len(node.args) == _NARGS.get(node.func.id, 1) and not node.keywords:
+                for a in node.args:
+                    emit(a)
                 ops.append((OP[node.func.id], 0))
             else:
                 raise ValueError(f"not a numeric load expression: {ast.dump(node)} in {src!r}")

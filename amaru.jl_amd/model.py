@@ -691,6 +691,11 @@ class DeviceModel:
         _lib.check(_lib.lib().amr_spmv_col_bytes(self._h, ctypes.byref(b)))
         return b.value
 
+    def set_spmv_col_bytes(self, nbytes):
+        """4: the SpMV reads the 32-bit columns; 2: the 16-bit offsets
+        (when every block's fits).  Results do not depend on it."""
+        _lib.check(_lib.lib().amr_set_spmv_col_bytes(self._h, int(nbytes)))
+
     def combine_KM(self, a, b):
         """The device K becomes a K + b M (the dynamic solver's pseudo-stiffness)."""
         _lib.check(_lib.lib().amr_combine_KM(self._h, float(a), float(b)))

@@ -418,6 +418,7 @@ class Server:
         self.workers = int(workers)
         self.in_use = 0
         self.peak_in_use = 0
+        self.peak_held = 0      # admitted bytes plus finished models not freed yet
         # main-loop time accounting (seconds): pool steps, resuming finished
         # members, starting admitted ones, idle waits
         self.t_loop = {"step": 0.0, "resume": 0.0, "start": 0.0, "wait": 0.0, "free": 0.0}
@@ -533,13 +534,20 @@ class Server:
                 while admit_q:
                     req = admit_q[0]
                     slots = self.pools.cap - len(running) - len(creating)
-                    fits = self.in_use + req.reserve <= self.budget or (not running and not creating)
+                    # finished models wait in the graveyard until an idle
+                    # point frees them: their bytes still occupy the device
+                    held = self.in_use + self._graveyard_bytes
+                    fits = held + req.reserve <= self.budget or (not running and not creating)
+                    if slots > 0 and not fits and self._graveyard and self.in_use + req.reserve <= self.budget:
+                        self._release()
+                        fits = True
                     if slots <= 0 or not fits:
                         break
                     admit_q.popleft()
                     with lock:
                         self.in_use += req.reserve
                         self.peak_in_use = max(self.peak_in_use, self.in_use)
+                        self.peak_held = max(self.peak_held, self.in_use + self._graveyard_bytes)
                     req.t_admit = clock()
                     creating.add(ex_dev.submit(device_build, req))
                 for f in [f for f in creating if f.done()]:

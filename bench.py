@@ -112,7 +112,31 @@ def cpu_baseline(n_cpu, threads):
     U[:nu] = x
     m.update_state(U, m.state)
     t3 = time.perf_counter()
-    return dict(nu=nu, ndofs=ndofs, t_assemble=t1 - t0, t_solve=t2 - t1, t_update=t3 - t2, total=t3 - t0)
+    R = K[nu:, :] @ U   # reactions K21 U1 + K22 U2
+    return dict(nu=nu, ndofs=ndofs, t_assemble=t1 - t0, t_solve=t2 - t1, t_update=t3 - t2, total=t3 - t0,
+                U=U, R=R)
+
+
+def direct_check(am, args, cb):
+    """The headline step's modes on the CPU sample's block (the largest one
+    the bench's SuperLU factorises in bounded time), against that FP64 direct
+    solve: relative L2 error of the displacements and of the reactions."""
+    mesh, model, bcs, Uex, Fex = block_problem(am, args.cpu_n)
+    nu = model.nu
+    assert nu == cb["nu"]
+    dev = am.DeviceModel(model, device=0)
+    dev.stage_begin(Uex, Fex)
+    out = {"block": f"HEX20 {args.cpu_n}^3, 27 IP", "unknowns": nu, "oracle": "SuperLU (FP64 direct) on K11"}
+    modes = {"headline": dict(precision=args.precision, precond=args.precond),
+             "plain_fp64": dict(precision="fp64", precond="jacobi"),
+             "mixed_f64acc_two_level": dict(precision="mixed64acc", precond="two_level")}
+    for name, kw in modes.items():
+        r = run_steps(dev, dev.opts(rtol=args.rtol, **kw), 1)[2]
+        dU, dF = dev.stage_get(last=True)
+        out[name] = {"mode": kw, "rel_err_U": rel_err(dU[:nu], cb["U"][:nu]),
+                     "rel_err_reactions": rel_err(dF[nu:], cb["R"]), "pcg_iterations": r["pcg_iterations"]}
+    dev.close()
+    return out
 
 
 def dof_solves(anas):
@@ -691,6 +715,8 @@ def main():
                                           f"{cb['nu']} unknowns: mount_K {cb['t_assemble']:.2f}s "
                                           f"({threads} OpenMP threads), SuperLU factor+solve "
                                           f"{cb['t_solve']:.2f}s (1 thread), update {cb['t_update']:.2f}s")}
+        out["rel_err_vs_fp64_direct"] = direct_check(am, args, cb)
+        log(f"direct oracle check: {out['rel_err_vs_fp64_direct']}")
     if args.post:
         out["post_processing"] = post_leg(am, dev, model, args)
     if args.mass:

@@ -336,6 +336,12 @@ int amr_get_coarse_space(amr_model *m, int32_t *agg, double *kc_inv);
  * block's column minus its row fits 16 bits (offsets stored beside the
  * 32-bit columns at model creation), else 4. */
 int amr_spmv_col_bytes(amr_model *m, int *bytes);
+/* Chooses the column indices the PCG SpMV reads: 4 drops the 16-bit
+ * offsets (the SpMV then reads the 32-bit columns, as on meshes whose
+ * offsets do not fit), 2 builds them again (AMR_ERR_ARG if one does not
+ * fit).  Both give bit-identical results.  Not while the model is solving in
+ * a pool. */
+int amr_set_spmv_col_bytes(amr_model *m, int bytes);
 
 /* Device memory held by the model (bytes), and the device's free and total
  * memory, for admission by HBM budget. */
@@ -355,10 +361,15 @@ int amr_mem_info(int device, int64_t *free_bytes, int64_t *total_bytes);
  * AMR_OP_CONST; at most AMR_EXPR_STACK values deep (checked before any
  * launch).  F: ncells x npts x 3.
  */
+/* The operations are those of the reference's op_dict (src/tools/expr.jl:
+ * 98-125) that give numbers; AMR_OP_IDIV is Julia's div (quotient truncated
+ * toward zero). */
 enum {
     AMR_OP_CONST = 0, AMR_OP_X, AMR_OP_Y, AMR_OP_Z, AMR_OP_T,                             /* push */
     AMR_OP_NEG, AMR_OP_SQRT, AMR_OP_ABS, AMR_OP_SIN, AMR_OP_COS, AMR_OP_EXP, AMR_OP_LOG,  /* unary */
-    AMR_OP_ADD, AMR_OP_SUB, AMR_OP_MUL, AMR_OP_DIV, AMR_OP_POW, AMR_OP_MOD                /* binary (MOD floored) */
+    AMR_OP_TAN, AMR_OP_LOG10,
+    AMR_OP_ADD, AMR_OP_SUB, AMR_OP_MUL, AMR_OP_DIV, AMR_OP_POW, AMR_OP_MAX, AMR_OP_MIN,   /* binary */
+    AMR_OP_IDIV
 };
 #define AMR_EXPR_STACK 16
 int amr_check_expr(const int32_t *ops, int nops, int nconsts);

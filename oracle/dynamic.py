@@ -68,7 +68,14 @@ def dyn_stage_solver(model, bcs, rho, nincs, tspan, tol=0.01, maxits=5, alpha=0.
             dUt = dUa + dUi
             dFin, state, msg = model.update_state(dUt, state_bk)
             if msg:
-                return False, msg
+                # the reference ignores the status (dyn-solver.jl:389) and
+                # goes on with a thread-dependent partial internal force; a
+                # failed update counts as an unbounded residue here
+                if log is not None:
+                    log.append((it, math.inf))
+                if it == 1:
+                    res1 = math.inf
+                break
             Fina = Fin + dFin
             Va = -V + 2 * dUt / dt
             Aa = -A + 4 * (dUt - V * dt) / dt ** 2
@@ -97,7 +104,8 @@ def dyn_stage_solver(model, bcs, rho, nincs, tspan, tol=0.01, maxits=5, alpha=0.
         if not converged:
             state = state_bk.copy()
             if autoinc:
-                q = min(max(1 + math.tanh(math.log10(tol / (res1 + EPS))), 0.2), 0.9)
+                # clamp(1 + tanh(log10(tol/res1)), 0.2, 0.9), dyn-solver.jl:515-516
+                q = 0.2 if math.isinf(res1) else min(max(1 + math.tanh(math.log10(tol / (res1 + EPS))), 0.2), 0.9)
                 dT = float(f"{q * dT:.3g}")
                 if dT < dTmin:
                     model.state = state_bk.copy()

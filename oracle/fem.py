@@ -7,6 +7,7 @@ element and material arithmetic is delegated to ``amaru_oracle.c``; the
 linear solve is SuperLU (``scipy.sparse.linalg.splu``), the FP64 direct
 oracle standing in for UMFPACK's ``lu(K11)`` of src/solver.jl:42-43.
 """
+import ast
 import math
 
 import numpy as np
@@ -23,28 +24,92 @@ EPS = 2.220446049250313e-16
 
 # --------------------------------------------------------------- expressions
 
+ARITH_TOL = 1e-6  # src/tools/expr.jl:96
+
+
+def _jl_div(a, b):
+    """Julia's div(a, b) on floats: (a - rem(a, b)) / b, truncated."""
+    return float(math.trunc((a - math.fmod(a, b)) / b))
+
+
+# op_dict (src/tools/expr.jl:98-125), one point at a time
+_OPS = {
+    "+": lambda *a: a[0] + a[1] if len(a) == 2 else +a[0],
+    "-": lambda *a: a[0] - a[1] if len(a) == 2 else -a[0],
+    "*": lambda a, b: a * b, "/": lambda a, b: a / b, "^": lambda a, b: a ** b,
+    "div": _jl_div,
+    ">": lambda a, b: a > b + ARITH_TOL, "<": lambda a, b: a < b - ARITH_TOL,
+    ">=": lambda a, b: a > b - ARITH_TOL, "<=": lambda a, b: a < b + ARITH_TOL,
+    "==": lambda a, b: abs(a - b) < ARITH_TOL, "!=": lambda a, b: abs(a - b) >= ARITH_TOL,
+    "abs": abs, "sin": math.sin, "cos": math.cos, "tan": math.tan, "exp": math.exp, "log": math.log,
+    "log10": math.log10, "sqrt": math.sqrt, "max": max, "min": min,
+}
+_PYOP = {"Add": "+", "Sub": "-", "Mult": "*", "Div": "/", "Pow": "^", "Gt": ">", "Lt": "<", "GtE": ">=",
+         "LtE": "<=", "Eq": "==", "NotEq": "!=", "USub": "-", "UAdd": "+"}
+
+
 def _expr(src):
-    """Compile a reference-style condition/value expression (Julia syntax
-    subset: && || ! and arithmetic on x, y, z, t)."""
+    """Parse a reference-style condition/value expression (Julia syntax:
+    && || comparisons and arithmetic on x, y, z, t) into a tree that
+    ``evaluate`` reduces as reduce! does (src/tools/expr.jl:141-176)."""
     if not isinstance(src, str):
         return src
-    s = src.replace("&&", " and ").replace("||", " or ")
-    s = s.replace("!=", "\x00").replace("!", " not ").replace("\x00", "!=")
-    s = s.replace("^", "**")
-    code = compile(s, "<expr>", "eval")
-    return code
+    s = src.replace("&&", " and ").replace("||", " or ").replace("^", "**")
+    if "!" in s.replace("!=", ""):
+        raise ValueError("operation ! not allowed in this context")
+    try:
+        return ("tree", ast.parse(s, mode="eval").body)
+    except SyntaxError as e:
+        raise ValueError(f"cannot parse {src!r}: {e}") from None
+
+
+def _reduce(node, env):
+    if isinstance(node, ast.Constant):
+        return node.value
+    if isinstance(node, ast.Name):
+        if node.id == "pi":
+            return math.pi
+        if node.id not in env:
+            raise ValueError(f"variable {node.id} not defined for this context")
+        return env[node.id]
+    if isinstance(node, ast.BoolOp):  # && / || short-circuit
+        want = isinstance(node.op, ast.Or)
+        for v in node.values:
+            if bool(_reduce(v, env)) == want:
+                return want
+        return not want
+    if isinstance(node, ast.Compare):  # :comparison chains
+        vals = [_reduce(node.left, env)] + [_reduce(c, env) for c in node.comparators]
+        for i, op in enumerate(node.ops):
+            if not _OPS[_PYOP[type(op).__name__]](vals[i], vals[i + 1]):
+                return False
+        return True
+    if isinstance(node, (ast.BinOp, ast.UnaryOp)):
+        name = _PYOP.get(type(node.op).__name__)
+        if name is None:
+            raise ValueError(f"operation {type(node.op).__name__} not allowed in this context")
+        args = [node.left, node.right] if isinstance(node, ast.BinOp) else [node.operand]
+        return _OPS[name](*[_reduce(a, env) for a in args])
+    if isinstance(node, ast.Call) and isinstance(node.func, ast.Name):
+        if node.func.id not in _OPS:
+            raise ValueError(f"operation {node.func.id} not allowed in this context")
+        args = [_reduce(a, env) for a in node.args]
+        try:
+            return _OPS[node.func.id](*args)
+        except TypeError:  # Julia: no method matching the argument count
+            raise ValueError(f"no method {node.func.id} for {len(args)} arguments") from None
+    raise ValueError(f"operation {ast.dump(node)} not allowed in this context")
 
 
 def evaluate(val, x=0.0, y=0.0, z=0.0, t=0.0):
-    """evaluate (src/tools/expr.jl): numbers pass through, expressions are
-    evaluated with the point coordinates and time bound."""
+    """evaluate (src/tools/expr.jl:178-181): numbers pass through,
+    expressions are reduced with the point coordinates and time bound."""
     if isinstance(val, (int, float)):
         return float(val)
     if callable(val):
         return float(val(x, y, z, t))
-    code = _expr(val) if isinstance(val, str) else val
-    return eval(code, {"__builtins__": {}, "sqrt": math.sqrt, "abs": abs, "pi": math.pi},
-                {"x": x, "y": y, "z": z, "t": t})
+    tree = _expr(val) if isinstance(val, str) else val
+    return _reduce(tree[1], {"x": x, "y": y, "z": z, "t": t})
 
 
 # -------------------------------------------------------------------- mesh

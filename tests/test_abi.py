@@ -46,14 +46,15 @@ def test_load_programs_are_checked_without_gpu(amaru):
     from amaru_amd import _lib
     from amaru_amd.expr import compile_program
     L = _lib.lib()
-    for e in (2.0, "3*z", "x^2 + sin(y) - 0.5*t", "-(x+y)*z/2 % 0.7", "sqrt(abs(x))*exp(-t)+log(2+y)"):
+    for e in (2.0, "3*z", "x^2 + sin(y) - 0.5*t", "-(x+y)*z/2 - div(x, 0.7)", "sqrt(abs(x))*exp(-t)+log(2+y)",
+              "max(tan(x), log10(1+y)) - min(z, t)"):
         ops, consts = compile_program(e)
         assert L.amr_check_expr(_lib.ptr(ops), len(ops), len(consts)) == 0
-    bad = [np.array([[12, 0]], dtype=np.int32),                 # binary operation on an empty stack
+    bad = [np.array([[14, 0]], dtype=np.int32),                 # binary operation on an empty stack
            np.array([[1, 0], [2, 0]], dtype=np.int32),          # leaves two values
            np.array([[0, 3]], dtype=np.int32),                  # constant index out of range
            np.array([[99, 0]], dtype=np.int32),                 # unknown operation
-           np.array([[1, 0]] * 17 + [[12, 0]] * 16, dtype=np.int32)]   # deeper than 16 values
+           np.array([[1, 0]] * 17 + [[14, 0]] * 16, dtype=np.int32)]   # deeper than 16 values
     for ops in bad:
         assert L.amr_check_expr(_lib.ptr(np.ascontiguousarray(ops)), len(ops), 1) == -1
     with pytest.raises(ValueError):

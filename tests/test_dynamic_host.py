@@ -109,3 +109,61 @@ def test_auto_increment_follows_the_oracle(amaru, monkeypatch):
     D.addstage(ana2, bcs, nincs=8, tspan=4e-3)
     ok, msg = D.dyn_solve(ana2, alpha=20.0, maxits=10)
     assert not ok and "did not converge" in msg
+
+
+def test_failed_update_cuts_the_step(amaru, monkeypatch):
+    """update_state! failing inside an increment (here: NaN internal forces
+    injected at the third update, i.e. the first iteration of the second
+    increment) is not an error of the stage: the iteration counts as an
+    unbounded residue, the increment is retried from the committed states
+    with dT cut by 0.2, and the stage finishes; the oracle driver with the
+    same injection takes the same path to the same displacements."""
+    from amaru_amd import dynamic as D
+    from amaru_amd._lib import AmaruError
+    from oracle import dynamic as OD
+
+    calls = {"dev": 0, "ora": 0}
+
+    class FailingDev(OracleDev):
+        def update_state(self, dUt):
+            calls["dev"] += 1
+            if calls["dev"] == 3:
+                raise AmaruError(-6, "solve_system!: NaN values in internal forces vector")
+            return super().update_state(dUt)
+
+    monkeypatch.setattr(D, "DeviceModel", FailingDev)
+    mesh = amaru.Mesh(amaru.Block([[0, 0, 0], [1.0, 0.1, 0.1]], nx=6, ny=2, nz=2, cellshape="HEX8"))
+    model = amaru.FEModel(mesh, [("all", amaru.MechSolid(rho=7850.0), amaru.LinearElastic(E=200e9, nu=0.3))])
+    ana = D.DynamicAnalysis(model)
+    bcs = [("x==0", amaru.SurfaceBC(ux=0, uy=0, uz=0)), ("x==1", amaru.SurfaceBC(tz=-1e6))]
+    D.addstage(ana, bcs, nincs=4, tspan=2e-3)
+    ok, msg = D.dyn_solve(ana, autoinc=True)
+    assert ok, msg
+    # increment 1 (it 1, 2), increment 2 fails at once, then the cut step
+    assert ana.log[2][1] == 1 and ana.log[2][2] == float("inf")
+    assert all(np.isfinite(r) for _, _, r in ana.log[3:])
+    # without auto-increment the failure ends the stage as non-converged
+    calls["dev"] = 0
+    model2 = amaru.FEModel(mesh, [("all", amaru.MechSolid(rho=7850.0), amaru.LinearElastic(E=200e9, nu=0.3))])
+    ana2 = D.DynamicAnalysis(model2)
+    D.addstage(ana2, bcs, nincs=4, tspan=2e-3)
+    ok, msg = D.dyn_solve(ana2)
+    assert not ok and "did not converge" in msg
+
+    om = fem.Model(mesh.coords, mesh.conn, O8, [("LinearElastic", (200e9, 0.3))])
+    real = om.update_state
+
+    def failing(dUt, st):
+        calls["ora"] += 1
+        if calls["ora"] == 3:
+            dFin, st2, _ = real(dUt, st)
+            return dFin * np.nan, st2, "solve_system!: NaN values in internal forces vector"
+        return real(dUt, st)
+
+    om.update_state = failing
+    log = []
+    ok, msg = OD.dyn_stage_solver(om, [("x==0", fem.SurfaceBC(ux=0, uy=0, uz=0)), ("x==1", fem.SurfaceBC(tz=-1e6))],
+                                  7850.0, nincs=4, tspan=2e-3, autoinc=True, log=log)
+    assert ok, msg
+    assert [it for _, it, _ in ana.log] == [it for it, _ in log]
+    assert np.linalg.norm(model.uvals - om.uvals) <= 1e-10 * np.linalg.norm(om.uvals)

@@ -12,7 +12,8 @@ import pytest
 
 pytestmark = pytest.mark.gpu
 
-VALUES = [2.5e5, "3*z", "x^2 + sin(y) - 0.5*t", "-1e6*(1-x)*exp(-z)"]
+VALUES = [2.5e5, "3*z", "x^2 + sin(y) - 0.5*t", "-1e6*(1-x)*exp(-z)",
+          "max(x, 0.3)*tan(z) - min(y, 0.5) + log10(2+x) + div(7*y, 0.3)"]
 
 
 def mesh(am, shape, seed=0):
@@ -90,7 +91,7 @@ def test_bad_programs_are_refused(amaru):
     D = np.zeros((1, 4, 2))
     w = np.ones(1)
     F = np.zeros_like(C)
-    bad = np.array([[12, 0]], dtype=np.int32)   # ADD with an empty stack
+    bad = np.array([[14, 0]], dtype=np.int32)   # ADD with an empty stack
     rc = _lib.lib().amr_cell_loads(0, 1, 4, 2, 1, _lib.ptr(C), _lib.ptr(N), _lib.ptr(D), _lib.ptr(w), 2,
                                    _lib.ptr(bad), None, 1, 0, 0.0, _lib.ptr(F))
     assert rc == -1 and "underflow" in _lib.lib().amr_last_error().decode()

@@ -308,3 +308,23 @@ def test_device_bytes_estimate_is_monotone(amaru):
     # Tet10 at 1e5 dofs: Ke (900 doubles per element) dominates, a few hundred MB
     assert 1e8 < prev < 2e9
     assert math.isfinite(prev)
+
+
+def test_server_counts_unfreed_models_in_admission(amaru, monkeypatch):
+    """Finished models are freed at idle points; until then their bytes
+    count against the budget, so admitted plus unfreed bytes never exceed
+    it (the server frees them early when a request would not fit)."""
+    from amaru_amd import analysis, pool as P
+    monkeypatch.setattr(analysis, "DeviceModel", FakeDev)
+    monkeypatch.setattr(P, "DeviceModel", FakeDev)
+    monkeypatch.setattr(P, "Pool", FakePool)
+    alone = analyses(amaru, 6)
+    est = max(P.estimate_device_bytes(a.model) for a in alone)
+    budget = int(2.5 * est)
+    reqs = [P.Request(i, 0.0, (lambda i=i: (analyses(amaru, 6)[i], {}))) for i in range(6)]
+    srv = P.Server(hbm_budget=budget, workers=2)
+    done = srv.serve(reqs)
+    ref = [amaru.solve(a) for a in alone]
+    assert [(r.ok, r.msg) for r in sorted(done, key=lambda r: r.rid)] == ref
+    assert est <= srv.peak_held <= budget
+    assert srv.in_use == 0
