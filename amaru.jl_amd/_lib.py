@@ -93,6 +93,7 @@ SIGNATURES = {
     "amr_cell_loads": (I32, [I32, ctypes.c_int64, I32, I32, I32, P, P, P, P, I32, P, P, I32, I32, D, P]),
     "amr_block_mesh": (I32, [I32, I32, P, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, P, P, P, P]),
     "amr_dof_map": (I32, [I32, ctypes.c_int64, P, P, P, P, P]),
+    "amr_dof_map_noalloc": (I32, [I32, ctypes.c_int64, P, P, P, P, P]),
     "amr_assemble_M": (I32, [P, P]),
     "amr_combine_KM": (I32, [P, D, D]),
     "amr_spmv_col_bytes": (I32, [P, P]),

@@ -11,6 +11,8 @@
 // r = -1 + 2 ((1 / (sub nx)) i), physical ones the HEX8 map of the corners
 // summed corner by corner, rounded to 8 decimals (round half to even of
 // x * 1e8, -0 dropped: src/node.jl:58-60).
+#include <algorithm>
+#include <mutex>
 #include <vector>
 
 #include "amr_internal.h"
@@ -147,7 +149,48 @@ __global__ void __launch_bounds__(256) dof_count_kernel(int64_t n3, const uint8_
     }
 }
 
-__global__ void __launch_bounds__(256) dof_rank_kernel(int64_t n3, int64_t nu, const uint8_t *__restrict__ active,
+// Exclusive tile offsets (unknowns and prescribed, as int64) from the
+// per-tile counts, and the totals sizes[0] = ndofs, sizes[1] = nu: one
+// workgroup; thread i sums a contiguous run of tiles, a scan over the
+// threads gives each run's base.
+__global__ void __launch_bounds__(1024) dof_scan_kernel(int64_t nt, const int32_t *__restrict__ cnt,
+                                                        int64_t *__restrict__ off, int64_t *__restrict__ sizes)
+{
+    __shared__ int64_t au[1024], ap[1024];
+    const int tid = threadIdx.x;
+    const int64_t per = (nt + 1023) / 1024;
+    const int64_t t0 = std::min<int64_t>(nt, tid * per), t1 = std::min<int64_t>(nt, t0 + per);
+    int64_t su = 0, sp = 0;
+    for (int64_t t = t0; t < t1; t++) {
+        su += cnt[t * 2];
+        sp += cnt[t * 2 + 1];
+    }
+    au[tid] = su;
+    ap[tid] = sp;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+        const int64_t a = tid >= o ? au[tid - o] : 0;
+        const int64_t b = tid >= o ? ap[tid - o] : 0;
+        __syncthreads();
+        au[tid] += a;
+        ap[tid] += b;
+        __syncthreads();
+    }
+    int64_t bu = au[tid] - su, bp = ap[tid] - sp;
+    for (int64_t t = t0; t < t1; t++) {
+        off[t * 2] = bu;
+        off[t * 2 + 1] = bp;
+        bu += cnt[t * 2];
+        bp += cnt[t * 2 + 1];
+    }
+    if (tid == 1023) {
+        sizes[0] = au[1023] + ap[1023];
+        sizes[1] = au[1023];
+    }
+}
+
+__global__ void __launch_bounds__(256) dof_rank_kernel(int64_t n3, const int64_t *__restrict__ sizes,
+                                                       const uint8_t *__restrict__ active,
                                                        const uint8_t *__restrict__ presc,
                                                        const int64_t *__restrict__ off, int32_t *__restrict__ eq)
 {
@@ -172,8 +215,22 @@ __global__ void __launch_bounds__(256) dof_rank_kernel(int64_t n3, int64_t nu, c
         eq[d] = -1;
         return;
     }
+    const int64_t nu = sizes[1];
     eq[d] = pr ? (int32_t)(nu + off[blockIdx.x * 2 + 1] + sp[threadIdx.x] - 1)
                : (int32_t)(off[blockIdx.x * 2] + su[threadIdx.x] - 1);
+}
+
+// The three DOF-map kernels on stream s: cnt (nt*2 int32), off (nt*2
+// int64) and sizes (2 int64) are work and result space in device memory.
+int launch_dof_map(int64_t n3, const uint8_t *active, const uint8_t *presc, int32_t *eq, int32_t *cnt, int64_t *off,
+                   int64_t *sizes, hipStream_t s)
+{
+    const int64_t nt = (n3 + 255) / 256;
+    hipLaunchKernelGGL(dof_count_kernel, dim3((unsigned)nt), dim3(256), 0, s, n3, active, presc, cnt);
+    hipLaunchKernelGGL(dof_scan_kernel, dim3(1), dim3(1024), 0, s, nt, cnt, off, sizes);
+    hipLaunchKernelGGL(dof_rank_kernel, dim3((unsigned)nt), dim3(256), 0, s, n3, sizes, active, presc, off, eq);
+    AMR_HIP(hipGetLastError());
+    return AMR_OK;
 }
 
 template <typename T>
@@ -283,7 +340,7 @@ extern "C" int amr_dof_map(int device, int64_t nnodes, const uint8_t *active, co
     bufs.p.push_back(dpre);
     if (!rc) rc = dmalloc(&dcnt, (size_t)nt * 2);
     bufs.p.push_back(dcnt);
-    if (!rc) rc = dmalloc(&doff, (size_t)nt * 2);
+    if (!rc) rc = dmalloc(&doff, (size_t)nt * 2 + 2);   // offsets, then ndofs and nu
     bufs.p.push_back(doff);
     if (!rc) rc = dmalloc(&deq, (size_t)n3);
     bufs.p.push_back(deq);
@@ -296,26 +353,76 @@ extern "C" int amr_dof_map(int device, int64_t nnodes, const uint8_t *active, co
     } guard{s};
     if (active) AMR_HIP(hipMemcpyAsync(dact, active, (size_t)nnodes, hipMemcpyHostToDevice, s));
     AMR_HIP(hipMemcpyAsync(dpre, presc, (size_t)n3, hipMemcpyHostToDevice, s));
-    hipLaunchKernelGGL(dof_count_kernel, dim3((unsigned)nt), dim3(256), 0, s, n3, dact, dpre, dcnt);
-    AMR_HIP(hipGetLastError());
-    std::vector<int32_t> cnt((size_t)nt * 2);
-    AMR_HIP(hipMemcpyAsync(cnt.data(), dcnt, sizeof(int32_t) * nt * 2, hipMemcpyDeviceToHost, s));
-    AMR_HIP(hipStreamSynchronize(s));
-    // exclusive tile offsets (host: one value per 256 dofs)
-    std::vector<int64_t> off((size_t)nt * 2);
-    int64_t au = 0, ap = 0;
-    for (int64_t t = 0; t < nt; t++) {
-        off[t * 2] = au;
-        off[t * 2 + 1] = ap;
-        au += cnt[t * 2];
-        ap += cnt[t * 2 + 1];
-    }
-    AMR_HIP(hipMemcpyAsync(doff, off.data(), sizeof(int64_t) * nt * 2, hipMemcpyHostToDevice, s));
-    hipLaunchKernelGGL(dof_rank_kernel, dim3((unsigned)nt), dim3(256), 0, s, n3, au, dact, dpre, doff, deq);
-    AMR_HIP(hipGetLastError());
+    if (int e = launch_dof_map(n3, dact, dpre, deq, dcnt, doff, doff + nt * 2, s)) return e;
+    int64_t sizes[2];
     AMR_HIP(hipMemcpyAsync(eq, deq, sizeof(int32_t) * n3, hipMemcpyDeviceToHost, s));
+    AMR_HIP(hipMemcpyAsync(sizes, doff + nt * 2, sizeof(sizes), hipMemcpyDeviceToHost, s));
     AMR_HIP(hipStreamSynchronize(s));
-    *ndofs = au + ap;
-    *nu = au;
+    *ndofs = sizes[0];
+    *nu = sizes[1];
     return AMR_OK;
+}
+
+// ---------------------------------------------------------------------------
+// amr_dof_map_noalloc: the same kernels with their work space in a static
+// device array instead of allocated buffers.  Layout: per-tile counts
+// (AMR_DOFMAP_NOALLOC_MAX_DOFS/256 x 2 int32), then per-tile offsets (same
+// count x 2 int64), then ndofs and nu are written to the caller's sizes.
+// Every device holds its own copy at its own address, looked up once per
+// device with hipGetSymbolAddress; host code uses only that address.
+static constexpr int64_t kScrTiles = AMR_DOFMAP_NOALLOC_MAX_DOFS / 256;
+static constexpr size_t kScrCnt = (size_t)kScrTiles * 2 * sizeof(int32_t);
+static constexpr size_t kScrBytes = kScrCnt + (size_t)kScrTiles * 2 * sizeof(int64_t);
+static __device__ __align__(256) unsigned char scratch[kScrBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+// the device's address of scratch; call with the device current
+int scratch_address(int device, unsigned char **out)
+{
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void *p = nullptr;
+        AMR_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch_ptr[device] = p;
+    }
+    *out = (unsigned char *)g_scratch_ptr[device];
+    return AMR_OK;
+}
+}  // namespace
+
+extern "C" int amr_dof_map_noalloc(int device, int64_t nnodes, const uint8_t *active, const uint8_t *presc,
+                                   int32_t *eq, int64_t *sizes, void *stream)
+{
+    if (nnodes < 0 || !presc || !eq || !sizes) {
+        amr_set_error("amr_dof_map_noalloc: null pointer or negative node count");
+        return AMR_ERR_ARG;
+    }
+    const int64_t n3 = nnodes * 3;
+    if (n3 > AMR_DOFMAP_NOALLOC_MAX_DOFS) {
+        amr_set_error("amr_dof_map_noalloc: " + std::to_string(n3) + " dofs exceed the scratch limit of " +
+                      std::to_string((int64_t)AMR_DOFMAP_NOALLOC_MAX_DOFS));
+        return AMR_ERR_ARG;
+    }
+    int ndev = 0;
+    AMR_HIP(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev || device >= kMaxDevices) {
+        amr_set_error("amr_dof_map_noalloc: no device " + std::to_string(device));
+        return AMR_ERR_ARG;
+    }
+    AMR_HIP(hipSetDevice(device));
+    unsigned char *scr = nullptr;
+    if (int rc = scratch_address(device, &scr)) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    if (n3 == 0) {
+        // ndofs = nu = 0 through the scan kernel over no tiles
+        hipLaunchKernelGGL(dof_scan_kernel, dim3(1), dim3(1024), 0, s, (int64_t)0, (const int32_t *)scr,
+                           (int64_t *)(scr + kScrCnt), sizes);
+        AMR_HIP(hipGetLastError());
+        return AMR_OK;
+    }
+    return launch_dof_map(n3, active, presc, eq, (int32_t *)scr, (int64_t *)(scr + kScrCnt), sizes, s);
 }

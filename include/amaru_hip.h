@@ -397,6 +397,23 @@ int amr_block_mesh(int device, int etype, const double *corners, int64_t nx, int
  */
 int amr_dof_map(int device, int64_t nnodes, const uint8_t *active, const uint8_t *presc, int32_t *eq,
                 int64_t *ndofs, int64_t *nu);
+/*
+ * amr_dof_map that allocates nothing.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, serving loops that keep their buffers, harnesses that forbid
+ * hipMalloc).  active (nnodes flags, or NULL: all), presc (nnodes x 3
+ * flags), eq (nnodes x 3 eq ids out) and sizes (2 int64 out: ndofs, nu) are
+ * DEVICE pointers; stream is a hipStream_t (NULL: the default stream).  The
+ * call queues three kernels on the stream and returns; results are those of
+ * amr_dof_map.  At most AMR_DOFMAP_NOALLOC_MAX_DOFS dofs (3 x nnodes), else
+ * AMR_ERR_ARG.  The kernels' work space (per 256-dof tile counts and
+ * offsets, 1.5 MiB) is one static device array per GPU, held until the
+ * process ends and shared by every call on that GPU: calls that overlap on
+ * one GPU race.
+ */
+#define AMR_DOFMAP_NOALLOC_MAX_DOFS (1 << 24)
+int amr_dof_map_noalloc(int device, int64_t nnodes, const uint8_t *active, const uint8_t *presc, int32_t *eq,
+                        int64_t *sizes, void *stream);
 
 /*
  * Output post-processing.  Values per IP are ip_state_vals of the solid

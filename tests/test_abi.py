@@ -59,3 +59,14 @@ def test_load_programs_are_checked_without_gpu(amaru):
         assert L.amr_check_expr(_lib.ptr(np.ascontiguousarray(ops)), len(ops), 1) == -1
     with pytest.raises(ValueError):
         compile_program("x==0 && y>1")
+
+
+def test_dof_map_noalloc_refuses_oversize_without_gpu(amaru):
+    """The allocation-free DOF map checks its work-space limit before any
+    device call."""
+    from amaru_amd import _lib
+    L = _lib.lib()
+    n = (1 << 24) // 3 + 1   # 3n dofs: one node over the limit
+    rc = L.amr_dof_map_noalloc(0, n, None, 1, 1, 1, None)
+    assert rc == -1 and "scratch limit" in L.amr_last_error().decode()
+    assert L.amr_dof_map_noalloc(0, 10, None, None, 1, 1, None) == -1
