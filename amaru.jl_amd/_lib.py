@@ -83,6 +83,7 @@ SIGNATURES = {
     "amr_pool_step_end": (I32, [P, I32, P, P, ctypes.POINTER(SolveInfo)]),
     "amr_pool_size": (I32, [P, P]),
     "amr_pool_timing": (I32, [P, I32, P, P]),
+    "amr_pool_set_graphs": (I32, [P, I32, P, P]),
     "amr_model_device_bytes": (I32, [P, P]),
     "amr_mem_info": (I32, [I32, P, P]),
     "amr_set_coarse_refresh": (I32, [P, I32]),
@@ -98,6 +99,11 @@ SIGNATURES = {
     "amr_combine_KM": (I32, [P, D, D]),
     "amr_spmv_col_bytes": (I32, [P, P]),
     "amr_set_spmv_col_bytes": (I32, [P, I32]),
+    "amr_set_scheme": (I32, [P, I32]),
+    "amr_set_elem_tiles": (I32, [P, I32, I32]),
+    "amr_set_ke_path": (I32, [P, I32]),
+    "amr_get_elem_tiles": (I32, [P, P, P]),
+    "amr_tune_elem_tiles": (I32, [P, P, P]),
     "amr_matvec": (I32, [P, I32, P, P]),
     "amr_get_M_csc": (I32, [P, P, P, P, P]),
     "amr_ip_values": (I32, [P, I32, P]),

@@ -12,7 +12,9 @@ the hot calls going through the C ABI:
     copyto!.(StateBk, State)           -> DeviceModel.commit
 
 The increment / Newton-Raphson / auto-increment logic is kept as the
-reference has it (scheme :FE).
+reference has it, with its predictor-corrector schemes :FE (default), :ME,
+:BE and :Ralston (src/mech/mech-solver.jl:280-349; the device runs both
+solves and updates of an iteration, ``amr_set_scheme``).
 """
 import math
 
@@ -21,6 +23,8 @@ import numpy as np
 from .model import DeviceModel
 
 EPS = 2.220446049250313e-16
+
+SCHEMES = ("FE", "ME", "BE", "Ralston")   # src/mech/mech-solver.jl:280-288
 
 SOLVER_DEFAULTS = dict(tol=0.01, rspan=0.01, rtol=0.01, dT0=0.01, dTmin=1e-7, dTmax=0.1, scheme="FE", maxits=5,
                        autoinc=False, quiet=True,
@@ -47,7 +51,9 @@ class MechAnalysis:
         self.stages = []
         self.loggers = []      # callables f(model, T) after each converged increment
         self.log = []          # (stage, inc, it, residue) like solve.log
-        self.stats = dict(nr_iterations=0, pcg_iterations=0, increments=0, dof_solves=0)
+        # T: pseudo-time reached in the last stage; cutbacks: increments
+        # retried with a smaller step (auto-increment)
+        self.stats = dict(nr_iterations=0, pcg_iterations=0, increments=0, dof_solves=0, T=0.0, cutbacks=0)
         self.dev = None
         self.opts = None       # amr_solve_opts of the running stage
 
@@ -78,8 +84,10 @@ def solver_args(**kwargs):
     if unknown:
         raise TypeError(f"solve!: unknown arguments {sorted(unknown)}")
     args = dict(SOLVER_DEFAULTS, **kwargs)
-    if args["scheme"] != "FE":
-        raise NotImplementedError("only the :FE predictor-corrector scheme is implemented on the device")
+    scheme = str(args["scheme"]).lstrip(":")
+    if scheme not in SCHEMES:
+        raise ValueError(f"solve!: scheme must be one of {sorted(SCHEMES)}")
+    args["scheme"] = scheme
     if not 1 <= args["maxits"] <= 10:
         raise ValueError("maxits must be in 1..10")
     return args
@@ -122,6 +130,8 @@ def _stage_solver(ana, stage, a):
     else:
         ana.dev.set_dofs()
     dev = ana.dev
+    ana.stats["T"] = 0.0
+    dev.set_scheme(a["scheme"])
     Uex, Fex = model.bc_vals(bcs)
     dev.stage_begin(Uex, Fex)
     ana.opts = DeviceModel.opts(precision=a["precision"], rtol=a["pcg_rtol"], maxiter=a["pcg_maxiter"],
@@ -185,6 +195,7 @@ def _stage_solver(ana, stage, a):
             ana.stats["increments"] += 1
             ana.stats["dof_solves"] += nu
             T += dT
+            ana.stats["T"] = T
             if T > Tcheck - dTmin:
                 Tcheck += dTcheck
             if ana.loggers:
@@ -209,6 +220,7 @@ def _stage_solver(ana, stage, a):
             inc -= 1
             dev.increment_reject()
             if autoinc:
+                ana.stats["cutbacks"] += 1
                 q = 1 + math.tanh(math.log10(tol / (res1 + EPS)))
                 q = min(max(q, 0.2), 0.9)
                 if syserror:

@@ -193,6 +193,13 @@ struct amr_model {
     double *mvx = nullptr, *mvy = nullptr;   // 3*nnodes operands of amr_matvec
     double *Ksave = nullptr;                 // K as it was before the last amr_combine_KM
     int64_t ksave_version = -1;
+    // predictor-corrector scheme of the NR iterations (AMR_SCHEME_*) and the
+    // predictor's tangent, kept while the corrector's is assembled
+    int scheme = AMR_SCHEME_FE;
+    double *Kpred = nullptr;
+    // element tilings (index into the shape's table; -1: the tuned or default one)
+    int ke_var = -1, up_var = -1;
+    int ke_path = AMR_KE_EXACT;   // element stiffness of the NR iterations (amr_set_ke_path)
 };
 
 enum { SV_UEX = 0, SV_FEX, SV_U, SV_F, SV_DFEX, SV_R, SV_RC, SV_DUA, SV_DUI, SV_DFIN, SV_DUT, SV_X };
@@ -203,17 +210,24 @@ int amr_vec_fill(amr_model *m, double *x, double v);
 int amr_vec_solve_prep(amr_model *m, const double *dUi, const double *R, double *u2, double *ky, double *b);
 int amr_vec_combine(amr_model *m, const double *x, double *dUi);
 int amr_vec_add(amr_model *m, const double *a, const double *b, double *out);
+int amr_vec_scale(amr_model *m, double a, const double *x, double *out);
+int amr_vec_nan_flag(amr_model *m, const double *v, int32_t *flag);  // *flag = any NaN in v
 int amr_vec_nr_post(amr_model *m, double *res, double *dui_max, double *dua_max, int *nan_flag);
 int amr_vec_commit(amr_model *m);
 int amr_vec_nr_post_async(amr_model *m, double *hs, int32_t *hc);
 int amr_nr_post_slots(int *res, int *errn, int *errd, int *nan);
 
 // kernels (kernels_elem.hip)
-int amr_launch_ke(amr_model *m);
+int amr_launch_ke(amr_model *m);                 // the model's path (amr_set_ke_path)
+int amr_launch_ke_path(amr_model *m, int path);  // AMR_KE_EXACT or AMR_KE_MFMA
+int amr_elem_tiles(amr_model *m, int *ach, int *epw);
+int amr_elem_tiles_set(amr_model *m, int ach, int epw);
+int amr_elem_tiles_tune(amr_model *m, int *ach, int *epw);
 int amr_launch_assemble(amr_model *m, bool flags);
 int amr_launch_update(amr_model *m, const double *dU_node /*device, node order*/, double *dFin_node);
 int amr_launch_mass(amr_model *m);  // Me from Nd, rho_d; then assembly into Mv, Mflag
 int amr_launch_combine_km(amr_model *m, double a, double b);  // K := a K + b M, preconditioners refreshed
+int amr_launch_combine_kk(amr_model *m, double a1, const double *K1, double a2);  // K := a1 K1 + a2 K
 
 // two-level coarse space (coarse.hip): build on first use, refresh Kc^-1
 // after every assembly; queued on the model's stream
@@ -245,6 +259,7 @@ int amr_pool_step_collect(amr_pool *p, std::vector<amr_pool_result> &out);
 int amr_pool_step_members(const amr_pool *p);  // members of the step in flight (0: none)
 int amr_pool_step_query(amr_pool *p, int *done);
 int amr_pool_set_timing(amr_pool *p, int enable, double *ms, int64_t *n);
+int amr_pool_graphs(amr_pool *p, int enable, int64_t *captures, int64_t *replays);
 int amr_spmv_full(amr_model *m, const double *x, double *y, int mask_mode);
 int amr_spmv_vals(amr_model *m, const double *vals /* kv_at layout */, const double *x, double *y, int mask_mode);
 int amr_choose_lanes(amr_model *m);

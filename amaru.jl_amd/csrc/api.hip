@@ -215,7 +215,7 @@ static void free_model(amr_model *m)
     void *ptrs[] = {m->X,    m->conn, m->emat, m->mats,  m->dNdR, m->wip,  m->st_cur, m->st_com, m->Ke,
                     m->rowptr, m->col, m->diag, m->cptr,  m->csrc, m->fptr, m->fsrc,   m->Kv,     m->Kv32,
                     m->Kflag, m->dinv, m->dinv32, m->mask, m->eqd,  m->dF,   m->estat,  m->err,    m->part,
-                    m->scal, m->binv, m->binv32, m->Me, m->Mv, m->Mflag, m->Nd, m->rho_d, m->mvx, m->mvy, m->Ksave, m->col16};
+                    m->scal, m->binv, m->binv32, m->Me, m->Mv, m->Mflag, m->Nd, m->rho_d, m->mvx, m->mvy, m->Ksave, m->col16, m->Kpred};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (double *p : m->v)
@@ -537,7 +537,7 @@ extern "C" int amr_assemble(amr_model *m)
     }
     AMR_HIP(hipSetDevice(m->device));
     AMR_HIP(hipMemsetAsync(m->err, 0x7f, sizeof(int32_t), m->stream));  // "no bad element"
-    TRY(amr_launch_ke(m));
+    TRY(amr_launch_ke_path(m, AMR_KE_EXACT));   // mount_K's pattern comes from the exact element matrices
     TRY(amr_launch_assemble(m, true));
     int32_t err = 0;
     TRY(d2h(&err, m->err, 1, m->stream));
@@ -1059,10 +1059,119 @@ extern "C" int amr_nr_finish(amr_model *m, const amr_solve_info *si, amr_nr_info
     return amr_nr_finish_end(m, info);
 }
 
+extern "C" int amr_set_scheme(amr_model *m, int scheme)
+{
+    if (!m || scheme < AMR_SCHEME_FE || scheme > AMR_SCHEME_RALSTON) {
+        amr_set_error("amr_set_scheme: unknown scheme");
+        return AMR_ERR_ARG;
+    }
+    if (amr_pool_is_member(m)) {
+        amr_set_error("the model is solving in a pool; its matrix and solver vectors are in use");
+        return AMR_ERR_ARG;
+    }
+    m->scheme = scheme;
+    return AMR_OK;
+}
+
+// The error words and the NaN flag of an update_state! queued on the
+// model's stream (waits for it); AMR_OK or the failure, message set.
+static int update_status(amr_model *m)
+{
+    int32_t herr[4];
+    TRY(d2h(herr, m->err, 4, m->stream));
+    AMR_HIP(hipStreamSynchronize(m->stream));
+    if (herr[0] != 0x7f7f7f7f || herr[2] != 0x7f7f7f7f) {
+        amr_set_error("Negative Jacobian determinant in cell " +
+                      std::to_string(herr[0] != 0x7f7f7f7f ? herr[0] : herr[2]));
+        return AMR_ERR_JACOBIAN;
+    }
+    if (herr[1] != 0x7f7f7f7f) {
+        amr_set_error("VonMisses: Negative value for √J2D (cell " + std::to_string(herr[1]) + ")");
+        return AMR_ERR_MATERIAL;
+    }
+    if (herr[3]) {
+        amr_set_error("solve_system!: NaN values in internal forces vector");
+        return AMR_ERR_NAN;
+    }
+    return AMR_OK;
+}
+
+// Predictor and corrector of the :ME, :BE and :Ralston schemes
+// (src/mech/mech-solver.jl:280-288, 325-349): ΔUitr = p1 ΔUi, Rtr = q11 R;
+// solve_system!(K, ΔUitr, Rtr); update_state!(ΔUa + ΔUitr) from the
+// committed state; K2 = mount_K at that state; K = a1 K + a2 K2;
+// solve_system!(K, ΔUi, R); then amr_nr_finish's update_state!(ΔUa + ΔUi),
+// ΔUa += ΔUi and residual.  The predictor's update must succeed (its
+// failure ends the iteration as the reference's syserror does).
+static int nr_iteration_pc(amr_model *m, int it, const amr_solve_opts *o, amr_nr_info *info)
+{
+    static const double P[4][4] = {{1.0, 1.0, 0.0, 0.0}, {1.0, 1.0, 0.5, 0.5}, {1.0, 1.0, 0.0, 1.0},
+                                   {2.0 / 3.0, 2.0 / 3.0, 0.25, 0.75}};
+    const double p1 = P[m->scheme][0], q11 = P[m->scheme][1], a1 = P[m->scheme][2], a2 = P[m->scheme][3];
+    double **v = m->sv;
+    double *dUitr = m->v[11], *Rtr = m->v[7];
+    amr_nr_info inf{};
+    auto fail = [&](int rc) {
+        inf.status = rc;
+        if (info) *info = inf;
+        return rc;
+    };
+    // predictor: mount_K at the current state, the scaled system
+    if (it > 1) TRY(amr_vec_fill(m, v[SV_DUI], 0.0));
+    AMR_HIP(hipMemsetAsync(m->err, 0x7f, 3 * sizeof(int32_t), m->stream));
+    TRY(amr_launch_ke(m));
+    TRY(amr_launch_assemble(m, false));
+    m->assembled = true;
+    TRY(amr_vec_scale(m, p1, v[SV_DUI], dUitr));
+    TRY(amr_vec_scale(m, q11, v[SV_R], Rtr));
+    amr_solve_info si{};
+    if (m->nu > 0) {
+        TRY(amr_vec_solve_prep(m, dUitr, Rtr, m->v[8], m->v[9], m->v[10]));
+        const int rc = amr_pcg_solve(m, m->v[10], v[SV_X], o, &si);
+        inf.pcg_iterations = si.iterations;
+        inf.replacements = si.refinements;
+        if (rc != AMR_OK) return rc == AMR_ERR_SOLVE ? fail(rc) : rc;
+        TRY(amr_vec_combine(m, v[SV_X], dUitr));
+    }
+    TRY(amr_vec_add(m, v[SV_DUA], dUitr, v[SV_DUT]));
+    TRY(amr_launch_update(m, v[SV_DUT], v[SV_DFIN]));
+    TRY(amr_vec_nan_flag(m, v[SV_DFIN], m->err + 3));
+    if (int rc = update_status(m)) return fail(rc);
+    // corrector: the predictor's tangent kept, mount_K at the predicted state
+    if (!m->Kpred) TRY(dalloc(m, &m->Kpred, (size_t)kv_size(m->pat.nblk)));
+    AMR_HIP(hipMemcpyAsync(m->Kpred, m->Kv, sizeof(double) * kv_size(m->pat.nblk), hipMemcpyDeviceToDevice,
+                           m->stream));
+    AMR_HIP(hipMemsetAsync(m->err, 0x7f, 3 * sizeof(int32_t), m->stream));
+    TRY(amr_launch_ke(m));
+    TRY(amr_launch_assemble(m, false));
+    TRY(amr_launch_combine_kk(m, a1, m->Kpred, a2));
+    amr_solve_info si2{};
+    if (m->nu > 0) {
+        TRY(amr_vec_solve_prep(m, v[SV_DUI], v[SV_R], m->v[8], m->v[9], m->v[10]));
+        const int rc = amr_pcg_solve(m, m->v[10], v[SV_X], o, &si2);
+        if (rc != AMR_OK && rc != AMR_ERR_SOLVE) return rc;
+    }
+    const int rc = amr_nr_finish(m, m->nu > 0 ? &si2 : nullptr, info);
+    if (info) {
+        info->pcg_iterations += si.iterations;
+        info->replacements += si.refinements;
+    }
+    return rc;
+}
+
 extern "C" int amr_nr_iteration(amr_model *m, int it, const amr_solve_opts *opts, amr_nr_info *info)
 {
-    TRY(amr_nr_prepare(m, it));
     amr_solve_opts o = opts ? *opts : default_opts();
+    if (m && m->scheme != AMR_SCHEME_FE) {
+        if (!m->sv[SV_U] || it < 1) return AMR_ERR_ARG;
+        if (amr_pool_is_member(m)) {
+            amr_set_error("the model is solving in a pool; its matrix and solver vectors are in use");
+            return AMR_ERR_ARG;
+        }
+        AMR_HIP(hipSetDevice(m->device));
+        return nr_iteration_pc(m, it, &o, info);
+    }
+    TRY(amr_nr_prepare(m, it));
     amr_solve_info si{};
     if (m->nu > 0) {
         const int rc = amr_pcg_solve(m, m->v[10], m->sv[SV_X], &o, &si);
@@ -1176,6 +1285,10 @@ extern "C" int amr_pool_add(amr_pool *p, amr_model *m, const amr_solve_opts *opt
         amr_set_error("amr_pool_add: the model has no unknowns; call amr_nr_finish with si = NULL");
         return AMR_ERR_ARG;
     }
+    if (m->scheme != AMR_SCHEME_FE) {
+        amr_set_error("amr_pool_add: pooled solves run scheme :FE; use amr_nr_iteration for the others");
+        return AMR_ERR_ARG;
+    }
     AMR_HIP(hipSetDevice(m->device));
     const amr_solve_opts o = opts ? *opts : default_opts();
     return amr_pool_join(p, m, m->v[10], m->sv[SV_X], &o);
@@ -1242,6 +1355,49 @@ extern "C" int amr_pool_timing(amr_pool *p, int enable, double *spmv_ms, int64_t
 {
     if (!p) return AMR_ERR_ARG;
     return amr_pool_set_timing(p, enable, spmv_ms, spmv_launches);
+}
+
+extern "C" int amr_set_ke_path(amr_model *m, int path)
+{
+    if (!m || (path != AMR_KE_EXACT && path != AMR_KE_MFMA)) {
+        amr_set_error("amr_set_ke_path: path must be AMR_KE_EXACT or AMR_KE_MFMA");
+        return AMR_ERR_ARG;
+    }
+    if (path == AMR_KE_MFMA && m->etype != AMR_HEX20) {
+        amr_set_error("amr_set_ke_path: the matrix-core path is built for HEX20 elements");
+        return AMR_ERR_ARG;
+    }
+    m->ke_path = path;
+    return AMR_OK;
+}
+
+extern "C" int amr_set_elem_tiles(amr_model *m, int ach, int epw)
+{
+    if (!m) return AMR_ERR_ARG;
+    return amr_elem_tiles_set(m, ach, epw);
+}
+
+extern "C" int amr_get_elem_tiles(amr_model *m, int *ach, int *epw)
+{
+    if (!m || !ach || !epw) return AMR_ERR_ARG;
+    return amr_elem_tiles(m, ach, epw);
+}
+
+extern "C" int amr_tune_elem_tiles(amr_model *m, int *ach, int *epw)
+{
+    if (!m || !ach || !epw) return AMR_ERR_ARG;
+    if (amr_pool_is_member(m)) {
+        amr_set_error("the model is solving in a pool; its matrix and solver vectors are in use");
+        return AMR_ERR_ARG;
+    }
+    AMR_HIP(hipSetDevice(m->device));
+    return amr_elem_tiles_tune(m, ach, epw);
+}
+
+extern "C" int amr_pool_set_graphs(amr_pool *p, int enable, int64_t *captures, int64_t *replays)
+{
+    if (!p) return AMR_ERR_ARG;
+    return amr_pool_graphs(p, enable, captures, replays);
 }
 
 extern "C" int amr_model_device_bytes(amr_model *m, int64_t *bytes)

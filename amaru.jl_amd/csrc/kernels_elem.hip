@@ -9,6 +9,8 @@
 // non-zero accumulator unchanged, so skipping them does not change a bit.
 #include <cmath>
 
+#include <mutex>
+
 #include "amr_internal.h"
 
 namespace {
@@ -353,6 +355,168 @@ __global__ void __launch_bounds__(64) ke_kernel(int64_t ne, const double *__rest
     if (lane == 0 && sBad) atomicMin(err, (int32_t)(e + 1));
 }
 
+// ---------------------------------- element stiffness on the matrix cores
+//
+// HEX20 (60 dofs): Ke = sum_q Bq' (coef_q D_q Bq) as one product
+// [B1' ... Bnq'] [c1 D1 B1; ...] with FP64 MFMA 16x16x4 tiles: the 60x60
+// result padded to 64x64 is 4x4 tiles of 16x16, of which the 10 on or above
+// the diagonal are computed and mirrored.  The inner dimension runs over the
+// IPs, 8 per IP (the six Mandel rows and two zero rows: two MFMA k-steps).
+// One 64-lane workgroup per element.  First the geometry of every IP (J, its
+// inverse and coef by lanes q, calcD for plastic states, dN/dX and dN/dX/sqrt2
+// per (IP, node)) is staged in LDS, so the element needs three barriers.
+// Every product has the exact kernel's operands; only the summation order
+// differs (the matrix cores accumulate in their own order), so Ke agrees with
+// the exact path to rounding, not bit for bit.
+//
+// MFMA f64 16x16x4 operands (one value per lane): A[m][k] at lane
+// m + 16 k, B[k][n] at lane n + 16 k; the 16x16 result D[4 (l/16) + r][l%16]
+// in element r of lane l's four-value accumulator.
+typedef double f64x4m __attribute__((ext_vector_type(4)));
+
+// Bq[row][m] (Mandel B, src/mech/elem/mech-solid.jl:106-118) for dof m =
+// 3 node + dir from the node's (g, h = g/sqrt2); rows 6, 7 and dofs >= 60 are 0
+__device__ inline double b_entry(int row, int m, const double *g, const double *h)
+{
+    if (m >= 60) return 0.0;
+    const int n = m / 3, d = m % 3;
+    const double *gn = g + n * 3, *hn = h + n * 3;
+    switch (row) {
+    case 0: return d == 0 ? gn[0] : 0.0;
+    case 1: return d == 1 ? gn[1] : 0.0;
+    case 2: return d == 2 ? gn[2] : 0.0;
+    case 3: return d == 1 ? hn[2] : d == 2 ? hn[1] : 0.0;
+    case 4: return d == 0 ? hn[2] : d == 2 ? hn[0] : 0.0;
+    case 5: return d == 0 ? hn[1] : d == 1 ? hn[0] : 0.0;
+    }
+    return 0.0;
+}
+
+// (D Bq)[row][n]: the exact kernel's three-term chains for column n
+__device__ inline double db_entry(int row, int n, const double *D, const double *g, const double *h)
+{
+    if (n >= 60 || row >= 6) return 0.0;
+    const int b = n / 3, c = n % 3;
+    const double *Dk = D + row * 6;
+    const double *gb = g + b * 3, *hb = h + b * 3;
+    if (c == 0) return fma(Dk[5], hb[1], fma(Dk[4], hb[2], fma(Dk[0], gb[0], 0.0)));
+    if (c == 1) return fma(Dk[5], hb[0], fma(Dk[3], hb[2], fma(Dk[1], gb[1], 0.0)));
+    return fma(Dk[4], hb[0], fma(Dk[3], hb[1], fma(Dk[2], gb[2], 0.0)));
+}
+
+template <int NQ, int NQC>
+__global__ void __launch_bounds__(64, 2) ke_mfma_hex20_kernel(int64_t ne, const double *__restrict__ X,
+                                                              const int32_t *__restrict__ conn,
+                                                              const int32_t *__restrict__ emat,
+                                                              const AmrMat *__restrict__ mats,
+                                                              const double *__restrict__ dNdR,
+                                                              const double *__restrict__ wip,
+                                                              const double *__restrict__ st, int64_t nip,
+                                                              int all_elastic, double *__restrict__ Ke,
+                                                              int32_t *__restrict__ err)
+{
+    constexpr int NN = 20;
+    static_assert(NQ % NQC == 0, "IP chunks");
+    const int64_t e = blockIdx.x;
+    const int lane = threadIdx.x;
+    __shared__ double sC[NN * 3];
+    __shared__ double sG[NQC][NN * 3], sH[NQC][NN * 3];
+    __shared__ double sInv[NQC][9], sCoef[NQC];
+    __shared__ double sD[NQC][36];
+    __shared__ int sBad;
+    for (int i = lane; i < NN * 3; i += 64) sC[i] = X[(int64_t)conn[e * NN + i / 3] * 3 + i % 3];
+    const AmrMat m = mats[emat[e]];
+    const bool elastic = all_elastic || m.kind == AMR_MAT_LE;
+    if (lane == 0) {
+        sBad = 0;
+        if (elastic) calc_De(m.p[0], m.p[1], sD[0]);
+    }
+    const int lr = lane % 16, lk = lane / 16;
+    f64x4m acc[10];
+#pragma unroll
+    for (int t = 0; t < 10; t++) acc[t] = f64x4m{0.0, 0.0, 0.0, 0.0};
+    for (int q0 = 0; q0 < NQ; q0 += NQC) {
+        __syncthreads();
+        // geometry of the chunk's IPs: lanes q
+        if (lane < NQC) {
+            const int q = q0 + lane;
+            const double *dq = dNdR + q * NN * 3;
+            double J[9], inv[9];
+            for (int i = 0; i < 3; i++)
+                for (int j = 0; j < 3; j++) {
+                    double a = 0.0;
+                    for (int n = 0; n < NN; n++) a = fma(sC[n * 3 + i], dq[n * 3 + j], a);
+                    J[i * 3 + j] = a;
+                }
+            const double det = inv3(J, inv);
+            if (!(det > 0.0)) sBad = 1;
+            for (int k = 0; k < 9; k++) sInv[lane][k] = inv[k];
+            sCoef[lane] = det * wip[q];
+            if (!elastic) {
+                double s14[AMR_NSTATE];
+                const int64_t ip = e * NQ + q;
+                for (int f = 0; f < AMR_NSTATE; f++) s14[f] = st[f * nip + ip];
+                calc_D(m, s14, sD[lane]);
+            }
+        }
+        __syncthreads();
+        for (int t = lane; t < NQC * NN; t += 64) {
+            const int qc = t / NN, n = t % NN;
+            const double *dq = dNdR + (q0 + qc) * NN * 3;
+            for (int j = 0; j < 3; j++) {
+                double a = 0.0;
+                for (int i = 0; i < 3; i++) a = fma(dq[n * 3 + i], sInv[qc][i * 3 + j], a);
+                sG[qc][n * 3 + j] = a;
+                sH[qc][n * 3 + j] = a / SR2;
+            }
+        }
+        __syncthreads();
+        for (int qc = 0; qc < NQC; qc++) {
+            const double c = sCoef[qc];
+            const double *D = sD[elastic ? 0 : qc];
+            const double *g = sG[qc], *h = sH[qc];
+#pragma unroll
+            for (int ks = 0; ks < 2; ks++) {
+                const int row = ks * 4 + lk;
+                double a[4], b[4];
+#pragma unroll
+                for (int t = 0; t < 4; t++) {
+                    a[t] = b_entry(row, t * 16 + lr, g, h);
+                    b[t] = c * db_entry(row, t * 16 + lr, D, g, h);
+                }
+                int idx = 0;
+#pragma unroll
+                for (int mt = 0; mt < 4; mt++)
+#pragma unroll
+                    for (int nt = mt; nt < 4; nt++) {
+                        acc[idx] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[mt], b[nt], acc[idx], 0, 0, 0);
+                        idx++;
+                    }
+            }
+        }
+    }
+    // Ke in the node-block layout [a][b][3x3]; the tiles below the diagonal
+    // are the mirror of those above
+    double *Ke_e = Ke + e * (int64_t)(NN * NN * 9);
+    int idx = 0;
+#pragma unroll
+    for (int mt = 0; mt < 4; mt++)
+#pragma unroll
+        for (int nt = mt; nt < 4; nt++) {
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int row = mt * 16 + 4 * lk + r, col = nt * 16 + lr;
+                if (row < 60 && col < 60) {
+                    const double v = acc[idx][r];
+                    Ke_e[((row / 3) * NN + col / 3) * 9 + (row % 3) * 3 + col % 3] = v;
+                    if (mt != nt) Ke_e[((col / 3) * NN + row / 3) * 9 + (col % 3) * 3 + row % 3] = v;
+                }
+            }
+            idx++;
+        }
+    if (lane == 0 && sBad) atomicMin(err, (int32_t)(e + 1));
+}
+
 // ----------------------------------------------------------- element mass
 //
 // One 64-lane workgroup per element.  Lanes q < NQ build J (the stiffness
@@ -491,6 +655,25 @@ __global__ void __launch_bounds__(256) combine_km_kernel(int64_t nblk, double a,
         Kv[i] = v;
         Kv32[kv32_at(k, s)] = (float)v;
         Kflag[i] = Kflag[i] | Mflag[i];
+    }
+}
+
+// K := a1 K1 + a2 K (the predictor-corrector tangent of mech_stage_solver!,
+// src/mech/mech-solver.jl:343: K = a1*K + a2*K2, K1 the predictor's tangent
+// saved before K2 was assembled over it): one thread per block; FP64 and
+// FP32 values written.
+__global__ void __launch_bounds__(256) combine_kk_kernel(int64_t nblk, double a1, const double *__restrict__ K1,
+                                                         double a2, double *__restrict__ Kv,
+                                                         float *__restrict__ Kv32)
+{
+    const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (k >= nblk) return;
+#pragma unroll
+    for (int s = 0; s < 9; s++) {
+        const int64_t i = kv_at(k, s);
+        const double v = a1 * K1[i] + a2 * Kv[i];
+        Kv[i] = v;
+        Kv32[kv32_at(k, s)] = (float)v;
     }
 }
 
@@ -720,17 +903,184 @@ int launch_update_t(amr_model *m, const double *dU)
     return AMR_OK;
 }
 
-}  // namespace
+// Element tiles.  Per element shape (HEX8 8 IP, HEX20 27 IP, HEX20 8 IP,
+// TET10 4 IP) three stiffness tilings (ACH row nodes per lane; NN * ceil(NN
+// / ACH) <= 64 lanes) and three update tilings (EPW elements per 256-thread
+// workgroup, EPW * NQ <= 256).  Each Ke entry and each IP update is the
+// same arithmetic in every tiling, so the choice changes speed only.
+constexpr int ACH_OPT[4][3] = {{1, 2, 4}, {7, 10, 20}, {7, 10, 20}, {2, 5, 10}};
+constexpr int EPW_OPT[4][3] = {{32, 16, 8}, {9, 4, 2}, {16, 8, 32}, {32, 16, 64}};
 
-int amr_launch_ke(amr_model *m)
+int shape_index(const amr_model *m)
 {
     switch (m->etype) {
-    case AMR_HEX8: return launch_ke_t<8, 8, 1>(m);
-    case AMR_HEX20: return m->nips == 27 ? launch_ke_t<20, 27, 7>(m) : launch_ke_t<20, 8, 7>(m);
-    case AMR_TET10: return launch_ke_t<10, 4, 2>(m);
+    case AMR_HEX8: return 0;
+    case AMR_HEX20: return m->nips == 27 ? 1 : 2;
+    case AMR_TET10: return 3;
     }
+    return -1;
+}
+
+// the process-wide choice of the tuner per shape (-1: not tuned)
+std::mutex g_tile_mu;
+int g_ke_var[4] = {-1, -1, -1, -1}, g_up_var[4] = {-1, -1, -1, -1};
+
+int ke_variant(const amr_model *m, int sh)
+{
+    if (m->ke_var >= 0) return m->ke_var;
+    std::lock_guard<std::mutex> lk(g_tile_mu);
+    return g_ke_var[sh] >= 0 ? g_ke_var[sh] : 0;
+}
+
+int up_variant(const amr_model *m, int sh)
+{
+    if (m->up_var >= 0) return m->up_var;
+    std::lock_guard<std::mutex> lk(g_tile_mu);
+    return g_up_var[sh] >= 0 ? g_up_var[sh] : 0;
+}
+
+int launch_ke_var(amr_model *m, int sh, int v)
+{
+#define KE3(NN, NQ, A0, A1, A2) \
+    return v == 0 ? launch_ke_t<NN, NQ, A0>(m) : v == 1 ? launch_ke_t<NN, NQ, A1>(m) : launch_ke_t<NN, NQ, A2>(m)
+    switch (sh) {
+    case 0: KE3(8, 8, 1, 2, 4);
+    case 1: KE3(20, 27, 7, 10, 20);
+    case 2: KE3(20, 8, 7, 10, 20);
+    case 3: KE3(10, 4, 2, 5, 10);
+    }
+#undef KE3
     amr_set_error("unknown element type");
     return AMR_ERR_ARG;
+}
+
+int launch_update_var(amr_model *m, int sh, int v, const double *dU)
+{
+#define UP3(NN, NQ, E0, E1, E2)                                                                        \
+    return v == 0 ? launch_update_t<NN, NQ, E0>(m, dU)                                               \
+                  : v == 1 ? launch_update_t<NN, NQ, E1>(m, dU) : launch_update_t<NN, NQ, E2>(m, dU)
+    switch (sh) {
+    case 0: UP3(8, 8, 32, 16, 8);
+    case 1: UP3(20, 27, 9, 4, 2);
+    case 2: UP3(20, 8, 16, 8, 32);
+    case 3: UP3(10, 4, 32, 16, 64);
+    }
+#undef UP3
+    amr_set_error("unknown element type");
+    return AMR_ERR_ARG;
+}
+
+}  // namespace
+
+int amr_launch_ke_path(amr_model *m, int path)
+{
+    const int sh = shape_index(m);
+    if (sh < 0) {
+        amr_set_error("unknown element type");
+        return AMR_ERR_ARG;
+    }
+    if (path == AMR_KE_MFMA && m->etype == AMR_HEX20) {
+        if (m->ne == 0) return AMR_OK;
+        if (m->nips == 27)
+            hipLaunchKernelGGL((ke_mfma_hex20_kernel<27, 9>), dim3((unsigned)m->ne), dim3(64), 0, m->stream, m->ne, m->X,
+                               m->conn, m->emat, m->mats, m->dNdR, m->wip, m->st_cur, m->nip_total,
+                               m->all_elastic ? 1 : 0, m->Ke, m->err);
+        else
+            hipLaunchKernelGGL((ke_mfma_hex20_kernel<8, 8>), dim3((unsigned)m->ne), dim3(64), 0, m->stream, m->ne, m->X,
+                               m->conn, m->emat, m->mats, m->dNdR, m->wip, m->st_cur, m->nip_total,
+                               m->all_elastic ? 1 : 0, m->Ke, m->err);
+        AMR_HIP(hipGetLastError());
+        return AMR_OK;
+    }
+    return launch_ke_var(m, sh, ke_variant(m, sh));
+}
+
+int amr_launch_ke(amr_model *m) { return amr_launch_ke_path(m, m->ke_path); }
+
+int amr_elem_tiles(amr_model *m, int *ach, int *epw)
+{
+    const int sh = shape_index(m);
+    if (sh < 0) return AMR_ERR_ARG;
+    *ach = ACH_OPT[sh][ke_variant(m, sh)];
+    *epw = EPW_OPT[sh][up_variant(m, sh)];
+    return AMR_OK;
+}
+
+int amr_elem_tiles_set(amr_model *m, int ach, int epw)
+{
+    const int sh = shape_index(m);
+    if (sh < 0) return AMR_ERR_ARG;
+    int kv = -1, uv = -1;
+    for (int i = 0; i < 3; i++) {
+        if (ACH_OPT[sh][i] == ach) kv = i;
+        if (EPW_OPT[sh][i] == epw) uv = i;
+    }
+    if ((ach != 0 && kv < 0) || (epw != 0 && uv < 0)) {
+        amr_set_error("amr_set_elem_tiles: no such tiling for this element shape");
+        return AMR_ERR_ARG;
+    }
+    m->ke_var = ach == 0 ? -1 : kv;
+    m->up_var = epw == 0 ? -1 : uv;
+    return AMR_OK;
+}
+
+// Times the three tilings of each kernel on the model (element stiffness
+// from the current states; the update from the committed states with a zero
+// displacement increment, written to the current states) and keeps the
+// fastest for this model and, process-wide, for its element shape.
+int amr_elem_tiles_tune(amr_model *m, int *ach, int *epw)
+{
+    const int sh = shape_index(m);
+    if (sh < 0) return AMR_ERR_ARG;
+    hipEvent_t e0, e1;
+    AMR_HIP(hipEventCreate(&e0));
+    AMR_HIP(hipEventCreate(&e1));
+    struct Ev {
+        hipEvent_t a, b;
+        ~Ev()
+        {
+            (void)hipEventDestroy(a);
+            (void)hipEventDestroy(b);
+        }
+    } guard{e0, e1};
+    double *dU = m->v[6];
+    const size_t n3 = (size_t)m->nnodes * 3;
+    AMR_HIP(hipMemsetAsync(dU, 0, n3 * sizeof(double), m->stream));
+    AMR_HIP(hipMemsetAsync(m->err, 0x7f, 3 * sizeof(int32_t), m->stream));
+    auto best_of = [&](auto &&launch, int &best) -> int {
+        float tbest = 0.0f;
+        best = 0;
+        for (int v = 0; v < 3; v++) {
+            if (int rc = launch(v)) return rc;   // warm-up
+            AMR_HIP(hipEventRecord(e0, m->stream));
+            for (int r = 0; r < 3; r++)
+                if (int rc = launch(v)) return rc;
+            AMR_HIP(hipEventRecord(e1, m->stream));
+            AMR_HIP(hipEventSynchronize(e1));
+            float ms = 0.0f;
+            AMR_HIP(hipEventElapsedTime(&ms, e0, e1));
+            if (v == 0 || ms < tbest) {
+                tbest = ms;
+                best = v;
+            }
+        }
+        return AMR_OK;
+    };
+    int kb = 0, ub = 0;
+    if (int rc = best_of([&](int v) { return launch_ke_var(m, sh, v); }, kb)) return rc;
+    if (int rc = best_of([&](int v) { return launch_update_var(m, sh, v, dU); }, ub)) return rc;
+    AMR_HIP(hipMemsetAsync(m->err, 0x7f, 3 * sizeof(int32_t), m->stream));
+    AMR_HIP(hipStreamSynchronize(m->stream));
+    m->ke_var = kb;
+    m->up_var = ub;
+    {
+        std::lock_guard<std::mutex> lk(g_tile_mu);
+        g_ke_var[sh] = kb;
+        g_up_var[sh] = ub;
+    }
+    *ach = ACH_OPT[sh][kb];
+    *epw = EPW_OPT[sh][ub];
+    return AMR_OK;
 }
 
 int amr_launch_assemble(amr_model *m, bool flags)
@@ -759,6 +1109,25 @@ int amr_launch_combine_km(amr_model *m, double a, double b)
     const unsigned grid = (unsigned)((m->pat.nblk + 255) / 256);
     hipLaunchKernelGGL(combine_km_kernel, dim3(grid), dim3(256), 0, m->stream, m->pat.nblk, a, b, m->Kv, m->Kv32,
                        m->Kflag, m->Mv, m->Mflag);
+    AMR_HIP(hipGetLastError());
+    const unsigned g2 = (unsigned)((m->nnodes * 3 + 255) / 256);
+    hipLaunchKernelGGL(jacobi_kernel, dim3(g2), dim3(256), 0, m->stream, m->nnodes, m->pat.nblk, m->diag, m->Kv, m->mask,
+                       m->dinv, m->dinv32);
+    AMR_HIP(hipGetLastError());
+    const unsigned g3 = (unsigned)((m->nnodes + 255) / 256);
+    hipLaunchKernelGGL(block_jacobi_kernel, dim3(g3), dim3(256), 0, m->stream, m->nnodes, m->pat.nblk, m->diag, m->Kv,
+                       m->mask, m->binv, m->binv32);
+    AMR_HIP(hipGetLastError());
+    return AMR_OK;
+}
+
+int amr_launch_combine_kk(amr_model *m, double a1, const double *K1, double a2)
+{
+    m->k_version++;
+    if (m->pat.nblk == 0) return AMR_OK;
+    const unsigned grid = (unsigned)((m->pat.nblk + 255) / 256);
+    hipLaunchKernelGGL(combine_kk_kernel, dim3(grid), dim3(256), 0, m->stream, m->pat.nblk, a1, K1, a2, m->Kv,
+                       m->Kv32);
     AMR_HIP(hipGetLastError());
     const unsigned g2 = (unsigned)((m->nnodes * 3 + 255) / 256);
     hipLaunchKernelGGL(jacobi_kernel, dim3(g2), dim3(256), 0, m->stream, m->nnodes, m->pat.nblk, m->diag, m->Kv, m->mask,
@@ -802,14 +1171,12 @@ int amr_launch_mass(amr_model *m)
 
 int amr_launch_update(amr_model *m, const double *dU, double *dFin)
 {
-    int rc;
-    switch (m->etype) {
-    case AMR_HEX8: rc = launch_update_t<8, 8, 32>(m, dU); break;
-    case AMR_HEX20: rc = m->nips == 27 ? launch_update_t<20, 27, 9>(m, dU) : launch_update_t<20, 8, 16>(m, dU); break;
-    case AMR_TET10: rc = launch_update_t<10, 4, 32>(m, dU); break;
-    default: amr_set_error("unknown element type"); return AMR_ERR_ARG;
+    const int sh = shape_index(m);
+    if (sh < 0) {
+        amr_set_error("unknown element type");
+        return AMR_ERR_ARG;
     }
-    if (rc) return rc;
+    if (int rc = launch_update_var(m, sh, up_variant(m, sh), dU)) return rc;
     unsigned g = (unsigned)((m->nnodes * 3 + 255) / 256);
     hipLaunchKernelGGL(force_gather_kernel, dim3(g), dim3(256), 0, m->stream, m->nnodes, m->nn, m->fptr, m->fsrc,
                        m->dF, dFin);

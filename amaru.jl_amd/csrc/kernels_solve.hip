@@ -37,6 +37,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 
 #include "amr_internal.h"
@@ -473,6 +474,7 @@ template <int L, typename VT, typename XT, typename AT>
 __global__ void __launch_bounds__(WGS) pcg_spmv_kernel(const SegDesc *__restrict__ D, const int2 *__restrict__ tl)
 {
     const int2 te = tl[xcd_tile(blockIdx.x, gridDim.x)];
+    if (te.x < 0) return;  // padding of a captured step's list
     const SegDesc &d = D[te.x];
     int32_t *ctl = d.ctl;
     if (ctl[C_DONE]) return;
@@ -497,21 +499,26 @@ __global__ void __launch_bounds__(WGS) pcg_spmv_kernel(const SegDesc *__restrict
     const VT *vals = seg_vals<VT>(d);
     const XT *px = seg_px<XT>(d);
     for (int i = threadIdx.x / L; i < nr; i += RPW) {
+        // lanes 0..2 of the row group fetch the row's mask and p entries
+        // before its blocks, so the row's tail waits on no load
+        const int64_t k3 = (r0 + i) * 3 + (sub < 3 ? sub : 0);
+        double mk = 0.0, pk = 0.0;
+        if (sub < 3) {
+            mk = gl(d.mask)[k3];
+            pk = gl(d.p)[k3];
+        }
         AT y[3];
         if (d.col16)
             row_sum<L, false, VT, XT, AT, int16_t>(srp[i], srp[i + 1], sub, d.col16, vals, px, y, r0 + i);
         else
             row_sum<L, false, VT, XT, AT>(srp[i], srp[i + 1], sub, d.col, vals, px, y);
-        if (sub == 0) {
-            double c = 0.0;
-            for (int e = 0; e < 3; e++) {
-                const int64_t k = (r0 + i) * 3 + e;
-                const double qk = gl(d.mask)[k] != 0.0 ? (double)y[e] : 0.0;
-                glw(d.q)[k] = qk;
-                c = fma(gl(d.p)[k], qk, c);
-            }
-            sdot[i] = c;
-        }
+        // lane e < 3 stores q of component e; lane 0 forms the row's p.q,
+        // fma(p2, q2, fma(p1, q1, fma(p0, q0, 0)))
+        const double qk = mk != 0.0 ? (double)(sub == 0 ? y[0] : sub == 1 ? y[1] : y[2]) : 0.0;
+        if (sub < 3) glw(d.q)[k3] = qk;
+        const double q1 = __shfl_down(qk, 1, L), q2 = __shfl_down(qk, 2, L);
+        const double p1 = __shfl_down(pk, 1, L), p2 = __shfl_down(pk, 2, L);
+        if (sub == 0) sdot[i] = fma(p2, q2, fma(p1, q1, fma(pk, qk, 0.0)));
     }
     __syncthreads();
     const double pq = lds_tree_sum(sdot, RP);
@@ -521,6 +528,7 @@ __global__ void __launch_bounds__(WGS) pcg_spmv_kernel(const SegDesc *__restrict
 __global__ void __launch_bounds__(WGV) pcg_update_kernel(const SegDesc *__restrict__ D, const int2 *__restrict__ vl)
 {
     const int2 ve = vl[blockIdx.x];
+    if (ve.x < 0) return;
     const SegDesc &d = D[ve.x];
     int32_t *ctl = d.ctl;
     if (ctl[C_DONE]) return;
@@ -597,6 +605,7 @@ __global__ void __launch_bounds__(WGV) pcg_update_kernel(const SegDesc *__restri
 __global__ void __launch_bounds__(WGV) pcg_dir_kernel(const SegDesc *__restrict__ D, const int2 *__restrict__ vl)
 {
     const int2 ve = vl[blockIdx.x];
+    if (ve.x < 0) return;
     const SegDesc &d = D[ve.x];
     int32_t *ctl = d.ctl;
     if (ctl[C_DONE]) return;
@@ -645,6 +654,7 @@ __global__ void __launch_bounds__(64) tl_restrict_kernel(const SegDesc *__restri
                                                          int mode)
 {
     const int2 e = al[blockIdx.x];
+    if (e.x < 0) return;
     const SegDesc &d = D[e.x];
     if (mode == 0 && d.ctl[C_DONE]) return;
     const int32_t a = e.y;
@@ -673,6 +683,7 @@ __global__ void __launch_bounds__(256) tl_coarse_kernel(const SegDesc *__restric
                                                         int mode)
 {
     const int2 e = wl[blockIdx.x];
+    if (e.x < 0) return;
     const SegDesc &d = D[e.x];
     if (mode == 0 && d.ctl[C_DONE]) return;
     const int pi = (d.ctl[C_ITERS] - 1) & 1;
@@ -830,9 +841,11 @@ __global__ void fill_kernel(int64_t n, double *__restrict__ x, double v)
 
 // Row tiles of a model.  The SpMV kernels run 4 workgroups per CU (8 waves
 // per SIMD), so ncu*4 tiles are resident at once: R is chosen so that the
-// tiles fill a whole number k of such rounds (k ~ rows / (128 * resident)),
-// avoiding a nearly empty last round, with R a multiple of 8, at least 64,
-// and at most MAX_GR tiles.  Vector kernels: at most MAX_GV workgroups.
+// tiles fill a whole number k of such rounds (k ~ rows / (64 * resident):
+// about 64 rows per tile, which measured 10% faster than 128 on the 40^3
+// HEX20 block), avoiding a nearly empty last round, with R a multiple of 8,
+// at least 64, and at most MAX_GR tiles.  Vector kernels: at most MAX_GV
+// workgroups.
 struct Geom {
     int R, ntiles, nvblk;
 };
@@ -842,7 +855,7 @@ Geom model_geom(const amr_model *m)
     Geom g;
     const int64_t n = std::max<int64_t>(1, m->nnodes);
     const int64_t res = (int64_t)std::max(1, m->ncu) * 4;
-    const int64_t k = std::max<int64_t>(1, (n + 64 * res) / (128 * res));
+    const int64_t k = std::max<int64_t>(1, (n + 32 * res) / (64 * res));
     int64_t R = (n + k * res - 1) / (k * res);
     R = std::max<int64_t>(64, (R + 7) / 8 * 8);
     if ((n + R - 1) / R > MAX_GR) R = ((n + MAX_GR - 1) / MAX_GR + 7) / 8 * 8;
@@ -960,6 +973,19 @@ struct amr_pool {
     std::vector<hipEvent_t> ev;
     double t_spmv_ms = 0.0;
     int64_t n_spmv = 0;
+    // captured iteration blocks of a step (check_every iterations of every
+    // PCG phase), keyed by their launch shapes; replayed while the shapes
+    // stay (work lists padded to size classes, members joining and leaving
+    // change only the lists' contents)
+    struct StepGraph {
+        std::vector<int64_t> key;
+        hipGraphExec_t exec = nullptr;
+        uint64_t used = 0;
+    };
+    std::vector<StepGraph> graphs;
+    bool use_graphs = true;
+    uint64_t graph_clock = 0;
+    int64_t graph_captures = 0, graph_replays = 0;
 };
 
 namespace {
@@ -981,11 +1007,19 @@ int pool_begin_host_writes(amr_pool *p)
     return AMR_OK;
 }
 
+void drop_graphs(amr_pool *p)
+{
+    for (auto &g : p->graphs)
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    p->graphs.clear();
+}
+
 int pool_reserve_lists(amr_pool *p, size_t bytes)
 {
     if (bytes <= p->lcap) return AMR_OK;
     // kernels queued earlier may still read the device list buffer
     if (int rc = pool_sync(p)) return rc;
+    drop_graphs(p);  // they point into the old buffer
     size_t ncap = std::max<size_t>(bytes, 2 * p->lcap);
     ncap = std::max<size_t>(ncap, 1 << 16);
     if (p->dL) AMR_HIP(hipFree(p->dL));
@@ -1163,6 +1197,24 @@ void build_groups(amr_pool *p, const std::vector<int> &slots, bool residual, std
         if (!ent[key].empty()) gr.push_back({key / 4, LANES[key % 4], rmax[key], 0, (unsigned)ent[key].size()});
 }
 
+// Work lists of a captured step are padded with empty entries (member -1,
+// which the kernels skip) to a size class: 64, then 2^k and 3*2^(k-1).
+size_t size_class(size_t n)
+{
+    if (n <= 64) return 64;
+    size_t c = 64;
+    while (c < n) {
+        if (c + c / 2 >= n) return c + c / 2;
+        c *= 2;
+    }
+    return c;
+}
+
+void pad_list(std::vector<int2> &v)
+{
+    if (!v.empty()) v.resize(size_class(v.size()), make_int2(-1, -1));
+}
+
 std::vector<int2> vec_list(amr_pool *p, const std::vector<int> &slots)
 {
     std::vector<int2> v;
@@ -1208,6 +1260,9 @@ int amr_pool_new(amr_pool **out, int device, int cap, int check, hipStream_t str
     p->cap = cap;
     p->check = std::max(1, std::min(check > 0 ? check : 32, 256));
     p->mem.assign(cap, Member());
+    // AMR_POOL_GRAPHS=0 launches steps kernel by kernel (for tools that do
+    // not follow graph launches)
+    if (const char *g = std::getenv("AMR_POOL_GRAPHS")) p->use_graphs = std::atoi(g) != 0;
     int rc = AMR_OK;
     auto fail = [&](hipError_t e) {
         amr_set_error(std::string("HIP error: ") + hipGetErrorString(e) + " creating a pool");
@@ -1254,6 +1309,7 @@ void amr_pool_free(amr_pool *p)
         if (e) (void)hipEventDestroy(e);
     if (p->ev_upload) (void)hipEventDestroy(p->ev_upload);
     if (p->ev_step) (void)hipEventDestroy(p->ev_step);
+    drop_graphs(p);
     if (p->own_stream && p->stream) (void)hipStreamDestroy(p->stream);
     delete p;
 }
@@ -1414,27 +1470,43 @@ int amr_pool_step_launch(amr_pool *p)
     const int check = p->check;
     hipStream_t st = p->stream;
 
-    // (a) lists for the initialisation and the iterations
+    // (a) lists for the iterations (first, so that their offsets depend on
+    // their own sizes only) and for the initialisation of joiners
+    const bool graph = p->use_graphs && !p->timing;
     std::vector<std::vector<int2>> ent;
     std::vector<Group> gr;
     if (int rc = pool_begin_host_writes(p)) return rc;
     build_groups(p, act, false, ent, gr);
-    std::vector<int2> vj = vec_list(p, join), va = vec_list(p, act);
+    std::vector<int2> va = vec_list(p, act), aa = agg_list(p, act), ca = cw_list(p, act);
+    if (graph) {
+        for (auto &e : ent) pad_list(e);
+        int gi = 0;
+        for (const auto &e : ent)
+            if (!e.empty()) {
+                gr[gi].n = (unsigned)e.size();
+                gr[gi].Rmax = (gr[gi].Rmax + 63) / 64 * 64;
+                gi++;
+            }
+        pad_list(va);
+        pad_list(aa);
+        pad_list(ca);
+    }
+    std::vector<int2> vj = vec_list(p, join);
     std::vector<int32_t> sj(join.begin(), join.end());
     std::vector<int2> aj = agg_list(p, join), cj = cw_list(p, join), pj = vec_list(p, tl_only(p, join));
-    std::vector<int2> aa = agg_list(p, act), ca = cw_list(p, act);
     size_t need = padded(vj) + padded(va) + padded(sj) + padded(aj) + padded(cj) + padded(pj) + padded(aa) +
                   padded(ca);
     for (const auto &e : ent) need += padded(e);
     if (int rc = pool_reserve_lists(p, need)) return rc;
     Lists Ls{p};
-    const size_t o_vj = Ls.push(vj), o_va = Ls.push(va), o_sj = Ls.push(sj);
-    const size_t o_aj = Ls.push(aj), o_cj = Ls.push(cj), o_pj = Ls.push(pj), o_aa = Ls.push(aa), o_ca = Ls.push(ca);
+    const size_t o_va = Ls.push(va), o_aa = Ls.push(aa), o_ca = Ls.push(ca);
     {
         int gi = 0;
         for (const auto &e : ent)
             if (!e.empty()) gr[gi++].off = Ls.push(e);
     }
+    const size_t o_vj = Ls.push(vj), o_sj = Ls.push(sj);
+    const size_t o_aj = Ls.push(aj), o_cj = Ls.push(cj), o_pj = Ls.push(pj);
     if (int rc = upload(p, Ls.off, p->desc_dirty)) return rc;
     if (!join.empty()) {
         hipLaunchKernelGGL(pcg_init_kernel, dim3((unsigned)vj.size()), dim3(WGV), 0, st, p->dD,
@@ -1445,20 +1517,63 @@ int amr_pool_step_launch(amr_pool *p)
     }
     if (p->timing)
         if (int rc = ensure_events(p)) return rc;
-    for (int j = 0; j < check; j++) {
-        if (p->timing) AMR_HIP(hipEventRecord(p->ev[2 * j], st));
-        for (const Group &g : gr) launch_pcg_spmv(g.kind, g.lanes, g.n, g.Rmax, st, p->dD, Ls.dev_at<int2>(g.off));
-        if (p->timing) AMR_HIP(hipEventRecord(p->ev[2 * j + 1], st));
-        hipLaunchKernelGGL(pcg_update_kernel, dim3((unsigned)va.size()), dim3(WGV), 0, st, p->dD,
-                           Ls.dev_at<int2>(o_va));
-        if (!aa.empty()) {
-            hipLaunchKernelGGL(tl_restrict_kernel, dim3((unsigned)aa.size()), dim3(64), 0, st, p->dD,
-                               Ls.dev_at<int2>(o_aa), 0);
-            hipLaunchKernelGGL(tl_coarse_kernel, dim3((unsigned)ca.size()), dim3(256), 0, st, p->dD,
-                               Ls.dev_at<int2>(o_ca), 0);
+    // (b) check_every iterations of every phase
+    auto iterations = [&]() {
+        for (int j = 0; j < check; j++) {
+            if (p->timing) (void)hipEventRecord(p->ev[2 * j], st);
+            for (const Group &g : gr)
+                launch_pcg_spmv(g.kind, g.lanes, g.n, g.Rmax, st, p->dD, Ls.dev_at<int2>(g.off));
+            if (p->timing) (void)hipEventRecord(p->ev[2 * j + 1], st);
+            hipLaunchKernelGGL(pcg_update_kernel, dim3((unsigned)va.size()), dim3(WGV), 0, st, p->dD,
+                               Ls.dev_at<int2>(o_va));
+            if (!aa.empty()) {
+                hipLaunchKernelGGL(tl_restrict_kernel, dim3((unsigned)aa.size()), dim3(64), 0, st, p->dD,
+                                   Ls.dev_at<int2>(o_aa), 0);
+                hipLaunchKernelGGL(tl_coarse_kernel, dim3((unsigned)ca.size()), dim3(256), 0, st, p->dD,
+                                   Ls.dev_at<int2>(o_ca), 0);
+            }
+            hipLaunchKernelGGL(pcg_dir_kernel, dim3((unsigned)va.size()), dim3(WGV), 0, st, p->dD,
+                               Ls.dev_at<int2>(o_va));
         }
-        hipLaunchKernelGGL(pcg_dir_kernel, dim3((unsigned)va.size()), dim3(WGV), 0, st, p->dD,
-                           Ls.dev_at<int2>(o_va));
+    };
+    if (!graph) {
+        iterations();
+    } else {
+        std::vector<int64_t> key = {check, (int64_t)(intptr_t)p->dL, (int64_t)va.size(), (int64_t)aa.size(),
+                                    (int64_t)ca.size()};
+        for (const Group &g : gr) key.insert(key.end(), {g.kind, g.lanes, g.Rmax, (int64_t)g.n});
+        amr_pool::StepGraph *hit = nullptr;
+        for (auto &g : p->graphs)
+            if (g.key == key) hit = &g;
+        if (!hit) {
+            // capture the iterations once for these shapes (no memset nodes:
+            // the kernels write every value they later read)
+            AMR_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
+            iterations();
+            const hipError_t le = hipGetLastError();
+            hipGraph_t g = nullptr;
+            const hipError_t ce = hipStreamEndCapture(st, &g);
+            AMR_HIP(le);
+            AMR_HIP(ce);
+            hipGraphExec_t ex = nullptr;
+            const hipError_t ie = hipGraphInstantiateWithFlags(&ex, g, 0);
+            (void)hipGraphDestroy(g);
+            AMR_HIP(ie);
+            if (p->graphs.size() >= 8) {
+                auto lru = std::min_element(p->graphs.begin(), p->graphs.end(),
+                                            [](const auto &a, const auto &b) { return a.used < b.used; });
+                AMR_HIP(hipStreamSynchronize(st));   // it may still be running
+                (void)hipGraphExecDestroy(lru->exec);
+                p->graphs.erase(lru);
+            }
+            p->graphs.push_back({key, ex, 0});
+            hit = &p->graphs.back();
+            p->graph_captures++;
+        } else {
+            p->graph_replays++;
+        }
+        hit->used = ++p->graph_clock;
+        AMR_HIP(hipGraphLaunch(hit->exec, st));
     }
     AMR_HIP(hipGetLastError());
     const size_t sbytes = sizeof(double) * SEG_DBL * p->cap;
@@ -1672,6 +1787,20 @@ int amr_pool_set_timing(amr_pool *p, int enable, double *ms, int64_t *n)
     p->timing = enable != 0;
     if (ms) *ms = p->t_spmv_ms;
     if (n) *n = p->n_spmv;
+    return AMR_OK;
+}
+
+int amr_pool_graphs(amr_pool *p, int enable, int64_t *captures, int64_t *replays)
+{
+    if (enable >= 0) {
+        if (p->in_flight) {
+            amr_set_error("amr_pool_graphs: a step is in flight");
+            return AMR_ERR_ARG;
+        }
+        p->use_graphs = enable != 0;
+    }
+    if (captures) *captures = p->graph_captures;
+    if (replays) *replays = p->graph_replays;
     return AMR_OK;
 }
 
@@ -1890,6 +2019,20 @@ __global__ void vadd_kernel(int64_t n, const double *__restrict__ a, const doubl
     for (int64_t i = (int64_t)blockIdx.x * WGV + threadIdx.x; i < n; i += (int64_t)gridDim.x * WGV) out[i] = a[i] + b[i];
 }
 
+__global__ void scale_kernel(int64_t n, double a, const double *__restrict__ x, double *__restrict__ out)
+{
+    for (int64_t i = (int64_t)blockIdx.x * WGV + threadIdx.x; i < n; i += (int64_t)gridDim.x * WGV) out[i] = a * x[i];
+}
+
+// flag = 1 if any entry of v is NaN (flag cleared by the caller)
+__global__ void nan_flag_kernel(int64_t n, const double *__restrict__ v, int32_t *__restrict__ flag)
+{
+    bool any = false;
+    for (int64_t i = (int64_t)blockIdx.x * WGV + threadIdx.x; i < n; i += (int64_t)gridDim.x * WGV)
+        any = any || v[i] != v[i];
+    if (__any(any) && threadIdx.x % 64 == 0) atomicOr(flag, 1);
+}
+
 // ΔUa += ΔUi; R = mask .* (ΔFex - ΔFin); per-workgroup max|R|, max|ΔUi|,
 // max|ΔUa| and a NaN-in-ΔFin flag
 __global__ void __launch_bounds__(WGV) nr_post_kernel(int64_t n, const double *__restrict__ mask,
@@ -1996,6 +2139,21 @@ int amr_vec_solve_prep(amr_model *m, const double *dUi, const double *R, double 
 int amr_vec_combine(amr_model *m, const double *x, double *dUi)
 {
     hipLaunchKernelGGL(combine_kernel, dim3(vec_grid(m)), dim3(WGV), 0, m->stream, m->nnodes * 3, m->mask, x, dUi);
+    AMR_HIP(hipGetLastError());
+    return AMR_OK;
+}
+
+int amr_vec_scale(amr_model *m, double a, const double *x, double *out)
+{
+    hipLaunchKernelGGL(scale_kernel, dim3(vec_grid(m)), dim3(WGV), 0, m->stream, m->nnodes * 3, a, x, out);
+    AMR_HIP(hipGetLastError());
+    return AMR_OK;
+}
+
+int amr_vec_nan_flag(amr_model *m, const double *v, int32_t *flag)
+{
+    AMR_HIP(hipMemsetAsync(flag, 0, sizeof(int32_t), m->stream));
+    hipLaunchKernelGGL(nan_flag_kernel, dim3(vec_grid(m)), dim3(WGV), 0, m->stream, m->nnodes * 3, v, flag);
     AMR_HIP(hipGetLastError());
     return AMR_OK;
 }

@@ -691,6 +691,34 @@ class DeviceModel:
         _lib.check(_lib.lib().amr_spmv_col_bytes(self._h, ctypes.byref(b)))
         return b.value
 
+    def set_ke_path(self, path):
+        """Element stiffness of the NR iterations: "exact" (bit for bit the
+        oracle's) or "mfma" (HEX20 on the FP64 matrix cores, equal to rounding)."""
+        _lib.check(_lib.lib().amr_set_ke_path(self._h, {"exact": 0, "mfma": 1}[path]))
+
+    def elem_tiles(self):
+        """(ach, epw): the element stiffness and update tilings in use."""
+        a, e = ctypes.c_int(0), ctypes.c_int(0)
+        _lib.check(_lib.lib().amr_get_elem_tiles(self._h, ctypes.byref(a), ctypes.byref(e)))
+        return a.value, e.value
+
+    def set_elem_tiles(self, ach=0, epw=0):
+        _lib.check(_lib.lib().amr_set_elem_tiles(self._h, int(ach), int(epw)))
+
+    def tune_elem_tiles(self):
+        """Times the tilings and keeps the fastest (for this element shape in
+        the process too); returns (ach, epw)."""
+        a, e = ctypes.c_int(0), ctypes.c_int(0)
+        _lib.check(_lib.lib().amr_tune_elem_tiles(self._h, ctypes.byref(a), ctypes.byref(e)))
+        return a.value, e.value
+
+    SCHEMES = {"FE": 0, "ME": 1, "BE": 2, "Ralston": 3}
+
+    def set_scheme(self, scheme):
+        """Predictor-corrector scheme of nr_iteration (mech_stage_solver!'s
+        scheme: FE, ME, BE or Ralston)."""
+        _lib.check(_lib.lib().amr_set_scheme(self._h, self.SCHEMES[str(scheme).lstrip(":")]))
+
     def set_spmv_col_bytes(self, nbytes):
         """4: the SpMV reads the 32-bit columns; 2: the 16-bit offsets
         (when every block's fits).  Results do not depend on it."""

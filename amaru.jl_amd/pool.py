@@ -102,6 +102,14 @@ class Pool:
         _lib.check(_lib.lib().amr_pool_step_end(self._h, self.cap, ctypes.byref(n), self._out_m, self._out_i))
         return self._results(n.value)
 
+    def set_graphs(self, enable=None):
+        """Steps as one captured HIP graph launch (True, the default) or
+        kernel by kernel (False); returns (captures, replays) so far."""
+        c, r = ctypes.c_int64(0), ctypes.c_int64(0)
+        flag = -1 if enable is None else (1 if enable else 0)
+        _lib.check(_lib.lib().amr_pool_set_graphs(self._h, flag, ctypes.byref(c), ctypes.byref(r)))
+        return c.value, r.value
+
     def timing(self, enable):
         ms = ctypes.c_double(0.0)
         n = ctypes.c_int64(0)
@@ -312,6 +320,8 @@ def run_analyses(analyses, pool=None, device=0, check_every=32, npools=1, **kwar
     (in ``pool``: a Pool or a PoolSet; by default in ``npools`` pools of the
     library's own).  Returns the list of (success, message)."""
     args = solver_args(device=device, **kwargs)
+    if args["scheme"] != "FE":
+        raise NotImplementedError("pooled solves run scheme FE; solve() runs the others")
     own = pool is None
     if own:
         ps = PoolSet(npools, device, max_members=max(1, len(analyses)), check_every=check_every)

@@ -239,14 +239,18 @@ def batched_leg(am, args):
 
 
 def footing_leg(am, args):
-    """Elastoplastic footing: the full Newton-Raphson analysis (20 increments
-    of prescribed settlement, tangent assembly and return mapping every NR
-    iteration) on the plain FP64 path and on the mixed two-level path."""
+    """Elastoplastic footing: the full Newton-Raphson analysis (prescribed
+    settlement in 20 increments, auto-increment by default as the reference's
+    remedy for non-convergence, tangent assembly and return mapping every NR
+    iteration) on the plain FP64 path and on the mixed two-level path.  The
+    perfectly plastic soil may stop converging before the full settlement:
+    then the pseudo-time reached and the cut-backs are reported."""
     from amaru_amd import workloads as W
     out = {}
+    autoinc = not args.footing_fixed
     for name, prec, pc in (("plain_fp64", "fp64", "jacobi"), ("mixed64acc_two_level", "mixed64acc", "two_level")):
         ana, kw = W.footing(n=args.footing, settlement=args.footing_settlement)
-        if args.footing_autoinc:
+        if autoinc:
             kw = dict(kw, autoinc=True)
         ana.model.configure_dofs(ana.stages[0].bcs)
         ana.dev = am.DeviceModel(ana.model)
@@ -255,20 +259,24 @@ def footing_leg(am, args):
         t0 = time.perf_counter()
         ok, msg = am.solve(ana, precision=prec, precond=pc, pcg_rtol=args.rtol, **kw)
         dt = time.perf_counter() - t0
-        if not ok:
-            raise RuntimeError(f"footing {name}: {msg}")
-        out[name] = {"value": ana.stats["dof_solves"] / dt, "seconds": dt, "nr_iterations": ana.stats["nr_iterations"],
-                     "pcg_iterations": ana.stats["pcg_iterations"], "increments": ana.stats["increments"],
+        st = ana.stats
+        out[name] = {"value": st["dof_solves"] / dt, "seconds": dt, "completed": bool(ok),
+                     "T_reached": st["T"], "settlement_reached_m": st["T"] * args.footing_settlement,
+                     "increments": st["increments"], "cutbacks": st["cutbacks"],
+                     "nr_iterations": st["nr_iterations"], "pcg_iterations": st["pcg_iterations"],
                      "plastic_ips": int((ana.model.state[:, 13] != 0).sum())}
+        if not ok:
+            out[name]["message"] = msg.strip()
         out[name + "_U"] = ana.model.uvals.copy()
-        log(f"footing {name}: {dt:.2f}s, {out[name]['value']:.4g} DOF-solves/s, NR {ana.stats['nr_iterations']}, "
-            f"PCG {ana.stats['pcg_iterations']}")
+        log(f"footing {name}: {dt:.2f}s, {'done' if ok else 'stopped'} at T={st['T']:.4f}, "
+            f"{out[name]['value']:.4g} DOF-solves/s, increments {st['increments']}, cut-backs {st['cutbacks']}, "
+            f"NR {st['nr_iterations']}, PCG {st['pcg_iterations']}")
         ana.dev.close()
     err = rel_err(out.pop("mixed64acc_two_level_U"), out.pop("plain_fp64_U"))
     nu = ana.model.nu
     return {"workload": f"Drucker-Prager footing, soil 10x10x5 m, {2 * args.footing}x{2 * args.footing}x"
-                        f"{args.footing} Hex8, 2x2 m footing settled {args.footing_settlement} m, 20 increments"
-                        + (" (auto-increment)" if args.footing_autoinc else ""),
+                        f"{args.footing} Hex8, 2x2 m footing settled {args.footing_settlement} m in 20 increments"
+                        + (", auto-increment" if autoinc else ", fixed increments"),
             "unknowns": nu, **out, "rel_err_mixed_vs_plain": err,
             "speedup_vs_plain_fp64": out["mixed64acc_two_level"]["value"] / out["plain_fp64"]["value"]}
 
@@ -566,6 +574,8 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=2)
     ap.add_argument("--precision", default="mixed64acc", choices=["mixed", "mixed64acc", "fp64"])
     ap.add_argument("--precond", default="jacobi", choices=["jacobi", "block_jacobi"])
+    ap.add_argument("--ke", default="exact", choices=["mfma", "exact"],
+                    help="element stiffness of the headline step: FP64 matrix cores or the bit-exact kernel")
     ap.add_argument("--variant-steps", type=int, default=1)
     ap.add_argument("--rtol", type=float, default=1e-10)
     ap.add_argument("--cpu-n", type=int, default=12, help="HEX20 elements per edge of the CPU sample")
@@ -578,9 +588,8 @@ def main():
     ap.add_argument("--serve-pools", type=int, default=3, help="pools stepping concurrently in the server")
     ap.add_argument("--serve", type=int, default=48, help="requests in the served leg (0 = skip)")
     ap.add_argument("--footing", type=int, default=30, help="footing leg: n for the (2n x 2n x n) block (0 = skip)")
-    ap.add_argument("--footing-settlement", type=float, default=0.02,
-                    help="0.1 m (the nominal value) collapses the perfectly plastic soil before the end")
-    ap.add_argument("--footing-autoinc", action="store_true")
+    ap.add_argument("--footing-settlement", type=float, default=0.1, help="footing settlement in m")
+    ap.add_argument("--footing-fixed", action="store_true", help="footing leg with fixed increments (no autoinc)")
     ap.add_argument("--post", type=int, default=3, help="timed nodal-recovery calls on the block (0 = skip)")
     ap.add_argument("--mass", type=int, default=3, help="timed mount_M calls on the block (0 = skip)")
     ap.add_argument("--meshgen", type=int, default=3, help="timed device mesh generations of the block (0 = skip)")
@@ -608,6 +617,10 @@ def main():
     mesh, model, bcs, Uex, Fex = block_problem(am, args.n)
     dev = am.DeviceModel(model, device=0)
     info = dev.info()
+    # element tiles timed once (model set-up, untimed); the stiffness of the
+    # NR iterations on the FP64 matrix cores unless --ke exact
+    tiles = dev.tune_elem_tiles()
+    dev.set_ke_path(args.ke)
     dev.stage_begin(Uex, Fex)
     log(f"model: {info['nnodes']} nodes, {info['ne']} elements, ndofs {model.ndofs}, nu {model.nu}, "
         f"{info['nblk']} blocks; setup {time.perf_counter() - t0:.1f}s")
@@ -637,7 +650,10 @@ def main():
     log(f"mixed: {np.mean(lat):.3f}s/step, (PCG its, replacements, fallback) {its}, relres {r['relres']:.2e}, "
         f"err {err_mixed:.2e}")
 
+    # the plain FP64 path is the drop-in one: bit-exact element stiffness too
+    dev.set_ke_path("exact")
     lat_p, its_p, rp = run_steps(dev, plain, args.plain_steps)
+    dev.set_ke_path(args.ke)
     Upl, _ = dev.stage_get(last=True)
     err_plain = float(np.linalg.norm(Upl - Uref) / np.linalg.norm(Uref))
     log(f"plain fp64: {np.mean(lat_p):.3f}s/step, PCG its {its_p}, err {err_plain:.2e}")
@@ -689,7 +705,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"HEX20 block {args.n}^3, 27 IP, E=30GPa nu=0.2, base clamped, top pressure 1MPa",
                    "ndofs": model.ndofs, "unknowns": nu, "rtol": args.rtol, "precond": args.precond,
-                   "step": "mount_K+solve_system!+update_state!", "spmv_lanes": lanes},
+                   "step": "mount_K+solve_system!+update_state!", "spmv_lanes": lanes,
+                   "element_stiffness": args.ke, "element_tiles": {"ach": tiles[0], "epw": tiles[1]}},
         "variants": extra,
         "latency_s": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
         "plain_fp64": {"value": plain_value, "ms_per_step": 1e3 * float(np.mean(lat_p)),

@@ -206,6 +206,18 @@ int amr_increment_begin(amr_model *m, double dT, double acr, int add_residual);
  * constitutive update is reported in info->status (and the return code). */
 int amr_nr_iteration(amr_model *m, int it, const amr_solve_opts *opts, amr_nr_info *info);
 
+/* Predictor-corrector scheme of amr_nr_iteration, mech_stage_solver!'s
+ * `scheme` keyword (src/mech/mech-solver.jl:280-288, 325-349).  :FE (the
+ * default): one mount_K, solve and update per iteration.  :ME, :BE,
+ * :Ralston, with (p1, q11, a1, a2) = (1, 1, 1/2, 1/2), (1, 1, 0, 1),
+ * (2/3, 2/3, 1/4, 3/4): solve_system!(K, p1 ΔUi, q11 R) and update_state!
+ * at ΔUa + that solution (the predictor), K2 = mount_K at the predicted
+ * state, K = a1 K + a2 K2, solve_system!(K, ΔUi, R) and update_state! at
+ * ΔUa + ΔUi (the corrector).  info->pcg_iterations counts both solves.
+ * Pools (amr_pool_add) run :FE only. */
+enum { AMR_SCHEME_FE = 0, AMR_SCHEME_ME = 1, AMR_SCHEME_BE = 2, AMR_SCHEME_RALSTON = 3 };
+int amr_set_scheme(amr_model *m, int scheme);
+
 /* Converged increment: U += ΔUa; F += ΔFin; Rc = (1-acr) Rc + R; state
  * commit (src/mech/mech-solver.jl:386-391). */
 int amr_increment_commit(amr_model *m);
@@ -228,6 +240,30 @@ int amr_tune_spmv_lanes(amr_model *m);
 /* out[0..2]: lanes chosen for the FP64, FP32 and FP32-storage/FP64-accumulation
  * SpMVs (0 = not chosen yet). */
 int amr_get_spmv_lanes(amr_model *m, int *out);
+
+/* Element tiles of the stiffness and update kernels: ach = row nodes per
+ * lane of the element stiffness (HEX8 1/2/4, HEX20 7/10/20, TET10 2/5/10),
+ * epw = elements per update workgroup (HEX8 32/16/8, HEX20 27 IP 9/4/2,
+ * HEX20 8 IP 16/8/32, TET10 32/16/64).  Every tiling gives bit-identical
+ * K, states and forces.  amr_set_elem_tiles: 0 keeps the tuned (or default,
+ * the first) choice.  amr_tune_elem_tiles times the three tilings of each
+ * kernel on the model (stiffness from the current states; update from the
+ * committed states with a zero increment, which it leaves as the current
+ * states: call it before an analysis or between increments), keeps the
+ * fastest for the model and, for every later model of the same element
+ * shape in the process, and returns them. */
+int amr_set_elem_tiles(amr_model *m, int ach, int epw);
+/* Element stiffness of the Newton-Raphson iterations (amr_nr_prepare,
+ * amr_nr_iteration; elem_stiffness, src/mech/elem/mech-solid.jl:124-166):
+ * AMR_KE_EXACT (the default) follows the oracle's operation order bit for
+ * bit; AMR_KE_MFMA (HEX20 only) forms Ke = sum_q Bq' (coef D) Bq on the
+ * FP64 matrix cores, equal to the exact one to rounding (the summation
+ * order differs).  amr_assemble (mount_K with its pattern) always uses the
+ * exact path. */
+enum { AMR_KE_EXACT = 0, AMR_KE_MFMA = 1 };
+int amr_set_ke_path(amr_model *m, int path);
+int amr_get_elem_tiles(amr_model *m, int *ach, int *epw);
+int amr_tune_elem_tiles(amr_model *m, int *ach, int *epw);
 
 /* Kernel timing of the PCG SpMV with HIP events on the model's stream:
  * enable = 1 starts (and clears) the counters, 0 stops.  Returns the
@@ -307,6 +343,15 @@ int amr_pool_size(amr_pool *p, int *members);
 
 /* SpMV kernel timing over the pool's steps, as amr_timing. */
 int amr_pool_timing(amr_pool *p, int enable, double *spmv_ms, int64_t *spmv_launches);
+
+/* A step's check_every iterations of every PCG phase run as one HIP graph
+ * launch: captured once per launch shape and replayed while members join
+ * and leave (the work lists are padded to size classes and re-uploaded
+ * each step; padding entries do nothing).  enable = 1 (the default) or 0
+ * switches it (-1 leaves it); *captures and *replays count the graphs
+ * captured and replayed so far (either may be NULL).  Results are the same
+ * bits either way; with amr_pool_timing on, steps launch kernel by kernel. */
+int amr_pool_set_graphs(amr_pool *p, int enable, int64_t *captures, int64_t *replays);
 
 /* Aggregate size of the two-level preconditioner: about target_nodes nodes
  * per aggregate, fewer aggregates than max_aggregates (defaults 256, 170).

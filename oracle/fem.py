@@ -593,11 +593,47 @@ def solve_system(K, U, F, nu):
 
 # ------------------------------------------------------------ stage solver
 
+# (p1, q11, a1, a2) of the predictor-corrector schemes (src/mech/mech-solver.jl:280-288)
+SCHEMES = {"FE": (1.0, 1.0, None, None), "ME": (1.0, 1.0, 0.5, 0.5), "BE": (1.0, 1.0, 0.0, 1.0),
+           "Ralston": (2 / 3, 2 / 3, 1 / 4, 3 / 4)}
+
+
+def nr_step(model, state_bk, state, dUa, dUi, R, nu, scheme="FE"):
+    """The solve and state update of one Newton-Raphson iteration of
+    mech_stage_solver! (src/mech/mech-solver.jl:325-349): the predictor
+    solve_system!(K, p1 ΔUi, q11 R) with K = mount_K at ``state`` and
+    update_state!(ΔUa + ΔUitr) from ``state_bk``; for :ME, :BE and :Ralston
+    the corrector with K = a1 K + a2 K2 (K2 = mount_K at the predicted
+    state), solve_system!(K, ΔUi, R) and update_state!(ΔUa + ΔUi).
+    ``dUi`` holds the iteration's essential values.  Returns (ΔUi, ΔFin,
+    state, message or None)."""
+    p1, q11, a1, a2 = SCHEMES[scheme]
+    K = model.mount_K(state)
+    dUitr = p1 * dUi
+    Rtr = q11 * R
+    msg = solve_system(K, dUitr, Rtr, nu)
+    if msg is not None:
+        return dUi, None, state, msg
+    dFin, state, msg = model.update_state(dUa + dUitr, state_bk)
+    if msg is not None or scheme == "FE":
+        return dUitr, dFin, state, msg
+    K2 = model.mount_K(state)
+    K = (a1 * K + a2 * K2).tocsr()
+    dUi = dUi.copy()
+    msg = solve_system(K, dUi, R.copy(), nu)
+    if msg is not None:
+        return dUi, None, state, msg
+    dFin, state, msg = model.update_state(dUa + dUi, state_bk)
+    return dUi, dFin, state, msg
+
+
 def solve_stage(model, bcs, nincs=1, nouts=0, tol=0.01, rtol=0.01, rspan=0.01, dT0=0.01, dTmin=1e-7,
-                dTmax=0.1, maxits=5, autoinc=False, on_increment=None, log=None):
-    """mech_stage_solver! (src/mech/mech-solver.jl:186-492), scheme :FE,
+                dTmax=0.1, maxits=5, autoinc=False, on_increment=None, log=None, scheme="FE"):
+    """mech_stage_solver! (src/mech/mech-solver.jl:186-492) with the
+    predictor-corrector schemes :FE, :ME, :BE, :Ralston (:319-349),
     including the stage_iterator! nincs/nouts adjustment (src/solver.jl:128-142).
     Returns (success, message)."""
+    p1, q11, a1, a2 = SCHEMES[scheme]
     if nouts > 0 and not autoinc:
         if nouts > nincs:
             nincs = nouts
@@ -652,19 +688,10 @@ def solve_stage(model, bcs, nincs=1, nouts=0, tol=0.01, rtol=0.01, rspan=0.01, d
             if it > 1:
                 dUi[:] = 0.0
             lastres = res
-            K = model.mount_K(state)
-            dUitr = 1.0 * dUi
-            Rtr = 1.0 * R
-            sysmsg = solve_system(K, dUitr, Rtr, nu)
+            dUi, dFin, state, sysmsg = nr_step(model, state_bk, state, dUa, dUi, R, nu, scheme)
             if sysmsg is not None:
                 syserror = True
                 break
-            dUt = dUa + dUitr
-            dFin, state, sysmsg = model.update_state(dUt, state_bk)
-            if sysmsg is not None:
-                syserror = True
-                break
-            dUi = dUitr
             dUa += dUi
             R = dFex - dFin
             R[nu:] = 0.0

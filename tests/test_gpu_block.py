@@ -92,3 +92,4 @@ def test_column_offsets_do_not_change_results(amaru):
     with pytest.raises(amaru.AmaruError):
         dev.set_spmv_col_bytes(3)
     dev.close()
+

@@ -45,3 +45,37 @@ def test_footing_matches_oracle(amaru, precision, precond):
         assert np.linalg.norm(a - b) <= 1e-7 * np.linalg.norm(b)
     assert np.abs(m.state - om.state).max() <= 1e-6 * np.abs(om.state).max()
     ana.dev.close()
+
+
+@pytest.mark.parametrize("precision,precond", [("fp64", "jacobi"), ("mixed64acc", "two_level")])
+def test_footing_full_settlement_autoinc_matches_oracle(amaru, precision, precond):
+    """The configured 0.1 m settlement with auto-increment
+    (src/mech/mech-solver.jl:461-480).  The perfectly plastic soil stops
+    converging before the end: the device takes the oracle's increments,
+    cut-backs and Newton-Raphson iterations to the same failure point, with
+    the same committed displacements and states there."""
+    from amaru_amd import workloads as W
+    ana, kw = W.footing(n=3, settlement=0.1)
+    ok, msg = amaru.solve(ana, precision=precision, precond=precond, pcg_rtol=1e-12, autoinc=True, **kw)
+    m = fem.Model(ana.model.mesh.coords, ana.model.mesh.conn, O8,
+                  [("DruckerPrager", (50e6, 0.3, 0.23094, 24000.0, 0.0))])
+    bcs = [("z==0", fem.SurfaceBC(ux=0, uy=0, uz=0)), ("x==0 || x==10", fem.SurfaceBC(ux=0)),
+           ("y==0 || y==10", fem.SurfaceBC(uy=0)),
+           ("z==5 && x>=4 && x<=6 && y>=4 && y<=6", fem.NodeBC(uz=-0.1))]
+    olog, Ts = [], []
+    ook, omsg = fem.solve_stage(m, bcs, nincs=20, maxits=5, autoinc=True, log=olog,
+                                on_increment=lambda mm, T: Ts.append(T))
+    assert (ok, ook) == (False, False), (msg, omsg)
+    assert [(inc, it) for _, inc, it, _ in ana.log] == [(inc, it) for inc, it, _ in olog]
+    attempts = sum(1 for _, it, _ in olog if it == 1)
+    assert ana.stats["increments"] == len(Ts) and ana.stats["cutbacks"] == attempts - len(Ts)
+    # the step growth after a converged increment, 1 + tanh(log10(tol/res1)),
+    # follows the first residual, which the device and the oracle agree on to
+    # about 1e-4 (different linear solvers), so T agrees to that
+    assert ana.stats["T"] == pytest.approx(Ts[-1], rel=1e-4) and 0.3 < Ts[-1] < 0.9
+    mm = ana.model
+    assert np.array_equal(mm.state[:, 13] != 0, m.state[:, 13] != 0)
+    # the settlement reached scales with T, so the fields agree to T's 1e-4
+    for a, b in ((mm.uvals, m.uvals), (mm.fvals, m.fvals)):
+        assert np.linalg.norm(a - b) <= 1e-4 * np.linalg.norm(b)
+    ana.dev.close()

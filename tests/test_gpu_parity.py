@@ -185,3 +185,37 @@ def test_spmv_lanes_do_not_change_results(amaru):
             assert it == outs[0][2]
             assert np.array_equal(Ug, outs[0][0]) and np.array_equal(Fg, outs[0][1])
     dev.close()
+
+
+TILES = {("HEX8", 0): ((1, 2, 4), (32, 16, 8)), ("HEX20", 27): ((7, 10, 20), (9, 4, 2)),
+         ("HEX20", 8): ((7, 10, 20), (16, 8, 32)), ("TET10", 0): ((2, 5, 10), (32, 16, 64))}
+
+
+@pytest.mark.parametrize("shape,nips", CASES)
+def test_element_tiles_do_not_change_results(amaru, shape, nips):
+    """Every stiffness tiling (row nodes per lane) and update tiling
+    (elements per workgroup) gives the oracle's bits; the tuner picks one of
+    them and later models of the shape use it."""
+    mesh, model, om = pair(amaru, shape, "VM", nips, n=(5, 3, 2), seed=2)
+    model.configure_dofs([("x==0", amaru.SurfaceBC(ux=0, uy=0, uz=0))])
+    om.configure_dofs([("x==0", fem.SurfaceBC(ux=0, uy=0, uz=0))])
+    dU = np.random.default_rng(9).normal(size=model.ndofs) * 4e-3
+    dFo, sto, msg = om.update_state(dU, om.state)
+    assert msg is None
+    Ko = om.elem_stiffness(sto)
+    dev = amaru.DeviceModel(model)
+    achs, epws = TILES[(shape, nips)]
+    for ach, epw in zip(achs, epws):
+        dev.set_elem_tiles(ach, epw)
+        assert dev.elem_tiles() == (ach, epw)
+        dev.restore()
+        assert np.array_equal(dev.update_state(dU), dFo)
+        assert np.array_equal(dev.state_get(), sto)
+        dev.assemble()
+        assert np.array_equal(dev.Ke(), Ko)
+    ach, epw = dev.tune_elem_tiles()
+    assert ach in achs and epw in epws and dev.elem_tiles() == (ach, epw)
+    dev.close()
+    dev2 = amaru.DeviceModel(model)
+    assert dev2.elem_tiles() == (ach, epw)     # the shape's tuned choice
+    dev2.close()

@@ -52,8 +52,11 @@ def results(ana):
     return m.uvals.copy(), m.fvals.copy(), m.state.copy(), dict(ana.stats)
 
 
-@pytest.mark.parametrize("precision", ["fp64", "mixed64acc", "mixed"])
-def test_pool_bit_identical_to_alone(amaru, precision):
+@pytest.mark.parametrize("precision,graphs", [("fp64", True), ("mixed64acc", True), ("mixed", True),
+                                              ("fp64", False), ("mixed64acc", False)])
+def test_pool_bit_identical_to_alone(amaru, precision, graphs):
+    """Also: a step's iterations launched as one captured graph (replayed
+    while members join and leave) or kernel by kernel give the same bits."""
     from amaru_amd import pool as P
     alone = []
     for ana, kw in small_set(amaru):
@@ -64,6 +67,7 @@ def test_pool_bit_identical_to_alone(amaru, precision):
     # pooled: one runner per distinct kwargs set, all models in one pool
     anas = small_set(amaru)
     pool = P.Pool(max_members=8)
+    pool.set_graphs(graphs)
     try:
         jobs, res = [], []
         for ana, kw in anas:
@@ -83,6 +87,11 @@ def test_pool_bit_identical_to_alone(amaru, precision):
                     live[jobs[i].ana.dev.handle.value] = i
         for j in jobs:
             assert j.result[0], j.result[1]
+        captures, replays = pool.set_graphs()
+        if graphs:
+            assert captures >= 1 and replays > captures
+        else:
+            assert captures == replays == 0
         for (ana, _), ref in zip(anas, alone):
             got = results(ana)
             assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1])

@@ -27,6 +27,9 @@ class FakeDev:
     def opts(**kw):
         return dict(kw)
 
+    def set_scheme(self, scheme):
+        assert scheme == "FE"
+
     def set_dofs(self):
         self.calls.append(("set_dofs",))
 
