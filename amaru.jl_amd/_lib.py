@@ -86,6 +86,7 @@ SIGNATURES = {
     "amr_pool_set_graphs": (I32, [P, I32, P, P]),
     "amr_model_device_bytes": (I32, [P, P]),
     "amr_mem_info": (I32, [I32, P, P]),
+    "amr_mem_trim": (I32, [I32]),
     "amr_set_coarse_refresh": (I32, [P, I32]),
     "amr_set_coarse": (I32, [P, I32, I32]),
     "amr_get_coarse": (I32, [P, P]),

@@ -15,7 +15,7 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libamaru_hip.so")
 BUILD = os.path.join(HERE, "_build")
 SOURCES = ["api.hip", "kernels_elem.hip", "kernels_solve.hip", "coarse.hip", "recovery.hip",
-           "mesh_gen.hip", "loads.hip", "model_build.cpp"]
+           "mesh_gen.hip", "loads.hip", "pattern.hip", "model_build.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

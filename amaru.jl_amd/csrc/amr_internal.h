@@ -67,6 +67,20 @@ struct AmrPattern {
 
 int amr_build_pattern(int nn, int64_t nnodes, int64_t ne, const int32_t *conn, AmrPattern &P);
 
+// Device-built pattern (pattern.hip): the same arrays as amr_build_pattern,
+// made on the device from the model's conn.  Returns AMR_PATTERN_ON_HOST when
+// a node's contributions exceed what one workgroup sorts (the host builder is
+// used then).  The host copies in m->pat (rowptr, col, diag) are downloaded
+// on first use by amr_pattern_host.
+constexpr int AMR_PATTERN_ON_HOST = 1;
+struct amr_model;
+int amr_build_pattern_device(amr_model *m);
+int amr_pattern_host(amr_model *m);
+
+// Device memory of a model, counted in m->dev_bytes; amr_release gives it back.
+int amr_dalloc(amr_model *m, void **p, size_t bytes);
+void amr_release(amr_model *m, void *p);
+
 // Host-built coarse space of the two-level preconditioner (model_build.cpp).
 struct AmrCoarse {
     int32_t nagg = 0;

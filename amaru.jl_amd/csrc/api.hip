@@ -1,9 +1,12 @@
 // C ABI of libamaru_hip.so (declared in include/amaru_hip.h).
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <mutex>
+#include <map>
+#include <unordered_map>
 
 #include "amr_internal.h"
 
@@ -164,20 +167,127 @@ static int nn_of(int etype)
 
 // ------------------------------------------------------------- helpers
 
+// Device memory of models.  A released block is kept in a per-device cache
+// and handed to the next request of about its size (best fit, at most 25%
+// larger), so that creating and destroying models in a serving loop does not
+// call hipMalloc or hipFree (hipFree waits for the whole device).  Blocks
+// are released only after the streams that used them are synchronised
+// (free_model, amr_pool_detach_model).  The cache holds at most
+// AMR_ARENA_BYTES (environment, default 32 GiB) per device; beyond that, and
+// when hipMalloc runs out of memory, cached blocks are freed.
+namespace {
+struct Arena {
+    std::multimap<size_t, void *> blocks;  // size -> cached block
+    size_t bytes = 0;
+};
+std::mutex g_dmem_mu;
+std::unordered_map<void *, std::pair<size_t, int>> g_dmem_live;  // block -> (size, device)
+std::unordered_map<int, Arena> g_arena;
+
+size_t arena_cap()
+{
+    static const size_t cap = [] {
+        const char *e = getenv("AMR_ARENA_BYTES");
+        return e ? (size_t)strtoull(e, nullptr, 10) : (size_t)32 << 30;
+    }();
+    return cap;
+}
+
+size_t arena_round(size_t b)
+{
+    if (b <= 65536) {
+        size_t r = 256;
+        while (r < b) r <<= 1;
+        return r;
+    }
+    return (b + 65535) & ~(size_t)65535;
+}
+
+// frees the cached blocks of a device (lock held, device current)
+void arena_trim_locked(int device)
+{
+    Arena &A = g_arena[device];
+    for (auto &kv : A.blocks) (void)hipFree(kv.second);
+    A.blocks.clear();
+    A.bytes = 0;
+}
+}  // namespace
+
+int amr_dalloc(amr_model *m, void **p, size_t bytes)
+{
+    const size_t need = arena_round(bytes ? bytes : 1);
+    const int dev = m->device;
+    {
+        std::lock_guard<std::mutex> lk(g_dmem_mu);
+        Arena &A = g_arena[dev];
+        auto it = A.blocks.lower_bound(need);
+        if (it != A.blocks.end() && it->first <= need + need / 4) {
+            const size_t sz = it->first;
+            *p = it->second;
+            A.blocks.erase(it);
+            A.bytes -= sz;
+            g_dmem_live[*p] = {sz, dev};
+            m->dev_bytes += (int64_t)sz;
+            return AMR_OK;
+        }
+    }
+    hipError_t e = hipMalloc(p, need);
+    if (e == hipErrorOutOfMemory) {
+        (void)hipGetLastError();
+        {
+            std::lock_guard<std::mutex> lk(g_dmem_mu);
+            arena_trim_locked(dev);
+        }
+        e = hipMalloc(p, need);
+    }
+    if (e != hipSuccess) {
+        *p = nullptr;
+        amr_set_error(std::string("hipMalloc of ") + std::to_string(need) + " bytes failed: " + hipGetErrorString(e));
+        return e == hipErrorOutOfMemory ? AMR_ERR_NOMEM : AMR_ERR_HIP;
+    }
+    {
+        std::lock_guard<std::mutex> lk(g_dmem_mu);
+        g_dmem_live[*p] = {need, dev};
+    }
+    m->dev_bytes += (int64_t)need;
+    return AMR_OK;
+}
+
+void amr_release(amr_model *m, void *p)
+{
+    if (!p) return;
+    std::lock_guard<std::mutex> lk(g_dmem_mu);
+    auto it = g_dmem_live.find(p);
+    if (it == g_dmem_live.end()) {
+        (void)hipFree(p);
+        return;
+    }
+    const size_t sz = it->second.first;
+    const int dev = it->second.second;
+    g_dmem_live.erase(it);
+    if (m) m->dev_bytes -= (int64_t)sz;
+    Arena &A = g_arena[dev];
+    if (A.bytes + sz <= arena_cap()) {
+        A.blocks.insert({sz, p});
+        A.bytes += sz;
+    } else {
+        (void)hipFree(p);
+    }
+}
+
+extern "C" int amr_mem_trim(int device)
+{
+    AMR_HIP(hipSetDevice(device));
+    std::lock_guard<std::mutex> lk(g_dmem_mu);
+    arena_trim_locked(device);
+    return AMR_OK;
+}
+
 // Device allocation charged to the model's byte count.
 template <typename T>
 static int dalloc(amr_model *m, T **p, size_t n)
 {
-    if (n == 0) n = 1;
-    hipError_t e = hipMalloc((void **)p, n * sizeof(T));
-    if (e != hipSuccess) {
-        *p = nullptr;
-        amr_set_error(std::string("hipMalloc of ") + std::to_string(n * sizeof(T)) + " bytes failed: " +
-                      hipGetErrorString(e));
-        return e == hipErrorOutOfMemory ? AMR_ERR_NOMEM : AMR_ERR_HIP;
-    }
-    m->dev_bytes += (int64_t)(n * sizeof(T));
-    return AMR_OK;
+    return amr_dalloc(m, (void **)p, (n ? n : 1) * sizeof(T));
 }
 
 template <typename T>
@@ -216,14 +326,10 @@ static void free_model(amr_model *m)
                     m->rowptr, m->col, m->diag, m->cptr,  m->csrc, m->fptr, m->fsrc,   m->Kv,     m->Kv32,
                     m->Kflag, m->dinv, m->dinv32, m->mask, m->eqd,  m->dF,   m->estat,  m->err,    m->part,
                     m->scal, m->binv, m->binv32, m->Me, m->Mv, m->Mflag, m->Nd, m->rho_d, m->mvx, m->mvy, m->Ksave, m->col16, m->Kpred};
-    for (void *p : ptrs)
-        if (p) (void)hipFree(p);
-    for (double *p : m->v)
-        if (p) (void)hipFree(p);
-    for (float *p : m->f)
-        if (p) (void)hipFree(p);
-    for (double *p : m->sv)
-        if (p) (void)hipFree(p);
+    for (void *p : ptrs) amr_release(m, p);
+    for (double *p : m->v) amr_release(m, p);
+    for (float *p : m->f) amr_release(m, p);
+    for (double *p : m->sv) amr_release(m, p);
     for (hipEvent_t e : m->ev)
         if (e) (void)hipEventDestroy(e);
     if (m->fin_h) (void)hipHostFree(m->fin_h);
@@ -234,10 +340,24 @@ static void free_model(amr_model *m)
     delete m;
 }
 
+int amr_pattern_host(amr_model *m)
+{
+    if ((int64_t)m->pat.rowptr.size() == m->nnodes + 1) return AMR_OK;
+    m->pat.rowptr.resize(m->nnodes + 1);
+    m->pat.col.resize(m->pat.nblk);
+    m->pat.diag.resize(m->nnodes);
+    TRY(d2h(m->pat.rowptr.data(), m->rowptr, m->pat.rowptr.size(), m->stream));
+    TRY(d2h(m->pat.col.data(), m->col, m->pat.col.size(), m->stream));
+    TRY(d2h(m->pat.diag.data(), m->diag, m->pat.diag.size(), m->stream));
+    AMR_HIP(hipStreamSynchronize(m->stream));
+    return AMR_OK;
+}
+
 // 16-bit column offsets (column - row) for the PCG SpMV, when every block's
 // offset fits; otherwise the SpMV reads the 32-bit columns.
 static int build_col16(amr_model *m, hipStream_t s)
 {
+    TRY(amr_pattern_host(m));
     std::vector<int16_t> c16(m->pat.col.size());
     for (int64_t r = 0; r < m->nnodes; r++)
         for (int32_t k = m->pat.rowptr[r]; k < m->pat.rowptr[r + 1]; k++) {
@@ -338,7 +458,6 @@ extern "C" int amr_model_create(amr_model **out, int device, int etype, int nips
     m->nip_total = ne * nq;
     m->nmats = nmats;
     m->all_elastic = all_el;
-    TRY(amr_build_pattern(nn, nnodes, ne, conn, m->pat));
 
     AMR_HIP(hipSetDevice(device));
     AMR_HIP(hipDeviceGetAttribute(&m->ncu, hipDeviceAttributeMultiprocessorCount, device));
@@ -355,12 +474,36 @@ extern "C" int amr_model_create(amr_model **out, int device, int etype, int nips
     TRY(dalloc(m.get(), &m->st_com, (size_t)AMR_NSTATE * m->nip_total));
     TRY(dalloc(m.get(), &m->Ke, (size_t)ne * nn * nn * 9));
     TRY(dalloc(m.get(), &m->rowptr, (size_t)nnodes + 1));
-    TRY(dalloc(m.get(), &m->col, (size_t)m->pat.nblk));
     TRY(dalloc(m.get(), &m->diag, (size_t)nnodes));
-    TRY(dalloc(m.get(), &m->cptr, (size_t)m->pat.nblk + 1));
-    TRY(dalloc(m.get(), &m->csrc, (size_t)m->pat.ncontrib));
     TRY(dalloc(m.get(), &m->fptr, (size_t)nnodes + 1));
-    TRY(dalloc(m.get(), &m->fsrc, m->pat.fsrc.size()));
+    TRY(dalloc(m.get(), &m->fsrc, (size_t)ne * nn));
+    TRY(dalloc(m.get(), &m->part, 24576));
+    hipStream_t s = m->stream;
+    TRY(h2d(m->conn, conn, (size_t)ne * nn, s));
+    // the sparsity pattern and gather maps, on the device (host builder for
+    // nodes with more contributions than one workgroup sorts)
+    int prc = amr_build_pattern_device(m.get());
+    if (prc == AMR_PATTERN_ON_HOST) {
+        TRY(amr_build_pattern(nn, nnodes, ne, conn, m->pat));
+        TRY(dalloc(m.get(), &m->col, (size_t)m->pat.nblk));
+        TRY(dalloc(m.get(), &m->cptr, (size_t)m->pat.nblk + 1));
+        TRY(dalloc(m.get(), &m->csrc, (size_t)m->pat.ncontrib));
+        TRY(h2d(m->rowptr, m->pat.rowptr.data(), m->pat.rowptr.size(), s));
+        TRY(h2d(m->col, m->pat.col.data(), m->pat.col.size(), s));
+        TRY(h2d(m->diag, m->pat.diag.data(), m->pat.diag.size(), s));
+        TRY(h2d(m->cptr, m->pat.cptr.data(), m->pat.cptr.size(), s));
+        TRY(h2d(m->csrc, m->pat.csrc.data(), m->pat.csrc.size(), s));
+        TRY(h2d(m->fptr, m->pat.fptr.data(), m->pat.fptr.size(), s));
+        TRY(h2d(m->fsrc, m->pat.fsrc.data(), m->pat.fsrc.size(), s));
+        TRY(build_col16(m.get(), s));
+        AMR_HIP(hipStreamSynchronize(s));   // the host vectors below are released
+        std::vector<int32_t>().swap(m->pat.csrc);
+        std::vector<int64_t>().swap(m->pat.cptr);
+        std::vector<int32_t>().swap(m->pat.fsrc);
+        std::vector<int32_t>().swap(m->pat.fptr);
+    } else if (prc) {
+        return prc;
+    }
     TRY(dalloc(m.get(), &m->Kv, (size_t)kv_size(m->pat.nblk)));
     TRY(dalloc(m.get(), &m->Kv32, (size_t)kv_size(m->pat.nblk)));
     TRY(dalloc(m.get(), &m->Kflag, (size_t)kv_size(m->pat.nblk)));
@@ -373,7 +516,6 @@ extern "C" int amr_model_create(amr_model **out, int device, int etype, int nips
     TRY(dalloc(m.get(), &m->dF, (size_t)ne * nn * 3));
     TRY(dalloc(m.get(), &m->estat, (size_t)ne));
     TRY(dalloc(m.get(), &m->err, 4));
-    TRY(dalloc(m.get(), &m->part, 24576));
     TRY(dalloc(m.get(), &m->scal, 64));
     for (double *&p : m->v) TRY(dalloc(m.get(), &p, n3));
     for (float *&p : m->f) TRY(dalloc(m.get(), &p, n3));
@@ -388,31 +530,17 @@ extern "C" int amr_model_create(amr_model **out, int device, int etype, int nips
         if (e == 0) m->elem_kind = k;
         else if (k != m->elem_kind) { m->elem_kind = -2; break; }
     }
-    hipStream_t s = m->stream;
     TRY(h2d(m->X, coords, n3, s));
     m->Xh.assign(coords, coords + n3);
-    TRY(h2d(m->conn, conn, (size_t)ne * nn, s));
     TRY(h2d(m->emat, emat, (size_t)ne, s));
     TRY(h2d(m->mats, mats.data(), mats.size(), s));
     TRY(h2d(m->dNdR, dNdR.data(), dNdR.size(), s));
     TRY(h2d(m->wip, w, (size_t)nq, s));
-    TRY(h2d(m->rowptr, m->pat.rowptr.data(), m->pat.rowptr.size(), s));
-    TRY(h2d(m->col, m->pat.col.data(), m->pat.col.size(), s));
-    TRY(build_col16(m.get(), s));
-    TRY(h2d(m->diag, m->pat.diag.data(), m->pat.diag.size(), s));
-    TRY(h2d(m->cptr, m->pat.cptr.data(), m->pat.cptr.size(), s));
-    TRY(h2d(m->csrc, m->pat.csrc.data(), m->pat.csrc.size(), s));
-    TRY(h2d(m->fptr, m->pat.fptr.data(), m->pat.fptr.size(), s));
-    TRY(h2d(m->fsrc, m->pat.fsrc.data(), m->pat.fsrc.size(), s));
     AMR_HIP(hipMemsetAsync(m->st_cur, 0, sizeof(double) * AMR_NSTATE * m->nip_total, s));
     AMR_HIP(hipMemsetAsync(m->st_com, 0, sizeof(double) * AMR_NSTATE * m->nip_total, s));
     AMR_HIP(hipMemsetAsync(m->scal, 0, sizeof(double) * 64, s));
     AMR_HIP(hipMemsetAsync(m->err, 0, 4 * sizeof(int32_t), s));
     TRY(upload_dofs(m.get(), eq_id, ndofs, nu));
-    // host copies of the big maps are no longer needed
-    std::vector<int32_t>().swap(m->pat.csrc);
-    std::vector<int64_t>().swap(m->pat.cptr);
-    std::vector<int32_t>().swap(m->pat.fsrc);
     *out = m.release();
     return AMR_OK;
 }
@@ -573,6 +701,7 @@ static int bsr_to_csc(amr_model *m, const double *dv, const uint8_t *dflag, int6
                       int32_t *rowval, double *nzval)
 {
     AMR_HIP(hipSetDevice(m->device));
+    TRY(amr_pattern_host(m));
     const int64_t nb = m->pat.nblk;
     std::vector<double> val((size_t)kv_size(nb));
     std::vector<uint8_t> fl((size_t)kv_size(nb));
@@ -690,9 +819,8 @@ extern "C" int amr_set_spmv_col_bytes(amr_model *m, int bytes)
     AMR_HIP(hipSetDevice(m->device));
     if (bytes == 4 && m->col16) {
         AMR_HIP(hipStreamSynchronize(m->stream));
-        AMR_HIP(hipFree(m->col16));
+        amr_release(m, m->col16);
         m->col16 = nullptr;
-        m->dev_bytes -= (int64_t)(m->pat.col.size() * sizeof(int16_t));
     } else if (bytes == 2 && !m->col16) {
         TRY(build_col16(m, m->stream));
         if (!m->col16) {
@@ -1363,8 +1491,8 @@ extern "C" int amr_set_ke_path(amr_model *m, int path)
         amr_set_error("amr_set_ke_path: path must be AMR_KE_EXACT or AMR_KE_MFMA");
         return AMR_ERR_ARG;
     }
-    if (path == AMR_KE_MFMA && m->etype != AMR_HEX20) {
-        amr_set_error("amr_set_ke_path: the matrix-core path is built for HEX20 elements");
+    if (path == AMR_KE_MFMA && m->etype != AMR_HEX20 && !m->all_elastic) {
+        amr_set_error("amr_set_ke_path: the matrix-core path takes HEX20 elements or LinearElastic models");
         return AMR_ERR_ARG;
     }
     m->ke_path = path;
@@ -1412,6 +1540,12 @@ extern "C" int amr_mem_info(int device, int64_t *free_bytes, int64_t *total_byte
     AMR_HIP(hipSetDevice(device));
     size_t f = 0, t = 0;
     AMR_HIP(hipMemGetInfo(&f, &t));
+    {
+        // blocks the library holds for reuse are free for models
+        std::lock_guard<std::mutex> lk(g_dmem_mu);
+        auto it = g_arena.find(device);
+        if (it != g_arena.end()) f += it->second.bytes;
+    }
     if (free_bytes) *free_bytes = (int64_t)f;
     if (total_bytes) *total_bytes = (int64_t)t;
     return AMR_OK;

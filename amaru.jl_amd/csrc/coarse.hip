@@ -132,47 +132,43 @@ __global__ void symmetrize_kernel(int32_t nc, double *__restrict__ A, const int3
 
 // ---- block Gauss-Jordan inversion (in place, row-major nc x nc) ----------
 // Step k, pivot block K = [k0, k0 + b):
-//   P    = A_KK^-1                              (gj_pivot, one workgroup)
+//   P    = A_KK^-1                              (gj_pivot, one wave)
 //   R    = P A_Kj for j outside K; C = A_iK     (gj_panel, old column panel)
 //   A_ij -= C_i R_j,  A_Kj = R_j,  A_iK = -C_i P,  A_KK = P      (gj_update)
 constexpr int GJB = 64;
 
-__global__ void __launch_bounds__(256) gj_pivot_kernel(int32_t nc, int32_t k0, int32_t b,
-                                                       const double *__restrict__ A, double *__restrict__ P,
-                                                       int32_t *__restrict__ info)
+__global__ void __launch_bounds__(64) gj_pivot_kernel(int32_t nc, int32_t k0, int32_t b,
+                                                      const double *__restrict__ A, double *__restrict__ P,
+                                                      int32_t *__restrict__ info)
 {
-    // thread: row i, columns j0 .. j0 + 15 of the pivot block, held in
-    // registers; row p and column p are published through LDS each step
-    __shared__ double rowp[GJB], colp[GJB];
-    const int i = threadIdx.x >> 2, j0 = (threadIdx.x & 3) * 16;
-    double v[16];
-    for (int q = 0; q < 16; q++)
-        v[q] = (i < b && j0 + q < b) ? A[(int64_t)(k0 + i) * nc + k0 + j0 + q] : 0.0;
+    // one wave: lane j holds column j of the pivot block in registers; the
+    // pivot row is each lane's own entry and the pivot column is read from
+    // lane p (readlane), so the 64 steps need no barrier
+    const int j = threadIdx.x;
+    double c[GJB];
+#pragma unroll
+    for (int i = 0; i < GJB; i++) c[i] = (i < b && j < b) ? A[(int64_t)(k0 + i) * nc + k0 + j] : 0.0;
     for (int p = 0; p < b; p++) {
-        if (i == p)
-            for (int q = 0; q < 16; q++) rowp[j0 + q] = v[q];
-        for (int q = 0; q < 16; q++)
-            if (j0 + q == p) colp[i] = v[q];
-        __syncthreads();
-        const double piv = rowp[p];
+        double rj = 0.0;
+#pragma unroll
+        for (int i = 0; i < GJB; i++) rj = i == p ? c[i] : rj;
+        const double piv = __shfl(rj, p);
         const bool ok = piv > 0.0 && std::isfinite(piv);
-        if (!ok && threadIdx.x == 0) info[0] = 1;
+        if (!ok && j == 0) info[0] = 1;
         const double ip = ok ? 1.0 / piv : 0.0;
-        const double ci = colp[i];
-        for (int q = 0; q < 16; q++) {
-            const int j = j0 + q;
-            const double rj = rowp[j];
+#pragma unroll
+        for (int i = 0; i < GJB; i++) {
+            const double ci = __shfl(c[i], p);
             double nv;
-            if (i == p && j == p) nv = ip;
-            else if (i == p) nv = rj * ip;
+            if (i == p) nv = j == p ? ip : rj * ip;
             else if (j == p) nv = -(ci * ip);
-            else nv = v[q] - ci * (rj * ip);
-            v[q] = nv;
+            else nv = c[i] - ci * (rj * ip);
+            c[i] = nv;
         }
-        __syncthreads();
     }
-    for (int q = 0; q < 16; q++)
-        if (i < b && j0 + q < b) P[i * GJB + j0 + q] = v[q];
+#pragma unroll
+    for (int i = 0; i < GJB; i++)
+        if (i < b && j < b) P[i * GJB + j] = c[i];
 }
 
 // one workgroup per 64-column tile t: R[:, tile] = P A_K,tile (t != k), and
@@ -255,13 +251,7 @@ template <typename T>
 int dev_upload(amr_model *m, T **d, const std::vector<T> &h)
 {
     const size_t n = std::max<size_t>(h.size(), 1);
-    hipError_t e = hipMalloc((void **)d, n * sizeof(T));
-    if (e != hipSuccess) {
-        *d = nullptr;
-        amr_set_error(std::string("hipMalloc (coarse space) failed: ") + hipGetErrorString(e));
-        return e == hipErrorOutOfMemory ? AMR_ERR_NOMEM : AMR_ERR_HIP;
-    }
-    m->dev_bytes += (int64_t)(n * sizeof(T));
+    if (int rc = amr_dalloc(m, (void **)d, n * sizeof(T))) return rc;
     if (!h.empty()) AMR_HIP(hipMemcpyAsync(*d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, m->stream));
     return AMR_OK;
 }
@@ -269,15 +259,7 @@ int dev_upload(amr_model *m, T **d, const std::vector<T> &h)
 template <typename T>
 int dev_alloc(amr_model *m, T **d, size_t n)
 {
-    n = std::max<size_t>(n, 1);
-    hipError_t e = hipMalloc((void **)d, n * sizeof(T));
-    if (e != hipSuccess) {
-        *d = nullptr;
-        amr_set_error(std::string("hipMalloc (coarse space) failed: ") + hipGetErrorString(e));
-        return e == hipErrorOutOfMemory ? AMR_ERR_NOMEM : AMR_ERR_HIP;
-    }
-    m->dev_bytes += (int64_t)(n * sizeof(T));
-    return AMR_OK;
+    return amr_dalloc(m, (void **)d, std::max<size_t>(n, 1) * sizeof(T));
 }
 
 int build(amr_model *m)
@@ -285,6 +267,7 @@ int build(amr_model *m)
     const int64_t n = m->nnodes;
     const int target = (int)std::max<int64_t>(m->coarse_target, (n + m->coarse_max - 1) / m->coarse_max);
     AmrCoarse C;
+    if (int rc = amr_pattern_host(m)) return rc;
     if (int rc = amr_build_coarse(n, m->Xh.data(), m->pat.rowptr.data(), m->pat.col.data(), target, C)) return rc;
     m->nagg = C.nagg;
     m->ncoarse = 6 * C.nagg;
@@ -357,7 +340,7 @@ int amr_coarse_setup(amr_model *m)
         const int nt = (nc + GJB - 1) / GJB;
         for (int k = 0; k < nt; k++) {
             const int b = std::min(GJB, nc - k * GJB);
-            hipLaunchKernelGGL(gj_pivot_kernel, dim3(1), dim3(256), 0, s, nc, k * GJB, b, m->Kinv, P, m->c_info);
+            hipLaunchKernelGGL(gj_pivot_kernel, dim3(1), dim3(64), 0, s, nc, k * GJB, b, m->Kinv, P, m->c_info);
             hipLaunchKernelGGL(gj_panel_kernel, dim3((unsigned)nt), dim3(256), 0, s, nc, k, b, m->Kinv, P, R, C);
             hipLaunchKernelGGL(gj_update_kernel, dim3((unsigned)nt, (unsigned)nt), dim3(256), 0, s, nc, k, b, m->Kinv,
                                P, R, C);
@@ -375,10 +358,14 @@ int amr_coarse_setup(amr_model *m)
 
 void amr_coarse_free(amr_model *m)
 {
-    void *ptrs[] = {m->c_agg, m->c_aggptr, m->c_aggnodes, m->c_prow, m->c_pagg, m->c_cba,  m->c_cbb, m->c_cbptr,
-                    m->c_cbslot, m->c_info, m->c_cent, m->c_pair, m->Kinv, m->c_vec, m->y_vec, m->c_work};
-    for (void *p : ptrs)
-        if (p) (void)hipFree(p);
+    void **ptrs[] = {(void **)&m->c_agg,  (void **)&m->c_aggptr, (void **)&m->c_aggnodes, (void **)&m->c_prow,
+                     (void **)&m->c_pagg, (void **)&m->c_cba,    (void **)&m->c_cbb,      (void **)&m->c_cbptr,
+                     (void **)&m->c_cbslot, (void **)&m->c_info, (void **)&m->c_cent,     (void **)&m->c_pair,
+                     (void **)&m->Kinv,   (void **)&m->c_vec,    (void **)&m->y_vec,      (void **)&m->c_work};
+    for (void **p : ptrs) {
+        amr_release(m, *p);
+        *p = nullptr;
+    }
     m->coarse_built = false;
     m->coarse_version = -1;
 }

@@ -370,7 +370,7 @@ __global__ void __launch_bounds__(64) ke_kernel(int64_t ne, const double *__rest
 // the exact path to rounding, not bit for bit.
 //
 // MFMA f64 16x16x4 operands (one value per lane): A[m][k] at lane
-// m + 16 k, B[k][n] at lane n + 16 k; the 16x16 result D[4 (l/16) + r][l%16]
+// m + 16 k, B[k][n] at lane n + 16 k; the 16x16 result D[l/16 + 4 r][l%16]
 // in element r of lane l's four-value accumulator.
 typedef double f64x4m __attribute__((ext_vector_type(4)));
 
@@ -505,7 +505,7 @@ __global__ void __launch_bounds__(64, 2) ke_mfma_hex20_kernel(int64_t ne, const 
         for (int nt = mt; nt < 4; nt++) {
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-                const int row = mt * 16 + 4 * lk + r, col = nt * 16 + lr;
+                const int row = mt * 16 + lk + 4 * r, col = nt * 16 + lr;
                 if (row < 60 && col < 60) {
                     const double v = acc[idx][r];
                     Ke_e[((row / 3) * NN + col / 3) * 9 + (row % 3) * 3 + col % 3] = v;
@@ -514,6 +514,138 @@ __global__ void __launch_bounds__(64, 2) ke_mfma_hex20_kernel(int64_t ne, const 
             }
             idx++;
         }
+    if (lane == 0 && sBad) atomicMin(err, (int32_t)(e + 1));
+}
+
+// Linear elastic elements (every material LinearElastic).  With the Lame
+// constants lambda = c nu, mu = c (1 - 2 nu) / 2 of the Mandel De
+// (c = E / ((1 + nu)(1 - 2 nu))), the node block (a, b) of Bq' De Bq is
+// lambda g_a g_b' + mu g_b g_a' + mu (g_a . g_b) I for the gradients g of
+// N_a, N_b at the IP, so
+//     Ke_ab[i][j] = lambda S_ab[i][j] + mu S_ab[j][i] + mu tr(S_ab) delta_ij,
+//     S_ab[i][j]  = sum_q coef_q g_a,i(q) g_b,j(q),
+// one product G' (C G) whose inner dimension is the IPs, four per MFMA
+// k-step (27 IPs: 7 k-steps).  Row (a, i) of S sits at 12 (a/4) + 4 i + a%4
+// and column (b, j) at 16 j + b%16 (nodes 16..19 in a second pass), so
+// that lane l holds all nine entries of S_ab for the nodes a = l/16 (mod 4)
+// and b = l%16 (mod 16) and forms its blocks of Ke in registers.  Same
+// operands as the exact kernel up to this algebra; the sums run in another
+// order, so Ke agrees with it to rounding.
+template <int NN, int NQ>
+__global__ void __launch_bounds__(64) ke_gram_kernel(int64_t ne, const double *__restrict__ X,
+                                                     const int32_t *__restrict__ conn,
+                                                     const int32_t *__restrict__ emat,
+                                                     const AmrMat *__restrict__ mats,
+                                                     const double *__restrict__ dNdR,
+                                                     const double *__restrict__ wip, double *__restrict__ Ke,
+                                                     int32_t *__restrict__ err)
+{
+    constexpr int KS = (NQ + 3) / 4;                   // k-steps over the IPs
+    constexpr int NAG = (NN + 3) / 4;                  // row node groups: a = 4 ag + l/16
+    constexpr int MT = (3 * (NAG - 1) + 2) / 4 + 1;    // row tiles
+    constexpr int NP = (NN + 15) / 16;                 // column passes of 16 nodes
+    const int64_t e = blockIdx.x;
+    const int lane = threadIdx.x, lr = lane % 16, lk = lane / 16;
+    __shared__ double sC[NN * 3];
+    __shared__ double sInv[NQ][9];
+    __shared__ double sCoef[4 * KS];
+    __shared__ double sG[4 * KS][NN * 3];
+    __shared__ int sBad;
+    for (int i = lane; i < NN * 3; i += 64) sC[i] = X[(int64_t)conn[e * NN + i / 3] * 3 + i % 3];
+    if (lane == 0) sBad = 0;
+    __syncthreads();
+    if (lane < NQ) {
+        const double *dq = dNdR + lane * NN * 3;
+        double J[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0}, inv[9];
+#pragma unroll 2
+        for (int n = 0; n < NN; n++) {
+            const double d0 = dq[n * 3], d1 = dq[n * 3 + 1], d2 = dq[n * 3 + 2];
+#pragma unroll
+            for (int i = 0; i < 3; i++) {
+                const double x = sC[n * 3 + i];
+                J[i * 3] = fma(x, d0, J[i * 3]);
+                J[i * 3 + 1] = fma(x, d1, J[i * 3 + 1]);
+                J[i * 3 + 2] = fma(x, d2, J[i * 3 + 2]);
+            }
+        }
+        const double det = inv3(J, inv);
+        if (!(det > 0.0)) sBad = 1;
+        for (int k = 0; k < 9; k++) sInv[lane][k] = inv[k];
+        sCoef[lane] = det * wip[lane];
+    } else if (lane < 4 * KS) {
+        sCoef[lane] = 0.0;
+    }
+    __syncthreads();
+    for (int t = lane; t < 4 * KS * NN; t += 64) {
+        const int q = t / NN, n = t % NN;
+        for (int j = 0; j < 3; j++) {
+            double a = 0.0;
+            if (q < NQ) {
+                const double *dq = dNdR + (q * NN + n) * 3;
+                for (int i = 0; i < 3; i++) a = fma(dq[i], sInv[q][i * 3 + j], a);
+            }
+            sG[q][n * 3 + j] = a;
+        }
+    }
+    __syncthreads();
+    // A operand rows: product row p = 16 mt + l%16 is (a, i) or padding
+    int aoff[MT];
+#pragma unroll
+    for (int mt = 0; mt < MT; mt++) {
+        const int p = 16 * mt + lr, u = p / 4, a = 4 * (u / 3) + p % 4;
+        aoff[mt] = (u < 3 * NAG && a < NN) ? a * 3 + u % 3 : -1;
+    }
+    const AmrMat mat = mats[emat[e]];
+    const double E = mat.p[0], nu = mat.p[1];
+    const double c = E / ((1.0 + nu) * (1.0 - 2.0 * nu));
+    const double lam = c * nu, mu = 0.5 * (c * (1.0 - 2.0 * nu));
+    double *Ke_e = Ke + e * (int64_t)(NN * NN * 9);
+#pragma unroll 1
+    for (int pp = 0; pp < NP; pp++) {
+        const int b = 16 * pp + lr;
+        f64x4m acc[MT][3];
+#pragma unroll
+        for (int mt = 0; mt < MT; mt++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) acc[mt][j] = f64x4m{0.0, 0.0, 0.0, 0.0};
+#pragma unroll 1
+        for (int s = 0; s < KS; s++) {
+            const int q = 4 * s + lk;
+            const double cq = sCoef[q];
+            double av[MT], bv[3];
+#pragma unroll
+            for (int mt = 0; mt < MT; mt++) av[mt] = aoff[mt] >= 0 ? sG[q][aoff[mt]] : 0.0;
+#pragma unroll
+            for (int j = 0; j < 3; j++) bv[j] = b < NN ? cq * sG[q][b * 3 + j] : 0.0;
+#pragma unroll
+            for (int mt = 0; mt < MT; mt++)
+#pragma unroll
+                for (int j = 0; j < 3; j++)
+                    acc[mt][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[mt], bv[j], acc[mt][j], 0, 0, 0);
+        }
+        if (b < NN) {
+#pragma unroll
+            for (int ag = 0; ag < NAG; ag++) {
+                const int a = 4 * ag + lk;
+                if (a >= NN) continue;
+                double S[3][3];
+#pragma unroll
+                for (int i = 0; i < 3; i++)
+#pragma unroll
+                    for (int j = 0; j < 3; j++) {
+                        const int u = 3 * ag + i;
+                        S[i][j] = acc[u / 4][j][u % 4];
+                    }
+                const double tr = S[0][0] + S[1][1] + S[2][2];
+                double *blk = Ke_e + ((int64_t)a * NN + b) * 9;
+#pragma unroll
+                for (int i = 0; i < 3; i++)
+#pragma unroll
+                    for (int j = 0; j < 3; j++)
+                        blk[i * 3 + j] = fma(lam, S[i][j], mu * S[j][i]) + (i == j ? mu * tr : 0.0);
+            }
+        }
+    }
     if (lane == 0 && sBad) atomicMin(err, (int32_t)(e + 1));
 }
 
@@ -978,6 +1110,26 @@ int amr_launch_ke_path(amr_model *m, int path)
     if (sh < 0) {
         amr_set_error("unknown element type");
         return AMR_ERR_ARG;
+    }
+    if (path == AMR_KE_MFMA && m->all_elastic) {
+        if (m->ne == 0) return AMR_OK;
+        const dim3 g((unsigned)m->ne), b(64);
+        if (m->etype == AMR_HEX20 && m->nips == 27)
+            hipLaunchKernelGGL((ke_gram_kernel<20, 27>), g, b, 0, m->stream, m->ne, m->X, m->conn, m->emat, m->mats,
+                               m->dNdR, m->wip, m->Ke, m->err);
+        else if (m->etype == AMR_HEX20 && m->nips == 8)
+            hipLaunchKernelGGL((ke_gram_kernel<20, 8>), g, b, 0, m->stream, m->ne, m->X, m->conn, m->emat, m->mats,
+                               m->dNdR, m->wip, m->Ke, m->err);
+        else if (m->etype == AMR_HEX8 && m->nips == 8)
+            hipLaunchKernelGGL((ke_gram_kernel<8, 8>), g, b, 0, m->stream, m->ne, m->X, m->conn, m->emat, m->mats,
+                               m->dNdR, m->wip, m->Ke, m->err);
+        else if (m->etype == AMR_TET10 && m->nips == 4)
+            hipLaunchKernelGGL((ke_gram_kernel<10, 4>), g, b, 0, m->stream, m->ne, m->X, m->conn, m->emat, m->mats,
+                               m->dNdR, m->wip, m->Ke, m->err);
+        else
+            return launch_ke_var(m, sh, ke_variant(m, sh));
+        AMR_HIP(hipGetLastError());
+        return AMR_OK;
     }
     if (path == AMR_KE_MFMA && m->etype == AMR_HEX20) {
         if (m->ne == 0) return AMR_OK;

@@ -109,17 +109,17 @@ template <int NT>
 __device__ inline double sum_partials(const double *__restrict__ P, int n)
 {
     double v = 0.0;
-    // four loads in flight per pass; the additions keep the order i, i+NT, ...
+    // up to 16 loads of a thread issued before its additions; the additions
+    // keep the order i, i+NT, ...
     const auto *Pg = gl(P);
-    int i = threadIdx.x;
-    for (; i + 3 * NT < n; i += 4 * NT) {
-        const double a0 = Pg[i], a1 = Pg[i + NT], a2 = Pg[i + 2 * NT], a3 = Pg[i + 3 * NT];
-        v += a0;
-        v += a1;
-        v += a2;
-        v += a3;
+    for (int i = threadIdx.x; i < n; i += 16 * NT) {
+        double a[16];
+#pragma unroll
+        for (int k = 0; k < 16; k++) a[k] = i + k * NT < n ? Pg[i + k * NT] : 0.0;
+#pragma unroll
+        for (int k = 0; k < 16; k++)
+            if (i + k * NT < n) v += a[k];
     }
-    for (; i < n; i += NT) v += Pg[i];
     return block_sum<NT>(v);
 }
 

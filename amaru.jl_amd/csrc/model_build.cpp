@@ -113,9 +113,10 @@ int amr_build_pattern(int nn, int64_t nnodes, int64_t ne, const int32_t *conn, A
 
 // ---------------------------------------------------------------------------
 // Coarse space of the two-level preconditioner: aggregates of nodes by
-// recursive coordinate bisection (split the longest extent at the median,
-// stable on node id, until every part has at most `target` nodes), and the
-// index lists that let the device form Kc = Phi^T K Phi deterministically:
+// recursive coordinate bisection (split the longest extent at the median of
+// (coordinate, node id), until every part has at most `target` nodes; a
+// linear-time selection per split), and the index lists that let the device
+// form Kc = Phi^T K Phi deterministically:
 //   pairs:  for each block row i, the distinct aggregates b of its columns
 //           (ascending), one output slot per (i, b);
 //   cblocks: for each coarse block (a, b), the slots of rows i in a with
@@ -126,9 +127,14 @@ int amr_build_coarse(int64_t nnodes, const double *X, const int32_t *rowptr, con
     if (target < 1) target = 1;
     C = AmrCoarse();
     C.agg.assign(nnodes, 0);
-    std::vector<int32_t> idx(nnodes);
-    for (int64_t i = 0; i < nnodes; i++) idx[i] = (int32_t)i;
-    // explicit stack of [lo, hi) ranges of idx; parts are numbered in the
+    // the nodes with their coordinates, permuted in place by the splits
+    struct Pt {
+        double x[3];
+        int32_t id;
+    };
+    std::vector<Pt> pts(nnodes);
+    for (int64_t i = 0; i < nnodes; i++) pts[i] = Pt{{X[i * 3], X[i * 3 + 1], X[i * 3 + 2]}, (int32_t)i};
+    // explicit stack of [lo, hi) ranges of pts; parts are numbered in the
     // order they are finished (depth first, lower half first)
     std::vector<std::pair<int64_t, int64_t>> st{{0, nnodes}};
     int32_t nagg = 0;
@@ -136,22 +142,25 @@ int amr_build_coarse(int64_t nnodes, const double *X, const int32_t *rowptr, con
         auto [lo, hi] = st.back();
         st.pop_back();
         if (hi - lo <= target) {
-            for (int64_t k = lo; k < hi; k++) C.agg[idx[k]] = nagg;
+            for (int64_t k = lo; k < hi; k++) C.agg[pts[k].id] = nagg;
             nagg++;
             continue;
         }
         double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
         for (int64_t k = lo; k < hi; k++)
             for (int d = 0; d < 3; d++) {
-                mn[d] = std::min(mn[d], X[idx[k] * 3 + d]);
-                mx[d] = std::max(mx[d], X[idx[k] * 3 + d]);
+                mn[d] = std::min(mn[d], pts[k].x[d]);
+                mx[d] = std::max(mx[d], pts[k].x[d]);
             }
         int ax = 0;
         for (int d = 1; d < 3; d++)
             if (mx[d] - mn[d] > mx[ax] - mn[ax]) ax = d;
-        std::stable_sort(idx.begin() + lo, idx.begin() + hi,
-                         [&](int32_t a, int32_t b) { return X[a * 3 + ax] < X[b * 3 + ax]; });
         const int64_t mid = lo + (hi - lo) / 2;
+        // the lower half is the (mid - lo) smallest by (coordinate, id), a
+        // set that does not depend on the order inside [lo, hi)
+        std::nth_element(pts.begin() + lo, pts.begin() + mid, pts.begin() + hi, [ax](const Pt &a, const Pt &b) {
+            return a.x[ax] < b.x[ax] || (a.x[ax] == b.x[ax] && a.id < b.id);
+        });
         st.push_back({mid, hi});
         st.push_back({lo, mid});
     }
@@ -173,29 +182,53 @@ int amr_build_coarse(int64_t nnodes, const double *X, const int32_t *rowptr, con
         const double w = 1.0 / (double)(C.aggptr[a + 1] - C.aggptr[a]);
         for (int d = 0; d < 3; d++) C.cent[a * 3 + d] = s[d] * w;
     }
-    // (row, aggregate) pairs
+    // (row, aggregate) pairs: the distinct aggregates of each row's columns
+    // (a stamp per aggregate drops repeats), ascending
     C.pptr.assign(nnodes + 1, 0);
     C.prow.clear();
-    std::vector<int32_t> seen;
+    std::vector<int32_t> seen, stamp(nagg, -1);
     for (int64_t i = 0; i < nnodes; i++) {
         seen.clear();
-        for (int32_t k = rowptr[i]; k < rowptr[i + 1]; k++) seen.push_back(C.agg[col[k]]);
+        for (int32_t k = rowptr[i]; k < rowptr[i + 1]; k++) {
+            const int32_t b = C.agg[col[k]];
+            if (stamp[b] != (int32_t)i) {
+                stamp[b] = (int32_t)i;
+                seen.push_back(b);
+            }
+        }
         std::sort(seen.begin(), seen.end());
-        seen.erase(std::unique(seen.begin(), seen.end()), seen.end());
         for (int32_t b : seen) {
             C.pagg.push_back(b);
             C.prow.push_back((int32_t)i);
         }
         C.pptr[i + 1] = (int32_t)C.pagg.size();
     }
-    // coarse blocks (a, b) ascending, slots ascending in row
+    // coarse blocks (a, b) ascending, slots ascending in row: a stable
+    // counting sort by b, then by a (slots are generated in row order)
     const int64_t np = (int64_t)C.pagg.size();
     std::vector<int64_t> key(np);
     for (int64_t i = 0; i < nnodes; i++)
         for (int32_t s = C.pptr[i]; s < C.pptr[i + 1]; s++) key[s] = (int64_t)C.agg[i] * nagg + C.pagg[s];
-    std::vector<int32_t> ord(np);
-    for (int64_t s = 0; s < np; s++) ord[s] = (int32_t)s;
-    std::stable_sort(ord.begin(), ord.end(), [&](int32_t x, int32_t y) { return key[x] < key[y]; });
+    std::vector<int32_t> ord(np), tmp(np);
+    {
+        std::vector<int64_t> cnt(nagg + 1);
+        auto pass = [&](const std::vector<int32_t> &in, std::vector<int32_t> &out, bool by_a) {
+            std::fill(cnt.begin(), cnt.end(), 0);
+            for (int64_t t = 0; t < np; t++) {
+                const int64_t k = key[in[t]];
+                cnt[(by_a ? k / nagg : k % nagg) + 1]++;
+            }
+            for (int32_t v = 0; v < nagg; v++) cnt[v + 1] += cnt[v];
+            for (int64_t t = 0; t < np; t++) {
+                const int64_t k = key[in[t]];
+                out[cnt[by_a ? k / nagg : k % nagg]++] = in[t];
+            }
+        };
+        for (int64_t t = 0; t < np; t++) tmp[t] = (int32_t)t;
+        pass(tmp, ord, false);
+        pass(ord, tmp, true);
+        ord.swap(tmp);
+    }
     C.cbptr.push_back(0);
     for (int64_t t = 0; t < np; t++) {
         if (t > 0 && key[ord[t]] != key[ord[t - 1]]) C.cbptr.push_back((int32_t)t);

@@ -432,6 +432,9 @@ class Server:
         # main-loop time accounting (seconds): pool steps, resuming finished
         # members, starting admitted ones, idle waits
         self.t_loop = {"step": 0.0, "resume": 0.0, "start": 0.0, "wait": 0.0, "free": 0.0}
+        # worker time (seconds, summed over requests): host construction and
+        # device set-up (upload, pattern, coarse space)
+        self.t_build = {"host": 0.0, "device": 0.0}
         # finished models whose device memory is released later (a release
         # can wait for the device), at idle points or when too much waits
         self._graveyard = []
@@ -488,15 +491,23 @@ class Server:
         t0 = time.perf_counter()
         clock = lambda: time.perf_counter() - t0  # noqa: E731
 
+        def timed(kind, ts):
+            with lock:
+                self.t_build[kind] += time.perf_counter() - ts
+
         def host_build(req):
+            ts = time.perf_counter()
             try:
                 ana, kw = req.build()
                 ana.model.configure_dofs(ana.stages[0].bcs)
                 return req, ana, kw, estimate_device_bytes(ana.model), None
             except Exception as e:
                 return req, None, None, 0, e
+            finally:
+                timed("host", ts)
 
         def device_build(req):
+            ts = time.perf_counter()
             try:
                 req.ana.dev = DeviceModel(req.ana.model, device=self.device)
                 if req.kwargs.get("precond") == "two_level":
@@ -504,6 +515,8 @@ class Server:
                 return req, None
             except Exception as e:
                 return req, e
+            finally:
+                timed("device", ts)
 
         nxt = 0
         building, creating = set(), set()

@@ -117,6 +117,36 @@ def cpu_baseline(n_cpu, threads):
                 U=U, R=R)
 
 
+def ke_probe(am, args):
+    """mount_K of an NR iteration on the block with each element-stiffness
+    path, five times each after one untimed call (for a kernel trace), and
+    the largest relative Frobenius difference of the two paths' Ke."""
+    import ctypes
+    mesh, model, bcs, Uex, Fex = block_problem(am, args.n)
+    dev = am.DeviceModel(model, device=0)
+    dev.stage_begin(Uex * 0, Fex)
+    dev.increment_begin(1.0, 1.0, False)
+    am.lib()
+    hip = ctypes.CDLL("libamdhip64.so.7")   # the runtime the library loaded
+    out, Ke = {}, {}
+    for path in ("exact", "mfma"):
+        dev.set_ke_path(path)
+        dev.nr_prepare(1)
+        assert hip.hipDeviceSynchronize() == 0
+        t = time.perf_counter()
+        for _ in range(5):
+            dev.nr_prepare(1)
+        assert hip.hipDeviceSynchronize() == 0
+        out[path] = (time.perf_counter() - t) / 5 * 1e3
+        Ke[path] = dev.Ke()
+        log(f"ke-probe: {path}: {out[path]:.2f} ms per mount_K")
+    d = np.linalg.norm((Ke["mfma"] - Ke["exact"]).reshape(len(Ke["exact"]), -1), axis=1)
+    d /= np.linalg.norm(Ke["exact"].reshape(len(Ke["exact"]), -1), axis=1)
+    log(f"ke-probe: max relative Frobenius difference {d.max():.3e}")
+    print(json.dumps({"mount_K_ms": out, "max_rel_frobenius": float(d.max())}))
+    dev.close()
+
+
 def direct_check(am, args, cb):
     """The headline step's modes on the CPU sample's block (the largest one
     the bench's SuperLU factorises in bounded time), against that FP64 direct
@@ -131,9 +161,12 @@ def direct_check(am, args, cb):
              "plain_fp64": dict(precision="fp64", precond="jacobi"),
              "mixed_f64acc_two_level": dict(precision="mixed64acc", precond="two_level")}
     for name, kw in modes.items():
+        # the element stiffness each mode has in the timed steps
+        dev.set_ke_path("exact" if name == "plain_fp64" else args.ke)
         r = run_steps(dev, dev.opts(rtol=args.rtol, **kw), 1)[2]
         dU, dF = dev.stage_get(last=True)
-        out[name] = {"mode": kw, "rel_err_U": rel_err(dU[:nu], cb["U"][:nu]),
+        out[name] = {"mode": dict(kw, element_stiffness="exact" if name == "plain_fp64" else args.ke),
+                     "rel_err_U": rel_err(dU[:nu], cb["U"][:nu]),
                      "rel_err_reactions": rel_err(dF[nu:], cb["R"]), "pcg_iterations": r["pcg_iterations"]}
     dev.close()
     return out
@@ -511,6 +544,7 @@ def served_leg(am, args):
                 "max_rel_err_vs_plain": err,
                 "latency_s": P.latency_stats(done), "peak_admitted_gb": srv.peak_in_use / 2 ** 30,
                 "pool_steps": srv.steps, "main_loop_s": {k: round(v, 4) for k, v in srv.t_loop.items()},
+                "worker_s": {k: round(v, 4) for k, v in srv.t_build.items()},
                 "phases_mean_s": {"arrival_to_admit": float(np.mean([r.t_admit - r.t_arrival for r in done])),
                                   "admit_to_device_ready": float(np.mean([r.t_start - r.t_admit for r in done])),
                                   "device_ready_to_done": float(np.mean([r.t_done - r.t_start for r in done]))}}
@@ -574,8 +608,11 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=2)
     ap.add_argument("--precision", default="mixed64acc", choices=["mixed", "mixed64acc", "fp64"])
     ap.add_argument("--precond", default="jacobi", choices=["jacobi", "block_jacobi"])
-    ap.add_argument("--ke", default="exact", choices=["mfma", "exact"],
-                    help="element stiffness of the headline step: FP64 matrix cores or the bit-exact kernel")
+    ap.add_argument("--ke", default="mfma", choices=["mfma", "exact"],
+                    help="element stiffness of the headline step: FP64 matrix cores (Gram form for the linear "
+                         "elastic block) or the bit-exact kernel")
+    ap.add_argument("--ke-probe", action="store_true",
+                    help="time mount_K with each element-stiffness path on the block and exit (for a kernel trace)")
     ap.add_argument("--variant-steps", type=int, default=1)
     ap.add_argument("--rtol", type=float, default=1e-10)
     ap.add_argument("--cpu-n", type=int, default=12, help="HEX20 elements per edge of the CPU sample")
@@ -612,6 +649,9 @@ def main():
     am = load_pkg()
     if args.pmc_probe:
         pmc_probe(am, args)
+        return
+    if args.ke_probe:
+        ke_probe(am, args)
         return
     t0 = time.perf_counter()
     mesh, model, bcs, Uex, Fex = block_problem(am, args.n)

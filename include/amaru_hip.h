@@ -256,10 +256,12 @@ int amr_set_elem_tiles(amr_model *m, int ach, int epw);
 /* Element stiffness of the Newton-Raphson iterations (amr_nr_prepare,
  * amr_nr_iteration; elem_stiffness, src/mech/elem/mech-solid.jl:124-166):
  * AMR_KE_EXACT (the default) follows the oracle's operation order bit for
- * bit; AMR_KE_MFMA (HEX20 only) forms Ke = sum_q Bq' (coef D) Bq on the
- * FP64 matrix cores, equal to the exact one to rounding (the summation
- * order differs).  amr_assemble (mount_K with its pattern) always uses the
- * exact path. */
+ * bit; AMR_KE_MFMA uses the FP64 matrix cores, equal to the exact path to
+ * rounding (the summation order differs): models whose materials are all
+ * LinearElastic form Ke_ab = lambda S_ab + mu S_ab' + mu tr(S_ab) I from
+ * S_ab = sum_q coef_q g_a g_b' (every shape and rule), other HEX20 models
+ * Ke = sum_q Bq' (coef D) Bq; the remaining cases keep the exact path.
+ * amr_assemble (mount_K with its pattern) always uses the exact path. */
 enum { AMR_KE_EXACT = 0, AMR_KE_MFMA = 1 };
 int amr_set_ke_path(amr_model *m, int path);
 int amr_get_elem_tiles(amr_model *m, int *ach, int *epw);
@@ -389,9 +391,15 @@ int amr_spmv_col_bytes(amr_model *m, int *bytes);
 int amr_set_spmv_col_bytes(amr_model *m, int bytes);
 
 /* Device memory held by the model (bytes), and the device's free and total
- * memory, for admission by HBM budget. */
+ * memory, for admission by HBM budget.  Model memory comes from a per-device
+ * cache: a destroyed model's blocks are kept for the next models (at most
+ * AMR_ARENA_BYTES per device, environment, default 32 GiB), so that a
+ * serving loop does not call hipMalloc/hipFree; amr_mem_info counts the
+ * cached blocks as free, and amr_mem_trim returns them to the device.  No
+ * counterpart in the reference (Julia's GC owns its arrays). */
 int amr_model_device_bytes(amr_model *m, int64_t *bytes);
 int amr_mem_info(int device, int64_t *free_bytes, int64_t *total_bytes);
+int amr_mem_trim(int device);
 
 /*
  * Distributed loads on the device, for many cells at once: the per-cell

@@ -93,3 +93,60 @@ def test_column_offsets_do_not_change_results(amaru):
         dev.set_spmv_col_bytes(3)
     dev.close()
 
+
+
+@pytest.mark.parametrize("shape,nips,mat", [("HEX20", 27, "LE"), ("HEX20", 8, "LE"), ("HEX20", 27, "VM"),
+                                            ("HEX8", 8, "LE"), ("TET10", 4, "LE")])
+def test_mfma_stiffness_matches_oracle(amaru, shape, nips, mat):
+    """The element stiffness on the FP64 matrix cores (the NR iterations'
+    path when selected): linear elastic models through the Gram form
+    lambda S + mu S' + mu tr(S) I of S = sum_q coef g g' (every shape), von
+    Mises HEX20 through sum_q B'(coef D)B from yielded states.  Every element
+    matrix within 1e-12 relative Frobenius of the oracle's (the summation
+    order differs); the exported K keeps the exact path's values and
+    pattern."""
+    import test_gpu_parity as T
+    mesh, model, om = T.pair(amaru, shape, mat, nips, n=(4, 3, 2), seed=3)
+    bcs = [("x==0", amaru.SurfaceBC(ux=0, uy=0, uz=0))]
+    model.configure_dofs(bcs)
+    om.configure_dofs([("x==0", fem.SurfaceBC(ux=0, uy=0, uz=0))])
+    dev = amaru.DeviceModel(model)
+    st = om.state
+    if mat == "VM":
+        dU = np.random.default_rng(11).normal(size=model.ndofs) * 4e-3
+        _, st, msg = om.update_state(dU, om.state)
+        assert msg is None and (st[:, 13] != 0).any()
+        dev.update_state(dU)    # trial states = the oracle's st
+    Ko = om.elem_stiffness(st)
+    dev.set_ke_path("mfma")
+    U, F = model.bc_vals(bcs)
+    dev.stage_begin(U, F)
+    dev.increment_begin(1.0, 1.0, False)
+    dev.nr_prepare(1)           # mount_K on the matrix cores
+    Km = dev.Ke()
+    err = np.linalg.norm((Km - Ko).reshape(len(Ko), -1), axis=1) / np.linalg.norm(Ko.reshape(len(Ko), -1), axis=1)
+    assert err.max() <= 1e-12, err.max()
+    # amr_assemble (mount_K with the reference's pattern) stays exact
+    dev.assemble()
+    assert np.array_equal(dev.Ke(), Ko)
+    dev.close()
+
+
+def test_mfma_block_solve_vs_superlu(amaru):
+    """The benchmark's step with the matrix-core stiffness on the 6^3 block:
+    displacements and reactions within 1e-8 of SuperLU."""
+    mesh, model, om, bcs, U, F = block(amaru, 6)
+    nu = model.nu
+    Ko = om.mount_K().tocsc()
+    Ud, Fd = U.copy(), F.copy()
+    assert fem.solve_system(Ko.tocsr(), Ud, Fd, nu) is None
+    dev = amaru.DeviceModel(model)
+    dev.set_ke_path("mfma")
+    dev.stage_begin(U * 0, F)
+    dev.increment_begin(1.0, 1.0, False)
+    r = dev.nr_iteration(1, dev.opts(precision="mixed64acc", rtol=1e-10))
+    assert r["status"] == 0
+    dUa, dFin = dev.stage_get(last=True)
+    assert np.linalg.norm(dUa[:nu] - Ud[:nu]) <= 1e-8 * np.linalg.norm(Ud[:nu])
+    assert np.linalg.norm(dFin[nu:] - Fd[nu:]) <= 1e-8 * np.linalg.norm(Fd[nu:])
+    dev.close()

@@ -2,7 +2,8 @@
 
 Pinned independently of the device code:
 * the aggregates equal a numpy restatement of the recursive coordinate
-  bisection rule;
+  bisection rule (split the longest extent at the median of (coordinate,
+  node id));
 * Kc^-1 equals the inverse of Phi^T K11 Phi built in numpy from the exported
   K (rotation modes from cross products), to 1e-9 relative;
 * solutions match SuperLU to 1e-8 in every precision, with several times
@@ -29,7 +30,8 @@ def rcb(X, target):
             continue
         sub = idx[lo:hi]
         ax = int(np.argmax(X[sub].max(0) - X[sub].min(0)))
-        idx[lo:hi] = sub[np.argsort(X[sub, ax], kind="stable")]
+        # lower half: the smallest by (coordinate, node id)
+        idx[lo:hi] = sub[np.lexsort((sub, X[sub, ax]))]
         mid = lo + (hi - lo) // 2
         st.append((mid, hi))
         st.append((lo, mid))
