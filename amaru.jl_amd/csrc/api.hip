@@ -10,6 +10,8 @@
 
 #include "amr_internal.h"
 
+#include <rocprofiler-sdk-roctx/roctx.h>
+
 static thread_local std::string g_err;
 
 void amr_set_error(const std::string &msg) { g_err = msg; }
@@ -160,6 +162,16 @@ static int ip_rule(int etype, int nips, double *R, double *w)
     return -1;
 }
 
+// A roctx range over an entry point, named after the reference call it
+// implements, so that a profile splits a step by phase (no cost without a
+// profiler attached).
+struct PhaseRange {
+    explicit PhaseRange(const char *name) { roctxRangePushA(name); }
+    ~PhaseRange() { roctxRangePop(); }
+    PhaseRange(const PhaseRange &) = delete;
+    PhaseRange &operator=(const PhaseRange &) = delete;
+};
+
 static int nn_of(int etype)
 {
     return etype == AMR_HEX8 ? 8 : etype == AMR_HEX20 ? 20 : etype == AMR_TET10 ? 10 : -1;
@@ -169,8 +181,9 @@ static int nn_of(int etype)
 
 // Device memory of models.  A released block is kept in a per-device cache
 // and handed to the next request of about its size (best fit, at most 25%
-// larger), so that creating and destroying models in a serving loop does not
-// call hipMalloc or hipFree (hipFree waits for the whole device).  Blocks
+// larger, zeroed on the model's stream), so that creating and destroying
+// models in a serving loop does not call hipMalloc or hipFree (hipFree waits
+// for the whole device).  Blocks
 // are released only after the streams that used them are synchronised
 // (free_model, amr_pool_detach_model).  The cache holds at most
 // AMR_ARENA_BYTES (environment, default 32 GiB) per device; beyond that, and
@@ -181,7 +194,12 @@ struct Arena {
     size_t bytes = 0;
 };
 std::mutex g_dmem_mu;
-std::unordered_map<void *, std::pair<size_t, int>> g_dmem_live;  // block -> (size, device)
+struct LiveBlock {
+    size_t size;     // bytes of the block
+    size_t charged;  // bytes the model asked for (its dev_bytes)
+    int device;
+};
+std::unordered_map<void *, LiveBlock> g_dmem_live;
 std::unordered_map<int, Arena> g_arena;
 
 size_t arena_cap()
@@ -215,7 +233,8 @@ void arena_trim_locked(int device)
 
 int amr_dalloc(amr_model *m, void **p, size_t bytes)
 {
-    const size_t need = arena_round(bytes ? bytes : 1);
+    if (bytes == 0) bytes = 1;
+    const size_t need = arena_round(bytes);
     const int dev = m->device;
     {
         std::lock_guard<std::mutex> lk(g_dmem_mu);
@@ -226,10 +245,16 @@ int amr_dalloc(amr_model *m, void **p, size_t bytes)
             *p = it->second;
             A.blocks.erase(it);
             A.bytes -= sz;
-            g_dmem_live[*p] = {sz, dev};
-            m->dev_bytes += (int64_t)sz;
-            return AMR_OK;
+            g_dmem_live[*p] = {sz, bytes, dev};
+            m->dev_bytes += (int64_t)bytes;
+        } else {
+            *p = nullptr;
         }
+    }
+    if (*p) {
+        // a reused block starts zeroed, as fresh device memory does
+        AMR_HIP(hipMemsetAsync(*p, 0, need, m->stream));
+        return AMR_OK;
     }
     hipError_t e = hipMalloc(p, need);
     if (e == hipErrorOutOfMemory) {
@@ -247,9 +272,9 @@ int amr_dalloc(amr_model *m, void **p, size_t bytes)
     }
     {
         std::lock_guard<std::mutex> lk(g_dmem_mu);
-        g_dmem_live[*p] = {need, dev};
+        g_dmem_live[*p] = {need, bytes, dev};
     }
-    m->dev_bytes += (int64_t)need;
+    m->dev_bytes += (int64_t)bytes;
     return AMR_OK;
 }
 
@@ -262,10 +287,10 @@ void amr_release(amr_model *m, void *p)
         (void)hipFree(p);
         return;
     }
-    const size_t sz = it->second.first;
-    const int dev = it->second.second;
+    const size_t sz = it->second.size;
+    const int dev = it->second.device;
+    if (m) m->dev_bytes -= (int64_t)it->second.charged;
     g_dmem_live.erase(it);
-    if (m) m->dev_bytes -= (int64_t)sz;
     Arena &A = g_arena[dev];
     if (A.bytes + sz <= arena_cap()) {
         A.blocks.insert({sz, p});
@@ -412,6 +437,7 @@ extern "C" int amr_model_create(amr_model **out, int device, int etype, int nips
                                 const int32_t *mat_kind, const double *mat_params, const int32_t *emat,
                                 const int32_t *eq_id, int64_t ndofs, int64_t nu)
 {
+    PhaseRange range_("amr_model_create");
     if (!out || !coords || !conn || !mat_kind || !mat_params || !emat || !eq_id || nnodes <= 0 || ne < 0 ||
         nmats <= 0) {
         amr_set_error("amr_model_create: null pointer or empty model");
@@ -658,6 +684,7 @@ extern "C" int amr_state_restore(amr_model *m)
 
 extern "C" int amr_assemble(amr_model *m)
 {
+    PhaseRange range_("mount_K");
     if (!m) return AMR_ERR_ARG;
     if (amr_pool_is_member(m)) {
         amr_set_error("the model is solving in a pool; its matrix and solver vectors are in use");
@@ -951,6 +978,7 @@ static amr_solve_opts default_opts()
 extern "C" int amr_solve_system(amr_model *m, double *U, double *F, const amr_solve_opts *opts,
                                 amr_solve_info *info)
 {
+    PhaseRange range_("solve_system!");
     if (!m || !U || !F) return AMR_ERR_ARG;
     if (amr_pool_is_member(m)) {
         amr_set_error("the model is solving in a pool; its matrix and solver vectors are in use");
@@ -1009,6 +1037,7 @@ extern "C" int amr_solve_system(amr_model *m, double *U, double *F, const amr_so
 
 extern "C" int amr_update_state(amr_model *m, const double *dUt, double *dFin)
 {
+    PhaseRange range_("update_state!");
     if (!m || !dUt || !dFin) return AMR_ERR_ARG;
     AMR_HIP(hipSetDevice(m->device));
     const int64_t n = m->nnodes * 3;
@@ -1074,6 +1103,7 @@ extern "C" int amr_increment_begin(amr_model *m, double dT, double acr, int add_
 
 extern "C" int amr_nr_prepare(amr_model *m, int it)
 {
+    PhaseRange range_("mount_K (NR)");
     if (!m || !m->sv[SV_U] || it < 1) return AMR_ERR_ARG;
     if (amr_pool_is_member(m)) {
         amr_set_error("the model is solving in a pool; its matrix and solver vectors are in use");
@@ -1094,6 +1124,7 @@ extern "C" int amr_nr_prepare(amr_model *m, int it)
 
 extern "C" int amr_nr_finish_begin(amr_model *m, const amr_solve_info *si)
 {
+    PhaseRange range_("update_state! (NR)");
     if (!m || !m->sv[SV_U]) return AMR_ERR_ARG;
     if (amr_pool_is_member(m)) {
         amr_set_error("the model is solving in a pool; its matrix and solver vectors are in use");
@@ -1289,6 +1320,7 @@ static int nr_iteration_pc(amr_model *m, int it, const amr_solve_opts *o, amr_nr
 
 extern "C" int amr_nr_iteration(amr_model *m, int it, const amr_solve_opts *opts, amr_nr_info *info)
 {
+    PhaseRange range_("NR iteration");
     amr_solve_opts o = opts ? *opts : default_opts();
     if (m && m->scheme != AMR_SCHEME_FE) {
         if (!m->sv[SV_U] || it < 1) return AMR_ERR_ARG;
@@ -1435,6 +1467,7 @@ static int copy_results(const std::vector<amr_pool_result> &res, int *nfinished,
 
 extern "C" int amr_pool_step(amr_pool *p, int capacity, int *nfinished, amr_model **models, amr_solve_info *infos)
 {
+    PhaseRange range_("pool step");
     if (!p || !nfinished) return AMR_ERR_ARG;
     if (capacity < amr_pool_count(p) || (capacity > 0 && (!models || !infos))) {
         amr_set_error("amr_pool_step: the output arrays must hold every member of the pool");
@@ -1448,6 +1481,7 @@ extern "C" int amr_pool_step(amr_pool *p, int capacity, int *nfinished, amr_mode
 
 extern "C" int amr_pool_step_begin(amr_pool *p)
 {
+    PhaseRange range_("pool step (queue)");
     if (!p) return AMR_ERR_ARG;
     return amr_pool_step_launch(p);
 }

@@ -338,6 +338,28 @@ __device__ inline double lds_tree_sum(double *s, int R)
     return t;
 }
 
+// lds_tree_sum's tree (R a power of two, 64 <= R <= 1024) done by the first
+// wave alone, after the barrier that completes s: lane i reduces
+// s[i + 64 m] over m in registers (the levels h >= 64, same pairs in the
+// same order), then the levels h = 32..1 by shuffles.  The result is in lane
+// 0; the other waves need not wait.
+__device__ inline double wave_tree_sum(const double *s, int R)
+{
+    const int M = R / 64;
+    double v[16];
+#pragma unroll
+    for (int m = 0; m < 16; m++) v[m] = m < M ? s[threadIdx.x + 64 * m] : 0.0;
+#pragma unroll
+    for (int H = 8; H > 0; H >>= 1)
+        if (H < M)
+#pragma unroll
+            for (int m = 0; m < H; m++) v[m] = v[m] + v[m + H];
+    double a = v[0];
+#pragma unroll
+    for (int h = 32; h > 0; h >>= 1) a = a + __shfl_down(a, h, 64);
+    return a;
+}
+
 // Dynamic LDS layout of the row-tile kernels: R+1 row pointers, then up to
 // four double arrays of pow2ceil(R) entries (sized for the largest R of the
 // launch).
@@ -521,8 +543,15 @@ __global__ void __launch_bounds__(WGS) pcg_spmv_kernel(const SegDesc *__restrict
         if (sub == 0) sdot[i] = fma(p2, q2, fma(p1, q1, fma(pk, qk, 0.0)));
     }
     __syncthreads();
-    const double pq = lds_tree_sum(sdot, RP);
-    if (threadIdx.x == 0) d.part[P_PQ + te.y] = pq;
+    if (RP <= 1024) {
+        if (threadIdx.x < 64) {
+            const double pq = wave_tree_sum(sdot, RP);
+            if (threadIdx.x == 0) d.part[P_PQ + te.y] = pq;
+        }
+    } else {
+        const double pq = lds_tree_sum(sdot, RP);
+        if (threadIdx.x == 0) d.part[P_PQ + te.y] = pq;
+    }
 }
 
 __global__ void __launch_bounds__(WGV) pcg_update_kernel(const SegDesc *__restrict__ D, const int2 *__restrict__ vl)

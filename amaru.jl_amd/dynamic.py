@@ -13,8 +13,12 @@ through the C ABI:
     solve_system!(Kp, dUi, dFp, nu)  -> DeviceModel.solve_system
     update_state!(elems, dUt, dt)    -> DeviceModel.update_state
 
-Supported as the reference runs them: the first stage (the reference sets
-up velocities and accelerations only there), no seismic input.  With
+Supported as the reference runs them: the first stage, no seismic input.
+The reference's stage solver assigns its velocity and acceleration vectors
+A and V only inside ``if stage.id == 1`` (src/mech/dyn-solver.jl:262-282)
+and reads them in every increment (:378), so a second stage stops there with
+an undefined-variable error; a second stage is refused here with that
+reason.  With
 auto-increment a failed increment is retried from the last converged state
 with dT = round(clamp(1 + tanh(log10(tol/res1)), 0.2, 0.9) dT, 3 digits);
 after a converged one the reference's growth factor is 1 (its iteration
@@ -79,7 +83,8 @@ def dyn_solve(ana, **kwargs):
         # its seismic path stops with an undefined-variable error
         raise NotImplementedError("dyn_solve: seismic input is not implemented")
     if len(ana.stages) != 1:
-        raise NotImplementedError("dyn_solve: one stage (the reference initialises velocities only in stage 1)")
+        raise NotImplementedError("dyn_solve: one stage (the reference's dyn_stage_solver! binds its velocities and "
+                                  "accelerations only in stage 1 and fails on a second stage)")
     st = ana.stages[0]
     st.status = "solving"
     ok, msg = _stage(ana, st, a)

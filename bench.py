@@ -176,6 +176,25 @@ def dof_solves(anas):
     return sum(a.stats["dof_solves"] for a in anas)
 
 
+class phase:
+    """A roctx range around a leg of the bench (a no-op without a profiler),
+    so that a kernel trace can be split by leg."""
+    _lib = None
+
+    def __init__(self, name):
+        self.name = name.encode()
+
+    def __enter__(self):
+        import ctypes
+        if phase._lib is None:
+            phase._lib = ctypes.CDLL("librocprofiler-sdk-roctx.so.1")
+        phase._lib.roctxRangePushA(ctypes.c_char_p(self.name))
+
+    def __exit__(self, *exc):
+        phase._lib.roctxRangePop()
+        return False
+
+
 def rel_err(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
 
@@ -560,11 +579,13 @@ def served_leg(am, args):
     finally:
         srv.close()
     # a batch submitted at once is scheduled largest first; a stream in arrival order
-    cap = serve(np.zeros(len(params)), "largest_first")
+    with phase("served capacity"):
+        cap = serve(np.zeros(len(params)), "largest_first")
     log(f"served capacity (all requests at t=0): {cap['value']:.4g} DOF-solves/s, "
         f"{cap['value'] / plain_rate:.2f}x plain")
     rate = args.serve_load * req_rate
-    lat = serve(W.arrival_times(args.serve_seed, len(params), rate), "fifo")
+    with phase("served poisson"):
+        lat = serve(W.arrival_times(args.serve_seed, len(params), rate), "fifo")
     log(f"served at {rate:.3g} requests/s: {lat['value']:.4g} DOF-solves/s, p50 {lat['latency_s']['p50_s']:.3f}s "
         f"p99 {lat['latency_s']['p99_s']:.3f}s")
     out = {"workload": f"{len(params)} requests (seed {args.serve_seed}): Hex8/Tet10 boxes 1e3-3e5 DOFs, 70% "
@@ -586,16 +607,24 @@ def pmc_probe(am, args):
     """One mixed-precision step on the block (its SpMV launches are what a
     counter pass reads) and a plain device copy of 1 GiB, the calibration
     kernel of known traffic (1 GiB read, 1 GiB written)."""
-    import torch
+    import ctypes
     mesh, model, bcs, Uex, Fex = block_problem(am, args.n)
     dev = am.DeviceModel(model, device=0)
+    dev.set_ke_path(args.ke)
     dev.stage_begin(Uex, Fex)
     run_steps(dev, dev.opts(precision=args.precision, rtol=args.rtol, precond=args.precond), 1)
     dev.close()
-    a = torch.ones(2 ** 27, dtype=torch.float64, device="cuda")
-    b = torch.empty_like(a)
-    b.copy_(a)
-    torch.cuda.synchronize()
+    # the copy through the HIP runtime the library uses (a device-to-device
+    # hipMemcpy runs the runtime's copy kernel)
+    hip = ctypes.CDLL("libamdhip64.so.7")
+    n = 1 << 30
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(n)) == 0
+    assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(n)) == 0
+    assert hip.hipMemset(a, 1, ctypes.c_size_t(n)) == 0
+    assert hip.hipMemcpy(b, a, ctypes.c_size_t(n), 3) == 0
+    assert hip.hipDeviceSynchronize() == 0
+    assert hip.hipFree(a) == 0 and hip.hipFree(b) == 0
     log("pmc probe done: one step, one 1 GiB copy")
 
 
@@ -683,7 +712,8 @@ def main():
 
     run_steps(dev, mixed, args.warmup)
     dev.timing(True)
-    lat, its, r = run_steps(dev, mixed, args.steps)
+    with phase("headline steps"):
+        lat, its, r = run_steps(dev, mixed, args.steps)
     spmv_ms, spmv_n = dev.timing(False)
     Umix, _ = dev.stage_get(last=True)
     err_mixed = float(np.linalg.norm(Umix - Uref) / np.linalg.norm(Uref))

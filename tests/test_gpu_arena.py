@@ -25,6 +25,7 @@ def run(amaru, shape, n):
     dev.increment_begin(1.0, 1.0, False)
     r = dev.nr_iteration(1, dev.opts(precision="mixed64acc", precond="two_level", rtol=1e-12))
     assert r["status"] == 0
+    dev.assemble()   # mount_K with the reference's pattern at the new state
     K = dev.K_csc()
     dU, dF = dev.stage_get(last=True)
     return dev, K, dU, dF, dev.state_get()
