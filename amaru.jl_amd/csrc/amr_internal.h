@@ -67,6 +67,10 @@ struct AmrPattern {
 
 int amr_build_pattern(int nn, int64_t nnodes, int64_t ne, const int32_t *conn, AmrPattern &P);
 
+// Doubles of a model's reduction-partials buffer (m->part; layout in
+// kernels_solve.hip).
+constexpr int AMR_PART_DBL = 25600;
+
 // Device-built pattern (pattern.hip): the same arrays as amr_build_pattern,
 // made on the device from the model's conn.  Returns AMR_PATTERN_ON_HOST when
 // a node's contributions exceed what one workgroup sorts (the host builder is
@@ -197,6 +201,7 @@ struct amr_model {
     // output post-processing (recovery.hip)
     std::vector<double> Nip;        // nips*nn shape functions at the IPs
     int elem_kind = -1;             // material kind of every element, -2 if they differ
+    std::vector<int8_t> ekind;      // material kind per element (models that mix kinds)
     struct AmrRecovery *rec = nullptr;
     // mass matrix (mount_M): one value per element node pair (the block is
     // that value times I3), assembled values and survivor flags in the kv_at

@@ -503,7 +503,7 @@ extern "C" int amr_model_create(amr_model **out, int device, int etype, int nips
     TRY(dalloc(m.get(), &m->diag, (size_t)nnodes));
     TRY(dalloc(m.get(), &m->fptr, (size_t)nnodes + 1));
     TRY(dalloc(m.get(), &m->fsrc, (size_t)ne * nn));
-    TRY(dalloc(m.get(), &m->part, 24576));
+    TRY(dalloc(m.get(), &m->part, AMR_PART_DBL));
     hipStream_t s = m->stream;
     TRY(h2d(m->conn, conn, (size_t)ne * nn, s));
     // the sparsity pattern and gather maps, on the device (host builder for
@@ -555,6 +555,10 @@ extern "C" int amr_model_create(amr_model **out, int device, int etype, int nips
         const int k = mat_kind[emat[e]];
         if (e == 0) m->elem_kind = k;
         else if (k != m->elem_kind) { m->elem_kind = -2; break; }
+    }
+    if (m->elem_kind == -2) {
+        m->ekind.resize(ne);
+        for (int64_t e = 0; e < ne; e++) m->ekind[e] = (int8_t)mat_kind[emat[e]];
     }
     TRY(h2d(m->X, coords, n3, s));
     m->Xh.assign(coords, coords + n3);

@@ -132,43 +132,43 @@ __global__ void symmetrize_kernel(int32_t nc, double *__restrict__ A, const int3
 
 // ---- block Gauss-Jordan inversion (in place, row-major nc x nc) ----------
 // Step k, pivot block K = [k0, k0 + b):
-//   P    = A_KK^-1                              (gj_pivot, one wave)
+//   P    = A_KK^-1                              (gj_pivot, one workgroup)
 //   R    = P A_Kj for j outside K; C = A_iK     (gj_panel, old column panel)
 //   A_ij -= C_i R_j,  A_Kj = R_j,  A_iK = -C_i P,  A_KK = P      (gj_update)
 constexpr int GJB = 64;
 
-__global__ void __launch_bounds__(64) gj_pivot_kernel(int32_t nc, int32_t k0, int32_t b,
-                                                      const double *__restrict__ A, double *__restrict__ P,
-                                                      int32_t *__restrict__ info)
+__global__ void __launch_bounds__(1024) gj_pivot_kernel(int32_t nc, int32_t k0, int32_t b,
+                                                        const double *__restrict__ A, double *__restrict__ P,
+                                                        int32_t *__restrict__ info)
 {
-    // one wave: lane j holds column j of the pivot block in registers; the
-    // pivot row is each lane's own entry and the pivot column is read from
-    // lane p (readlane), so the 64 steps need no barrier
-    const int j = threadIdx.x;
-    double c[GJB];
-#pragma unroll
-    for (int i = 0; i < GJB; i++) c[i] = (i < b && j < b) ? A[(int64_t)(k0 + i) * nc + k0 + j] : 0.0;
+    // the pivot block in LDS; thread (i, j0) updates entries (i, j0..j0+3):
+    // each step reads the pivot row and column, then (after a barrier)
+    // writes the new values
+    __shared__ double M[GJB][GJB + 1];
+    const int i = threadIdx.x >> 4, j0 = (threadIdx.x & 15) * 4;
+    for (int k = 0; k < 4; k++) M[i][j0 + k] = (i < b && j0 + k < b) ? A[(int64_t)(k0 + i) * nc + k0 + j0 + k] : 0.0;
+    __syncthreads();
     for (int p = 0; p < b; p++) {
-        double rj = 0.0;
-#pragma unroll
-        for (int i = 0; i < GJB; i++) rj = i == p ? c[i] : rj;
-        const double piv = __shfl(rj, p);
+        const double piv = M[p][p];
         const bool ok = piv > 0.0 && std::isfinite(piv);
-        if (!ok && j == 0) info[0] = 1;
+        if (!ok && threadIdx.x == 0) info[0] = 1;
         const double ip = ok ? 1.0 / piv : 0.0;
-#pragma unroll
-        for (int i = 0; i < GJB; i++) {
-            const double ci = __shfl(c[i], p);
-            double nv;
-            if (i == p) nv = j == p ? ip : rj * ip;
-            else if (j == p) nv = -(ci * ip);
-            else nv = c[i] - ci * (rj * ip);
-            c[i] = nv;
+        const double ci = M[i][p];
+        double nv[4];
+        for (int k = 0; k < 4; k++) {
+            const int j = j0 + k;
+            const double rj = M[p][j], v = M[i][j];
+            if (i == p && j == p) nv[k] = ip;
+            else if (i == p) nv[k] = rj * ip;
+            else if (j == p) nv[k] = -(ci * ip);
+            else nv[k] = v - ci * (rj * ip);
         }
+        __syncthreads();
+        for (int k = 0; k < 4; k++) M[i][j0 + k] = nv[k];
+        __syncthreads();
     }
-#pragma unroll
-    for (int i = 0; i < GJB; i++)
-        if (i < b && j < b) P[i * GJB + j] = c[i];
+    for (int k = 0; k < 4; k++)
+        if (i < b && j0 + k < b) P[i * GJB + j0 + k] = M[i][j0 + k];
 }
 
 // one workgroup per 64-column tile t: R[:, tile] = P A_K,tile (t != k), and
@@ -340,7 +340,7 @@ int amr_coarse_setup(amr_model *m)
         const int nt = (nc + GJB - 1) / GJB;
         for (int k = 0; k < nt; k++) {
             const int b = std::min(GJB, nc - k * GJB);
-            hipLaunchKernelGGL(gj_pivot_kernel, dim3(1), dim3(64), 0, s, nc, k * GJB, b, m->Kinv, P, m->c_info);
+            hipLaunchKernelGGL(gj_pivot_kernel, dim3(1), dim3(1024), 0, s, nc, k * GJB, b, m->Kinv, P, m->c_info);
             hipLaunchKernelGGL(gj_panel_kernel, dim3((unsigned)nt), dim3(256), 0, s, nc, k, b, m->Kinv, P, R, C);
             hipLaunchKernelGGL(gj_update_kernel, dim3((unsigned)nt, (unsigned)nt), dim3(256), 0, s, nc, k, b, m->Kinv,
                                P, R, C);

@@ -46,7 +46,7 @@ namespace {
 
 constexpr int WGV = 256;  // vector kernels
 constexpr int WGS = 512;  // SpMV kernels
-constexpr int MAX_GR = 4096, MAX_GV = 512;
+constexpr int MAX_GR = 4096, MAX_GV = 1024;
 
 // scalar and control slots of one model: S_NSCAL doubles, then C_NCTL ints
 enum { S_RZ = 0, S_RR, S_BB, S_THR, S_KX, S_RES, S_ERRN, S_ERRD, S_GAP, S_NSCAL = 16 };
@@ -60,8 +60,9 @@ enum { C_DONE = 0, C_ITERS, C_BREAK, C_NAN, C_STOP, C_NCTL = 16 };
 constexpr int SEG_DBL = 32;  // doubles per model in a pool's scalar array
 static_assert(S_NSCAL + C_NCTL / 2 <= SEG_DBL, "scalar block");
 // partial buffers inside m->part
-enum { P_PQ = 0, P_RZ0 = 4096, P_RZ1 = 4608, P_RR0 = 5120, P_RR1 = 5632, P_CY0 = 6144, P_CY1 = 6656, P_A = 8192,
-       P_B = 12288, P_C = 16384, P_D = 20480, P_END = 24576 };
+enum { P_PQ = 0, P_RZ0 = 4096, P_RZ1 = 5120, P_RR0 = 6144, P_RR1 = 7168, P_CY0 = 8192, P_CY1 = 8704, P_A = 9216,
+       P_B = 13312, P_C = 17408, P_D = 21504, P_END = 25600 };
+static_assert(P_END == AMR_PART_DBL, "partials layout");
 constexpr int MAX_CW = 512;  // coarse-product workgroups per model (4 coarse rows each)
 
 // Pointers taken from a descriptor are generic; device buffers are all in
@@ -562,6 +563,23 @@ __global__ void __launch_bounds__(WGV) pcg_update_kernel(const SegDesc *__restri
     int32_t *ctl = d.ctl;
     if (ctl[C_DONE]) return;
     const int pi = (ctl[C_ITERS] - 1) & 1;
+    auto *__restrict__ x = glw(d.x);
+    auto *__restrict__ r = glw(d.r);
+    auto *__restrict__ z = glw(d.z);
+    const auto *__restrict__ p = gl(d.p);
+    const auto *__restrict__ q = gl(d.q);
+    const int64_t stride = (int64_t)d.nvblk * WGV;
+    // the first node's operands are loaded before the partial sums, so that
+    // their latency overlaps the reductions
+    int64_t nd = (int64_t)ve.y * WGV + threadIdx.x;
+    double pv[3] = {0.0, 0.0, 0.0}, qv[3] = {0.0, 0.0, 0.0}, xv[3] = {0.0, 0.0, 0.0}, rv0[3] = {0.0, 0.0, 0.0};
+    if (nd < d.nnodes)
+        for (int c = 0; c < 3; c++) {
+            pv[c] = p[nd * 3 + c];
+            qv[c] = q[nd * 3 + c];
+            xv[c] = x[nd * 3 + c];
+            rv0[c] = r[nd * 3 + c];
+        }
     const double pq = sum_partials<WGV>(d.part + P_PQ, d.ntiles);
     const double rz = sum_rz(d, 1 - pi);
     if (!(pq > 0.0) || !std::isfinite(pq)) {
@@ -573,15 +591,8 @@ __global__ void __launch_bounds__(WGV) pcg_update_kernel(const SegDesc *__restri
     }
     const double a = rz / pq;
     double rzn = 0.0, rr = 0.0;
-    auto *__restrict__ x = glw(d.x);
-    auto *__restrict__ r = glw(d.r);
-    auto *__restrict__ z = glw(d.z);
-    const auto *__restrict__ p = gl(d.p);
-    const auto *__restrict__ q = gl(d.q);
-    const int64_t stride = (int64_t)d.nvblk * WGV;
-    // two nodes per pass (nd, nd + stride), all loads issued before any
-    // store; the dot contributions keep the one-node-at-a-time order
-    auto node = [&](int64_t nd, const double pv[3], const double qv[3], const double xv[3], const double rv0[3]) {
+    // one node per pass; the dot contributions in node order
+    while (nd < d.nnodes) {
         double rv[3], zv[3];
         for (int c = 0; c < 3; c++) {
             const int64_t i = nd * 3 + c;
@@ -595,33 +606,14 @@ __global__ void __launch_bounds__(WGV) pcg_update_kernel(const SegDesc *__restri
             rzn = fma(rv[c], zv[c], rzn);
             rr = fma(rv[c], rv[c], rr);
         }
-    };
-    int64_t nd = (int64_t)ve.y * WGV + threadIdx.x;
-    for (; nd + stride < d.nnodes; nd += 2 * stride) {
-        const int64_t n1 = nd + stride;
-        double pa[3], qa[3], xa[3], ra[3], pb[3], qb[3], xb[3], rb[3];
-        for (int c = 0; c < 3; c++) {
-            pa[c] = p[nd * 3 + c];
-            qa[c] = q[nd * 3 + c];
-            xa[c] = x[nd * 3 + c];
-            ra[c] = r[nd * 3 + c];
-            pb[c] = p[n1 * 3 + c];
-            qb[c] = q[n1 * 3 + c];
-            xb[c] = x[n1 * 3 + c];
-            rb[c] = r[n1 * 3 + c];
-        }
-        node(nd, pa, qa, xa, ra);
-        node(n1, pb, qb, xb, rb);
-    }
-    if (nd < d.nnodes) {
-        double pa[3], qa[3], xa[3], ra[3];
-        for (int c = 0; c < 3; c++) {
-            pa[c] = p[nd * 3 + c];
-            qa[c] = q[nd * 3 + c];
-            xa[c] = x[nd * 3 + c];
-            ra[c] = r[nd * 3 + c];
-        }
-        node(nd, pa, qa, xa, ra);
+        nd += stride;
+        if (nd < d.nnodes)
+            for (int c = 0; c < 3; c++) {
+                pv[c] = p[nd * 3 + c];
+                qv[c] = q[nd * 3 + c];
+                xv[c] = x[nd * 3 + c];
+                rv0[c] = r[nd * 3 + c];
+            }
     }
     rzn = block_sum<WGV>(rzn);
     rr = block_sum<WGV>(rr);
@@ -639,6 +631,17 @@ __global__ void __launch_bounds__(WGV) pcg_dir_kernel(const SegDesc *__restrict_
     int32_t *ctl = d.ctl;
     if (ctl[C_DONE]) return;
     const int it = ctl[C_ITERS], pi = (it - 1) & 1;
+    auto *__restrict__ p = glw(d.p);
+    auto *__restrict__ p32 = d.wp32 ? glw(d.p32) : nullptr;
+    const auto *__restrict__ z = gl(d.z);
+    const int64_t n = d.nnodes * 3, stride = (int64_t)d.nvblk * WGV;
+    // the first dof's operands (and coarse term) before the partial sums
+    int64_t i = (int64_t)ve.y * WGV + threadIdx.x;
+    double pv = 0.0, zv = 0.0;
+    if (i < n) {
+        pv = p[i];
+        zv = d.tl ? z[i] + coarse_term(d, i / 3, (int)(i % 3)) : z[i];
+    }
     const double rzn = sum_rz(d, pi);
     const double rzo = sum_rz(d, 1 - pi);
     const double rr = sum_partials<WGV>(d.part + (pi ? P_RR1 : P_RR0), d.nvblk);
@@ -648,29 +651,15 @@ __global__ void __launch_bounds__(WGV) pcg_dir_kernel(const SegDesc *__restrict_
         d.scal[S_RZ] = rzn;
         if (rr <= d.scal[S_THR] || it >= d.maxiter || !std::isfinite(rr)) ctl[C_STOP] = 1;
     }
-    auto *__restrict__ p = glw(d.p);
-    auto *__restrict__ p32 = d.wp32 ? glw(d.p32) : nullptr;
-    const auto *__restrict__ z = gl(d.z);
-    const int64_t n = d.nnodes * 3, stride = (int64_t)d.nvblk * WGV;
-    int64_t i = (int64_t)ve.y * WGV + threadIdx.x;
-    for (; i + stride < n; i += 2 * stride) {
-        const int64_t j = i + stride;
-        const double pi0 = p[i], pj0 = p[j];
-        const double zi = d.tl ? z[i] + coarse_term(d, i / 3, (int)(i % 3)) : z[i];
-        const double zj = d.tl ? z[j] + coarse_term(d, j / 3, (int)(j % 3)) : z[j];
-        const double pn = fma(beta, pi0, zi), pm = fma(beta, pj0, zj);
-        p[i] = pn;
-        p[j] = pm;
-        if (p32) {
-            p32[i] = (float)pn;
-            p32[j] = (float)pm;
-        }
-    }
-    if (i < n) {
-        const double zi = d.tl ? z[i] + coarse_term(d, i / 3, (int)(i % 3)) : z[i];
-        const double pn = fma(beta, p[i], zi);
+    while (i < n) {
+        const double pn = fma(beta, pv, zv);
         p[i] = pn;
         if (p32) p32[i] = (float)pn;
+        i += stride;
+        if (i < n) {
+            pv = p[i];
+            zv = d.tl ? z[i] + coarse_term(d, i / 3, (int)(i % 3)) : z[i];
+        }
     }
 }
 

@@ -250,7 +250,7 @@ int amr_build_pattern_device(amr_model *m)
         return AMR_ERR_ARG;
     }
     int32_t *flags = (int32_t *)m->part, *stmp = flags + 4;
-    if ((int64_t)(24576 * 2 - 4) < scan_tmp_ints(n + 1)) {
+    if ((int64_t)(AMR_PART_DBL * 2 - 4) < scan_tmp_ints(n + 1)) {
         amr_set_error("model too large for the pattern scan");
         return AMR_ERR_ARG;
     }

@@ -19,6 +19,13 @@
 //                   better conditioned)
 //   node_eval_kernel one thread per (node, field): the patches' polynomials at
 //                   the node, summed in patch order, divided by their count.
+// A model that mixes material kinds has fields that only some elements
+// carry (the plastic strain of von Mises, the Drucker-Prager extras): as the
+// reference (fe-model.jl:614-680), each such field is fitted on the
+// subpatch of the elements that carry it and averaged over the subpatches
+// that reach a node.  The subpatches are fit units of their own, after the
+// patches; the nodes keep two patch lists, one for the fields every element
+// carries and one for the extras.
 // Every sum has a fixed order: the results do not depend on the launch.
 #include <algorithm>
 #include <cmath>
@@ -187,10 +194,10 @@ __global__ void __launch_bounds__(64) patch_fit_kernel(int nq, const int32_t *__
 #pragma unroll
         for (int r = 0; r < RPL; r++) part += v[r] * v[r];
         const double akk = __shfl(a[0][k], k, 64);
-        const double sigma = wave_sum(part) + akk * akk;
-        const double alpha = -copysign(sqrt(sigma), akk);
+        const double below = wave_sum(part);   // squares of column k below row k
+        const double alpha = -copysign(sqrt(below + akk * akk), akk);
         const double vk = akk - alpha;
-        const double vv = sigma - akk * akk + vk * vk;
+        const double vv = below + vk * vk;      // v'v of the reflector as applied
         rmax = fmax(rmax, fabs(alpha));
         if (!(fabs(alpha) > 1e-13 * rmax) || vv == 0.0) deficient = true;
         if (lane == k) v[0] = vk;
@@ -212,7 +219,111 @@ __global__ void __launch_bounds__(64) patch_fit_kernel(int nq, const int32_t *__
     }
     __syncthreads();
     if (deficient) {
-        if (lane == 0) bad[p] = 1;
+        // rank-deficient basis (small subpatches of a field some elements
+        // carry): pinv's minimum-norm fit, which depends on the basis, so it
+        // is formed as the reference forms it, on the raw coordinates.
+        // One-sided Jacobi SVD of M's columns (rows over the wave), singular
+        // values at or below eps * min(m, nt) * smax dropped (Julia's pinv
+        // tolerance); the coefficients are then written for the centred,
+        // scaled basis node_eval_kernel evaluates (an exact re-expansion).
+        __shared__ double Mr[MAXM][NT], Vr[NT][NT];
+#pragma unroll
+        for (int r = 0; r < RPL; r++) {
+            const int i = lane + 64 * r;
+            double t[NT] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+            if (i < m) {
+                const int64_t ip = (int64_t)pelem[e0 + i / nq] * nq + i % nq;
+                reg_terms(ipx[ip * 3], ipx[ip * 3 + 1], ipx[ip * 3 + 2], t);
+            }
+#pragma unroll
+            for (int c = 0; c < NT; c++) Mr[i][c] = (i < m && c < nt) ? t[c] : 0.0;
+        }
+        if (lane < NT)
+            for (int c = 0; c < NT; c++) Vr[lane][c] = lane == c ? 1.0 : 0.0;
+        __syncthreads();
+        for (int sweep = 0; sweep < 60; sweep++) {
+            bool rotated = false;
+            for (int ca = 0; ca < nt; ca++)
+                for (int cb = ca + 1; cb < nt; cb++) {
+                    double al = 0.0, be = 0.0, ga = 0.0;
+#pragma unroll
+                    for (int r = 0; r < RPL; r++) {
+                        const int i = lane + 64 * r;
+                        al += Mr[i][ca] * Mr[i][ca];
+                        be += Mr[i][cb] * Mr[i][cb];
+                        ga += Mr[i][ca] * Mr[i][cb];
+                    }
+                    al = wave_sum(al);
+                    be = wave_sum(be);
+                    ga = wave_sum(ga);
+                    if (!(fabs(ga) > 1e-15 * sqrt(al * be))) continue;
+                    rotated = true;
+                    const double zeta = (be - al) / (2.0 * ga);
+                    const double t = copysign(1.0, zeta) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+                    const double c = 1.0 / sqrt(1.0 + t * t), sn = c * t;
+#pragma unroll
+                    for (int r = 0; r < RPL; r++) {
+                        const int i = lane + 64 * r;
+                        const double ua = Mr[i][ca], ub = Mr[i][cb];
+                        Mr[i][ca] = c * ua - sn * ub;
+                        Mr[i][cb] = sn * ua + c * ub;
+                    }
+                    if (lane < NT) {
+                        const double va = Vr[lane][ca], vb = Vr[lane][cb];
+                        Vr[lane][ca] = c * va - sn * vb;
+                        Vr[lane][cb] = sn * va + c * vb;
+                    }
+                }
+            if (!rotated) break;
+        }
+        __syncthreads();
+        double s2[NT], smax = 0.0;
+#pragma unroll
+        for (int j = 0; j < NT; j++) {
+            double n2 = 0.0;
+#pragma unroll
+            for (int r = 0; r < RPL; r++) n2 += Mr[lane + 64 * r][j] * Mr[lane + 64 * r][j];
+            s2[j] = j < nt ? wave_sum(n2) : 0.0;
+            smax = fmax(smax, sqrt(s2[j]));
+        }
+        const double tol = 2.220446049250313e-16 * (double)(m < nt ? m : nt) * smax;
+        const double cx = pcen[p * 4], cy = pcen[p * 4 + 1], cz = pcen[p * 4 + 2], h = pcen[p * 4 + 3];
+        for (int f = 0; f < NF; f++) {
+            // y_j = (M_j . w) / s_j^2 (M_j = U_j s_j), a = V y
+            double y[NT];
+#pragma unroll
+            for (int j = 0; j < NT; j++) {
+                double d = 0.0;
+#pragma unroll
+                for (int r = 0; r < RPL; r++) {
+                    const int i = lane + 64 * r;
+                    if (i < m) {
+                        const int64_t ip = (int64_t)pelem[e0 + i / nq] * nq + i % nq;
+                        d += Mr[i][j] * vals[ip * NF + f];
+                    }
+                }
+                d = wave_sum(d);
+                y[j] = (j < nt && sqrt(s2[j]) > tol) ? d / s2[j] : 0.0;
+            }
+            double a[NT];
+#pragma unroll
+            for (int t = 0; t < NT; t++) {
+                double v = 0.0;
+#pragma unroll
+                for (int j = 0; j < NT; j++) v += Vr[t][j] * y[j];
+                a[t] = t < nt ? v : 0.0;
+            }
+            // x = c + h (xi) etc.: the same polynomial in the centred, scaled terms
+            const double b[NT] = {a[0] + a[1] * cx + a[2] * cy + a[3] * cz + a[4] * cx * cy + a[5] * cy * cz +
+                                      a[6] * cx * cz,
+                                  h * (a[1] + a[4] * cy + a[6] * cz), h * (a[2] + a[4] * cx + a[5] * cz),
+                                  h * (a[3] + a[5] * cy + a[6] * cx), h * h * a[4], h * h * a[5], h * h * a[6]};
+            if (lane == 0) {
+                double *cf = coef + (size_t)p * NT * NF + f;
+                for (int t = 0; t < NT; t++) cf[t * NF] = b[t];
+            }
+        }
+        if (lane == 0 && !(smax > 0.0)) bad[p] = 1;
         return;
     }
     if (lane < NF) {
@@ -238,8 +349,10 @@ __global__ void __launch_bounds__(64) patch_fit_kernel(int nq, const int32_t *__
 // One thread per (node, field): the field's value from each patch reaching
 // the node, summed in patch order, divided by the number of patches.
 __global__ void __launch_bounds__(256) node_eval_kernel(int64_t nnodes, int nq, const double *__restrict__ X,
-                                                         const int32_t *__restrict__ nptr,
-                                                         const int32_t *__restrict__ npat,
+                                                         const int32_t *__restrict__ nptr0,
+                                                         const int32_t *__restrict__ npat0,
+                                                         const int32_t *__restrict__ nptrx,
+                                                         const int32_t *__restrict__ npatx, int fsplit,
                                                          const int32_t *__restrict__ pptr,
                                                          const double *__restrict__ pcen,
                                                          const double *__restrict__ coef, double *__restrict__ out)
@@ -248,6 +361,7 @@ __global__ void __launch_bounds__(256) node_eval_kernel(int64_t nnodes, int nq, 
     if (id >= nnodes * NF) return;
     const int64_t n = id / NF;
     const int f = (int)(id - n * NF);
+    const int32_t *nptr = f < fsplit ? nptr0 : nptrx, *npat = f < fsplit ? npat0 : npatx;
     double acc = 0.0;
     const int j0 = nptr[n], j1 = nptr[n + 1];
     for (int j = j0; j < j1; j++) {
@@ -291,10 +405,12 @@ int alloc(amr_model *m, T **p, size_t n, int64_t &bytes)
 }  // namespace
 
 struct AmrRecovery {
-    int32_t npatch = 0, maxm = 0;
+    int32_t npatch = 0, maxm = 0;   // fit units: patches, then subpatches of the extra fields
+    int32_t fsplit = NF;            // fields below it use nptr/npat, the rest nptrx/npatx
     int64_t bytes = 0;
     std::vector<int32_t> reps;   // patches reaching each node
     int32_t *pptr = nullptr, *pelem = nullptr, *nptr = nullptr, *npat = nullptr, *bad = nullptr;
+    int32_t *nptrx = nullptr, *npatx = nullptr;
     double *pcen = nullptr, *ipx = nullptr, *vals = nullptr, *coef = nullptr, *nout = nullptr;
 };
 
@@ -302,7 +418,8 @@ void amr_recovery_free(amr_model *m)
 {
     AmrRecovery *r = m->rec;
     if (!r) return;
-    void *ptrs[] = {r->pptr, r->pelem, r->nptr, r->npat, r->bad, r->pcen, r->ipx, r->vals, r->coef, r->nout};
+    void *ptrs[] = {r->pptr, r->pelem, r->nptr, r->npat, r->bad,  r->pcen,
+                    r->ipx,  r->vals,  r->coef, r->nout, r->nptrx, r->npatx};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     m->dev_bytes -= r->bytes;
@@ -434,16 +551,63 @@ static int build_recovery(amr_model *m)
         pcen.insert(pcen.end(), {c[0], c[1], c[2], h > 0.0 ? h : 1.0});
         pn_patch.resize(pn_node.size(), p);
     }
+    // a model that mixes material kinds: the subpatches of the elements that
+    // carry the extra fields (those of its one plastic kind), as fit units
+    // after the patches, with their nodes in order of first appearance
+    const int32_t npatch0 = (int32_t)owner.size();
+    int32_t fsplit = NF;
+    std::vector<int32_t> px_node, px_unit;
+    if (m->elem_kind == -2) {
+        int plastic = -1;
+        for (int8_t k : m->ekind)
+            if (k != AMR_MAT_LE) {
+                if (plastic >= 0 && k != plastic) {
+                    amr_set_error("amr_nodal_recovery: von Mises and Drucker-Prager elements in one model (their "
+                                  "extra output fields share columns)");
+                    return AMR_ERR_ARG;
+                }
+                plastic = k;
+            }
+        fsplit = 16;   // the stress_strain_dict fields, carried by every element
+        std::fill(seen.begin(), seen.end(), -1);
+        for (int32_t p = 0; p < npatch0 && plastic >= 0; p++) {
+            const int32_t u = (int32_t)pptr.size() - 1;
+            const size_t before = pelem.size();
+            for (int32_t i = pptr[p]; i < pptr[p + 1]; i++) {
+                const int32_t e = pelem[i];
+                if (m->ekind[e] == plastic) pelem.push_back(e);
+            }
+            if (pelem.size() == before) continue;
+            pptr.push_back((int32_t)pelem.size());
+            for (int k = 0; k < 4; k++) pcen.push_back(pcen[(size_t)p * 4 + k]);
+            for (size_t i = before; i < pelem.size(); i++)
+                for (int a = 0; a < nn; a++) {
+                    const int32_t v = conn[(size_t)pelem[i] * nn + a];
+                    if (seen[v] == u) continue;
+                    seen[v] = u;
+                    px_node.push_back(v);
+                    px_unit.push_back(u);
+                }
+        }
+    }
     // node -> patches, ascending patch (the reference's summation order)
     AmrRecovery *r = m->rec ? m->rec : (m->rec = new AmrRecovery);   // amr_ip_values may have made it
-    r->npatch = (int32_t)owner.size();
+    r->npatch = (int32_t)pptr.size() - 1;
     r->maxm = maxm;
+    r->fsplit = fsplit;
     r->reps.assign(nnodes, 0);
     for (int32_t v : pn_node) r->reps[v]++;
     std::vector<int32_t> nptr(nnodes + 1, 0);
     for (int64_t n = 0; n < nnodes; n++) nptr[n + 1] = nptr[n] + r->reps[n];
     std::vector<int32_t> npat(pn_node.size()), fill(nptr.begin(), nptr.end() - 1);
     for (size_t i = 0; i < pn_node.size(); i++) npat[fill[pn_node[i]]++] = pn_patch[i];
+    std::vector<int32_t> nptrx(nnodes + 1, 0), npatx(px_node.size());
+    for (int32_t v : px_node) nptrx[v + 1]++;
+    for (int64_t n = 0; n < nnodes; n++) nptrx[n + 1] += nptrx[n];
+    {
+        std::vector<int32_t> fx(nptrx.begin(), nptrx.end() - 1);
+        for (size_t i = 0; i < px_node.size(); i++) npatx[fx[px_node[i]]++] = px_unit[i];
+    }
 
     const int np = r->npatch;
     int rc = AMR_OK;
@@ -451,6 +615,8 @@ static int build_recovery(amr_model *m)
     if (!rc) rc = alloc(m, &r->pelem, pelem.size(), r->bytes);
     if (!rc) rc = alloc(m, &r->nptr, nptr.size(), r->bytes);
     if (!rc) rc = alloc(m, &r->npat, npat.size(), r->bytes);
+    if (!rc) rc = alloc(m, &r->nptrx, nptrx.size(), r->bytes);
+    if (!rc) rc = alloc(m, &r->npatx, npatx.size(), r->bytes);
     if (!rc) rc = alloc(m, &r->bad, (size_t)np, r->bytes);
     if (!rc) rc = alloc(m, &r->pcen, pcen.size(), r->bytes);
     if (!rc) rc = alloc(m, &r->ipx, ipx.size(), r->bytes);
@@ -468,6 +634,7 @@ static int build_recovery(amr_model *m)
         size_t n;
     } ups[] = {{r->pptr, pptr.data(), pptr.size() * 4},   {r->pelem, pelem.data(), pelem.size() * 4},
                {r->nptr, nptr.data(), nptr.size() * 4},   {r->npat, npat.data(), npat.size() * 4},
+               {r->nptrx, nptrx.data(), nptrx.size() * 4}, {r->npatx, npatx.data(), npatx.size() * 4},
                {r->pcen, pcen.data(), pcen.size() * 8},   {r->ipx, ipx.data(), ipx.size() * 8}};
     hipError_t e = hipSuccess;
     for (const Up &u : ups)
@@ -499,11 +666,6 @@ extern "C" int amr_ip_values(amr_model *m, int committed, double *out)
 extern "C" int amr_nodal_recovery(amr_model *m, int committed, double *out, int32_t *reps)
 {
     if (!m || !out) return AMR_ERR_ARG;
-    if (m->elem_kind == -2) {
-        amr_set_error("amr_nodal_recovery: the elements use different material kinds (per-field subpatches are "
-                      "not supported)");
-        return AMR_ERR_ARG;
-    }
     AMR_HIP(hipSetDevice(m->device));
     if (!m->rec || !m->rec->pptr) TRY(build_recovery(m));
     AmrRecovery *r = m->rec;
@@ -518,6 +680,7 @@ extern "C" int amr_nodal_recovery(amr_model *m, int committed, double *out, int3
     }
     const unsigned g = (unsigned)((m->nnodes * NF + 255) / 256);
     hipLaunchKernelGGL(node_eval_kernel, dim3(g), dim3(256), 0, s, m->nnodes, m->nips, m->X, r->nptr, r->npat,
+                       r->nptrx, r->npatx, r->fsplit,
                        r->pptr, r->pcen, r->coef, r->nout);
     AMR_HIP(hipGetLastError());
     AMR_HIP(hipMemcpyAsync(out, r->nout, sizeof(double) * NF * m->nnodes, hipMemcpyDeviceToHost, s));

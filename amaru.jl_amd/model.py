@@ -435,9 +435,13 @@ def output_fields(model):
     (stress_strain_dict, src/tools/tensors.jl:190-213, plus the material's
     own entries)."""
     kinds = {type(m).__name__ for m in model.mats}
-    if len(kinds) != 1:
-        raise ValueError("output fields: the model mixes material kinds")
-    return list(enumerate(OUTPUT_FIELDS)) + list(_EXTRA_FIELDS[kinds.pop()])
+    plastic = kinds - {"LinearElastic"}
+    if len(plastic) > 1:
+        raise ValueError("output fields: von Mises and Drucker-Prager in one model")
+    # a model that mixes kinds: the stress_strain_dict fields of every
+    # element, then the plastic kind's extras (recovered on the subpatches of
+    # the elements that carry them)
+    return list(enumerate(OUTPUT_FIELDS)) + list(_EXTRA_FIELDS[plastic.pop() if plastic else "LinearElastic"])
 
 
 class DeviceModel:

@@ -371,7 +371,7 @@ def estimate_device_bytes(model):
     per_node = 888
     per_elem = 4 * nn + 4 + 72 * nn * nn + 24 * nn + 4 + 4 * nn + 224 * nips + 4 * nn * nn
     per_blk = 4 + 8 + 72 + 36 + 9
-    return int(per_node * nnodes + per_elem * ne + per_blk * (nblk + 64) + 24576 * 8 + 4096)
+    return int(per_node * nnodes + per_elem * ne + per_blk * (nblk + 64) + 25600 * 8 + 4096)
 
 
 class Request:

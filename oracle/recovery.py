@@ -110,29 +110,43 @@ def patches(conn, etype, nnodes):
     return inner
 
 
-def nodal_patch_recovery(X, conn, etype, W, nips=0):
+def nodal_patch_recovery(X, conn, etype, W, nips=0, has=None):
     """(nnodes, nf) recovered nodal values of the IP values ``W`` (ne*nips,
-    nf), and the number of patches that reached each node."""
+    nf), and the number of patches that reached each node.
+
+    ``has`` (ne, nf) marks the fields each element carries (default: all).
+    A field is fitted on the subpatch of the patch's elements that carry it
+    and averaged over the subpatches that reach a node (fe-model.jl:614-680:
+    ``subpatch``, ``V_reps[node, field]``)."""
     nnodes = len(X)
     R, _ = rule(etype, nips)
     nq = len(R)
+    nf = W.shape[1]
     XI = ip_coords(X, conn, etype, nips)
-    V = np.zeros((nnodes, W.shape[1]))
-    reps = np.zeros(nnodes, dtype=np.int64)
+    if has is None:
+        has = np.ones((len(conn), nf), dtype=bool)
+    V = np.zeros((nnodes, nf))
+    reps = np.zeros((nnodes, nf), dtype=np.int64)
     for patch in patches(conn, etype, nnodes):
         if not patch:
             continue
-        ips = np.concatenate([np.arange(e * nq, (e + 1) * nq) for e in patch])
-        nodes = list(dict.fromkeys(int(n) for e in patch for n in conn[e]))
-        m = len(ips)
-        nterms = 7 if m >= 7 else 4 if m >= 4 else 1
-        M = reg_terms(XI[ips], nterms)
-        invM = np.linalg.pinv(M, rcond=EPS * min(M.shape))
-        A = invM @ W[ips]
-        Vp = reg_terms(X[nodes], nterms) @ A
-        V[nodes] += Vp
-        reps[nodes] += 1
+        last = None
+        for f in range(nf):
+            sub = [e for e in patch if has[e, f]]
+            if not sub:
+                continue
+            if sub != last:
+                last = sub
+                ips = np.concatenate([np.arange(e * nq, (e + 1) * nq) for e in sub])
+                nodes = list(dict.fromkeys(int(n) for e in sub for n in conn[e]))
+                m = len(ips)
+                nterms = 7 if m >= 7 else 4 if m >= 4 else 1
+                M = reg_terms(XI[ips], nterms)
+                invM = np.linalg.pinv(M, rcond=EPS * min(M.shape))
+                N = reg_terms(X[nodes], nterms)
+            V[nodes, f] += N @ (invM @ W[ips, f])
+            reps[nodes, f] += 1
     with np.errstate(invalid="ignore", divide="ignore"):
-        out = V / reps[:, None]
+        out = V / reps
     out[np.isnan(out)] = 0.0
-    return out, reps
+    return out, reps[:, 0]

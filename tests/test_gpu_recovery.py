@@ -125,13 +125,43 @@ def test_node_fields_after_a_solve(amaru, tmp_path):
     dev.close()
 
 
-def test_mixed_material_kinds_are_refused(amaru):
+@pytest.mark.parametrize("shape,plastic", [("HEX8", "VM"), ("TET10", "VM"), ("TET10", "DP"), ("HEX20", "DP")])
+def test_mixed_material_kinds_match_oracle(amaru, shape, plastic):
+    """Linear elastic and plastic elements in one model: the stress and
+    strain fields are fitted on whole patches, the plastic kind's extras on
+    the subpatches of the plastic elements (fe-model.jl:614-680), with pinv's
+    minimum-norm fit where a subpatch's basis is rank-deficient (small TET10
+    subpatches at the interface)."""
+    mesh = amaru.Mesh(amaru.Block([[0, 0, 0], [2.0, 1.2, 0.9]], nx=4, ny=3, nz=2, cellshape=shape))
+    model = amaru.FEModel(mesh, [("all", amaru.MechSolid, material(amaru, "LE")),
+                                 ("x>=1", amaru.MechSolid, material(amaru, plastic))])
+    model.configure_dofs([("x==0", amaru.SurfaceBC(ux=0, uy=0, uz=0))])
+    dev = amaru.DeviceModel(model)
+    nip = model.ne * model.nips
+    st = random_states(nip, 5)
+    dev.state_set(st)
+    ekind = np.array([KIND["LE"] if type(model.mats[k]).__name__ == "LinearElastic" else KIND[plastic]
+                      for k in model.emat])
+    assert (ekind == 0).any() and (ekind != 0).any()
+    Vo = R.ip_values(st, np.repeat(ekind, model.nips))
+    has = np.ones((model.ne, Vo.shape[1]), bool)
+    has[:, 16:] = (ekind != 0)[:, None]
+    N, reps = dev.nodal_values()
+    No, repso = R.nodal_patch_recovery(model.coords, model.conn, OSHAPE[shape], Vo, 0, has=has)
+    assert np.array_equal(reps, repso)
+    scale = np.maximum(np.abs(No).max(0), 1e-300)
+    err = (np.abs(N - No) / scale).max()
+    assert err <= 1e-9, f"max scaled difference {err:.3e}"
+    dev.close()
+
+
+def test_von_mises_with_drucker_prager_is_refused(amaru):
     mesh = amaru.Mesh(amaru.Block([[0, 0, 0], [1, 1, 1]], nx=2, ny=1, nz=1, cellshape="HEX8"))
-    model = amaru.FEModel(mesh, [("all", amaru.MechSolid, amaru.LinearElastic(E=1e9, nu=0.3)),
-                                 ("x>=0.5", amaru.MechSolid, amaru.VonMises(E=1e9, nu=0.3, fy=1e6))])
+    model = amaru.FEModel(mesh, [("all", amaru.MechSolid, material(amaru, "VM")),
+                                 ("x>=0.5", amaru.MechSolid, material(amaru, "DP"))])
     model.configure_dofs([("x==0", amaru.SurfaceBC(ux=0, uy=0, uz=0))])
     dev = amaru.DeviceModel(model)
     dev.ip_values()   # per-IP values need no patches
-    with pytest.raises(amaru.AmaruError, match="material kinds"):
+    with pytest.raises(amaru.AmaruError, match="share columns"):
         dev.nodal_values()
     dev.close()

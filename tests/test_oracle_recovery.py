@@ -130,3 +130,42 @@ def test_recovery_patch_counts():
     _, reps = R.nodal_patch_recovery(X, conn, HEX8, np.zeros((len(conn) * 8, 1)))
     layer = np.rint(X[:, 0] * 3).astype(int)
     assert np.array_equal(reps, np.array([4, 8, 8, 4])[layer])
+
+
+@pytest.mark.parametrize("etype,n,nips", [(HEX8, (4, 3, 2), 0), (TET10, (3, 2, 2), 0), (HEX20, (3, 2, 2), 8)])
+def test_subpatches_of_fields_some_elements_carry(etype, n, nips):
+    """A field carried only by the elements with x-centre > 1 (fe-model.jl:
+    614-680): fitted on the subpatch of those elements, it reproduces a
+    linear field exactly at every node that only full-rank subpatches reach
+    (a rank-deficient one takes pinv's minimum-norm fit), and is 0 at the
+    nodes no subpatch reaches; a field every element carries is unchanged by
+    the split.  Fitting the whole patch (zeros included) would break both."""
+    X, conn = fem.block_mesh([0, 0, 0], [2.0, 1.2, 0.9], *n, etype)
+    XI = R.ip_coords(X, conn, etype, nips)
+    nq = len(XI) // len(conn)
+    right = X[conn].mean(1)[:, 0] > 1.0
+    lin = lambda P: 0.5 + 2 * P[:, 0] - P[:, 1] + 3 * P[:, 2]  # noqa: E731
+    carry = np.repeat(right, nq)
+    W = np.stack([poly(XI), np.where(carry, lin(XI), 0.0)], 1)
+    has = np.stack([np.ones(len(conn), bool), right], 1)
+    V, reps = R.nodal_patch_recovery(X, conn, etype, W, nips, has=has)
+    V0, reps0 = R.nodal_patch_recovery(X, conn, etype, W[:, :1], nips)
+    assert np.array_equal(reps, reps0) and np.array_equal(V[:, 0], V0[:, 0])
+    # the nodes of right elements in patches that hold one; those reached by
+    # a rank-deficient subpatch
+    reached = np.zeros(len(X), bool)
+    deficient = np.zeros(len(X), bool)
+    for patch in R.patches(conn, etype, len(X)):
+        sub = [e for e in patch if right[e]]
+        if not sub:
+            continue
+        ips = np.concatenate([np.arange(e * nq, (e + 1) * nq) for e in sub])
+        nt = 7 if len(ips) >= 7 else 4 if len(ips) >= 4 else 1
+        nodes = np.unique(conn[sub])
+        reached[nodes] = True
+        if np.linalg.matrix_rank(R.reg_terms(XI[ips], nt)) < nt:
+            deficient[nodes] = True
+    full = reached & ~deficient
+    assert full.sum() >= 10 and not reached.all()
+    assert np.allclose(V[full, 1], lin(X[full]), rtol=0, atol=1e-11 * np.abs(lin(X)).max())
+    assert np.all(V[~reached, 1] == 0.0)
