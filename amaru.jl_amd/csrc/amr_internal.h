@@ -182,7 +182,7 @@ struct amr_model {
     int fin_rc = 0;
     amr_nr_info fin_info{};
     // stage_begin's host vectors stay alive until their copies are done
-    std::vector<double> sb_host[2];
+    double *sb_pin = nullptr;  // pinned host staging of the stage targets (2 x 3 nnodes)
     hipEvent_t sb_event = nullptr;
     hipEvent_t ev_handoff = nullptr;   // orders the switch between its own and the pool's stream
     hipEvent_t fin_event = nullptr;    // end of the queued NR finish (its read-back)

@@ -358,7 +358,11 @@ static void free_model(amr_model *m)
     for (hipEvent_t e : m->ev)
         if (e) (void)hipEventDestroy(e);
     if (m->fin_h) (void)hipHostFree(m->fin_h);
-    if (m->sb_event) (void)hipEventDestroy(m->sb_event);
+    if (m->sb_event) {
+        (void)hipEventSynchronize(m->sb_event);
+        (void)hipEventDestroy(m->sb_event);
+    }
+    if (m->sb_pin) (void)hipHostFree(m->sb_pin);
     if (m->ev_handoff) (void)hipEventDestroy(m->ev_handoff);
     if (m->fin_event) (void)hipEventDestroy(m->fin_event);
     if (m->own_stream) (void)hipStreamDestroy(m->own_stream);
@@ -639,6 +643,28 @@ static std::mutex g_stage_mu;
 static double *g_stage[64] = {};
 static size_t g_stage_cap[64] = {};
 
+// Read-backs of result vectors go through pinned host memory (one buffer
+// per device, grown as needed; a copy into pageable memory takes the
+// runtime's slower path).  The caller holds g_pin_mu until it has copied
+// the data out.
+static std::mutex g_pin_mu;
+static double *g_pin[64] = {};
+static size_t g_pin_cap[64] = {};
+
+static int pinned_buffer(int device, size_t count, double **out)
+{
+    if (g_pin_cap[device] < count) {
+        if (g_pin[device]) AMR_HIP(hipHostFree(g_pin[device]));
+        g_pin[device] = nullptr;
+        g_pin_cap[device] = 0;
+        const size_t cap = std::max(count, (size_t)1 << 16);
+        AMR_HIP(hipHostMalloc((void **)&g_pin[device], cap * sizeof(double), hipHostMallocDefault));
+        g_pin_cap[device] = cap;
+    }
+    *out = g_pin[device];
+    return AMR_OK;
+}
+
 extern "C" int amr_state_get(amr_model *m, double *st, int committed)
 {
     if (!m || !st) return AMR_ERR_ARG;
@@ -661,8 +687,12 @@ extern "C" int amr_state_get(amr_model *m, double *st, int committed)
     hipLaunchKernelGGL(state_aos_kernel, dim3(g), dim3(256), 0, m->stream, n, committed ? m->st_com : m->st_cur,
                        buf);
     AMR_HIP(hipGetLastError());
-    TRY(d2h(st, buf, cnt, m->stream));
+    std::lock_guard<std::mutex> lp(g_pin_mu);
+    double *pin;
+    TRY(pinned_buffer(m->device, cnt, &pin));
+    TRY(d2h(pin, buf, cnt, m->stream));
     AMR_HIP(hipStreamSynchronize(m->stream));
+    std::memcpy(st, pin, cnt * sizeof(double));
     return AMR_OK;
 }
 
@@ -931,14 +961,18 @@ extern "C" int amr_get_M_csc(amr_model *m, int64_t *nnz, int64_t *colptr, int32_
 
 // ------------------------------------------------------- solve_system!
 
-static void eq_to_node(const amr_model *m, const double *eqv, std::vector<double> &node)
+static void eq_to_node(const amr_model *m, const double *eqv, double *node)
 {
-    node.assign(m->nnodes * 3, 0.0);
-    for (int64_t k = 0; k < m->nnodes * 3; k++)
-        if (m->eq[k] >= 0) node[k] = eqv[m->eq[k]];
+    for (int64_t k = 0; k < m->nnodes * 3; k++) node[k] = m->eq[k] >= 0 ? eqv[m->eq[k]] : 0.0;
 }
 
-static void node_to_eq(const amr_model *m, const std::vector<double> &node, double *eqv)
+static void eq_to_node(const amr_model *m, const double *eqv, std::vector<double> &node)
+{
+    node.resize(m->nnodes * 3);
+    eq_to_node(m, eqv, node.data());
+}
+
+static void node_to_eq(const amr_model *m, const double *node, double *eqv)
 {
     for (int64_t k = 0; k < m->nnodes * 3; k++)
         if (m->eq[k] >= 0) eqv[m->eq[k]] = node[k];
@@ -1056,7 +1090,7 @@ extern "C" int amr_update_state(amr_model *m, const double *dUt, double *dFin)
     TRY(d2h(st.data(), m->estat, (size_t)m->ne, m->stream));
     AMR_HIP(hipStreamSynchronize(m->stream));
     std::fill(dFin, dFin + m->ndofs, 0.0);
-    node_to_eq(m, fn, dFin);
+    node_to_eq(m, fn.data(), dFin);
     for (int64_t e = 0; e < m->ne; e++)
         if (st[e]) {
             if (st[e] == 2) {
@@ -1083,14 +1117,17 @@ extern "C" int amr_stage_begin(amr_model *m, const double *Uex, const double *Fe
     const size_t n = (size_t)m->nnodes * 3;
     for (double *&p : m->sv)
         if (!p) TRY(dalloc(m, &p, n));
-    // the host vectors live in the model until the copies from them are done
-    // (waited for here before they are overwritten by the next stage)
-    if (!m->sb_event) AMR_HIP(hipEventCreateWithFlags(&m->sb_event, hipEventDisableTiming));
-    else AMR_HIP(hipEventSynchronize(m->sb_event));
-    eq_to_node(m, Uex, m->sb_host[0]);
-    eq_to_node(m, Fex, m->sb_host[1]);
-    TRY(h2d(m->sv[SV_UEX], m->sb_host[0].data(), n, m->stream));
-    TRY(h2d(m->sv[SV_FEX], m->sb_host[1].data(), n, m->stream));
+    // the pinned host buffer lives in the model until the copies from it are
+    // done (waited for here before the next stage overwrites it)
+    if (!m->sb_event) {
+        AMR_HIP(hipEventCreateWithFlags(&m->sb_event, hipEventDisableTiming));
+        AMR_HIP(hipHostMalloc((void **)&m->sb_pin, 2 * n * sizeof(double), hipHostMallocDefault));
+    } else
+        AMR_HIP(hipEventSynchronize(m->sb_event));
+    eq_to_node(m, Uex, m->sb_pin);
+    eq_to_node(m, Fex, m->sb_pin + n);
+    TRY(h2d(m->sv[SV_UEX], m->sb_pin, n, m->stream));
+    TRY(h2d(m->sv[SV_FEX], m->sb_pin + n, n, m->stream));
     AMR_HIP(hipEventRecord(m->sb_event, m->stream));
     for (int k : {SV_U, SV_F, SV_RC, SV_DUA, SV_DUI, SV_DFIN, SV_R, SV_DFEX}) TRY(amr_vec_fill(m, m->sv[k], 0.0));
     return AMR_OK;
@@ -1363,12 +1400,14 @@ extern "C" int amr_stage_get(amr_model *m, double *U, double *F, int last)
     if (!m || !m->sv[SV_U]) return AMR_ERR_ARG;
     AMR_HIP(hipSetDevice(m->device));
     const size_t n = (size_t)m->nnodes * 3;
-    std::vector<double> a(n), b(n);
-    TRY(d2h(a.data(), m->sv[last ? SV_DUA : SV_U], n, m->stream));
-    TRY(d2h(b.data(), m->sv[last ? SV_DFIN : SV_F], n, m->stream));
+    std::lock_guard<std::mutex> lp(g_pin_mu);
+    double *pin;
+    TRY(pinned_buffer(m->device, 2 * n, &pin));
+    TRY(d2h(pin, m->sv[last ? SV_DUA : SV_U], n, m->stream));
+    TRY(d2h(pin + n, m->sv[last ? SV_DFIN : SV_F], n, m->stream));
     AMR_HIP(hipStreamSynchronize(m->stream));
-    if (U) node_to_eq(m, a, U);
-    if (F) node_to_eq(m, b, F);
+    if (U) node_to_eq(m, pin, U);
+    if (F) node_to_eq(m, pin + n, F);
     return AMR_OK;
 }
 

@@ -246,7 +246,8 @@ def batched_leg(am, args):
         pool = P.PoolSet(args.batch_pools, 0, max_members=len(anas), check_every=32)
         pool.timing(True)
         t2 = time.perf_counter()
-        out = P.run_analyses(anas, pool=pool, precision=prec, precond=pc, pcg_rtol=args.rtol)
+        with phase(f"batched {key}"):
+            out = P.run_analyses(anas, pool=pool, precision=prec, precond=pc, pcg_rtol=args.rtol)
         t_b = time.perf_counter() - t2
         ms, nsp = pool.timing(False)
         if not all(ok for ok, _ in out):
