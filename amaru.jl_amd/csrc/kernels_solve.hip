@@ -49,7 +49,9 @@ constexpr int WGS = 512;  // SpMV kernels
 constexpr int MAX_GR = 4096, MAX_GV = 1024;
 
 // scalar and control slots of one model: S_NSCAL doubles, then C_NCTL ints
-enum { S_RZ = 0, S_RR, S_BB, S_THR, S_KX, S_RES, S_ERRN, S_ERRD, S_GAP, S_NSCAL = 16 };
+// S_RZP0/1: r.z published per parity slot (the sum of that slot's partials)
+enum { S_RZ = 0, S_RR, S_BB, S_THR, S_KX, S_RES, S_ERRN, S_ERRD, S_GAP, S_RZP0, S_RZP1, S_NSCAL = 16 };
+__device__ inline int rz_pub(int slot) { return slot ? S_RZP1 : S_RZP0; }
 // C_DONE is read by every workgroup at kernel entry and written only where
 // all workgroups of the model in the running launch agree on it
 // (breakdown); the convergence test writes C_STOP, which the next SpMV
@@ -476,9 +478,10 @@ __global__ void __launch_bounds__(WGS) residual_kernel(const SegDesc *__restrict
 //   pcg_spmv:   q = A p, partial p.q                          -> P_PQ
 //   pcg_update: a = rz_k / pq; x += a p; r -= a q; z = M r;
 //               partials rz_{k+1}, rr                         -> P_RZ[pi], P_RR[pi]
-//   pcg_dir:    b = rz_{k+1} / rz_k; p = z + b p; convergence test
-// rz_k is the sum of P_RZ[1-pi] (written by iteration k-1, by the
-// initialisation, or by a residual replacement).
+//   pcg_dir:    b = rz_{k+1} / rz_k; p = z + b p; convergence test;
+//               publishes rz_{k+1} in scalar slot rz_pub(pi)
+// rz_k is the sum of P_RZ[1-pi], published in rz_pub(1-pi) by iteration
+// k-1's pcg_dir, by the initialisation or by a residual replacement.
 
 template <typename VT>
 __device__ inline const VT *seg_vals(const SegDesc &d);
@@ -581,7 +584,7 @@ __global__ void __launch_bounds__(WGV) pcg_update_kernel(const SegDesc *__restri
             rv0[c] = r[nd * 3 + c];
         }
     const double pq = sum_partials<WGV>(d.part + P_PQ, d.ntiles);
-    const double rz = sum_rz(d, 1 - pi);
+    const double rz = d.scal[rz_pub(1 - pi)];
     if (!(pq > 0.0) || !std::isfinite(pq)) {
         if (ve.y == 0 && threadIdx.x == 0) {
             ctl[C_BREAK] = 1;
@@ -642,14 +645,19 @@ __global__ void __launch_bounds__(WGV) pcg_dir_kernel(const SegDesc *__restrict_
         pv = p[i];
         zv = d.tl ? z[i] + coarse_term(d, i / 3, (int)(i % 3)) : z[i];
     }
+    // rz of the previous iteration was published in the other parity slot;
+    // the first workgroup publishes this one's and tests convergence
     const double rzn = sum_rz(d, pi);
-    const double rzo = sum_rz(d, 1 - pi);
-    const double rr = sum_partials<WGV>(d.part + (pi ? P_RR1 : P_RR0), d.nvblk);
+    const double rzo = d.scal[rz_pub(1 - pi)];
     const double beta = rzn / rzo;
-    if (ve.y == 0 && threadIdx.x == 0) {
-        d.scal[S_RR] = rr;
-        d.scal[S_RZ] = rzn;
-        if (rr <= d.scal[S_THR] || it >= d.maxiter || !std::isfinite(rr)) ctl[C_STOP] = 1;
+    if (ve.y == 0) {
+        const double rr = sum_partials<WGV>(d.part + (pi ? P_RR1 : P_RR0), d.nvblk);
+        if (threadIdx.x == 0) {
+            d.scal[S_RR] = rr;
+            d.scal[S_RZ] = rzn;
+            d.scal[rz_pub(pi)] = rzn;
+            if (rr <= d.scal[S_THR] || it >= d.maxiter || !std::isfinite(rr)) ctl[C_STOP] = 1;
+        }
     }
     while (i < n) {
         const double pn = fma(beta, pv, zv);
@@ -814,6 +822,7 @@ __global__ void __launch_bounds__(WGV) pcg_finalize_kernel(const SegDesc *__rest
         const double rr = sum_partials<WGV>(d.part + P_RR1, d.nvblk);
         if (threadIdx.x == 0) {
             scal[S_RZ] = rz;
+            scal[rz_pub(1)] = rz;
             scal[S_RR] = rr;
             scal[S_BB] = rr;
             scal[S_KX] = 0.0;
@@ -840,6 +849,7 @@ __global__ void __launch_bounds__(WGV) pcg_finalize_kernel(const SegDesc *__rest
         const double rz = sum_rz(d, slot_pi);
         if (threadIdx.x == 0) {
             scal[S_RZ] = rz;
+            scal[rz_pub(slot_pi)] = rz;
             ctl[C_DONE] = 0;
             ctl[C_STOP] = 0;
             ctl[C_BREAK] = 0;
