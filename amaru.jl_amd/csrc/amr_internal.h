@@ -248,10 +248,24 @@ int amr_launch_mass(amr_model *m);  // Me from Nd, rho_d; then assembly into Mv,
 int amr_launch_combine_km(amr_model *m, double a, double b);  // K := a K + b M, preconditioners refreshed
 int amr_launch_combine_kk(amr_model *m, double a1, const double *K1, double a2);  // K := a1 K1 + a2 K
 
-// two-level coarse space (coarse.hip): build on first use, refresh Kc^-1
-// after every assembly; queued on the model's stream
-int amr_coarse_setup(amr_model *m);
-int amr_coarse_build(amr_model *m);  // the index structures only (host work, one wait)
+// Pinned host buffer and its device copy for the uploads of one stream: the
+// host side is rewritten only after the previous upload from it completed.
+struct AmrStaging {
+    char *d = nullptr, *h = nullptr;
+    size_t cap = 0;
+    hipEvent_t ev = nullptr;  // recorded after the last upload
+    bool pending = false;
+};
+void amr_staging_free(AmrStaging &s);
+
+// two-level coarse space (coarse.hip): the index structures are built on
+// first use (host work, one wait); Kc^-1 is formed again when
+// amr_coarse_due says so, for any number of models at once
+// (amr_coarse_form: one launch per phase and block step for all of them,
+// queued on stream s, after which their coarse operators are current)
+int amr_coarse_build(amr_model *m);
+bool amr_coarse_due(const amr_model *m);
+int amr_coarse_form(hipStream_t s, amr_model *const *ms, int n, AmrStaging &stg);
 void amr_coarse_free(amr_model *m);
 
 // output post-processing (recovery.hip)
@@ -279,6 +293,7 @@ int amr_pool_step_members(const amr_pool *p);  // members of the step in flight 
 int amr_pool_step_query(amr_pool *p, int *done);
 int amr_pool_set_timing(amr_pool *p, int enable, double *ms, int64_t *n);
 int amr_pool_graphs(amr_pool *p, int enable, int64_t *captures, int64_t *replays);
+int amr_pool_coarse_counts(amr_pool *p, int64_t *batches, int64_t *models);
 int amr_spmv_full(amr_model *m, const double *x, double *y, int mask_mode);
 int amr_spmv_vals(amr_model *m, const double *vals /* kv_at layout */, const double *x, double *y, int mask_mode);
 int amr_choose_lanes(amr_model *m);

@@ -16,12 +16,15 @@
 // Gauss-Jordan elimination without pivoting (Kc is symmetric positive
 // definite, so every pivot block is), 64x64 pivot blocks, three kernels per
 // block step, every sum in a fixed order: the inverse is bit-reproducible
-// whatever else runs on the device.  It is symmetrised by mirroring its upper
+// whatever else runs on the device.  Every kernel works through a list of
+// (model, index) entries, so the coarse operators of all models joining a
+// pool at one step are formed by one launch per phase and block step.  It is symmetrised by mirroring its upper
 // triangle; each PCG iteration applies Kc^-1 as a dense product.  A coarse
 // matrix with a pivot that is not positive (degenerate aggregates) is
 // replaced by zero, which leaves plain Jacobi.
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 
 #include "amr_internal.h"
 
@@ -44,24 +47,36 @@ __device__ inline void rigid_modes(const double d[3], const double mk[3], double
         for (int q = 0; q < 6; q++) R[r][q] *= mk[r];
 }
 
-__global__ void __launch_bounds__(256) galerkin_pairs_kernel(int64_t npairs, const int32_t *__restrict__ prow,
-                                                             const int32_t *__restrict__ pagg,
-                                                             const int32_t *__restrict__ agg,
-                                                             const double *__restrict__ cent,
-                                                             const double *__restrict__ X,
-                                                             const double *__restrict__ mask,
-                                                             const int32_t *__restrict__ rowptr,
-                                                             const int32_t *__restrict__ col,
-                                                             const double *__restrict__ Kv, double *__restrict__ out)
+// One model's coarse formation, as the batched kernels see it.  Every
+// kernel below takes the jobs and a work list of (job, index) entries, so
+// one launch serves any number of models; each entry does the same
+// arithmetic as a launch for its model alone would.
+struct CoarseJob {
+    int32_t nc, nt, ncb;
+    int64_t npairs;
+    const int32_t *prow, *pagg, *agg, *rowptr, *col, *cba, *cbb, *cbptr, *cbslot;
+    const double *cent, *X, *mask, *Kv;
+    double *pair, *Kc, *P, *R, *C;
+    int32_t *info;
+};
+
+// entry (job, b): pairs s = 256 b + thread.  Pair s = (block row i,
+// aggregate b): U = R_i^T sum_{j in row i, agg(j) = b} K_ij R_j, 36 values
+__global__ void __launch_bounds__(256) galerkin_pairs_kernel(const CoarseJob *__restrict__ J,
+                                                             const int2 *__restrict__ wl)
 {
-    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= npairs) return;
-    const int32_t i = prow[s], b = pagg[s];
+    const int2 e = wl[blockIdx.x];
+    const CoarseJob &q = J[e.x];
+    const int64_t s = (int64_t)e.y * 256 + threadIdx.x;
+    if (s >= q.npairs) return;
+    const int32_t *__restrict__ agg = q.agg;
+    const double *__restrict__ X = q.X, *__restrict__ cent = q.cent, *__restrict__ mask = q.mask;
+    const int32_t i = q.prow[s], b = q.pagg[s];
     double W[3][6];
     for (int r = 0; r < 3; r++)
-        for (int q = 0; q < 6; q++) W[r][q] = 0.0;
-    for (int32_t k = rowptr[i]; k < rowptr[i + 1]; k++) {
-        const int32_t j = col[k];
+        for (int c = 0; c < 6; c++) W[r][c] = 0.0;
+    for (int32_t k = q.rowptr[i]; k < q.rowptr[i + 1]; k++) {
+        const int32_t j = q.col[k];
         if (agg[j] != b) continue;
         double d[3], mk[3], R[3][6], Kb[9];
         for (int c = 0; c < 3; c++) {
@@ -69,10 +84,10 @@ __global__ void __launch_bounds__(256) galerkin_pairs_kernel(int64_t npairs, con
             mk[c] = mask[(int64_t)j * 3 + c];
         }
         rigid_modes(d, mk, R);
-        for (int e = 0; e < 9; e++) Kb[e] = Kv[kv_at(k, e)];
+        for (int t = 0; t < 9; t++) Kb[t] = q.Kv[kv_at(k, t)];
         for (int r = 0; r < 3; r++)
-            for (int q = 0; q < 6; q++)
-                W[r][q] = fma(Kb[r * 3 + 2], R[2][q], fma(Kb[r * 3 + 1], R[1][q], fma(Kb[r * 3], R[0][q], W[r][q])));
+            for (int c = 0; c < 6; c++)
+                W[r][c] = fma(Kb[r * 3 + 2], R[2][c], fma(Kb[r * 3 + 1], R[1][c], fma(Kb[r * 3], R[0][c], W[r][c])));
     }
     const int32_t a = agg[i];
     double d[3], mk[3], R[3][6];
@@ -82,52 +97,63 @@ __global__ void __launch_bounds__(256) galerkin_pairs_kernel(int64_t npairs, con
     }
     rigid_modes(d, mk, R);
     for (int p = 0; p < 6; p++)
-        for (int q = 0; q < 6; q++)
-            out[s * 36 + p * 6 + q] = fma(R[2][p], W[2][q], fma(R[1][p], W[1][q], R[0][p] * W[0][q]));
+        for (int c = 0; c < 6; c++)
+            q.pair[s * 36 + p * 6 + c] = fma(R[2][p], W[2][c], fma(R[1][p], W[1][c], R[0][p] * W[0][c]));
 }
 
-__global__ void __launch_bounds__(256) galerkin_blocks_kernel(int32_t ncb, int32_t nc, const int32_t *__restrict__ cba,
-                                                              const int32_t *__restrict__ cbb,
-                                                              const int32_t *__restrict__ cbptr,
-                                                              const int32_t *__restrict__ cbslot,
-                                                              const double *__restrict__ pair, double *__restrict__ Kc)
+// entry (job, b): t = 256 b + thread = (coarse block cb, entry): Kc_ab is
+// the sum of its pairs in ascending row
+__global__ void __launch_bounds__(256) galerkin_blocks_kernel(const CoarseJob *__restrict__ J,
+                                                              const int2 *__restrict__ wl)
 {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (int64_t)ncb * 36) return;
-    const int32_t cb = (int32_t)(t / 36), e = (int32_t)(t % 36);
+    const int2 e = wl[blockIdx.x];
+    const CoarseJob &q = J[e.x];
+    const int64_t t = (int64_t)e.y * 256 + threadIdx.x;
+    if (t >= (int64_t)q.ncb * 36) return;
+    const int32_t cb = (int32_t)(t / 36), c = (int32_t)(t % 36);
     double acc = 0.0;
-    for (int32_t k = cbptr[cb]; k < cbptr[cb + 1]; k++) acc += pair[(int64_t)cbslot[k] * 36 + e];
-    Kc[(int64_t)(6 * cba[cb] + e / 6) * nc + 6 * cbb[cb] + e % 6] = acc;
+    for (int32_t k = q.cbptr[cb]; k < q.cbptr[cb + 1]; k++) acc += q.pair[(int64_t)q.cbslot[k] * 36 + c];
+    q.Kc[(int64_t)(6 * q.cba[cb] + c / 6) * q.nc + 6 * q.cbb[cb] + c % 6] = acc;
 }
 
-__global__ void zero_kernel(int64_t n, double *__restrict__ x)
+// entry (job, b): Kc[4096 b .. 4096 (b + 1)) = 0
+__global__ void __launch_bounds__(256) zero_kernel(const CoarseJob *__restrict__ J, const int2 *__restrict__ wl)
 {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        x[i] = 0.0;
+    const int2 e = wl[blockIdx.x];
+    const CoarseJob &q = J[e.x];
+    const int64_t n = (int64_t)q.nc * q.nc, b0 = (int64_t)e.y * 4096;
+    const int64_t b1 = n < b0 + 4096 ? n : b0 + 4096;
+    for (int64_t i = b0 + threadIdx.x; i < b1; i += 256) q.Kc[i] = 0.0;
 }
 
-// coarse dofs whose basis vector vanishes (all dofs of the aggregate
-// prescribed) get a unit diagonal; their right-hand side is always zero
-__global__ void fix_diag_kernel(int32_t nc, double *__restrict__ Kc, int32_t *__restrict__ info)
+// entry (job, b): coarse dof k = 256 b + thread.  Coarse dofs whose basis
+// vector vanishes (all dofs of the aggregate prescribed) get a unit
+// diagonal; their right-hand side is always zero
+__global__ void __launch_bounds__(256) fix_diag_kernel(const CoarseJob *__restrict__ J, const int2 *__restrict__ wl)
 {
-    const int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k == 0) info[0] = info[1] = 0;
-    if (k < nc && Kc[(int64_t)k * nc + k] == 0.0) Kc[(int64_t)k * nc + k] = 1.0;
+    const int2 e = wl[blockIdx.x];
+    const CoarseJob &q = J[e.x];
+    const int32_t k = e.y * 256 + (int32_t)threadIdx.x;
+    if (k == 0) q.info[0] = q.info[1] = 0;
+    if (k < q.nc && q.Kc[(int64_t)k * q.nc + k] == 0.0) q.Kc[(int64_t)k * q.nc + k] = 1.0;
 }
 
-// Mirror the upper triangle of the inverse onto the lower one (the
-// elimination leaves them equal up to rounding), or clear everything if a
-// pivot failed
-__global__ void symmetrize_kernel(int32_t nc, double *__restrict__ A, const int32_t *__restrict__ info)
+// entry (job, b): entry t = 256 b + thread of the inverse.  Mirror the upper
+// triangle onto the lower one (the elimination leaves them equal up to
+// rounding), or clear everything if a pivot failed
+__global__ void __launch_bounds__(256) symmetrize_kernel(const CoarseJob *__restrict__ J, const int2 *__restrict__ wl)
 {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int2 e = wl[blockIdx.x];
+    const CoarseJob &q = J[e.x];
+    const int32_t nc = q.nc;
+    const int64_t t = (int64_t)e.y * 256 + threadIdx.x;
     if (t >= (int64_t)nc * nc) return;
     const int32_t i = (int32_t)(t / nc), j = (int32_t)(t % nc);
-    if (info[0] != 0) {
-        A[t] = 0.0;
+    if (q.info[0] != 0) {
+        q.Kc[t] = 0.0;
         return;
     }
-    if (i < j) A[(int64_t)j * nc + i] = A[(int64_t)i * nc + j];
+    if (i < j) q.Kc[(int64_t)j * nc + i] = q.Kc[(int64_t)i * nc + j];
 }
 
 // ---- block Gauss-Jordan inversion (in place, row-major nc x nc) ----------
@@ -135,78 +161,91 @@ __global__ void symmetrize_kernel(int32_t nc, double *__restrict__ A, const int3
 //   P    = A_KK^-1                              (gj_pivot, one workgroup)
 //   R    = P A_Kj for j outside K; C = A_iK     (gj_panel, old column panel)
 //   A_ij -= C_i R_j,  A_Kj = R_j,  A_iK = -C_i P,  A_KK = P      (gj_update)
+// A launch does step k for every listed model with more than k block rows.
 constexpr int GJB = 64;
 
-__global__ void __launch_bounds__(1024) gj_pivot_kernel(int32_t nc, int32_t k0, int32_t b,
-                                                        const double *__restrict__ A, double *__restrict__ P,
-                                                        int32_t *__restrict__ info)
+// entry (job, -): the pivot block of step k
+__global__ void __launch_bounds__(1024) gj_pivot_kernel(const CoarseJob *__restrict__ J, const int2 *__restrict__ wl,
+                                                        int32_t k)
 {
+    const CoarseJob &q = J[wl[blockIdx.x].x];
+    const int32_t nc = q.nc, k0 = k * GJB, b = nc - k0 < GJB ? nc - k0 : GJB;
+    const double *__restrict__ A = q.Kc;
+    double *__restrict__ P = q.P;
     // the pivot block in LDS; thread (i, j0) updates entries (i, j0..j0+3):
     // each step reads the pivot row and column, then (after a barrier)
     // writes the new values
     __shared__ double M[GJB][GJB + 1];
     const int i = threadIdx.x >> 4, j0 = (threadIdx.x & 15) * 4;
-    for (int k = 0; k < 4; k++) M[i][j0 + k] = (i < b && j0 + k < b) ? A[(int64_t)(k0 + i) * nc + k0 + j0 + k] : 0.0;
+    for (int c = 0; c < 4; c++) M[i][j0 + c] = (i < b && j0 + c < b) ? A[(int64_t)(k0 + i) * nc + k0 + j0 + c] : 0.0;
     __syncthreads();
     for (int p = 0; p < b; p++) {
         const double piv = M[p][p];
         const bool ok = piv > 0.0 && std::isfinite(piv);
-        if (!ok && threadIdx.x == 0) info[0] = 1;
+        if (!ok && threadIdx.x == 0) q.info[0] = 1;
         const double ip = ok ? 1.0 / piv : 0.0;
         const double ci = M[i][p];
         double nv[4];
-        for (int k = 0; k < 4; k++) {
-            const int j = j0 + k;
+        for (int c = 0; c < 4; c++) {
+            const int j = j0 + c;
             const double rj = M[p][j], v = M[i][j];
-            if (i == p && j == p) nv[k] = ip;
-            else if (i == p) nv[k] = rj * ip;
-            else if (j == p) nv[k] = -(ci * ip);
-            else nv[k] = v - ci * (rj * ip);
+            if (i == p && j == p) nv[c] = ip;
+            else if (i == p) nv[c] = rj * ip;
+            else if (j == p) nv[c] = -(ci * ip);
+            else nv[c] = v - ci * (rj * ip);
         }
         __syncthreads();
-        for (int k = 0; k < 4; k++) M[i][j0 + k] = nv[k];
+        for (int c = 0; c < 4; c++) M[i][j0 + c] = nv[c];
         __syncthreads();
     }
-    for (int k = 0; k < 4; k++)
-        if (i < b && j0 + k < b) P[i * GJB + j0 + k] = M[i][j0 + k];
+    for (int c = 0; c < 4; c++)
+        if (i < b && j0 + c < b) P[i * GJB + j0 + c] = M[i][j0 + c];
 }
 
-// one workgroup per 64-column tile t: R[:, tile] = P A_K,tile (t != k), and
-// C[tile rows, :] = A_tile,K
-__global__ void __launch_bounds__(256) gj_panel_kernel(int32_t nc, int32_t k, int32_t b,
-                                                       const double *__restrict__ A, const double *__restrict__ P,
-                                                       double *__restrict__ R, double *__restrict__ C)
+// entry (job, t), one per 64-column tile t: R[:, tile] = P A_K,tile (t != k),
+// and C[tile rows, :] = A_tile,K
+__global__ void __launch_bounds__(256) gj_panel_kernel(const CoarseJob *__restrict__ J, const int2 *__restrict__ wl,
+                                                       int32_t k)
 {
+    const int2 e = wl[blockIdx.x];
+    const CoarseJob &q = J[e.x];
+    const int32_t nc = q.nc, t = e.y, k0 = k * GJB, c0 = t * GJB, b = nc - k0 < GJB ? nc - k0 : GJB;
+    const double *__restrict__ A = q.Kc, *__restrict__ P = q.P;
+    double *__restrict__ R = q.R, *__restrict__ C = q.C;
     __shared__ double Ps[GJB][GJB + 1];
     __shared__ double As[GJB][GJB + 1];
-    const int t = blockIdx.x, k0 = k * GJB, c0 = t * GJB;
     const int w = std::min(GJB, nc - c0);  // columns (and rows) of this tile
     const int r = threadIdx.x >> 2, q0 = (threadIdx.x & 3) * 16;  // 64 rows x 4 quarters
     if (r < w)
-        for (int q = q0; q < q0 + 16 && q < b; q++) C[(int64_t)(c0 + r) * GJB + q] = A[(int64_t)(c0 + r) * nc + k0 + q];
+        for (int c = q0; c < q0 + 16 && c < b; c++) C[(int64_t)(c0 + r) * GJB + c] = A[(int64_t)(c0 + r) * nc + k0 + c];
     if (t == k) return;
     if (r < b)
-        for (int q = q0; q < q0 + 16; q++) {
-            if (q < b) Ps[r][q] = P[r * GJB + q];
-            if (q < w) As[r][q] = A[(int64_t)(k0 + r) * nc + c0 + q];
+        for (int c = q0; c < q0 + 16; c++) {
+            if (c < b) Ps[r][c] = P[r * GJB + c];
+            if (c < w) As[r][c] = A[(int64_t)(k0 + r) * nc + c0 + c];
         }
     __syncthreads();
     if (r < b)
-        for (int q = q0; q < q0 + 16 && q < w; q++) {
+        for (int c = q0; c < q0 + 16 && c < w; c++) {
             double acc = 0.0;
-            for (int l = 0; l < b; l++) acc = fma(Ps[r][l], As[l][q], acc);
-            R[(int64_t)r * nc + c0 + q] = acc;
+            for (int l = 0; l < b; l++) acc = fma(Ps[r][l], As[l][c], acc);
+            R[(int64_t)r * nc + c0 + c] = acc;
         }
 }
 
-// one workgroup per 64x64 tile (ti, tj) of A, 4x4 entries per thread
-__global__ void __launch_bounds__(256) gj_update_kernel(int32_t nc, int32_t k, int32_t b, double *__restrict__ A,
-                                                        const double *__restrict__ P, const double *__restrict__ R,
-                                                        const double *__restrict__ C)
+// entry (job, ti * nt + tj), one per 64x64 tile (ti, tj) of A, 4x4 entries
+// per thread
+__global__ void __launch_bounds__(256) gj_update_kernel(const CoarseJob *__restrict__ J, const int2 *__restrict__ wl,
+                                                        int32_t k)
 {
+    const int2 e = wl[blockIdx.x];
+    const CoarseJob &q = J[e.x];
+    const int32_t nc = q.nc, ti = e.y / q.nt, tj = e.y % q.nt, k0 = k * GJB;
+    const int32_t b = nc - k0 < GJB ? nc - k0 : GJB;
+    double *__restrict__ A = q.Kc;
+    const double *__restrict__ P = q.P, *__restrict__ R = q.R, *__restrict__ C = q.C;
     __shared__ double Cs[GJB][GJB + 1];
     __shared__ double Bs[GJB][GJB + 1];
-    const int tj = blockIdx.x, ti = blockIdx.y, k0 = k * GJB;
     const int r0 = ti * GJB, c0 = tj * GJB;
     const int hr = std::min(GJB, nc - r0), hc = std::min(GJB, nc - c0);
     const int rr = 4 * (threadIdx.x / 16), cc = 4 * (threadIdx.x % 16);
@@ -227,9 +266,9 @@ __global__ void __launch_bounds__(256) gj_update_kernel(int32_t nc, int32_t k, i
     double acc[4][4] = {};
     for (int l = 0; l < b; l++) {
         double cv[4], bv[4];
-        for (int q = 0; q < 4; q++) {
-            cv[q] = Cs[rr + q][l];
-            bv[q] = Bs[l][cc + q];
+        for (int s = 0; s < 4; s++) {
+            cv[s] = Cs[rr + s][l];
+            bv[s] = Bs[l][cc + s];
         }
         for (int x = 0; x < 4; x++)
             for (int y = 0; y < 4; y++) acc[x][y] = fma(cv[x], bv[y], acc[x][y]);
@@ -243,10 +282,6 @@ __global__ void __launch_bounds__(256) gj_update_kernel(int32_t nc, int32_t k, i
         }
 }
 
-// One rocBLAS handle per device, shared by every model.  The handle's
-// device workspace is shared too, so its uses are chained on the device
-// with an event: each factorisation waits for the previous one, whatever
-// stream either ran on.
 template <typename T>
 int dev_upload(amr_model *m, T **d, const std::vector<T> &h)
 {
@@ -307,52 +342,165 @@ int amr_coarse_build(amr_model *m)
     return build(m);
 }
 
-int amr_coarse_setup(amr_model *m)
+bool amr_coarse_due(const amr_model *m)
 {
-    if (!m->coarse_built)
-        if (int rc = build(m)) return rc;
     // The coarse operator only preconditions: it is formed from the K of the
     // first assembly of each DOF map, and for models with plastic materials
     // again at the first solve of each load increment (a linear elastic K
     // does not change between assemblies)
-    if (m->coarse_version == m->k_version) return AMR_OK;
+    if (m->coarse_version == m->k_version) return false;
     const bool same_inc = m->all_elastic || m->coarse_inc == m->inc_version;
-    if (m->coarse_version >= 0 && m->coarse_dofs == m->dof_version &&
-        (m->coarse_refresh == AMR_COARSE_PER_DOF_MAP ||
-         (m->coarse_refresh == AMR_COARSE_PER_INCREMENT && same_inc)))
-        return AMR_OK;
-    hipStream_t s = m->stream;
-    const int32_t nc = m->ncoarse;
-    const int64_t nn = (int64_t)nc * nc;
-    hipLaunchKernelGGL(zero_kernel, dim3((unsigned)std::min<int64_t>((nn + 255) / 256, 4096)), dim3(256), 0, s, nn,
-                       m->Kinv);
-    if (m->npairs > 0)
-        hipLaunchKernelGGL(galerkin_pairs_kernel, dim3((unsigned)((m->npairs + 255) / 256)), dim3(256), 0, s,
-                           m->npairs, m->c_prow, m->c_pagg, m->c_agg, m->c_cent, m->X, m->mask, m->rowptr, m->col,
-                           m->Kv, m->c_pair);
-    if (m->ncb > 0)
-        hipLaunchKernelGGL(galerkin_blocks_kernel, dim3((unsigned)(((int64_t)m->ncb * 36 + 255) / 256)), dim3(256),
-                           0, s, m->ncb, nc, m->c_cba, m->c_cbb, m->c_cbptr, m->c_cbslot, m->c_pair, m->Kinv);
-    hipLaunchKernelGGL(fix_diag_kernel, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, s, nc, m->Kinv, m->c_info);
-    AMR_HIP(hipGetLastError());
-    {
-        double *P = m->c_work, *R = P + GJB * GJB, *C = R + (size_t)GJB * nc;
-        const int nt = (nc + GJB - 1) / GJB;
-        for (int k = 0; k < nt; k++) {
-            const int b = std::min(GJB, nc - k * GJB);
-            hipLaunchKernelGGL(gj_pivot_kernel, dim3(1), dim3(1024), 0, s, nc, k * GJB, b, m->Kinv, P, m->c_info);
-            hipLaunchKernelGGL(gj_panel_kernel, dim3((unsigned)nt), dim3(256), 0, s, nc, k, b, m->Kinv, P, R, C);
-            hipLaunchKernelGGL(gj_update_kernel, dim3((unsigned)nt, (unsigned)nt), dim3(256), 0, s, nc, k, b, m->Kinv,
-                               P, R, C);
-        }
-        AMR_HIP(hipGetLastError());
+    return !(m->coarse_version >= 0 && m->coarse_dofs == m->dof_version &&
+             (m->coarse_refresh == AMR_COARSE_PER_DOF_MAP ||
+              (m->coarse_refresh == AMR_COARSE_PER_INCREMENT && same_inc)));
+}
+
+void amr_staging_free(AmrStaging &s)
+{
+    if (s.pending && s.ev) (void)hipEventSynchronize(s.ev);
+    if (s.d) (void)hipFree(s.d);
+    if (s.h) (void)hipHostFree(s.h);
+    if (s.ev) (void)hipEventDestroy(s.ev);
+    s = AmrStaging();
+}
+
+namespace {
+
+// room for `bytes` in the staging buffers of stream st (the host side free
+// to write)
+int staging_reserve(AmrStaging &s, hipStream_t st, size_t bytes)
+{
+    if (!s.ev) AMR_HIP(hipEventCreateWithFlags(&s.ev, hipEventDisableTiming));
+    if (s.pending) {
+        AMR_HIP(hipEventSynchronize(s.ev));
+        s.pending = false;
     }
-    hipLaunchKernelGGL(symmetrize_kernel, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, s, nc, m->Kinv,
-                       m->c_info);
+    if (bytes <= s.cap) return AMR_OK;
+    // kernels queued earlier on st may still read the device buffer
+    AMR_HIP(hipStreamSynchronize(st));
+    if (s.d) AMR_HIP(hipFree(s.d));
+    if (s.h) AMR_HIP(hipHostFree(s.h));
+    s.d = s.h = nullptr;
+    s.cap = 0;
+    const size_t cap = std::max<size_t>(bytes + bytes / 2, 1 << 16);
+    AMR_HIP(hipMalloc(&s.d, cap));
+    AMR_HIP(hipHostMalloc(&s.h, cap, hipHostMallocDefault));
+    s.cap = cap;
+    return AMR_OK;
+}
+
+}  // namespace
+
+int amr_coarse_form(hipStream_t s, amr_model *const *ms, int n, AmrStaging &stg)
+{
+    if (n <= 0) return AMR_OK;
+    std::vector<CoarseJob> jobs(n);
+    std::vector<int2> zl, pl, bl, fl, sl;
+    int maxnt = 0;
+    for (int j = 0; j < n; j++) {
+        amr_model *m = ms[j];
+        if (!m->coarse_built)
+            if (int rc = build(m)) return rc;
+        CoarseJob &q = jobs[j];
+        q.nc = m->ncoarse;
+        q.nt = (q.nc + GJB - 1) / GJB;
+        q.ncb = m->ncb;
+        q.npairs = m->npairs;
+        q.prow = m->c_prow;
+        q.pagg = m->c_pagg;
+        q.agg = m->c_agg;
+        q.rowptr = m->rowptr;
+        q.col = m->col;
+        q.cba = m->c_cba;
+        q.cbb = m->c_cbb;
+        q.cbptr = m->c_cbptr;
+        q.cbslot = m->c_cbslot;
+        q.cent = m->c_cent;
+        q.X = m->X;
+        q.mask = m->mask;
+        q.Kv = m->Kv;
+        q.pair = m->c_pair;
+        q.Kc = m->Kinv;
+        q.P = m->c_work;
+        q.R = q.P + GJB * GJB;
+        q.C = q.R + (size_t)GJB * q.nc;
+        q.info = m->c_info;
+        maxnt = std::max(maxnt, q.nt);
+        const int64_t nn = (int64_t)q.nc * q.nc;
+        for (int64_t b = 0; b < (nn + 4095) / 4096; b++) zl.push_back(make_int2(j, (int)b));
+        for (int64_t b = 0; b < (q.npairs + 255) / 256; b++) pl.push_back(make_int2(j, (int)b));
+        for (int64_t b = 0; b < ((int64_t)q.ncb * 36 + 255) / 256; b++) bl.push_back(make_int2(j, (int)b));
+        for (int64_t b = 0; b < (q.nc + 255) / 256; b++) fl.push_back(make_int2(j, (int)b));
+        for (int64_t b = 0; b < (nn + 255) / 256; b++) sl.push_back(make_int2(j, (int)b));
+    }
+    // block step k: the pivots, panels and tiles of the models with more
+    // than k block rows
+    std::vector<std::vector<int2>> pv(maxnt), pn(maxnt), up(maxnt);
+    for (int k = 0; k < maxnt; k++)
+        for (int j = 0; j < n; j++) {
+            const int nt = jobs[j].nt;
+            if (k >= nt) continue;
+            pv[k].push_back(make_int2(j, 0));
+            for (int t = 0; t < nt; t++) pn[k].push_back(make_int2(j, t));
+            for (int t = 0; t < nt * nt; t++) up[k].push_back(make_int2(j, t));
+        }
+    // one upload: the jobs, then every list (16-byte aligned offsets)
+    size_t off = 0;
+    auto place = [&off](size_t bytes) {
+        const size_t o = off;
+        off = (off + bytes + 15) & ~(size_t)15;
+        return o;
+    };
+    const size_t o_jobs = place(jobs.size() * sizeof(CoarseJob));
+    const size_t o_z = place(zl.size() * sizeof(int2)), o_p = place(pl.size() * sizeof(int2)),
+                 o_b = place(bl.size() * sizeof(int2)), o_f = place(fl.size() * sizeof(int2)),
+                 o_s = place(sl.size() * sizeof(int2));
+    std::vector<size_t> o_pv(maxnt), o_pn(maxnt), o_up(maxnt);
+    for (int k = 0; k < maxnt; k++) {
+        o_pv[k] = place(pv[k].size() * sizeof(int2));
+        o_pn[k] = place(pn[k].size() * sizeof(int2));
+        o_up[k] = place(up[k].size() * sizeof(int2));
+    }
+    if (int rc = staging_reserve(stg, s, off)) return rc;
+    auto put = [&stg](size_t o, const void *src, size_t bytes) {
+        if (bytes) std::memcpy(stg.h + o, src, bytes);
+    };
+    put(o_jobs, jobs.data(), jobs.size() * sizeof(CoarseJob));
+    put(o_z, zl.data(), zl.size() * sizeof(int2));
+    put(o_p, pl.data(), pl.size() * sizeof(int2));
+    put(o_b, bl.data(), bl.size() * sizeof(int2));
+    put(o_f, fl.data(), fl.size() * sizeof(int2));
+    put(o_s, sl.data(), sl.size() * sizeof(int2));
+    for (int k = 0; k < maxnt; k++) {
+        put(o_pv[k], pv[k].data(), pv[k].size() * sizeof(int2));
+        put(o_pn[k], pn[k].data(), pn[k].size() * sizeof(int2));
+        put(o_up[k], up[k].data(), up[k].size() * sizeof(int2));
+    }
+    AMR_HIP(hipMemcpyAsync(stg.d, stg.h, off, hipMemcpyHostToDevice, s));
+    AMR_HIP(hipEventRecord(stg.ev, s));
+    stg.pending = true;
+    const CoarseJob *dj = (const CoarseJob *)(stg.d + o_jobs);
+    auto wl = [&stg](size_t o) { return (const int2 *)(stg.d + o); };
+    auto grid = [](size_t n_) { return dim3((unsigned)n_); };
+    if (!zl.empty()) hipLaunchKernelGGL(zero_kernel, grid(zl.size()), dim3(256), 0, s, dj, wl(o_z));
+    if (!pl.empty()) hipLaunchKernelGGL(galerkin_pairs_kernel, grid(pl.size()), dim3(256), 0, s, dj, wl(o_p));
+    if (!bl.empty()) hipLaunchKernelGGL(galerkin_blocks_kernel, grid(bl.size()), dim3(256), 0, s, dj, wl(o_b));
+    if (!fl.empty()) hipLaunchKernelGGL(fix_diag_kernel, grid(fl.size()), dim3(256), 0, s, dj, wl(o_f));
     AMR_HIP(hipGetLastError());
-    m->coarse_version = m->k_version;
-    m->coarse_dofs = m->dof_version;
-    m->coarse_inc = m->inc_version;
+    for (int k = 0; k < maxnt; k++) {
+        hipLaunchKernelGGL(gj_pivot_kernel, grid(pv[k].size()), dim3(1024), 0, s, dj, wl(o_pv[k]), k);
+        hipLaunchKernelGGL(gj_panel_kernel, grid(pn[k].size()), dim3(256), 0, s, dj, wl(o_pn[k]), k);
+        hipLaunchKernelGGL(gj_update_kernel, grid(up[k].size()), dim3(256), 0, s, dj, wl(o_up[k]), k);
+    }
+    AMR_HIP(hipGetLastError());
+    if (!sl.empty()) hipLaunchKernelGGL(symmetrize_kernel, grid(sl.size()), dim3(256), 0, s, dj, wl(o_s));
+    AMR_HIP(hipGetLastError());
+    for (int j = 0; j < n; j++) {
+        amr_model *m = ms[j];
+        m->coarse_version = m->k_version;
+        m->coarse_dofs = m->dof_version;
+        m->coarse_inc = m->inc_version;
+    }
     return AMR_OK;
 }
 

@@ -958,15 +958,16 @@ struct Member {
     double bb = 0.0, rr_ref = 0.0, rr_true_prev = INFINITY, relres = 0.0;
     int converged = 0, broke = 0, replacements = 0, iters = 0, fell_back = 0, breakdowns = 0;
     int last_rep = 0;  // iteration of the last residual replacement
+    bool form = false;  // its coarse operator is to be formed before it starts
     // per-step scratch
     bool rep = false, done = false;
     double rr_rec = 0.0;
 };
 
 // M_WAIT: joined, but work the model queued on its own stream before
-// joining (assembly, coarse operator) may still be running; it starts at the
-// first step that finds that work done, so that a slow joiner does not hold
-// up the members already solving
+// joining (assembly), or the formation of its coarse operator, may still be
+// running; it starts at the first step that finds that work done, so that a
+// slow joiner does not hold up the members already solving
 enum { M_FREE = 0, M_JOIN, M_ACTIVE, M_WAIT };
 
 }  // namespace
@@ -1014,6 +1015,13 @@ struct amr_pool {
     bool use_graphs = true;
     uint64_t graph_clock = 0;
     int64_t graph_captures = 0, graph_replays = 0;
+    // coarse operators of two-level joiners are formed in batches: on the
+    // side stream for joiners that start at a later step (the members
+    // already solving are not held up), on the pool's stream for those that
+    // start at once
+    hipStream_t side = nullptr;
+    AmrStaging stg_side, stg_pool;
+    int64_t coarse_batches = 0, coarse_formed = 0;
 };
 
 namespace {
@@ -1337,6 +1345,10 @@ void amr_pool_free(amr_pool *p)
         if (e) (void)hipEventDestroy(e);
     if (p->ev_upload) (void)hipEventDestroy(p->ev_upload);
     if (p->ev_step) (void)hipEventDestroy(p->ev_step);
+    if (p->side) (void)hipStreamSynchronize(p->side);
+    amr_staging_free(p->stg_side);
+    amr_staging_free(p->stg_pool);
+    if (p->side) (void)hipStreamDestroy(p->side);
     drop_graphs(p);
     if (p->own_stream && p->stream) (void)hipStreamDestroy(p->stream);
     delete p;
@@ -1369,8 +1381,10 @@ int amr_pool_detach_model(amr_model *m)
             member = true;
         }
     if (member || m->stream != m->own_stream) {
-        // its kernels may still be queued on the pool's stream
+        // its kernels may still be queued on the pool's stream (or, for its
+        // coarse operator, on the side stream)
         AMR_HIP(hipStreamSynchronize(p->stream));
+        if (p->side) AMR_HIP(hipStreamSynchronize(p->side));
         p->fl_act.erase(std::remove_if(p->fl_act.begin(), p->fl_act.end(),
                                        [&](int s) { return p->mem[s].state == M_FREE; }),
                         p->fl_act.end());
@@ -1417,14 +1431,17 @@ int amr_pool_join(amr_pool *p, amr_model *m, const double *b, double *x, const a
         return AMR_ERR_ARG;
     }
     if (int rc = amr_choose_lanes(m)) return rc;
+    bool form = false;
     if (o->precond == AMR_PRECOND_TWO_LEVEL) {
-        if (int rc = amr_coarse_setup(m)) return rc;
+        if (int rc = amr_coarse_build(m)) return rc;
         if ((m->ncoarse + 3) / 4 > MAX_CW) {
             amr_set_error("two-level preconditioner: too many aggregates for the coarse product");
             return AMR_ERR_ARG;
         }
+        form = amr_coarse_due(m);
     }
     Member M;
+    M.form = form;
     M.m = m;
     M.o = *o;
     M.b = b;
@@ -1464,14 +1481,18 @@ int amr_pool_step_launch(amr_pool *p)
     AMR_HIP(hipSetDevice(p->device));
     // joiners whose own work is done start now; if nobody else is solving,
     // the waiting ones start too, the pool's stream waiting for them
-    std::vector<int> waiting;
+    // (a joiner whose coarse operator is due is formed first: on the side
+    // stream when others can start now, else on the pool's stream)
+    std::vector<int> waiting, side;
     bool ready = false;
     for (int s = 0; s < p->cap; s++) {
         Member &M = p->mem[s];
         if (M.state == M_WAIT) {
             const hipError_t q = hipEventQuery(M.m->ev_handoff);
-            if (q == hipSuccess) M.state = M_JOIN;
-            else if (q == hipErrorNotReady) {
+            if (q == hipSuccess) {
+                if (M.form) side.push_back(s);
+                else M.state = M_JOIN;
+            } else if (q == hipErrorNotReady) {
                 (void)hipGetLastError();  // not an error: clear it
                 waiting.push_back(s);
             } else {
@@ -1480,11 +1501,40 @@ int amr_pool_step_launch(amr_pool *p)
         }
         ready = ready || M.state == M_JOIN || M.state == M_ACTIVE;
     }
-    if (!ready)
+    if (!ready) {
         for (int s : waiting) {
             AMR_HIP(hipStreamWaitEvent(p->stream, p->mem[s].m->ev_handoff, 0));
             p->mem[s].state = M_JOIN;
         }
+        for (int s : side) p->mem[s].state = M_JOIN;
+        side.clear();
+    }
+    if (!side.empty()) {
+        if (!p->side) AMR_HIP(hipStreamCreateWithFlags(&p->side, hipStreamNonBlocking));
+        std::vector<amr_model *> ms;
+        for (int s : side) ms.push_back(p->mem[s].m);
+        if (int rc = amr_coarse_form(p->side, ms.data(), (int)ms.size(), p->stg_side)) return rc;
+        p->coarse_batches++;
+        p->coarse_formed += (int64_t)ms.size();
+        // they start at the first step that finds the formation done
+        for (int s : side) {
+            AMR_HIP(hipEventRecord(p->mem[s].m->ev_handoff, p->side));
+            p->mem[s].form = false;
+        }
+    }
+    {
+        std::vector<amr_model *> ms;
+        for (int s = 0; s < p->cap; s++)
+            if (p->mem[s].state == M_JOIN && p->mem[s].form) {
+                ms.push_back(p->mem[s].m);
+                p->mem[s].form = false;
+            }
+        if (!ms.empty()) {
+            if (int rc = amr_coarse_form(p->stream, ms.data(), (int)ms.size(), p->stg_pool)) return rc;
+            p->coarse_batches++;
+            p->coarse_formed += (int64_t)ms.size();
+        }
+    }
     std::vector<int> join, act;
     for (int s = 0; s < p->cap; s++) {
         if (p->mem[s].state == M_JOIN) join.push_back(s);
@@ -1829,6 +1879,13 @@ int amr_pool_graphs(amr_pool *p, int enable, int64_t *captures, int64_t *replays
     }
     if (captures) *captures = p->graph_captures;
     if (replays) *replays = p->graph_replays;
+    return AMR_OK;
+}
+
+int amr_pool_coarse_counts(amr_pool *p, int64_t *batches, int64_t *models)
+{
+    if (batches) *batches = p->coarse_batches;
+    if (models) *models = p->coarse_formed;
     return AMR_OK;
 }
 

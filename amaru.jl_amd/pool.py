@@ -110,6 +110,12 @@ class Pool:
         _lib.check(_lib.lib().amr_pool_set_graphs(self._h, flag, ctypes.byref(c), ctypes.byref(r)))
         return c.value, r.value
 
+    def coarse_batches(self):
+        """(batched coarse-operator formations, models they formed) so far."""
+        b, m = ctypes.c_int64(0), ctypes.c_int64(0)
+        _lib.check(_lib.lib().amr_pool_coarse_batches(self._h, ctypes.byref(b), ctypes.byref(m)))
+        return b.value, m.value
+
     def timing(self, enable):
         ms = ctypes.c_double(0.0)
         n = ctypes.c_int64(0)

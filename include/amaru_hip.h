@@ -354,6 +354,13 @@ int amr_pool_timing(amr_pool *p, int enable, double *spmv_ms, int64_t *spmv_laun
  * captured and replayed so far (either may be NULL).  Results are the same
  * bits either way; with amr_pool_timing on, steps launch kernel by kernel. */
 int amr_pool_set_graphs(amr_pool *p, int enable, int64_t *captures, int64_t *replays);
+/* Two-level coarse operators are formed when their models start solving in
+ * the pool, for all models that start at one step together: one launch per
+ * phase (Galerkin product, each Gauss-Jordan block step) for all of them.
+ * *batches counts those batched formations so far and *models the models
+ * they formed (either may be NULL).  No reference counterpart: the
+ * reference factorises each system on its own (src/solver.jl:5-79). */
+int amr_pool_coarse_batches(amr_pool *p, int64_t *batches, int64_t *models);
 
 /* Aggregate size of the two-level preconditioner: about target_nodes nodes
  * per aggregate, fewer aggregates than max_aggregates (defaults 256, 170).

@@ -169,3 +169,45 @@ def test_coarse_refresh_policies(amaru):
     dev.solve_system(U * 0.0, F, precision="fp64", precond="two_level", rtol=1e-10)
     assert np.array_equal(dev.coarse_space()[1], out["dof_map"][1])
     dev.close()
+
+
+def test_batched_formation_matches_alone(amaru):
+    """Six cantilevers with coarse spaces of 1 to 16 Gauss-Jordan block rows
+    (30 to 1020 coarse dofs) start solving in one pool at the same step:
+    their coarse operators are formed in one batch, and each Kc^-1 and
+    displacement field equals the one of solving that model alone."""
+    from amaru_amd import pool as P, workloads as W
+    params = W.cantilever_params(11, 6)
+    targets = [4000, 400, 120, 60, 24, 16]
+
+    def build(i):
+        ana = W.cantilever(params[i])
+        ana.model.configure_dofs(ana.stages[0].bcs)
+        ana.dev = amaru.DeviceModel(ana.model)
+        ana.dev.set_coarse(targets[i], 170)
+        return ana
+
+    kw = dict(precision="fp64", precond="two_level", pcg_rtol=1e-10)
+    alone = []
+    for i in range(6):
+        ana = build(i)
+        ok, msg = amaru.solve(ana, pcg_check_every=16, **kw)
+        assert ok, msg
+        alone.append((ana.dev.coarse_space()[1], ana.model.uvals.copy()))
+        ana.dev.close()
+    ncs = [k.shape[0] for k, _ in alone]
+    assert min(ncs) <= 64 and max(ncs) > 15 * 64, ncs
+    anas = [build(i) for i in range(6)]
+    ps = P.PoolSet(1, 0, max_members=6, check_every=16)
+    try:
+        out = P.run_analyses(anas, pool=ps, **kw)
+        assert all(ok for ok, _ in out), out
+        assert ps.pools[0].coarse_batches() == (1, 6)
+        for ana, (kinv, U) in zip(anas, alone):
+            assert np.array_equal(ana.dev.coarse_space()[1], kinv)
+            assert np.array_equal(ana.model.uvals, U)
+    finally:
+        for ana in anas:
+            ps.pools[0].forget(ana.dev)
+            ana.dev.close()
+        ps.close()
