@@ -84,7 +84,7 @@ SIGNATURES = {
     "amr_pool_size": (I32, [P, P]),
     "amr_pool_timing": (I32, [P, I32, P, P]),
     "amr_pool_set_graphs": (I32, [P, I32, P, P]),
-    "amr_pool_coarse_batches": (I32, [P, P, P]),
+    "amr_pool_stats": (I32, [P, P]),
     "amr_model_device_bytes": (I32, [P, P]),
     "amr_mem_info": (I32, [I32, P, P]),
     "amr_mem_trim": (I32, [I32]),

@@ -293,7 +293,7 @@ int amr_pool_step_members(const amr_pool *p);  // members of the step in flight 
 int amr_pool_step_query(amr_pool *p, int *done);
 int amr_pool_set_timing(amr_pool *p, int enable, double *ms, int64_t *n);
 int amr_pool_graphs(amr_pool *p, int enable, int64_t *captures, int64_t *replays);
-int amr_pool_coarse_counts(amr_pool *p, int64_t *batches, int64_t *models);
+int amr_pool_counts(amr_pool *p, int64_t *out);
 int amr_spmv_full(amr_model *m, const double *x, double *y, int mask_mode);
 int amr_spmv_vals(amr_model *m, const double *vals /* kv_at layout */, const double *x, double *y, int mask_mode);
 int amr_choose_lanes(amr_model *m);

@@ -1605,10 +1605,10 @@ extern "C" int amr_pool_set_graphs(amr_pool *p, int enable, int64_t *captures, i
     return amr_pool_graphs(p, enable, captures, replays);
 }
 
-extern "C" int amr_pool_coarse_batches(amr_pool *p, int64_t *batches, int64_t *models)
+extern "C" int amr_pool_stats(amr_pool *p, int64_t *out)
 {
-    if (!p) return AMR_ERR_ARG;
-    return amr_pool_coarse_counts(p, batches, models);
+    if (!p || !out) return AMR_ERR_ARG;
+    return amr_pool_counts(p, out);
 }
 
 extern "C" int amr_model_device_bytes(amr_model *m, int64_t *bytes)

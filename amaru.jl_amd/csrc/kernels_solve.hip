@@ -1012,7 +1012,7 @@ struct amr_pool {
         uint64_t used = 0;
     };
     std::vector<StepGraph> graphs;
-    bool use_graphs = true;
+    bool use_graphs = false;
     uint64_t graph_clock = 0;
     int64_t graph_captures = 0, graph_replays = 0;
     // coarse operators of two-level joiners are formed in batches: on the
@@ -1022,6 +1022,7 @@ struct amr_pool {
     hipStream_t side = nullptr;
     AmrStaging stg_side, stg_pool;
     int64_t coarse_batches = 0, coarse_formed = 0;
+    int64_t iter_kernels = 0, iter_submits = 0;  // of the steps' PCG iterations
 };
 
 namespace {
@@ -1296,8 +1297,10 @@ int amr_pool_new(amr_pool **out, int device, int cap, int check, hipStream_t str
     p->cap = cap;
     p->check = std::max(1, std::min(check > 0 ? check : 32, 256));
     p->mem.assign(cap, Member());
-    // AMR_POOL_GRAPHS=0 launches steps kernel by kernel (for tools that do
-    // not follow graph launches)
+    // steps launch kernel by kernel unless AMR_POOL_GRAPHS=1 (or
+    // amr_pool_set_graphs) asks for captured ones: on MI355X the replayed
+    // graphs ran the pooled workloads 4-12% slower than the same kernels
+    // launched one by one
     if (const char *g = std::getenv("AMR_POOL_GRAPHS")) p->use_graphs = std::atoi(g) != 0;
     int rc = AMR_OK;
     auto fail = [&](hipError_t e) {
@@ -1614,6 +1617,11 @@ int amr_pool_step_launch(amr_pool *p)
                                Ls.dev_at<int2>(o_va));
         }
     };
+    // kernels of the step's iterations, and the host submissions that queue
+    // them: one each kernel by kernel, one graph launch when captured
+    const int64_t iter_kernels = (int64_t)check * ((int64_t)gr.size() + 2 + (aa.empty() ? 0 : 2));
+    p->iter_kernels += iter_kernels;
+    p->iter_submits += graph ? 1 : iter_kernels;
     if (!graph) {
         iterations();
     } else {
@@ -1882,10 +1890,12 @@ int amr_pool_graphs(amr_pool *p, int enable, int64_t *captures, int64_t *replays
     return AMR_OK;
 }
 
-int amr_pool_coarse_counts(amr_pool *p, int64_t *batches, int64_t *models)
+int amr_pool_counts(amr_pool *p, int64_t *out)
 {
-    if (batches) *batches = p->coarse_batches;
-    if (models) *models = p->coarse_formed;
+    out[0] = p->coarse_batches;
+    out[1] = p->coarse_formed;
+    out[2] = p->iter_kernels;
+    out[3] = p->iter_submits;
     return AMR_OK;
 }
 

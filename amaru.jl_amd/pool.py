@@ -103,18 +103,27 @@ class Pool:
         return self._results(n.value)
 
     def set_graphs(self, enable=None):
-        """Steps as one captured HIP graph launch (True, the default) or
-        kernel by kernel (False); returns (captures, replays) so far."""
+        """Steps as one captured HIP graph launch (True) or kernel by kernel
+        (False, the default: faster on MI355X); returns (captures, replays)
+        so far."""
         c, r = ctypes.c_int64(0), ctypes.c_int64(0)
         flag = -1 if enable is None else (1 if enable else 0)
         _lib.check(_lib.lib().amr_pool_set_graphs(self._h, flag, ctypes.byref(c), ctypes.byref(r)))
         return c.value, r.value
 
+    def stats(self):
+        """Counters so far: batched coarse-operator formations and the models
+        they formed; kernels of the steps' PCG iterations and the host
+        submissions that queued them (graph launches when captured)."""
+        out = np.zeros(4, dtype=np.int64)
+        _lib.check(_lib.lib().amr_pool_stats(self._h, _lib.ptr(out)))
+        return {"coarse_batches": int(out[0]), "coarse_models": int(out[1]), "iteration_kernels": int(out[2]),
+                "host_submissions": int(out[3])}
+
     def coarse_batches(self):
         """(batched coarse-operator formations, models they formed) so far."""
-        b, m = ctypes.c_int64(0), ctypes.c_int64(0)
-        _lib.check(_lib.lib().amr_pool_coarse_batches(self._h, ctypes.byref(b), ctypes.byref(m)))
-        return b.value, m.value
+        st = self.stats()
+        return st["coarse_batches"], st["coarse_models"]
 
     def timing(self, enable):
         ms = ctypes.c_double(0.0)
@@ -260,6 +269,15 @@ class PoolSet:
 
     def size(self):
         return sum(p.size() for p in self.pools)
+
+    def stats(self):
+        """Pool.stats summed over the pools, with the steps launched."""
+        out = {}
+        for p in self.pools:
+            for k, v in p.stats().items():
+                out[k] = out.get(k, 0) + v
+        out["steps"] = self.launches
+        return out
 
     def in_flight(self):
         return any(p.in_flight for p in self.pipes)

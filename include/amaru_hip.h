@@ -346,21 +346,30 @@ int amr_pool_size(amr_pool *p, int *members);
 /* SpMV kernel timing over the pool's steps, as amr_timing. */
 int amr_pool_timing(amr_pool *p, int enable, double *spmv_ms, int64_t *spmv_launches);
 
-/* A step's check_every iterations of every PCG phase run as one HIP graph
- * launch: captured once per launch shape and replayed while members join
- * and leave (the work lists are padded to size classes and re-uploaded
- * each step; padding entries do nothing).  enable = 1 (the default) or 0
- * switches it (-1 leaves it); *captures and *replays count the graphs
+/* With enable = 1 a step's check_every iterations of every PCG phase run
+ * as one HIP graph launch: captured once per launch shape and replayed
+ * while members join and leave (the work lists are padded to size classes
+ * and re-uploaded each step; padding entries do nothing).  enable = 0 (the
+ * default, also set by the environment variable AMR_POOL_GRAPHS) launches
+ * them kernel by kernel: on MI355X the replays ran the pooled workloads
+ * 4-12% slower.  -1 leaves it; *captures and *replays count the graphs
  * captured and replayed so far (either may be NULL).  Results are the same
- * bits either way; with amr_pool_timing on, steps launch kernel by kernel. */
+ * bits either way; with amr_pool_timing on, steps launch kernel by kernel
+ * (the SpMVs between event records). */
 int amr_pool_set_graphs(amr_pool *p, int enable, int64_t *captures, int64_t *replays);
-/* Two-level coarse operators are formed when their models start solving in
- * the pool, for all models that start at one step together: one launch per
- * phase (Galerkin product, each Gauss-Jordan block step) for all of them.
- * *batches counts those batched formations so far and *models the models
- * they formed (either may be NULL).  No reference counterpart: the
- * reference factorises each system on its own (src/solver.jl:5-79). */
-int amr_pool_coarse_batches(amr_pool *p, int64_t *batches, int64_t *models);
+/* Counters of the pool so far, out[0..3]:
+ *   [0] batched formations of two-level coarse operators: they are formed
+ *       when their models start solving, for all models that start at one
+ *       step together, one launch per phase (Galerkin product, each
+ *       Gauss-Jordan block step) for all of them;
+ *   [1] models those formations formed;
+ *   [2] kernels of the steps' PCG iterations (every phase of every
+ *       iteration; each serves all members);
+ *   [3] host submissions that queued them (one per kernel when a step runs
+ *       kernel by kernel, one per graph launch when it is captured).
+ * No reference counterpart: the reference solves each system on its own
+ * (src/solver.jl:5-79). */
+int amr_pool_stats(amr_pool *p, int64_t *out);
 
 /* Aggregate size of the two-level preconditioner: about target_nodes nodes
  * per aggregate, fewer aggregates than max_aggregates (defaults 256, 170).

@@ -88,10 +88,15 @@ def test_pool_bit_identical_to_alone(amaru, precision, graphs):
         for j in jobs:
             assert j.result[0], j.result[1]
         captures, replays = pool.set_graphs()
+        st = pool.stats()
         if graphs:
             assert captures >= 1 and replays > captures
+            # one graph launch per step queues all its iterations' kernels
+            assert st["host_submissions"] == captures + replays
+            assert st["iteration_kernels"] >= 32 * st["host_submissions"]
         else:
             assert captures == replays == 0
+            assert st["host_submissions"] == st["iteration_kernels"] > 0
         for (ana, _), ref in zip(anas, alone):
             got = results(ana)
             assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1])
