@@ -688,16 +688,34 @@ __global__ void __launch_bounds__(64) tl_restrict_kernel(const SegDesc *__restri
     const auto *aggptr = gl(d.aggptr), *aggnodes = gl(d.aggnodes);
     const double cx = cent[a * 3], cy = cent[a * 3 + 1], cz = cent[a * 3 + 2];
     double v[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    for (int32_t k = aggptr[a] + threadIdx.x; k < aggptr[a + 1]; k += 64) {
-        const int64_t nd = aggnodes[k];
-        const double r0 = rv[nd * 3], r1 = rv[nd * 3 + 1], r2 = rv[nd * 3 + 2];
-        const double dx = X[nd * 3] - cx, dy = X[nd * 3 + 1] - cy, dz = X[nd * 3 + 2] - cz;
-        v[0] += r0;
-        v[1] += r1;
-        v[2] += r2;
-        v[3] += fma(dy, r2, -dz * r1);
-        v[4] += fma(dz, r0, -dx * r2);
-        v[5] += fma(dx, r1, -dy * r0);
+    // a lane's nodes k = k0 + lane + 64 j in order of j, eight at a time:
+    // the eight nodes' loads are issued before their sums
+    constexpr int NB = 8;
+    const int32_t k1 = aggptr[a + 1];
+    for (int32_t kb = aggptr[a] + threadIdx.x; kb < k1; kb += 64 * NB) {
+        int64_t nd[NB];
+#pragma unroll
+        for (int j = 0; j < NB; j++) nd[j] = kb + 64 * j < k1 ? aggnodes[kb + 64 * j] : -1;
+        double r[NB][3], x[NB][3];
+#pragma unroll
+        for (int j = 0; j < NB; j++)
+            if (nd[j] >= 0)
+                for (int c = 0; c < 3; c++) {
+                    r[j][c] = rv[nd[j] * 3 + c];
+                    x[j][c] = X[nd[j] * 3 + c];
+                }
+#pragma unroll
+        for (int j = 0; j < NB; j++)
+            if (nd[j] >= 0) {
+                const double r0 = r[j][0], r1 = r[j][1], r2 = r[j][2];
+                const double dx = x[j][0] - cx, dy = x[j][1] - cy, dz = x[j][2] - cz;
+                v[0] += r0;
+                v[1] += r1;
+                v[2] += r2;
+                v[3] += fma(dy, r2, -dz * r1);
+                v[4] += fma(dz, r0, -dx * r2);
+                v[5] += fma(dx, r1, -dy * r0);
+            }
     }
     for (int q = 0; q < 6; q++)
         for (int off = 32; off > 0; off >>= 1) v[q] += __shfl_down(v[q], off, 64);
@@ -717,8 +735,21 @@ __global__ void __launch_bounds__(256) tl_coarse_kernel(const SegDesc *__restric
     double acc = 0.0;
     const auto *cvec = gl(d.cvec);
     if (row < d.nc) {
+        // columns c = lane + 64 j in order of j, sixteen loads at a time
         const auto *Kr = gl(d.Kinv) + (int64_t)row * d.nc;
-        for (int c = lane; c < d.nc; c += 64) acc = fma(Kr[c], cvec[c], acc);
+        constexpr int NB = 16;
+        for (int cb = lane; cb < d.nc; cb += 64 * NB) {
+            double kv[NB], cv[NB];
+#pragma unroll
+            for (int j = 0; j < NB; j++) {
+                const bool in = cb + 64 * j < d.nc;
+                kv[j] = in ? Kr[cb + 64 * j] : 0.0;
+                cv[j] = in ? cvec[cb + 64 * j] : 0.0;
+            }
+#pragma unroll
+            for (int j = 0; j < NB; j++)
+                if (cb + 64 * j < d.nc) acc = fma(kv[j], cv[j], acc);
+        }
     }
     for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
     __shared__ double cyw[4];
