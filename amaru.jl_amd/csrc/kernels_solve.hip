@@ -320,6 +320,62 @@ __device__ inline void row_sum(int32_t rb, int32_t re, int sub, const CT *__rest
     }
 }
 
+// row_sum<L, false> (y = K x) and row_sum<L, true> (a = |K| |x|) of the
+// same row in one pass over its blocks: each sum has the same slots and the
+// same tree as alone, so the same bits.
+template <int L, typename VT, typename XT, typename AT, typename CT = int32_t>
+__device__ inline void row_sum_pair(int32_t rb, int32_t re, int sub, const CT *__restrict__ col,
+                                    const VT *__restrict__ vals, const XT *__restrict__ x, AT y[3], AT a[3],
+                                    int64_t row = 0)
+{
+    constexpr int NS = 64 / L;
+    AT acc[NS][3], abs_[NS][3];
+#pragma unroll
+    for (int t = 0; t < NS; t++)
+#pragma unroll
+        for (int d = 0; d < 3; d++) acc[t][d] = abs_[t][d] = (AT)0;
+    for (int32_t base = rb; base < re; base += 64) {
+#pragma unroll
+        for (int t = 0; t < NS; t++) {
+            const int32_t k = base + sub + L * t;
+            if (k < re) {
+                const int64_t c = block_col<CT>(col, k, row);
+                const auto *xc = gl(x) + c * 3;
+                const AT x0 = (AT)xc[0], x1 = (AT)xc[1], x2 = (AT)xc[2];
+                AT v[9];
+                load_block<VT, AT>(vals, k, v);
+                acc[t][0] = fma(v[2], x2, fma(v[1], x1, fma(v[0], x0, acc[t][0])));
+                acc[t][1] = fma(v[5], x2, fma(v[4], x1, fma(v[3], x0, acc[t][1])));
+                acc[t][2] = fma(v[8], x2, fma(v[7], x1, fma(v[6], x0, acc[t][2])));
+                abs_[t][0] += fabs(v[0] * x0) + fabs(v[1] * x1) + fabs(v[2] * x2);
+                abs_[t][1] += fabs(v[3] * x0) + fabs(v[4] * x1) + fabs(v[5] * x2);
+                abs_[t][2] += fabs(v[6] * x0) + fabs(v[7] * x1) + fabs(v[8] * x2);
+            }
+        }
+    }
+#pragma unroll
+    for (int off = 32; off >= L; off >>= 1) {
+#pragma unroll
+        for (int t = 0; t < off / L; t++)
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                acc[t][d] = acc[t][d] + acc[t + off / L][d];
+                abs_[t][d] = abs_[t][d] + abs_[t + off / L][d];
+            }
+    }
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        AT u = acc[0][d], w = abs_[0][d];
+#pragma unroll
+        for (int off = L / 2; off > 0; off >>= 1) {
+            u += __shfl_xor(u, off, L);
+            w += __shfl_xor(w, off, L);
+        }
+        y[d] = u;
+        a[d] = w;
+    }
+}
+
 // Smallest power of two >= R: LDS reduction arrays are that long, zero
 // padded, so the tree is the same fixed binary tree for any R.
 __host__ __device__ inline int pow2ceil(int R)
@@ -438,8 +494,11 @@ __global__ void __launch_bounds__(WGS) residual_kernel(const SegDesc *__restrict
     const double *__restrict__ mask = d.mask;
     for (int i = threadIdx.x / L; i < nr; i += RPW) {
         double y[3], a[3];
-        row_sum<L, false, double, double, double>(srp[i], srp[i + 1], sub, d.col, d.Kv, d.x, y);
-        row_sum<L, true, double, double, double>(srp[i], srp[i + 1], sub, d.col, d.Kv, d.x, a);
+        if (d.col16)
+            row_sum_pair<L, double, double, double, int16_t>(srp[i], srp[i + 1], sub, d.col16, d.Kv, d.x, y, a,
+                                                             r0 + i);
+        else
+            row_sum_pair<L, double, double, double>(srp[i], srp[i + 1], sub, d.col, d.Kv, d.x, y, a);
         if (sub == 0) {
             double rr = 0.0, bb = 0.0, kx = 0.0, gap = 0.0;
             for (int c = 0; c < 3; c++) {
@@ -1054,6 +1113,7 @@ struct amr_pool {
     AmrStaging stg_side, stg_pool;
     int64_t coarse_batches = 0, coarse_formed = 0;
     int64_t iter_kernels = 0, iter_submits = 0;  // of the steps' PCG iterations
+    int64_t member_kernels = 0;  // the same iterations launched member by member
 };
 
 namespace {
@@ -1653,6 +1713,10 @@ int amr_pool_step_launch(amr_pool *p)
     const int64_t iter_kernels = (int64_t)check * ((int64_t)gr.size() + 2 + (aa.empty() ? 0 : 2));
     p->iter_kernels += iter_kernels;
     p->iter_submits += graph ? 1 : iter_kernels;
+    // launching each member alone would take its own SpMV, update and
+    // direction kernels (and restrict and coarse product, two-level) per
+    // iteration
+    for (int s : act) p->member_kernels += (int64_t)check * (p->hD[s].tl ? 5 : 3);
     if (!graph) {
         iterations();
     } else {
@@ -1927,6 +1991,7 @@ int amr_pool_counts(amr_pool *p, int64_t *out)
     out[1] = p->coarse_formed;
     out[2] = p->iter_kernels;
     out[3] = p->iter_submits;
+    out[4] = p->member_kernels;
     return AMR_OK;
 }
 

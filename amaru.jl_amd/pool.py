@@ -113,12 +113,13 @@ class Pool:
 
     def stats(self):
         """Counters so far: batched coarse-operator formations and the models
-        they formed; kernels of the steps' PCG iterations and the host
-        submissions that queued them (graph launches when captured)."""
-        out = np.zeros(4, dtype=np.int64)
+        they formed; kernels of the steps' PCG iterations, the host
+        submissions that queued them (graph launches when captured), and the
+        kernels the same iterations would take launched member by member."""
+        out = np.zeros(5, dtype=np.int64)
         _lib.check(_lib.lib().amr_pool_stats(self._h, _lib.ptr(out)))
         return {"coarse_batches": int(out[0]), "coarse_models": int(out[1]), "iteration_kernels": int(out[2]),
-                "host_submissions": int(out[3])}
+                "host_submissions": int(out[3]), "member_by_member_kernels": int(out[4])}
 
     def coarse_batches(self):
         """(batched coarse-operator formations, models they formed) so far."""

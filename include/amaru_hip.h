@@ -357,7 +357,7 @@ int amr_pool_timing(amr_pool *p, int enable, double *spmv_ms, int64_t *spmv_laun
  * bits either way; with amr_pool_timing on, steps launch kernel by kernel
  * (the SpMVs between event records). */
 int amr_pool_set_graphs(amr_pool *p, int enable, int64_t *captures, int64_t *replays);
-/* Counters of the pool so far, out[0..3]:
+/* Counters of the pool so far, out[0..4]:
  *   [0] batched formations of two-level coarse operators: they are formed
  *       when their models start solving, for all models that start at one
  *       step together, one launch per phase (Galerkin product, each
@@ -366,7 +366,10 @@ int amr_pool_set_graphs(amr_pool *p, int enable, int64_t *captures, int64_t *rep
  *   [2] kernels of the steps' PCG iterations (every phase of every
  *       iteration; each serves all members);
  *   [3] host submissions that queued them (one per kernel when a step runs
- *       kernel by kernel, one per graph launch when it is captured).
+ *       kernel by kernel, one per graph launch when it is captured);
+ *   [4] kernels the same iterations would take launched member by member
+ *       (SpMV, update and direction per member and iteration, two-level
+ *       members also restrict and coarse product).
  * No reference counterpart: the reference solves each system on its own
  * (src/solver.jl:5-79). */
 int amr_pool_stats(amr_pool *p, int64_t *out);

@@ -89,6 +89,7 @@ def test_pool_bit_identical_to_alone(amaru, precision, graphs):
             assert j.result[0], j.result[1]
         captures, replays = pool.set_graphs()
         st = pool.stats()
+        assert st["member_by_member_kernels"] > st["iteration_kernels"]
         if graphs:
             assert captures >= 1 and replays > captures
             # one graph launch per step queues all its iterations' kernels
