@@ -133,6 +133,8 @@ def _stage_solver(ana, stage, a):
     ana.stats["T"] = 0.0
     dev.set_scheme(a["scheme"])
     Uex, Fex = model.bc_vals(bcs)
+    if model.__dict__.get("_state_dev") is dev:
+        model.state  # noqa: B018  (the previous stage's states, before this one commits)
     dev.stage_begin(Uex, Fex)
     ana.opts = DeviceModel.opts(precision=a["precision"], rtol=a["pcg_rtol"], maxiter=a["pcg_maxiter"],
                                 precond=a["precond"], replace_ratio=a["pcg_replace_ratio"],
@@ -233,5 +235,5 @@ def _stage_solver(ana, stage, a):
                 status = "Solver did not converge. Try `autoinc=true`. " + (sysmsg or "")
                 break
     sync_nodal()
-    model.state = dev.state_get(committed=True)
+    model.state_on_device(dev)
     return status is None, status

@@ -145,6 +145,8 @@ struct amr_model {
     // solver workspace (3*nnodes each)
     double *v[12] = {};
     float *f[8] = {};
+    double *v_block = nullptr;  // v and f are slices of these
+    float *f_block = nullptr;
     double *part = nullptr;       // reduction partials
     double *scal = nullptr;       // device scalars
     int nparts = 0;
@@ -183,12 +185,14 @@ struct amr_model {
     amr_nr_info fin_info{};
     // stage_begin's host vectors stay alive until their copies are done
     double *sb_pin = nullptr;  // pinned host staging of the stage targets (2 x 3 nnodes)
+    size_t sb_pin_bytes = 0;
     hipEvent_t sb_event = nullptr;
     hipEvent_t ev_handoff = nullptr;   // orders the switch between its own and the pool's stream
     hipEvent_t fin_event = nullptr;    // end of the queued NR finish (its read-back)
     bool assembled = false;
     // device-resident stage vectors (node order): Uex Fex U F dFex R Rc dUa dUi dFin dUt x
     double *sv[12] = {};
+    double *sv_block = nullptr;  // sv are slices of it
     double acr = 0.0;
     // SpMV timing with events on the model stream
     bool timing = false;

@@ -337,6 +337,45 @@ static int d2h(T *h, const T *d, size_t n, hipStream_t s)
         if (_rc) return _rc;   \
     } while (0)
 
+// Pinned host blocks kept for reuse by later models (pinning memory costs
+// about as much as the small copies it serves): best fit within twice the
+// request, at most 256 MiB kept.
+static std::mutex g_hpin_mu;
+static std::multimap<size_t, void *> g_hpin_free;
+static size_t g_hpin_kept = 0;
+
+static int pinned_get(size_t bytes, void **out, size_t *got)
+{
+    {
+        std::lock_guard<std::mutex> lk(g_hpin_mu);
+        auto it = g_hpin_free.lower_bound(bytes);
+        if (it != g_hpin_free.end() && it->first <= 2 * bytes) {
+            *out = it->second;
+            *got = it->first;
+            g_hpin_kept -= it->first;
+            g_hpin_free.erase(it);
+            return AMR_OK;
+        }
+    }
+    AMR_HIP(hipHostMalloc(out, bytes, hipHostMallocPortable));
+    *got = bytes;
+    return AMR_OK;
+}
+
+static void pinned_put(void *p, size_t bytes)
+{
+    if (!p) return;
+    {
+        std::lock_guard<std::mutex> lk(g_hpin_mu);
+        if (g_hpin_kept + bytes <= ((size_t)256 << 20)) {
+            g_hpin_free.emplace(bytes, p);
+            g_hpin_kept += bytes;
+            return;
+        }
+    }
+    (void)hipHostFree(p);
+}
+
 static void free_model(amr_model *m)
 {
     if (!m) return;
@@ -352,17 +391,17 @@ static void free_model(amr_model *m)
                     m->Kflag, m->dinv, m->dinv32, m->mask, m->eqd,  m->dF,   m->estat,  m->err,    m->part,
                     m->scal, m->binv, m->binv32, m->Me, m->Mv, m->Mflag, m->Nd, m->rho_d, m->mvx, m->mvy, m->Ksave, m->col16, m->Kpred};
     for (void *p : ptrs) amr_release(m, p);
-    for (double *p : m->v) amr_release(m, p);
-    for (float *p : m->f) amr_release(m, p);
-    for (double *p : m->sv) amr_release(m, p);
+    amr_release(m, m->v_block);
+    amr_release(m, m->f_block);
+    amr_release(m, m->sv_block);
     for (hipEvent_t e : m->ev)
         if (e) (void)hipEventDestroy(e);
-    if (m->fin_h) (void)hipHostFree(m->fin_h);
+    pinned_put(m->fin_h, sizeof(amr_model::FinHost));
     if (m->sb_event) {
         (void)hipEventSynchronize(m->sb_event);
         (void)hipEventDestroy(m->sb_event);
     }
-    if (m->sb_pin) (void)hipHostFree(m->sb_pin);
+    pinned_put(m->sb_pin, m->sb_pin_bytes);
     if (m->ev_handoff) (void)hipEventDestroy(m->ev_handoff);
     if (m->fin_event) (void)hipEventDestroy(m->fin_event);
     if (m->own_stream) (void)hipStreamDestroy(m->own_stream);
@@ -547,8 +586,14 @@ extern "C" int amr_model_create(amr_model **out, int device, int etype, int nips
     TRY(dalloc(m.get(), &m->estat, (size_t)ne));
     TRY(dalloc(m.get(), &m->err, 4));
     TRY(dalloc(m.get(), &m->scal, 64));
-    for (double *&p : m->v) TRY(dalloc(m.get(), &p, n3));
-    for (float *&p : m->f) TRY(dalloc(m.get(), &p, n3));
+    {
+        // the work vectors as slices of one block each (one allocation)
+        const size_t n3p = ((size_t)n3 + 63) / 64 * 64;
+        TRY(dalloc(m.get(), &m->v_block, n3p * 12));
+        TRY(dalloc(m.get(), &m->f_block, n3p * 8));
+        for (int k = 0; k < 12; k++) m->v[k] = m->v_block + n3p * k;
+        for (int k = 0; k < 8; k++) m->f[k] = m->f_block + n3p * k;
+    }
 
     std::vector<double> dNdR((size_t)nq * nn * 3);
     for (int q = 0; q < nq; q++) shape_deriv(etype, R + 3 * q, dNdR.data() + (size_t)q * nn * 3);
@@ -1115,13 +1160,17 @@ extern "C" int amr_stage_begin(amr_model *m, const double *Uex, const double *Fe
     if (!m || !Uex || !Fex) return AMR_ERR_ARG;
     AMR_HIP(hipSetDevice(m->device));
     const size_t n = (size_t)m->nnodes * 3;
-    for (double *&p : m->sv)
-        if (!p) TRY(dalloc(m, &p, n));
+    if (!m->sv_block) {
+        // the stage vectors as slices of one block (one allocation)
+        const size_t np = (n + 63) / 64 * 64, ns = sizeof(m->sv) / sizeof(m->sv[0]);
+        TRY(dalloc(m, &m->sv_block, np * ns));
+        for (size_t k = 0; k < ns; k++) m->sv[k] = m->sv_block + np * k;
+    }
     // the pinned host buffer lives in the model until the copies from it are
     // done (waited for here before the next stage overwrites it)
     if (!m->sb_event) {
         AMR_HIP(hipEventCreateWithFlags(&m->sb_event, hipEventDisableTiming));
-        AMR_HIP(hipHostMalloc((void **)&m->sb_pin, 2 * n * sizeof(double), hipHostMallocDefault));
+        TRY(pinned_get(2 * n * sizeof(double), (void **)&m->sb_pin, &m->sb_pin_bytes));
     } else
         AMR_HIP(hipEventSynchronize(m->sb_event));
     eq_to_node(m, Uex, m->sb_pin);
@@ -1176,7 +1225,10 @@ extern "C" int amr_nr_finish_begin(amr_model *m, const amr_solve_info *si)
         return AMR_ERR_ARG;
     }
     AMR_HIP(hipSetDevice(m->device));
-    if (!m->fin_h) AMR_HIP(hipHostMalloc((void **)&m->fin_h, sizeof(amr_model::FinHost), hipHostMallocDefault));
+    if (!m->fin_h) {
+        size_t got = 0;
+        TRY(pinned_get(sizeof(amr_model::FinHost), (void **)&m->fin_h, &got));
+    }
     amr_nr_info inf{};
     double **v = m->sv;
     int rc = AMR_OK;

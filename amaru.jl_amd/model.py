@@ -267,6 +267,27 @@ class FEModel:
         self.fvals = np.zeros((self.nnodes, 3))   # dof.vals[fx,fy,fz]
         self.eq = None
 
+    # The IP states (one row of NSTATE values per IP).  After an analysis
+    # stage they are the device model's committed states, copied to the host
+    # when first read (or when that device model is closed or starts another
+    # stage), so that analyses whose states nobody reads pay no copy.
+    @property
+    def state(self):
+        dev = self.__dict__.get("_state_dev")
+        if dev is not None:
+            self._state_dev = None
+            self._state = dev.state_get(committed=True)
+        return self._state
+
+    @state.setter
+    def state(self, st):
+        self._state_dev = None
+        self._state = st
+
+    def state_on_device(self, dev):
+        """The states are dev's committed ones (read when first needed)."""
+        self._state_dev = dev
+
     def _rule(self, nips):
         if self.shape in (HEX8, HEX20):
             if nips in (0, 8):
@@ -474,6 +495,8 @@ class DeviceModel:
 
     def close(self):
         if self._h:
+            if self.model.__dict__.get("_state_dev") is self:
+                self.model.state  # noqa: B018  (copy the committed states first)
             _lib.lib().amr_model_destroy(self._h)
             self._h = None
 
