@@ -230,6 +230,7 @@ enum { SV_UEX = 0, SV_FEX, SV_U, SV_F, SV_DFEX, SV_R, SV_RC, SV_DUA, SV_DUI, SV_
 // stage vector kernels (kernels_solve.hip)
 int amr_vec_inc_begin(amr_model *m, double dT, double acr, int add);
 int amr_vec_fill(amr_model *m, double *x, double v);
+int amr_vec_fill_n(amr_model *m, double *x, int64_t n, double v);  // n values from x
 int amr_vec_solve_prep(amr_model *m, const double *dUi, const double *R, double *u2, double *ky, double *b);
 int amr_vec_combine(amr_model *m, const double *x, double *dUi);
 int amr_vec_add(amr_model *m, const double *a, const double *b, double *out);

@@ -1178,7 +1178,11 @@ extern "C" int amr_stage_begin(amr_model *m, const double *Uex, const double *Fe
     TRY(h2d(m->sv[SV_UEX], m->sb_pin, n, m->stream));
     TRY(h2d(m->sv[SV_FEX], m->sb_pin + n, n, m->stream));
     AMR_HIP(hipEventRecord(m->sb_event, m->stream));
-    for (int k : {SV_U, SV_F, SV_RC, SV_DUA, SV_DUI, SV_DFIN, SV_R, SV_DFEX}) TRY(amr_vec_fill(m, m->sv[k], 0.0));
+    // SV_U .. SV_DFIN are consecutive slices: one fill
+    static_assert(SV_DFIN - SV_U == 7 && SV_F == SV_U + 1 && SV_DFEX == SV_U + 2 && SV_R == SV_U + 3 &&
+                      SV_RC == SV_U + 4 && SV_DUA == SV_U + 5 && SV_DUI == SV_U + 6,
+                  "stage vectors zeroed at a stage's start");
+    TRY(amr_vec_fill_n(m, m->sv[SV_U], (int64_t)(m->sv[SV_DFIN] - m->sv[SV_U]) + (int64_t)n, 0.0));
     return AMR_OK;
 }
 

@@ -2314,6 +2314,14 @@ int amr_vec_fill(amr_model *m, double *x, double val)
     return AMR_OK;
 }
 
+int amr_vec_fill_n(amr_model *m, double *x, int64_t n, double val)
+{
+    const int64_t g = std::max<int64_t>(1, std::min<int64_t>((n + WGV - 1) / WGV, 4 * MAX_GV));
+    hipLaunchKernelGGL(fill_kernel, dim3((unsigned)g), dim3(WGV), 0, m->stream, n, x, val);
+    AMR_HIP(hipGetLastError());
+    return AMR_OK;
+}
+
 // b = mask .* (R - K (prescribed part of dUi)); u2/ky are scratch
 int amr_vec_solve_prep(amr_model *m, const double *dUi, const double *R, double *u2, double *ky, double *b)
 {
