@@ -186,6 +186,7 @@ struct amr_model {
     // stage_begin's host vectors stay alive until their copies are done
     double *sb_pin = nullptr;  // pinned host staging of the stage targets (2 x 3 nnodes)
     size_t sb_pin_bytes = 0;
+    bool sb_fresh = false;  // sb_event not recorded yet
     hipEvent_t sb_event = nullptr;
     hipEvent_t ev_handoff = nullptr;   // orders the switch between its own and the pool's stream
     hipEvent_t fin_event = nullptr;    // end of the queued NR finish (its read-back)

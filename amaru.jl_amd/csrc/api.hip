@@ -475,6 +475,29 @@ static int upload_dofs(amr_model *m, const int32_t *eq_id, int64_t ndofs, int64_
     return AMR_OK;
 }
 
+// The stage driver's device vectors, pinned staging and event: allocated
+// with the model (so that starting a stage allocates nothing).
+static int stage_alloc(amr_model *m)
+{
+    const size_t n = (size_t)m->nnodes * 3;
+    if (!m->sv_block) {
+        // the stage vectors as slices of one block (one allocation)
+        const size_t np = (n + 63) / 64 * 64, ns = sizeof(m->sv) / sizeof(m->sv[0]);
+        TRY(dalloc(m, &m->sv_block, np * ns));
+        for (size_t k = 0; k < ns; k++) m->sv[k] = m->sv_block + np * k;
+    }
+    if (!m->sb_event) {
+        AMR_HIP(hipEventCreateWithFlags(&m->sb_event, hipEventDisableTiming));
+        TRY(pinned_get(2 * n * sizeof(double), (void **)&m->sb_pin, &m->sb_pin_bytes));
+        m->sb_fresh = true;
+    }
+    if (!m->fin_h) {
+        size_t got = 0;
+        TRY(pinned_get(sizeof(amr_model::FinHost), (void **)&m->fin_h, &got));
+    }
+    return AMR_OK;
+}
+
 extern "C" int amr_model_create(amr_model **out, int device, int etype, int nips, int64_t nnodes,
                                 const double *coords, int64_t ne, const int32_t *conn, int nmats,
                                 const int32_t *mat_kind, const double *mat_params, const int32_t *emat,
@@ -620,6 +643,7 @@ extern "C" int amr_model_create(amr_model **out, int device, int etype, int nips
     AMR_HIP(hipMemsetAsync(m->scal, 0, sizeof(double) * 64, s));
     AMR_HIP(hipMemsetAsync(m->err, 0, 4 * sizeof(int32_t), s));
     TRY(upload_dofs(m.get(), eq_id, ndofs, nu));
+    TRY(stage_alloc(m.get()));
     *out = m.release();
     return AMR_OK;
 }
@@ -1160,19 +1184,11 @@ extern "C" int amr_stage_begin(amr_model *m, const double *Uex, const double *Fe
     if (!m || !Uex || !Fex) return AMR_ERR_ARG;
     AMR_HIP(hipSetDevice(m->device));
     const size_t n = (size_t)m->nnodes * 3;
-    if (!m->sv_block) {
-        // the stage vectors as slices of one block (one allocation)
-        const size_t np = (n + 63) / 64 * 64, ns = sizeof(m->sv) / sizeof(m->sv[0]);
-        TRY(dalloc(m, &m->sv_block, np * ns));
-        for (size_t k = 0; k < ns; k++) m->sv[k] = m->sv_block + np * k;
-    }
+    TRY(stage_alloc(m));
     // the pinned host buffer lives in the model until the copies from it are
     // done (waited for here before the next stage overwrites it)
-    if (!m->sb_event) {
-        AMR_HIP(hipEventCreateWithFlags(&m->sb_event, hipEventDisableTiming));
-        TRY(pinned_get(2 * n * sizeof(double), (void **)&m->sb_pin, &m->sb_pin_bytes));
-    } else
-        AMR_HIP(hipEventSynchronize(m->sb_event));
+    if (!m->sb_fresh) AMR_HIP(hipEventSynchronize(m->sb_event));
+    m->sb_fresh = false;
     eq_to_node(m, Uex, m->sb_pin);
     eq_to_node(m, Fex, m->sb_pin + n);
     TRY(h2d(m->sv[SV_UEX], m->sb_pin, n, m->stream));
@@ -1229,10 +1245,7 @@ extern "C" int amr_nr_finish_begin(amr_model *m, const amr_solve_info *si)
         return AMR_ERR_ARG;
     }
     AMR_HIP(hipSetDevice(m->device));
-    if (!m->fin_h) {
-        size_t got = 0;
-        TRY(pinned_get(sizeof(amr_model::FinHost), (void **)&m->fin_h, &got));
-    }
+    TRY(stage_alloc(m));
     amr_nr_info inf{};
     double **v = m->sv;
     int rc = AMR_OK;

@@ -495,10 +495,17 @@ class Server:
         self.in_use -= req.bytes if req.bytes else req.reserve
 
     def _release(self):
+        """Detach the finished models from the pools here; destroy them on
+        the closing thread when serving (a destroy waits for the model's
+        stream and copies its committed states to the host)."""
         ts = time.perf_counter()
+        closer = getattr(self, "_closer", None)
         for dev in self._graveyard:
-            self.pools.forget(dev)
-            dev.close()
+            self.pools.detach(dev)
+            if closer is not None:
+                closer.submit(dev.close)
+            else:
+                dev.close()
         self._graveyard.clear()
         self._graveyard_bytes = 0
         self.t_loop["free"] += time.perf_counter() - ts
@@ -513,6 +520,7 @@ class Server:
         # that admitted requests are not queued behind arrivals' host builds
         ex = concurrent.futures.ThreadPoolExecutor(max_workers=self.workers)
         ex_dev = concurrent.futures.ThreadPoolExecutor(max_workers=max(1, self.workers // 2))
+        self._closer = concurrent.futures.ThreadPoolExecutor(max_workers=1)
         lock = threading.Lock()
         t0 = time.perf_counter()
         clock = lambda: time.perf_counter() - t0  # noqa: E731
@@ -680,6 +688,8 @@ class Server:
                 ex.shutdown(wait=True)
                 ex_dev.shutdown(wait=True)
                 self._release()
+                self._closer.shutdown(wait=True)
+                self._closer = None
         return reqs
 
 
