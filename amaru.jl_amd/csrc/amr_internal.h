@@ -69,7 +69,7 @@ int amr_build_pattern(int nn, int64_t nnodes, int64_t ne, const int32_t *conn, A
 
 // Doubles of a model's reduction-partials buffer (m->part; layout in
 // kernels_solve.hip).
-constexpr int AMR_PART_DBL = 25600;
+constexpr int AMR_PART_DBL = 46080;
 
 // Device-built pattern (pattern.hip): the same arrays as amr_build_pattern,
 // made on the device from the model's conn.  Returns AMR_PATTERN_ON_HOST when
@@ -107,7 +107,7 @@ struct amr_pool;
 // Per-model device resources.
 struct amr_model {
     int etype = 0, nn = 0, nips = 0, device = 0;
-    int ncu = 256;  // compute units of the device (row-tile sizing)
+    int ncu = 256;  // compute units of the device
     int64_t nnodes = 0, ne = 0, ndofs = 0, nu = 0, nip_total = 0;
     AmrPattern pat;
     std::vector<int32_t> eq;      // nnodes*3 eq ids (-1: no dof)

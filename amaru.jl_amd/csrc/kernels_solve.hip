@@ -46,7 +46,7 @@ namespace {
 
 constexpr int WGV = 256;  // vector kernels
 constexpr int WGS = 512;  // SpMV kernels
-constexpr int MAX_GR = 4096, MAX_GV = 1024;
+constexpr int MAX_GR = 8192, MAX_GV = 1024;
 
 // scalar and control slots of one model: S_NSCAL doubles, then C_NCTL ints
 // S_RZP0/1: r.z published per parity slot (the sum of that slot's partials)
@@ -62,8 +62,8 @@ enum { C_DONE = 0, C_ITERS, C_BREAK, C_NAN, C_STOP, C_NCTL = 16 };
 constexpr int SEG_DBL = 32;  // doubles per model in a pool's scalar array
 static_assert(S_NSCAL + C_NCTL / 2 <= SEG_DBL, "scalar block");
 // partial buffers inside m->part
-enum { P_PQ = 0, P_RZ0 = 4096, P_RZ1 = 5120, P_RR0 = 6144, P_RR1 = 7168, P_CY0 = 8192, P_CY1 = 8704, P_A = 9216,
-       P_B = 13312, P_C = 17408, P_D = 21504, P_END = 25600 };
+enum { P_PQ = 0, P_RZ0 = 8192, P_RZ1 = 9216, P_RR0 = 10240, P_RR1 = 11264, P_CY0 = 12288, P_CY1 = 12800, P_A = 13312,
+       P_B = 21504, P_C = 29696, P_D = 37888, P_END = 46080 };
 static_assert(P_END == AMR_PART_DBL, "partials layout");
 constexpr int MAX_CW = 512;  // coarse-product workgroups per model (4 coarse rows each)
 
@@ -957,13 +957,14 @@ __global__ void fill_kernel(int64_t n, double *__restrict__ x, double v)
     for (int64_t i = (int64_t)blockIdx.x * WGV + threadIdx.x; i < n; i += (int64_t)gridDim.x * WGV) x[i] = v;
 }
 
-// Row tiles of a model.  The SpMV kernels run 4 workgroups per CU (8 waves
-// per SIMD), so ncu*4 tiles are resident at once: R is chosen so that the
-// tiles fill a whole number k of such rounds (k ~ rows / (64 * resident):
-// about 64 rows per tile, which measured 10% faster than 128 on the 40^3
-// HEX20 block), avoiding a nearly empty last round, with R a multiple of 8,
-// at least 64, and at most MAX_GR tiles.  Vector kernels: at most MAX_GV
-// workgroups.
+// Row tiles of a model: R = 64 rows, a whole number of the row passes of a
+// 512-thread workgroup at every lanes value (16 rows per pass at 32 lanes,
+// 8 at 64, 32 at 16, 64 at 8); models of more than 64 * MAX_GR nodes take
+// the next multiple of 64 that keeps MAX_GR tiles.  On the 40^3 HEX20 block
+// (L = 32) the SpMV took 153.9 us with R = 64 and 154.5 us with 48 against
+// 155.8-160.0 us with 40, 56, 72, 80 and 96 (rows that leave the last pass
+// partly idle, or longer tiles) and 162.6 us with 128.  Vector kernels: at
+// most MAX_GV workgroups.
 struct Geom {
     int R, ntiles, nvblk;
 };
@@ -972,11 +973,8 @@ Geom model_geom(const amr_model *m)
 {
     Geom g;
     const int64_t n = std::max<int64_t>(1, m->nnodes);
-    const int64_t res = (int64_t)std::max(1, m->ncu) * 4;
-    const int64_t k = std::max<int64_t>(1, (n + 32 * res) / (64 * res));
-    int64_t R = (n + k * res - 1) / (k * res);
-    R = std::max<int64_t>(64, (R + 7) / 8 * 8);
-    if ((n + R - 1) / R > MAX_GR) R = ((n + MAX_GR - 1) / MAX_GR + 7) / 8 * 8;
+    int64_t R = 64;
+    if ((n + R - 1) / R > MAX_GR) R = ((n + MAX_GR - 1) / MAX_GR + 63) / 64 * 64;
     g.R = (int)R;
     g.ntiles = (int)((n + R - 1) / R);
     const int64_t nv = (m->nnodes * 3 + WGV - 1) / WGV;
@@ -2260,8 +2258,9 @@ __global__ void __launch_bounds__(WGV) nr_finalize_kernel(int nv, const double *
                                                           int32_t *__restrict__ ctl, double *__restrict__ scal)
 {
     double v[4] = {0.0, 0.0, 0.0, 0.0};
+    const int off[4] = {P_A, P_B, P_C, P_D};
     for (int i = threadIdx.x; i < nv; i += WGV)
-        for (int k = 0; k < 4; k++) v[k] = fmax(v[k], part[P_A + k * 4096 + i]);
+        for (int k = 0; k < 4; k++) v[k] = fmax(v[k], part[off[k] + i]);
     for (int k = 0; k < 4; k++) v[k] = block_max<WGV>(v[k]);
     if (threadIdx.x == 0) {
         scal[S_RES] = v[0];

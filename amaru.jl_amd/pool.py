@@ -397,7 +397,7 @@ def estimate_device_bytes(model):
     per_elem = 4 * nn + 4 + 72 * nn * nn + 24 * nn + 4 + 4 * nn + 224 * nips + 4 * nn * nn
     per_blk = 4 + 8 + 72 + 36 + 9
     # + the padding of the 32 vector slices (to 64 entries each)
-    return int(per_node * nnodes + per_elem * ne + per_blk * (nblk + 64) + 25600 * 8 + 4096 + 24 * 64 * 8 + 8 * 64 * 4)
+    return int(per_node * nnodes + per_elem * ne + per_blk * (nblk + 64) + 46080 * 8 + 4096 + 24 * 64 * 8 + 8 * 64 * 4)
 
 
 class Request:
