@@ -71,6 +71,8 @@ SIGNATURES = {
     "amr_nr_prepare": (I32, [P, I32]),
     "amr_nr_finish": (I32, [P, ctypes.POINTER(SolveInfo), ctypes.POINTER(NRInfo)]),
     "amr_nr_finish_begin": (I32, [P, ctypes.POINTER(SolveInfo)]),
+    "amr_nr_prepare_many": (I32, [P, P, I32]),
+    "amr_nr_finish_begin_many": (I32, [P, P, I32]),
     "amr_nr_finish_end": (I32, [P, ctypes.POINTER(NRInfo)]),
     "amr_pool_create": (I32, [P, I32, I32, I32]),
     "amr_pool_destroy": (I32, [P]),

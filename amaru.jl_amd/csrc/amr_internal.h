@@ -188,6 +188,7 @@ struct amr_model {
     size_t sb_pin_bytes = 0;
     bool sb_fresh = false;  // sb_event not recorded yet
     hipEvent_t sb_event = nullptr;
+    hipEvent_t ev_batch = nullptr;  // the model's queued work, for a batched launch to wait on
     hipEvent_t ev_handoff = nullptr;   // orders the switch between its own and the pool's stream
     hipEvent_t fin_event = nullptr;    // end of the queued NR finish (its read-back)
     bool assembled = false;
@@ -263,6 +264,16 @@ struct AmrStaging {
     bool pending = false;
 };
 void amr_staging_free(AmrStaging &s);
+int amr_staging_reserve(AmrStaging &s, hipStream_t st, size_t bytes);  // waits for the last upload; may grow
+
+// The element phase of several models in one launch per kernel family (the
+// same arithmetic as launching each model alone), queued on stream s:
+// mount_K (error words reset, element stiffness by each model's path and
+// tiles, assembly, Jacobi and nodal block-Jacobi inverses) and update_state!
+// (IP updates by each model's tiles, then the internal-force gather)
+int amr_batch_mount_K(amr_model *const *ms, int n, hipStream_t s, AmrStaging &stg);
+int amr_batch_update(amr_model *const *ms, const double *const *dU, double *const *dFin, int n, hipStream_t s,
+                     AmrStaging &stg);
 
 // two-level coarse space (coarse.hip): the index structures are built on
 // first use (host work, one wait); Kc^-1 is formed again when

@@ -403,6 +403,7 @@ static void free_model(amr_model *m)
     }
     pinned_put(m->sb_pin, m->sb_pin_bytes);
     if (m->ev_handoff) (void)hipEventDestroy(m->ev_handoff);
+    if (m->ev_batch) (void)hipEventDestroy(m->ev_batch);
     if (m->fin_event) (void)hipEventDestroy(m->fin_event);
     if (m->own_stream) (void)hipStreamDestroy(m->own_stream);
     delete m;
@@ -1275,6 +1276,182 @@ extern "C" int amr_nr_finish_begin(amr_model *m, const amr_solve_info *si)
     m->fin_info = inf;
     m->fin_rc = rc;
     m->fin_pending = true;
+    return AMR_OK;
+}
+
+// ---------------------------------------------- several models at once
+//
+// amr_nr_prepare_many / amr_nr_finish_begin_many: the same as calling the
+// single-model entry point on each model in turn, with the element kernels
+// of all the models in one launch per kernel family.  The batched launches
+// run on the models' common stream (a pool's, for members that left it
+// together), else on the device's batch stream, ordered after and before the
+// models' own streams with events.
+
+namespace {
+
+// Staging of the job tables: one buffer per stream, so that an upload is
+// queued behind the kernels that read that buffer's previous contents (a
+// buffer shared by two streams could be overwritten while the other
+// stream's kernels still read it).
+struct BatchCtx {
+    std::mutex mu;
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;
+    std::map<hipStream_t, AmrStaging> stg;
+};
+BatchCtx g_batch[64];
+
+int batch_check(amr_model *const *ms, int n, int *device)
+{
+    for (int i = 0; i < n; i++) {
+        amr_model *m = ms[i];
+        if (!m || !m->sv[SV_U]) return AMR_ERR_ARG;
+        if (amr_pool_is_member(m)) {
+            amr_set_error("the model is solving in a pool; its matrix and solver vectors are in use");
+            return AMR_ERR_ARG;
+        }
+        if (i > 0 && m->device != ms[0]->device) {
+            amr_set_error("the models of one batched call must be on one device");
+            return AMR_ERR_ARG;
+        }
+        for (int k = 0; k < i; k++)
+            if (ms[k] == m) {
+                amr_set_error("a model appears twice in one batched call");
+                return AMR_ERR_ARG;
+            }
+    }
+    *device = ms[0]->device;
+    return AMR_OK;
+}
+
+// the stream of the batched launches, ordered after the models' queued work
+int batch_enter(amr_model *const *ms, int n, BatchCtx &b, hipStream_t *s)
+{
+    bool same = true;
+    for (int i = 1; i < n; i++) same = same && ms[i]->stream == ms[0]->stream;
+    if (same) {
+        *s = ms[0]->stream;
+        return AMR_OK;
+    }
+    if (!b.stream) {
+        AMR_HIP(hipStreamCreateWithFlags(&b.stream, hipStreamNonBlocking));
+        AMR_HIP(hipEventCreateWithFlags(&b.done, hipEventDisableTiming));
+    }
+    for (int i = 0; i < n; i++) {
+        amr_model *m = ms[i];
+        if (!m->ev_batch) AMR_HIP(hipEventCreateWithFlags(&m->ev_batch, hipEventDisableTiming));
+        AMR_HIP(hipEventRecord(m->ev_batch, m->stream));
+        AMR_HIP(hipStreamWaitEvent(b.stream, m->ev_batch, 0));
+    }
+    *s = b.stream;
+    return AMR_OK;
+}
+
+// the models' streams ordered after the batched launches
+int batch_leave(amr_model *const *ms, int n, BatchCtx &b, hipStream_t s)
+{
+    if (s != b.stream) return AMR_OK;
+    AMR_HIP(hipEventRecord(b.done, s));
+    for (int i = 0; i < n; i++) AMR_HIP(hipStreamWaitEvent(ms[i]->stream, b.done, 0));
+    return AMR_OK;
+}
+
+}  // namespace
+
+extern "C" int amr_nr_prepare_many(amr_model **ms, const int *its, int n)
+{
+    PhaseRange range_("mount_K (NR, batched)");
+    if (n <= 0) return AMR_OK;
+    if (!ms || !its) return AMR_ERR_ARG;
+    int dev = 0;
+    TRY(batch_check(ms, n, &dev));
+    for (int i = 0; i < n; i++)
+        if (its[i] < 1) return AMR_ERR_ARG;
+    if (dev < 0 || dev >= 64) return AMR_ERR_ARG;
+    AMR_HIP(hipSetDevice(dev));
+    for (int i = 0; i < n; i++)
+        if (its[i] > 1) TRY(amr_vec_fill(ms[i], ms[i]->sv[SV_DUI], 0.0));
+    {
+        BatchCtx &b = g_batch[dev];
+        std::lock_guard<std::mutex> lk(b.mu);
+        hipStream_t s;
+        TRY(batch_enter(ms, n, b, &s));
+        TRY(amr_batch_mount_K(ms, n, s, b.stg[s]));
+        TRY(batch_leave(ms, n, b, s));
+    }
+    for (int i = 0; i < n; i++) {
+        amr_model *m = ms[i];
+        m->assembled = true;
+        // right-hand side of solve_system!(K, ΔUitr, Rtr, nu): R - K12 ΔUi2
+        if (m->nu > 0) TRY(amr_vec_solve_prep(m, m->sv[SV_DUI], m->sv[SV_R], m->v[8], m->v[9], m->v[10]));
+    }
+    return AMR_OK;
+}
+
+extern "C" int amr_nr_finish_begin_many(amr_model **ms, const amr_solve_info *si, int n)
+{
+    PhaseRange range_("update_state! (NR, batched)");
+    if (n <= 0) return AMR_OK;
+    if (!ms) return AMR_ERR_ARG;
+    int dev = 0;
+    TRY(batch_check(ms, n, &dev));
+    if (dev < 0 || dev >= 64) return AMR_ERR_ARG;
+    for (int i = 0; i < n; i++) {
+        if (ms[i]->fin_pending) {
+            amr_set_error("amr_nr_finish_begin: the previous finish has not been completed");
+            return AMR_ERR_ARG;
+        }
+        if (ms[i]->nu > 0 && !si) {
+            amr_set_error("amr_nr_finish: the model has unknowns; pass the outcome of its linear solve");
+            return AMR_ERR_ARG;
+        }
+    }
+    AMR_HIP(hipSetDevice(dev));
+    std::vector<amr_model *> ok;
+    std::vector<const double *> dU;
+    std::vector<double *> dFin;
+    std::vector<amr_nr_info> inf(n);
+    std::vector<int> rcs(n, AMR_OK);
+    for (int i = 0; i < n; i++) {
+        amr_model *m = ms[i];
+        TRY(stage_alloc(m));
+        double **v = m->sv;
+        if (m->nu > 0) {
+            inf[i].pcg_iterations = si[i].iterations;
+            inf[i].relres = si[i].relres;
+            inf[i].replacements = si[i].refinements;
+            inf[i].fallback = si[i].fallback;
+            if (!si[i].converged) rcs[i] = AMR_ERR_SOLVE;
+            else TRY(amr_vec_combine(m, v[SV_X], v[SV_DUI]));
+        }
+        if (rcs[i] == AMR_OK) {
+            // update_state!(ΔUa + ΔUitr) from the committed state
+            TRY(amr_vec_add(m, v[SV_DUA], v[SV_DUI], v[SV_DUT]));
+            ok.push_back(m);
+            dU.push_back(v[SV_DUT]);
+            dFin.push_back(v[SV_DFIN]);
+        }
+    }
+    if (!ok.empty()) {
+        BatchCtx &b = g_batch[dev];
+        std::lock_guard<std::mutex> lk(b.mu);
+        hipStream_t s;
+        TRY(batch_enter(ok.data(), (int)ok.size(), b, &s));
+        TRY(amr_batch_update(ok.data(), dU.data(), dFin.data(), (int)ok.size(), s, b.stg[s]));
+        TRY(batch_leave(ok.data(), (int)ok.size(), b, s));
+    }
+    for (amr_model *m : ok) {
+        TRY(amr_vec_nr_post_async(m, m->fin_h->s, m->fin_h->c));
+        AMR_HIP(hipMemcpyAsync(m->fin_h->err, m->err, 3 * sizeof(int32_t), hipMemcpyDeviceToHost, m->stream));
+        if (!m->fin_event) AMR_HIP(hipEventCreateWithFlags(&m->fin_event, hipEventDisableTiming));
+        AMR_HIP(hipEventRecord(m->fin_event, m->stream));
+    }
+    for (int i = 0; i < n; i++) {
+        ms[i]->fin_info = inf[i];
+        ms[i]->fin_rc = rcs[i];
+        ms[i]->fin_pending = true;
+    }
     return AMR_OK;
 }
 

@@ -364,11 +364,9 @@ void amr_staging_free(AmrStaging &s)
     s = AmrStaging();
 }
 
-namespace {
-
 // room for `bytes` in the staging buffers of stream st (the host side free
 // to write)
-int staging_reserve(AmrStaging &s, hipStream_t st, size_t bytes)
+int amr_staging_reserve(AmrStaging &s, hipStream_t st, size_t bytes)
 {
     if (!s.ev) AMR_HIP(hipEventCreateWithFlags(&s.ev, hipEventDisableTiming));
     if (s.pending) {
@@ -388,8 +386,6 @@ int staging_reserve(AmrStaging &s, hipStream_t st, size_t bytes)
     s.cap = cap;
     return AMR_OK;
 }
-
-}  // namespace
 
 int amr_coarse_form(hipStream_t s, amr_model *const *ms, int n, AmrStaging &stg)
 {
@@ -461,7 +457,7 @@ int amr_coarse_form(hipStream_t s, amr_model *const *ms, int n, AmrStaging &stg)
         o_pn[k] = place(pn[k].size() * sizeof(int2));
         o_up[k] = place(up[k].size() * sizeof(int2));
     }
-    if (int rc = staging_reserve(stg, s, off)) return rc;
+    if (int rc = amr_staging_reserve(stg, s, off)) return rc;
     auto put = [&stg](size_t o, const void *src, size_t bytes) {
         if (bytes) std::memcpy(stg.h + o, src, bytes);
     };

@@ -8,6 +8,8 @@
 // the Mandel B matrix are skipped; an fma with a zero factor leaves a
 // non-zero accumulator unchanged, so skipping them does not change a bit.
 #include <cmath>
+#include <cstring>
+#include <map>
 
 #include <mutex>
 
@@ -234,18 +236,14 @@ __device__ inline double inv3_entry(const double *J, int k, double *det_out)
 // column node (54 fma) and updates its blocks (36 fma per block).
 
 template <int NN, int NQ, int ACH>
-__global__ void __launch_bounds__(64) ke_kernel(int64_t ne, const double *__restrict__ X,
-                                                const int32_t *__restrict__ conn,
-                                                const int32_t *__restrict__ emat,
-                                                const AmrMat *__restrict__ mats,
-                                                const double *__restrict__ dNdR,
-                                                const double *__restrict__ wip,
-                                                const double *__restrict__ st, int64_t nip,
-                                                int all_elastic, double *__restrict__ Ke,
-                                                int32_t *__restrict__ err)
+__device__ inline void ke_kernel_at(int64_t bid_, int64_t ne, const double *__restrict__ X,
+                                    const int32_t *__restrict__ conn, const int32_t *__restrict__ emat,
+                                    const AmrMat *__restrict__ mats, const double *__restrict__ dNdR,
+                                    const double *__restrict__ wip, const double *__restrict__ st, int64_t nip,
+                                    int all_elastic, double *__restrict__ Ke, int32_t *__restrict__ err)
 {
     constexpr int NCH = (NN + ACH - 1) / ACH;
-    const int64_t e = blockIdx.x;
+    const int64_t e = bid_;
     const int lane = threadIdx.x;
     __shared__ double sC[NN * 3];
     __shared__ double sJ[9], sInv[9], sCoef;
@@ -355,6 +353,16 @@ __global__ void __launch_bounds__(64) ke_kernel(int64_t ne, const double *__rest
     if (lane == 0 && sBad) atomicMin(err, (int32_t)(e + 1));
 }
 
+template <int NN, int NQ, int ACH>
+__global__ void __launch_bounds__(64) ke_kernel(int64_t ne, const double *__restrict__ X,
+                                  const int32_t *__restrict__ conn, const int32_t *__restrict__ emat,
+                                  const AmrMat *__restrict__ mats, const double *__restrict__ dNdR,
+                                  const double *__restrict__ wip, const double *__restrict__ st, int64_t nip,
+                                  int all_elastic, double *__restrict__ Ke, int32_t *__restrict__ err)
+{
+    ke_kernel_at<NN, NQ, ACH>(blockIdx.x, ne, X, conn, emat, mats, dNdR, wip, st, nip, all_elastic, Ke, err);
+}
+
 // ---------------------------------- element stiffness on the matrix cores
 //
 // HEX20 (60 dofs): Ke = sum_q Bq' (coef_q D_q Bq) as one product
@@ -405,19 +413,16 @@ __device__ inline double db_entry(int row, int n, const double *D, const double 
 }
 
 template <int NQ, int NQC>
-__global__ void __launch_bounds__(64, 2) ke_mfma_hex20_kernel(int64_t ne, const double *__restrict__ X,
-                                                              const int32_t *__restrict__ conn,
-                                                              const int32_t *__restrict__ emat,
-                                                              const AmrMat *__restrict__ mats,
-                                                              const double *__restrict__ dNdR,
-                                                              const double *__restrict__ wip,
-                                                              const double *__restrict__ st, int64_t nip,
-                                                              int all_elastic, double *__restrict__ Ke,
-                                                              int32_t *__restrict__ err)
+__device__ inline void ke_mfma_hex20_kernel_at(int64_t bid_, int64_t ne, const double *__restrict__ X,
+                                               const int32_t *__restrict__ conn, const int32_t *__restrict__ emat,
+                                               const AmrMat *__restrict__ mats, const double *__restrict__ dNdR,
+                                               const double *__restrict__ wip, const double *__restrict__ st,
+                                               int64_t nip, int all_elastic, double *__restrict__ Ke,
+                                               int32_t *__restrict__ err)
 {
     constexpr int NN = 20;
     static_assert(NQ % NQC == 0, "IP chunks");
-    const int64_t e = blockIdx.x;
+    const int64_t e = bid_;
     const int lane = threadIdx.x;
     __shared__ double sC[NN * 3];
     __shared__ double sG[NQC][NN * 3], sH[NQC][NN * 3];
@@ -517,6 +522,16 @@ __global__ void __launch_bounds__(64, 2) ke_mfma_hex20_kernel(int64_t ne, const 
     if (lane == 0 && sBad) atomicMin(err, (int32_t)(e + 1));
 }
 
+template <int NQ, int NQC>
+__global__ void __launch_bounds__(64, 2) ke_mfma_hex20_kernel(int64_t ne, const double *__restrict__ X,
+                                  const int32_t *__restrict__ conn, const int32_t *__restrict__ emat,
+                                  const AmrMat *__restrict__ mats, const double *__restrict__ dNdR,
+                                  const double *__restrict__ wip, const double *__restrict__ st, int64_t nip,
+                                  int all_elastic, double *__restrict__ Ke, int32_t *__restrict__ err)
+{
+    ke_mfma_hex20_kernel_at<NQ, NQC>(blockIdx.x, ne, X, conn, emat, mats, dNdR, wip, st, nip, all_elastic, Ke, err);
+}
+
 // Linear elastic elements (every material LinearElastic).  With the Lame
 // constants lambda = c nu, mu = c (1 - 2 nu) / 2 of the Mandel De
 // (c = E / ((1 + nu)(1 - 2 nu))), the node block (a, b) of Bq' De Bq is
@@ -532,19 +547,17 @@ __global__ void __launch_bounds__(64, 2) ke_mfma_hex20_kernel(int64_t ne, const 
 // operands as the exact kernel up to this algebra; the sums run in another
 // order, so Ke agrees with it to rounding.
 template <int NN, int NQ>
-__global__ void __launch_bounds__(64) ke_gram_kernel(int64_t ne, const double *__restrict__ X,
-                                                     const int32_t *__restrict__ conn,
-                                                     const int32_t *__restrict__ emat,
-                                                     const AmrMat *__restrict__ mats,
-                                                     const double *__restrict__ dNdR,
-                                                     const double *__restrict__ wip, double *__restrict__ Ke,
-                                                     int32_t *__restrict__ err)
+__device__ inline void ke_gram_kernel_at(int64_t bid_, int64_t ne, const double *__restrict__ X,
+                                         const int32_t *__restrict__ conn, const int32_t *__restrict__ emat,
+                                         const AmrMat *__restrict__ mats, const double *__restrict__ dNdR,
+                                         const double *__restrict__ wip, double *__restrict__ Ke,
+                                         int32_t *__restrict__ err)
 {
     constexpr int KS = (NQ + 3) / 4;                   // k-steps over the IPs
     constexpr int NAG = (NN + 3) / 4;                  // row node groups: a = 4 ag + l/16
     constexpr int MT = (3 * (NAG - 1) + 2) / 4 + 1;    // row tiles
     constexpr int NP = (NN + 15) / 16;                 // column passes of 16 nodes
-    const int64_t e = blockIdx.x;
+    const int64_t e = bid_;
     const int lane = threadIdx.x, lr = lane % 16, lk = lane / 16;
     __shared__ double sC[NN * 3];
     __shared__ double sInv[NQ][9];
@@ -649,6 +662,15 @@ __global__ void __launch_bounds__(64) ke_gram_kernel(int64_t ne, const double *_
     if (lane == 0 && sBad) atomicMin(err, (int32_t)(e + 1));
 }
 
+template <int NN, int NQ>
+__global__ void __launch_bounds__(64) ke_gram_kernel(int64_t ne, const double *__restrict__ X,
+                                  const int32_t *__restrict__ conn, const int32_t *__restrict__ emat,
+                                  const AmrMat *__restrict__ mats, const double *__restrict__ dNdR,
+                                  const double *__restrict__ wip, double *__restrict__ Ke, int32_t *__restrict__ err)
+{
+    ke_gram_kernel_at<NN, NQ>(blockIdx.x, ne, X, conn, emat, mats, dNdR, wip, Ke, err);
+}
+
 // ----------------------------------------------------------- element mass
 //
 // One 64-lane workgroup per element.  Lanes q < NQ build J (the stiffness
@@ -735,13 +757,12 @@ __global__ void __launch_bounds__(256) mass_assemble_kernel(int64_t nblk, const 
 // 0.0 (0.0 + v1 == v1).  Writes FP64 and FP32 values and, when asked, the
 // survivor flags that define the reference's scalar pattern.  Values are
 // stored in the kv_at(k, s) chunked layout.
-__global__ void __launch_bounds__(256) assemble_kernel(int64_t nblk, const int64_t *__restrict__ cptr,
-                                                       const int32_t *__restrict__ csrc,
-                                                       const double *__restrict__ Ke,
-                                                       double *__restrict__ Kv, float *__restrict__ Kv32,
-                                                       uint8_t *__restrict__ flags)
+__device__ inline void assemble_kernel_at(int64_t bid_, int64_t nblk, const int64_t *__restrict__ cptr,
+                                          const int32_t *__restrict__ csrc, const double *__restrict__ Ke,
+                                          double *__restrict__ Kv, float *__restrict__ Kv32,
+                                          uint8_t *__restrict__ flags)
 {
-    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t k = (int64_t)bid_ * blockDim.x + threadIdx.x;
     if (k >= nblk) return;
     double acc[9];
     unsigned fl = 0;
@@ -768,6 +789,13 @@ __global__ void __launch_bounds__(256) assemble_kernel(int64_t nblk, const int64
 #pragma unroll
         for (int s = 0; s < 9; s++) flags[kv_at(k, s)] = (fl >> s) & 1u;
     }
+}
+
+__global__ void __launch_bounds__(256) assemble_kernel(int64_t nblk, const int64_t *__restrict__ cptr,
+                                  const int32_t *__restrict__ csrc, const double *__restrict__ Ke,
+                                  double *__restrict__ Kv, float *__restrict__ Kv32, uint8_t *__restrict__ flags)
+{
+    assemble_kernel_at(blockIdx.x, nblk, cptr, csrc, Ke, Kv, Kv32, flags);
 }
 
 // Kp = a K + b M in K's storage (the pseudo-stiffness of the dynamic
@@ -810,11 +838,11 @@ __global__ void __launch_bounds__(256) combine_kk_kernel(int64_t nblk, double a1
 }
 
 // Jacobi inverse diagonal (0 at prescribed dofs)
-__global__ void jacobi_kernel(int64_t nnodes, int64_t nblk, const int32_t *__restrict__ diag, const double *__restrict__ Kv,
-                              const double *__restrict__ mask, double *__restrict__ dinv,
-                              float *__restrict__ dinv32)
+__device__ inline void jacobi_kernel_at(int64_t bid_, int64_t nnodes, int64_t nblk, const int32_t *__restrict__ diag,
+                                        const double *__restrict__ Kv, const double *__restrict__ mask,
+                                        double *__restrict__ dinv, float *__restrict__ dinv32)
 {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t i = (int64_t)bid_ * blockDim.x + threadIdx.x;
     if (i >= nnodes * 3) return;
     int64_t n = i / 3;
     int d = (int)(i % 3);
@@ -825,14 +853,22 @@ __global__ void jacobi_kernel(int64_t nnodes, int64_t nblk, const int32_t *__res
     dinv32[i] = fr ? 1.0f / (float)kd : 0.0f;
 }
 
+__global__ void jacobi_kernel(int64_t nnodes, int64_t nblk, const int32_t *__restrict__ diag,
+                              const double *__restrict__ Kv, const double *__restrict__ mask,
+                              double *__restrict__ dinv, float *__restrict__ dinv32)
+{
+    jacobi_kernel_at(blockIdx.x, nnodes, nblk, diag, Kv, mask, dinv, dinv32);
+}
+
 // 3x3 nodal block Jacobi: inverse of each node's (symmetrised) diagonal
 // block restricted to the node's free dofs; rows/columns of prescribed dofs
 // are zero.
-__global__ void block_jacobi_kernel(int64_t nnodes, int64_t nblk, const int32_t *__restrict__ diag,
-                                    const double *__restrict__ Kv, const double *__restrict__ mask,
-                                    double *__restrict__ binv, float *__restrict__ binv32)
+__device__ inline void block_jacobi_kernel_at(int64_t bid_, int64_t nnodes, int64_t nblk,
+                                              const int32_t *__restrict__ diag, const double *__restrict__ Kv,
+                                              const double *__restrict__ mask, double *__restrict__ binv,
+                                              float *__restrict__ binv32)
 {
-    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t n = (int64_t)bid_ * blockDim.x + threadIdx.x;
     if (n >= nnodes) return;
     const int32_t db = diag[n];
     double A[3][3], I[3][3];
@@ -867,6 +903,13 @@ __global__ void block_jacobi_kernel(int64_t nnodes, int64_t nblk, const int32_t 
     }
 }
 
+__global__ void block_jacobi_kernel(int64_t nnodes, int64_t nblk, const int32_t *__restrict__ diag,
+                                    const double *__restrict__ Kv, const double *__restrict__ mask,
+                                    double *__restrict__ binv, float *__restrict__ binv32)
+{
+    block_jacobi_kernel_at(blockIdx.x, nnodes, nblk, diag, Kv, mask, binv, binv32);
+}
+
 // ------------------------------------------------------------- IP update
 //
 // EPW elements per 256-thread workgroup.  Phase A: one thread per IP builds
@@ -877,17 +920,13 @@ __global__ void block_jacobi_kernel(int64_t nnodes, int64_t nblk, const int32_t 
 // accumulates dF[c] = fma(coef_q, (B' dsig_q)[c], dF[c]) over the IPs in
 // order.
 template <int NN, int NQ, int EPW>
-__global__ void __launch_bounds__(256) update_kernel(int64_t ne, const double *__restrict__ X,
-                                                     const int32_t *__restrict__ conn,
-                                                     const int32_t *__restrict__ emat,
-                                                     const AmrMat *__restrict__ mats,
-                                                     const double *__restrict__ dNdR,
-                                                     const double *__restrict__ wip,
-                                                     const double *__restrict__ st_in,
-                                                     double *__restrict__ st_out, int64_t nip,
-                                                     const double *__restrict__ dU,
-                                                     double *__restrict__ dF,
-                                                     int32_t *__restrict__ estat, int32_t *__restrict__ err)
+__device__ inline void update_kernel_at(int64_t bid_, int64_t ne, const double *__restrict__ X,
+                                        const int32_t *__restrict__ conn, const int32_t *__restrict__ emat,
+                                        const AmrMat *__restrict__ mats, const double *__restrict__ dNdR,
+                                        const double *__restrict__ wip, const double *__restrict__ st_in,
+                                        double *__restrict__ st_out, int64_t nip, const double *__restrict__ dU,
+                                        double *__restrict__ dF, int32_t *__restrict__ estat,
+                                        int32_t *__restrict__ err)
 {
     constexpr int ND = 3 * NN;
     __shared__ double sC[EPW][NN * 3];
@@ -896,7 +935,7 @@ __global__ void __launch_bounds__(256) update_kernel(int64_t ne, const double *_
     __shared__ double sCoef[EPW * NQ];
     __shared__ double sDs[EPW * NQ][6];
     __shared__ int sStat[EPW * NQ];
-    const int64_t e0 = (int64_t)blockIdx.x * EPW;
+    const int64_t e0 = (int64_t)bid_ * EPW;
     const int tid = threadIdx.x;
 
     for (int i = tid; i < EPW * NN * 3; i += blockDim.x) {
@@ -998,18 +1037,37 @@ __global__ void __launch_bounds__(256) update_kernel(int64_t ne, const double *_
     }
 }
 
-// dFin[3n+d] = sum over (e, la) incident to node n, in element order
-__global__ void force_gather_kernel(int64_t nnodes, int nn, const int32_t *__restrict__ fptr,
-                                    const int32_t *__restrict__ fsrc, const double *__restrict__ dF,
-                                    double *__restrict__ out)
+template <int NN, int NQ, int EPW>
+__global__ void __launch_bounds__(256) update_kernel(int64_t ne, const double *__restrict__ X,
+                                  const int32_t *__restrict__ conn, const int32_t *__restrict__ emat,
+                                  const AmrMat *__restrict__ mats, const double *__restrict__ dNdR,
+                                  const double *__restrict__ wip, const double *__restrict__ st_in,
+                                  double *__restrict__ st_out, int64_t nip, const double *__restrict__ dU,
+                                  double *__restrict__ dF, int32_t *__restrict__ estat, int32_t *__restrict__ err)
 {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    update_kernel_at<NN, NQ, EPW>(blockIdx.x, ne, X, conn, emat, mats, dNdR, wip, st_in, st_out, nip, dU, dF, estat,
+                                  err);
+}
+
+// dFin[3n+d] = sum over (e, la) incident to node n, in element order
+__device__ inline void force_gather_kernel_at(int64_t bid_, int64_t nnodes, int nn, const int32_t *__restrict__ fptr,
+                                              const int32_t *__restrict__ fsrc, const double *__restrict__ dF,
+                                              double *__restrict__ out)
+{
+    int64_t i = (int64_t)bid_ * blockDim.x + threadIdx.x;
     if (i >= nnodes * 3) return;
     int64_t n = i / 3;
     int d = (int)(i % 3);
     double acc = 0.0;
     for (int32_t p = fptr[n]; p < fptr[n + 1]; p++) acc += dF[(int64_t)fsrc[p] * 3 + d];
     out[i] = acc;
+}
+
+__global__ void force_gather_kernel(int64_t nnodes, int nn, const int32_t *__restrict__ fptr,
+                                    const int32_t *__restrict__ fsrc, const double *__restrict__ dF,
+                                    double *__restrict__ out)
+{
+    force_gather_kernel_at(blockIdx.x, nnodes, nn, fptr, fsrc, dF, out);
 }
 
 template <int NN, int NQ, int ACH>
@@ -1333,5 +1391,394 @@ int amr_launch_update(amr_model *m, const double *dU, double *dFin)
     hipLaunchKernelGGL(force_gather_kernel, dim3(g), dim3(256), 0, m->stream, m->nnodes, m->nn, m->fptr, m->fsrc,
                        m->dF, dFin);
     AMR_HIP(hipGetLastError());
+    return AMR_OK;
+}
+
+// ------------------------------------------------ several models per launch
+//
+// The element phase of a set of models (a pool's members starting or ending
+// a Newton-Raphson iteration together): each launch covers every model's
+// workgroups, in model order; a workgroup finds its model in the launch's
+// start table (binary search) and runs the single-model kernel's body on
+// its own workgroup index, so every value is computed as in the model's own
+// launch.
+
+namespace {
+
+struct ElemJob {
+    int64_t ne, nip, nblk, nnodes;
+    int nn, all_elastic;
+    const double *X;
+    const int32_t *conn, *emat;
+    const AmrMat *mats;
+    const double *dNdR, *wip, *st_cur, *st_com;
+    double *st_out, *Ke;
+    int32_t *err;
+    const int64_t *cptr;
+    const int32_t *csrc, *diag, *fptr, *fsrc;
+    double *Kv;
+    float *Kv32;
+    const double *mask;
+    double *dinv, *binv;
+    float *dinv32, *binv32;
+    const double *dU;
+    double *dF, *dFin;
+    int32_t *estat;
+};
+
+__device__ inline int job_of(const int64_t *__restrict__ start, int n, int64_t b)
+{
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) / 2;
+        if (start[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+#define AMR_JOB(J, start, n)                                  \
+    const int j_ = job_of(start, n, (int64_t)blockIdx.x);     \
+    const ElemJob &q = J[j_];                                 \
+    const int64_t b_ = (int64_t)blockIdx.x - start[j_]
+
+// the error words of every model: first failing element "none" (the bytes
+// 0x7f of the single-model reset)
+__global__ void err_reset_kernel(const ElemJob *__restrict__ J, int n)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < n)
+        for (int k = 0; k < 3; k++) J[j].err[k] = 0x7f7f7f7f;
+}
+
+template <int NN, int NQ, int ACH>
+__global__ void __launch_bounds__(64) ke_batch_kernel(const ElemJob *__restrict__ J, const int64_t *__restrict__ start,
+                                                      int n)
+{
+    AMR_JOB(J, start, n);
+    ke_kernel_at<NN, NQ, ACH>(b_, q.ne, q.X, q.conn, q.emat, q.mats, q.dNdR, q.wip, q.st_cur, q.nip, q.all_elastic,
+                              q.Ke, q.err);
+}
+
+template <int NN, int NQ>
+__global__ void __launch_bounds__(64) ke_gram_batch_kernel(const ElemJob *__restrict__ J,
+                                                           const int64_t *__restrict__ start, int n)
+{
+    AMR_JOB(J, start, n);
+    ke_gram_kernel_at<NN, NQ>(b_, q.ne, q.X, q.conn, q.emat, q.mats, q.dNdR, q.wip, q.Ke, q.err);
+}
+
+template <int NQ, int NQC>
+__global__ void __launch_bounds__(64, 2) ke_mfma_hex20_batch_kernel(const ElemJob *__restrict__ J,
+                                                                    const int64_t *__restrict__ start, int n)
+{
+    AMR_JOB(J, start, n);
+    ke_mfma_hex20_kernel_at<NQ, NQC>(b_, q.ne, q.X, q.conn, q.emat, q.mats, q.dNdR, q.wip, q.st_cur, q.nip,
+                                     q.all_elastic, q.Ke, q.err);
+}
+
+__global__ void __launch_bounds__(256) assemble_batch_kernel(const ElemJob *__restrict__ J,
+                                                             const int64_t *__restrict__ start, int n)
+{
+    AMR_JOB(J, start, n);
+    assemble_kernel_at(b_, q.nblk, q.cptr, q.csrc, q.Ke, q.Kv, q.Kv32, nullptr);
+}
+
+__global__ void __launch_bounds__(256) jacobi_batch_kernel(const ElemJob *__restrict__ J,
+                                                           const int64_t *__restrict__ start, int n)
+{
+    AMR_JOB(J, start, n);
+    jacobi_kernel_at(b_, q.nnodes, q.nblk, q.diag, q.Kv, q.mask, q.dinv, q.dinv32);
+}
+
+__global__ void __launch_bounds__(256) block_jacobi_batch_kernel(const ElemJob *__restrict__ J,
+                                                                 const int64_t *__restrict__ start, int n)
+{
+    AMR_JOB(J, start, n);
+    block_jacobi_kernel_at(b_, q.nnodes, q.nblk, q.diag, q.Kv, q.mask, q.binv, q.binv32);
+}
+
+template <int NN, int NQ, int EPW>
+__global__ void __launch_bounds__(256) update_batch_kernel(const ElemJob *__restrict__ J,
+                                                           const int64_t *__restrict__ start, int n)
+{
+    AMR_JOB(J, start, n);
+    update_kernel_at<NN, NQ, EPW>(b_, q.ne, q.X, q.conn, q.emat, q.mats, q.dNdR, q.wip, q.st_com, q.st_out, q.nip,
+                                  q.dU, q.dF, q.estat, q.err);
+}
+
+__global__ void __launch_bounds__(256) force_gather_batch_kernel(const ElemJob *__restrict__ J,
+                                                                 const int64_t *__restrict__ start, int n)
+{
+    AMR_JOB(J, start, n);
+    force_gather_kernel_at(b_, q.nnodes, q.nn, q.fptr, q.fsrc, q.dF, q.dFin);
+}
+
+#undef AMR_JOB
+
+ElemJob elem_job(const amr_model *m)
+{
+    ElemJob q{};
+    q.ne = m->ne;
+    q.nip = m->nip_total;
+    q.nblk = m->pat.nblk;
+    q.nnodes = m->nnodes;
+    q.nn = m->nn;
+    q.all_elastic = m->all_elastic ? 1 : 0;
+    q.X = m->X;
+    q.conn = m->conn;
+    q.emat = m->emat;
+    q.mats = m->mats;
+    q.dNdR = m->dNdR;
+    q.wip = m->wip;
+    q.st_cur = m->st_cur;
+    q.st_com = m->st_com;
+    q.st_out = m->st_cur;
+    q.Ke = m->Ke;
+    q.err = m->err;
+    q.cptr = m->cptr;
+    q.csrc = m->csrc;
+    q.diag = m->diag;
+    q.fptr = m->fptr;
+    q.fsrc = m->fsrc;
+    q.Kv = m->Kv;
+    q.Kv32 = m->Kv32;
+    q.mask = m->mask;
+    q.dinv = m->dinv;
+    q.binv = m->binv;
+    q.dinv32 = m->dinv32;
+    q.binv32 = m->binv32;
+    q.dF = m->dF;
+    q.estat = m->estat;
+    return q;
+}
+
+// The stiffness kernel a model's launch would use: 0..11 the exact kernel
+// (shape * 3 + tiling), 100 + shape the Gram-form matrix-core kernel, 200 /
+// 201 the HEX20 matrix-core kernel with 27 / 8 IPs.
+int ke_key(const amr_model *m)
+{
+    const int sh = shape_index(m);
+    if (m->ke_path == AMR_KE_MFMA && m->all_elastic) return 100 + sh;
+    if (m->ke_path == AMR_KE_MFMA && m->etype == AMR_HEX20) return m->nips == 27 ? 200 : 201;
+    return sh * 3 + ke_variant(m, sh);
+}
+
+int up_key(const amr_model *m)
+{
+    const int sh = shape_index(m);
+    return sh * 3 + up_variant(m, sh);
+}
+
+// One upload of the jobs and every launch's start table; returns the device
+// addresses (offsets into the staging buffer)
+struct BatchUpload {
+    size_t off = 0;
+    std::vector<std::pair<size_t, std::vector<char>>> parts;
+    template <typename T>
+    size_t add(const std::vector<T> &v)
+    {
+        const size_t o = off;
+        std::vector<char> b(v.size() * sizeof(T));
+        if (!v.empty()) std::memcpy(b.data(), v.data(), b.size());
+        parts.emplace_back(o, std::move(b));
+        off = (off + v.size() * sizeof(T) + 15) & ~(size_t)15;
+        return o;
+    }
+    int send(AmrStaging &stg, hipStream_t s)
+    {
+        if (int rc = amr_staging_reserve(stg, s, std::max<size_t>(off, 16))) return rc;
+        for (const auto &p : parts)
+            if (!p.second.empty()) std::memcpy(stg.h + p.first, p.second.data(), p.second.size());
+        AMR_HIP(hipMemcpyAsync(stg.d, stg.h, off, hipMemcpyHostToDevice, s));
+        AMR_HIP(hipEventRecord(stg.ev, s));
+        stg.pending = true;
+        return AMR_OK;
+    }
+};
+
+// start table of a launch over `idx` (indices into the jobs) with `wgs(j)`
+// workgroups for job j; the jobs of the launch are copied in that order
+template <typename F>
+std::vector<int64_t> starts(const std::vector<int> &idx, F wgs)
+{
+    std::vector<int64_t> st(idx.size() + 1, 0);
+    for (size_t i = 0; i < idx.size(); i++) st[i + 1] = st[i] + wgs(idx[i]);
+    return st;
+}
+
+int launch_ke_batch(int key, const ElemJob *J, const int64_t *st, int n, unsigned grid, hipStream_t s)
+{
+#define KEB(NN, NQ, A) hipLaunchKernelGGL((ke_batch_kernel<NN, NQ, A>), dim3(grid), dim3(64), 0, s, J, st, n)
+    switch (key) {
+    case 0: KEB(8, 8, 1); break;
+    case 1: KEB(8, 8, 2); break;
+    case 2: KEB(8, 8, 4); break;
+    case 3: KEB(20, 27, 7); break;
+    case 4: KEB(20, 27, 10); break;
+    case 5: KEB(20, 27, 20); break;
+    case 6: KEB(20, 8, 7); break;
+    case 7: KEB(20, 8, 10); break;
+    case 8: KEB(20, 8, 20); break;
+    case 9: KEB(10, 4, 2); break;
+    case 10: KEB(10, 4, 5); break;
+    case 11: KEB(10, 4, 10); break;
+    case 100: hipLaunchKernelGGL((ke_gram_batch_kernel<8, 8>), dim3(grid), dim3(64), 0, s, J, st, n); break;
+    case 101: hipLaunchKernelGGL((ke_gram_batch_kernel<20, 27>), dim3(grid), dim3(64), 0, s, J, st, n); break;
+    case 102: hipLaunchKernelGGL((ke_gram_batch_kernel<20, 8>), dim3(grid), dim3(64), 0, s, J, st, n); break;
+    case 103: hipLaunchKernelGGL((ke_gram_batch_kernel<10, 4>), dim3(grid), dim3(64), 0, s, J, st, n); break;
+    case 200: hipLaunchKernelGGL((ke_mfma_hex20_batch_kernel<27, 9>), dim3(grid), dim3(64), 0, s, J, st, n); break;
+    case 201: hipLaunchKernelGGL((ke_mfma_hex20_batch_kernel<8, 8>), dim3(grid), dim3(64), 0, s, J, st, n); break;
+    default: amr_set_error("batched element stiffness: unknown kernel"); return AMR_ERR_ARG;
+    }
+#undef KEB
+    AMR_HIP(hipGetLastError());
+    return AMR_OK;
+}
+
+int launch_update_batch(int key, const ElemJob *J, const int64_t *st, int n, unsigned grid, hipStream_t s)
+{
+#define UPB(NN, NQ, E) hipLaunchKernelGGL((update_batch_kernel<NN, NQ, E>), dim3(grid), dim3(256), 0, s, J, st, n)
+    switch (key) {
+    case 0: UPB(8, 8, 32); break;
+    case 1: UPB(8, 8, 16); break;
+    case 2: UPB(8, 8, 8); break;
+    case 3: UPB(20, 27, 9); break;
+    case 4: UPB(20, 27, 4); break;
+    case 5: UPB(20, 27, 2); break;
+    case 6: UPB(20, 8, 16); break;
+    case 7: UPB(20, 8, 8); break;
+    case 8: UPB(20, 8, 32); break;
+    case 9: UPB(10, 4, 32); break;
+    case 10: UPB(10, 4, 16); break;
+    case 11: UPB(10, 4, 64); break;
+    default: amr_set_error("batched update: unknown kernel"); return AMR_ERR_ARG;
+    }
+#undef UPB
+    AMR_HIP(hipGetLastError());
+    return AMR_OK;
+}
+
+// workgroups of a model's update launch with tiling key k
+int64_t update_wgs(int key, int64_t ne)
+{
+    constexpr int EPW[12] = {32, 16, 8, 9, 4, 2, 16, 8, 32, 32, 16, 64};
+    return (ne + EPW[key] - 1) / EPW[key];
+}
+
+}  // namespace
+
+int amr_batch_mount_K(amr_model *const *ms, int n, hipStream_t s, AmrStaging &stg)
+{
+    if (n <= 0) return AMR_OK;
+    std::vector<ElemJob> jobs(n);
+    for (int j = 0; j < n; j++) {
+        if (shape_index(ms[j]) < 0) {
+            amr_set_error("unknown element type");
+            return AMR_ERR_ARG;
+        }
+        jobs[j] = elem_job(ms[j]);
+    }
+    // the stiffness launches: one per kernel the models use
+    std::map<int, std::vector<int>> groups;
+    for (int j = 0; j < n; j++)
+        if (ms[j]->ne > 0) groups[ke_key(ms[j])].push_back(j);
+    std::vector<int> all, withk;
+    for (int j = 0; j < n; j++) {
+        all.push_back(j);
+        if (jobs[j].nblk > 0) withk.push_back(j);
+    }
+    BatchUpload up;
+    const size_t o_all = up.add(jobs);
+    struct L {
+        int key;
+        size_t o_jobs, o_st;
+        int n;
+        int64_t wgs;
+    };
+    std::vector<L> ke;
+    auto add_launch = [&](const std::vector<int> &idx, auto wgs, int key) {
+        std::vector<ElemJob> jj;
+        for (int j : idx) jj.push_back(jobs[j]);
+        const std::vector<int64_t> st = starts(idx, wgs);
+        L l{key, up.add(jj), 0, (int)idx.size(), st.back()};
+        l.o_st = up.add(st);
+        return l;
+    };
+    for (auto &g : groups) ke.push_back(add_launch(g.second, [&](int j) { return jobs[j].ne; }, g.first));
+    const L as = add_launch(withk, [&](int j) { return (jobs[j].nblk + 255) / 256; }, 0);
+    const L jc = add_launch(withk, [&](int j) { return (jobs[j].nnodes * 3 + 255) / 256; }, 0);
+    const L bj = add_launch(withk, [&](int j) { return (jobs[j].nnodes + 255) / 256; }, 0);
+    if (int rc = up.send(stg, s)) return rc;
+    auto dj = [&](size_t o) { return (const ElemJob *)(stg.d + o); };
+    auto ds = [&](size_t o) { return (const int64_t *)(stg.d + o); };
+    hipLaunchKernelGGL(err_reset_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, dj(o_all), n);
+    AMR_HIP(hipGetLastError());
+    for (const L &l : ke)
+        if (l.wgs > 0)
+            if (int rc = launch_ke_batch(l.key, dj(l.o_jobs), ds(l.o_st), l.n, (unsigned)l.wgs, s)) return rc;
+    if (as.wgs > 0) {
+        hipLaunchKernelGGL(assemble_batch_kernel, dim3((unsigned)as.wgs), dim3(256), 0, s, dj(as.o_jobs),
+                           ds(as.o_st), as.n);
+        hipLaunchKernelGGL(jacobi_batch_kernel, dim3((unsigned)jc.wgs), dim3(256), 0, s, dj(jc.o_jobs), ds(jc.o_st),
+                           jc.n);
+        hipLaunchKernelGGL(block_jacobi_batch_kernel, dim3((unsigned)bj.wgs), dim3(256), 0, s, dj(bj.o_jobs),
+                           ds(bj.o_st), bj.n);
+        AMR_HIP(hipGetLastError());
+    }
+    for (int j = 0; j < n; j++) ms[j]->k_version++;
+    return AMR_OK;
+}
+
+int amr_batch_update(amr_model *const *ms, const double *const *dU, double *const *dFin, int n, hipStream_t s,
+                     AmrStaging &stg)
+{
+    if (n <= 0) return AMR_OK;
+    std::vector<ElemJob> jobs(n);
+    std::map<int, std::vector<int>> groups;
+    std::vector<int> all;
+    for (int j = 0; j < n; j++) {
+        if (shape_index(ms[j]) < 0) {
+            amr_set_error("unknown element type");
+            return AMR_ERR_ARG;
+        }
+        jobs[j] = elem_job(ms[j]);
+        jobs[j].dU = dU[j];
+        jobs[j].dFin = dFin[j];
+        if (ms[j]->ne > 0) groups[up_key(ms[j])].push_back(j);
+        all.push_back(j);
+    }
+    BatchUpload up;
+    struct L {
+        int key;
+        size_t o_jobs, o_st;
+        int n;
+        int64_t wgs;
+    };
+    auto add_launch = [&](const std::vector<int> &idx, auto wgs, int key) {
+        std::vector<ElemJob> jj;
+        for (int j : idx) jj.push_back(jobs[j]);
+        const std::vector<int64_t> st = starts(idx, wgs);
+        L l{key, up.add(jj), 0, (int)idx.size(), st.back()};
+        l.o_st = up.add(st);
+        return l;
+    };
+    std::vector<L> ul;
+    for (auto &g : groups) {
+        const int key = g.first;
+        ul.push_back(add_launch(g.second, [&](int j) { return update_wgs(key, jobs[j].ne); }, key));
+    }
+    const L fg = add_launch(all, [&](int j) { return (jobs[j].nnodes * 3 + 255) / 256; }, 0);
+    if (int rc = up.send(stg, s)) return rc;
+    auto dj = [&](size_t o) { return (const ElemJob *)(stg.d + o); };
+    auto ds = [&](size_t o) { return (const int64_t *)(stg.d + o); };
+    for (const L &l : ul)
+        if (l.wgs > 0)
+            if (int rc = launch_update_batch(l.key, dj(l.o_jobs), ds(l.o_st), l.n, (unsigned)l.wgs, s)) return rc;
+    if (fg.wgs > 0) {
+        hipLaunchKernelGGL(force_gather_batch_kernel, dim3((unsigned)fg.wgs), dim3(256), 0, s, dj(fg.o_jobs),
+                           ds(fg.o_st), fg.n);
+        AMR_HIP(hipGetLastError());
+    }
     return AMR_OK;
 }

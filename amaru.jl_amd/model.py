@@ -615,6 +615,30 @@ class DeviceModel:
         si = ctypes.byref(solve_info) if solve_info is not None else None
         _lib.check(_lib.lib().amr_nr_finish_begin(self._h, si))
 
+    @staticmethod
+    def nr_prepare_many(devs, its):
+        """nr_prepare(it) of every model, their element kernels batched (one
+        launch per kernel family for all of them)."""
+        import ctypes
+        n = len(devs)
+        if n == 0:
+            return
+        hs = (ctypes.c_void_p * n)(*[d._h.value for d in devs])
+        iv = (ctypes.c_int * n)(*[int(i) for i in its])
+        _lib.check(_lib.lib().amr_nr_prepare_many(hs, iv, n))
+
+    @staticmethod
+    def nr_finish_begin_many(devs, solve_infos):
+        """nr_finish_begin of every model (a SolveInfo each), their update
+        kernels batched."""
+        import ctypes
+        n = len(devs)
+        if n == 0:
+            return
+        hs = (ctypes.c_void_p * n)(*[d._h.value for d in devs])
+        si = (_lib.SolveInfo * n)(*solve_infos)
+        _lib.check(_lib.lib().amr_nr_finish_begin_many(hs, si, n))
+
     def nr_finish_end(self):
         """Second half of nr_finish: wait and return what nr_iteration returns."""
         import ctypes

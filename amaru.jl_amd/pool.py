@@ -157,55 +157,111 @@ class Job:
     def resume_end(self):
         return self._advance(self.ana.dev.nr_finish_end())
 
-
-    def _advance(self, r, first=False):
+    def next_solve(self, r=None, first=False):
+        """Advances the machine (sending it r) to its next Newton-Raphson
+        iteration that needs a pooled linear solve and returns that
+        iteration's number, or None when the analysis has ended (result
+        set).  The iterations of a model without unknowns are run here."""
         try:
             it = next(self.gen) if first else self.gen.send(r)
             while True:
                 dev = self.ana.dev
                 self.pool.attach(dev)
-                dev.nr_prepare(it)
                 if dev.model.nu == 0:
+                    dev.nr_prepare(it)
                     it = self.gen.send(dev.nr_finish(None))
                     continue
-                self.pool.add(dev, self.ana.opts)
-                return False
+                return it
         except StopIteration as stop:
             self.result = stop.value
             self.done = True
+            return None
+
+    def join(self):
+        """The prepared iteration's linear solve joins the pool."""
+        self.pool.add(self.ana.dev, self.ana.opts)
+
+    def _advance(self, r, first=False):
+        it = self.next_solve(r, first)
+        if it is None:
             return True
+        self.ana.dev.nr_prepare(it)
+        self.join()
+        return False
+
+
+def prepare_and_join(pairs):
+    """[(job, it)]: mount_K and the right-hand side of every job's iteration
+    it, their element kernels batched across the models, then each job's
+    solve joins its pool."""
+    if not pairs:
+        return
+    devs = [j.ana.dev for j, _ in pairs]
+    type(devs[0]).nr_prepare_many(devs, [it for _, it in pairs])
+    for j, _ in pairs:
+        j.join()
+
+
+def start_all(jobs):
+    """Job.start for every job, their first mount_K batched; returns each
+    job's done flag."""
+    pairs, done = [], []
+    for j in jobs:
+        it = j.next_solve(first=True)
+        done.append(it is None)
+        if it is not None:
+            pairs.append((j, it))
+    prepare_and_join(pairs)
+    return done
 
 
 def resume_all(finished, jobs_of, catch=False, between=None):
     """Resume the jobs of the members a step finished: all NR finishes are
-    queued first and then completed, so they share one wait; ``between()``
-    runs after the finishes are queued (the next pool step is launched
-    there, so that it overlaps the host work of completing them).  Yields
-    (handle, job, done), or with catch=True (handle, job, done, exception)
-    so that one failing job does not stop the others."""
+    queued first (their update kernels batched across the models) and then
+    completed, so they share one wait; ``between()`` runs after the finishes
+    are queued (the next pool step is launched there, so that it overlaps
+    the host work of completing them); the next iterations' mount_K are
+    batched too.  Yields (handle, job, done), or with catch=True (handle,
+    job, done, exception) so that one failing job does not stop the
+    others."""
     jobs = [(h, jobs_of(h)) for h, _ in finished]
     errs = {}
-    for (h, job), (_, si) in zip(jobs, finished):
-        try:
-            job.resume_begin(si)
-        except Exception as e:
-            if not catch:
-                raise
-            errs[h] = e
+    try:
+        if jobs:
+            devs = [job.ana.dev for _, job in jobs]
+            type(devs[0]).nr_finish_begin_many(devs, [si for _, si in finished])
+    except Exception as e:
+        if not catch:
+            raise
+        errs = {h: e for h, _ in jobs}
     if between is not None:
         between()
+    out, pairs = [], []
     for h, job in jobs:
         if h in errs:
-            yield h, job, True, errs[h]
+            out.append((h, job, True, errs[h]))
             continue
         try:
-            done = job.resume_end()
+            it = job.next_solve(job.ana.dev.nr_finish_end())
         except Exception as e:
             if not catch:
                 raise
-            yield h, job, True, e
+            out.append((h, job, True, e))
             continue
-        yield (h, job, done, None) if catch else (h, job, done)
+        if it is None:
+            out.append((h, job, True, None))
+        else:
+            pairs.append((job, it))
+            out.append((h, job, False, None))
+    try:
+        prepare_and_join(pairs)
+    except Exception as e:
+        if not catch:
+            raise
+        failed = {id(j) for j, _ in pairs}
+        out = [(h, job, True, e) if id(job) in failed else r for (h, job, d, x), r in zip(out, out)]
+    for h, job, done, exc in out:
+        yield (h, job, done, exc) if catch else (h, job, done)
 
 
 class _Pipeline:
@@ -357,12 +413,12 @@ def run_analyses(analyses, pool=None, device=0, check_every=32, npools=1, **kwar
         results = [None] * len(analyses)
         order = sorted(range(len(analyses)), key=lambda i: -work_weight(analyses[i]))
         home = {i: ps.assign(work_weight(analyses[i])) for i in order}
-        for i, ana in enumerate(analyses):
-            job = Job(ana, args, ps.pools[home[i]])
-            if job.start():
+        started = [Job(ana, args, ps.pools[home[i]]) for i, ana in enumerate(analyses)]
+        for i, (job, done) in enumerate(zip(started, start_all(started))):
+            if done:
                 results[i] = job.result
             else:
-                jobs[ana.dev.handle.value] = (i, job)
+                jobs[job.ana.dev.handle.value] = (i, job)
         while jobs:
             ps.launch_idle()
             for k, fin in ps.collect(block=True):
@@ -645,13 +701,15 @@ class Server:
                     self.t_loop["step"] += time.perf_counter() - ts
                     ts = time.perf_counter()
                     for k, fin in fins:
-                        for h, si in fin:
-                            req = running.pop(h)
-                            try:
-                                req.job.resume_begin(si)
-                                pending.append((req, None))
-                            except Exception as e:
-                                pending.append((req, e))
+                        # one batched NR finish (update kernels) per pool
+                        done_reqs = [running.pop(h) for h, _ in fin]
+                        try:
+                            if done_reqs:
+                                devs = [r.ana.dev for r in done_reqs]
+                                type(devs[0]).nr_finish_begin_many(devs, [si for _, si in fin])
+                            pending.extend((r, None) for r in done_reqs)
+                        except Exception as e:
+                            pending.extend((r, e) for r in done_reqs)
                         self.pools.relauncher(k)()
                     if pending:
                         req, exc = pending.popleft()

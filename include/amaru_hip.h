@@ -288,6 +288,17 @@ int amr_nr_finish(amr_model *m, const amr_solve_info *si, amr_nr_info *info);
  * all, so they share one wait. */
 int amr_nr_finish_begin(amr_model *m, const amr_solve_info *si);
 int amr_nr_finish_end(amr_model *m, amr_nr_info *info);
+/* The same for n models at once (replacing the per-element loops of
+ * mount_K and update_state!, src/mech/mech-solver.jl:78-144, for every
+ * model): amr_nr_prepare_many equals amr_nr_prepare(ms[i], its[i]) for each
+ * i, amr_nr_finish_begin_many equals amr_nr_finish_begin(ms[i], &si[i])
+ * (si may be NULL when no model has unknowns), with the element kernels of
+ * all n models in one launch per kernel family (element stiffness per kernel
+ * variant, assembly, Jacobi inverses; IP updates per tiling, force gather).
+ * Results are the same bits.  The models are on one device and none is
+ * solving in a pool; each appears once. */
+int amr_nr_prepare_many(amr_model **ms, const int *its, int n);
+int amr_nr_finish_begin_many(amr_model **ms, const amr_solve_info *si, int n);
 
 /* ------------------------------------------------------------------------
  * Pools: the linear solves of many models advanced together, one launch

@@ -68,6 +68,16 @@ class FakeDev:
     def nr_finish_end(self):
         return self._result(self._pending)
 
+    @staticmethod
+    def nr_prepare_many(devs, its):
+        for d, it in zip(devs, its):
+            d.nr_prepare(it)
+
+    @staticmethod
+    def nr_finish_begin_many(devs, sis):
+        for d, si in zip(devs, sis):
+            d.nr_finish_begin(si)
+
     def increment_commit(self):
         self.calls.append(("commit",))
 
