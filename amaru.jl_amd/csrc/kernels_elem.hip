@@ -235,12 +235,15 @@ __device__ inline double inv3_entry(const double *J, int k, double *det_out)
 // evaluates calcD for plastic states, then every lane forms DB for its
 // column node (54 fma) and updates its blocks (36 fma per block).
 
+// Each kernel's body is a device function of its workgroup index, inlined
+// into the single-model kernel (the workgroup index is blockIdx.x) and into
+// the batched kernel of several models (the index within its model).
 template <int NN, int NQ, int ACH>
-__device__ inline void ke_kernel_at(int64_t bid_, int64_t ne, const double *__restrict__ X,
-                                    const int32_t *__restrict__ conn, const int32_t *__restrict__ emat,
-                                    const AmrMat *__restrict__ mats, const double *__restrict__ dNdR,
-                                    const double *__restrict__ wip, const double *__restrict__ st, int64_t nip,
-                                    int all_elastic, double *__restrict__ Ke, int32_t *__restrict__ err)
+__device__ __attribute__((always_inline)) inline void
+ke_kernel_at(int64_t bid_, int64_t ne, const double *__restrict__ X, const int32_t *__restrict__ conn,
+             const int32_t *__restrict__ emat, const AmrMat *__restrict__ mats, const double *__restrict__ dNdR,
+             const double *__restrict__ wip, const double *__restrict__ st, int64_t nip, int all_elastic,
+             double *__restrict__ Ke, int32_t *__restrict__ err)
 {
     constexpr int NCH = (NN + ACH - 1) / ACH;
     const int64_t e = bid_;
@@ -413,12 +416,12 @@ __device__ inline double db_entry(int row, int n, const double *D, const double 
 }
 
 template <int NQ, int NQC>
-__device__ inline void ke_mfma_hex20_kernel_at(int64_t bid_, int64_t ne, const double *__restrict__ X,
-                                               const int32_t *__restrict__ conn, const int32_t *__restrict__ emat,
-                                               const AmrMat *__restrict__ mats, const double *__restrict__ dNdR,
-                                               const double *__restrict__ wip, const double *__restrict__ st,
-                                               int64_t nip, int all_elastic, double *__restrict__ Ke,
-                                               int32_t *__restrict__ err)
+__device__ __attribute__((always_inline)) inline void
+ke_mfma_hex20_kernel_at(int64_t bid_, int64_t ne, const double *__restrict__ X, const int32_t *__restrict__ conn,
+                        const int32_t *__restrict__ emat, const AmrMat *__restrict__ mats,
+                        const double *__restrict__ dNdR, const double *__restrict__ wip,
+                        const double *__restrict__ st, int64_t nip, int all_elastic, double *__restrict__ Ke,
+                        int32_t *__restrict__ err)
 {
     constexpr int NN = 20;
     static_assert(NQ % NQC == 0, "IP chunks");
@@ -547,11 +550,11 @@ __global__ void __launch_bounds__(64, 2) ke_mfma_hex20_kernel(int64_t ne, const 
 // operands as the exact kernel up to this algebra; the sums run in another
 // order, so Ke agrees with it to rounding.
 template <int NN, int NQ>
-__device__ inline void ke_gram_kernel_at(int64_t bid_, int64_t ne, const double *__restrict__ X,
-                                         const int32_t *__restrict__ conn, const int32_t *__restrict__ emat,
-                                         const AmrMat *__restrict__ mats, const double *__restrict__ dNdR,
-                                         const double *__restrict__ wip, double *__restrict__ Ke,
-                                         int32_t *__restrict__ err)
+__device__ __attribute__((always_inline)) inline void
+ke_gram_kernel_at(int64_t bid_, int64_t ne, const double *__restrict__ X, const int32_t *__restrict__ conn,
+                  const int32_t *__restrict__ emat, const AmrMat *__restrict__ mats,
+                  const double *__restrict__ dNdR, const double *__restrict__ wip, double *__restrict__ Ke,
+                  int32_t *__restrict__ err)
 {
     constexpr int KS = (NQ + 3) / 4;                   // k-steps over the IPs
     constexpr int NAG = (NN + 3) / 4;                  // row node groups: a = 4 ag + l/16
@@ -757,10 +760,10 @@ __global__ void __launch_bounds__(256) mass_assemble_kernel(int64_t nblk, const 
 // 0.0 (0.0 + v1 == v1).  Writes FP64 and FP32 values and, when asked, the
 // survivor flags that define the reference's scalar pattern.  Values are
 // stored in the kv_at(k, s) chunked layout.
-__device__ inline void assemble_kernel_at(int64_t bid_, int64_t nblk, const int64_t *__restrict__ cptr,
-                                          const int32_t *__restrict__ csrc, const double *__restrict__ Ke,
-                                          double *__restrict__ Kv, float *__restrict__ Kv32,
-                                          uint8_t *__restrict__ flags)
+__device__ __attribute__((always_inline)) inline void
+assemble_kernel_at(int64_t bid_, int64_t nblk, const int64_t *__restrict__ cptr, const int32_t *__restrict__ csrc,
+                   const double *__restrict__ Ke, double *__restrict__ Kv, float *__restrict__ Kv32,
+                   uint8_t *__restrict__ flags)
 {
     int64_t k = (int64_t)bid_ * blockDim.x + threadIdx.x;
     if (k >= nblk) return;
@@ -838,9 +841,10 @@ __global__ void __launch_bounds__(256) combine_kk_kernel(int64_t nblk, double a1
 }
 
 // Jacobi inverse diagonal (0 at prescribed dofs)
-__device__ inline void jacobi_kernel_at(int64_t bid_, int64_t nnodes, int64_t nblk, const int32_t *__restrict__ diag,
-                                        const double *__restrict__ Kv, const double *__restrict__ mask,
-                                        double *__restrict__ dinv, float *__restrict__ dinv32)
+__device__ __attribute__((always_inline)) inline void
+jacobi_kernel_at(int64_t bid_, int64_t nnodes, int64_t nblk, const int32_t *__restrict__ diag,
+                 const double *__restrict__ Kv, const double *__restrict__ mask, double *__restrict__ dinv,
+                 float *__restrict__ dinv32)
 {
     int64_t i = (int64_t)bid_ * blockDim.x + threadIdx.x;
     if (i >= nnodes * 3) return;
@@ -863,10 +867,10 @@ __global__ void jacobi_kernel(int64_t nnodes, int64_t nblk, const int32_t *__res
 // 3x3 nodal block Jacobi: inverse of each node's (symmetrised) diagonal
 // block restricted to the node's free dofs; rows/columns of prescribed dofs
 // are zero.
-__device__ inline void block_jacobi_kernel_at(int64_t bid_, int64_t nnodes, int64_t nblk,
-                                              const int32_t *__restrict__ diag, const double *__restrict__ Kv,
-                                              const double *__restrict__ mask, double *__restrict__ binv,
-                                              float *__restrict__ binv32)
+__device__ __attribute__((always_inline)) inline void
+block_jacobi_kernel_at(int64_t bid_, int64_t nnodes, int64_t nblk, const int32_t *__restrict__ diag,
+                       const double *__restrict__ Kv, const double *__restrict__ mask, double *__restrict__ binv,
+                       float *__restrict__ binv32)
 {
     const int64_t n = (int64_t)bid_ * blockDim.x + threadIdx.x;
     if (n >= nnodes) return;
@@ -920,13 +924,12 @@ __global__ void block_jacobi_kernel(int64_t nnodes, int64_t nblk, const int32_t 
 // accumulates dF[c] = fma(coef_q, (B' dsig_q)[c], dF[c]) over the IPs in
 // order.
 template <int NN, int NQ, int EPW>
-__device__ inline void update_kernel_at(int64_t bid_, int64_t ne, const double *__restrict__ X,
-                                        const int32_t *__restrict__ conn, const int32_t *__restrict__ emat,
-                                        const AmrMat *__restrict__ mats, const double *__restrict__ dNdR,
-                                        const double *__restrict__ wip, const double *__restrict__ st_in,
-                                        double *__restrict__ st_out, int64_t nip, const double *__restrict__ dU,
-                                        double *__restrict__ dF, int32_t *__restrict__ estat,
-                                        int32_t *__restrict__ err)
+__device__ __attribute__((always_inline)) inline void
+update_kernel_at(int64_t bid_, int64_t ne, const double *__restrict__ X, const int32_t *__restrict__ conn,
+                 const int32_t *__restrict__ emat, const AmrMat *__restrict__ mats, const double *__restrict__ dNdR,
+                 const double *__restrict__ wip, const double *__restrict__ st_in, double *__restrict__ st_out,
+                 int64_t nip, const double *__restrict__ dU, double *__restrict__ dF, int32_t *__restrict__ estat,
+                 int32_t *__restrict__ err)
 {
     constexpr int ND = 3 * NN;
     __shared__ double sC[EPW][NN * 3];
@@ -1050,9 +1053,9 @@ __global__ void __launch_bounds__(256) update_kernel(int64_t ne, const double *_
 }
 
 // dFin[3n+d] = sum over (e, la) incident to node n, in element order
-__device__ inline void force_gather_kernel_at(int64_t bid_, int64_t nnodes, int nn, const int32_t *__restrict__ fptr,
-                                              const int32_t *__restrict__ fsrc, const double *__restrict__ dF,
-                                              double *__restrict__ out)
+__device__ __attribute__((always_inline)) inline void
+force_gather_kernel_at(int64_t bid_, int64_t nnodes, int nn, const int32_t *__restrict__ fptr,
+                       const int32_t *__restrict__ fsrc, const double *__restrict__ dF, double *__restrict__ out)
 {
     int64_t i = (int64_t)bid_ * blockDim.x + threadIdx.x;
     if (i >= nnodes * 3) return;
@@ -1405,36 +1408,49 @@ int amr_launch_update(amr_model *m, const double *dU, double *dFin)
 
 namespace {
 
+#define AMR_AS1 __attribute__((address_space(1)))
+// Pointer fields in the global address space: a kernel converts them to
+// the bodies' generic parameters with one cast the compiler sees through,
+// so the bodies use global (not flat) memory instructions, as with kernel
+// arguments.
 struct ElemJob {
     int64_t ne, nip, nblk, nnodes;
     int nn, all_elastic;
-    const double *X;
-    const int32_t *conn, *emat;
-    const AmrMat *mats;
-    const double *dNdR, *wip, *st_cur, *st_com;
-    double *st_out, *Ke;
-    int32_t *err;
-    const int64_t *cptr;
-    const int32_t *csrc, *diag, *fptr, *fsrc;
-    double *Kv;
-    float *Kv32;
-    const double *mask;
-    double *dinv, *binv;
-    float *dinv32, *binv32;
-    const double *dU;
-    double *dF, *dFin;
-    int32_t *estat;
+    const AMR_AS1 double *X;
+    const AMR_AS1 int32_t *conn, *emat;
+    const AMR_AS1 AmrMat *mats;
+    const AMR_AS1 double *dNdR, *wip, *st_cur, *st_com;
+    AMR_AS1 double *st_out, *Ke;
+    AMR_AS1 int32_t *err;
+    const AMR_AS1 int64_t *cptr;
+    const AMR_AS1 int32_t *csrc, *diag, *fptr, *fsrc;
+    AMR_AS1 double *Kv;
+    AMR_AS1 float *Kv32;
+    const AMR_AS1 double *mask;
+    AMR_AS1 double *dinv, *binv;
+    AMR_AS1 float *dinv32, *binv32;
+    const AMR_AS1 double *dU;
+    AMR_AS1 double *dF, *dFin;
+    AMR_AS1 int32_t *estat;
 };
+
+// The launch's start table holds n + 1 workgroup offsets, then (from entry
+// n + 1 on) the job of every 1,024th workgroup: a workgroup reads that hint
+// and steps forward over the (few) jobs that start before it, a short chain
+// of dependent loads where a binary search would take log2(n).
+constexpr int HINT_SHIFT = 10;
 
 __device__ inline int job_of(const int64_t *__restrict__ start, int n, int64_t b)
 {
-    int lo = 0, hi = n - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) / 2;
-        if (start[mid] <= b) lo = mid;
-        else hi = mid - 1;
-    }
-    return lo;
+    int j = (int)start[n + 1 + (b >> HINT_SHIFT)];
+    while (j + 1 < n && start[j + 1] <= b) j++;
+    return j;
+}
+
+template <typename T>
+__device__ inline T *gen(AMR_AS1 T *p)
+{
+    return (T *)p;
 }
 
 #define AMR_JOB(J, start, n)                                  \
@@ -1456,8 +1472,9 @@ __global__ void __launch_bounds__(64) ke_batch_kernel(const ElemJob *__restrict_
                                                       int n)
 {
     AMR_JOB(J, start, n);
-    ke_kernel_at<NN, NQ, ACH>(b_, q.ne, q.X, q.conn, q.emat, q.mats, q.dNdR, q.wip, q.st_cur, q.nip, q.all_elastic,
-                              q.Ke, q.err);
+    ke_kernel_at<NN, NQ, ACH>(b_, q.ne, gen(q.X), gen(q.conn), gen(q.emat), gen(q.mats), gen(q.dNdR), gen(q.wip),
+                              gen(q.st_cur), q.nip, q.all_elastic,
+                              gen(q.Ke), gen(q.err));
 }
 
 template <int NN, int NQ>
@@ -1465,7 +1482,8 @@ __global__ void __launch_bounds__(64) ke_gram_batch_kernel(const ElemJob *__rest
                                                            const int64_t *__restrict__ start, int n)
 {
     AMR_JOB(J, start, n);
-    ke_gram_kernel_at<NN, NQ>(b_, q.ne, q.X, q.conn, q.emat, q.mats, q.dNdR, q.wip, q.Ke, q.err);
+    ke_gram_kernel_at<NN, NQ>(b_, q.ne, gen(q.X), gen(q.conn), gen(q.emat), gen(q.mats), gen(q.dNdR), gen(q.wip),
+                              gen(q.Ke), gen(q.err));
 }
 
 template <int NQ, int NQC>
@@ -1473,29 +1491,30 @@ __global__ void __launch_bounds__(64, 2) ke_mfma_hex20_batch_kernel(const ElemJo
                                                                     const int64_t *__restrict__ start, int n)
 {
     AMR_JOB(J, start, n);
-    ke_mfma_hex20_kernel_at<NQ, NQC>(b_, q.ne, q.X, q.conn, q.emat, q.mats, q.dNdR, q.wip, q.st_cur, q.nip,
-                                     q.all_elastic, q.Ke, q.err);
+    ke_mfma_hex20_kernel_at<NQ, NQC>(b_, q.ne, gen(q.X), gen(q.conn), gen(q.emat), gen(q.mats), gen(q.dNdR),
+                                     gen(q.wip), gen(q.st_cur), q.nip,
+                                     q.all_elastic, gen(q.Ke), gen(q.err));
 }
 
 __global__ void __launch_bounds__(256) assemble_batch_kernel(const ElemJob *__restrict__ J,
                                                              const int64_t *__restrict__ start, int n)
 {
     AMR_JOB(J, start, n);
-    assemble_kernel_at(b_, q.nblk, q.cptr, q.csrc, q.Ke, q.Kv, q.Kv32, nullptr);
+    assemble_kernel_at(b_, q.nblk, gen(q.cptr), gen(q.csrc), gen(q.Ke), gen(q.Kv), gen(q.Kv32), nullptr);
 }
 
 __global__ void __launch_bounds__(256) jacobi_batch_kernel(const ElemJob *__restrict__ J,
                                                            const int64_t *__restrict__ start, int n)
 {
     AMR_JOB(J, start, n);
-    jacobi_kernel_at(b_, q.nnodes, q.nblk, q.diag, q.Kv, q.mask, q.dinv, q.dinv32);
+    jacobi_kernel_at(b_, q.nnodes, q.nblk, gen(q.diag), gen(q.Kv), gen(q.mask), gen(q.dinv), gen(q.dinv32));
 }
 
 __global__ void __launch_bounds__(256) block_jacobi_batch_kernel(const ElemJob *__restrict__ J,
                                                                  const int64_t *__restrict__ start, int n)
 {
     AMR_JOB(J, start, n);
-    block_jacobi_kernel_at(b_, q.nnodes, q.nblk, q.diag, q.Kv, q.mask, q.binv, q.binv32);
+    block_jacobi_kernel_at(b_, q.nnodes, q.nblk, gen(q.diag), gen(q.Kv), gen(q.mask), gen(q.binv), gen(q.binv32));
 }
 
 template <int NN, int NQ, int EPW>
@@ -1503,18 +1522,25 @@ __global__ void __launch_bounds__(256) update_batch_kernel(const ElemJob *__rest
                                                            const int64_t *__restrict__ start, int n)
 {
     AMR_JOB(J, start, n);
-    update_kernel_at<NN, NQ, EPW>(b_, q.ne, q.X, q.conn, q.emat, q.mats, q.dNdR, q.wip, q.st_com, q.st_out, q.nip,
-                                  q.dU, q.dF, q.estat, q.err);
+    update_kernel_at<NN, NQ, EPW>(b_, q.ne, gen(q.X), gen(q.conn), gen(q.emat), gen(q.mats), gen(q.dNdR), gen(q.wip),
+                                  gen(q.st_com), gen(q.st_out), q.nip,
+                                  gen(q.dU), gen(q.dF), gen(q.estat), gen(q.err));
 }
 
 __global__ void __launch_bounds__(256) force_gather_batch_kernel(const ElemJob *__restrict__ J,
                                                                  const int64_t *__restrict__ start, int n)
 {
     AMR_JOB(J, start, n);
-    force_gather_kernel_at(b_, q.nnodes, q.nn, q.fptr, q.fsrc, q.dF, q.dFin);
+    force_gather_kernel_at(b_, q.nnodes, q.nn, gen(q.fptr), gen(q.fsrc), gen(q.dF), gen(q.dFin));
 }
 
 #undef AMR_JOB
+
+template <typename T>
+AMR_AS1 T *as1(T *p)
+{
+    return (AMR_AS1 T *)p;
+}
 
 ElemJob elem_job(const amr_model *m)
 {
@@ -1525,31 +1551,31 @@ ElemJob elem_job(const amr_model *m)
     q.nnodes = m->nnodes;
     q.nn = m->nn;
     q.all_elastic = m->all_elastic ? 1 : 0;
-    q.X = m->X;
-    q.conn = m->conn;
-    q.emat = m->emat;
-    q.mats = m->mats;
-    q.dNdR = m->dNdR;
-    q.wip = m->wip;
-    q.st_cur = m->st_cur;
-    q.st_com = m->st_com;
-    q.st_out = m->st_cur;
-    q.Ke = m->Ke;
-    q.err = m->err;
-    q.cptr = m->cptr;
-    q.csrc = m->csrc;
-    q.diag = m->diag;
-    q.fptr = m->fptr;
-    q.fsrc = m->fsrc;
-    q.Kv = m->Kv;
-    q.Kv32 = m->Kv32;
-    q.mask = m->mask;
-    q.dinv = m->dinv;
-    q.binv = m->binv;
-    q.dinv32 = m->dinv32;
-    q.binv32 = m->binv32;
-    q.dF = m->dF;
-    q.estat = m->estat;
+    q.X = as1(m->X);
+    q.conn = as1(m->conn);
+    q.emat = as1(m->emat);
+    q.mats = as1(m->mats);
+    q.dNdR = as1(m->dNdR);
+    q.wip = as1(m->wip);
+    q.st_cur = as1(m->st_cur);
+    q.st_com = as1(m->st_com);
+    q.st_out = as1(m->st_cur);
+    q.Ke = as1(m->Ke);
+    q.err = as1(m->err);
+    q.cptr = as1(m->cptr);
+    q.csrc = as1(m->csrc);
+    q.diag = as1(m->diag);
+    q.fptr = as1(m->fptr);
+    q.fsrc = as1(m->fsrc);
+    q.Kv = as1(m->Kv);
+    q.Kv32 = as1(m->Kv32);
+    q.mask = as1(m->mask);
+    q.dinv = as1(m->dinv);
+    q.binv = as1(m->binv);
+    q.dinv32 = as1(m->dinv32);
+    q.binv32 = as1(m->binv32);
+    q.dF = as1(m->dF);
+    q.estat = as1(m->estat);
     return q;
 }
 
@@ -1598,12 +1624,21 @@ struct BatchUpload {
 };
 
 // start table of a launch over `idx` (indices into the jobs) with `wgs(j)`
-// workgroups for job j; the jobs of the launch are copied in that order
+// workgroups for job j (the jobs of the launch are copied in that order):
+// the offsets, then the job of every 1,024th workgroup (job_of)
 template <typename F>
 std::vector<int64_t> starts(const std::vector<int> &idx, F wgs)
 {
-    std::vector<int64_t> st(idx.size() + 1, 0);
-    for (size_t i = 0; i < idx.size(); i++) st[i + 1] = st[i] + wgs(idx[i]);
+    const size_t n = idx.size();
+    std::vector<int64_t> st(n + 1, 0);
+    for (size_t i = 0; i < n; i++) st[i + 1] = st[i] + wgs(idx[i]);
+    const int64_t total = st[n], nh = (total >> HINT_SHIFT) + 1;
+    size_t j = 0;
+    for (int64_t k = 0; k < nh; k++) {
+        const int64_t b = k << HINT_SHIFT;
+        while (j + 1 < n && st[j + 1] <= b) j++;
+        st.push_back((int64_t)j);
+    }
     return st;
 }
 
@@ -1701,7 +1736,7 @@ int amr_batch_mount_K(amr_model *const *ms, int n, hipStream_t s, AmrStaging &st
         std::vector<ElemJob> jj;
         for (int j : idx) jj.push_back(jobs[j]);
         const std::vector<int64_t> st = starts(idx, wgs);
-        L l{key, up.add(jj), 0, (int)idx.size(), st.back()};
+        L l{key, up.add(jj), 0, (int)idx.size(), st[idx.size()]};
         l.o_st = up.add(st);
         return l;
     };
@@ -1743,8 +1778,8 @@ int amr_batch_update(amr_model *const *ms, const double *const *dU, double *cons
             return AMR_ERR_ARG;
         }
         jobs[j] = elem_job(ms[j]);
-        jobs[j].dU = dU[j];
-        jobs[j].dFin = dFin[j];
+        jobs[j].dU = as1(dU[j]);
+        jobs[j].dFin = as1(dFin[j]);
         if (ms[j]->ne > 0) groups[up_key(ms[j])].push_back(j);
         all.push_back(j);
     }
@@ -1759,7 +1794,7 @@ int amr_batch_update(amr_model *const *ms, const double *const *dU, double *cons
         std::vector<ElemJob> jj;
         for (int j : idx) jj.push_back(jobs[j]);
         const std::vector<int64_t> st = starts(idx, wgs);
-        L l{key, up.add(jj), 0, (int)idx.size(), st.back()};
+        L l{key, up.add(jj), 0, (int)idx.size(), st[idx.size()]};
         l.o_st = up.add(st);
         return l;
     };
