@@ -82,6 +82,7 @@ SIGNATURES = {
     "amr_pool_step": (I32, [P, I32, P, P, ctypes.POINTER(SolveInfo)]),
     "amr_pool_step_begin": (I32, [P]),
     "amr_pool_step_done": (I32, [P, P]),
+    "amr_pool_release_left": (I32, [P]),
     "amr_pool_step_end": (I32, [P, I32, P, P, ctypes.POINTER(SolveInfo)]),
     "amr_pool_size": (I32, [P, P]),
     "amr_pool_timing": (I32, [P, I32, P, P]),

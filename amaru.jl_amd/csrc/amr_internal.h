@@ -308,6 +308,7 @@ int amr_pool_step_launch(amr_pool *p);
 int amr_pool_step_collect(amr_pool *p, std::vector<amr_pool_result> &out);
 int amr_pool_step_members(const amr_pool *p);  // members of the step in flight (0: none)
 int amr_pool_step_query(amr_pool *p, int *done);
+int amr_pool_hand_back(amr_pool *p);
 int amr_pool_set_timing(amr_pool *p, int enable, double *ms, int64_t *n);
 int amr_pool_graphs(amr_pool *p, int enable, int64_t *captures, int64_t *replays);
 int amr_pool_counts(amr_pool *p, int64_t *out);

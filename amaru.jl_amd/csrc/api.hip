@@ -1781,6 +1781,12 @@ extern "C" int amr_pool_step_done(amr_pool *p, int *done)
     return amr_pool_step_query(p, done);
 }
 
+extern "C" int amr_pool_release_left(amr_pool *p)
+{
+    if (!p) return AMR_ERR_ARG;
+    return amr_pool_hand_back(p);
+}
+
 extern "C" int amr_pool_step_end(amr_pool *p, int capacity, int *nfinished, amr_model **models,
                                  amr_solve_info *infos)
 {

@@ -1081,9 +1081,9 @@ struct amr_pool {
     hipEvent_t ev_step = nullptr;    // recorded after the step's scalar read-back
     std::vector<int> fl_act;         // members of the in-flight step
     // models that left the pool at the last step and still run on its
-    // stream; they return to their own streams when the next step is queued,
-    // so that the work queued for them in between (the NR finish) runs ahead
-    // of that step
+    // stream; they return to their own streams when the next step is queued
+    // (or at amr_pool_hand_back), so that the work queued for them in
+    // between (the NR finish) runs ahead of that step
     std::vector<amr_model *> leaving;
     // SpMV timing
     bool timing = false;
@@ -1632,10 +1632,7 @@ int amr_pool_step_launch(amr_pool *p)
         if (p->mem[s].state == M_JOIN) join.push_back(s);
         if (p->mem[s].state == M_JOIN || p->mem[s].state == M_ACTIVE) act.push_back(s);
     }
-    for (amr_model *m : p->leaving)
-        if (m->stream == p->stream && m->pool == p && !amr_pool_is_member(m))
-            if (int rc = handoff(m, p->stream, m->own_stream)) return rc;
-    p->leaving.clear();
+    if (int rc = amr_pool_hand_back(p)) return rc;
     if (act.empty()) return AMR_OK;
     const int check = p->check;
     hipStream_t st = p->stream;
@@ -1932,6 +1929,19 @@ int amr_pool_step_collect(amr_pool *p, std::vector<amr_pool_result> &out)
     for (size_t i = first; i < out.size(); i++)
         if (out[i].m->stream != out[i].m->own_stream) p->leaving.push_back(out[i].m);
     return rc;
+}
+
+// The models that left the pool at its last step return to their own
+// streams, after the work queued for them on the pool's stream so far.
+int amr_pool_hand_back(amr_pool *p)
+{
+    if (p->leaving.empty()) return AMR_OK;
+    AMR_HIP(hipSetDevice(p->device));
+    for (amr_model *m : p->leaving)
+        if (m->stream == p->stream && m->pool == p && !amr_pool_is_member(m))
+            if (int rc = handoff(m, p->stream, m->own_stream)) return rc;
+    p->leaving.clear();
+    return AMR_OK;
 }
 
 int amr_pool_step_members(const amr_pool *p) { return p->in_flight ? (int)p->fl_act.size() : 0; }

@@ -18,6 +18,7 @@ between convergence and precision decisions).
 import collections
 import concurrent.futures
 import ctypes
+import sys
 import threading
 import time
 
@@ -90,6 +91,12 @@ class Pool:
         step_end start at the next step."""
         _lib.check(_lib.lib().amr_pool_step_begin(self._h))
 
+    def release_left(self):
+        """The models that left at the last step_end continue on their own
+        streams from now on (else from the next step_begin); call it before
+        continuing such a model on another thread."""
+        _lib.check(_lib.lib().amr_pool_release_left(self._h))
+
     def step_done(self):
         """True when step_end would not wait."""
         d = ctypes.c_int(1)
@@ -157,16 +164,39 @@ class Job:
     def resume_end(self):
         return self._advance(self.ana.dev.nr_finish_end())
 
-    def next_solve(self, r=None, first=False):
+    def prestart(self):
+        """The part of start() that touches no pool, for a worker thread:
+        the machine runs to its first pooled solve (stage set-up, boundary
+        values) and that iteration's mount_K and right-hand side are queued
+        on the model's own stream; returns True when the analysis has
+        already ended.  join_started() completes the start."""
+        it = self.next_solve(first=True, attach=False)
+        if it is None:
+            return True
+        self.ana.dev.nr_prepare(it)
+        return False
+
+    def join_started(self):
+        """After prestart(): the prepared solve joins the pool (on the
+        thread that drives the pool)."""
+        if self.done:
+            return True
+        self.pool.attach(self.ana.dev)
+        self.join()
+        return False
+
+    def next_solve(self, r=None, first=False, attach=True):
         """Advances the machine (sending it r) to its next Newton-Raphson
         iteration that needs a pooled linear solve and returns that
         iteration's number, or None when the analysis has ended (result
-        set).  The iterations of a model without unknowns are run here."""
+        set).  The iterations of a model without unknowns are run here.
+        With attach the model is attached to the job's pool."""
         try:
             it = next(self.gen) if first else self.gen.send(r)
             while True:
                 dev = self.ana.dev
-                self.pool.attach(dev)
+                if attach:
+                    self.pool.attach(dev)
                 if dev.model.nu == 0:
                     dev.nr_prepare(it)
                     it = self.gen.send(dev.nr_finish(None))
@@ -494,10 +524,22 @@ class Server:
     """
 
     def __init__(self, device=0, max_members=256, check_every=16, hbm_budget=None, workers=4, npools=3,
-                 order="fifo"):
+                 order="fifo", start_on_workers=True, resume_on_workers=True, switch_interval=None):
         if order not in ("fifo", "largest_first"):
             raise ValueError("order must be 'fifo' or 'largest_first'")
         self.order = order
+        # an admitted request's analysis runs to its first solve (and queues
+        # that iteration's mount_K) on the device worker; the serving loop
+        # only joins it to its pool
+        self.start_on_workers = bool(start_on_workers)
+        # a member whose solve finished completes its NR iteration and
+        # prepares the next one (or ends its analysis) on a worker; the
+        # serving loop joins it to its pool again
+        self.resume_on_workers = bool(resume_on_workers)
+        # the interpreter's thread switch interval while serving (seconds;
+        # None leaves it): the serving loop waits for the interpreter lock
+        # after every library call, held meanwhile by the build workers
+        self.switch_interval = switch_interval
         self.device = device
         npools = max(1, int(npools))
         self.pools = PoolSet(npools, device, max_members=-(-int(max_members) // npools), check_every=check_every)
@@ -516,7 +558,7 @@ class Server:
         self.t_loop = {"step": 0.0, "resume": 0.0, "start": 0.0, "wait": 0.0, "free": 0.0}
         # worker time (seconds, summed over requests): host construction and
         # device set-up (upload, pattern, coarse space)
-        self.t_build = {"host": 0.0, "device": 0.0}
+        self.t_build = {"host": 0.0, "device": 0.0, "resume": 0.0}
         # finished models whose device memory is released later (a release
         # can wait for the device), at idle points or when too much waits
         self._graveyard = []
@@ -577,7 +619,11 @@ class Server:
         ex = concurrent.futures.ThreadPoolExecutor(max_workers=self.workers)
         ex_dev = concurrent.futures.ThreadPoolExecutor(max_workers=max(1, self.workers // 2))
         self._closer = concurrent.futures.ThreadPoolExecutor(max_workers=1)
+        ex_res = concurrent.futures.ThreadPoolExecutor(max_workers=2)
         lock = threading.Lock()
+        switch0 = sys.getswitchinterval()
+        if self.switch_interval is not None:
+            sys.setswitchinterval(float(self.switch_interval))
         t0 = time.perf_counter()
         clock = lambda: time.perf_counter() - t0  # noqa: E731
 
@@ -602,14 +648,31 @@ class Server:
                 req.ana.dev = DeviceModel(req.ana.model, device=self.device)
                 if req.kwargs.get("precond") == "two_level":
                     req.ana.dev.set_coarse()   # aggregates built here, off the serving loop
+                if self.start_on_workers:
+                    # stage set-up and the first mount_K, off the serving loop
+                    req.job.prestart()
                 return req, None
             except Exception as e:
                 return req, e
             finally:
                 timed("device", ts)
 
+        def resume_prep(req):
+            # the model left its pool (release_left), so its work runs on its
+            # own stream, which no other thread uses meanwhile
+            ts = time.perf_counter()
+            try:
+                it = req.job.next_solve(req.ana.dev.nr_finish_end(), attach=False)
+                if it is not None:
+                    req.ana.dev.nr_prepare(it)
+                return req, None
+            except Exception as e:
+                return req, e
+            finally:
+                timed("resume", ts)
+
         nxt = 0
-        building, creating = set(), set()
+        building, creating, resuming = set(), set(), set()
         admit_q = collections.deque()
         arrived = []   # arrived, host build not started (largest_first only)
         running = {}
@@ -662,6 +725,11 @@ class Server:
                         self.peak_in_use = max(self.peak_in_use, self.in_use)
                         self.peak_held = max(self.peak_held, self.in_use + self._graveyard_bytes)
                     req.t_admit = clock()
+                    if self.start_on_workers:
+                        req.weight = work_weight(req.ana)
+                        req.home = self.pools.assign(req.weight)
+                        req.job = Job(req.ana, solver_args(device=self.device, **req.kwargs),
+                                      self.pools.pools[req.home])
                     creating.add(ex_dev.submit(device_build, req))
                 for f in [f for f in creating if f.done()]:
                     creating.discard(f)
@@ -673,11 +741,14 @@ class Server:
                         self.peak_in_use = max(self.peak_in_use, self.in_use)
                         req.t_start = clock()
                         try:
-                            req.weight = work_weight(req.ana)
-                            req.home = self.pools.assign(req.weight)
-                            req.job = Job(req.ana, solver_args(device=self.device, **req.kwargs),
-                                          self.pools.pools[req.home])
-                            done = req.job.start()
+                            if self.start_on_workers:
+                                done = req.job.join_started()
+                            else:
+                                req.weight = work_weight(req.ana)
+                                req.home = self.pools.assign(req.weight)
+                                req.job = Job(req.ana, solver_args(device=self.device, **req.kwargs),
+                                              self.pools.pools[req.home])
+                                done = req.job.start()
                         except Exception as e:
                             exc, done = e, True
                     else:
@@ -688,7 +759,7 @@ class Server:
                     else:
                         running[req.ana.dev.handle.value] = req
                     self.t_loop["start"] += time.perf_counter() - ts
-                if pending or self.pools.size() > 0 or self.pools.in_flight():
+                if pending or resuming or self.pools.size() > 0 or self.pools.in_flight():
                     # collect every pool whose step has finished: queue its
                     # finished members' NR updates, queue its next step, and
                     # leave completing the updates to the passes below, one
@@ -697,7 +768,7 @@ class Server:
                     # pool's members
                     self.pools.launch_idle()
                     ts = time.perf_counter()
-                    fins = self.pools.collect(block=not pending)
+                    fins = self.pools.collect(block=not pending and not resuming)
                     self.t_loop["step"] += time.perf_counter() - ts
                     ts = time.perf_counter()
                     for k, fin in fins:
@@ -707,10 +778,34 @@ class Server:
                             if done_reqs:
                                 devs = [r.ana.dev for r in done_reqs]
                                 type(devs[0]).nr_finish_begin_many(devs, [si for _, si in fin])
-                            pending.extend((r, None) for r in done_reqs)
+                                if self.resume_on_workers:
+                                    self.pools.pools[k].release_left()
+                                    resuming.update(ex_res.submit(resume_prep, r) for r in done_reqs)
+                                else:
+                                    pending.extend((r, None) for r in done_reqs)
                         except Exception as e:
                             pending.extend((r, e) for r in done_reqs)
                         self.pools.relauncher(k)()
+                    ready = [f for f in resuming if f.done()]
+                    if not fins and not ready and not pending and resuming:
+                        # nothing to do until a worker or a pool step ends
+                        concurrent.futures.wait(resuming, timeout=2e-4,
+                                                return_when=concurrent.futures.FIRST_COMPLETED)
+                        ready = [f for f in resuming if f.done()]
+                    for f in ready:
+                        resuming.discard(f)
+                        req, exc = f.result()
+                        done = True
+                        if exc is None:
+                            try:
+                                done = req.job.join_started()
+                            except Exception as e:
+                                exc = e
+                        if done:
+                            self._finish(req, clock(), exc)
+                            ndone += 1
+                        else:
+                            running[req.ana.dev.handle.value] = req
                     if pending:
                         req, exc = pending.popleft()
                         done = True
@@ -745,9 +840,11 @@ class Server:
             finally:
                 ex.shutdown(wait=True)
                 ex_dev.shutdown(wait=True)
+                ex_res.shutdown(wait=True)
                 self._release()
                 self._closer.shutdown(wait=True)
                 self._closer = None
+                sys.setswitchinterval(switch0)
         return reqs
 
 

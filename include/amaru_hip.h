@@ -350,6 +350,15 @@ int amr_pool_step_end(amr_pool *p, int capacity, int *nfinished, amr_model **mod
  * the device (amr_pool_step_end would not wait), else 0.  For hosts that
  * drive several pools from one thread. */
 int amr_pool_step_done(amr_pool *p, int *done);
+/* The models that left the pool at its last step_end (their solves
+ * finished) run their following work on the pool's stream until the next
+ * step is queued, so that the work queued for them in between (the NR
+ * finish) runs ahead of that step.  amr_pool_release_left moves them back to
+ * their own streams now, after the work queued for them so far: a host that
+ * continues such a model on another thread calls it first, on the thread
+ * that drives the pool.  A pool call of its own: the reference has no
+ * counterpart. */
+int amr_pool_release_left(amr_pool *p);
 
 /* Members currently solving. */
 int amr_pool_size(amr_pool *p, int *members);

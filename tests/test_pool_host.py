@@ -116,6 +116,9 @@ class FakePool:
     def detach(self, dev):
         self.attached.discard(dev.handle.value)
 
+    def release_left(self):
+        pass
+
     def forget(self, dev):
         self.attached.discard(dev.handle.value)
 
@@ -211,7 +214,8 @@ def test_run_analyses_interleaves_like_alone(amaru, monkeypatch):
         assert a.stats == b.stats
 
 
-def test_server_isolates_failures_and_returns_budget(amaru, monkeypatch):
+@pytest.mark.parametrize("on_workers", [False, True], ids=["loop", "workers"])
+def test_server_isolates_failures_and_returns_budget(amaru, monkeypatch, on_workers):
     """A request whose construction raises fails alone; the others finish
     with the results of solving alone; the admitted bytes return to zero."""
     from amaru_amd import analysis, pool as P
@@ -230,7 +234,8 @@ def test_server_isolates_failures_and_returns_budget(amaru, monkeypatch):
     reqs = [P.Request(i, 0.001 * i, builder(i)) for i in range(4)]
     est = max(P.estimate_device_bytes(a.model) for a in alone)
     budget = int(2.5 * est)   # room for two models at once
-    srv = P.Server(hbm_budget=budget, workers=2)
+    # on_workers: the analyses' starts and NR resumes run on worker threads
+    srv = P.Server(hbm_budget=budget, workers=2, start_on_workers=on_workers, resume_on_workers=on_workers)
     done = srv.serve(reqs)
     bad = [r for r in done if r.rid == 2][0]
     assert not bad.ok and "bad request" in bad.msg
