@@ -60,6 +60,20 @@ struct CoarseJob {
     int32_t *info;
 };
 
+// Pointers taken from a job are generic; the buffers are all in the global
+// address space, and saying so lets the compiler use global (not flat)
+// memory instructions.
+template <typename T>
+__device__ inline const __attribute__((address_space(1))) T *cg(const T *p)
+{
+    return (const __attribute__((address_space(1))) T *)p;
+}
+template <typename T>
+__device__ inline __attribute__((address_space(1))) T *cgw(T *p)
+{
+    return (__attribute__((address_space(1))) T *)p;
+}
+
 // entry (job, b): pairs s = 256 b + thread.  Pair s = (block row i,
 // aggregate b): U = R_i^T sum_{j in row i, agg(j) = b} K_ij R_j, 36 values
 __global__ void __launch_bounds__(256) galerkin_pairs_kernel(const CoarseJob *__restrict__ J,
@@ -69,14 +83,16 @@ __global__ void __launch_bounds__(256) galerkin_pairs_kernel(const CoarseJob *__
     const CoarseJob &q = J[e.x];
     const int64_t s = (int64_t)e.y * 256 + threadIdx.x;
     if (s >= q.npairs) return;
-    const int32_t *__restrict__ agg = q.agg;
-    const double *__restrict__ X = q.X, *__restrict__ cent = q.cent, *__restrict__ mask = q.mask;
-    const int32_t i = q.prow[s], b = q.pagg[s];
+    const auto *__restrict__ agg = cg(q.agg);
+    const auto *__restrict__ X = cg(q.X), *__restrict__ cent = cg(q.cent), *__restrict__ mask = cg(q.mask);
+    const auto *__restrict__ rowptr = cg(q.rowptr), *__restrict__ col = cg(q.col);
+    const auto *__restrict__ Kv = cg(q.Kv);
+    const int32_t i = cg(q.prow)[s], b = cg(q.pagg)[s];
     double W[3][6];
     for (int r = 0; r < 3; r++)
         for (int c = 0; c < 6; c++) W[r][c] = 0.0;
-    for (int32_t k = q.rowptr[i]; k < q.rowptr[i + 1]; k++) {
-        const int32_t j = q.col[k];
+    for (int32_t k = rowptr[i]; k < rowptr[i + 1]; k++) {
+        const int32_t j = col[k];
         if (agg[j] != b) continue;
         double d[3], mk[3], R[3][6], Kb[9];
         for (int c = 0; c < 3; c++) {
@@ -84,7 +100,7 @@ __global__ void __launch_bounds__(256) galerkin_pairs_kernel(const CoarseJob *__
             mk[c] = mask[(int64_t)j * 3 + c];
         }
         rigid_modes(d, mk, R);
-        for (int t = 0; t < 9; t++) Kb[t] = q.Kv[kv_at(k, t)];
+        for (int t = 0; t < 9; t++) Kb[t] = Kv[kv_at(k, t)];
         for (int r = 0; r < 3; r++)
             for (int c = 0; c < 6; c++)
                 W[r][c] = fma(Kb[r * 3 + 2], R[2][c], fma(Kb[r * 3 + 1], R[1][c], fma(Kb[r * 3], R[0][c], W[r][c])));
@@ -98,7 +114,7 @@ __global__ void __launch_bounds__(256) galerkin_pairs_kernel(const CoarseJob *__
     rigid_modes(d, mk, R);
     for (int p = 0; p < 6; p++)
         for (int c = 0; c < 6; c++)
-            q.pair[s * 36 + p * 6 + c] = fma(R[2][p], W[2][c], fma(R[1][p], W[1][c], R[0][p] * W[0][c]));
+            cgw(q.pair)[s * 36 + p * 6 + c] = fma(R[2][p], W[2][c], fma(R[1][p], W[1][c], R[0][p] * W[0][c]));
 }
 
 // entry (job, b): t = 256 b + thread = (coarse block cb, entry): Kc_ab is
@@ -112,8 +128,10 @@ __global__ void __launch_bounds__(256) galerkin_blocks_kernel(const CoarseJob *_
     if (t >= (int64_t)q.ncb * 36) return;
     const int32_t cb = (int32_t)(t / 36), c = (int32_t)(t % 36);
     double acc = 0.0;
-    for (int32_t k = q.cbptr[cb]; k < q.cbptr[cb + 1]; k++) acc += q.pair[(int64_t)q.cbslot[k] * 36 + c];
-    q.Kc[(int64_t)(6 * q.cba[cb] + c / 6) * q.nc + 6 * q.cbb[cb] + c % 6] = acc;
+    const auto *__restrict__ cbptr = cg(q.cbptr), *__restrict__ cbslot = cg(q.cbslot);
+    const auto *__restrict__ pair = cg(q.pair);
+    for (int32_t k = cbptr[cb]; k < cbptr[cb + 1]; k++) acc += pair[(int64_t)cbslot[k] * 36 + c];
+    cgw(q.Kc)[(int64_t)(6 * cg(q.cba)[cb] + c / 6) * q.nc + 6 * cg(q.cbb)[cb] + c % 6] = acc;
 }
 
 // entry (job, b): Kc[4096 b .. 4096 (b + 1)) = 0
@@ -123,7 +141,8 @@ __global__ void __launch_bounds__(256) zero_kernel(const CoarseJob *__restrict__
     const CoarseJob &q = J[e.x];
     const int64_t n = (int64_t)q.nc * q.nc, b0 = (int64_t)e.y * 4096;
     const int64_t b1 = n < b0 + 4096 ? n : b0 + 4096;
-    for (int64_t i = b0 + threadIdx.x; i < b1; i += 256) q.Kc[i] = 0.0;
+    auto *__restrict__ Kc = cgw(q.Kc);
+    for (int64_t i = b0 + threadIdx.x; i < b1; i += 256) Kc[i] = 0.0;
 }
 
 // entry (job, b): coarse dof k = 256 b + thread.  Coarse dofs whose basis
@@ -134,8 +153,9 @@ __global__ void __launch_bounds__(256) fix_diag_kernel(const CoarseJob *__restri
     const int2 e = wl[blockIdx.x];
     const CoarseJob &q = J[e.x];
     const int32_t k = e.y * 256 + (int32_t)threadIdx.x;
-    if (k == 0) q.info[0] = q.info[1] = 0;
-    if (k < q.nc && q.Kc[(int64_t)k * q.nc + k] == 0.0) q.Kc[(int64_t)k * q.nc + k] = 1.0;
+    auto *__restrict__ Kc = cgw(q.Kc);
+    if (k == 0) cgw(q.info)[0] = cgw(q.info)[1] = 0;
+    if (k < q.nc && Kc[(int64_t)k * q.nc + k] == 0.0) Kc[(int64_t)k * q.nc + k] = 1.0;
 }
 
 // entry (job, b): entry t = 256 b + thread of the inverse.  Mirror the upper
@@ -149,11 +169,12 @@ __global__ void __launch_bounds__(256) symmetrize_kernel(const CoarseJob *__rest
     const int64_t t = (int64_t)e.y * 256 + threadIdx.x;
     if (t >= (int64_t)nc * nc) return;
     const int32_t i = (int32_t)(t / nc), j = (int32_t)(t % nc);
-    if (q.info[0] != 0) {
-        q.Kc[t] = 0.0;
+    auto *__restrict__ Kc = cgw(q.Kc);
+    if (cg(q.info)[0] != 0) {
+        Kc[t] = 0.0;
         return;
     }
-    if (i < j) q.Kc[(int64_t)j * nc + i] = q.Kc[(int64_t)i * nc + j];
+    if (i < j) Kc[(int64_t)j * nc + i] = Kc[(int64_t)i * nc + j];
 }
 
 // ---- block Gauss-Jordan inversion (in place, row-major nc x nc) ----------
@@ -170,8 +191,8 @@ __global__ void __launch_bounds__(1024) gj_pivot_kernel(const CoarseJob *__restr
 {
     const CoarseJob &q = J[wl[blockIdx.x].x];
     const int32_t nc = q.nc, k0 = k * GJB, b = nc - k0 < GJB ? nc - k0 : GJB;
-    const double *__restrict__ A = q.Kc;
-    double *__restrict__ P = q.P;
+    const auto *__restrict__ A = cg(q.Kc);
+    auto *__restrict__ P = cgw(q.P);
     // the pivot block in LDS; thread (i, j0) updates entries (i, j0..j0+3):
     // each step reads the pivot row and column, then (after a barrier)
     // writes the new values
@@ -182,7 +203,7 @@ __global__ void __launch_bounds__(1024) gj_pivot_kernel(const CoarseJob *__restr
     for (int p = 0; p < b; p++) {
         const double piv = M[p][p];
         const bool ok = piv > 0.0 && std::isfinite(piv);
-        if (!ok && threadIdx.x == 0) q.info[0] = 1;
+        if (!ok && threadIdx.x == 0) cgw(q.info)[0] = 1;
         const double ip = ok ? 1.0 / piv : 0.0;
         const double ci = M[i][p];
         double nv[4];
@@ -210,8 +231,8 @@ __global__ void __launch_bounds__(256) gj_panel_kernel(const CoarseJob *__restri
     const int2 e = wl[blockIdx.x];
     const CoarseJob &q = J[e.x];
     const int32_t nc = q.nc, t = e.y, k0 = k * GJB, c0 = t * GJB, b = nc - k0 < GJB ? nc - k0 : GJB;
-    const double *__restrict__ A = q.Kc, *__restrict__ P = q.P;
-    double *__restrict__ R = q.R, *__restrict__ C = q.C;
+    const auto *__restrict__ A = cg(q.Kc), *__restrict__ P = cg(q.P);
+    auto *__restrict__ R = cgw(q.R), *__restrict__ C = cgw(q.C);
     __shared__ double Ps[GJB][GJB + 1];
     __shared__ double As[GJB][GJB + 1];
     const int w = std::min(GJB, nc - c0);  // columns (and rows) of this tile
@@ -242,8 +263,8 @@ __global__ void __launch_bounds__(256) gj_update_kernel(const CoarseJob *__restr
     const CoarseJob &q = J[e.x];
     const int32_t nc = q.nc, ti = e.y / q.nt, tj = e.y % q.nt, k0 = k * GJB;
     const int32_t b = nc - k0 < GJB ? nc - k0 : GJB;
-    double *__restrict__ A = q.Kc;
-    const double *__restrict__ P = q.P, *__restrict__ R = q.R, *__restrict__ C = q.C;
+    auto *__restrict__ A = cgw(q.Kc);
+    const auto *__restrict__ P = cg(q.P), *__restrict__ R = cg(q.R), *__restrict__ C = cg(q.C);
     __shared__ double Cs[GJB][GJB + 1];
     __shared__ double Bs[GJB][GJB + 1];
     const int r0 = ti * GJB, c0 = tj * GJB;
@@ -277,7 +298,7 @@ __global__ void __launch_bounds__(256) gj_update_kernel(const CoarseJob *__restr
         for (int y = 0; y < 4; y++) {
             const int r = rr + x, c = cc + y;
             if (r >= hr || c >= hc) continue;
-            double *a = A + (int64_t)(r0 + r) * nc + c0 + c;
+            auto *a = A + (int64_t)(r0 + r) * nc + c0 + c;
             *a = tj == k ? -acc[x][y] : *a - acc[x][y];
         }
 }

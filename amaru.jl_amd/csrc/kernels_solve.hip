@@ -490,8 +490,8 @@ __global__ void __launch_bounds__(WGS) residual_kernel(const SegDesc *__restrict
     const int nr = load_rows(d.rowptr, d.nnodes, R, te.y, srp, r0);
     constexpr int RPW = WGS / L;
     const int sub = threadIdx.x % L;
-    const double *__restrict__ b = d.b;
-    const double *__restrict__ mask = d.mask;
+    const auto *__restrict__ b = gl(d.b);
+    const auto *__restrict__ mask = gl(d.mask);
     for (int i = threadIdx.x / L; i < nr; i += RPW) {
         double y[3], a[3];
         if (d.col16)
@@ -507,11 +507,11 @@ __global__ void __launch_bounds__(WGS) residual_kernel(const SegDesc *__restrict
                 const double ri = fr ? b[k] - y[c] : 0.0;
                 const double bi = fr ? b[k] : 0.0;
                 const double ai = fr ? a[c] + fabs(b[k]) : 0.0;
-                d.rt[k] = ri;
+                glw(d.rt)[k] = ri;
                 rr = fma(ri, ri, rr);
                 bb = fma(bi, bi, bb);
                 kx = fma(ai, ai, kx);
-                const double g = ri - d.r[k];
+                const double g = ri - gl(d.r)[k];
                 gap = fma(g, g, gap);
             }
             s_rr[i] = rr;
@@ -524,10 +524,10 @@ __global__ void __launch_bounds__(WGS) residual_kernel(const SegDesc *__restrict
     const double trr = lds_tree_sum(s_rr, RP), tbb = lds_tree_sum(s_bb, RP), tkx = lds_tree_sum(s_kx, RP),
                  tgap = lds_tree_sum(s_gap, RP);
     if (threadIdx.x == 0) {
-        d.part[P_A + te.y] = trr;
-        d.part[P_B + te.y] = tbb;
-        d.part[P_C + te.y] = tkx;
-        d.part[P_D + te.y] = tgap;
+        glw(d.part)[P_A + te.y] = trr;
+        glw(d.part)[P_B + te.y] = tbb;
+        glw(d.part)[P_C + te.y] = tkx;
+        glw(d.part)[P_D + te.y] = tgap;
     }
 }
 
@@ -561,7 +561,7 @@ __global__ void __launch_bounds__(WGS) pcg_spmv_kernel(const SegDesc *__restrict
     const int2 te = tl[xcd_tile(blockIdx.x, gridDim.x)];
     if (te.x < 0) return;  // padding of a captured step's list
     const SegDesc &d = D[te.x];
-    int32_t *ctl = d.ctl;
+    auto *ctl = glw(d.ctl);
     if (ctl[C_DONE]) return;
     if (ctl[C_STOP]) {
         // converged in the previous iteration: every workgroup leaves, and
@@ -609,11 +609,11 @@ __global__ void __launch_bounds__(WGS) pcg_spmv_kernel(const SegDesc *__restrict
     if (RP <= 1024) {
         if (threadIdx.x < 64) {
             const double pq = wave_tree_sum(sdot, RP);
-            if (threadIdx.x == 0) d.part[P_PQ + te.y] = pq;
+            if (threadIdx.x == 0) glw(d.part)[P_PQ + te.y] = pq;
         }
     } else {
         const double pq = lds_tree_sum(sdot, RP);
-        if (threadIdx.x == 0) d.part[P_PQ + te.y] = pq;
+        if (threadIdx.x == 0) glw(d.part)[P_PQ + te.y] = pq;
     }
 }
 
@@ -622,7 +622,7 @@ __global__ void __launch_bounds__(WGV) pcg_update_kernel(const SegDesc *__restri
     const int2 ve = vl[blockIdx.x];
     if (ve.x < 0) return;
     const SegDesc &d = D[ve.x];
-    int32_t *ctl = d.ctl;
+    auto *ctl = glw(d.ctl);
     if (ctl[C_DONE]) return;
     const int pi = (ctl[C_ITERS] - 1) & 1;
     auto *__restrict__ x = glw(d.x);
@@ -643,7 +643,7 @@ __global__ void __launch_bounds__(WGV) pcg_update_kernel(const SegDesc *__restri
             rv0[c] = r[nd * 3 + c];
         }
     const double pq = sum_partials<WGV>(d.part + P_PQ, d.ntiles);
-    const double rz = d.scal[rz_pub(1 - pi)];
+    const double rz = glw(d.scal)[rz_pub(1 - pi)];
     if (!(pq > 0.0) || !std::isfinite(pq)) {
         if (ve.y == 0 && threadIdx.x == 0) {
             ctl[C_BREAK] = 1;
@@ -680,8 +680,8 @@ __global__ void __launch_bounds__(WGV) pcg_update_kernel(const SegDesc *__restri
     rzn = block_sum<WGV>(rzn);
     rr = block_sum<WGV>(rr);
     if (threadIdx.x == 0) {
-        d.part[(pi ? P_RZ1 : P_RZ0) + ve.y] = rzn;
-        d.part[(pi ? P_RR1 : P_RR0) + ve.y] = rr;
+        glw(d.part)[(pi ? P_RZ1 : P_RZ0) + ve.y] = rzn;
+        glw(d.part)[(pi ? P_RR1 : P_RR0) + ve.y] = rr;
     }
 }
 
@@ -690,7 +690,7 @@ __global__ void __launch_bounds__(WGV) pcg_dir_kernel(const SegDesc *__restrict_
     const int2 ve = vl[blockIdx.x];
     if (ve.x < 0) return;
     const SegDesc &d = D[ve.x];
-    int32_t *ctl = d.ctl;
+    auto *ctl = glw(d.ctl);
     if (ctl[C_DONE]) return;
     const int it = ctl[C_ITERS], pi = (it - 1) & 1;
     auto *__restrict__ p = glw(d.p);
@@ -707,15 +707,15 @@ __global__ void __launch_bounds__(WGV) pcg_dir_kernel(const SegDesc *__restrict_
     // rz of the previous iteration was published in the other parity slot;
     // the first workgroup publishes this one's and tests convergence
     const double rzn = sum_rz(d, pi);
-    const double rzo = d.scal[rz_pub(1 - pi)];
+    const double rzo = glw(d.scal)[rz_pub(1 - pi)];
     const double beta = rzn / rzo;
     if (ve.y == 0) {
         const double rr = sum_partials<WGV>(d.part + (pi ? P_RR1 : P_RR0), d.nvblk);
         if (threadIdx.x == 0) {
-            d.scal[S_RR] = rr;
-            d.scal[S_RZ] = rzn;
-            d.scal[rz_pub(pi)] = rzn;
-            if (rr <= d.scal[S_THR] || it >= d.maxiter || !std::isfinite(rr)) ctl[C_STOP] = 1;
+            glw(d.scal)[S_RR] = rr;
+            glw(d.scal)[S_RZ] = rzn;
+            glw(d.scal)[rz_pub(pi)] = rzn;
+            if (rr <= glw(d.scal)[S_THR] || it >= d.maxiter || !std::isfinite(rr)) ctl[C_STOP] = 1;
         }
     }
     while (i < n) {
@@ -741,7 +741,7 @@ __global__ void __launch_bounds__(64) tl_restrict_kernel(const SegDesc *__restri
     const int2 e = al[blockIdx.x];
     if (e.x < 0) return;
     const SegDesc &d = D[e.x];
-    if (mode == 0 && d.ctl[C_DONE]) return;
+    if (mode == 0 && glw(d.ctl)[C_DONE]) return;
     const int32_t a = e.y;
     const auto *cent = gl(d.cent), *X = gl(d.X), *rv = gl(d.r);
     const auto *aggptr = gl(d.aggptr), *aggnodes = gl(d.aggnodes);
@@ -788,8 +788,8 @@ __global__ void __launch_bounds__(256) tl_coarse_kernel(const SegDesc *__restric
     const int2 e = wl[blockIdx.x];
     if (e.x < 0) return;
     const SegDesc &d = D[e.x];
-    if (mode == 0 && d.ctl[C_DONE]) return;
-    const int pi = (d.ctl[C_ITERS] - 1) & 1;
+    if (mode == 0 && glw(d.ctl)[C_DONE]) return;
+    const int pi = (glw(d.ctl)[C_ITERS] - 1) & 1;
     const int row = e.y * 4 + (int)(threadIdx.x / 64), lane = threadIdx.x % 64;
     double acc = 0.0;
     const auto *cvec = gl(d.cvec);
@@ -817,7 +817,7 @@ __global__ void __launch_bounds__(256) tl_coarse_kernel(const SegDesc *__restric
         cyw[threadIdx.x / 64] = row < d.nc ? cvec[row] * acc : 0.0;
     }
     __syncthreads();
-    if (threadIdx.x == 0) d.part[(pi ? P_CY1 : P_CY0) + e.y] = ((cyw[0] + cyw[1]) + cyw[2]) + cyw[3];
+    if (threadIdx.x == 0) glw(d.part)[(pi ? P_CY1 : P_CY0) + e.y] = ((cyw[0] + cyw[1]) + cyw[2]) + cyw[3];
 }
 
 __global__ void __launch_bounds__(WGV) tl_prolong_kernel(const SegDesc *__restrict__ D, const int2 *__restrict__ vl)
@@ -840,7 +840,7 @@ __global__ void __launch_bounds__(WGV) pcg_init_kernel(const SegDesc *__restrict
     const SegDesc &d = D[ve.x];
     // iteration 0: the two-level kernels after this one write the parity
     // slot of C_ITERS - 1, i.e. slot 1, like the Jacobi partials below
-    if (ve.y == 0 && threadIdx.x == 0) d.ctl[C_ITERS] = 0;
+    if (ve.y == 0 && threadIdx.x == 0) glw(d.ctl)[C_ITERS] = 0;
     double rz = 0.0, rr = 0.0;
     float *__restrict__ p32 = d.wp32 ? d.p32 : nullptr;
     const int64_t stride = (int64_t)d.nvblk * WGV;
@@ -862,8 +862,8 @@ __global__ void __launch_bounds__(WGV) pcg_init_kernel(const SegDesc *__restrict
     rz = block_sum<WGV>(rz);
     rr = block_sum<WGV>(rr);
     if (threadIdx.x == 0) {
-        d.part[P_RZ1 + ve.y] = rz;
-        d.part[P_RR1 + ve.y] = rr;
+        glw(d.part)[P_RZ1 + ve.y] = rz;
+        glw(d.part)[P_RR1 + ve.y] = rr;
     }
 }
 
@@ -874,7 +874,7 @@ __global__ void __launch_bounds__(WGV) pcg_rez_kernel(const SegDesc *__restrict_
 {
     const int2 ve = vl[blockIdx.x];
     const SegDesc &d = D[ve.x];
-    const int slot_pi = 1 - (d.ctl[C_ITERS] & 1);
+    const int slot_pi = 1 - (glw(d.ctl)[C_ITERS] & 1);
     float *__restrict__ p32 = d.wp32 ? d.p32 : nullptr;
     double rz = 0.0;
     const int64_t stride = (int64_t)d.nvblk * WGV;
@@ -894,7 +894,7 @@ __global__ void __launch_bounds__(WGV) pcg_rez_kernel(const SegDesc *__restrict_
         }
     }
     rz = block_sum<WGV>(rz);
-    if (threadIdx.x == 0) d.part[(slot_pi ? P_RZ1 : P_RZ0) + ve.y] = rz;
+    if (threadIdx.x == 0) glw(d.part)[(slot_pi ? P_RZ1 : P_RZ0) + ve.y] = rz;
 }
 
 // One workgroup per listed model publishes scalars from partials.
@@ -905,7 +905,7 @@ __global__ void __launch_bounds__(WGV) pcg_finalize_kernel(const SegDesc *__rest
                                                            const int32_t *__restrict__ sl, int mode)
 {
     const SegDesc &d = D[sl[blockIdx.x]];
-    int32_t *ctl = d.ctl;
+    auto *ctl = glw(d.ctl);
     double *scal = d.scal;
     if (mode == 0) {
         const double rz = sum_rz(d, 1);
