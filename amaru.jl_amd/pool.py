@@ -18,7 +18,6 @@ between convergence and precision decisions).
 import collections
 import concurrent.futures
 import ctypes
-import sys
 import threading
 import time
 
@@ -524,7 +523,7 @@ class Server:
     """
 
     def __init__(self, device=0, max_members=256, check_every=16, hbm_budget=None, workers=4, npools=3,
-                 order="fifo", start_on_workers=True, resume_on_workers=True, switch_interval=None):
+                 order="fifo", start_on_workers=True, resume_on_workers=True):
         if order not in ("fifo", "largest_first"):
             raise ValueError("order must be 'fifo' or 'largest_first'")
         self.order = order
@@ -536,10 +535,6 @@ class Server:
         # prepares the next one (or ends its analysis) on a worker; the
         # serving loop joins it to its pool again
         self.resume_on_workers = bool(resume_on_workers)
-        # the interpreter's thread switch interval while serving (seconds;
-        # None leaves it): the serving loop waits for the interpreter lock
-        # after every library call, held meanwhile by the build workers
-        self.switch_interval = switch_interval
         self.device = device
         npools = max(1, int(npools))
         self.pools = PoolSet(npools, device, max_members=-(-int(max_members) // npools), check_every=check_every)
@@ -621,9 +616,6 @@ class Server:
         self._closer = concurrent.futures.ThreadPoolExecutor(max_workers=1)
         ex_res = concurrent.futures.ThreadPoolExecutor(max_workers=2)
         lock = threading.Lock()
-        switch0 = sys.getswitchinterval()
-        if self.switch_interval is not None:
-            sys.setswitchinterval(float(self.switch_interval))
         t0 = time.perf_counter()
         clock = lambda: time.perf_counter() - t0  # noqa: E731
 
@@ -844,7 +836,6 @@ class Server:
                 self._release()
                 self._closer.shutdown(wait=True)
                 self._closer = None
-                sys.setswitchinterval(switch0)
         return reqs
 
 
