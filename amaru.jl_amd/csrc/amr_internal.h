@@ -169,7 +169,7 @@ struct amr_model {
     int coarse_refresh = AMR_COARSE_PER_DOF_MAP;  // when the coarse operator is formed again
     int32_t nagg = 0, ncoarse = 0, ncb = 0;
     int64_t npairs = 0;
-    int32_t *c_agg = nullptr, *c_aggptr = nullptr, *c_aggnodes = nullptr, *c_prow = nullptr, *c_pagg = nullptr;
+    int32_t *c_agg = nullptr, *c_aggptr = nullptr, *c_aggnodes = nullptr, *c_pptr = nullptr, *c_pagg = nullptr;
     int32_t *c_cba = nullptr, *c_cbb = nullptr, *c_cbptr = nullptr, *c_cbslot = nullptr, *c_info = nullptr;
     double *c_cent = nullptr, *c_pair = nullptr, *Kinv = nullptr, *c_vec = nullptr, *y_vec = nullptr;
     double *c_work = nullptr;  // workspace of the coarse inversion

@@ -53,8 +53,8 @@ __device__ inline void rigid_modes(const double d[3], const double mk[3], double
 // arithmetic as a launch for its model alone would.
 struct CoarseJob {
     int32_t nc, nt, ncb;
-    int64_t npairs;
-    const int32_t *prow, *pagg, *agg, *rowptr, *col, *cba, *cbb, *cbptr, *cbslot;
+    int64_t npairs, nrows;
+    const int32_t *pptr, *pagg, *agg, *rowptr, *col, *cba, *cbb, *cbptr, *cbslot;
     const double *cent, *X, *mask, *Kv;
     double *pair, *Kc, *P, *R, *C;
     int32_t *info;
@@ -74,47 +74,61 @@ __device__ inline __attribute__((address_space(1))) T *cgw(T *p)
     return (__attribute__((address_space(1))) T *)p;
 }
 
-// entry (job, b): pairs s = 256 b + thread.  Pair s = (block row i,
-// aggregate b): U = R_i^T sum_{j in row i, agg(j) = b} K_ij R_j, 36 values
+// entry (job, g): block rows 16 g .. 16 g + 15, four consecutive rows per
+// wave.  Pair s = (row i, aggregate
+// b) of the row's pairs: U = R_i^T sum_{j in row i, agg(j) = b} K_ij R_j,
+// 36 values.  Lane l takes the row's blocks l, l + 64, ... (in that order)
+// and the wave sums the lanes' 3 x 6 products by a fixed tree.
 __global__ void __launch_bounds__(256) galerkin_pairs_kernel(const CoarseJob *__restrict__ J,
                                                              const int2 *__restrict__ wl)
 {
     const int2 e = wl[blockIdx.x];
     const CoarseJob &q = J[e.x];
-    const int64_t s = (int64_t)e.y * 256 + threadIdx.x;
-    if (s >= q.npairs) return;
+    const int lane = threadIdx.x % 64;
     const auto *__restrict__ agg = cg(q.agg);
     const auto *__restrict__ X = cg(q.X), *__restrict__ cent = cg(q.cent), *__restrict__ mask = cg(q.mask);
     const auto *__restrict__ rowptr = cg(q.rowptr), *__restrict__ col = cg(q.col);
     const auto *__restrict__ Kv = cg(q.Kv);
-    const int32_t i = cg(q.prow)[s], b = cg(q.pagg)[s];
-    double W[3][6];
-    for (int r = 0; r < 3; r++)
-        for (int c = 0; c < 6; c++) W[r][c] = 0.0;
-    for (int32_t k = rowptr[i]; k < rowptr[i + 1]; k++) {
-        const int32_t j = col[k];
-        if (agg[j] != b) continue;
-        double d[3], mk[3], R[3][6], Kb[9];
-        for (int c = 0; c < 3; c++) {
-            d[c] = X[(int64_t)j * 3 + c] - cent[(int64_t)b * 3 + c];
-            mk[c] = mask[(int64_t)j * 3 + c];
+    for (int64_t i = (int64_t)e.y * 16 + (threadIdx.x / 64) * 4, i1 = i + 4; i < i1 && i < q.nrows; i++) {
+        const int32_t k0 = rowptr[i], k1 = rowptr[i + 1], s0 = cg(q.pptr)[i], s1 = cg(q.pptr)[i + 1];
+        for (int32_t s = s0; s < s1; s++) {
+            const int32_t b = cg(q.pagg)[s];
+            double W[3][6];
+            for (int r = 0; r < 3; r++)
+                for (int c = 0; c < 6; c++) W[r][c] = 0.0;
+            for (int32_t k = k0 + lane; k < k1; k += 64) {
+                const int32_t j = col[k];
+                if (agg[j] != b) continue;
+                double d[3], mk[3], R[3][6], Kb[9];
+                for (int c = 0; c < 3; c++) {
+                    d[c] = X[(int64_t)j * 3 + c] - cent[(int64_t)b * 3 + c];
+                    mk[c] = mask[(int64_t)j * 3 + c];
+                }
+                rigid_modes(d, mk, R);
+                for (int t = 0; t < 9; t++) Kb[t] = Kv[kv_at(k, t)];
+                for (int r = 0; r < 3; r++)
+                    for (int c = 0; c < 6; c++)
+                        W[r][c] = fma(Kb[r * 3 + 2], R[2][c],
+                                      fma(Kb[r * 3 + 1], R[1][c], fma(Kb[r * 3], R[0][c], W[r][c])));
+            }
+            for (int r = 0; r < 3; r++)
+                for (int c = 0; c < 6; c++)
+                    for (int off = 32; off > 0; off >>= 1) W[r][c] += __shfl_xor(W[r][c], off, 64);
+            if (lane == 0) {
+                const int32_t a = agg[i];
+                double d[3], mk[3], R[3][6];
+                for (int c = 0; c < 3; c++) {
+                    d[c] = X[i * 3 + c] - cent[(int64_t)a * 3 + c];
+                    mk[c] = mask[i * 3 + c];
+                }
+                rigid_modes(d, mk, R);
+                for (int p = 0; p < 6; p++)
+                    for (int c = 0; c < 6; c++)
+                        cgw(q.pair)[(int64_t)s * 36 + p * 6 + c] =
+                            fma(R[2][p], W[2][c], fma(R[1][p], W[1][c], R[0][p] * W[0][c]));
+            }
         }
-        rigid_modes(d, mk, R);
-        for (int t = 0; t < 9; t++) Kb[t] = Kv[kv_at(k, t)];
-        for (int r = 0; r < 3; r++)
-            for (int c = 0; c < 6; c++)
-                W[r][c] = fma(Kb[r * 3 + 2], R[2][c], fma(Kb[r * 3 + 1], R[1][c], fma(Kb[r * 3], R[0][c], W[r][c])));
     }
-    const int32_t a = agg[i];
-    double d[3], mk[3], R[3][6];
-    for (int c = 0; c < 3; c++) {
-        d[c] = X[(int64_t)i * 3 + c] - cent[(int64_t)a * 3 + c];
-        mk[c] = mask[(int64_t)i * 3 + c];
-    }
-    rigid_modes(d, mk, R);
-    for (int p = 0; p < 6; p++)
-        for (int c = 0; c < 6; c++)
-            cgw(q.pair)[s * 36 + p * 6 + c] = fma(R[2][p], W[2][c], fma(R[1][p], W[1][c], R[0][p] * W[0][c]));
 }
 
 // entry (job, b): t = 256 b + thread = (coarse block cb, entry): Kc_ab is
@@ -333,7 +347,7 @@ int build(amr_model *m)
     if (int rc = dev_upload(m, &m->c_agg, C.agg)) return rc;
     if (int rc = dev_upload(m, &m->c_aggptr, C.aggptr)) return rc;
     if (int rc = dev_upload(m, &m->c_aggnodes, C.aggnodes)) return rc;
-    if (int rc = dev_upload(m, &m->c_prow, C.prow)) return rc;
+    if (int rc = dev_upload(m, &m->c_pptr, C.pptr)) return rc;
     if (int rc = dev_upload(m, &m->c_pagg, C.pagg)) return rc;
     if (int rc = dev_upload(m, &m->c_cba, C.cba)) return rc;
     if (int rc = dev_upload(m, &m->c_cbb, C.cbb)) return rc;
@@ -423,7 +437,8 @@ int amr_coarse_form(hipStream_t s, amr_model *const *ms, int n, AmrStaging &stg)
         q.nt = (q.nc + GJB - 1) / GJB;
         q.ncb = m->ncb;
         q.npairs = m->npairs;
-        q.prow = m->c_prow;
+        q.nrows = m->nnodes;
+        q.pptr = m->c_pptr;
         q.pagg = m->c_pagg;
         q.agg = m->c_agg;
         q.rowptr = m->rowptr;
@@ -445,7 +460,7 @@ int amr_coarse_form(hipStream_t s, amr_model *const *ms, int n, AmrStaging &stg)
         maxnt = std::max(maxnt, q.nt);
         const int64_t nn = (int64_t)q.nc * q.nc;
         for (int64_t b = 0; b < (nn + 4095) / 4096; b++) zl.push_back(make_int2(j, (int)b));
-        for (int64_t b = 0; b < (q.npairs + 255) / 256; b++) pl.push_back(make_int2(j, (int)b));
+        for (int64_t g = 0; g < (q.nrows + 15) / 16; g++) pl.push_back(make_int2(j, (int)g));
         for (int64_t b = 0; b < ((int64_t)q.ncb * 36 + 255) / 256; b++) bl.push_back(make_int2(j, (int)b));
         for (int64_t b = 0; b < (q.nc + 255) / 256; b++) fl.push_back(make_int2(j, (int)b));
         for (int64_t b = 0; b < (nn + 255) / 256; b++) sl.push_back(make_int2(j, (int)b));
@@ -523,7 +538,7 @@ int amr_coarse_form(hipStream_t s, amr_model *const *ms, int n, AmrStaging &stg)
 
 void amr_coarse_free(amr_model *m)
 {
-    void **ptrs[] = {(void **)&m->c_agg,  (void **)&m->c_aggptr, (void **)&m->c_aggnodes, (void **)&m->c_prow,
+    void **ptrs[] = {(void **)&m->c_agg,  (void **)&m->c_aggptr, (void **)&m->c_aggnodes, (void **)&m->c_pptr,
                      (void **)&m->c_pagg, (void **)&m->c_cba,    (void **)&m->c_cbb,      (void **)&m->c_cbptr,
                      (void **)&m->c_cbslot, (void **)&m->c_info, (void **)&m->c_cent,     (void **)&m->c_pair,
                      (void **)&m->Kinv,   (void **)&m->c_vec,    (void **)&m->y_vec,      (void **)&m->c_work};
