@@ -131,21 +131,33 @@ __global__ void __launch_bounds__(256) galerkin_pairs_kernel(const CoarseJob *__
     }
 }
 
-// entry (job, b): t = 256 b + thread = (coarse block cb, entry): Kc_ab is
-// the sum of its pairs in ascending row
+// entry (job, g): coarse blocks cb = 4 g + wave.  Kc_ab is the sum of the
+// block's pairs: lane l takes pairs l, l + 64, ... of the block's list (in
+// that order), the wave sums the lanes' 36 values by a fixed tree, and
+// lane c < 36 writes entry c.
 __global__ void __launch_bounds__(256) galerkin_blocks_kernel(const CoarseJob *__restrict__ J,
                                                               const int2 *__restrict__ wl)
 {
     const int2 e = wl[blockIdx.x];
     const CoarseJob &q = J[e.x];
-    const int64_t t = (int64_t)e.y * 256 + threadIdx.x;
-    if (t >= (int64_t)q.ncb * 36) return;
-    const int32_t cb = (int32_t)(t / 36), c = (int32_t)(t % 36);
-    double acc = 0.0;
+    const int32_t cb = e.y * 4 + (int32_t)(threadIdx.x / 64);
+    const int lane = threadIdx.x % 64;
+    if (cb >= q.ncb) return;  // the whole wave
     const auto *__restrict__ cbptr = cg(q.cbptr), *__restrict__ cbslot = cg(q.cbslot);
     const auto *__restrict__ pair = cg(q.pair);
-    for (int32_t k = cbptr[cb]; k < cbptr[cb + 1]; k++) acc += pair[(int64_t)cbslot[k] * 36 + c];
-    cgw(q.Kc)[(int64_t)(6 * cg(q.cba)[cb] + c / 6) * q.nc + 6 * cg(q.cbb)[cb] + c % 6] = acc;
+    double acc[36];
+    for (int c = 0; c < 36; c++) acc[c] = 0.0;
+    for (int32_t k = cbptr[cb] + lane; k < cbptr[cb + 1]; k += 64) {
+        const auto *pv = pair + (int64_t)cbslot[k] * 36;
+        for (int c = 0; c < 36; c++) acc[c] += pv[c];
+    }
+    double out = 0.0;
+    for (int c = 0; c < 36; c++) {
+        for (int off = 32; off > 0; off >>= 1) acc[c] += __shfl_xor(acc[c], off, 64);
+        if (lane == c) out = acc[c];
+    }
+    if (lane < 36)
+        cgw(q.Kc)[(int64_t)(6 * cg(q.cba)[cb] + lane / 6) * q.nc + 6 * cg(q.cbb)[cb] + lane % 6] = out;
 }
 
 // entry (job, b): Kc[4096 b .. 4096 (b + 1)) = 0
@@ -461,7 +473,7 @@ int amr_coarse_form(hipStream_t s, amr_model *const *ms, int n, AmrStaging &stg)
         const int64_t nn = (int64_t)q.nc * q.nc;
         for (int64_t b = 0; b < (nn + 4095) / 4096; b++) zl.push_back(make_int2(j, (int)b));
         for (int64_t g = 0; g < (q.nrows + 15) / 16; g++) pl.push_back(make_int2(j, (int)g));
-        for (int64_t b = 0; b < ((int64_t)q.ncb * 36 + 255) / 256; b++) bl.push_back(make_int2(j, (int)b));
+        for (int64_t g = 0; g < ((int64_t)q.ncb + 3) / 4; g++) bl.push_back(make_int2(j, (int)g));
         for (int64_t b = 0; b < (q.nc + 255) / 256; b++) fl.push_back(make_int2(j, (int)b));
         for (int64_t b = 0; b < (nn + 255) / 256; b++) sl.push_back(make_int2(j, (int)b));
     }
